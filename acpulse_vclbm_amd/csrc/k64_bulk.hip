@@ -1,0 +1,153 @@
+// Fused collide + push kernel, float64 tier.
+//
+// One thread per node, one workgroup per grid row (x is the fastest index), so
+// that every cache line of the destination arrays is written by a single
+// workgroup: the x-shifted stores of a row stay inside that row.  A (time t) is
+// read once, B (time t+1) written once: 2*19*8 = 304 bytes per node and step.
+//
+// Semantics follow Lattice::timeStepping* (Lattice.cpp:1524-1580): interior
+// nodes collide in place and push all 18 moving populations to their neighbours
+// (FluidNode::transport, FluidNode.h:22-32); boundary nodes do not collide and
+// push only the links that point inward in every boundary dimension
+// (BoundaryNode::transport with the pointers of Lattice::setConnectivity); every
+// node copies its rest population.  B entries that nothing pushes keep their
+// previous contents, as in the reference.  In periodic mode every node is
+// interior and the neighbours wrap.
+#include "k64.hpp"
+
+namespace acp {
+
+template <int OP, int ORDER, bool CUBIC, bool PERIODIC>
+__global__ __launch_bounds__(512) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                              double* __restrict__ B, double* __restrict__ psi,
+                                              const double* __restrict__ M, int nx, int ny, int nz, int64_t N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx) return;
+    const int j = blockIdx.y, kz = blockIdx.z;
+    const int64_t sxy = (int64_t)nx * ny;
+    const int64_t l = i + (int64_t)nx * j + sxy * kz;
+    const K64& k = *kp;
+
+    double f[19];
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = A[q * N + l];
+    });
+
+    int bi = 0, bj = 0, bk = 0;
+    if constexpr (!PERIODIC) {
+        bi = i == 0 ? 1 : (i == nx - 1 ? -1 : 0);
+        bj = j == 0 ? 1 : (j == ny - 1 ? -1 : 0);
+        bk = kz == 0 ? 1 : (kz == nz - 1 ? -1 : 0);
+    }
+    const bool boundary = (bi | bj | bk) != 0;
+
+    // neighbour offsets (wrapped in periodic mode)
+    int64_t oxp = 1, oxm = -1, oyp = nx, oym = -(int64_t)nx, ozp = sxy, ozm = -sxy;
+    if constexpr (PERIODIC) {
+        if (i == nx - 1) oxp = 1 - nx;
+        if (i == 0) oxm = nx - 1;
+        if (j == ny - 1) oyp = -(int64_t)nx * (ny - 1);
+        if (j == 0) oym = (int64_t)nx * (ny - 1);
+        if (kz == nz - 1) ozp = -sxy * (nz - 1);
+        if (kz == 0) ozm = sxy * (nz - 1);
+    }
+
+    if (!boundary) {
+        if constexpr (OP == OP_SRT) {
+            collide_srt64<ORDER, false>(k, f, nullptr);
+        } else if constexpr (OP == OP_RR || OP == OP_HRR) {
+            double ps[19];
+            if constexpr (CUBIC) {
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    ps[q] = psi[q * N + l];
+                });
+            }
+            double nb[6][4];
+            if constexpr (OP == OP_HRR || CUBIC) {
+                const int64_t o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+#pragma unroll
+                for (int m = 0; m < 6; ++m)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) nb[m][v] = M[v * N + l + o[m]];
+            }
+            collide_rr64<ORDER, OP == OP_HRR, CUBIC>(k, f, ps, nb);
+            if constexpr (OP == OP_HRR || CUBIC) {
+                if (psi != nullptr) {
+                    static_for<0, 19>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        psi[q * N + l] = ps[q];
+                    });
+                }
+            }
+        } else if constexpr (OP == OP_MRT_RM) {
+            collide_mrt_rm64(k, f);
+        } else if constexpr (OP == OP_MRT_GS) {
+            collide_mrt_gs64(k, f);
+        }
+    }
+
+    static_for<0, 18>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (!PERIODIC) {
+            if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
+        }
+        int64_t d = l;
+        if constexpr (EX[q] > 0) d += oxp;
+        if constexpr (EX[q] < 0) d += oxm;
+        if constexpr (EY[q] > 0) d += oyp;
+        if constexpr (EY[q] < 0) d += oym;
+        if constexpr (EZ[q] > 0) d += ozp;
+        if constexpr (EZ[q] < 0) d += ozm;
+        B[q * N + d] = f[q];
+    });
+    B[18 * N + l] = f[18];
+}
+
+template <int OP, int ORDER, bool CUBIC, bool PERIODIC>
+static hipError_t go(const K64* k, const double* A, double* B, double* psi, const double* M, const Grid& g,
+                     hipStream_t s) {
+    const int bx = g.nx >= 512 ? 512 : ((g.nx + 63) / 64) * 64;
+    dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
+    hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC>), grid, block, 0, s, k, A, B, psi, M, g.nx, g.ny, g.nz,
+                       g.N);
+    return hipGetLastError();
+}
+
+template <int OP, int ORDER, bool CUBIC>
+static hipError_t go_p(bool periodic, const K64* k, const double* A, double* B, double* psi, const double* M,
+                       const Grid& g, hipStream_t s) {
+    return periodic ? go<OP, ORDER, CUBIC, true>(k, A, B, psi, M, g, s)
+                    : go<OP, ORDER, CUBIC, false>(k, A, B, psi, M, g, s);
+}
+
+template <int OP, bool CUBIC>
+static hipError_t go_o(int order, bool periodic, const K64* k, const double* A, double* B, double* psi,
+                       const double* M, const Grid& g, hipStream_t s) {
+    switch (order) {
+        case 2: return go_p<OP, 2, CUBIC>(periodic, k, A, B, psi, M, g, s);
+        case 3: return go_p<OP, 3, CUBIC>(periodic, k, A, B, psi, M, g, s);
+        case 4: return go_p<OP, 4, CUBIC>(periodic, k, A, B, psi, M, g, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64* k, const double* A, double* B,
+                         double* psi, const double* macro, const Grid& g, hipStream_t s) {
+    switch (op) {
+        case OP_SRT: return go_o<OP_SRT, false>(order, periodic, k, A, B, psi, macro, g, s);
+        case OP_RR:
+            return cubic ? go_o<OP_RR, true>(order, periodic, k, A, B, psi, macro, g, s)
+                         : go_o<OP_RR, false>(order, periodic, k, A, B, psi, macro, g, s);
+        case OP_HRR:
+            return cubic ? go_o<OP_HRR, true>(order, periodic, k, A, B, psi, macro, g, s)
+                         : go_o<OP_HRR, false>(order, periodic, k, A, B, psi, macro, g, s);
+        case OP_MRT_RM: return go_p<OP_MRT_RM, 3, false>(periodic, k, A, B, psi, macro, g, s);
+        case OP_MRT_GS: return go_p<OP_MRT_GS, 3, false>(periodic, k, A, B, psi, macro, g, s);
+        case OP_STREAM: return go_p<OP_STREAM, 3, false>(periodic, k, A, B, psi, macro, g, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace acp

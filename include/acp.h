@@ -1,0 +1,150 @@
+/*
+ * acp.h -- C ABI of the MI355X D3Q19 lattice-Boltzmann engine (libacpulse.so).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  This is
+ * the boundary a caller of the reference solver (aleksandrShukman/acPulse_vcLBM)
+ * binds instead of its Control / Lattice classes; each entry point names the
+ * reference interface it replaces.  Populations are exchanged as float64 arrays
+ * in the layout f[q][k][j][i] (q in the reference's direction order, i fastest),
+ * i.e. index q*N + i + nx*(j + ny*k) with N = nx*ny*nz.
+ *
+ * Threading: one host thread per case handle at a time; different handles are
+ * independent.  All device work of a case is issued on the stream given to
+ * acp_case_create (or a stream the library creates when NULL is passed).
+ */
+#ifndef ACP_H
+#define ACP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference exits the process on configuration errors
+ * (Control.cpp:147, 164; Node.cpp:206, 436); this library returns them. */
+enum {
+    ACP_OK = 0,
+    ACP_E_ARG = 1,         /* invalid argument or unsupported parameter combination */
+    ACP_E_CONFIG = 2,      /* case file unreadable, key missing, bad value */
+    ACP_E_OOM = 3,         /* device memory allocation failed */
+    ACP_E_HIP = 4,         /* a HIP runtime call failed */
+    ACP_E_STATE = 5,       /* call out of order (e.g. step before init) */
+    ACP_E_UNSUPPORTED = 6, /* feature not available in this build (e.g. MaxLevel > 0) */
+    ACP_E_IO = 7           /* output file could not be written */
+};
+
+/* Collision-Model (BoundaryConditions.txt:31-44; Lattice.cpp:2158-2183) */
+enum { ACP_OP_SRT = 0, ACP_OP_MRT_GS = 1, ACP_OP_MRT_RM = 2, ACP_OP_RR = 3, ACP_OP_HRR = 4 };
+/* Storage / arithmetic tier.  F64 reproduces the reference arithmetic; the shifted
+ * tiers store f_i - w_i rho0 (scaled) and collide in float32, in lattice units. */
+enum { ACP_PREC_F64 = 0, ACP_PREC_F32_SHIFTED = 1, ACP_PREC_F16_SHIFTED = 2 };
+/* Domain boundary: the reference applies LODI on all six faces (Lattice.cpp:1582-1624). */
+enum { ACP_BC_LODI = 0, ACP_BC_PERIODIC = 1 };
+/* MRT operators: the reference computes the relaxed populations and discards them
+ * (Node.cpp:2603-2604, 2833-2834), i.e. MRT steps are pure streaming + LODI. */
+enum { ACP_MRT_AS_REFERENCE = 0, ACP_MRT_RELAXED = 1 };
+
+/* Case description: the fields of Control (Control.h:77-127) plus the options
+ * this engine adds.  Lengths in metres, rates in SI as in the case file. */
+typedef struct acp_case_desc {
+    int32_t nx, ny, nz;         /* nodes (Control.cpp:43-45) */
+    int32_t op;                 /* ACP_OP_* */
+    int32_t order;              /* Order-Of-Equilibrium 2|3|4 */
+    int32_t cubic;              /* Cubic-Mach-Correction yes=1 */
+    int32_t transverse;         /* Transverse yes=1 */
+    int32_t nonreflecting;      /* Non-Reflecting yes=1 (selects arrays in VTK output only) */
+    int32_t boundary;           /* ACP_BC_* */
+    int32_t mrt_mode;           /* ACP_MRT_* */
+    int32_t precision;          /* ACP_PREC_* */
+    int32_t max_level;          /* MaxLevel; 0 only */
+    int32_t timesteps;          /* Max-Time-Step */
+    int32_t write_interval;     /* WriteInterval */
+    int32_t threads;            /* Threads (kept for output parity; unused) */
+    double rho0;                /* Density */
+    double nu;                  /* KinematicViscosity */
+    double cs;                  /* SpeedOfSound */
+    double spacing;             /* Spacing */
+    double mach;                /* MACH-Number */
+    double k1;                  /* K1-Relaxation */
+    double k2;                  /* K2-Relaxation (parsed, unused as in the reference) */
+    double sigma;               /* Hybrid-Parameter */
+    double domain[3];           /* Domain-Size-X/Y/Z */
+    double f16_scale;           /* storage scale of the FP16 tier (per lattice) */
+    char save_as[1024];         /* Save-as (rest of line) */
+    char collision_model[32];   /* Collision-Model as written */
+} acp_case_desc;
+
+typedef struct acp_case acp_case;
+
+/* Case handle facts for admission and reporting. */
+typedef struct acp_case_info {
+    int64_t nodes;
+    int64_t device_bytes;       /* device memory held by the case */
+    int64_t bytes_per_node_step;/* algorithmic HBM bytes of the bulk kernel per node and step */
+    int64_t lodi_nodes;         /* boundary nodes handled by the LODI kernel */
+    int32_t lodi_launches;      /* LODI kernel launches per step */
+    int32_t steps_done;
+    double omega;               /* collision frequency (Control.cpp:54) */
+    double dt;                  /* time step (Control.cpp:53) */
+} acp_case_info;
+
+/* Last error text of the calling thread ("" when none). */
+const char* acp_last_error(void);
+
+/* Control::Control(string) (Control.cpp:8-132): parse a BoundaryConditions.txt
+ * with the reference's first-match keyword semantics and derive node counts.
+ * Also reads the optional keys this engine adds (Precision, Boundary, MRT-Mode,
+ * FP16-Scale); unknown keys are ignored as in the reference. */
+int acp_control_read(const char* path, acp_case_desc* out);
+
+/* Defaults of the reference's BoundaryConditions.txt for every field. */
+void acp_case_desc_default(acp_case_desc* out);
+
+/* Lattice::Lattice(Control&) (Lattice.h:23-57): allocate device state for one
+ * uniform-level case.  `stream` is a hipStream_t (NULL: the library creates one). */
+int acp_case_create(const acp_case_desc* desc, void* stream, acp_case** out);
+void acp_case_destroy(acp_case* c);
+int acp_case_info_get(const acp_case* c, acp_case_info* out);
+
+/* Lattice::initialization(Control&) (Lattice.cpp:1397-1493): the reference's
+ * hard-coded Gaussian pulse, feq_N(rho, 0) in both arrays. */
+int acp_case_init_reference(acp_case* c);
+/* Same initialization with a parameterised pulse in node units:
+ * rho = rho0 (1 + amp exp(-ln2 ((i-x0)^2 + (j-y0)^2) / b^2)), u = 0. */
+int acp_case_init_pulse(acp_case* c, double amp, double x0, double y0, double b);
+/* Random fluctuation field on the device: rho = rho0 (1 + rho_amp n0),
+ * u_a = u_amp c_s sqrt(3) n_a with n ~ N(0,1) from Philox-4x32-10 (key = seed,
+ * counter = node index) and Box-Muller; populations feq_N(rho, u). */
+int acp_case_init_random(acp_case* c, uint64_t seed, double rho_amp, double u_amp);
+/* Replace the populations of both arrays (host float64 [19][N]). */
+int acp_case_set_populations(acp_case* c, const double* f);
+
+/* Lattice::solve(bool&) (Lattice.cpp:2158-2183) n times.  Asynchronous on the
+ * case stream; errors of earlier work surface on the next synchronising call. */
+int acp_case_step(acp_case* c, int32_t n);
+/* Block until the case stream is idle. */
+int acp_case_sync(acp_case* c);
+
+/* Lattice::getDistribution(bool&, ...) (Lattice.h:119) for the whole current
+ * array: host float64 [19][N]. */
+int acp_case_get_populations(acp_case* c, double* f);
+/* Density and velocity of every node from the current array (Node.cpp:11-30,
+ * 85-118, without cubic-Mach terms): rho [N], u [3][N], float64. */
+int acp_case_get_macros(acp_case* c, double* rho, double* u);
+/* Total mass sum over all nodes of sum_q f_q, reduced in float64 on the device. */
+int acp_case_mass(acp_case* c, double* mass);
+/* Probes: nodes whose density (float64) is recorded on the device after every
+ * step; read back the recorded series [steps][nprobe] and clear it. */
+int acp_case_set_probes(acp_case* c, const int64_t* nodes, int32_t nprobe, int32_t capacity);
+int acp_case_read_probes(acp_case* c, double* out, int32_t* nsteps);
+
+/* Lattice::writeResultsVTK(string, bool&, bool&) (Lattice.cpp:4094-4349) for a
+ * single level: legacy ASCII VTK of Pressure, Density, Omega and Velocity. */
+int acp_case_write_vtk(acp_case* c, const char* filename);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,960 @@
+/*
+ * acp_oracle.c -- CPU restatement of the reference D3Q19 uniform-grid step.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP path.  Nothing in the
+ * product links or loads this file.
+ *
+ * Floating-point association follows the reference expressions term by term
+ * (left-to-right sums, the same grouping of products, no FMA) so that the
+ * results are bit-identical to the reference build; terms whose lattice
+ * factor is an exact zero are skipped, which cannot change an IEEE sum.
+ */
+#include "acp_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdio.h>
+
+/* D3Q19 in the reference order (Control.cpp:58-123), as integer unit vectors. */
+static const int E[19][3] = {
+    {1, 0, 0},  {0, 1, 0},  {-1, 0, 0}, {0, -1, 0}, {0, 0, 1},   {0, 0, -1}, {1, 0, 1},
+    {1, 0, -1}, {0, 1, 1},  {0, 1, -1}, {-1, 0, 1}, {-1, 0, -1}, {0, -1, 1}, {0, -1, -1},
+    {-1, 1, 0}, {1, -1, 0}, {1, 1, 0},  {-1, -1, 0}, {0, 0, 0}};
+
+/* ------------------------------------------------------------------ constants */
+
+void acpo__gs_matrix(int M[19][19], int pw[19]);
+
+static long gcdl(long a, long b) { if (a < 0) a = -a; if (b < 0) b = -b; while (b) { long t = a % b; a = b; b = t; } return a; }
+
+int acpo_consts_init(acpo_consts* k, const acpo_params* p) {
+    memset(k, 0, sizeof(*k));
+    const double cs = p->cs;
+    k->cs = cs;
+    k->c = cs * sqrt(3.);                                                  /* Control.cpp:52 */
+    k->dt = p->spacing / k->c;                                             /* Control.cpp:53 */
+    /* Lattice.cpp:1404 (level 0): the per-node collision frequency */
+    k->omega = (pow(cs, 2.) * (k->dt / pow(2., 0))) / (p->nu + 0.5 * pow(cs, 2.) * (k->dt / pow(2., 0)));
+    for (int q = 0; q < 19; ++q) {
+        k->w[q] = q < 6 ? 1. / 18. : (q < 18 ? 1. / 36. : 1. / 3.);          /* Control.cpp:126-131 */
+        for (int a = 0; a < 3; ++a) k->xi[q][a] = E[q][a] == 0 ? 0. : (E[q][a] > 0 ? k->c : -k->c);
+    }
+    k->cs2 = pow(cs, 2.);
+    k->rcs2 = 1. / k->cs2;
+    k->rcs4 = 1. / pow(cs, 4.);
+    k->rcs6 = 1. / pow(cs, 6.);
+    k->rcs8 = 1. / pow(cs, 8.);
+    const double o6 = 1. / 6., t7 = 2. / 7., t5 = 2. / 5.;
+    for (int q = 0; q < 19; ++q) {
+        const double x = k->xi[q][0], y = k->xi[q][1], z = k->xi[q][2];
+        k->hx[q] = x; k->hy[q] = y; k->hz[q] = z;
+        /* Node.cpp:1016-1021 */
+        const double xx = x * x - k->cs2, yy = y * y - k->cs2, zz = z * z - k->cs2;
+        k->hxx[q] = xx; k->hyy[q] = yy; k->hzz[q] = zz;
+        k->hxy[q] = x * y; k->hxz[q] = x * z; k->hyz[q] = y * z;
+        /* Node.cpp:2514-2516 */
+        k->pxx[q] = pow(x, 2.) - k->cs2; k->pyy[q] = pow(y, 2.) - k->cs2; k->pzz[q] = pow(z, 2.) - k->cs2;
+        /* Node.cpp:1024-1037 */
+        const double XXY = xx * y, XYY = yy * x, XXZ = xx * z, XZZ = zz * x, YYZ = yy * z, YZZ = zz * y;
+        k->h3[0][q] = XXY + YZZ; k->h3[1][q] = XXZ + YYZ; k->h3[2][q] = XYY + XZZ;
+        k->h3[3][q] = YZZ - XXY; k->h3[4][q] = YYZ - XXZ; k->h3[5][q] = XZZ - XYY;
+        /* Node.cpp:1040-1052 */
+        const double a = xx * yy + o6 * zz;
+        const double b = xx * zz + o6 * yy + t7 * a;
+        const double c = yy * zz + o6 * xx + t7 * a + t5 * b;
+        k->h4[0][q] = a; k->h4[1][q] = b; k->h4[2][q] = c;
+    }
+    k->rec_cs4_strain = 1. / (k->cs2 * k->cs2);
+    k->inv_pre = -((k->dt * k->cs2) / (k->omega * p->nu));
+    k->nu = p->nu; k->sigma = p->sigma; k->rho0 = p->rho0; k->spacing = p->spacing;
+    k->lodi_cs2 = pow(cs, 2.);
+    k->dom_len[0] = p->nx * p->spacing; k->dom_len[1] = p->ny * p->spacing; k->dom_len[2] = p->nz * p->spacing;
+    k->ma = p->mach; k->ma2 = p->mach * p->mach; k->k1 = p->k1;
+    k->c2 = pow(k->c, 2.0); k->c3 = pow(k->c, 3.0); k->c4 = pow(k->c, 4.0);
+    k->ix = 1. / k->c; k->ix2 = pow(k->ix, 2.); k->ix3 = pow(k->ix, 3.); k->ix4 = pow(k->ix, 4.);
+
+    /* Gram-Schmidt inverse: the basis of calcAllMoments_GSbasis (Node.cpp:2934-3004) is
+     * orthogonal under the lattice weights, so M^-1[q][k] = w_q M_k[q] / sum_q' w_q' M_k[q']^2;
+     * every entry reduces to +-1/den and carries the row's 1/c power. */
+    {
+        int M[19][19];
+        int pw[19];
+        acpo__gs_matrix(M, pw);
+        const long wn[19] = {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+        const long wd[19] = {18, 18, 18, 18, 18, 18, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36, 3};
+        const double xp[5] = {1.0, k->ix, k->ix2, k->ix3, k->ix4};
+        for (int r = 0; r < 19; ++r) {
+            long nn = 0, nd = 1; /* norm = sum w M^2 as a fraction */
+            for (int q = 0; q < 19; ++q) {
+                long tn = wn[q] * M[r][q] * M[r][q], td = wd[q];
+                nn = nn * td + tn * nd; nd = nd * td;
+                long g = gcdl(nn, nd); if (g) { nn /= g; nd /= g; }
+            }
+            for (int q = 0; q < 19; ++q) {
+                if (M[r][q] == 0) { k->gs_pre[r][q] = 0.; continue; }
+                long en = wn[q] * M[r][q] * nd, ed = wd[q] * nn;
+                long g = gcdl(en, ed); en /= g; ed /= g;
+                if (ed < 0) { ed = -ed; en = -en; }
+                if (en != 1 && en != -1) return -1;
+                double v = (double)en / (double)ed;
+                k->gs_pre[r][q] = r == 0 ? v : v * xp[pw[r]];
+            }
+        }
+    }
+    return 0;
+}
+
+/* --------------------------------------------------------- equilibria (a4) */
+
+static inline void feq_coeffs(const double u[3], double a2[6], double a3[6], double a4[3]) {
+    const double ux = u[0], uy = u[1], uz = u[2];
+    /* Node.cpp:1001-1008 */
+    a2[0] = ux * ux; a2[1] = uy * uy; a2[2] = uz * uz;
+    a2[3] = ux * uy; a2[4] = ux * uz; a2[5] = uy * uz;
+    /* Node.cpp:1010-1024 */
+    const double XXY = uy * a2[0], XYY = ux * a2[1], XXZ = uz * a2[0], XZZ = ux * a2[2], YYZ = uz * a2[1], YZZ = uy * a2[2];
+    a3[0] = XXY + YZZ; a3[1] = XXZ + YYZ; a3[2] = XYY + XZZ;
+    a3[3] = YZZ - XXY; a3[4] = YYZ - XXZ; a3[5] = XZZ - XYY;
+    /* Node.cpp:1026-1039 */
+    const double o6 = 1. / 6., t7 = 2. / 7., t5 = 2. / 5.;
+    const double XXYY = uy * XXY, XXZZ = uz * XXZ, YYZZ = uy * YZZ;
+    const double A = XXYY + o6 * a2[2];
+    const double B = XXZZ + o6 * a2[1] + t7 * A;
+    const double C = YYZZ + o6 * a2[0] + t7 * A + t5 * B;
+    a4[0] = A; a4[1] = B; a4[2] = C;
+}
+
+/* Node.cpp:739-813 (order 2), 816-949 (order 3), 952-1132 (order 4) */
+void acpo_feq(const acpo_consts* k, int order, double rho, const double u[3], double out[19]) {
+    double a2[6], a3[6], a4[3];
+    feq_coeffs(u, a2, a3, a4);
+    const double c2 = 1. / 2. * k->rcs4, c3 = 1. / 6. * k->rcs6, c4 = 1. / 4. * k->rcs8;
+    const double e7 = 8. / 7., e45 = 56. / 45., e27 = 40. / 27.;
+    for (int q = 0; q < 19; ++q) {
+        double s1 = 0.0;
+        int first = 1;
+        if (E[q][0]) { s1 = k->hx[q] * u[0]; first = 0; }
+        if (E[q][1]) { s1 = first ? k->hy[q] * u[1] : s1 + k->hy[q] * u[1]; first = 0; }
+        if (E[q][2]) { s1 = first ? k->hz[q] * u[2] : s1 + k->hz[q] * u[2]; first = 0; }
+        double s2 = k->hxx[q] * a2[0] + k->hyy[q] * a2[1] + k->hzz[q] * a2[2];
+        if (E[q][0] && E[q][1]) s2 = s2 + k->hxy[q] * a2[3] * 2.;
+        if (E[q][0] && E[q][2]) s2 = s2 + k->hxz[q] * a2[4] * 2.;
+        if (E[q][1] && E[q][2]) s2 = s2 + k->hyz[q] * a2[5] * 2.;
+        double x = 1. * 1. + k->rcs2 * s1 + c2 * s2;
+        if (order >= 3) {
+            double s3 = k->h3[0][q] * a3[0] * 3. + k->h3[1][q] * a3[1] * 3. + k->h3[2][q] * a3[2] * 3. +
+                        k->h3[3][q] * a3[3] + k->h3[4][q] * a3[4] + k->h3[5][q] * a3[5];
+            x = x + c3 * s3;
+        }
+        if (order >= 4) {
+            double s4 = e7 * k->h4[0][q] * a4[0] + e45 * k->h4[1][q] * a4[1] + e27 * k->h4[2][q] * a4[2];
+            x = x + c4 * s4;
+        }
+        out[q] = rho * k->w[q] * x;
+    }
+}
+
+/* Node.cpp:11-30 and 85-118: density and velocity, with the 0.5*dt*psi terms when psi != NULL */
+void acpo_moments(const acpo_consts* k, const double f[19], const double* psi, double* rho, double u[3]) {
+    double r = 0., vx = 0., vy = 0., vz = 0.;
+    if (!psi) {
+        for (int q = 0; q < 19; ++q) r += f[q];
+        for (int q = 0; q < 19; ++q) {
+            if (E[q][0]) vx += k->xi[q][0] * f[q];
+            if (E[q][1]) vy += k->xi[q][1] * f[q];
+            if (E[q][2]) vz += k->xi[q][2] * f[q];
+        }
+    } else {
+        const double h = 0.5 * k->dt;
+        for (int q = 0; q < 19; ++q) r += f[q] + h * psi[q];
+        for (int q = 0; q < 19; ++q) {
+            vx += k->xi[q][0] * f[q] + h * k->xi[q][0] * psi[q];
+            vy += k->xi[q][1] * f[q] + h * k->xi[q][1] * psi[q];
+            vz += k->xi[q][2] * f[q] + h * k->xi[q][2] * psi[q];
+        }
+    }
+    *rho = r;
+    u[0] = vx / r; u[1] = vy / r; u[2] = vz / r;
+}
+
+/* Node.cpp:2493-2558; pi = {xx, yy, zz, xy, xz, yz} */
+void acpo_pi1(const acpo_consts* k, const double f[19], const double feq[19], const double* psi, double pi[6]) {
+    double xx = 0., yy = 0., zz = 0., xy = 0., xz = 0., yz = 0.;
+    const double h = 0.5 * k->dt;
+    for (int q = 0; q < 19; ++q) {
+        const double d = psi ? f[q] - feq[q] + h * psi[q] : f[q] - feq[q];
+        xx += k->pxx[q] * d; yy += k->pyy[q] * d; zz += k->pzz[q] * d;
+        if (E[q][0] && E[q][1]) xy += k->hxy[q] * d;
+        if (E[q][0] && E[q][2]) xz += k->hxz[q] * d;
+        if (E[q][1] && E[q][2]) yz += k->hyz[q] * d;
+    }
+    pi[0] = xx; pi[1] = yy; pi[2] = zz; pi[3] = xy; pi[4] = xz; pi[5] = yz;
+}
+
+/* Recursive off-equilibrium from second-order coefficients a2 = {xx,yy,zz,xy,xz,yz}:
+ * calcRRoffEquilibrium{2,3,4}_GHbasis (Node.cpp:1136-1427) with a2 = Pi1, and
+ * calcHRRoffEquilibrium{2,3,4}_GHbasis (Node.cpp:1430-1747) with a2 = the hybrid blend. */
+void acpo_offeq(const acpo_consts* k, int order, const double a2[6], const double u[3], double out[19]) {
+    const double XX = a2[0], YY = a2[1], ZZ = a2[2], XY = a2[3], XZ = a2[4], YZ = a2[5];
+    const double ux = u[0], uy = u[1], uz = u[2];
+    /* Node.cpp:1217-1230 */
+    const double tXXY = 2. * ux * XY + uy * XX, tXYY = 2. * uy * XY + ux * YY, tXXZ = 2. * ux * XZ + uz * XX;
+    const double tXZZ = 2. * uz * XZ + ux * ZZ, tYYZ = 2. * uy * YZ + uz * YY, tYZZ = 2. * uz * YZ + uy * ZZ;
+    const double b3[6] = {tXXY + tYZZ, tXXZ + tYYZ, tXYY + tXZZ, tYZZ - tXXY, tYYZ - tXXZ, tXZZ - tXYY};
+    /* Node.cpp:1345-1357 */
+    const double o6 = 1. / 6., t7 = 2. / 7., t5 = 2. / 5.;
+    const double fXXYY = uy * uy * XX + ux * ux * YY + 4. * ux * uy * XY;
+    const double fXXZZ = uz * uz * XX + ux * ux * ZZ + 4. * ux * uz * XZ;
+    const double fYYZZ = uy * uy * ZZ + uz * uz * YY + 4. * uy * uz * YZ;
+    const double A = fXXYY + o6 * ZZ;
+    const double B = fXXZZ + o6 * YY + t7 * A;
+    const double C = fYYZZ + o6 * XX + t7 * A + t5 * B;
+    const double c2 = 1. / 2. * k->rcs4, c3 = 1. / 6. * k->rcs6, c4 = 1. / 4. * k->rcs8;
+    const double e7 = 8. / 7., e45 = 56. / 45., e27 = 40. / 27.;
+    for (int q = 0; q < 19; ++q) {
+        double s2 = k->hxx[q] * XX + k->hyy[q] * YY + k->hzz[q] * ZZ;
+        if (E[q][0] && E[q][1]) s2 = s2 + k->hxy[q] * XY * 2.;
+        if (E[q][0] && E[q][2]) s2 = s2 + k->hxz[q] * XZ * 2.;
+        if (E[q][1] && E[q][2]) s2 = s2 + k->hyz[q] * YZ * 2.;
+        double x = c2 * s2;
+        if (order >= 3) {
+            double s3 = k->h3[0][q] * b3[0] * 3. + k->h3[1][q] * b3[1] * 3. + k->h3[2][q] * b3[2] * 3. +
+                        k->h3[3][q] * b3[3] + k->h3[4][q] * b3[4] + k->h3[5][q] * b3[5];
+            x = x + c3 * s3;
+        }
+        if (order >= 4) {
+            double s4 = e7 * k->h4[0][q] * A + e45 * k->h4[1][q] * B + e27 * k->h4[2][q] * C;
+            x = x + c4 * s4;
+        }
+        out[q] = k->w[q] * x;
+    }
+}
+
+/* Node.cpp:1750-1889.  nb_macro[m] = pre-collision (rho,ux,uy,uz) of neighbour m in
+ * directions 0..5 (+x,+y,-x,-y,+z,-z).  S = {xx,yy,zz,xy,xz,yz}. */
+void acpo_strain_psi(const acpo_consts* k, const double nb[6][4], double S[6], double psi[19]) {
+    double P[10][6];
+    double d[6];
+    for (int m = 0; m < 6; ++m) {
+        const double r = nb[m][0], vx = nb[m][1], vy = nb[m][2], vz = nb[m][3];
+        P[0][m] = r * vx * vx * vx;        /* XXX */
+        P[1][m] = 0.5 * r * vy * vz * vz; /* XXY */
+        P[2][m] = 0.5 * r * vz * vy * vy; /* XXZ */
+        P[3][m] = 0.5 * r * vx * vz * vz; /* XYY */
+        P[4][m] = r * vy * vy * vy;        /* YYY */
+        P[5][m] = 0.5 * r * vz * vx * vx; /* YYZ */
+        P[6][m] = 0.5 * r * vx * vy * vy; /* XZZ */
+        P[7][m] = 0.5 * r * vy * vx * vx; /* YZZ */
+        P[8][m] = r * vz * vz * vz;        /* ZZZ */
+        P[9][m] = r * vx * vy * vz;        /* XYZ */
+        d[m] = k->spacing / pow(2., 0);
+    }
+    const double dx = d[0] + d[2], dy = d[1] + d[3], dz = d[4] + d[5];
+    enum { XXX, XXY, XXZ, XYY, YYY, YYZ, XZZ, YZZ, ZZZ, XYZ };
+    const double XXX_dx = (P[XXX][0] - P[XXX][2]) / dx, XYY_dx = (P[XYY][0] - P[XYY][2]) / dx;
+    const double XZZ_dx = (P[XZZ][0] - P[XZZ][2]) / dx, XXY_dx = (P[XXY][0] - P[XXY][2]) / dx;
+    const double XXZ_dx = (P[XXZ][0] - P[XXZ][2]) / dx, XYZ_dx = (P[XYZ][0] - P[XYZ][2]) / dx;
+    const double XXY_dy = (P[XXY][1] - P[XXY][3]) / dy, YYY_dy = (P[YYY][1] - P[YYY][3]) / dy;
+    const double YZZ_dy = (P[YZZ][1] - P[YZZ][3]) / dy, XYY_dy = (P[XYY][1] - P[XYY][3]) / dy;
+    const double XYZ_dy = (P[XYZ][1] - P[XYZ][3]) / dy, YYZ_dy = (P[YYZ][1] - P[YYZ][3]) / dy;
+    const double XXZ_dz = (P[XXZ][4] - P[XXZ][5]) / dz, YYZ_dz = (P[YYZ][4] - P[YYZ][5]) / dz;
+    const double ZZZ_dz = (P[ZZZ][4] - P[ZZZ][5]) / dz, XYZ_dz = (P[XYZ][4] - P[XYZ][5]) / dz;
+    const double XZZ_dz = (P[XZZ][4] - P[XZZ][5]) / dz, YZZ_dz = (P[YZZ][4] - P[YZZ][5]) / dz;
+    const double gxx = XXX_dx + XXY_dy + XXZ_dz, gyy = XYY_dx + YYY_dy + YYZ_dz, gzz = XZZ_dx + YZZ_dy + ZZZ_dz;
+    const double gxy = XXY_dx + XYY_dy + XYZ_dz, gxz = XXZ_dx + XYZ_dy + XZZ_dz, gyz = XYZ_dx + YYZ_dy + YZZ_dz;
+    for (int q = 0; q < 19; ++q) {
+        double s = k->pxx[q] * gxx + k->pyy[q] * gyy + k->pzz[q] * gzz;
+        if (E[q][0] && E[q][1]) s = s + k->hxy[q] * gxy * 2.;
+        if (E[q][0] && E[q][2]) s = s + k->hxz[q] * gxz * 2.;
+        if (E[q][1] && E[q][2]) s = s + k->hyz[q] * gyz * 2.;
+        psi[q] = 0.5 * k->w[q] * k->rec_cs4_strain * s;
+    }
+    S[0] = (nb[0][1] - nb[2][1]) / dx;
+    S[1] = (nb[1][2] - nb[3][2]) / dy;
+    S[2] = (nb[4][3] - nb[5][3]) / dz;
+    S[3] = 0.5 * ((nb[0][2] - nb[2][2]) / dx + (nb[1][1] - nb[3][1]) / dy);
+    S[4] = 0.5 * ((nb[0][3] - nb[2][3]) / dx + (nb[4][1] - nb[5][1]) / dz);
+    S[5] = 0.5 * ((nb[1][3] - nb[3][3]) / dy + (nb[4][2] - nb[5][2]) / dz);
+}
+
+/* ------------------------------------------------------------- MRT pieces */
+
+/* Gram-Schmidt moment rows (Node.cpp:2934-3004) in lattice units and the power of c
+ * each row carries. */
+void acpo__gs_matrix(int M[19][19], int pw[19]) {
+    memset(M, 0, sizeof(int) * 19 * 19);
+    for (int q = 0; q < 19; ++q) {
+        const int x = E[q][0], y = E[q][1], z = E[q][2];
+        const int e2 = x * x + y * y + z * z;
+        M[0][q] = 1;
+        M[1][q] = e2 == 2 ? 1 : (e2 == 0 ? -1 : 0);
+        M[2][q] = e2 == 1 ? -2 : 1;
+        const int axis = e2 == 1 ? -2 : 1;
+        M[3][q] = x; M[4][q] = axis * x;
+        M[5][q] = y; M[6][q] = axis * y;
+        M[7][q] = z; M[8][q] = axis * z;
+        M[9][q] = 3 * x * x - e2;
+        M[10][q] = e2 == 1 ? -M[9][q] : (e2 == 2 ? M[9][q] : 0);
+        M[11][q] = y * y - z * z;
+        M[12][q] = e2 == 1 ? -M[11][q] : (e2 == 2 ? M[11][q] : 0);
+        M[13][q] = x * y; M[14][q] = y * z; M[15][q] = x * z;
+        M[16][q] = x * (y * y - z * z);
+        M[17][q] = y * (z * z - x * x);
+        M[18][q] = z * (x * x - y * y);
+    }
+    const int p[19] = {0, 2, 4, 1, 3, 1, 3, 1, 3, 2, 4, 2, 4, 2, 2, 2, 3, 3, 3};
+    memcpy(pw, p, sizeof(p));
+}
+
+void acpo_gs_moments(const acpo_consts* k, const double f[19], double m[19]) {
+    int M[19][19], pw[19];
+    acpo__gs_matrix(M, pw);
+    const double cp[5] = {1.0, k->c, pow(k->c, 2.), pow(k->c, 3.), pow(k->c, 4.)};
+    for (int r = 0; r < 19; ++r) {
+        double s = 0.;
+        for (int q = 0; q < 19; ++q) {
+            const int a = M[r][q];
+            if (a == 1) s += f[q];
+            else if (a == -1) s -= f[q];
+            else if (a == 2) s += 2. * f[q];
+            else if (a == -2) s -= 2. * f[q];
+        }
+        m[r] = r == 0 ? s : s * cp[pw[r]];
+    }
+}
+
+/* Node.cpp:2838-2867 */
+void acpo_gs_eqmoments(double rho, double jx, double jy, double jz, double meq[19]) {
+    memset(meq, 0, sizeof(double) * 19);
+    meq[0] = rho; meq[3] = jx; meq[5] = jy; meq[7] = jz;
+    const double ir = 1. / rho, xx = jx * jx, yy = jy * jy, zz = jz * jz;
+    meq[1] = (xx + yy + zz) * ir;
+    meq[9] = (2. * xx - yy - zz) * ir;
+    meq[11] = (yy - zz) * ir;
+    meq[13] = jx * jy * ir;
+    meq[14] = jy * jz * ir;
+    meq[15] = jx * jz * ir;
+}
+
+/* Node.cpp:2870-2931 */
+void acpo_gs_inverse(const acpo_consts* k, const double m[19], double f[19]) {
+    for (int q = 0; q < 19; ++q) {
+        double s = 0.;
+        int first = 1;
+        for (int r = 0; r < 19; ++r) {
+            if (k->gs_pre[r][q] == 0.) continue;
+            const double t = k->gs_pre[r][q] * m[r];
+            s = first ? t : s + t;
+            first = 0;
+        }
+        f[q] = s;
+    }
+}
+
+/* raw-moment rows (Node.cpp:2693-2762): exponents (a,b,c) of M_abc, trace and N moments special */
+static void rm_row(int r, int q, int* coef, int* pw) {
+    const int x = E[q][0], y = E[q][1], z = E[q][2];
+    const int X = x * x, Y = y * y, Z = z * z;
+    switch (r) {
+        case 0: *coef = 1; *pw = 0; break;
+        case 1: *coef = x; *pw = 1; break;
+        case 2: *coef = y; *pw = 1; break;
+        case 3: *coef = z; *pw = 1; break;
+        case 4: *coef = X + Y + Z; *pw = 2; break;
+        case 5: *coef = X - Z; *pw = 2; break;
+        case 6: *coef = Y - Z; *pw = 2; break;
+        case 7: *coef = x * y; *pw = 2; break;
+        case 8: *coef = x * z; *pw = 2; break;
+        case 9: *coef = y * z; *pw = 2; break;
+        case 10: *coef = X * z; *pw = 3; break;
+        case 11: *coef = x * Z; *pw = 3; break;
+        case 12: *coef = Y * z; *pw = 3; break;
+        case 13: *coef = y * Z; *pw = 3; break;
+        case 14: *coef = X * y; *pw = 3; break;
+        case 15: *coef = x * Y; *pw = 3; break;
+        case 16: *coef = X * Z; *pw = 4; break;
+        case 17: *coef = X * Y; *pw = 4; break;
+        default: *coef = Y * Z; *pw = 4; break;
+    }
+}
+
+void acpo_rm_moments(const acpo_consts* k, const double f[19], double m[19]) {
+    const double cp[5] = {1.0, k->c, k->c2, k->c3, k->c4};
+    for (int r = 0; r < 19; ++r) {
+        double s = 0.;
+        int pw = 0;
+        if (r == 4) {
+            /* Node.cpp:2713-2718: axis links, then twice the diagonal links */
+            for (int q = 0; q < 6; ++q) s += f[q];
+            double t = 0.;
+            for (int q = 6; q < 18; ++q) t += f[q];
+            s += 2. * t;
+            pw = 2;
+        } else {
+            for (int q = 0; q < 19; ++q) {
+                int a;
+                rm_row(r, q, &a, &pw);
+                if (a == 1) s += f[q];
+                else if (a == -1) s -= f[q];
+            }
+        }
+        m[r] = r == 0 ? s : s * cp[pw];
+    }
+}
+
+/* Coreixas' extended raw-moment equilibrium (Node.cpp:2650-2690), written per link class:
+ * rest, axis link along a (sign s, transverse axes b,c) and diagonal link in (a,b). */
+void acpo_rm_feq4(const acpo_consts* k, double rho, const double u[3], double out[19]) {
+    const double x1 = k->ix, x2 = pow(k->ix, 2.), x3 = pow(k->ix, 3.), x4 = pow(k->ix, 4.);
+    const double S[3] = {u[0] * u[0], u[1] * u[1], u[2] * u[2]};
+    for (int q = 0; q < 19; ++q) {
+        int ax[2], sg[2], n = 0;
+        for (int a = 0; a < 3; ++a)
+            if (E[q][a]) { ax[n] = a; sg[n] = E[q][a]; ++n; }
+        double t;
+        if (n == 0) {
+            t = 1 - (S[0] + S[1] + S[2]) * x2 + 3 * (S[0] * S[1] + S[0] * S[2] + S[1] * S[2]) * x4;
+        } else if (n == 1) {
+            const int a = ax[0], b = (a == 0) ? 1 : 0, c = (a == 2) ? 1 : 2;
+            const double ua = sg[0] * u[a];
+            t = 1 + 3 * ua * x1 + 3 * (S[a] - S[b] - S[c]) * x2 - 9 * ua * (S[b] + S[c]) * x3 - 9 * S[a] * (S[b] + S[c]) * x4;
+        } else {
+            const int a = ax[0], b = ax[1];
+            const double ua = sg[0] * u[a], ub = sg[1] * u[b];
+            t = 1 + 3 * (ua + ub) * x1 + 3 * (S[a] + S[b]) * x2 + 9 * ua * ub * x2 + 9 * (S[a] * ub + ua * S[b]) * x3 +
+                9 * S[a] * S[b] * x4;
+        }
+        out[q] = rho * k->w[q] * t;
+    }
+}
+
+/* Inverse of the raw-moment transform (Node.cpp:2765-2795), solved per link from the
+ * moment definitions: rest, axis and diagonal links each see a fixed set of moments. */
+void acpo_rm_inverse(const acpo_consts* k, const double m[19], double f[19]) {
+    const double x1 = k->ix, x2 = k->ix2, x3 = k->ix3, x4 = k->ix4;
+    /* axis links: f = w-free combination of M_a, trace/N second moments, M_a(bb+cc), M_aabb */
+    const double o6 = 1. / 6.;
+    /* second moments along each axis, (1/6)(T + 2 N_a - ...) with T the trace and N the normal differences */
+    const double Pxx = o6 * (m[4] + 2. * m[5] - m[6]), Pyy = o6 * (m[4] - m[5] + 2. * m[6]), Pzz = o6 * (m[4] - m[5] - m[6]);
+    f[0] = 0.5 * m[1] * x1 + Pxx * x2 - 0.5 * (m[11] + m[15]) * x3 - 0.5 * (m[16] + m[17]) * x4;
+    f[2] = -0.5 * m[1] * x1 + Pxx * x2 + 0.5 * (m[11] + m[15]) * x3 - 0.5 * (m[16] + m[17]) * x4;
+    f[1] = 0.5 * m[2] * x1 + Pyy * x2 - 0.5 * (m[13] + m[14]) * x3 - 0.5 * (m[17] + m[18]) * x4;
+    f[3] = -0.5 * m[2] * x1 + Pyy * x2 + 0.5 * (m[13] + m[14]) * x3 - 0.5 * (m[17] + m[18]) * x4;
+    f[4] = 0.5 * m[3] * x1 + Pzz * x2 - 0.5 * (m[10] + m[12]) * x3 - 0.5 * (m[16] + m[18]) * x4;
+    f[5] = -0.5 * m[3] * x1 + Pzz * x2 + 0.5 * (m[10] + m[12]) * x3 - 0.5 * (m[16] + m[18]) * x4;
+    /* diagonal links in the (a,b) plane: M_ab, M_aab, M_abb, M_aabb */
+    struct { int q, sa, sb, mab, maab, mabb, m4; } D[12] = {
+        {6, 1, 1, 8, 10, 11, 16},   {7, 1, -1, 8, 10, 11, 16},   {10, -1, 1, 8, 10, 11, 16}, {11, -1, -1, 8, 10, 11, 16},
+        {8, 1, 1, 9, 12, 13, 18},   {9, 1, -1, 9, 12, 13, 18},   {12, -1, 1, 9, 12, 13, 18}, {13, -1, -1, 9, 12, 13, 18},
+        {16, 1, 1, 7, 14, 15, 17},  {15, 1, -1, 7, 14, 15, 17},  {14, -1, 1, 7, 14, 15, 17}, {17, -1, -1, 7, 14, 15, 17}};
+    for (int i = 0; i < 12; ++i) {
+        const double sab = (double)(D[i].sa * D[i].sb);
+        /* M_aab carries the sign of b, M_abb the sign of a */
+        f[D[i].q] = 0.25 * sab * m[D[i].mab] * x2 + 0.25 * (D[i].sb * m[D[i].maab] + D[i].sa * m[D[i].mabb]) * x3 +
+                    0.25 * m[D[i].m4] * x4;
+    }
+    f[18] = m[0] - m[4] * x2 + (m[16] + m[17] + m[18]) * x4;
+}
+
+/* ------------------------------------------------------------- collisions */
+
+void acpo_collide(const acpo_consts* k, const acpo_params* p, double f[19], double* psi, const double nb[6][4]) {
+    const int cubic = p->cubic;
+    const double h = 0.5 * k->dt;
+    const double om = k->omega;
+    switch (p->op) {
+        case ACPO_SRT: { /* Node.cpp:511-557 */
+            double rho, u[3], eq[19];
+            acpo_moments(k, f, cubic ? psi : NULL, &rho, u);
+            acpo_feq(k, p->order, rho, u, eq);
+            for (int q = 0; q < 19; ++q) f[q] += om * (eq[q] - f[q]);
+            break;
+        }
+        case ACPO_RR:
+        case ACPO_HRR: { /* Node.cpp:560-650, 653-736 */
+            double rho, u[3], eq[19], a2[6], f1[19];
+            acpo_moments(k, f, cubic ? psi : NULL, &rho, u);
+            acpo_feq(k, p->order, rho, u, eq);
+            double S[6] = {0};
+            if (p->op == ACPO_HRR || cubic) {
+                double ps[19];
+                acpo_strain_psi(k, nb, S, ps);
+                if (psi) memcpy(psi, ps, sizeof(ps));
+            }
+            acpo_pi1(k, f, eq, cubic ? psi : NULL, a2);
+            if (p->op == ACPO_HRR) {
+                double dens, uu[3];
+                acpo_moments(k, f, cubic ? psi : NULL, &dens, uu); /* Node.cpp:1444 */
+                (void)uu;
+                const double dyn = k->nu * dens;
+                for (int i = 0; i < 6; ++i) a2[i] = k->sigma * a2[i] + (1. - k->sigma) * k->inv_pre * (2. * dyn * S[i]);
+            }
+            acpo_offeq(k, p->order, a2, u, f1);
+            if (cubic)
+                for (int q = 0; q < 19; ++q) f[q] = eq[q] + (1 - om) * f1[q] + h * psi[q];
+            else
+                for (int q = 0; q < 19; ++q) f[q] = eq[q] + (1 - om) * f1[q];
+            break;
+        }
+        case ACPO_MRT_RM: { /* Node.cpp:2565-2605; the reference discards the result */
+            if (!p->mrt_mode) break;
+            double m[19], eq[19], meq[19], u[3];
+            acpo_rm_moments(k, f, m);
+            u[0] = m[1] / m[0]; u[1] = m[2] / m[0]; u[2] = m[3] / m[0];
+            acpo_rm_feq4(k, m[0], u, eq);
+            acpo_rm_moments(k, eq, meq);
+            for (int r = 0; r < 19; ++r) {
+                const double rate = (r >= 5 && r <= 9) ? om : 1.;
+                m[r] += rate * (meq[r] - m[r]);
+            }
+            acpo_rm_inverse(k, m, f);
+            break;
+        }
+        case ACPO_MRT_GS: { /* Node.cpp:2798-2835; the reference discards the result */
+            if (!p->mrt_mode) break;
+            double m[19], meq[19], dm[19], df[19];
+            acpo_gs_moments(k, f, m);
+            acpo_gs_eqmoments(m[0], m[3], m[5], m[7], meq);
+            for (int r = 0; r < 19; ++r) {
+                double rate = 1.;
+                if (r == 0 || r == 3 || r == 5 || r == 7) rate = 0.;
+                if (r == 1 || r == 9 || r == 11 || r == 13 || r == 14 || r == 15) rate = om;
+                dm[r] = rate == 0. ? 0. : rate * (meq[r] - m[r]);
+            }
+            acpo_gs_inverse(k, dm, df);
+            for (int q = 0; q < 19; ++q) f[q] += df[q];
+            break;
+        }
+        default: break;
+    }
+}
+
+/* ------------------------------------------------------------- lattice */
+
+struct acpo_case {
+    acpo_params p;
+    acpo_consts k;
+    int64_t N;
+    double *A, *B, *psi, *macro;
+    uint32_t* mask;      /* push mask per node, bit q (0..18) */
+    uint8_t* interior;   /* 1 for nodes that collide */
+    /* LODI metadata in reference order */
+    int64_t nl;
+    int64_t *l_node, *l_n1, *l_n2, *l_t[4];
+    int32_t *l_dir, *l_corner, *l_phase;
+    int32_t max_phase;
+    int64_t* ph_start;  /* [max_phase + 1] offsets into ph_order */
+    int64_t* ph_order;  /* LODI slots sorted by phase, reference order within a phase */
+};
+
+static inline int64_t lin(const acpo_case* c, int i, int j, int k) {
+    return (int64_t)i + (int64_t)c->p.nx * ((int64_t)j + (int64_t)c->p.ny * k);
+}
+
+typedef struct { int n1[3], n2[3], t[4][3], dir, corner, has_t; } lodi_geo;
+
+/* Normal / second-normal / tangent neighbours, normal direction and corner flag of a
+ * boundary node, as Lattice::setConnectivity assigns them (Lattice.cpp:500-1392). */
+static void boundary_geo(int nx, int ny, int nz, int i, int j, int k, lodi_geo* g) {
+    const int im = nx - 1, jm = ny - 1, km = nz - 1;
+    memset(g, 0, sizeof(*g));
+    const int bi = (i == 0) ? -1 : (i == im ? 1 : 0);
+    const int bj = (j == 0) ? -1 : (j == jm ? 1 : 0);
+    const int bk = (k == 0) ? -1 : (k == km ? 1 : 0);
+    const int nb = (bi != 0) + (bj != 0) + (bk != 0);
+#define SET3(v, a, b, c) do { (v)[0] = (a); (v)[1] = (b); (v)[2] = (c); } while (0)
+    if (nb == 1) {
+        g->corner = 0; g->has_t = 1;
+        if (bk) { /* z faces (Lattice.cpp:500-596) */
+            g->dir = bk < 0 ? 4 : 5;
+            SET3(g->n1, i, j, k - bk); SET3(g->n2, i, j, k - 2 * bk);
+            SET3(g->t[0], i + 1, j, k); SET3(g->t[1], i - 1, j, k); SET3(g->t[2], i, j + 1, k); SET3(g->t[3], i, j - 1, k);
+        } else if (bj) { /* y faces (Lattice.cpp:598-690) */
+            g->dir = bj < 0 ? 1 : 3;
+            SET3(g->n1, i, j - bj, k); SET3(g->n2, i, j - 2 * bj, k);
+            SET3(g->t[0], i + 1, j, k); SET3(g->t[1], i - 1, j, k); SET3(g->t[2], i, j, k + 1); SET3(g->t[3], i, j, k - 1);
+        } else { /* x faces (Lattice.cpp:826-908) */
+            g->dir = bi < 0 ? 0 : 2;
+            SET3(g->n1, i - bi, j, k); SET3(g->n2, i - 2 * bi, j, k);
+            SET3(g->t[0], i, j + 1, k); SET3(g->t[1], i, j - 1, k); SET3(g->t[2], i, j, k + 1); SET3(g->t[3], i, j, k - 1);
+        }
+        return;
+    }
+    g->corner = 1; g->has_t = 0;
+    if (bi == 0) { /* edges along x (Lattice.cpp:692-824): normal along the inward yz diagonal */
+        g->dir = bj < 0 ? 1 : 3;
+        SET3(g->n1, i, j - bj, k - bk); SET3(g->n2, i, j - 2 * bj, k - 2 * bk);
+        return;
+    }
+    if (bi < 0) { /* inlet edges and corners (Lattice.cpp:910-1160) */
+        g->dir = 0;
+        if (nb == 3) { SET3(g->n1, 1, j, k); SET3(g->n2, 2, j, k); }
+        else { SET3(g->n1, 1, j - bj, k - bk); SET3(g->n2, 2, j - 2 * bj, k - 2 * bk); }
+        return;
+    }
+    /* outlet edges and corners (Lattice.cpp:1162-1392) */
+    if (bk) g->dir = bk < 0 ? 4 : 5;
+    else g->dir = bj < 0 ? 1 : 3;
+    SET3(g->n1, im - 1, j - bj, k - bk); SET3(g->n2, im - 2, j - 2 * bj, k - 2 * bk);
+#undef SET3
+}
+
+static int is_boundary(const acpo_params* p, int i, int j, int k) {
+    return i == 0 || j == 0 || k == 0 || i == p->nx - 1 || j == p->ny - 1 || k == p->nz - 1;
+}
+
+/* Push mask (Lattice.cpp:244-471 interior; 500-1392 boundary): interior nodes push all 18
+ * links; a boundary node pushes exactly the links that point inward in every dimension in
+ * which it lies on the boundary. */
+static uint32_t push_mask(const acpo_params* p, int i, int j, int k) {
+    if (p->periodic || !is_boundary(p, i, j, k)) return 0x7FFFFu;
+    const int b[3] = {i == 0 ? 1 : (i == p->nx - 1 ? -1 : 0), j == 0 ? 1 : (j == p->ny - 1 ? -1 : 0),
+                      k == 0 ? 1 : (k == p->nz - 1 ? -1 : 0)};
+    uint32_t m = 1u << 18;
+    for (int q = 0; q < 18; ++q) {
+        int ok = 1;
+        for (int a = 0; a < 3; ++a)
+            if (b[a] != 0 && E[q][a] != b[a]) ok = 0;
+        if (ok) m |= 1u << q;
+    }
+    return m;
+}
+
+acpo_case* acpo_create(const acpo_params* p) {
+    if (p->nx < 3 || p->ny < 3 || p->nz < 3) return NULL;
+    if (p->order < 2 || p->order > 4) return NULL;
+    acpo_case* c = (acpo_case*)calloc(1, sizeof(acpo_case));
+    c->p = *p;
+    if (acpo_consts_init(&c->k, p) != 0) { free(c); return NULL; }
+    const int64_t N = (int64_t)p->nx * p->ny * p->nz;
+    c->N = N;
+    c->A = (double*)calloc((size_t)19 * N, sizeof(double));
+    c->B = (double*)calloc((size_t)19 * N, sizeof(double));
+    c->psi = (double*)calloc((size_t)19 * N, sizeof(double));
+    c->macro = (double*)calloc((size_t)4 * N, sizeof(double));
+    c->mask = (uint32_t*)calloc((size_t)N, sizeof(uint32_t));
+    c->interior = (uint8_t*)calloc((size_t)N, 1);
+    for (int kk = 0; kk < p->nz; ++kk)
+        for (int jj = 0; jj < p->ny; ++jj)
+            for (int ii = 0; ii < p->nx; ++ii) {
+                const int64_t l = lin(c, ii, jj, kk);
+                c->mask[l] = push_mask(p, ii, jj, kk);
+                c->interior[l] = (uint8_t)(p->periodic || !is_boundary(p, ii, jj, kk));
+            }
+    if (!p->periodic) {
+        /* boundary nodes in the order of Lattice::timeSteppingSRT's LODI loops
+         * (Lattice.cpp:1582-1624): x faces, then z faces, then y faces */
+        const int im = p->nx - 1, jm = p->ny - 1, km = p->nz - 1;
+        int64_t cap = 2 * ((int64_t)p->ny * p->nz + (int64_t)p->nx * p->ny + (int64_t)p->nx * p->nz);
+        c->l_node = (int64_t*)calloc(cap, sizeof(int64_t));
+        c->l_n1 = (int64_t*)calloc(cap, sizeof(int64_t));
+        c->l_n2 = (int64_t*)calloc(cap, sizeof(int64_t));
+        for (int t = 0; t < 4; ++t) c->l_t[t] = (int64_t*)calloc(cap, sizeof(int64_t));
+        c->l_dir = (int32_t*)calloc(cap, sizeof(int32_t));
+        c->l_corner = (int32_t*)calloc(cap, sizeof(int32_t));
+        c->l_phase = (int32_t*)calloc(cap, sizeof(int32_t));
+        int64_t n = 0;
+#define ADD(I, J, K)                                                                              \
+    do {                                                                                          \
+        lodi_geo g;                                                                               \
+        boundary_geo(p->nx, p->ny, p->nz, (I), (J), (K), &g);                                     \
+        c->l_node[n] = lin(c, (I), (J), (K));                                                     \
+        c->l_n1[n] = lin(c, g.n1[0], g.n1[1], g.n1[2]);                                           \
+        c->l_n2[n] = lin(c, g.n2[0], g.n2[1], g.n2[2]);                                           \
+        for (int t = 0; t < 4; ++t) c->l_t[t][n] = g.has_t ? lin(c, g.t[t][0], g.t[t][1], g.t[t][2]) : -1; \
+        c->l_dir[n] = g.dir;                                                                      \
+        c->l_corner[n] = g.corner;                                                                \
+        ++n;                                                                                      \
+    } while (0)
+        for (int j = 0; j <= jm; ++j)
+            for (int k = 0; k <= km; ++k) { ADD(0, j, k); ADD(im, j, k); }
+        for (int i = 1; i <= im - 1; ++i)
+            for (int j = 0; j <= jm; ++j) { ADD(i, j, 0); ADD(i, j, km); }
+        for (int i = 1; i <= im - 1; ++i)
+            for (int k = 1; k <= km - 1; ++k) { ADD(i, 0, k); ADD(i, jm, k); }
+#undef ADD
+        c->nl = n;
+        /* Phases: a node that reads the B array of a boundary node handled earlier must
+         * run after it, one that reads a boundary node handled later must run before it. */
+        int32_t* pos = (int32_t*)malloc(sizeof(int32_t) * (size_t)N);
+        int32_t* lb = (int32_t*)calloc((size_t)n, sizeof(int32_t));
+        for (int64_t l = 0; l < N; ++l) pos[l] = -1;
+        for (int64_t s = 0; s < n; ++s) pos[c->l_node[s]] = (int32_t)s;
+        for (int64_t s = 0; s < n; ++s) {
+            int32_t ph = lb[s] > 1 ? lb[s] : 1;
+            const int64_t rd[2] = {c->l_n1[s], c->l_n2[s]};
+            for (int r = 0; r < 2; ++r) {
+                const int32_t o = pos[rd[r]];
+                if (o < 0 || o == s) continue;
+                if (o < s) { if (c->l_phase[o] + 1 > ph) ph = c->l_phase[o] + 1; }
+            }
+            c->l_phase[s] = ph;
+            for (int r = 0; r < 2; ++r) {
+                const int32_t o = pos[rd[r]];
+                if (o > s && lb[o] < ph + 1) lb[o] = ph + 1;
+            }
+        }
+        free(pos);
+        free(lb);
+        int32_t mp = 0;
+        for (int64_t s = 0; s < n; ++s) if (c->l_phase[s] > mp) mp = c->l_phase[s];
+        c->max_phase = mp;
+        c->ph_start = (int64_t*)calloc((size_t)mp + 1, sizeof(int64_t));
+        c->ph_order = (int64_t*)calloc((size_t)n + 1, sizeof(int64_t));
+        int64_t t = 0;
+        for (int32_t ph = 1; ph <= mp; ++ph) {
+            c->ph_start[ph - 1] = t;
+            for (int64_t s = 0; s < n; ++s) if (c->l_phase[s] == ph) c->ph_order[t++] = s;
+        }
+        c->ph_start[mp] = t;
+    }
+    return c;
+}
+
+void acpo_destroy(acpo_case* c) {
+    if (!c) return;
+    free(c->A); free(c->B); free(c->psi); free(c->macro); free(c->mask); free(c->interior);
+    free(c->l_node); free(c->l_n1); free(c->l_n2);
+    for (int t = 0; t < 4; ++t) free(c->l_t[t]);
+    free(c->l_dir); free(c->l_corner); free(c->l_phase);
+    free(c->ph_start); free(c->ph_order);
+    free(c);
+}
+
+void acpo_set_state(acpo_case* c, const double* f) {
+    memcpy(c->A, f, sizeof(double) * 19 * (size_t)c->N);
+    memcpy(c->B, f, sizeof(double) * 19 * (size_t)c->N);
+    memset(c->psi, 0, sizeof(double) * 19 * (size_t)c->N);
+}
+void acpo_get_state(const acpo_case* c, double* f) { memcpy(f, c->A, sizeof(double) * 19 * (size_t)c->N); }
+void acpo_get_psi(const acpo_case* c, double* psi) { memcpy(psi, c->psi, sizeof(double) * 19 * (size_t)c->N); }
+const acpo_consts* acpo_case_consts(const acpo_case* c) { return &c->k; }
+int64_t acpo_lodi_count(const acpo_case* c) { return c->nl; }
+void acpo_lodi_schedule(const acpo_case* c, int64_t* node, int32_t* phase) {
+    for (int64_t s = 0; s < c->nl; ++s) { node[s] = c->l_node[s]; phase[s] = c->l_phase[s]; }
+}
+
+static inline void load19(const double* F, int64_t N, int64_t l, double f[19]) {
+    for (int q = 0; q < 19; ++q) f[q] = F[(size_t)q * N + l];
+}
+
+/* Node.cpp:229-504: one LODI node.  own: A of the node, n1/n2: B of the normal
+ * neighbours (+ their psi when cubic), t: A of the four tangent neighbours. */
+static void lodi_node(const acpo_case* c, int64_t s, double* Bout) {
+    const acpo_consts* k = &c->k;
+    const int64_t N = c->N;
+    const int cubic = c->p.cubic;
+    double f[19], g[19], psi1[19], psi2[19];
+    double rb, vb[3], r1, v1[3], r2, v2[3];
+    load19(c->A, N, c->l_node[s], f);
+    acpo_moments(k, f, NULL, &rb, vb); /* boundary nodes carry psi = 0 */
+    load19(c->B, N, c->l_n1[s], g);
+    if (cubic) load19(c->psi, N, c->l_n1[s], psi1);
+    acpo_moments(k, g, cubic ? psi1 : NULL, &r1, v1);
+    load19(c->B, N, c->l_n2[s], g);
+    if (cubic) load19(c->psi, N, c->l_n2[s], psi2);
+    acpo_moments(k, g, cubic ? psi2 : NULL, &r2, v2);
+    const double ls = k->spacing * pow(2., -0);
+    double dr = (4. * r1 - r2 - 3. * rb) / (-2. * ls);
+    double vg[3];
+    for (int a = 0; a < 3; ++a) vg[a] = (4. * v1[a] - v2[a] - 3. * vb[a]) / (-2. * ls);
+    double dT[2] = {0., 0.};
+    double vgT[2][3] = {{0., 0., 0.}, {0., 0., 0.}};
+    if (!c->l_corner[s] && c->p.transverse) {
+        double rt[4], vt[4][3];
+        for (int t = 0; t < 4; ++t) {
+            load19(c->A, N, c->l_t[t][s], g);
+            acpo_moments(k, g, NULL, &rt[t], vt[t]);
+        }
+        dT[0] = (rt[0] - rt[1]) / (2. * ls);
+        dT[1] = (rt[2] - rt[3]) / (2. * ls);
+        for (int a = 0; a < 3; ++a) {
+            vgT[0][a] = (vt[0][a] - vt[1][a]) / (2. * ls);
+            vgT[1][a] = (vt[2][a] - vt[3][a]) / (2. * ls);
+        }
+    }
+    const int dir = c->l_dir[s];
+    const int neg = (dir == 0 || dir == 1 || dir == 4);
+    if (neg) { dr *= -1.; vg[0] *= -1.; vg[1] *= -1.; vg[2] *= -1.; }
+    /* local frame: normal axis first (Node.cpp:303-413) */
+    int perm[3];
+    int axis = (dir == 0 || dir == 2) ? 0 : ((dir == 1 || dir == 3) ? 1 : 2);
+    if (axis == 0) { perm[0] = 0; perm[1] = 1; perm[2] = 2; }
+    else if (axis == 1) { perm[0] = 1; perm[1] = 0; perm[2] = 2; }
+    else { perm[0] = 2; perm[1] = 0; perm[2] = 1; }
+    double svg[3], svb[3], svgT[2][3];
+    for (int a = 0; a < 3; ++a) {
+        svg[a] = vg[perm[a]]; svb[a] = vb[perm[a]];
+        svgT[0][a] = vgT[0][perm[a]]; svgT[1][a] = vgT[1][perm[a]];
+    }
+    const double L = k->dom_len[axis];
+    /* Node.cpp:447-504 */
+    const double cs = k->cs, cs2 = k->lodi_cs2;
+    const double csDivL = cs / L;
+    const double K1 = k->k1, K2 = k->ma, MaSq = k->ma2;
+    const double rout = k->rho0;
+    const double dtl = k->dt / pow(2., 0);
+    const double Tex = 0.;
+    const double T1 = -(cs2 * (rb * (svgT[0][1] + svgT[1][2]) + svb[1] * dT[0] + svb[2] * dT[1]) -
+                        rb * cs * (svb[1] * svgT[0][0] + svb[2] * svgT[1][0]));
+    const double T5 = -(cs2 * (rb * (svgT[0][1] + svgT[1][2]) + svb[1] * dT[0] + svb[2] * dT[1]) +
+                        rb * cs * (svb[1] * svgT[0][0] + svb[2] * svgT[1][0]));
+    double L1, L5;
+    if (neg) {
+        L5 = K1 * (1 - MaSq) * csDivL * cs2 * (rb - rout) - K2 * (T5 - Tex) + T5;
+        L1 = (svb[0] - cs) * cs * (cs * dr - rb * svg[0]);
+    } else {
+        L1 = K1 * (1 - MaSq) * csDivL * cs2 * (rb - rout) - K2 * (T1 - Tex) + T1;
+        L5 = (svb[0] + cs) * cs * (cs * dr + rb * svg[0]);
+    }
+    const double L3 = svb[0] * svg[1];
+    const double L4 = svb[0] * svg[2];
+    const double T3 = -(svb[1] * svgT[0][1] + svb[2] * svgT[1][1]) - (cs2 / rb) * dT[0];
+    const double T4 = -(svb[1] * svgT[0][2] + svb[2] * svgT[1][2]) - (cs2 / rb) * dT[1];
+    const double rho = rb - dtl / (2. * cs2) * (L5 + L1) + dtl / (2. * cs2) * (T5 + T1);
+    double lv[3];
+    lv[0] = svb[0] - dtl / (2. * rb * cs) * (L5 - L1) + dtl / (2. * rb * cs) * (T5 - T1);
+    lv[1] = svb[1] - dtl * L3 + dtl * T3;
+    lv[2] = svb[2] - dtl * L4 + dtl * T4;
+    double u[3];
+    for (int a = 0; a < 3; ++a) u[perm[a]] = lv[a];
+    double eq[19];
+    acpo_feq(k, c->p.order, rho, u, eq);
+    for (int q = 0; q < 19; ++q) Bout[(size_t)q * N + c->l_node[s]] = eq[q];
+}
+
+static void macro_pass(acpo_case* c) {
+    const int64_t N = c->N;
+#pragma omp parallel for schedule(static) num_threads(c->p.nthreads > 0 ? c->p.nthreads : 1)
+    for (int64_t l = 0; l < N; ++l) {
+        double f[19], r, u[3];
+        load19(c->A, N, l, f);
+        acpo_moments(&c->k, f, NULL, &r, u);
+        c->macro[l] = r; c->macro[N + l] = u[0]; c->macro[2 * N + l] = u[1]; c->macro[3 * N + l] = u[2];
+    }
+}
+
+static void one_step(acpo_case* c) {
+    const acpo_params* p = &c->p;
+    const int64_t N = c->N;
+    const int nx = p->nx, ny = p->ny, nz = p->nz;
+    const int need_nb = (p->op == ACPO_HRR) || ((p->op == ACPO_RR) && p->cubic);
+    if (need_nb) macro_pass(c);
+    /* collide + push (Lattice.cpp:1524-1580) */
+#pragma omp parallel for schedule(static) num_threads(p->nthreads > 0 ? p->nthreads : 1)
+    for (int kk = 0; kk < nz; ++kk)
+        for (int jj = 0; jj < ny; ++jj)
+            for (int ii = 0; ii < nx; ++ii) {
+                const int64_t l = lin(c, ii, jj, kk);
+                double f[19];
+                load19(c->A, N, l, f);
+                if (c->interior[l]) {
+                    double nb[6][4];
+                    double ps[19];
+                    if (need_nb) {
+                        for (int m = 0; m < 6; ++m) {
+                            int a = ii + E[m][0], b = jj + E[m][1], d = kk + E[m][2];
+                            if (p->periodic) { a = (a + nx) % nx; b = (b + ny) % ny; d = (d + nz) % nz; }
+                            const int64_t o = lin(c, a, b, d);
+                            for (int v = 0; v < 4; ++v) nb[m][v] = c->macro[(size_t)v * N + o];
+                        }
+                    }
+                    for (int q = 0; q < 19; ++q) ps[q] = c->psi[(size_t)q * N + l];
+                    acpo_collide(&c->k, p, f, ps, (const double(*)[4])nb);
+                    for (int q = 0; q < 19; ++q) c->psi[(size_t)q * N + l] = ps[q];
+                }
+                const uint32_t m = c->mask[l];
+                for (int q = 0; q < 18; ++q) {
+                    if (!(m & (1u << q))) continue;
+                    int a = ii + E[q][0], b = jj + E[q][1], d = kk + E[q][2];
+                    if (p->periodic) { a = (a + nx) % nx; b = (b + ny) % ny; d = (d + nz) % nz; }
+                    c->B[(size_t)q * N + lin(c, a, b, d)] = f[q];
+                }
+                c->B[(size_t)18 * N + l] = f[18];
+            }
+    /* LODI passes (Lattice.cpp:1582-1624).  Nodes of one phase neither read nor
+     * write each other's B entries, so each phase runs in parallel; phases run in
+     * order, which reproduces the reference's sequential pass order. */
+    if (!p->periodic)
+        for (int32_t ph = 1; ph <= c->max_phase; ++ph) {
+            const int64_t b = c->ph_start[ph - 1], e = c->ph_start[ph];
+#pragma omp parallel for schedule(static) num_threads(p->nthreads > 0 ? p->nthreads : 1)
+            for (int64_t t = b; t < e; ++t) lodi_node(c, c->ph_order[t], c->B);
+        }
+    double* t = c->A; c->A = c->B; c->B = t;
+}
+
+void acpo_step(acpo_case* c, int n, const int64_t* probe_idx, int nprobe, double* probe_out) {
+    for (int s = 0; s < n; ++s) {
+        one_step(c);
+        for (int i = 0; i < nprobe; ++i) {
+            double r = 0.;
+            for (int q = 0; q < 19; ++q) r += c->A[(size_t)q * c->N + probe_idx[i]];
+            probe_out[(size_t)s * nprobe + i] = r;
+        }
+    }
+}
+
+void acpo_geometry(const acpo_case* c, int64_t* rows) {
+    const acpo_params* p = &c->p;
+    const int64_t N = c->N;
+    for (int64_t l = 0; l < N; ++l) {
+        int64_t* r = rows + 27 * l;
+        for (int t = 0; t < 27; ++t) r[t] = -1;
+        r[24] = 0; r[25] = 0; r[26] = c->interior[l] ? 0 : 1;
+        const int i = (int)(l % p->nx), j = (int)((l / p->nx) % p->ny), k = (int)(l / ((int64_t)p->nx * p->ny));
+        for (int q = 0; q < 18; ++q) {
+            if (!(c->mask[l] & (1u << q))) continue;
+            int a = i + E[q][0], b = j + E[q][1], d = k + E[q][2];
+            if (p->periodic) { a = (a + p->nx) % p->nx; b = (b + p->ny) % p->ny; d = (d + p->nz) % p->nz; }
+            r[q] = lin(c, a, b, d);
+        }
+    }
+    for (int64_t s = 0; s < c->nl; ++s) {
+        int64_t* r = rows + 27 * c->l_node[s];
+        r[18] = c->l_n1[s]; r[19] = c->l_n2[s];
+        for (int t = 0; t < 4; ++t) r[20 + t] = c->l_t[t][s];
+        r[24] = c->l_dir[s]; r[25] = c->l_corner[s];
+    }
+}
+
+/* Lattice.cpp:1397-1493 for level 0 */
+void acpo_reference_ic(const acpo_params* p, double domx, double domy, double* f) {
+    acpo_consts k;
+    acpo_consts_init(&k, p);
+    const int64_t N = (int64_t)p->nx * p->ny * p->nz;
+    const double Rc = 0.06, A = 0.01, xc = -8. * Rc, yc = 0.;
+    (void)yc;
+    for (int kk = 0; kk < p->nz; ++kk)
+        for (int jj = 0; jj < p->ny; ++jj)
+            for (int ii = 0; ii < p->nx; ++ii) {
+                const double x = ii * (p->spacing / pow(2., 0)), y = jj * (p->spacing / pow(2., 0));
+                const double xn = x - domx / 2., yn = y - domy / 2.;
+                const double xr = xn - xc;
+                const double rho = p->rho0 * (1 + A * exp(-(xr * xr + yn * yn) / (2 * Rc * Rc)));
+                const double u[3] = {0., 0., 0.};
+                double eq[19];
+                acpo_feq(&k, p->order, rho, u, eq);
+                const int64_t l = (int64_t)ii + (int64_t)p->nx * ((int64_t)jj + (int64_t)p->ny * kk);
+                for (int q = 0; q < 19; ++q) f[(size_t)q * N + l] = eq[q];
+            }
+}
+
+void acpo_pulse_ic(const acpo_params* p, double amp, double x0, double y0, double b, double* f) {
+    acpo_consts k;
+    acpo_consts_init(&k, p);
+    const int64_t N = (int64_t)p->nx * p->ny * p->nz;
+    const double ln2 = 0.69314718055994530942;
+    for (int kk = 0; kk < p->nz; ++kk)
+        for (int jj = 0; jj < p->ny; ++jj)
+            for (int ii = 0; ii < p->nx; ++ii) {
+                const double dx = ii - x0, dy = jj - y0;
+                const double rho = p->rho0 * (1. + amp * exp(-ln2 * (dx * dx + dy * dy) / (b * b)));
+                const double u[3] = {0., 0., 0.};
+                double eq[19];
+                acpo_feq(&k, p->order, rho, u, eq);
+                const int64_t l = (int64_t)ii + (int64_t)p->nx * ((int64_t)jj + (int64_t)p->ny * kk);
+                for (int q = 0; q < 19; ++q) f[(size_t)q * N + l] = eq[q];
+            }
+}

@@ -1,0 +1,104 @@
+/*
+ * acp_oracle.h -- CPU restatement of the reference D3Q19 uniform-grid step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker used by tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py.  Nothing in
+ * the product (acpulse_vclbm_amd/) links, loads or calls it.
+ *
+ * Every function restates the arithmetic of the reference solver
+ * (aleksandrShukman/acPulse_vcLBM, the .cpp files under acPulse_vcLBM/) in plain C on a
+ * structure-of-arrays lattice, keeping the reference's floating-point
+ * association so that results are bit-identical to the reference build.
+ * File:line citations name the reference function each piece follows.
+ */
+#ifndef ACP_ORACLE_H
+#define ACP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ACPO_SRT = 0, ACPO_MRT_GS = 1, ACPO_MRT_RM = 2, ACPO_RR = 3, ACPO_HRR = 4 };
+
+typedef struct acpo_params {
+    int32_t nx, ny, nz;
+    int32_t op;         /* ACPO_* collision model (Lattice.cpp:2158-2183) */
+    int32_t order;      /* Order-Of-Equilibrium 2|3|4 */
+    int32_t cubic;      /* Cubic-Mach-Correction yes=1 */
+    int32_t transverse; /* Transverse yes=1 */
+    int32_t periodic;   /* 0: LODI on all six faces (the reference), 1: fully periodic (new) */
+    int32_t mrt_mode;   /* 0: as the reference (result discarded -> identity), 1: relaxation applied */
+    int32_t nthreads;
+    double rho0, nu, cs, spacing, mach, k1, sigma;
+} acpo_params;
+
+/* Constants derived once per case, as Control::Control (Control.cpp:39-131) and
+ * the per-call pow() chains of Node.cpp compute them. */
+typedef struct acpo_consts {
+    double c, dt, omega;
+    double w[19];
+    double xi[19][3];
+    double cs2, rcs2, rcs4, rcs6, rcs8;
+    double hx[19], hy[19], hz[19];
+    double hxx[19], hyy[19], hzz[19], hxy[19], hxz[19], hyz[19];  /* xi*xi - cs2 (Node.cpp:788-793) */
+    double pxx[19], pyy[19], pzz[19];                           /* pow(xi,2.) - cs2 (Node.cpp:2514-2516) */
+    double h3[6][19];  /* XXYpYZZ, XXZpYYZ, XYYpXZZ, YZZmXXY, YYZmXXZ, XZZmXYY */
+    double h4[3][19];  /* XXYY, XXZZ, YYZZ fully orthogonal */
+    double rec_cs4_strain; /* 1/(cs2*cs2) (Node.cpp:1758) */
+    double inv_pre;        /* -((dt*cs2)/(omega*nu)) (Node.cpp:1449) */
+    double nu, sigma, rho0, spacing, cs;
+    double lodi_cs2, dom_len[3], ma, ma2, k1;
+    /* raw-moment and Gram-Schmidt MRT constants */
+    double c2, c3, c4, ix, ix2, ix3, ix4;
+    double gs_pre[19][19];
+} acpo_consts;
+
+int  acpo_consts_init(acpo_consts* k, const acpo_params* p);
+
+/* ---- per-node building blocks (inputs/outputs are 19-vectors in reference order) ---- */
+void acpo_feq(const acpo_consts* k, int order, double rho, const double u[3], double out[19]);
+void acpo_moments(const acpo_consts* k, const double f[19], const double* psi, double* rho, double u[3]);
+void acpo_pi1(const acpo_consts* k, const double f[19], const double feq[19], const double* psi, double pi[6]);
+void acpo_offeq(const acpo_consts* k, int order, const double a2[6], const double u[3], double out[19]);
+void acpo_strain_psi(const acpo_consts* k, const double nb_macro[6][4], double S[6], double psi[19]);
+void acpo_rm_moments(const acpo_consts* k, const double f[19], double m[19]);
+void acpo_rm_feq4(const acpo_consts* k, double rho, const double u[3], double out[19]);
+void acpo_rm_inverse(const acpo_consts* k, const double m[19], double f[19]);
+void acpo_gs_moments(const acpo_consts* k, const double f[19], double m[19]);
+void acpo_gs_eqmoments(double rho, double jx, double jy, double jz, double meq[19]);
+void acpo_gs_inverse(const acpo_consts* k, const double m[19], double f[19]);
+
+/* Collide one interior node in place (Node.cpp:511-736, 2565-2835).  nb_macro holds the
+ * pre-collision (rho,ux,uy,uz) of the +x,+y,-x,-y,+z,-z neighbours (HRR only); psi is the
+ * node's cubic-Mach state (read and updated; may be NULL when unused). */
+void acpo_collide(const acpo_consts* k, const acpo_params* p, double f[19], double* psi,
+                  const double nb_macro[6][4]);
+
+/* ---- whole lattice ---- */
+typedef struct acpo_case acpo_case;
+acpo_case* acpo_create(const acpo_params* p);
+void acpo_destroy(acpo_case* c);
+void acpo_set_state(acpo_case* c, const double* f);   /* [19][N], both arrays (Lattice.cpp:1457-1475) */
+void acpo_get_state(const acpo_case* c, double* f);   /* current array [19][N] */
+void acpo_get_psi(const acpo_case* c, double* psi);   /* [19][N] */
+/* Runs n steps; after each step writes sum_q f_q of the current array at each probe node. */
+void acpo_step(acpo_case* c, int n, const int64_t* probe_idx, int nprobe, double* probe_out);
+/* Connectivity rows in the reference driver's 27-column layout. */
+void acpo_geometry(const acpo_case* c, int64_t* rows);
+/* LODI schedule: for each boundary node in reference order, its linear index and phase. */
+int64_t acpo_lodi_count(const acpo_case* c);
+void acpo_lodi_schedule(const acpo_case* c, int64_t* node, int32_t* phase);
+const acpo_consts* acpo_case_consts(const acpo_case* c);
+
+/* Reference initial condition (Lattice.cpp:1397-1493): feq_N of the hard-coded pulse. */
+void acpo_reference_ic(const acpo_params* p, double domx, double domy, double* f);
+/* Parameterised pulse in node units: rho = rho0 (1 + amp exp(-ln2 ((i-x0)^2 + (j-y0)^2) / b^2)),
+ * u = 0, populations feq_N (the initialization of Lattice.cpp:1457-1475 with a new profile). */
+void acpo_pulse_ic(const acpo_params* p, double amp, double x0, double y0, double b, double* f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""ctypes binding of the CPU oracle (oracle/_build/libacp_oracle.so).
+
+Test infrastructure: the oracle is the checker for the HIP path and is only
+loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libacp_oracle.so")
+
+OPS = {"SRT": 0, "MRT-GS": 1, "MRT-RM": 2, "RR": 3, "HRR": 4}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+        ("op", ctypes.c_int32), ("order", ctypes.c_int32), ("cubic", ctypes.c_int32),
+        ("transverse", ctypes.c_int32), ("periodic", ctypes.c_int32), ("mrt_mode", ctypes.c_int32),
+        ("nthreads", ctypes.c_int32),
+        ("rho0", ctypes.c_double), ("nu", ctypes.c_double), ("cs", ctypes.c_double),
+        ("spacing", ctypes.c_double), ("mach", ctypes.c_double), ("k1", ctypes.c_double),
+        ("sigma", ctypes.c_double),
+    ]
+
+
+class Consts(ctypes.Structure):
+    _fields_ = [
+        ("c", ctypes.c_double), ("dt", ctypes.c_double), ("omega", ctypes.c_double),
+        ("w", ctypes.c_double * 19), ("xi", (ctypes.c_double * 3) * 19),
+        ("cs2", ctypes.c_double), ("rcs2", ctypes.c_double), ("rcs4", ctypes.c_double),
+        ("rcs6", ctypes.c_double), ("rcs8", ctypes.c_double),
+        ("hx", ctypes.c_double * 19), ("hy", ctypes.c_double * 19), ("hz", ctypes.c_double * 19),
+        ("hxx", ctypes.c_double * 19), ("hyy", ctypes.c_double * 19), ("hzz", ctypes.c_double * 19),
+        ("hxy", ctypes.c_double * 19), ("hxz", ctypes.c_double * 19), ("hyz", ctypes.c_double * 19),
+        ("pxx", ctypes.c_double * 19), ("pyy", ctypes.c_double * 19), ("pzz", ctypes.c_double * 19),
+        ("h3", (ctypes.c_double * 19) * 6), ("h4", (ctypes.c_double * 19) * 3),
+        ("rec_cs4_strain", ctypes.c_double), ("inv_pre", ctypes.c_double),
+        ("nu", ctypes.c_double), ("sigma", ctypes.c_double), ("rho0", ctypes.c_double),
+        ("spacing", ctypes.c_double), ("cs", ctypes.c_double),
+        ("lodi_cs2", ctypes.c_double), ("dom_len", ctypes.c_double * 3), ("ma", ctypes.c_double),
+        ("ma2", ctypes.c_double), ("k1", ctypes.c_double),
+        ("c2", ctypes.c_double), ("c3", ctypes.c_double), ("c4", ctypes.c_double),
+        ("ix", ctypes.c_double), ("ix2", ctypes.c_double), ("ix3", ctypes.c_double), ("ix4", ctypes.c_double),
+        ("gs_pre", (ctypes.c_double * 19) * 19),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library missing: run __graft_entry__.build() (make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER(Params)
+        K = ctypes.POINTER(Consts)
+        dp = ctypes.POINTER(ctypes.c_double)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        L.acpo_consts_init.argtypes = [K, P]
+        L.acpo_feq.argtypes = [K, ctypes.c_int, ctypes.c_double, dp, dp]
+        L.acpo_moments.argtypes = [K, dp, dp, dp, dp]
+        L.acpo_pi1.argtypes = [K, dp, dp, dp, dp]
+        L.acpo_offeq.argtypes = [K, ctypes.c_int, dp, dp, dp]
+        L.acpo_strain_psi.argtypes = [K, dp, dp, dp]
+        for n in ("acpo_rm_moments", "acpo_gs_moments", "acpo_rm_inverse", "acpo_gs_inverse"):
+            getattr(L, n).argtypes = [K, dp, dp]
+        L.acpo_rm_feq4.argtypes = [K, ctypes.c_double, dp, dp]
+        L.acpo_gs_eqmoments.argtypes = [ctypes.c_double] * 4 + [dp]
+        L.acpo_collide.argtypes = [K, P, dp, dp, dp]
+        L.acpo_create.argtypes = [P]
+        L.acpo_create.restype = ctypes.c_void_p
+        L.acpo_destroy.argtypes = [ctypes.c_void_p]
+        L.acpo_set_state.argtypes = [ctypes.c_void_p, dp]
+        L.acpo_get_state.argtypes = [ctypes.c_void_p, dp]
+        L.acpo_get_psi.argtypes = [ctypes.c_void_p, dp]
+        L.acpo_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_int, dp]
+        L.acpo_geometry.argtypes = [ctypes.c_void_p, ip]
+        L.acpo_lodi_count.argtypes = [ctypes.c_void_p]
+        L.acpo_lodi_count.restype = ctypes.c_int64
+        L.acpo_lodi_schedule.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(ctypes.c_int32)]
+        L.acpo_reference_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, dp]
+        L.acpo_pulse_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        _lib = L
+    return _lib
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def make_params(nx, ny, nz, op="SRT", order=3, cubic=False, transverse=False, periodic=False,
+                mrt_mode=0, nthreads=8, rho0=1.17621, nu=None, tau=0.6, cs=347.3, spacing=0.01,
+                mach=0.0, k1=1.0, sigma=0.98):
+    if nu is None:
+        dt = spacing / (cs * 3 ** 0.5)
+        nu = (tau - 0.5) * cs * cs * dt
+    return Params(nx, ny, nz, OPS[op] if isinstance(op, str) else op, order, int(cubic), int(transverse),
+                  int(periodic), int(mrt_mode), nthreads, rho0, nu, cs, spacing, mach, k1, sigma)
+
+
+def consts(p):
+    k = Consts()
+    rc = lib().acpo_consts_init(ctypes.byref(k), ctypes.byref(p))
+    assert rc == 0
+    return k
+
+
+class OracleCase:
+    """Whole-lattice oracle run: reference step semantics on a [19][N] state."""
+
+    def __init__(self, p):
+        self.p = p
+        self.N = p.nx * p.ny * p.nz
+        self.h = lib().acpo_create(ctypes.byref(p))
+        if not self.h:
+            raise ValueError("oracle rejected the case parameters")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().acpo_destroy(self.h)
+            self.h = None
+
+    def set_state(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(19, self.N)
+        lib().acpo_set_state(self.h, dptr(f))
+
+    def state(self):
+        f = np.empty((19, self.N))
+        lib().acpo_get_state(self.h, dptr(f))
+        return f
+
+    def psi(self):
+        f = np.empty((19, self.N))
+        lib().acpo_get_psi(self.h, dptr(f))
+        return f
+
+    def step(self, n, probes=()):
+        idx = np.array([i + self.p.nx * (j + self.p.ny * k) for (i, j, k) in probes] or [0], dtype=np.int64)
+        out = np.zeros(max(1, n * len(probes)))
+        lib().acpo_step(self.h, n, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(probes), dptr(out))
+        return out[: n * len(probes)].reshape(n, len(probes)) if probes else None
+
+    def geometry(self):
+        rows = np.empty((self.N, 27), dtype=np.int64)
+        lib().acpo_geometry(self.h, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+        return rows
+
+    def lodi_schedule(self):
+        n = lib().acpo_lodi_count(self.h)
+        node = np.empty(n, dtype=np.int64)
+        ph = np.empty(n, dtype=np.int32)
+        lib().acpo_lodi_schedule(self.h, node.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                 ph.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        return node, ph
+
+
+def feq(p, order, rho, u):
+    k = consts(p)
+    out = np.empty(19)
+    uu = np.ascontiguousarray(u, dtype=np.float64)
+    lib().acpo_feq(ctypes.byref(k), order, rho, dptr(uu), dptr(out))
+    return out
+
+
+def equilibrium_field(p, rho, u):
+    """feq_N for every node; rho [N], u [3][N] -> [19][N]."""
+    k = consts(p)
+    N = rho.size
+    f = np.empty((19, N))
+    out = np.empty(19)
+    L = lib()
+    for n in range(N):
+        uu = np.ascontiguousarray(u[:, n])
+        L.acpo_feq(ctypes.byref(k), p.order, float(rho[n]), dptr(uu), dptr(out))
+        f[:, n] = out
+    return f
+
+
+def pulse_ic(p, amp, x0, y0, b):
+    f = np.empty((19, p.nx * p.ny * p.nz))
+    lib().acpo_pulse_ic(ctypes.byref(p), amp, x0, y0, b, dptr(f))
+    return f
+
+
+def reference_ic(p, domx, domy):
+    f = np.empty((19, p.nx * p.ny * p.nz))
+    lib().acpo_reference_ic(ctypes.byref(p), domx, domy, dptr(f))
+    return f
+
+
+def macros(p, f):
+    """(rho, ux, uy, uz) per node in the reference's units (no psi terms)."""
+    k = consts(p)
+    xi = np.array(k.xi)
+    rho = np.zeros(f.shape[1])
+    for q in range(19):
+        rho += f[q]
+    u = np.zeros((3, f.shape[1]))
+    for a in range(3):
+        for q in range(19):
+            if xi[q][a] != 0.0:
+                u[a] += xi[q][a] * f[q]
+        u[a] /= rho
+    return rho, u
