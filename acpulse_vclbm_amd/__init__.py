@@ -1,0 +1,281 @@
+"""MI355X D3Q19 lattice-Boltzmann engine: Python side of the C ABI (include/acp.h).
+
+Mirrors the reference solver's driver interface (aleksandrShukman/acPulse_vcLBM,
+main.cpp:46-157): ``Control(path)`` reads a BoundaryConditions.txt,
+``Lattice(control)`` allocates the case, ``initialization(control)``,
+``solve(alternating)`` and ``getDistribution(...)`` behave as there.  All work
+runs in libacpulse.so; this module only marshals arguments.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _capi
+from ._capi import (ACP_OK, AcpError, BOUNDARY, CaseDesc, CaseInfo, MRT_MODE, OP, PREC, check, dptr, lib)
+
+__all__ = ["Case", "Control", "Lattice", "CaseDesc", "AcpError", "make_desc", "lib", "lodi_schedule"]
+
+
+def make_desc(nx, ny, nz, op="SRT", order=3, cubic=False, transverse=False, boundary="LODI",
+              mrt_mode="REFERENCE", precision="F64", rho0=1.17621, cs=347.3, spacing=0.01, nu=None, tau=None,
+              mach=0.0, k1=1.0, sigma=0.98, f16_scale=1000.0, timesteps=0, write_interval=1000000,
+              save_as="acp_case"):
+    """Case description with the reference's SI parameters.  Either ``nu`` (m^2/s)
+    or ``tau`` (relaxation time, tau = 1/2 + nu/(cs^2 dt)) gives the viscosity."""
+    d = CaseDesc()
+    lib().acp_case_desc_default(ctypes.byref(d))
+    d.nx, d.ny, d.nz = int(nx), int(ny), int(nz)
+    d.op = OP[op] if isinstance(op, str) else int(op)
+    d.order = int(order)
+    d.cubic, d.transverse = int(bool(cubic)), int(bool(transverse))
+    d.boundary = BOUNDARY[boundary] if isinstance(boundary, str) else int(boundary)
+    d.mrt_mode = MRT_MODE[mrt_mode] if isinstance(mrt_mode, str) else int(mrt_mode)
+    d.precision = PREC[precision] if isinstance(precision, str) else int(precision)
+    d.rho0, d.cs, d.spacing = float(rho0), float(cs), float(spacing)
+    if nu is None:
+        if tau is None:
+            raise ValueError("give nu or tau")
+        dt = spacing / (cs * math.sqrt(3.0))
+        nu = (tau - 0.5) * cs * cs * dt
+    d.nu = float(nu)
+    d.mach, d.k1, d.sigma, d.f16_scale = float(mach), float(k1), float(sigma), float(f16_scale)
+    d.timesteps, d.write_interval = int(timesteps), int(write_interval)
+    d.domain[0], d.domain[1], d.domain[2] = (nx - 0.5) * spacing, (ny - 0.5) * spacing, (nz - 0.5) * spacing
+    d.save_as = save_as.encode()
+    d.collision_model = (op if isinstance(op, str) else "SRT").encode()
+    return d
+
+
+def read_case_file(path):
+    """acp_control_read: Control::Control semantics (Control.cpp:8-132)."""
+    d = CaseDesc()
+    check(lib().acp_control_read(str(path).encode(), ctypes.byref(d)))
+    return d
+
+
+class Case:
+    """One device-resident case (acp_case_* of include/acp.h)."""
+
+    def __init__(self, desc, stream=None):
+        self.desc = desc
+        self.h = ctypes.c_void_p()
+        st = ctypes.c_void_p(int(stream)) if stream else None
+        check(lib().acp_case_create(ctypes.byref(desc), st, ctypes.byref(self.h)))
+        self.nx, self.ny, self.nz = desc.nx, desc.ny, desc.nz
+        self.N = self.nx * self.ny * self.nz
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().acp_case_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def node(self, i, j, k):
+        return i + self.nx * (j + self.ny * k)
+
+    def info(self):
+        inf = CaseInfo()
+        check(lib().acp_case_info_get(self.h, ctypes.byref(inf)))
+        return inf
+
+    def init_reference(self):
+        check(lib().acp_case_init_reference(self.h))
+
+    def init_pulse(self, amp, x0, y0, b):
+        check(lib().acp_case_init_pulse(self.h, amp, x0, y0, b))
+
+    def init_random(self, seed, rho_amp, u_amp):
+        check(lib().acp_case_init_random(self.h, seed, rho_amp, u_amp))
+
+    def set_populations(self, f):
+        f = np.ascontiguousarray(f, dtype=np.float64).reshape(19, self.N)
+        check(lib().acp_case_set_populations(self.h, dptr(f)))
+
+    def step(self, n=1):
+        check(lib().acp_case_step(self.h, int(n)))
+
+    def sync(self):
+        check(lib().acp_case_sync(self.h))
+
+    def populations(self):
+        f = np.empty((19, self.N))
+        check(lib().acp_case_get_populations(self.h, dptr(f)))
+        return f
+
+    def macros(self):
+        rho = np.empty(self.N)
+        u = np.empty((3, self.N))
+        check(lib().acp_case_get_macros(self.h, dptr(rho), dptr(u)))
+        return rho, u
+
+    def mass(self):
+        m = ctypes.c_double()
+        check(lib().acp_case_mass(self.h, ctypes.byref(m)))
+        return m.value
+
+    def set_probes(self, points, capacity):
+        nodes = np.array([self.node(*p) for p in points], dtype=np.int64)
+        check(lib().acp_case_set_probes(self.h, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                        len(nodes), int(capacity)))
+        self._nprobe = len(nodes)
+        self._probe_cap = int(capacity)
+
+    def read_probes(self):
+        out = np.empty((max(self._probe_cap, 1), max(self._nprobe, 1)))
+        n = ctypes.c_int32()
+        check(lib().acp_case_read_probes(self.h, dptr(out), ctypes.byref(n)))
+        return out[: n.value, : self._nprobe].copy()
+
+
+def lodi_schedule(nx, ny, nz):
+    """(nodes, phases) of the LODI pass in processing order."""
+    L = lib()
+    n = L.acp_lodi_schedule(nx, ny, nz, None, None, 0)
+    nodes = np.empty(n, dtype=np.int64)
+    ph = np.empty(n, dtype=np.int32)
+    L.acp_lodi_schedule(nx, ny, nz, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                        ph.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n)
+    return nodes, ph
+
+
+class Control:
+    """Control(string) of the reference (Control.h:17-75): case file reader and getters."""
+
+    def __init__(self, filename):
+        self.desc = read_case_file(filename)
+        self.filename = filename
+        self._time = 0.0
+
+    def getNumberOfNodesX(self):
+        return self.desc.nx
+
+    def getNumberOfNodesY(self):
+        return self.desc.ny
+
+    def getNumberOfNodesZ(self):
+        return self.desc.nz
+
+    def getLevelMax(self):
+        return self.desc.max_level
+
+    def getTimeStepMax(self):
+        return self.desc.timesteps
+
+    def getWriteInterval(self):
+        return self.desc.write_interval
+
+    def getNumberOfThreads(self):
+        return self.desc.threads
+
+    def getOrderOfEquilibrium(self):
+        return self.desc.order
+
+    def getSpacing(self):
+        return self.desc.spacing
+
+    def getHybridParameter(self):
+        return self.desc.sigma
+
+    def getDensity(self):
+        return self.desc.rho0
+
+    def getMachNumber(self):
+        return self.desc.mach
+
+    def getSpeedOfSound(self):
+        return self.desc.cs
+
+    def getKinematicViscosity(self):
+        return self.desc.nu
+
+    def getMolecularVelocity(self):
+        return self.desc.cs * math.sqrt(3.0)
+
+    def getTimeStep(self):
+        return self.desc.spacing / self.getMolecularVelocity()
+
+    def getCollisionFrequency(self):
+        cs2, dt = self.desc.cs ** 2, self.getTimeStep()
+        return (cs2 * dt) / (self.desc.nu + 0.5 * cs2 * dt)
+
+    def getReynoldsNumber(self):
+        return 0.0
+
+    def getChannelSizeX_(self):
+        return self.desc.domain[0]
+
+    def getChannelSizeY_(self):
+        return self.desc.domain[1]
+
+    def getChannelSizeZ_(self):
+        return self.desc.domain[2]
+
+    def getLODIK1relaxation(self):
+        return self.desc.k1
+
+    def getLODIK2relaxation(self):
+        return self.desc.k2
+
+    def getCaseName(self):
+        return self.desc.save_as.decode()
+
+    def getCollisionModel(self):
+        return self.desc.collision_model.decode()
+
+    def getReflectionHandling(self):
+        return "yes" if self.desc.nonreflecting else "no"
+
+    def getTransverseInclusion(self):
+        return "yes" if self.desc.transverse else "no"
+
+    def getCubicMachCorrection(self):
+        return "yes" if self.desc.cubic else "no"
+
+    def setTime_(self, t):
+        self._time = float(t)
+
+    def getTime_(self):
+        return self._time
+
+
+class Lattice:
+    """Lattice(Control&) of the reference (Lattice.h:23-120) on the device.
+
+    The reference keeps two population arrays and a ping-pong flag owned by the
+    caller (main.cpp:55-56, 155-156); here the case tracks the current array and
+    ``solve(alternating)`` checks that the caller's flag agrees with it."""
+
+    def __init__(self, control, stream=None):
+        self.bc = control
+        self.case = Case(control.desc, stream=stream)
+        self._alt = True
+
+    def initialization(self, control=None):
+        self.case.init_reference()
+        self._alt = True
+
+    def solve(self, alternating):
+        if bool(alternating) != self._alt:
+            raise AcpError(_capi.ACP_E_STATE, "ping-pong flag does not match the current array")
+        self.case.step(1)
+        self._alt = not self._alt
+
+    def getDistribution(self, alternating, dir, level, i, j, k):
+        if level != 0:
+            raise AcpError(_capi.ACP_E_UNSUPPORTED, "single level only")
+        if bool(alternating) != self._alt:
+            raise AcpError(_capi.ACP_E_STATE, "only the current array is kept on the device")
+        f = self.case.populations()
+        return float(f[dir, self.case.node(i, j, k)])

@@ -1,0 +1,66 @@
+// Internal declarations shared by the host-side translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "acp.h"
+#include "k64.hpp"
+#include "kshift.hpp"
+
+namespace acp {
+
+int fail(int code, const std::string& msg);
+void clear_error();
+int hip_fail(hipError_t e, const char* where);
+
+// Constants of one case, derived from the case description exactly as the
+// reference derives them (Control.cpp:39-131, Lattice.cpp:1404, Node.cpp).
+void k64_init(K64& k, const acp_case_desc& d);
+void kshift_init(KShift& k, const acp_case_desc& d);
+
+// Boundary-node schedule of the LODI pass: nodes in the reference's pass order
+// (x faces, z faces, y faces; Lattice.cpp:1582-1624) grouped into phases so that
+// a node reading another boundary node's B entries sees them exactly as the
+// reference's sequential loops do.
+struct LodiSchedule {
+    std::vector<LodiNode> nodes;      // sorted by phase
+    std::vector<int64_t> phase_start; // [nphase + 1]
+};
+LodiSchedule build_lodi_schedule(int nx, int ny, int nz);
+
+}  // namespace acp
+
+struct acp_case {
+    acp_case_desc d;
+    acp::Grid g;
+    int precision;
+    int bulk_op;
+    bool periodic, need_macro, need_psi;
+    // float64 tier
+    acp::K64 hk;
+    acp::K64* dk = nullptr;
+    double *A = nullptr, *B = nullptr, *psi = nullptr, *M = nullptr;
+    // shifted tiers
+    acp::KShift hks;
+    acp::KShift* dks = nullptr;
+    void *SA = nullptr, *SB = nullptr;
+    float* SM = nullptr;
+    // LODI
+    acp::LodiNode* lodi = nullptr;
+    std::vector<int64_t> phase_start;
+    // probes and reductions
+    int64_t* probe_nodes = nullptr;
+    double* probe_buf = nullptr;
+    int32_t nprobe = 0, probe_cap = 0, probe_count = 0;
+    double* partial = nullptr;
+    int partial_blocks = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    bool initialized = false;
+    int32_t steps = 0;
+    int64_t device_bytes = 0;
+};

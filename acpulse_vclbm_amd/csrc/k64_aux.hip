@@ -1,0 +1,283 @@
+// Float64 auxiliary kernels: LODI boundary pass, HRR macro pre-pass, device
+// initial condition, probes and the mass reduction.
+#include "k64.hpp"
+
+namespace acp {
+
+__device__ __forceinline__ void load19(const double* __restrict__ F, int64_t N, int64_t l, double f[19]) {
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = F[q * N + l];
+    });
+}
+
+// ------------------------------------------------------------------ LODI
+// Node::LODIpressureOutlet + LODIpressureCalcVelocity (Node.cpp:229-504) for one
+// boundary node: own state from A (time t), normal neighbours from B (time t+1,
+// plus their cubic-Mach terms), tangent neighbours from A; writes feq_N of the
+// characteristic update into the node's B entries.
+template <int ORDER, bool CUBIC>
+__global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                              double* __restrict__ B, const double* __restrict__ psi,
+                                              const LodiNode* __restrict__ nodes, int64_t count, int64_t N,
+                                              int transverse) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const K64& k = *kp;
+    const LodiNode ln = nodes[s];
+    double f[19], ps[19];
+    double rb, vb[3], r1, v1[3], r2, v2[3];
+    load19(A, N, ln.node, f);
+    moments64(k, f, rb, vb[0], vb[1], vb[2]);
+    load19(B, N, ln.n1, f);
+    if constexpr (CUBIC) {
+        load19(psi, N, ln.n1, ps);
+        moments64_psi(k, f, ps, r1, v1[0], v1[1], v1[2]);
+    } else {
+        moments64(k, f, r1, v1[0], v1[1], v1[2]);
+    }
+    load19(B, N, ln.n2, f);
+    if constexpr (CUBIC) {
+        load19(psi, N, ln.n2, ps);
+        moments64_psi(k, f, ps, r2, v2[0], v2[1], v2[2]);
+    } else {
+        moments64(k, f, r2, v2[0], v2[1], v2[2]);
+    }
+    const double ls = k.ls;
+    double dr = (4. * r1 - r2 - 3. * rb) / (-2. * ls);
+    double vg[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vg[a] = (4. * v1[a] - v2[a] - 3. * vb[a]) / (-2. * ls);
+    double dT[2] = {0., 0.};
+    double vgT[2][3] = {{0., 0., 0.}, {0., 0., 0.}};
+    if (!ln.corner && transverse) {
+        double rt[4], vt[4][3];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            load19(A, N, ln.t[t], f);
+            moments64(k, f, rt[t], vt[t][0], vt[t][1], vt[t][2]);
+        }
+        dT[0] = (rt[0] - rt[1]) / (2. * ls);
+        dT[1] = (rt[2] - rt[3]) / (2. * ls);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vgT[0][a] = (vt[0][a] - vt[1][a]) / (2. * ls);
+            vgT[1][a] = (vt[2][a] - vt[3][a]) / (2. * ls);
+        }
+    }
+    const int dir = ln.dir;
+    const bool neg = (dir == 0 || dir == 1 || dir == 4);
+    if (neg) {
+        dr *= -1.;
+        vg[0] *= -1.;
+        vg[1] *= -1.;
+        vg[2] *= -1.;
+    }
+    // local frame with the normal axis first (Node.cpp:303-413)
+    const int axis = (dir == 0 || dir == 2) ? 0 : ((dir == 1 || dir == 3) ? 1 : 2);
+    const int p0 = axis, p1 = axis == 0 ? 1 : 0, p2 = axis == 2 ? 1 : 2;
+    const double svg[3] = {vg[p0], vg[p1], vg[p2]};
+    const double svb[3] = {vb[p0], vb[p1], vb[p2]};
+    const double svgT[2][3] = {{vgT[0][p0], vgT[0][p1], vgT[0][p2]}, {vgT[1][p0], vgT[1][p1], vgT[1][p2]}};
+    const double L = k.dom_len[axis];
+    const double cs = k.cs, cs2 = k.lodi_cs2;
+    const double csDivL = cs / L;
+    const double K1 = k.k1, K2 = k.ma, MaSq = k.ma2;
+    const double rout = k.rho0;
+    const double dtl = k.dt_l;
+    const double Tex = 0.;
+    const double T1 = -(cs2 * (rb * (svgT[0][1] + svgT[1][2]) + svb[1] * dT[0] + svb[2] * dT[1]) -
+                        rb * cs * (svb[1] * svgT[0][0] + svb[2] * svgT[1][0]));
+    const double T5 = -(cs2 * (rb * (svgT[0][1] + svgT[1][2]) + svb[1] * dT[0] + svb[2] * dT[1]) +
+                        rb * cs * (svb[1] * svgT[0][0] + svb[2] * svgT[1][0]));
+    double L1, L5;
+    if (neg) {
+        L5 = K1 * (1 - MaSq) * csDivL * cs2 * (rb - rout) - K2 * (T5 - Tex) + T5;
+        L1 = (svb[0] - cs) * cs * (cs * dr - rb * svg[0]);
+    } else {
+        L1 = K1 * (1 - MaSq) * csDivL * cs2 * (rb - rout) - K2 * (T1 - Tex) + T1;
+        L5 = (svb[0] + cs) * cs * (cs * dr + rb * svg[0]);
+    }
+    const double L3 = svb[0] * svg[1];
+    const double L4 = svb[0] * svg[2];
+    const double T3 = -(svb[1] * svgT[0][1] + svb[2] * svgT[1][1]) - (cs2 / rb) * dT[0];
+    const double T4 = -(svb[1] * svgT[0][2] + svb[2] * svgT[1][2]) - (cs2 / rb) * dT[1];
+    const double rho = rb - dtl / (2. * cs2) * (L5 + L1) + dtl / (2. * cs2) * (T5 + T1);
+    const double lv0 = svb[0] - dtl / (2. * rb * cs) * (L5 - L1) + dtl / (2. * rb * cs) * (T5 - T1);
+    const double lv1 = svb[1] - dtl * L3 + dtl * T3;
+    const double lv2 = svb[2] - dtl * L4 + dtl * T4;
+    double u[3];
+    u[p0] = lv0;
+    u[p1] = lv1;
+    u[p2] = lv2;
+    double eq[19];
+    feq64<ORDER>(k, rho, u[0], u[1], u[2], eq);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        B[q * N + ln.node] = eq[q];
+    });
+}
+
+hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, const double* A, double* B,
+                         const double* psi, const LodiNode* nodes, int64_t count, int64_t N, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const int bs = 256;
+    dim3 grid((unsigned)((count + bs - 1) / bs));
+    const int tr = transverse ? 1 : 0;
+#define L64(O, C) hipLaunchKernelGGL((lodi64<O, C>), grid, dim3(bs), 0, s, k, A, B, psi, nodes, count, N, tr)
+    if (cubic) {
+        if (order == 2) L64(2, true);
+        else if (order == 3) L64(3, true);
+        else L64(4, true);
+    } else {
+        if (order == 2) L64(2, false);
+        else if (order == 3) L64(3, false);
+        else L64(4, false);
+    }
+#undef L64
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ macro pre-pass
+// Pre-collision density and velocity of every node (calcDensityPreCol /
+// calcVelocityPreCol, Node.cpp:32-51, 121-154) for the HRR strain stencil.
+__global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                               double* __restrict__ M, int64_t N) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    double f[19], r, ux, uy, uz;
+    load19(A, N, l, f);
+    moments64(*kp, f, r, ux, uy, uz);
+    M[l] = r;
+    M[N + l] = ux;
+    M[2 * N + l] = uy;
+    M[3 * N + l] = uz;
+}
+
+hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Grid& g, bool, hipStream_t s) {
+    const int bs = 256;
+    hipLaunchKernelGGL(macro64, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, k, A, macro, g.N);
+    return hipGetLastError();
+}
+
+// macros in the output layout rho[N], u[3][N]
+__global__ __launch_bounds__(256) void macros_out64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                                    double* __restrict__ rho, double* __restrict__ u, int64_t N) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    double f[19], r, ux, uy, uz;
+    load19(A, N, l, f);
+    moments64(*kp, f, r, ux, uy, uz);
+    rho[l] = r;
+    u[l] = ux;
+    u[N + l] = uy;
+    u[2 * N + l] = uz;
+}
+
+hipError_t launch_macros_out64(const K64* k, const double* A, const Grid& g, double* rho, double* u,
+                               hipStream_t s) {
+    const int bs = 256;
+    hipLaunchKernelGGL(macros_out64, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, k, A, rho, u, g.N);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ random initial condition
+// Philox-4x32-10 (Salmon et al., SC'11): counter = node index, key = seed.
+__device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += W0;
+        k1 += W1;
+    }
+}
+
+template <int ORDER>
+__global__ __launch_bounds__(256) void init_random64(const K64* __restrict__ kp, double* __restrict__ A,
+                                                     double* __restrict__ B, int64_t N, uint64_t seed, double rho0,
+                                                     double rho_amp, double u_amp) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    uint32_t c[4] = {(uint32_t)l, (uint32_t)((uint64_t)l >> 32), 0u, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double uu[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) uu[a] = ((double)c[a] + 0.5) * (1.0 / 4294967296.0);
+    const double two_pi = 6.283185307179586476925286766559;
+    const double r0 = sqrt(-2.0 * log(uu[0])), r1 = sqrt(-2.0 * log(uu[2]));
+    const double n0 = r0 * cos(two_pi * uu[1]), n1 = r0 * sin(two_pi * uu[1]);
+    const double n2 = r1 * cos(two_pi * uu[3]), n3 = r1 * sin(two_pi * uu[3]);
+    const K64& k = *kp;
+    const double rho = rho0 * (1.0 + rho_amp * n0);
+    const double us = u_amp * k.c;
+    double eq[19];
+    feq64<ORDER>(k, rho, us * n1, us * n2, us * n3, eq);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        A[q * N + l] = eq[q];
+        B[q * N + l] = eq[q];
+    });
+}
+
+hipError_t launch_init_random64(int order, const K64* k, double* A, double* B, const Grid& g, uint64_t seed,
+                               double rho0, double rho_amp, double u_amp, hipStream_t s) {
+    const int bs = 256;
+    dim3 grid((unsigned)((g.N + bs - 1) / bs));
+    if (order == 2) hipLaunchKernelGGL(init_random64<2>, grid, dim3(bs), 0, s, k, A, B, g.N, seed, rho0, rho_amp, u_amp);
+    else if (order == 3) hipLaunchKernelGGL(init_random64<3>, grid, dim3(bs), 0, s, k, A, B, g.N, seed, rho0, rho_amp, u_amp);
+    else hipLaunchKernelGGL(init_random64<4>, grid, dim3(bs), 0, s, k, A, B, g.N, seed, rho0, rho_amp, u_amp);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ probes and mass
+__global__ void probe64(const double* __restrict__ A, int64_t N, const int64_t* __restrict__ nodes, int nprobe,
+                        double* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprobe) return;
+    const int64_t l = nodes[p];
+    double r = 0.;
+#pragma unroll
+    for (int q = 0; q < 19; ++q) r += A[q * N + l];
+    out[p] = r;
+}
+
+hipError_t launch_probe64(const double* A, int64_t N, const int64_t* nodes, int nprobe, double* out,
+                          hipStream_t s) {
+    if (nprobe <= 0) return hipSuccess;
+    hipLaunchKernelGGL(probe64, dim3((nprobe + 63) / 64), dim3(64), 0, s, A, N, nodes, nprobe, out);
+    return hipGetLastError();
+}
+
+// per-block partial sums of sum_q f_q in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void mass64(const double* __restrict__ A, int64_t N, double* __restrict__ partial) {
+    __shared__ double sh[256];
+    double acc = 0.;
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < N; l += (int64_t)gridDim.x * blockDim.x) {
+        double r = 0.;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) r += A[q * N + l];
+        acc += r;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+hipError_t launch_mass64(const double* A, int64_t N, double* partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(mass64, dim3(nblocks), dim3(256), 0, s, A, N, partial);
+    return hipGetLastError();
+}
+
+}  // namespace acp

@@ -1,0 +1,575 @@
+// Shifted-population tiers: float32 collision in lattice units on stored
+// deviations g_q = s (f_q/rho0 - w_q), float32 (s = 1) or float16 storage.
+//
+// The kernels follow the float64 tier step for step (collide + push with the
+// same push rule, the same LODI pass order and phases) but read/write 4 or 2
+// bytes per population: 2*19*4 = 152 and 2*19*2 = 76 bytes per node and step.
+// Global reductions (mass, probes) and conversions to SI are done in float64.
+#include "kshift.hpp"
+#include "physf.hpp"
+
+namespace acp {
+
+template <typename T>
+__device__ __forceinline__ float ld(const T* __restrict__ p) {
+    return (float)(*p);
+}
+
+template <typename T>
+__device__ __forceinline__ void load_dev(const T* __restrict__ F, int64_t N, int64_t l, float inv_s, float d[19]) {
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        d[q] = (float)F[q * N + l] * inv_s;
+    });
+}
+
+// ------------------------------------------------------------------ bulk collide + push
+template <typename T, int OP, int ORDER, bool PERIODIC>
+__global__ __launch_bounds__(512) void bulk_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                  T* __restrict__ B, const float* __restrict__ M, int nx, int ny,
+                                                  int nz, int64_t N) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nx) return;
+    const int j = blockIdx.y, kz = blockIdx.z;
+    const int64_t sxy = (int64_t)nx * ny;
+    const int64_t l = i + (int64_t)nx * j + sxy * kz;
+    const KShift& k = *kp;
+    float g[19];
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        g[q] = (float)A[q * N + l];
+    });
+    int bi = 0, bj = 0, bk = 0;
+    if constexpr (!PERIODIC) {
+        bi = i == 0 ? 1 : (i == nx - 1 ? -1 : 0);
+        bj = j == 0 ? 1 : (j == ny - 1 ? -1 : 0);
+        bk = kz == 0 ? 1 : (kz == nz - 1 ? -1 : 0);
+    }
+    const bool boundary = (bi | bj | bk) != 0;
+    int64_t oxp = 1, oxm = -1, oyp = nx, oym = -(int64_t)nx, ozp = sxy, ozm = -sxy;
+    if constexpr (PERIODIC) {
+        if (i == nx - 1) oxp = 1 - nx;
+        if (i == 0) oxm = nx - 1;
+        if (j == ny - 1) oyp = -(int64_t)nx * (ny - 1);
+        if (j == 0) oym = (int64_t)nx * (ny - 1);
+        if (kz == nz - 1) ozp = -sxy * (nz - 1);
+        if (kz == 0) ozm = sxy * (nz - 1);
+    }
+    if constexpr (OP != OP_STREAM) {
+        if (!boundary) {
+            float d[19];
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                d[q] = g[q] * k.inv_s;
+            });
+            float dr, ux, uy, uz;
+            moments_dev(d, dr, ux, uy, uz);
+            float eq[19];
+            if constexpr (OP == OP_MRT_RM) rm_feq_dev(dr, ux, uy, uz, eq);
+            else if constexpr (OP != OP_MRT_GS) feq_dev<ORDER>(dr, ux, uy, uz, eq);
+            const float om1 = 1.f - k.omega;
+            if constexpr (OP == OP_SRT) {
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    d[q] = eq[q] + om1 * (d[q] - eq[q]);
+                });
+            } else if constexpr (OP == OP_RR || OP == OP_HRR) {
+                float fneq[19], a2[6], f1[19];
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    fneq[q] = d[q] - eq[q];
+                });
+                pi1_dev(fneq, a2);
+                if constexpr (OP == OP_HRR) {
+                    // FD strain of the neighbours' pre-collision velocity (Node.cpp:1750-1889), dx = 1
+                    const int64_t o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+                    float v[6][3];
+#pragma unroll
+                    for (int m = 0; m < 6; ++m)
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) v[m][a] = M[(a + 1) * N + l + o[m]];
+                    const float S[6] = {0.5f * (v[0][0] - v[2][0]), 0.5f * (v[1][1] - v[3][1]),
+                                        0.5f * (v[4][2] - v[5][2]),
+                                        0.5f * (0.5f * (v[0][1] - v[2][1]) + 0.5f * (v[1][0] - v[3][0])),
+                                        0.5f * (0.5f * (v[0][2] - v[2][2]) + 0.5f * (v[4][0] - v[5][0])),
+                                        0.5f * (0.5f * (v[1][2] - v[3][2]) + 0.5f * (v[4][1] - v[5][1]))};
+                    const float c = (1.f - k.sigma) * k.hrr_pre * (1.f + dr);
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) a2[a] = k.sigma * a2[a] + c * S[a];
+                }
+                offeq_dev<ORDER>(a2, ux, uy, uz, f1);
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    d[q] = eq[q] + om1 * f1[q];
+                });
+            } else if constexpr (OP == OP_MRT_RM) {
+                // feq_RM + (1-omega) M^-1 (non-equilibrium of N_xz, N_yz, Pi_xy, Pi_xz, Pi_yz);
+                // every other raw moment relaxes with rate 1 (Node.cpp:2573-2600)
+                float n5 = 0.f, n6 = 0.f, nxy = 0.f, nxz = 0.f, nyz = 0.f;
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    const float f = d[q] - eq[q];
+                    constexpr int c5 = EX[q] * EX[q] - EZ[q] * EZ[q], c6 = EY[q] * EY[q] - EZ[q] * EZ[q];
+                    if constexpr (c5 > 0) n5 += f;
+                    if constexpr (c5 < 0) n5 -= f;
+                    if constexpr (c6 > 0) n6 += f;
+                    if constexpr (c6 < 0) n6 -= f;
+                    if constexpr (EX[q] * EY[q] > 0) nxy += f;
+                    if constexpr (EX[q] * EY[q] < 0) nxy -= f;
+                    if constexpr (EX[q] * EZ[q] > 0) nxz += f;
+                    if constexpr (EX[q] * EZ[q] < 0) nxz -= f;
+                    if constexpr (EY[q] * EZ[q] > 0) nyz += f;
+                    if constexpr (EY[q] * EZ[q] < 0) nyz -= f;
+                });
+                const float t3 = 1.f / 3.f, t6 = 1.f / 6.f;
+                const float ax = om1 * (t3 * n5 - t6 * n6), ay = om1 * (t3 * n6 - t6 * n5), az = -om1 * t6 * (n5 + n6);
+                const float dxy = 0.25f * om1 * nxy, dxz = 0.25f * om1 * nxz, dyz = 0.25f * om1 * nyz;
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    float add = 0.f;
+                    if constexpr (EX[q] != 0 && EY[q] == 0 && EZ[q] == 0) add = ax;
+                    if constexpr (EY[q] != 0 && EX[q] == 0 && EZ[q] == 0) add = ay;
+                    if constexpr (EZ[q] != 0 && EX[q] == 0 && EY[q] == 0) add = az;
+                    if constexpr (EX[q] * EY[q] != 0) add = (float)(EX[q] * EY[q]) * dxy;
+                    if constexpr (EX[q] * EZ[q] != 0) add = (float)(EX[q] * EZ[q]) * dxz;
+                    if constexpr (EY[q] * EZ[q] != 0) add = (float)(EY[q] * EZ[q]) * dyz;
+                    d[q] = eq[q] + add;
+                });
+            } else if constexpr (OP == OP_MRT_GS) {
+                // Gram-Schmidt moments of the deviations (the rest state has only m0 = 1),
+                // relaxation (Node.cpp:2798-2835) and inverse transform f += M^-1 dm
+                float m[19];
+                static_for<0, 19>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    float s = 0.f;
+                    static_for<0, 19>([&](auto qc) {
+                        constexpr int q = decltype(qc)::value;
+                        constexpr int a = gsf_coef(r, q);
+                        if constexpr (a != 0) s += (float)a * d[q];
+                    });
+                    m[r] = s;
+                });
+                const float rho = 1.f + m[0];
+                const float ir = 1.f / rho, jx = m[3], jy = m[5], jz = m[7];
+                const float xx = jx * jx, yy = jy * jy, zz = jz * jz;
+                float meq[19] = {0.f};
+                meq[1] = (xx + yy + zz) * ir;
+                meq[9] = (2.f * xx - yy - zz) * ir;
+                meq[11] = (yy - zz) * ir;
+                meq[13] = jx * jy * ir;
+                meq[14] = jy * jz * ir;
+                meq[15] = jx * jz * ir;
+                float dm[19];
+                static_for<0, 19>([&](auto rc) {
+                    constexpr int r = decltype(rc)::value;
+                    if constexpr (r == 0 || r == 3 || r == 5 || r == 7) dm[r] = 0.f;
+                    else if constexpr (r == 1 || r == 9 || r == 11 || r == 13 || r == 14 || r == 15)
+                        dm[r] = k.omega * (meq[r] - m[r]);
+                    else dm[r] = meq[r] - m[r];
+                });
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    float s = 0.f;
+                    static_for<0, 19>([&](auto rc) {
+                        constexpr int r = decltype(rc)::value;
+                        constexpr int a = gsf_coef(r, q);
+                        if constexpr (a != 0) {
+                            constexpr float coef = wf(q) * (float)a / gsf_norm(r);
+                            s += coef * dm[r];
+                        }
+                    });
+                    d[q] += s;
+                });
+            }
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                g[q] = d[q] * k.s;
+            });
+        }
+    }
+    static_for<0, 18>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (!PERIODIC) {
+            if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
+        }
+        int64_t dd = l;
+        if constexpr (EX[q] > 0) dd += oxp;
+        if constexpr (EX[q] < 0) dd += oxm;
+        if constexpr (EY[q] > 0) dd += oyp;
+        if constexpr (EY[q] < 0) dd += oym;
+        if constexpr (EZ[q] > 0) dd += ozp;
+        if constexpr (EZ[q] < 0) dd += ozm;
+        B[q * N + dd] = (T)g[q];
+    });
+    B[18 * N + l] = (T)g[18];
+}
+
+template <typename T, int OP, int ORDER>
+static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B, const float* M, const Grid& g,
+                          hipStream_t s) {
+    const int bx = g.nx >= 512 ? 512 : ((g.nx + 63) / 64) * 64;
+    dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
+    if (periodic)
+        hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, true>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny,
+                           g.nz, g.N);
+    else
+        hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, false>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx,
+                           g.ny, g.nz, g.N);
+    return hipGetLastError();
+}
+
+template <typename T>
+static hipError_t bulk_t(int op, int order, bool periodic, const KShift* k, const void* A, void* B, const float* M,
+                         const Grid& g, hipStream_t s) {
+#define BO(OPC)                                                                  \
+    switch (order) {                                                             \
+        case 2: return bulk_go<T, OPC, 2>(periodic, k, A, B, M, g, s);           \
+        case 3: return bulk_go<T, OPC, 3>(periodic, k, A, B, M, g, s);           \
+        case 4: return bulk_go<T, OPC, 4>(periodic, k, A, B, M, g, s);           \
+        default: return hipErrorInvalidValue;                                    \
+    }
+    switch (op) {
+        case OP_SRT: BO(OP_SRT)
+        case OP_RR: BO(OP_RR)
+        case OP_HRR: BO(OP_HRR)
+        case OP_MRT_RM: return bulk_go<T, OP_MRT_RM, 3>(periodic, k, A, B, M, g, s);
+        case OP_MRT_GS: return bulk_go<T, OP_MRT_GS, 3>(periodic, k, A, B, M, g, s);
+        case OP_STREAM: return bulk_go<T, OP_STREAM, 3>(periodic, k, A, B, M, g, s);
+        default: return hipErrorInvalidValue;
+    }
+#undef BO
+}
+
+hipError_t launch_bulk_shift(int prec, int op, int order, bool periodic, const KShift* k, const void* A, void* B,
+                             const float* macro, const Grid& g, hipStream_t s) {
+    if (prec == ACP_PREC_F32_SHIFTED) return bulk_t<float>(op, order, periodic, k, A, B, macro, g, s);
+    return bulk_t<_Float16>(op, order, periodic, k, A, B, macro, g, s);
+}
+
+// ------------------------------------------------------------------ HRR pre-pass
+template <typename T>
+__global__ __launch_bounds__(256) void macro_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                   float* __restrict__ M, int64_t N) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    float d[19], dr, ux, uy, uz;
+    load_dev(A, N, l, kp->inv_s, d);
+    moments_dev(d, dr, ux, uy, uz);
+    M[l] = dr;
+    M[N + l] = ux;
+    M[2 * N + l] = uy;
+    M[3 * N + l] = uz;
+}
+
+hipError_t launch_macro_shift(int prec, const KShift* k, const void* A, float* macro, const Grid& g, hipStream_t s) {
+    dim3 grid((unsigned)((g.N + 255) / 256));
+    if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(macro_shift<float>, grid, dim3(256), 0, s, k, (const float*)A, macro, g.N);
+    else
+        hipLaunchKernelGGL(macro_shift<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)A, macro, g.N);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ LODI
+// Node.cpp:229-504 in lattice units on deviations: (rho_b - rho0) and the density
+// differences of the one-sided gradient are formed from the stored deviations.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                  T* __restrict__ B, const LodiNode* __restrict__ nodes, int64_t count,
+                                                  int64_t N, int transverse) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const KShift& k = *kp;
+    const LodiNode ln = nodes[s];
+    float d[19];
+    float rb, vb[3], r1, v1[3], r2, v2[3];
+    load_dev(A, N, ln.node, k.inv_s, d);
+    moments_dev(d, rb, vb[0], vb[1], vb[2]);
+    load_dev(B, N, ln.n1, k.inv_s, d);
+    moments_dev(d, r1, v1[0], v1[1], v1[2]);
+    load_dev(B, N, ln.n2, k.inv_s, d);
+    moments_dev(d, r2, v2[0], v2[1], v2[2]);
+    float dr = (4.f * r1 - r2 - 3.f * rb) * -0.5f;
+    float vg[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vg[a] = (4.f * v1[a] - v2[a] - 3.f * vb[a]) * -0.5f;
+    float dT[2] = {0.f, 0.f};
+    float vgT[2][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    if (!ln.corner && transverse) {
+        float rt[4], vt[4][3];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            load_dev(A, N, ln.t[t], k.inv_s, d);
+            moments_dev(d, rt[t], vt[t][0], vt[t][1], vt[t][2]);
+        }
+        dT[0] = 0.5f * (rt[0] - rt[1]);
+        dT[1] = 0.5f * (rt[2] - rt[3]);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            vgT[0][a] = 0.5f * (vt[0][a] - vt[1][a]);
+            vgT[1][a] = 0.5f * (vt[2][a] - vt[3][a]);
+        }
+    }
+    const int dir = ln.dir;
+    const bool neg = (dir == 0 || dir == 1 || dir == 4);
+    if (neg) {
+        dr = -dr;
+        vg[0] = -vg[0];
+        vg[1] = -vg[1];
+        vg[2] = -vg[2];
+    }
+    const int axis = (dir == 0 || dir == 2) ? 0 : ((dir == 1 || dir == 3) ? 1 : 2);
+    const int p0 = axis, p1 = axis == 0 ? 1 : 0, p2 = axis == 2 ? 1 : 2;
+    const float svg[3] = {vg[p0], vg[p1], vg[p2]};
+    const float svb[3] = {vb[p0], vb[p1], vb[p2]};
+    const float svgT[2][3] = {{vgT[0][p0], vgT[0][p1], vgT[0][p2]}, {vgT[1][p0], vgT[1][p1], vgT[1][p2]}};
+    const float cs = 0.57735026918962576f, cs2 = 1.f / 3.f;
+    const float csDivL = cs * k.inv_len[axis];
+    const float rho_b = 1.f + rb;
+    const float T1 = -(cs2 * (rho_b * (svgT[0][1] + svgT[1][2]) + svb[1] * dT[0] + svb[2] * dT[1]) -
+                       rho_b * cs * (svb[1] * svgT[0][0] + svb[2] * svgT[1][0]));
+    const float T5 = -(cs2 * (rho_b * (svgT[0][1] + svgT[1][2]) + svb[1] * dT[0] + svb[2] * dT[1]) +
+                       rho_b * cs * (svb[1] * svgT[0][0] + svb[2] * svgT[1][0]));
+    const float Lin = k.k1 * (1.f - k.ma2) * csDivL * cs2 * rb;
+    float L1, L5;
+    if (neg) {
+        L5 = Lin - k.ma * T5 + T5;
+        L1 = (svb[0] - cs) * cs * (cs * dr - rho_b * svg[0]);
+    } else {
+        L1 = Lin - k.ma * T1 + T1;
+        L5 = (svb[0] + cs) * cs * (cs * dr + rho_b * svg[0]);
+    }
+    const float L3 = svb[0] * svg[1], L4 = svb[0] * svg[2];
+    const float T3 = -(svb[1] * svgT[0][1] + svb[2] * svgT[1][1]) - (cs2 / rho_b) * dT[0];
+    const float T4 = -(svb[1] * svgT[0][2] + svb[2] * svgT[1][2]) - (cs2 / rho_b) * dT[1];
+    const float dr_new = rb - 1.5f * (L5 + L1) + 1.5f * (T5 + T1);
+    const float c2 = 1.f / (2.f * rho_b * cs);
+    float u[3];
+    u[p0] = svb[0] - c2 * (L5 - L1) + c2 * (T5 - T1);
+    u[p1] = svb[1] - L3 + T3;
+    u[p2] = svb[2] - L4 + T4;
+    float eq[19];
+    feq_dev<ORDER>(dr_new, u[0], u[1], u[2], eq);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        B[q * N + ln.node] = (T)(eq[q] * k.s);
+    });
+}
+
+hipError_t launch_lodi_shift(int prec, int order, bool transverse, const KShift* k, const void* A, void* B,
+                             const LodiNode* nodes, int64_t count, int64_t N, hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    dim3 grid((unsigned)((count + 255) / 256)), block(256);
+    const int tr = transverse ? 1 : 0;
+#define LS(T, O) hipLaunchKernelGGL((lodi_shift<T, O>), grid, block, 0, s, k, (const T*)A, (T*)B, nodes, count, N, tr)
+    if (prec == ACP_PREC_F32_SHIFTED) {
+        if (order == 2) LS(float, 2);
+        else if (order == 3) LS(float, 3);
+        else LS(float, 4);
+    } else {
+        if (order == 2) LS(_Float16, 2);
+        else if (order == 3) LS(_Float16, 3);
+        else LS(_Float16, 4);
+    }
+#undef LS
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ conversions
+template <typename T>
+__global__ void to_store(const KShift* __restrict__ kp, const double* __restrict__ src, T* __restrict__ dA,
+                         T* __restrict__ dB, int64_t n, double w) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const T v = (T)((double)kp->s * (src[l] / kp->rho0 - w));
+    dA[l] = v;
+    if (dB) dB[l] = v;
+}
+
+template <typename T>
+__global__ void from_store(const KShift* __restrict__ kp, const T* __restrict__ src, double* __restrict__ dst,
+                           int64_t n, double w) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    dst[l] = kp->rho0 * (w + (double)src[l] / (double)kp->s);
+}
+
+hipError_t launch_to_store(int prec, const KShift* k, const double* src, void* dA, void* dB, int64_t n, int q,
+                           hipStream_t s) {
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(to_store<float>, grid, dim3(256), 0, s, k, src, (float*)dA, (float*)dB, n, weight(q));
+    else
+        hipLaunchKernelGGL(to_store<_Float16>, grid, dim3(256), 0, s, k, src, (_Float16*)dA, (_Float16*)dB, n,
+                           weight(q));
+    return hipGetLastError();
+}
+
+hipError_t launch_from_store(int prec, const KShift* k, const void* src, double* dst, int64_t n, int q,
+                             hipStream_t s) {
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(from_store<float>, grid, dim3(256), 0, s, k, (const float*)src, dst, n, weight(q));
+    else
+        hipLaunchKernelGGL(from_store<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)src, dst, n, weight(q));
+    return hipGetLastError();
+}
+
+// density and velocity in float64 (SI) from the stored deviations
+template <typename T>
+__device__ __forceinline__ void macro_f64(const KShift& k, const T* __restrict__ A, int64_t N, int64_t l, double& rho,
+                                          double& ux, double& uy, double& uz) {
+    double r = 0., jx = 0., jy = 0., jz = 0.;
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const double v = (double)A[q * N + l];
+        r += v;
+        if constexpr (EX[q] != 0) jx += EX[q] * v;
+        if constexpr (EY[q] != 0) jy += EY[q] * v;
+        if constexpr (EZ[q] != 0) jz += EZ[q] * v;
+    });
+    const double is = 1.0 / (double)k.s;
+    const double rl = 1.0 + r * is;
+    rho = k.rho0 * rl;
+    ux = jx * is / rl * k.c_si;
+    uy = jy * is / rl * k.c_si;
+    uz = jz * is / rl * k.c_si;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void macros_out_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                        double* __restrict__ rho, double* __restrict__ u, int64_t N) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    double r, ux, uy, uz;
+    macro_f64(*kp, A, N, l, r, ux, uy, uz);
+    rho[l] = r;
+    u[l] = ux;
+    u[N + l] = uy;
+    u[2 * N + l] = uz;
+}
+
+hipError_t launch_macros_out_shift(int prec, const KShift* k, const void* A, const Grid& g, double* rho, double* u,
+                                   hipStream_t s) {
+    dim3 grid((unsigned)((g.N + 255) / 256));
+    if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(macros_out_shift<float>, grid, dim3(256), 0, s, k, (const float*)A, rho, u, g.N);
+    else
+        hipLaunchKernelGGL(macros_out_shift<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)A, rho, u, g.N);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ random initial condition
+__device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    const uint32_t M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)M0 * c[0], p1 = (uint64_t)M1 * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[1] = (uint32_t)p1;
+        c[3] = (uint32_t)p0;
+        c[0] = n0;
+        c[2] = n2;
+        k0 += W0;
+        k1 += W1;
+    }
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(256) void init_random_shift(const KShift* __restrict__ kp, T* __restrict__ A,
+                                                         T* __restrict__ B, int64_t N, uint64_t seed, double rho_amp,
+                                                         double u_amp) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= N) return;
+    uint32_t c[4] = {(uint32_t)l, (uint32_t)((uint64_t)l >> 32), 0u, 0u};
+    philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    double uu[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) uu[a] = ((double)c[a] + 0.5) * (1.0 / 4294967296.0);
+    const double two_pi = 6.283185307179586476925286766559;
+    const double r0 = sqrt(-2.0 * log(uu[0])), r1 = sqrt(-2.0 * log(uu[2]));
+    const double n0 = r0 * cos(two_pi * uu[1]), n1 = r0 * sin(two_pi * uu[1]);
+    const double n2 = r1 * cos(two_pi * uu[3]), n3 = r1 * sin(two_pi * uu[3]);
+    float eq[19];
+    feq_dev<ORDER>((float)(rho_amp * n0), (float)(u_amp * n1), (float)(u_amp * n2), (float)(u_amp * n3), eq);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const T v = (T)(eq[q] * kp->s);
+        A[q * N + l] = v;
+        B[q * N + l] = v;
+    });
+}
+
+hipError_t launch_init_random_shift(int prec, int order, const KShift* k, void* A, void* B, const Grid& g,
+                                    uint64_t seed, double rho_amp, double u_amp, hipStream_t s) {
+    dim3 grid((unsigned)((g.N + 255) / 256)), block(256);
+#define IR(T, O) hipLaunchKernelGGL((init_random_shift<T, O>), grid, block, 0, s, k, (T*)A, (T*)B, g.N, seed, rho_amp, u_amp)
+    if (prec == ACP_PREC_F32_SHIFTED) {
+        if (order == 2) IR(float, 2);
+        else if (order == 3) IR(float, 3);
+        else IR(float, 4);
+    } else {
+        if (order == 2) IR(_Float16, 2);
+        else if (order == 3) IR(_Float16, 3);
+        else IR(_Float16, 4);
+    }
+#undef IR
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ probes and mass (float64)
+template <typename T>
+__global__ void probe_shift(const KShift* __restrict__ kp, const T* __restrict__ A, int64_t N,
+                            const int64_t* __restrict__ nodes, int nprobe, double* __restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprobe) return;
+    double r, ux, uy, uz;
+    macro_f64(*kp, A, N, nodes[p], r, ux, uy, uz);
+    out[p] = r;
+}
+
+hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, int64_t N, const int64_t* nodes, int nprobe,
+                              double* out, hipStream_t s) {
+    if (nprobe <= 0) return hipSuccess;
+    dim3 grid((nprobe + 63) / 64), block(64);
+    if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(probe_shift<float>, grid, block, 0, s, k, (const float*)A, N, nodes, nprobe, out);
+    else
+        hipLaunchKernelGGL(probe_shift<_Float16>, grid, block, 0, s, k, (const _Float16*)A, N, nodes, nprobe, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void mass_shift(const KShift* __restrict__ kp, const T* __restrict__ A, int64_t N,
+                                                  double* __restrict__ partial) {
+    __shared__ double sh[256];
+    double acc = 0.;
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < N; l += (int64_t)gridDim.x * blockDim.x) {
+        double r = 0.;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) r += (double)A[q * N + l];
+        acc += r;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        // sum over this block's nodes of rho0 (1 + sum_q g_q / s): the node count part is
+        // added on the host from the grid size, here only the deviation part
+        partial[blockIdx.x] = kp->rho0 * sh[0] / (double)kp->s;
+    }
+}
+
+hipError_t launch_mass_shift(int prec, const KShift* k, const void* A, int64_t N, double* partial, int nblocks,
+                             hipStream_t s) {
+    if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(mass_shift<float>, dim3(nblocks), dim3(256), 0, s, k, (const float*)A, N, partial);
+    else
+        hipLaunchKernelGGL(mass_shift<_Float16>, dim3(nblocks), dim3(256), 0, s, k, (const _Float16*)A, N, partial);
+    return hipGetLastError();
+}
+
+}  // namespace acp

@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: D3Q19 raw-moment MRT on a periodic 512^3 grid.
+
+Workload (one GPU): 512^3 = 134,217,728 nodes, MRT raw-moment collision with
+the relaxation applied, periodic, initial rho = 1 + 1e-3 N(0,1) and
+u = 1e-3 N(0,1) (lattice units, Philox seed 0), tau = 0.503.  A step is one
+collide+push pass over all nodes.  W warm-up steps, then K timed steps between
+HIP events recorded on the case stream.  Runs the float64 tier (the reported
+value) and the float32-shifted tier (reported beside it).
+
+GLUPS counts every node of the grid per step.  The roofline figure uses the
+algorithmic bytes of the bulk kernel, 2*19*8 = 304 B (float64) and 2*19*4 =
+152 B (float32) per node and step, over the measured average step time; with
+no boundary pass on a periodic grid the step is one kernel launch.
+"""
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def run_tier(acp, torch, precision, n, steps, warmup, seed=0):
+    stream = torch.cuda.Stream()
+    d = acp.make_desc(n, n, n, op="MRT-RM", order=3, boundary="PERIODIC", mrt_mode="RELAXED",
+                      precision=precision, tau=0.503)
+    case = acp.Case(d, stream=stream.cuda_stream)
+    try:
+        case.init_random(seed, 1e-3, 1e-3)
+        m0 = case.mass()
+        case.step(warmup)
+        case.sync()
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        case.step(steps)
+        t1.record(stream)
+        t1.synchronize()
+        torch.cuda.synchronize()
+        ms = t0.elapsed_time(t1)
+        m1 = case.mass()
+        info = case.info()
+    finally:
+        case.close()
+        del case
+        torch.cuda.synchronize()
+    ms_step = ms / steps
+    nodes = n ** 3
+    glups = nodes / (ms_step * 1e-3) / 1e9
+    bpn = 304 if precision == "F64" else (152 if precision == "F32-SHIFTED" else 76)
+    achieved = nodes * bpn / (ms_step * 1e-3) / 1e9
+    return {
+        "glups": glups, "ms_per_step": ms_step, "bytes_per_node_step": bpn, "achieved_gbs": achieved,
+        "frac": achieved / HBM_PEAK_GBS, "mass_drift_rel": abs(m1 - m0) / m0, "device_bytes": info.device_bytes,
+    }
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
+    read from the committed summary (profiles/<name>), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"hbm_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(sample_n=96, budget_s=12.0):
+    """The CPU oracle (C, OpenMP) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_lib as ol
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, 16))
+    p = ol.make_params(sample_n, sample_n, sample_n, op="MRT-RM", order=3, periodic=True, mrt_mode=1, tau=0.503,
+                       nthreads=threads, cs=1.0 / math.sqrt(3.0), spacing=1.0, rho0=1.0)
+    N = sample_n ** 3
+    rng = np.random.default_rng(0)
+    rho = 1.0 + 1e-3 * rng.standard_normal(N)
+    u = 1e-3 * rng.standard_normal((3, N))
+    c = ol.OracleCase(p)
+    c.set_state(ol.equilibrium_field(p, rho, u))
+    c.step(1)
+    steps, t = 0, 0.0
+    while t < budget_s and steps < 200:
+        t0 = time.perf_counter()
+        c.step(2)
+        t += time.perf_counter() - t0
+        steps += 2
+    glups = N * steps / t / 1e9
+    return {"value": glups, "unit": "GLUPS", "cores": threads, "kind": "port",
+            "sample": f"{sample_n}^3 periodic MRT-RM relaxed, float64 oracle, {steps} steps in {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tiers", default="F64,F32-SHIFTED")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
+              "(a single case is sized for one MI355X; see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    import acpulse_vclbm_amd as acp
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU", file=sys.stderr)
+        sys.exit(1)
+    torch.cuda.set_device(0)
+    tiers = a.tiers.split(",")
+    res = {}
+    for t in tiers:
+        res[t] = run_tier(acp, torch, t, a.n, a.steps, a.warmup)
+        print(f"# {t}: {res[t]['glups']:.2f} GLUPS, {res[t]['ms_per_step']:.3f} ms/step, "
+              f"{100 * res[t]['frac']:.1f}% of HBM peak", file=sys.stderr, flush=True)
+    main_t = "F64" if "F64" in res else tiers[0]
+    r = res[main_t]
+    cpu = None if a.no_cpu_baseline else cpu_baseline()
+    out = {
+        "metric": "GLUPS & %HBM roofline, FP64 vs FP32-shifted; p50/p99 case latency",
+        "value": round(r["glups"], 4),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(r["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if main_t == "F64" else "f32",
+        "data": "synthetic (Philox seed 0 fluctuation field)",
+        "config": {"workload": f"D3Q19 MRT raw-moment (relaxation applied), {a.n}^3 periodic, tau 0.503, "
+                               f"rho=1+1e-3 N(0,1), u=1e-3 N(0,1)", "grid": [a.n, a.n, a.n],
+                   "nodes": a.n ** 3, "tiers": tiers},
+        "roofline": {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(r["frac"], 4),
+                     "traffic": traffic_from_profiles(f"bulk64_mrtrm_periodic_{a.n}.txt")},
+        "tiers": {t: {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res[t].items()} for t in res},
+        "cpu_baseline": cpu,
+    }
+    if "F32-SHIFTED" in res and "F64" in res:
+        out["fp32_shifted_over_fp64"] = round(res["F32-SHIFTED"]["glups"] / res["F64"]["glups"], 3)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -140,9 +140,11 @@ int acp_case_mass(acp_case* c, double* mass);
 int acp_case_set_probes(acp_case* c, const int64_t* nodes, int32_t nprobe, int32_t capacity);
 int acp_case_read_probes(acp_case* c, double* out, int32_t* nsteps);
 
-/* Lattice::writeResultsVTK(string, bool&, bool&) (Lattice.cpp:4094-4349) for a
- * single level: legacy ASCII VTK of Pressure, Density, Omega and Velocity. */
-int acp_case_write_vtk(acp_case* c, const char* filename);
+/* Introspection, no counterpart in the reference: the boundary nodes of an
+ * nx*ny*nz LODI case in the order the engine processes them (phase by phase;
+ * within a phase in the reference's pass order, Lattice.cpp:1582-1624) and the
+ * phase of each.  Returns the node count; writes at most `cap` entries. */
+int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t* nodes, int32_t* phases, int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -958,3 +958,15 @@ void acpo_pulse_ic(const acpo_params* p, double amp, double x0, double y0, doubl
                 for (int q = 0; q < 19; ++q) f[(size_t)q * N + l] = eq[q];
             }
 }
+
+void acpo_equilibrium_field(const acpo_params* p, int64_t N, const double* rho, const double* u, double* f) {
+    acpo_consts k;
+    acpo_consts_init(&k, p);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads > 0 ? p->nthreads : 1)
+    for (int64_t l = 0; l < N; ++l) {
+        const double uu[3] = {u[l], u[N + l], u[2 * N + l]};
+        double eq[19];
+        acpo_feq(&k, p->order, rho[l], uu, eq);
+        for (int q = 0; q < 19; ++q) f[(size_t)q * N + l] = eq[q];
+    }
+}

@@ -94,6 +94,8 @@ const acpo_consts* acpo_case_consts(const acpo_case* c);
 
 /* Reference initial condition (Lattice.cpp:1397-1493): feq_N of the hard-coded pulse. */
 void acpo_reference_ic(const acpo_params* p, double domx, double domy, double* f);
+/* feq_N(rho[l], u[.][l]) for N nodes: rho [N], u [3][N] -> f [19][N] */
+void acpo_equilibrium_field(const acpo_params* p, int64_t N, const double* rho, const double* u, double* f);
 /* Parameterised pulse in node units: rho = rho0 (1 + amp exp(-ln2 ((i-x0)^2 + (j-y0)^2) / b^2)),
  * u = 0, populations feq_N (the initialization of Lattice.cpp:1457-1475 with a new profile). */
 void acpo_pulse_ic(const acpo_params* p, double amp, double x0, double y0, double b, double* f);

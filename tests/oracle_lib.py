@@ -84,6 +84,7 @@ def lib():
         L.acpo_lodi_count.restype = ctypes.c_int64
         L.acpo_lodi_schedule.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(ctypes.c_int32)]
         L.acpo_reference_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, dp]
+        L.acpo_equilibrium_field.argtypes = [P, ctypes.c_int64, dp, dp, dp]
         L.acpo_pulse_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         _lib = L
     return _lib
@@ -170,15 +171,11 @@ def feq(p, order, rho, u):
 
 def equilibrium_field(p, rho, u):
     """feq_N for every node; rho [N], u [3][N] -> [19][N]."""
-    k = consts(p)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    u = np.ascontiguousarray(u, dtype=np.float64)
     N = rho.size
     f = np.empty((19, N))
-    out = np.empty(19)
-    L = lib()
-    for n in range(N):
-        uu = np.ascontiguousarray(u[:, n])
-        L.acpo_feq(ctypes.byref(k), p.order, float(rho[n]), dptr(uu), dptr(out))
-        f[:, n] = out
+    lib().acpo_equilibrium_field(ctypes.byref(p), N, dptr(rho), dptr(u), dptr(f))
     return f
 
 

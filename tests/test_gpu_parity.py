@@ -1,0 +1,240 @@
+"""HIP path (libacpulse.so via the C ABI) against the reference fixtures and the oracle.
+
+Float64 tier: bit-identical results are required (np.array_equal), the
+reference and the oracle agree bit for bit and the kernels keep the reference's
+floating-point association with FMA contraction disabled.
+Shifted tiers: tolerances stated per test, against the float64 oracle.
+"""
+import glob
+import hashlib
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import acpulse_vclbm_amd as acp
+import oracle_lib as ol
+from make_golden import CS, DX, RHO0, params_of
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+pytestmark = pytest.mark.gpu
+
+
+def desc_of(spec, **over):
+    kw = dict(op=spec["op"], order=spec["order"], cubic=spec.get("cubic", False),
+              transverse=spec.get("transverse", False), rho0=RHO0, cs=CS, spacing=spec.get("spacing", DX),
+              mach=spec.get("mach", 0.0), k1=spec.get("k1", 1.0), sigma=spec.get("sigma", 0.98))
+    if spec.get("nu") is not None:
+        kw["nu"] = spec["nu"]
+    else:
+        kw["tau"] = spec.get("tau", 0.6)
+    kw.update(over)
+    return acp.make_desc(spec["nx"], spec["ny"], spec["nz"], **kw)
+
+
+STEP_CASES = sorted(glob.glob(os.path.join(GOLD, "step_*.npz")))
+
+
+@pytest.mark.parametrize("path", STEP_CASES, ids=[os.path.basename(p)[5:-4] for p in STEP_CASES])
+def test_reference_step_fixtures_bitexact(gpu, path):
+    z = np.load(path)
+    spec = json.loads(str(z["spec"]))
+    print("case", os.path.basename(path), flush=True)
+    with acp.Case(desc_of(spec)) as c:
+        c.set_populations(z["f0"])
+        c.set_probes([tuple(p) for p in z["probes"]], int(z["nsteps"]))
+        c.step(int(z["nsteps"]))
+        f = c.populations()
+        pr = c.read_probes()
+    assert np.array_equal(f, z["f"]), f"max |df| = {np.abs(f - z['f']).max():.3e}"
+    assert np.array_equal(pr, z["probe"])
+
+
+def _big(name):
+    return np.load(os.path.join(GOLD, f"big_{name}.npz"))
+
+
+def test_config1r_bgk_500_steps(gpu):
+    """200x200x3 BGK, tau = 0.5003, pulse 1e-3 at (100,100), LODI faces: probe series
+    and the full final state equal the reference's (SHA-256 of the population array)."""
+    z = _big("cfg1r_srt")
+    spec = json.loads(str(z["spec"]))
+    p = params_of(spec)
+    with acp.Case(desc_of(spec)) as c:
+        c.init_pulse(1e-3, 100.0, 100.0, 10.0)
+        c.set_probes([tuple(x) for x in z["probes"]], 500)
+        c.step(500)
+        f = c.populations()
+        pr = c.read_probes()
+    assert np.array_equal(pr, z["probe"])
+    assert hashlib.sha256(f.tobytes()).hexdigest() == str(z["sha256"])
+    rho = f.sum(axis=0).reshape(p.nz, p.ny, p.nx)
+    assert np.array_equal(rho[1], z["rho_plane"])
+
+
+def test_config1r_hrr_transverse_500_steps(gpu):
+    z = _big("cfg1r_hrr_tr")
+    spec = json.loads(str(z["spec"]))
+    with acp.Case(desc_of(spec)) as c:
+        c.init_pulse(1e-3, 100.0, 100.0, 10.0)
+        c.set_probes([tuple(x) for x in z["probes"]], 500)
+        c.step(500)
+        f = c.populations()
+        pr = c.read_probes()
+    assert np.array_equal(pr, z["probe"])
+    assert hashlib.sha256(f.tobytes()).hexdigest() == str(z["sha256"])
+
+
+def test_reference_initialization_pulse(gpu):
+    """Lattice::initialization's hard-coded pulse computed by the library, 60 BGK steps."""
+    z = _big("refpulse_srt")
+    spec = json.loads(str(z["spec"]))
+    with acp.Case(desc_of(spec)) as c:
+        c.init_reference()
+        c.set_probes([tuple(x) for x in z["probes"]], 60)
+        c.step(60)
+        f = c.populations()
+        pr = c.read_probes()
+    assert np.array_equal(pr, z["probe"])
+    assert hashlib.sha256(f.tobytes()).hexdigest() == str(z["sha256"])
+
+
+def test_config3r_mrtrm_48cubed(gpu):
+    z = _big("cfg3r_mrtrm")
+    spec = json.loads(str(z["spec"]))
+    p = params_of(spec)
+    N = 48 ** 3
+    rng = np.random.default_rng(0)
+    rho = RHO0 * (1.0 + 1e-3 * rng.standard_normal(N))
+    u = 1e-3 * CS * math.sqrt(3.0) * rng.standard_normal((3, N))
+    ic = ol.equilibrium_field(p, rho, u)
+    with acp.Case(desc_of(spec)) as c:
+        c.set_populations(ic)
+        c.set_probes([tuple(x) for x in z["probes"]], 50)
+        c.step(50)
+        f = c.populations()
+        pr = c.read_probes()
+    assert np.array_equal(pr, z["probe"])
+    assert hashlib.sha256(f.tobytes()).hexdigest() == str(z["sha256"])
+
+
+def _random_f(p, seed, amp_u=0.05):
+    rng = np.random.default_rng(seed)
+    N = p.nx * p.ny * p.nz
+    rho = p.rho0 * (1.0 + 0.01 * rng.uniform(-1, 1, N))
+    u = amp_u * p.cs * rng.uniform(-1, 1, (3, N))
+    f = ol.equilibrium_field(p, rho, u)
+    w = np.array([1 / 18] * 6 + [1 / 36] * 12 + [1 / 3])
+    return np.ascontiguousarray(f + p.rho0 * w[:, None] * 0.003 * rng.uniform(-1, 1, (19, N)))
+
+
+NEW_MODES = [
+    ("SRT", 3, "PERIODIC", "REFERENCE"), ("RR", 4, "PERIODIC", "REFERENCE"), ("HRR", 3, "PERIODIC", "REFERENCE"),
+    ("MRT-RM", 3, "PERIODIC", "RELAXED"), ("MRT-GS", 3, "PERIODIC", "RELAXED"),
+    ("MRT-RM", 3, "LODI", "RELAXED"), ("MRT-GS", 2, "LODI", "RELAXED"), ("HRR", 2, "PERIODIC", "REFERENCE"),
+]
+
+
+@pytest.mark.parametrize("op,order,bnd,mrt", NEW_MODES, ids=[f"{a}-o{b}-{c}-{d}" for a, b, c, d in NEW_MODES])
+def test_new_modes_match_oracle_bitexact(gpu, op, order, bnd, mrt):
+    """Periodic domains and relaxed MRT have no reference output; the oracle
+    restates them with the same arithmetic, so the GPU must match it exactly."""
+    nx, ny, nz = 16, 12, 10
+    periodic = bnd == "PERIODIC"
+    relaxed = mrt == "RELAXED"
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=periodic, mrt_mode=int(relaxed), tau=0.56,
+                       transverse=True)
+    f0 = _random_f(p, 11)
+    with acp.Case(acp.make_desc(nx, ny, nz, op=op, order=order, boundary=bnd, mrt_mode=mrt, tau=0.56,
+                                transverse=True)) as c:
+        c.set_populations(f0)
+        c.step(20)
+        f = c.populations()
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(20)
+    assert np.array_equal(f, o.state()), f"max |df| = {np.abs(f - o.state()).max():.3e}"
+
+
+def test_periodic_mass_and_translation(gpu):
+    """Periodic BGK: exact shift equivariance, mass conserved to round-off."""
+    nx, ny, nz = 16, 12, 10
+    p = ol.make_params(nx, ny, nz, op="SRT", periodic=True, tau=0.7)
+    f0 = _random_f(p, 3).reshape(19, nz, ny, nx)
+    shifted = np.roll(f0, (2, -3, 5), axis=(1, 2, 3))
+    d = acp.make_desc(nx, ny, nz, op="SRT", boundary="PERIODIC", tau=0.7)
+    with acp.Case(d) as a, acp.Case(d) as b:
+        a.set_populations(f0.reshape(19, -1))
+        b.set_populations(shifted.reshape(19, -1))
+        m0 = a.mass()
+        a.step(30)
+        b.step(30)
+        fa = a.populations().reshape(19, nz, ny, nx)
+        fb = b.populations().reshape(19, nz, ny, nx)
+        m1 = a.mass()
+    assert np.array_equal(np.roll(fa, (2, -3, 5), axis=(1, 2, 3)), fb)
+    assert abs(m1 - m0) / m0 < 1e-13
+
+
+def test_init_random_statistics(gpu):
+    """Philox/Box-Muller field: density and velocity fluctuations have the requested
+    standard deviation and the populations are the equilibrium of (rho, u)."""
+    d = acp.make_desc(64, 64, 64, op="MRT-RM", boundary="PERIODIC", mrt_mode="RELAXED", tau=0.503)
+    with acp.Case(d) as c:
+        c.init_random(0, 1e-3, 1e-3)
+        rho, u = c.macros()
+    cs_lu = CS * math.sqrt(3.0)
+    drho = rho / RHO0 - 1.0
+    assert abs(drho.std() - 1e-3) < 2e-5 and abs(drho.mean()) < 2e-5
+    for a in range(3):
+        ua = u[a] / cs_lu
+        assert abs(ua.std() - 1e-3) < 2e-5 and abs(ua.mean()) < 2e-5
+
+
+@pytest.mark.parametrize("prec,tol", [("F32-SHIFTED", 2e-4), ("F16-SHIFTED", 2e-2)])
+@pytest.mark.parametrize("op", ["SRT", "HRR", "MRT-RM"])
+def test_shifted_tiers_against_f64(gpu, prec, tol, op):
+    """Quasi-2D pulse (amplitude 1e-3) with LODI faces, 300 steps: the density
+    perturbation of the shifted tier against the float64 oracle, relative to its
+    maximum: max|drho_tier - drho_64| / max|drho_64| <= tol (2e-4 for float32,
+    2e-2 for float16 at scale 1e3)."""
+    nx, ny, nz, steps = 96, 80, 3, 300
+    mrt = "RELAXED" if op == "MRT-RM" else "REFERENCE"
+    p = ol.make_params(nx, ny, nz, op=op, order=3, mrt_mode=int(op == "MRT-RM"), tau=0.52, transverse=True,
+                       sigma=0.98)
+    f0 = ol.pulse_ic(p, 1e-3, 48.0, 40.0, 6.0)
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(steps)
+    r64, _ = ol.macros(p, o.state())
+    d = acp.make_desc(nx, ny, nz, op=op, order=3, precision=prec, mrt_mode=mrt, tau=0.52, transverse=True,
+                      f16_scale=1000.0)
+    with acp.Case(d) as c:
+        c.set_populations(f0)
+        c.step(steps)
+        rho, _ = c.macros()
+    d64 = r64 / RHO0 - 1.0
+    dt = rho / RHO0 - 1.0
+    err = np.abs(dt - d64).max() / np.abs(d64).max()
+    print(f"{prec} {op}: relative density error {err:.3e}")
+    assert err <= tol
+
+
+def test_probe_and_mass_f64(gpu):
+    nx, ny, nz = 20, 18, 6
+    p = ol.make_params(nx, ny, nz, op="RR", order=3, tau=0.6)
+    f0 = _random_f(p, 5)
+    with acp.Case(acp.make_desc(nx, ny, nz, op="RR", tau=0.6)) as c:
+        c.set_populations(f0)
+        c.set_probes([(3, 4, 2), (10, 9, 3)], 10)
+        c.step(10)
+        pr = c.read_probes()
+        f = c.populations()
+        m = c.mass()
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    po = o.step(10, [(3, 4, 2), (10, 9, 3)])
+    assert np.array_equal(pr, po)
+    assert abs(m - f.sum()) <= 1e-12 * abs(m)
