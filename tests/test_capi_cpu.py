@@ -1,0 +1,123 @@
+"""Host-side checks of libacpulse.so that need no GPU: it loads, exports every
+function include/acp.h declares, reads case files with the reference's
+semantics (Control.cpp:8-273), and builds the LODI schedule the reference's
+pass order implies."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import acpulse_vclbm_amd as acp
+from acpulse_vclbm_amd import _capi
+import oracle_lib as ol
+from make_golden import bc_text
+
+
+def test_library_exports_every_header_symbol():
+    L = acp.lib()
+    names = _capi.header_symbols()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def _write(tmp_path, text, name="bc.txt"):
+    p = tmp_path / name
+    p.write_text(text)
+    return str(p)
+
+
+def test_case_file_semantics(tmp_path):
+    txt = bc_text(20, 12, 3, op="HRR", order=4, transverse=True, cubic=True, tau=0.55, mach=0.1, k1=0.5,
+                  sigma=0.9)
+    # commented-out alternatives before the real key, trailing comments and ';'
+    txt = "//Collision-Model SRT\n#Order-Of-Equilibrium 2\n" + txt
+    txt = txt.replace("Density 1.17621", "Density 1.17621;")
+    txt = txt.replace("SpeedOfSound 347.3", "SpeedOfSound 347.3; //!air")
+    txt = txt.replace("Save-as /tmp/acp_golden", "Save-as   /tmp/my results/case one")
+    d = acp.read_case_file(_write(tmp_path, txt))
+    assert (d.nx, d.ny, d.nz) == (20, 12, 3)
+    assert d.op == _capi.OP["HRR"] and d.order == 4 and d.cubic == 1 and d.transverse == 1
+    assert d.rho0 == 1.17621 and d.cs == 347.3 and abs(d.mach - 0.1) < 1e-15 and d.k1 == 0.5 and d.sigma == 0.9
+    assert d.save_as.decode() == "/tmp/my results/case one"
+    assert d.precision == _capi.PREC["F64"] and d.boundary == _capi.BOUNDARY["LODI"]
+    c = acp.Control(_write(tmp_path, txt, "b2.txt"))
+    assert c.getCollisionModel() == "HRR" and c.getNumberOfNodesX() == 20 and c.getTransverseInclusion() == "yes"
+
+
+def test_first_match_wins(tmp_path):
+    txt = bc_text(10, 10, 3) + "Collision-Model RR\nMax-Time-Step 7\n"
+    d = acp.read_case_file(_write(tmp_path, txt))
+    assert d.op == _capi.OP["SRT"] and d.timesteps == 10
+
+
+def test_node_count_rounding(tmp_path):
+    """numberOfNodes = ceil(L / dx + 1e-8) (Control.cpp:43-45)."""
+    txt = bc_text(10, 10, 3).replace("Domain-Size-X 0.095", "Domain-Size-X 0.1")
+    d = acp.read_case_file(_write(tmp_path, txt))
+    assert d.nx == 11
+
+
+def test_engine_keys(tmp_path):
+    txt = bc_text(10, 10, 3) + "Precision F32-SHIFTED\nBoundary PERIODIC\nMRT-Mode RELAXED\nFP16-Scale 500\n"
+    d = acp.read_case_file(_write(tmp_path, txt))
+    assert d.precision == 1 and d.boundary == 1 and d.mrt_mode == 1 and d.f16_scale == 500.0
+
+
+@pytest.mark.parametrize("bad,code", [
+    ("Collision-Model SRT", _capi.ACP_E_CONFIG),          # key removed -> missing
+    ("Order-Of-Equilibrium 3", _capi.ACP_E_CONFIG),
+])
+def test_missing_keys_are_errors(tmp_path, bad, code):
+    txt = bc_text(10, 10, 3).replace(bad + "\n", "")
+    with pytest.raises(acp.AcpError) as e:
+        acp.read_case_file(_write(tmp_path, txt))
+    assert e.value.code == code
+
+
+def test_bad_values_are_errors(tmp_path):
+    with pytest.raises(acp.AcpError):
+        acp.read_case_file(_write(tmp_path, bc_text(10, 10, 3).replace("Order-Of-Equilibrium 3", "Order-Of-Equilibrium 5")))
+    with pytest.raises(acp.AcpError):
+        acp.read_case_file(_write(tmp_path, bc_text(10, 10, 3).replace("Collision-Model SRT", "Collision-Model XYZ")))
+    with pytest.raises(acp.AcpError):
+        acp.read_case_file(str(tmp_path / "does-not-exist.txt"))
+
+
+@pytest.mark.parametrize("dims", [(9, 7, 3), (8, 3, 6), (7, 6, 5), (4, 5, 4), (3, 3, 3), (33, 17, 3)])
+def test_lodi_schedule_matches_oracle(dims):
+    """Same boundary nodes, same phases, same order within each phase as the
+    oracle's schedule (which reproduces the reference bit for bit)."""
+    nodes, ph = acp.lodi_schedule(*dims)
+    c = ol.OracleCase(ol.make_params(*dims))
+    onode, oph = c.lodi_schedule()
+    order = np.argsort(oph, kind="stable")
+    assert np.array_equal(nodes, onode[order])
+    assert np.array_equal(ph, oph[order])
+    nx, ny, nz = dims
+    assert len(nodes) == nx * ny * nz - (nx - 2) * (ny - 2) * (nz - 2)
+
+
+def test_create_rejects_bad_descriptions():
+    for kw, code in [(dict(nx=2), _capi.ACP_E_ARG), (dict(order=5), _capi.ACP_E_CONFIG)]:
+        d = acp.make_desc(kw.get("nx", 8), 8, 8, tau=0.6)
+        if "order" in kw:
+            d.order = kw["order"]
+        h = ctypes.c_void_p()
+        rc = acp.lib().acp_case_create(ctypes.byref(d), None, ctypes.byref(h))
+        assert rc == code and not h.value
+    d = acp.make_desc(8, 8, 8, tau=0.6, precision="F32-SHIFTED", cubic=True)
+    h = ctypes.c_void_p()
+    assert acp.lib().acp_case_create(ctypes.byref(d), None, ctypes.byref(h)) == _capi.ACP_E_UNSUPPORTED
+    d = acp.make_desc(8, 8, 8, tau=0.6)
+    d.max_level = 1
+    assert acp.lib().acp_case_create(ctypes.byref(d), None, ctypes.byref(h)) == _capi.ACP_E_UNSUPPORTED
+
+
+def test_null_arguments_do_not_crash():
+    L = acp.lib()
+    assert L.acp_control_read(None, None) == _capi.ACP_E_ARG
+    assert L.acp_case_step(None, 1) == _capi.ACP_E_ARG
+    assert L.acp_case_info_get(None, None) == _capi.ACP_E_ARG
+    L.acp_case_destroy(None)
