@@ -82,6 +82,8 @@ def lib():
         "acp_case_mass": (ctypes.c_int, [vp, dp]),
         "acp_case_set_probes": (ctypes.c_int, [vp, ctypes.POINTER(i64), i32, i32]),
         "acp_case_read_probes": (ctypes.c_int, [vp, dp, ctypes.POINTER(i32)]),
+        "acp_case_write_vtk": (ctypes.c_int, [vp, ctypes.c_char_p]),
+        "acp_write_vtk_host": (ctypes.c_int, [D, dp, dp, dp, ctypes.c_char_p]),
         "acp_lodi_schedule": (ctypes.c_int64, [i32, i32, i32, ctypes.POINTER(i64), ctypes.POINTER(i32), i64]),
     }
     for name, (res, args) in sig.items():

@@ -697,3 +697,51 @@ extern "C" int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t
             }
     return n;
 }
+
+namespace {
+int download_array(acp_case* c, bool current, double* f) {
+    const int64_t N = c->g.N;
+    if (!shifted(c)) {
+        HIPCHK(hipMemcpyAsync(f, current ? c->A : c->B, sizeof(double) * 19 * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        return ACP_OK;
+    }
+    const int64_t chunk = std::min<int64_t>(N, (int64_t)1 << 24);
+    double* stage = nullptr;
+    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * chunk));
+    const size_t eb = elem_bytes(c->precision);
+    const char* src = (const char*)(current ? c->SA : c->SB);
+    int rc = ACP_OK;
+    for (int q = 0; q < 19 && rc == ACP_OK; ++q)
+        for (int64_t l0 = 0; l0 < N && rc == ACP_OK; l0 += chunk) {
+            const int64_t n = std::min(chunk, N - l0);
+            hipError_t e = acp::launch_from_store(c->precision, c->dks, src + ((size_t)q * N + l0) * eb, stage, n, q,
+                                                  c->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(f + (size_t)q * N + l0, stage, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = acp::hip_fail(e, "population download");
+        }
+    (void)hipFree(stage);
+    return rc;
+}
+}  // namespace
+
+extern "C" int acp_case_write_vtk(acp_case* c, const char* stem) {
+    if (int rc = check_case(c)) return rc;
+    if (!stem) return fail(ACP_E_ARG, "null file name");
+    const int64_t N = c->g.N;
+    std::vector<double> cur((size_t)19 * N), prev;
+    if (int rc = download_array(c, true, cur.data())) return rc;
+    if (c->d.nonreflecting) {
+        prev.resize((size_t)19 * N);
+        if (int rc = download_array(c, false, prev.data())) return rc;
+    }
+    std::vector<double> psi;
+    if (c->psi) {
+        psi.resize((size_t)19 * N);
+        HIPCHK(hipMemcpy(psi.data(), c->psi, sizeof(double) * 19 * N, hipMemcpyDeviceToHost));
+    }
+    return acp::write_vtk_host(c->d, c->hk, cur.data(), prev.empty() ? nullptr : prev.data(),
+                               psi.empty() ? nullptr : psi.data(), stem);
+}

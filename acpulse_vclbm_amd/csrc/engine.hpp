@@ -32,6 +32,11 @@ struct LodiSchedule {
 };
 LodiSchedule build_lodi_schedule(int nx, int ny, int nz);
 
+// VTK output of one level from host float64 arrays (current array, the other
+// array for the Non-Reflecting boundary quirk, cubic-Mach psi or null).
+int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, const double* fprev, const double* psi,
+                   const char* stem);
+
 }  // namespace acp
 
 struct acp_case {

@@ -140,6 +140,16 @@ int acp_case_mass(acp_case* c, double* mass);
 int acp_case_set_probes(acp_case* c, const int64_t* nodes, int32_t nprobe, int32_t capacity);
 int acp_case_read_probes(acp_case* c, double* out, int32_t* nsteps);
 
+/* Lattice::writeResultsVTK(string, bool&, bool&) (Lattice.cpp:2212-2243, 4094-4349)
+ * for one level: writes <stem>.vtk with the reference's byte layout from the
+ * current state (and, for Non-Reflecting yes, the previous array of boundary
+ * nodes with i > 0, as the reference prints them). */
+int acp_case_write_vtk(acp_case* c, const char* stem);
+/* The same writer on host arrays (float64 [19][N] current and previous arrays,
+ * cubic-Mach psi or NULL), e.g. for states computed elsewhere. */
+int acp_write_vtk_host(const acp_case_desc* desc, const double* fcur, const double* fprev, const double* psi,
+                       const char* stem);
+
 /* Introspection, no counterpart in the reference: the boundary nodes of an
  * nx*ny*nz LODI case in the order the engine processes them (phase by phase;
  * within a phase in the reference's pass order, Lattice.cpp:1582-1624) and the

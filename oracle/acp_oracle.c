@@ -728,6 +728,7 @@ void acpo_set_state(acpo_case* c, const double* f) {
     memset(c->psi, 0, sizeof(double) * 19 * (size_t)c->N);
 }
 void acpo_get_state(const acpo_case* c, double* f) { memcpy(f, c->A, sizeof(double) * 19 * (size_t)c->N); }
+void acpo_get_prev(const acpo_case* c, double* f) { memcpy(f, c->B, sizeof(double) * 19 * (size_t)c->N); }
 void acpo_get_psi(const acpo_case* c, double* psi) { memcpy(psi, c->psi, sizeof(double) * 19 * (size_t)c->N); }
 const acpo_consts* acpo_case_consts(const acpo_case* c) { return &c->k; }
 int64_t acpo_lodi_count(const acpo_case* c) { return c->nl; }

@@ -83,6 +83,7 @@ void acpo_destroy(acpo_case* c);
 void acpo_set_state(acpo_case* c, const double* f);   /* [19][N], both arrays (Lattice.cpp:1457-1475) */
 void acpo_get_state(const acpo_case* c, double* f);   /* current array [19][N] */
 void acpo_get_psi(const acpo_case* c, double* psi);   /* [19][N] */
+void acpo_get_prev(const acpo_case* c, double* f);    /* the other array [19][N] */
 /* Runs n steps; after each step writes sum_q f_q of the current array at each probe node. */
 void acpo_step(acpo_case* c, int n, const int64_t* probe_idx, int nprobe, double* probe_out);
 /* Connectivity rows in the reference driver's 27-column layout. */

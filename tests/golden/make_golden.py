@@ -42,7 +42,7 @@ def nu_for_tau(tau, cs=CS, dx=DX):
 
 
 def bc_text(nx, ny, nz, op="SRT", order=3, cubic=False, transverse=False, tau=0.6, mach=0.0, k1=1.0,
-            sigma=0.98, threads=8, spacing=DX, nu=None, steps=10):
+            sigma=0.98, threads=8, spacing=DX, nu=None, steps=10, nonreflecting=True):
     if nu is None:
         nu = nu_for_tau(tau, dx=spacing)
     yn = lambda b: "yes" if b else "no"  # noqa: E731
@@ -50,7 +50,7 @@ def bc_text(nx, ny, nz, op="SRT", order=3, cubic=False, transverse=False, tau=0.
         f"Max-Time-Step {steps}", "WriteInterval 1000000", "Save-as /tmp/acp_golden", f"Threads {threads}",
         f"Density {RHO0!r}", f"KinematicViscosity {nu!r}", f"SpeedOfSound {CS!r}", f"MACH-Number {mach!r}",
         f"Order-Of-Equilibrium {order}", f"Collision-Model {op}", f"Hybrid-Parameter {sigma!r}",
-        f"Cubic-Mach-Correction {yn(cubic)}", "Coupling-Approach STD", "Non-Reflecting yes",
+        f"Cubic-Mach-Correction {yn(cubic)}", "Coupling-Approach STD", f"Non-Reflecting {yn(nonreflecting)}",
         f"Transverse {yn(transverse)}", f"K1-Relaxation {k1!r}", "K2-Relaxation 1.0", "F2C-Filter NOFI",
         "Coarse-to-Fine fAll", "Fine-to-Coarse cAll", "Hanging-Node-Reconstruction hAll", "Scaling-Style SRTSTYLE",
         f"Domain-Size-X {(nx - 0.5) * spacing!r}", f"Domain-Size-Y {(ny - 0.5) * spacing!r}",
@@ -139,15 +139,41 @@ def make_big_case(name, spec, nsteps, ic, tmp, outdir, probes, plane_k=1, spacin
                         ic_kind="reference" if ic is None else "injected")
 
 
+def make_vtk_case(name, spec, nsteps, seed, tmp, outdir, nonreflecting):
+    """The reference's own writeResultsVTK output after nsteps (Lattice.cpp:4094-4349)."""
+    d = os.path.join(tmp, name)
+    os.makedirs(d, exist_ok=True)
+    nx, ny, nz = spec["nx"], spec["ny"], spec["nz"]
+    with open(os.path.join(d, "bc.txt"), "w") as fh:
+        fh.write(bc_text(nx, ny, nz, op=spec["op"], order=spec["order"], cubic=spec.get("cubic", False),
+                         transverse=spec.get("transverse", False), tau=spec.get("tau", 0.6),
+                         nonreflecting=nonreflecting))
+    f0 = random_ic(spec, seed)
+    f0.tofile(os.path.join(d, "ic.bin"))
+    run_driver(["vtk", "bc.txt", str(nsteps), "out", "--ic", "ic.bin"], d)
+    txt = open(os.path.join(d, "out.vtk"), "rb").read()
+    np.savez_compressed(os.path.join(outdir, f"vtk_{name}.npz"), spec=spec_json(spec), nsteps=nsteps, f0=f0,
+                        nonreflecting=int(nonreflecting), vtk=np.frombuffer(txt, dtype=np.uint8))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=HERE)
     ap.add_argument("--skip-big", action="store_true")
+    ap.add_argument("--only-vtk", action="store_true")
     a = ap.parse_args()
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: make -C oracle ref")
     out = a.out
     with tempfile.TemporaryDirectory() as tmp:
+        make_vtk_case("srt_q2d_nonrefl", dict(nx=9, ny=7, nz=3, op="SRT", order=3), 7, 101, tmp, out, True)
+        make_vtk_case("hrr_3d_refl", dict(nx=7, ny=6, nz=5, op="HRR", order=3, transverse=True), 5, 102, tmp, out,
+                      False)
+        make_vtk_case("rr_cubic_3d_nonrefl", dict(nx=7, ny=6, nz=5, op="RR", order=3, cubic=True), 6, 103, tmp, out,
+                      True)
+        print("vtk cases", flush=True)
+        if a.only_vtk:
+            return
         # connectivity
         for (nx, ny, nz) in [(7, 6, 5), (9, 7, 3), (8, 3, 6), (4, 5, 4), (5, 4, 3), (6, 6, 6)]:
             d = os.path.join(tmp, f"g{nx}{ny}{nz}")

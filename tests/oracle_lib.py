@@ -78,6 +78,7 @@ def lib():
         L.acpo_set_state.argtypes = [ctypes.c_void_p, dp]
         L.acpo_get_state.argtypes = [ctypes.c_void_p, dp]
         L.acpo_get_psi.argtypes = [ctypes.c_void_p, dp]
+        L.acpo_get_prev.argtypes = [ctypes.c_void_p, dp]
         L.acpo_step.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_int, dp]
         L.acpo_geometry.argtypes = [ctypes.c_void_p, ip]
         L.acpo_lodi_count.argtypes = [ctypes.c_void_p]
@@ -134,6 +135,11 @@ class OracleCase:
     def state(self):
         f = np.empty((19, self.N))
         lib().acpo_get_state(self.h, dptr(f))
+        return f
+
+    def prev(self):
+        f = np.empty((19, self.N))
+        lib().acpo_get_prev(self.h, dptr(f))
         return f
 
     def psi(self):

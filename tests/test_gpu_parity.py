@@ -238,3 +238,50 @@ def test_probe_and_mass_f64(gpu):
     po = o.step(10, [(3, 4, 2), (10, 9, 3)])
     assert np.array_equal(pr, po)
     assert abs(m - f.sum()) <= 1e-12 * abs(m)
+
+
+VTK_CASES = sorted(glob.glob(os.path.join(GOLD, "vtk_*.npz")))
+
+
+@pytest.mark.parametrize("path", VTK_CASES, ids=[os.path.basename(p)[4:-4] for p in VTK_CASES])
+def test_case_vtk_output_matches_reference_bytes(gpu, tmp_path, path):
+    """acp_case_write_vtk after GPU steps equals the reference's writeResultsVTK file."""
+    z = np.load(path)
+    spec = json.loads(str(z["spec"]))
+    d = desc_of(spec)
+    d.nonreflecting = int(z["nonreflecting"])
+    stem = str(tmp_path / "gpu")
+    with acp.Case(d) as c:
+        c.set_populations(z["f0"])
+        c.step(int(z["nsteps"]))
+        acp._capi.check(acp.lib().acp_case_write_vtk(c.h, stem.encode()))
+    assert open(stem + ".vtk", "rb").read() == z["vtk"].tobytes()
+
+
+def test_acp_run_driver_end_to_end(gpu, tmp_path):
+    """The command-line driver: the reference's main() loop on a case file, writing
+    <Save-as>_<t>.vtk at t = 0 and every WriteInterval; the last file equals the
+    oracle state written by the same writer."""
+    import subprocess
+    from make_golden import bc_text
+    nx, ny, nz, steps = 40, 30, 3, 6
+    txt = bc_text(nx, ny, nz, op="SRT", order=3, tau=0.55, spacing=0.02, steps=steps)
+    txt = txt.replace("WriteInterval 1000000", "WriteInterval 3").replace("Save-as /tmp/acp_golden",
+                                                                            f"Save-as {tmp_path}/run")
+    (tmp_path / "bc.txt").write_text(txt)
+    exe = os.path.join(os.path.dirname(acp._capi.LIB_PATH), "acp_run")
+    r = subprocess.run([exe, str(tmp_path / "bc.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "LB acoustic pulse solver" in r.stdout and "given maximum number of time steps reached" in r.stdout
+    assert "solved 100.00% of the case" in r.stdout
+    for t in (0, 3, 6):
+        assert os.path.exists(tmp_path / f"run_{t}.vtk")
+    p = ol.make_params(nx, ny, nz, op="SRT", order=3, tau=0.55, spacing=0.02)
+    o = ol.OracleCase(p)
+    o.set_state(ol.reference_ic(p, (nx - 0.5) * 0.02, (ny - 0.5) * 0.02))
+    o.step(steps)
+    cur, prev = o.state(), o.prev()
+    d = acp.read_case_file(str(tmp_path / "bc.txt"))
+    acp._capi.check(acp.lib().acp_write_vtk_host(d, acp._capi.dptr(cur), acp._capi.dptr(prev), None,
+                                                 str(tmp_path / "oracle").encode()))
+    assert open(tmp_path / "run_6.vtk", "rb").read() == open(tmp_path / "oracle.vtk", "rb").read()
