@@ -150,6 +150,15 @@ def lodi_schedule(nx, ny, nz):
     return nodes, ph
 
 
+def copy_probe(planes, plane_elems, reps=5, stream=None):
+    """Mean milliseconds of one device copy of `planes` (1 or 19) float64
+    planes of `plane_elems` elements in the engine's layout (acp_copy_probe)."""
+    L = _capi.lib()
+    ms = ctypes.c_double()
+    _capi.check(L.acp_copy_probe(int(planes), int(plane_elems), int(reps), stream, ctypes.byref(ms)))
+    return ms.value
+
+
 class Control:
     """Control(string) of the reference (Control.h:17-75): case file reader and getters."""
 

@@ -14,24 +14,28 @@
 // previous contents, as in the reference.  In periodic mode every node is
 // interior and the neighbours wrap.
 #include "k64.hpp"
+#include "plane.hpp"
 
 namespace acp {
 
-template <int OP, int ORDER, bool CUBIC, bool PERIODIC>
-__global__ __launch_bounds__(512) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                              double* __restrict__ B, double* __restrict__ psi,
-                                              const double* __restrict__ M, int nx, int ny, int nz, int64_t N) {
+template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW>
+__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                                     double* __restrict__ B, double* __restrict__ psi,
+                                                     const double* __restrict__ M, int nx, int ny, int nz,
+                                                     int64_t N) {
+    using P = Plane<NARROW>;
+    using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nx) return;
     const int j = blockIdx.y, kz = blockIdx.z;
-    const int64_t sxy = (int64_t)nx * ny;
-    const int64_t l = i + (int64_t)nx * j + sxy * kz;
+    const I sxy = (I)nx * (I)ny;
+    const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
     const K64& k = *kp;
 
     double f[19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = A[q * N + l];
+        f[q] = P::at(A, q, l, N);
     });
 
     int bi = 0, bj = 0, bk = 0;
@@ -42,15 +46,16 @@ __global__ __launch_bounds__(512) void bulk64(const K64* __restrict__ kp, const 
     }
     const bool boundary = (bi | bj | bk) != 0;
 
-    // neighbour offsets (wrapped in periodic mode)
-    int64_t oxp = 1, oxm = -1, oyp = nx, oym = -(int64_t)nx, ozp = sxy, ozm = -sxy;
+    // neighbour offsets (wrapped in periodic mode); unsigned wrap-around in the
+    // narrow form lands on the right index because the final index is in range
+    I oxp = 1, oxm = (I)-1, oyp = (I)nx, oym = (I)0 - (I)nx, ozp = sxy, ozm = (I)0 - sxy;
     if constexpr (PERIODIC) {
-        if (i == nx - 1) oxp = 1 - nx;
-        if (i == 0) oxm = nx - 1;
-        if (j == ny - 1) oyp = -(int64_t)nx * (ny - 1);
-        if (j == 0) oym = (int64_t)nx * (ny - 1);
-        if (kz == nz - 1) ozp = -sxy * (nz - 1);
-        if (kz == 0) ozm = sxy * (nz - 1);
+        if (i == nx - 1) oxp = (I)0 - (I)(nx - 1);
+        if (i == 0) oxm = (I)(nx - 1);
+        if (j == ny - 1) oyp = (I)0 - (I)nx * (I)(ny - 1);
+        if (j == 0) oym = (I)nx * (I)(ny - 1);
+        if (kz == nz - 1) ozp = (I)0 - sxy * (I)(nz - 1);
+        if (kz == 0) ozm = sxy * (I)(nz - 1);
     }
 
     if (!boundary) {
@@ -61,23 +66,23 @@ __global__ __launch_bounds__(512) void bulk64(const K64* __restrict__ kp, const 
             if constexpr (CUBIC) {
                 static_for<0, 19>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    ps[q] = psi[q * N + l];
+                    ps[q] = P::at(psi, q, l, N);
                 });
             }
             double nb[6][4];
             if constexpr (OP == OP_HRR || CUBIC) {
-                const int64_t o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+                const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
 #pragma unroll
                 for (int m = 0; m < 6; ++m)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) nb[m][v] = M[v * N + l + o[m]];
+                    for (int v = 0; v < 4; ++v) nb[m][v] = P::at(M, v, (I)(l + o[m]), N);
             }
             collide_rr64<ORDER, OP == OP_HRR, CUBIC>(k, f, ps, nb);
             if constexpr (OP == OP_HRR || CUBIC) {
                 if (psi != nullptr) {
                     static_for<0, 19>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        psi[q * N + l] = ps[q];
+                        P::at(psi, q, l, N) = ps[q];
                     });
                 }
             }
@@ -93,25 +98,29 @@ __global__ __launch_bounds__(512) void bulk64(const K64* __restrict__ kp, const 
         if constexpr (!PERIODIC) {
             if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
         }
-        int64_t d = l;
+        I d = l;
         if constexpr (EX[q] > 0) d += oxp;
         if constexpr (EX[q] < 0) d += oxm;
         if constexpr (EY[q] > 0) d += oyp;
         if constexpr (EY[q] < 0) d += oym;
         if constexpr (EZ[q] > 0) d += ozp;
         if constexpr (EZ[q] < 0) d += ozm;
-        B[q * N + d] = f[q];
+        P::at(B, q, d, N) = f[q];
     });
-    B[18 * N + l] = f[18];
+    P::at(B, 18, l, N) = f[18];
 }
 
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC>
 static hipError_t go(const K64* k, const double* A, double* B, double* psi, const double* M, const Grid& g,
                      hipStream_t s) {
-    const int bx = g.nx >= 512 ? 512 : ((g.nx + 63) / 64) * 64;
+    const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
-    hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC>), grid, block, 0, s, k, A, B, psi, M, g.nx, g.ny, g.nz,
-                       g.N);
+    if (narrow_planes(g.N, sizeof(double)))
+        hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, true>), grid, block, 0, s, k, A, B, psi, M, g.nx,
+                           g.ny, g.nz, g.N);
+    else
+        hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, false>), grid, block, 0, s, k, A, B, psi, M, g.nx,
+                           g.ny, g.nz, g.N);
     return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 // Global reductions (mass, probes) and conversions to SI are done in float64.
 #include "kshift.hpp"
 #include "physf.hpp"
+#include "plane.hpp"
 
 namespace acp {
 
@@ -24,20 +25,22 @@ __device__ __forceinline__ void load_dev(const T* __restrict__ F, int64_t N, int
 }
 
 // ------------------------------------------------------------------ bulk collide + push
-template <typename T, int OP, int ORDER, bool PERIODIC>
-__global__ __launch_bounds__(512) void bulk_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
-                                                  T* __restrict__ B, const float* __restrict__ M, int nx, int ny,
-                                                  int nz, int64_t N) {
+template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW>
+__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
+    const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
+    int ny, int nz, int64_t N) {
+    using P = Plane<NARROW>;
+    using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nx) return;
     const int j = blockIdx.y, kz = blockIdx.z;
-    const int64_t sxy = (int64_t)nx * ny;
-    const int64_t l = i + (int64_t)nx * j + sxy * kz;
+    const I sxy = (I)nx * (I)ny;
+    const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
     const KShift& k = *kp;
     float g[19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        g[q] = (float)A[q * N + l];
+        g[q] = (float)P::at(A, q, l, N);
     });
     int bi = 0, bj = 0, bk = 0;
     if constexpr (!PERIODIC) {
@@ -46,14 +49,16 @@ __global__ __launch_bounds__(512) void bulk_shift(const KShift* __restrict__ kp,
         bk = kz == 0 ? 1 : (kz == nz - 1 ? -1 : 0);
     }
     const bool boundary = (bi | bj | bk) != 0;
-    int64_t oxp = 1, oxm = -1, oyp = nx, oym = -(int64_t)nx, ozp = sxy, ozm = -sxy;
+    // neighbour offsets; unsigned wrap-around in the narrow form is exact because
+    // every final index is in range
+    I oxp = 1, oxm = (I)-1, oyp = (I)nx, oym = (I)0 - (I)nx, ozp = sxy, ozm = (I)0 - sxy;
     if constexpr (PERIODIC) {
-        if (i == nx - 1) oxp = 1 - nx;
-        if (i == 0) oxm = nx - 1;
-        if (j == ny - 1) oyp = -(int64_t)nx * (ny - 1);
-        if (j == 0) oym = (int64_t)nx * (ny - 1);
-        if (kz == nz - 1) ozp = -sxy * (nz - 1);
-        if (kz == 0) ozm = sxy * (nz - 1);
+        if (i == nx - 1) oxp = (I)0 - (I)(nx - 1);
+        if (i == 0) oxm = (I)(nx - 1);
+        if (j == ny - 1) oyp = (I)0 - (I)nx * (I)(ny - 1);
+        if (j == 0) oym = (I)nx * (I)(ny - 1);
+        if (kz == nz - 1) ozp = (I)0 - sxy * (I)(nz - 1);
+        if (kz == 0) ozm = sxy * (I)(nz - 1);
     }
     if constexpr (OP != OP_STREAM) {
         if (!boundary) {
@@ -82,12 +87,12 @@ __global__ __launch_bounds__(512) void bulk_shift(const KShift* __restrict__ kp,
                 pi1_dev(fneq, a2);
                 if constexpr (OP == OP_HRR) {
                     // FD strain of the neighbours' pre-collision velocity (Node.cpp:1750-1889), dx = 1
-                    const int64_t o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+                    const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
                     float v[6][3];
 #pragma unroll
                     for (int m = 0; m < 6; ++m)
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) v[m][a] = M[(a + 1) * N + l + o[m]];
+                        for (int a = 0; a < 3; ++a) v[m][a] = P::at(M, a + 1, (I)(l + o[m]), N);
                     const float S[6] = {0.5f * (v[0][0] - v[2][0]), 0.5f * (v[1][1] - v[3][1]),
                                         0.5f * (v[4][2] - v[5][2]),
                                         0.5f * (0.5f * (v[0][1] - v[2][1]) + 0.5f * (v[1][0] - v[3][0])),
@@ -192,29 +197,36 @@ __global__ __launch_bounds__(512) void bulk_shift(const KShift* __restrict__ kp,
         if constexpr (!PERIODIC) {
             if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
         }
-        int64_t dd = l;
+        I dd = l;
         if constexpr (EX[q] > 0) dd += oxp;
         if constexpr (EX[q] < 0) dd += oxm;
         if constexpr (EY[q] > 0) dd += oyp;
         if constexpr (EY[q] < 0) dd += oym;
         if constexpr (EZ[q] > 0) dd += ozp;
         if constexpr (EZ[q] < 0) dd += ozm;
-        B[q * N + dd] = (T)g[q];
+        P::at(B, q, dd, N) = (T)g[q];
     });
-    B[18 * N + l] = (T)g[18];
+    P::at(B, 18, l, N) = (T)g[18];
 }
 
 template <typename T, int OP, int ORDER>
 static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B, const float* M, const Grid& g,
                           hipStream_t s) {
-    const int bx = g.nx >= 512 ? 512 : ((g.nx + 63) / 64) * 64;
+    const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
-    if (periodic)
-        hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, true>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny,
-                           g.nz, g.N);
-    else
-        hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, false>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx,
-                           g.ny, g.nz, g.N);
+    // the macro array M is float32 like a float32 population plane
+    const bool narrow = narrow_planes(g.N, sizeof(float) > sizeof(T) ? sizeof(float) : sizeof(T));
+#define BS(PER, NAR)                                                                                             \
+    hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
+                       g.ny, g.nz, g.N)
+    if (periodic) {
+        if (narrow) BS(true, true);
+        else BS(true, false);
+    } else {
+        if (narrow) BS(false, true);
+        else BS(false, false);
+    }
+#undef BS
     return hipGetLastError();
 }
 

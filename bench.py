@@ -63,6 +63,18 @@ def run_tier(acp, torch, precision, n, steps, warmup, seed=0):
     }
 
 
+def copy_ceiling(acp, torch, n, reps=5):
+    """Device-copy bandwidth for the same bytes as one float64 step: 19 planes
+    of n^3 float64 copied one element of every plane per thread (the bulk
+    kernel's access pattern), and one plane of 19 n^3 elements."""
+    stream = torch.cuda.Stream()
+    nodes = n ** 3
+    nbytes = 2 * 19 * 8 * nodes
+    ms19 = acp.copy_probe(19, nodes, reps, stream.cuda_stream)
+    ms1 = acp.copy_probe(1, 19 * nodes, reps, stream.cuda_stream)
+    return {"planes19_gbs": nbytes / (ms19 * 1e-3) / 1e9, "planes1_gbs": nbytes / (ms1 * 1e-3) / 1e9}
+
+
 def traffic_from_profiles(name):
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
     read from the committed summary (profiles/<name>), or None."""
@@ -109,6 +121,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tiers", default="F64,F32-SHIFTED")
+    ap.add_argument("--no-copy-ceiling", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run three 4 GiB single-plane device copies (counter calibration) and exit")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this engine has no multi-GPU path in this build "
@@ -122,6 +137,12 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    if a.calibrate_copy:
+        elems = 512 * 1024 * 1024
+        ms = acp.copy_probe(1, elems, 2, torch.cuda.current_stream().cuda_stream)
+        print(json.dumps({"calibration_copy": {"bytes_read": elems * 8, "bytes_written": elems * 8,
+                                               "launches": 3, "ms_per_copy": round(ms, 4)}}))
+        return
     tiers = a.tiers.split(",")
     res = {}
     for t in tiers:
@@ -130,6 +151,10 @@ def main():
               f"{100 * res[t]['frac']:.1f}% of HBM peak", file=sys.stderr, flush=True)
     main_t = "F64" if "F64" in res else tiers[0]
     r = res[main_t]
+    ceil = None if a.no_copy_ceiling else copy_ceiling(acp, torch, a.n)
+    if ceil is not None:
+        print(f"# copy ceiling: 19 planes {ceil['planes19_gbs']:.0f} GB/s, 1 plane {ceil['planes1_gbs']:.0f} GB/s",
+              file=sys.stderr, flush=True)
     cpu = None if a.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "GLUPS & %HBM roofline, FP64 vs FP32-shifted; p50/p99 case latency",
@@ -151,6 +176,10 @@ def main():
                      "frac": round(r["frac"], 4),
                      "traffic": traffic_from_profiles(f"bulk64_mrtrm_periodic_{a.n}.txt")},
         "tiers": {t: {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res[t].items()} for t in res},
+        "copy_ceiling": None if ceil is None else {
+            "planes19_gbs": round(ceil["planes19_gbs"], 1), "planes1_gbs": round(ceil["planes1_gbs"], 1),
+            "frac_of_planes19": round(r["achieved_gbs"] / ceil["planes19_gbs"], 4),
+            "frac_of_planes1": round(r["achieved_gbs"] / ceil["planes1_gbs"], 4)},
         "cpu_baseline": cpu,
     }
     if "F32-SHIFTED" in res and "F64" in res:

@@ -156,6 +156,14 @@ int acp_write_vtk_host(const acp_case_desc* desc, const double* fcur, const doub
  * phase of each.  Returns the node count; writes at most `cap` entries. */
 int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t* nodes, int32_t* phases, int64_t cap);
 
+/* Measurement helper, no counterpart in the reference: the device-copy
+ * ceiling for the engine's memory layout.  Copies `planes` (1 or 19) float64
+ * planes of `plane_elems` elements each, one element of every plane per thread
+ * (the access pattern of the bulk kernel without the neighbour shift), `reps`
+ * times on `stream` after one untimed copy; writes the mean milliseconds per
+ * copy.  Allocates and frees its own two buffers. */
+int acp_copy_probe(int32_t planes, int64_t plane_elems, int32_t reps, void* stream, double* ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif
