@@ -138,6 +138,10 @@ class Case:
         check(lib().acp_case_read_probes(self.h, dptr(out), ctypes.byref(n)))
         return out[: n.value, : self._nprobe].copy()
 
+    def write_vtk(self, stem):
+        """Lattice::writeResultsVTK (Lattice.cpp:2212-2243, 4094-4349): <stem>.vtk."""
+        check(lib().acp_case_write_vtk(self.h, str(stem).encode()))
+
 
 def lodi_schedule(nx, ny, nz):
     """(nodes, phases) of the LODI pass in processing order."""
