@@ -138,6 +138,16 @@ class Case:
         check(lib().acp_case_read_probes(self.h, dptr(out), ctypes.byref(n)))
         return out[: n.value, : self._nprobe].copy()
 
+    def enable_graphs(self, chunk):
+        """Run every full chunk of `chunk` (even) steps as one captured-graph replay; 0 switches off."""
+        check(lib().acp_case_enable_graphs(self.h, int(chunk)))
+
+    def checksum(self):
+        """Order-independent 64-bit fingerprint of the current population array."""
+        v = ctypes.c_uint64()
+        check(lib().acp_case_checksum(self.h, ctypes.byref(v)))
+        return v.value
+
     def write_vtk(self, stem):
         """Lattice::writeResultsVTK (Lattice.cpp:2212-2243, 4094-4349): <stem>.vtk."""
         check(lib().acp_case_write_vtk(self.h, str(stem).encode()))
@@ -152,6 +162,13 @@ def lodi_schedule(nx, ny, nz):
     L.acp_lodi_schedule(nx, ny, nz, nodes.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                         ph.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), n)
     return nodes, ph
+
+
+def footprint(desc, nprobe=0, probe_capacity=0):
+    """Device bytes a case of `desc` holds with `nprobe` probes of `probe_capacity` steps (acp_case_footprint)."""
+    b = ctypes.c_int64()
+    check(lib().acp_case_footprint(ctypes.byref(desc), int(nprobe), int(probe_capacity), ctypes.byref(b)))
+    return b.value
 
 
 def copy_probe(planes, plane_elems, reps=5, stream=None):

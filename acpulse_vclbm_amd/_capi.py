@@ -44,6 +44,42 @@ class CaseInfo(ctypes.Structure):
     ]
 
 
+class ServerDesc(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_active", ctypes.c_int32),
+                ("graph_chunk", ctypes.c_int32), ("launch_steps", ctypes.c_int32), ("chunks_ahead", ctypes.c_int32),
+                ("admit_threads", ctypes.c_int32)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [
+        ("desc", CaseDesc), ("steps", ctypes.c_int32), ("init_kind", ctypes.c_int32),
+        ("amp", ctypes.c_double), ("x0", ctypes.c_double), ("y0", ctypes.c_double), ("halfwidth", ctypes.c_double),
+        ("seed", ctypes.c_uint64), ("rho_amp", ctypes.c_double), ("u_amp", ctypes.c_double),
+        ("probe", ctypes.c_int32 * 3), ("probe_out", ctypes.POINTER(ctypes.c_double)),
+        ("arrival", ctypes.c_double), ("tag", ctypes.c_uint64),
+    ]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [
+        ("tag", ctypes.c_uint64), ("status", ctypes.c_int32), ("steps", ctypes.c_int32),
+        ("device_bytes", ctypes.c_int64), ("t_arrival", ctypes.c_double), ("t_submit", ctypes.c_double),
+        ("t_admit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double),
+        ("mass", ctypes.c_double), ("checksum", ctypes.c_uint64), ("error", ctypes.c_char * 128),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_int64), ("completed", ctypes.c_int64), ("queued", ctypes.c_int64),
+        ("active", ctypes.c_int32), ("peak_active", ctypes.c_int32), ("bytes_in_use", ctypes.c_int64),
+        ("peak_bytes", ctypes.c_int64), ("budget_bytes", ctypes.c_int64), ("clock", ctypes.c_double),
+    ]
+
+
+INIT = {"REFERENCE": 0, "PULSE": 1, "RANDOM": 2}
+
+
 class AcpError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"acp error {code}: {msg}")
@@ -86,6 +122,16 @@ def lib():
         "acp_write_vtk_host": (ctypes.c_int, [D, dp, dp, dp, ctypes.c_char_p]),
         "acp_lodi_schedule": (ctypes.c_int64, [i32, i32, i32, ctypes.POINTER(i64), ctypes.POINTER(i32), i64]),
         "acp_copy_probe": (ctypes.c_int, [i32, i64, i32, vp, dp]),
+        "acp_case_enable_graphs": (ctypes.c_int, [vp, i32]),
+        "acp_case_footprint": (ctypes.c_int, [D, i32, i32, ctypes.POINTER(i64)]),
+        "acp_case_create_in": (ctypes.c_int, [D, vp, vp, i64, ctypes.POINTER(vp)]),
+        "acp_case_checksum": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "acp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(vp)]),
+        "acp_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request)]),
+        "acp_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result), i32, i32, ctypes.POINTER(i32)]),
+        "acp_server_get_stats": (ctypes.c_int, [vp, ctypes.POINTER(ServerStats)]),
+        "acp_server_clock": (ctypes.c_double, [vp]),
+        "acp_server_destroy": (None, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

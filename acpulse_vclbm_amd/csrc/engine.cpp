@@ -162,12 +162,27 @@ void kshift_init(KShift& k, const acp_case_desc& d) {
 LodiSchedule build_lodi_schedule(int nx, int ny, int nz) {
     const int im = nx - 1, jm = ny - 1, km = nz - 1;
     auto lin = [&](int i, int j, int k) { return (int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k); };
-    std::vector<LodiNode> seq;
-    seq.reserve(2 * ((size_t)ny * nz + (size_t)nx * ny + (size_t)nx * nz));
+    // Position of a boundary node in the pass order below (x faces by (j, k), then
+    // z faces by (i, j), then y faces by (i, k)); -1 for an interior node.
+    const int64_t z0 = 2 * (int64_t)ny * nz, y0 = z0 + 2 * (int64_t)(nx - 2) * ny;
+    auto pos3 = [&](int i, int j, int k) -> int64_t {
+        if (i == 0 || i == im) return 2 * ((int64_t)j * nz + k) + (i == im);
+        if (k == 0 || k == km) return z0 + 2 * ((int64_t)(i - 1) * ny + j) + (k == km);
+        if (j == 0 || j == jm) return y0 + 2 * ((int64_t)(i - 1) * (nz - 2) + (k - 1)) + (j == jm);
+        return -1;
+    };
+    auto neighbour = [&](int64_t& node, int64_t& pos, int i, int j, int k) {
+        node = lin(i, j, k);
+        pos = pos3(i, j, k);
+    };
+    std::vector<int64_t> rd1, rd2;  // pass-order positions of the normal neighbours
+    const size_t nbnd = 2 * ((size_t)ny * nz + (size_t)nx * ny + (size_t)nx * nz);
+    rd1.reserve(nbnd);
+    rd2.reserve(nbnd);
     // Normal, second-normal and tangent neighbours, inward normal direction and
     // corner flag as Lattice::setConnectivity assigns them (Lattice.cpp:500-1392).
-    auto add = [&](int i, int j, int k) {
-        LodiNode n;
+    auto record = [&](int i, int j, int k, LodiNode& n, int64_t& p1, int64_t& p2) {
+        p1 = p2 = -1;
         n.node = lin(i, j, k);
         for (int t = 0; t < 4; ++t) n.t[t] = -1;
         const int bi = i == 0 ? -1 : (i == im ? 1 : 0);
@@ -178,42 +193,48 @@ LodiSchedule build_lodi_schedule(int nx, int ny, int nz) {
             n.corner = 0;
             if (bk) {  // z faces
                 n.dir = bk < 0 ? 4 : 5;
-                n.n1 = lin(i, j, k - bk);
-                n.n2 = lin(i, j, k - 2 * bk);
+                neighbour(n.n1, p1, i, j, k - bk);
+                neighbour(n.n2, p2, i, j, k - 2 * bk);
                 n.t[0] = lin(i + 1, j, k); n.t[1] = lin(i - 1, j, k); n.t[2] = lin(i, j + 1, k); n.t[3] = lin(i, j - 1, k);
             } else if (bj) {  // y faces
                 n.dir = bj < 0 ? 1 : 3;
-                n.n1 = lin(i, j - bj, k);
-                n.n2 = lin(i, j - 2 * bj, k);
+                neighbour(n.n1, p1, i, j - bj, k);
+                neighbour(n.n2, p2, i, j - 2 * bj, k);
                 n.t[0] = lin(i + 1, j, k); n.t[1] = lin(i - 1, j, k); n.t[2] = lin(i, j, k + 1); n.t[3] = lin(i, j, k - 1);
             } else {  // x faces
                 n.dir = bi < 0 ? 0 : 2;
-                n.n1 = lin(i - bi, j, k);
-                n.n2 = lin(i - 2 * bi, j, k);
+                neighbour(n.n1, p1, i - bi, j, k);
+                neighbour(n.n2, p2, i - 2 * bi, j, k);
                 n.t[0] = lin(i, j + 1, k); n.t[1] = lin(i, j - 1, k); n.t[2] = lin(i, j, k + 1); n.t[3] = lin(i, j, k - 1);
             }
         } else {
             n.corner = 1;
             if (bi == 0) {  // edges along x: normal along the inward yz diagonal
                 n.dir = bj < 0 ? 1 : 3;
-                n.n1 = lin(i, j - bj, k - bk);
-                n.n2 = lin(i, j - 2 * bj, k - 2 * bk);
+                neighbour(n.n1, p1, i, j - bj, k - bk);
+                neighbour(n.n2, p2, i, j - 2 * bj, k - 2 * bk);
             } else if (bi < 0) {  // inlet edges and corners
                 n.dir = 0;
                 if (nb == 3) {
-                    n.n1 = lin(1, j, k);
-                    n.n2 = lin(2, j, k);
+                    neighbour(n.n1, p1, 1, j, k);
+                    neighbour(n.n2, p2, 2, j, k);
                 } else {
-                    n.n1 = lin(1, j - bj, k - bk);
-                    n.n2 = lin(2, j - 2 * bj, k - 2 * bk);
+                    neighbour(n.n1, p1, 1, j - bj, k - bk);
+                    neighbour(n.n2, p2, 2, j - 2 * bj, k - 2 * bk);
                 }
             } else {  // outlet edges and corners
                 n.dir = bk ? (bk < 0 ? 4 : 5) : (bj < 0 ? 1 : 3);
-                n.n1 = lin(im - 1, j - bj, k - bk);
-                n.n2 = lin(im - 2, j - 2 * bj, k - 2 * bk);
+                neighbour(n.n1, p1, im - 1, j - bj, k - bk);
+                neighbour(n.n2, p2, im - 2, j - 2 * bj, k - 2 * bk);
             }
         }
-        seq.push_back(n);
+    };
+    auto add = [&](int i, int j, int k) {
+        LodiNode n;
+        int64_t p1, p2;
+        record(i, j, k, n, p1, p2);
+        rd1.push_back(p1);
+        rd2.push_back(p2);
     };
     for (int j = 0; j <= jm; ++j)
         for (int k = 0; k <= km; ++k) {
@@ -233,35 +254,43 @@ LodiSchedule build_lodi_schedule(int nx, int ny, int nz) {
     // phases: a node that reads the B entries of a boundary node handled before it
     // goes to a later phase; one that reads a node handled after it forces that
     // node into a later phase.
-    const size_t n = seq.size();
-    std::unordered_map<int64_t, int32_t> pos;
-    pos.reserve(n * 2);
-    for (size_t s = 0; s < n; ++s) pos[seq[s].node] = (int32_t)s;
+    const size_t n = rd1.size();
     std::vector<int32_t> phase(n, 1), lb(n, 1);
     int32_t maxph = 1;
     for (size_t s = 0; s < n; ++s) {
         int32_t ph = lb[s];
-        const int64_t rd[2] = {seq[s].n1, seq[s].n2};
-        for (int r = 0; r < 2; ++r) {
-            auto it = pos.find(rd[r]);
-            if (it == pos.end() || it->second == (int32_t)s) continue;
-            if ((size_t)it->second < s) ph = std::max(ph, phase[it->second] + 1);
-        }
+        const int64_t rd[2] = {rd1[s], rd2[s]};
+        for (int r = 0; r < 2; ++r)
+            if (rd[r] >= 0 && rd[r] < (int64_t)s) ph = std::max(ph, phase[rd[r]] + 1);
         phase[s] = ph;
         maxph = std::max(maxph, ph);
-        for (int r = 0; r < 2; ++r) {
-            auto it = pos.find(rd[r]);
-            if (it != pos.end() && (size_t)it->second > s) lb[it->second] = std::max(lb[it->second], ph + 1);
-        }
+        for (int r = 0; r < 2; ++r)
+            if (rd[r] > (int64_t)s) lb[rd[r]] = std::max(lb[rd[r]], ph + 1);
     }
+    // Emit each phase in linear node order (i fastest): nodes of one phase are
+    // independent, and this order makes consecutive threads of the LODI kernel
+    // touch consecutive nodes.  Only boundary nodes are visited.
+    auto visit = [&](auto&& f) {
+        for (int k = 0; k <= km; ++k)
+            for (int j = 0; j <= jm; ++j) {
+                if (k == 0 || k == km || j == 0 || j == jm) {
+                    for (int i = 0; i <= im; ++i) f(i, j, k);
+                } else {
+                    f(0, j, k);
+                    f(im, j, k);
+                }
+            }
+    };
+    std::vector<int64_t> cursor((size_t)maxph + 2, 0);
+    visit([&](int i, int j, int k) { ++cursor[phase[pos3(i, j, k)] + 1]; });
+    for (int32_t ph = 1; ph <= maxph; ++ph) cursor[ph + 1] += cursor[ph];
     LodiSchedule out;
-    out.nodes.reserve(n);
-    out.phase_start.push_back(0);
-    for (int32_t ph = 1; ph <= maxph; ++ph) {
-        for (size_t s = 0; s < n; ++s)
-            if (phase[s] == ph) out.nodes.push_back(seq[s]);
-        out.phase_start.push_back((int64_t)out.nodes.size());
-    }
+    out.phase_start.assign(cursor.begin() + 1, cursor.end());
+    out.nodes.resize(n);
+    visit([&](int i, int j, int k) {
+        int64_t p1, p2;
+        record(i, j, k, out.nodes[cursor[phase[pos3(i, j, k)]]++], p1, p2);
+    });
     return out;
 }
 
@@ -279,23 +308,72 @@ size_t elem_bytes(int prec) { return prec == ACP_PREC_F64 ? 8 : (prec == ACP_PRE
 template <typename T>
 int dalloc(acp_case* c, T** p, size_t count) {
     const size_t bytes = count * sizeof(T);
-    hipError_t e = hipMalloc((void**)p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return fail(ACP_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
-                                   hipGetErrorString(e));
+    const int64_t rounded = acp::round_alloc((int64_t)bytes);
+    if (c->arena) {
+        if (c->arena_used + rounded > c->arena_size) {
+            *p = nullptr;
+            return fail(ACP_E_OOM, "case region too small: needs more than " + std::to_string(c->arena_size) +
+                                       " bytes");
+        }
+        *p = reinterpret_cast<T*>(c->arena + c->arena_used);
+        c->arena_used += rounded;
+    } else {
+        hipError_t e = hipMalloc((void**)p, bytes);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            (void)hipGetLastError();
+            return fail(ACP_E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed: " +
+                                       hipGetErrorString(e));
+        }
     }
-    c->device_bytes += (int64_t)bytes;
+    c->device_bytes += rounded;
     return ACP_OK;
 }
 
+// Give back a buffer obtained from dalloc (nothing to do inside a region).
+template <typename T>
+void dfree(acp_case* c, T*& p) {
+    if (p && !c->arena) (void)hipFree(p);
+    p = nullptr;
+}
+
+void drop_graphs(acp_case* c) {
+    for (auto& g : c->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
 void release(acp_case* c) {
+    drop_graphs(c);
     void* ptrs[] = {c->dk, c->A, c->B, c->psi, c->M, c->dks, c->SA, c->SB, c->SM, c->lodi, c->probe_nodes,
-                    c->probe_buf, c->partial};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
+                    c->probe_buf, c->probe_slot, c->partial, c->stage};
+    if (!c->arena)
+        for (void* p : ptrs)
+            if (p) (void)hipFree(p);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// Host-to-device copy ordered on the case's own stream (never the null stream,
+// so that it neither waits for nor disturbs work other threads issue on other
+// streams), complete on return.
+hipError_t upload(acp_case* c, void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    return e;
+}
+
+// After a new initial state: step and probe counters back to zero.
+int restart(acp_case* c) {
+    c->initialized = true;
+    c->steps = 0;
+    c->probe_count = 0;
+    if (c->probe_slot) {
+        HIPCHK(hipMemsetAsync(c->probe_slot, 0, sizeof(int32_t), c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return ACP_OK;
 }
 
 // Upload host float64 populations [19][N] (SI units) into both arrays.
@@ -313,9 +391,8 @@ int upload_f64(acp_case* c, const double* f) {
         return ACP_OK;
     }
     // staged conversion, one q-plane segment at a time
-    const int64_t chunk = std::min<int64_t>(N, (int64_t)1 << 24);
-    double* stage = nullptr;
-    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * chunk));
+    const int64_t chunk = c->stage_elems;
+    double* stage = c->stage;
     const size_t eb = elem_bytes(c->precision);
     int rc = ACP_OK;
     for (int q = 0; q < 19 && rc == ACP_OK; ++q)
@@ -329,7 +406,6 @@ int upload_f64(acp_case* c, const double* f) {
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
             if (e != hipSuccess) rc = acp::hip_fail(e, "population upload");
         }
-    (void)hipFree(stage);
     return rc;
 }
 
@@ -338,39 +414,38 @@ int check_case(const acp_case* c) {
     return ACP_OK;
 }
 
-// host-side initial condition: rho(i,j) with u = 0, populations feq_N, uploaded in chunks
+// Initial condition rho(i, j) (the same on every k) with u = 0, populations
+// feq_N: the host evaluates the density one band of rows at a time into the
+// staging buffer; the device evaluates the equilibria and fills both arrays.
 template <typename RhoFn>
 int init_rho_field(acp_case* c, RhoFn rho_of) {
     const acp::Grid& g = c->g;
-    const int64_t N = g.N;
-    std::vector<double> f((size_t)19 * N);
-    acp::K64 k = c->hk;
-    for (int kz = 0; kz < g.nz; ++kz)
-        for (int j = 0; j < g.ny; ++j)
-            for (int i = 0; i < g.nx; ++i) {
-                const int64_t l = (int64_t)i + (int64_t)g.nx * ((int64_t)j + (int64_t)g.ny * kz);
-                const double rho = rho_of(i, j, kz);
-                double eq[19];
-                if (c->d.order == 2) acp::feq64<2>(k, rho, 0., 0., 0., eq);
-                else if (c->d.order == 3) acp::feq64<3>(k, rho, 0., 0., 0., eq);
-                else acp::feq64<4>(k, rho, 0., 0., 0., eq);
-                for (int q = 0; q < 19; ++q) f[(size_t)q * N + l] = eq[q];
-            }
-    int rc = upload_f64(c, f.data());
-    if (rc == ACP_OK) {
-        c->initialized = true;
-        c->steps = 0;
-        c->probe_count = 0;
+    const int rows = (int)std::min<int64_t>(g.ny, c->stage_elems / g.nx);
+    if (rows < 1) return fail(ACP_E_ARG, "grid rows longer than the staging buffer");
+    std::vector<double> band((size_t)rows * g.nx);
+    void* A = shifted(c) ? c->SA : (void*)c->A;
+    void* B = shifted(c) ? c->SB : (void*)c->B;
+    for (int j0 = 0; j0 < g.ny; j0 += rows) {
+        const int nr = std::min(rows, g.ny - j0);
+        for (int j = 0; j < nr; ++j)
+            for (int i = 0; i < g.nx; ++i) band[(size_t)j * g.nx + i] = rho_of(i, j0 + j, 0);
+        // the upload waits on the stream, so the previous band's kernel is done with the stage
+        HIPCHK(upload(c, c->stage, band.data(), sizeof(double) * (size_t)nr * g.nx));
+        HIPCHK(acp::launch_init_rows(c->precision, c->d.order, c->dk, c->dks, c->stage, j0, nr, g, A, B, c->stream));
     }
-    return rc;
+    if (c->psi) HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(double) * 19 * g.N, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return restart(c);
 }
 
 }  // namespace
 
-extern "C" int acp_case_create(const acp_case_desc* desc, void* stream, acp_case** out) {
-    if (!desc || !out) return fail(ACP_E_ARG, "acp_case_create: null argument");
-    *out = nullptr;
-    const acp_case_desc& d = *desc;
+namespace {
+
+// float64 staging buffer of the shifted tiers' conversions: 1 Mi elements (8 MiB)
+constexpr int64_t kStageElems = (int64_t)1 << 20;
+
+int validate_desc(const acp_case_desc& d) {
     if (d.max_level != 0)
         return fail(ACP_E_UNSUPPORTED, "MaxLevel > 0: two-level refinement is not part of this engine");
     if (d.nx < 3 || d.ny < 3 || d.nz < 3) return fail(ACP_E_ARG, "grid needs at least 3 nodes per direction");
@@ -383,8 +458,70 @@ extern "C" int acp_case_create(const acp_case_desc* desc, void* stream, acp_case
     if (!(d.cs > 0) || !(d.spacing > 0) || !(d.nu > 0) || !(d.rho0 > 0))
         return fail(ACP_E_ARG, "SpeedOfSound, Spacing, KinematicViscosity and Density must be positive");
     if (d.precision == ACP_PREC_F16_SHIFTED && !(d.f16_scale > 0)) return fail(ACP_E_ARG, "FP16 scale must be positive");
+    return ACP_OK;
+}
+
+// Device bytes a case of `d` holds: the same buffers, in the same 256-byte
+// units, as acp_case_create allocates, plus nprobe probes of `cap` steps.
+int64_t footprint_bytes(const acp_case_desc& d, int32_t nprobe, int32_t cap) {
+    using acp::round_alloc;
+    const int64_t N = (int64_t)d.nx * d.ny * d.nz;
+    const bool need_macro = d.op == ACP_OP_HRR || (d.op == ACP_OP_RR && d.cubic);
+    const bool need_psi = d.cubic && (d.op == ACP_OP_RR || d.op == ACP_OP_HRR);
+    int64_t b = round_alloc(sizeof(acp::K64));
+    if (d.precision == ACP_PREC_F64) {
+        b += 2 * round_alloc(19 * N * 8);
+        if (need_psi) b += round_alloc(19 * N * 8);
+        if (need_macro) b += round_alloc(4 * N * 8);
+    } else {
+        const int64_t eb = d.precision == ACP_PREC_F32_SHIFTED ? 4 : 2;
+        b += round_alloc(sizeof(acp::KShift)) + 2 * round_alloc(19 * N * eb);
+        if (d.op == ACP_OP_HRR) b += round_alloc(4 * N * 4);
+    }
+    b += round_alloc(std::min<int64_t>(N, kStageElems) * 8);
+    if (d.boundary != ACP_BC_PERIODIC)
+        b += round_alloc((N - (int64_t)(d.nx - 2) * (d.ny - 2) * (d.nz - 2)) * (int64_t)sizeof(acp::LodiNode));
+    b += round_alloc(std::min<int64_t>(1024, (N + 255) / 256) * 8);
+    if (nprobe > 0)
+        b += round_alloc((int64_t)nprobe * 8) + round_alloc((int64_t)nprobe * std::max(cap, 1) * 8) + round_alloc(4);
+    return b;
+}
+
+int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t region_bytes, acp_case** out);
+
+}  // namespace
+
+extern "C" int acp_case_footprint(const acp_case_desc* desc, int32_t nprobe, int32_t probe_capacity,
+                                  int64_t* bytes) {
+    if (!desc || !bytes || nprobe < 0 || probe_capacity < 0) return fail(ACP_E_ARG, "acp_case_footprint: bad argument");
+    if (int rc = validate_desc(*desc)) return rc;
+    *bytes = footprint_bytes(*desc, nprobe, probe_capacity);
+    acp::clear_error();
+    return ACP_OK;
+}
+
+extern "C" int acp_case_create(const acp_case_desc* desc, void* stream, acp_case** out) {
+    return create_case(desc, stream, nullptr, 0, out);
+}
+
+extern "C" int acp_case_create_in(const acp_case_desc* desc, void* stream, void* region, int64_t region_bytes,
+                                  acp_case** out) {
+    if (!region || region_bytes <= 0 || ((uintptr_t)region % 256) != 0)
+        return fail(ACP_E_ARG, "acp_case_create_in: region must be non-null, 256-byte aligned and non-empty");
+    return create_case(desc, stream, region, region_bytes, out);
+}
+
+namespace {
+
+int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t region_bytes, acp_case** out) {
+    if (!desc || !out) return fail(ACP_E_ARG, "acp_case_create: null argument");
+    *out = nullptr;
+    const acp_case_desc& d = *desc;
+    if (int rc = validate_desc(d)) return rc;
 
     acp_case* c = new acp_case();
+    c->arena = (char*)region;
+    c->arena_size = region_bytes;
     c->d = d;
     c->g = acp::Grid{d.nx, d.ny, d.nz, (int64_t)d.nx * d.ny * d.nz};
     c->precision = d.precision;
@@ -423,7 +560,7 @@ extern "C" int acp_case_create(const acp_case_desc* desc, void* stream, acp_case
             return bail(rc);
         if (c->need_psi && (rc = dalloc(c, &c->psi, (size_t)19 * N))) return bail(rc);
         if (c->need_macro && (rc = dalloc(c, &c->M, (size_t)4 * N))) return bail(rc);
-        hipError_t e = hipMemcpy(c->dk, &c->hk, sizeof(acp::K64), hipMemcpyHostToDevice);
+        hipError_t e = upload(c, c->dk, &c->hk, sizeof(acp::K64));
         if (e != hipSuccess) return bail(acp::hip_fail(e, "constants upload"));
     } else {
         const size_t eb = elem_bytes(c->precision);
@@ -434,27 +571,32 @@ extern "C" int acp_case_create(const acp_case_desc* desc, void* stream, acp_case
         c->SA = a;
         c->SB = b;
         if (d.op == ACP_OP_HRR && (rc = dalloc(c, &c->SM, (size_t)4 * N))) return bail(rc);
-        hipError_t e = hipMemcpy(c->dks, &c->hks, sizeof(acp::KShift), hipMemcpyHostToDevice);
+        hipError_t e = upload(c, c->dks, &c->hks, sizeof(acp::KShift));
         if (e != hipSuccess) return bail(acp::hip_fail(e, "constants upload"));
         // the float64 constants are used by the conversions of the reference initial condition
         if ((rc = dalloc(c, &c->dk, 1))) return bail(rc);
-        e = hipMemcpy(c->dk, &c->hk, sizeof(acp::K64), hipMemcpyHostToDevice);
+        e = upload(c, c->dk, &c->hk, sizeof(acp::K64));
         if (e != hipSuccess) return bail(acp::hip_fail(e, "constants upload"));
     }
     if (!c->periodic) {
         acp::LodiSchedule sch = acp::build_lodi_schedule(d.nx, d.ny, d.nz);
         c->phase_start = sch.phase_start;
         if ((rc = dalloc(c, &c->lodi, sch.nodes.size()))) return bail(rc);
-        hipError_t e = hipMemcpy(c->lodi, sch.nodes.data(), sizeof(acp::LodiNode) * sch.nodes.size(),
-                                 hipMemcpyHostToDevice);
+        hipError_t e = upload(c, c->lodi, sch.nodes.data(), sizeof(acp::LodiNode) * sch.nodes.size());
         if (e != hipSuccess) return bail(acp::hip_fail(e, "LODI schedule upload"));
     }
     c->partial_blocks = (int)std::min<int64_t>(1024, (N + 255) / 256);
     if ((rc = dalloc(c, &c->partial, (size_t)c->partial_blocks))) return bail(rc);
+    c->stage_elems = std::min<int64_t>(N, kStageElems);
+    if ((rc = dalloc(c, &c->stage, (size_t)c->stage_elems))) return bail(rc);
+    c->probe_mark = c->arena_used;
+    c->arr0 = shifted(c) ? c->SA : (const void*)c->A;
     *out = c;
     acp::clear_error();
     return ACP_OK;
 }
+
+}  // namespace
 
 extern "C" void acp_case_destroy(acp_case* c) {
     if (!c) return;
@@ -517,30 +659,21 @@ extern "C" int acp_case_init_random(acp_case* c, uint64_t seed, double rho_amp, 
     }
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) return acp::hip_fail(e, "acp_case_init_random");
-    c->initialized = true;
-    c->steps = 0;
-    c->probe_count = 0;
-    return ACP_OK;
+    return restart(c);
 }
 
 extern "C" int acp_case_set_populations(acp_case* c, const double* f) {
     if (int rc = check_case(c)) return rc;
     if (!f) return fail(ACP_E_ARG, "null population array");
     int rc = upload_f64(c, f);
-    if (rc == ACP_OK) {
-        c->initialized = true;
-        c->steps = 0;
-        c->probe_count = 0;
-    }
+    if (rc == ACP_OK) rc = restart(c);
     return rc;
 }
 
-extern "C" int acp_case_step(acp_case* c, int32_t n) {
-    if (int rc = check_case(c)) return rc;
-    if (!c->initialized) return fail(ACP_E_STATE, "acp_case_step before initialisation");
-    if (n < 0) return fail(ACP_E_ARG, "negative step count");
-    if (c->nprobe > 0 && c->probe_count + n > c->probe_cap)
-        return fail(ACP_E_STATE, "probe buffer full: read the probes first");
+namespace {
+
+// Launch n steps on the case stream, one kernel sequence per step.
+int step_eager(acp_case* c, int32_t n) {
     const int64_t N = c->g.N;
     const int nph = c->phase_start.empty() ? 0 : (int)c->phase_start.size() - 1;
     for (int32_t s = 0; s < n; ++s) {
@@ -555,8 +688,8 @@ extern "C" int acp_case_step(acp_case* c, int32_t n) {
                                        c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], N,
                                        c->stream);
             if (e == hipSuccess && c->nprobe > 0)
-                e = acp::launch_probe64(c->B, N, c->probe_nodes, c->nprobe,
-                                        c->probe_buf + (size_t)c->probe_count * c->nprobe, c->stream);
+                e = acp::launch_probe64(c->B, N, c->probe_nodes, c->nprobe, c->probe_buf, c->probe_slot,
+                                        c->probe_cap, c->stream);
             std::swap(c->A, c->B);
         } else {
             if (c->SM) e = acp::launch_macro_shift(c->precision, c->dks, c->SA, c->SM, c->g, c->stream);
@@ -568,14 +701,86 @@ extern "C" int acp_case_step(acp_case* c, int32_t n) {
                                            c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], N,
                                            c->stream);
             if (e == hipSuccess && c->nprobe > 0)
-                e = acp::launch_probe_shift(c->precision, c->dks, c->SB, N, c->probe_nodes, c->nprobe,
-                                            c->probe_buf + (size_t)c->probe_count * c->nprobe, c->stream);
+                e = acp::launch_probe_shift(c->precision, c->dks, c->SB, N, c->probe_nodes, c->nprobe, c->probe_buf,
+                                            c->probe_slot, c->probe_cap, c->stream);
             std::swap(c->SA, c->SB);
         }
         if (e != hipSuccess) return acp::hip_fail(e, "acp_case_step launch");
         if (c->nprobe > 0) ++c->probe_count;
         ++c->steps;
     }
+    return ACP_OK;
+}
+
+int parity(const acp_case* c) { return (shifted(c) ? c->SA : (const void*)c->A) == c->arr0 ? 0 : 1; }
+
+// One chunk of graph_chunk steps (an even count, so the arrays end where they
+// started) as a captured graph; captured on first use for each parity.
+int step_graph_chunk(acp_case* c) {
+    const int p = parity(c);
+    if (!c->gexec[p]) {
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        const int32_t steps0 = c->steps, count0 = c->probe_count;
+        const int rc = step_eager(c, c->graph_chunk);
+        hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->steps = steps0;
+        c->probe_count = count0;
+        if (rc != ACP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&c->gexec[p], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->gexec[p] = nullptr;
+            return acp::hip_fail(e, "step graph capture");
+        }
+    }
+    HIPCHK(hipGraphLaunch(c->gexec[p], c->stream));
+    c->steps += c->graph_chunk;
+    if (c->nprobe > 0) c->probe_count += c->graph_chunk;
+    return ACP_OK;
+}
+
+}  // namespace
+
+extern "C" int acp_case_step(acp_case* c, int32_t n) {
+    if (int rc = check_case(c)) return rc;
+    if (!c->initialized) return fail(ACP_E_STATE, "acp_case_step before initialisation");
+    if (n < 0) return fail(ACP_E_ARG, "negative step count");
+    if (c->nprobe > 0 && c->probe_count + n > c->probe_cap)
+        return fail(ACP_E_STATE, "probe buffer full: read the probes first");
+    while (c->graph_chunk > 0 && n >= c->graph_chunk) {
+        if (int rc = step_graph_chunk(c)) return rc;
+        n -= c->graph_chunk;
+    }
+    return step_eager(c, n);
+}
+
+extern "C" int acp_case_enable_graphs(acp_case* c, int32_t chunk) {
+    if (int rc = check_case(c)) return rc;
+    if (chunk < 0 || chunk == 1 || (chunk % 2) != 0)
+        return fail(ACP_E_ARG, "graph chunk must be 0 (off) or an even number of steps");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->graph_chunk = chunk;
+    return ACP_OK;
+}
+
+extern "C" int acp_case_checksum(acp_case* c, uint64_t* out) {
+    if (int rc = check_case(c)) return rc;
+    if (!out) return fail(ACP_E_ARG, "null checksum output");
+    const void* a = shifted(c) ? c->SA : (const void*)c->A;
+    const int eb = shifted(c) ? (int)elem_bytes(c->precision) : 8;
+    unsigned long long* part = reinterpret_cast<unsigned long long*>(c->partial);
+    HIPCHK(acp::launch_checksum(a, eb, 19 * c->g.N, part, c->partial_blocks, c->stream));
+    std::vector<unsigned long long> h(c->partial_blocks);
+    HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    uint64_t sum = 0;
+    for (unsigned long long v : h) sum += (uint64_t)v;
+    *out = sum;
     return ACP_OK;
 }
 
@@ -594,9 +799,8 @@ extern "C" int acp_case_get_populations(acp_case* c, double* f) {
         HIPCHK(hipStreamSynchronize(c->stream));
         return ACP_OK;
     }
-    const int64_t chunk = std::min<int64_t>(N, (int64_t)1 << 24);
-    double* stage = nullptr;
-    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * chunk));
+    const int64_t chunk = c->stage_elems;
+    double* stage = c->stage;
     const size_t eb = elem_bytes(c->precision);
     int rc = ACP_OK;
     for (int q = 0; q < 19 && rc == ACP_OK; ++q)
@@ -609,7 +813,6 @@ extern "C" int acp_case_get_populations(acp_case* c, double* f) {
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
             if (e != hipSuccess) rc = acp::hip_fail(e, "population download");
         }
-    (void)hipFree(stage);
     return rc;
 }
 
@@ -653,17 +856,34 @@ extern "C" int acp_case_set_probes(acp_case* c, const int64_t* nodes, int32_t np
     for (int32_t p = 0; p < nprobe; ++p)
         if (nodes[p] < 0 || nodes[p] >= c->g.N) return fail(ACP_E_ARG, "probe node out of range");
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->probe_nodes) (void)hipFree(c->probe_nodes);
-    if (c->probe_buf) (void)hipFree(c->probe_buf);
-    c->probe_nodes = nullptr;
-    c->probe_buf = nullptr;
+    drop_graphs(c);  // captured graphs hold the old buffers
+    dfree(c, c->probe_nodes);
+    dfree(c, c->probe_buf);
+    dfree(c, c->probe_slot);
+    c->device_bytes -= c->probe_bytes;
+    c->probe_bytes = 0;
+    if (c->arena) c->arena_used = c->probe_mark;
     c->nprobe = 0;
     c->probe_cap = 0;
     c->probe_count = 0;
     if (nprobe == 0) return ACP_OK;
-    HIPCHK(hipMalloc((void**)&c->probe_nodes, sizeof(int64_t) * nprobe));
-    HIPCHK(hipMalloc((void**)&c->probe_buf, sizeof(double) * (size_t)nprobe * std::max(capacity, 1)));
-    HIPCHK(hipMemcpy(c->probe_nodes, nodes, sizeof(int64_t) * nprobe, hipMemcpyHostToDevice));
+    const int64_t before = c->device_bytes;
+    int rc = dalloc(c, &c->probe_nodes, (size_t)nprobe);
+    if (rc == ACP_OK) rc = dalloc(c, &c->probe_buf, (size_t)nprobe * std::max(capacity, 1));
+    if (rc == ACP_OK) rc = dalloc(c, &c->probe_slot, 1);
+    c->probe_bytes = c->device_bytes - before;
+    if (rc != ACP_OK) {
+        dfree(c, c->probe_nodes);
+        dfree(c, c->probe_buf);
+        dfree(c, c->probe_slot);
+        c->device_bytes = before;
+        c->probe_bytes = 0;
+        if (c->arena) c->arena_used = c->probe_mark;
+        return rc;
+    }
+    HIPCHK(hipMemcpyAsync(c->probe_nodes, nodes, sizeof(int64_t) * nprobe, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->probe_slot, 0, sizeof(int32_t), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->nprobe = nprobe;
     c->probe_cap = capacity;
     return ACP_OK;
@@ -677,8 +897,9 @@ extern "C" int acp_case_read_probes(acp_case* c, double* out, int32_t* nsteps) {
         if (!out) return fail(ACP_E_ARG, "null probe output");
         HIPCHK(hipMemcpyAsync(out, c->probe_buf, sizeof(double) * (size_t)c->probe_count * c->nprobe,
                               hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
     }
+    if (c->probe_slot) HIPCHK(hipMemsetAsync(c->probe_slot, 0, sizeof(int32_t), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     c->probe_count = 0;
     return ACP_OK;
 }
@@ -706,9 +927,8 @@ int download_array(acp_case* c, bool current, double* f) {
         HIPCHK(hipStreamSynchronize(c->stream));
         return ACP_OK;
     }
-    const int64_t chunk = std::min<int64_t>(N, (int64_t)1 << 24);
-    double* stage = nullptr;
-    HIPCHK(hipMalloc((void**)&stage, sizeof(double) * chunk));
+    const int64_t chunk = c->stage_elems;
+    double* stage = c->stage;
     const size_t eb = elem_bytes(c->precision);
     const char* src = (const char*)(current ? c->SA : c->SB);
     int rc = ACP_OK;
@@ -722,7 +942,6 @@ int download_array(acp_case* c, bool current, double* f) {
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
             if (e != hipSuccess) rc = acp::hip_fail(e, "population download");
         }
-    (void)hipFree(stage);
     return rc;
 }
 }  // namespace
@@ -740,7 +959,8 @@ extern "C" int acp_case_write_vtk(acp_case* c, const char* stem) {
     std::vector<double> psi;
     if (c->psi) {
         psi.resize((size_t)19 * N);
-        HIPCHK(hipMemcpy(psi.data(), c->psi, sizeof(double) * 19 * N, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpyAsync(psi.data(), c->psi, sizeof(double) * 19 * N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
     }
     return acp::write_vtk_host(c->d, c->hk, cur.data(), prev.empty() ? nullptr : prev.data(),
                                psi.empty() ? nullptr : psi.data(), stem);

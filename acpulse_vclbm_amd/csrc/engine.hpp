@@ -14,6 +14,10 @@
 namespace acp {
 
 int fail(int code, const std::string& msg);
+
+// Every device buffer of a case is accounted (and placed inside a caller's
+// region) in multiples of 256 bytes.
+inline int64_t round_alloc(int64_t bytes) { return (bytes + 255) / 256 * 256; }
 void clear_error();
 int hip_fail(hipError_t e, const char* where);
 
@@ -60,7 +64,19 @@ struct acp_case {
     // probes and reductions
     int64_t* probe_nodes = nullptr;
     double* probe_buf = nullptr;
+    int32_t* probe_slot = nullptr;  // device row counter of the probe series
     int32_t nprobe = 0, probe_cap = 0, probe_count = 0;
+    int64_t probe_bytes = 0;
+    // staging buffer of the shifted tiers' float64 conversions
+    double* stage = nullptr;
+    int64_t stage_elems = 0;
+    // caller-owned device region the case lives in (acp_case_create_in), or null
+    char* arena = nullptr;
+    int64_t arena_size = 0, arena_used = 0, probe_mark = 0;
+    // captured step chunks: one executable graph per ping-pong parity
+    int32_t graph_chunk = 0;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    const void* arr0 = nullptr;  // the array that is current at parity 0
     double* partial = nullptr;
     int partial_blocks = 0;
     hipStream_t stream = nullptr;

@@ -35,7 +35,9 @@ hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, c
 hipError_t launch_init_random64(int order, const K64* k, double* A, double* B, const Grid& g, uint64_t seed,
                                double rho0, double rho_amp, double u_amp, hipStream_t s);
 hipError_t launch_probe64(const double* A, int64_t N, const int64_t* nodes, int nprobe, double* out,
-                          hipStream_t s);
+                          int32_t* slot, int32_t cap, hipStream_t s);
+hipError_t launch_checksum(const void* a, int elem_bytes, int64_t n, unsigned long long* partial, int nblocks,
+                           hipStream_t s);
 hipError_t launch_mass64(const double* A, int64_t N, double* partial, int nblocks, hipStream_t s);
 hipError_t launch_macros_out64(const K64* k, const double* A, const Grid& g, double* rho, double* u,
                                hipStream_t s);

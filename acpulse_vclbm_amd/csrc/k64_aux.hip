@@ -1,6 +1,7 @@
 // Float64 auxiliary kernels: LODI boundary pass, HRR macro pre-pass, device
 // initial condition, probes and the mass reduction.
 #include "k64.hpp"
+#include "kshift.hpp"
 
 namespace acp {
 
@@ -237,22 +238,121 @@ hipError_t launch_init_random64(int order, const K64* k, double* A, double* B, c
     return hipGetLastError();
 }
 
+// Equilibrium initial state from a density field given per (i, j) for a band
+// of rows [j0, j0 + rows), u = 0, every k: feq_N(rho, 0) in float64 with the
+// host's association (this translation unit has FMA contraction off), written
+// to both arrays, float64 or converted to the shifted storage exactly as the
+// population upload converts it, (T)(s * (f / rho0 - w_q)).
+template <typename T, int ORDER>
+__global__ __launch_bounds__(256) void init_rows(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
+                                                 const double* __restrict__ rho_rows, int j0, int rows, int nx, int ny,
+                                                 int nz, T* __restrict__ A, T* __restrict__ B, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t band = (int64_t)rows * nx;
+    if (t >= band * nz) return;
+    const int64_t kz = t / band, r = t - kz * band;
+    const int64_t l = (int64_t)j0 * nx + r + kz * (int64_t)nx * ny;
+    double eq[19];
+    feq64<ORDER>(*kp, rho_rows[r], 0., 0., 0., eq);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        T v;
+        if constexpr (sizeof(T) == 8) v = eq[q];
+        else v = (T)((double)ksp->s * (eq[q] / ksp->rho0 - weight(q)));
+        A[q * N + l] = v;
+        B[q * N + l] = v;
+    });
+}
+
+hipError_t launch_init_rows(int prec, int order, const K64* k, const KShift* ks, const double* rho_rows, int j0,
+                            int rows, const Grid& g, void* A, void* B, hipStream_t s) {
+    const int64_t n = (int64_t)rows * g.nx * g.nz;
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+#define IR(T, O)                                                                                                  \
+    hipLaunchKernelGGL((init_rows<T, O>), grid, block, 0, s, k, ks, rho_rows, j0, rows, g.nx, g.ny, g.nz, (T*)A, \
+                       (T*)B, g.N)
+#define IRO(T)                     \
+    if (order == 2) IR(T, 2);      \
+    else if (order == 3) IR(T, 3); \
+    else IR(T, 4)
+    if (prec == ACP_PREC_F64) IRO(double);
+    else if (prec == ACP_PREC_F32_SHIFTED) IRO(float);
+    else IRO(_Float16);
+#undef IRO
+#undef IR
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ probes and mass
-__global__ void probe64(const double* __restrict__ A, int64_t N, const int64_t* __restrict__ nodes, int nprobe,
-                        double* __restrict__ out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nprobe) return;
-    const int64_t l = nodes[p];
-    double r = 0.;
+// One block: every thread reads the slot counter, writes its probes into row
+// `slot` of the series (if below the capacity), and thread 0 advances the
+// counter after the barrier.  Keeping the row index on the device lets a
+// captured graph of several steps write a new row on every replay.
+__global__ __launch_bounds__(1024) void probe64(const double* __restrict__ A, int64_t N,
+                                                const int64_t* __restrict__ nodes, int nprobe,
+                                                double* __restrict__ out, int32_t* __restrict__ slot, int32_t cap) {
+    const int32_t row = *slot;
+    for (int p = threadIdx.x; p < nprobe; p += blockDim.x) {
+        const int64_t l = nodes[p];
+        double r = 0.;
 #pragma unroll
-    for (int q = 0; q < 19; ++q) r += A[q * N + l];
-    out[p] = r;
+        for (int q = 0; q < 19; ++q) r += A[q * N + l];
+        if (row < cap) out[(int64_t)row * nprobe + p] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = row + 1;
 }
 
 hipError_t launch_probe64(const double* A, int64_t N, const int64_t* nodes, int nprobe, double* out,
-                          hipStream_t s) {
+                          int32_t* slot, int32_t cap, hipStream_t s) {
     if (nprobe <= 0) return hipSuccess;
-    hipLaunchKernelGGL(probe64, dim3((nprobe + 63) / 64), dim3(64), 0, s, A, N, nodes, nprobe, out);
+    const int bs = nprobe >= 1024 ? 1024 : ((nprobe + 63) / 64) * 64;
+    hipLaunchKernelGGL(probe64, dim3(1), dim3(bs), 0, s, A, N, nodes, nprobe, out, slot, cap);
+    return hipGetLastError();
+}
+
+// Order-independent 64-bit fingerprint of an array of n elements of T: the
+// wrapping sum over elements of a mix of (bit pattern, index).  Integer sums
+// are associative, so the result does not depend on the reduction order.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void checksum_k(const T* __restrict__ a, int64_t n,
+                                                  unsigned long long* __restrict__ partial) {
+    __shared__ unsigned long long sh[256];
+    uint64_t acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t bits = 0;
+        if constexpr (sizeof(T) == 8) bits = __builtin_bit_cast(uint64_t, a[i]);
+        else if constexpr (sizeof(T) == 4) bits = __builtin_bit_cast(uint32_t, a[i]);
+        else bits = __builtin_bit_cast(uint16_t, a[i]);
+        acc += mix64(bits ^ ((uint64_t)i * 0x9e3779b97f4a7c15ull));
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+hipError_t launch_checksum(const void* a, int elem_bytes, int64_t n, unsigned long long* partial, int nblocks,
+                           hipStream_t s) {
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(checksum_k<double>, dim3(nblocks), dim3(256), 0, s, (const double*)a, n, partial);
+    else if (elem_bytes == 4)
+        hipLaunchKernelGGL(checksum_k<float>, dim3(nblocks), dim3(256), 0, s, (const float*)a, n, partial);
+    else
+        hipLaunchKernelGGL(checksum_k<_Float16>, dim3(nblocks), dim3(256), 0, s, (const _Float16*)a, n, partial);
     return hipGetLastError();
 }
 

@@ -530,24 +530,32 @@ hipError_t launch_init_random_shift(int prec, int order, const KShift* k, void* 
 }
 
 // ------------------------------------------------------------------ probes and mass (float64)
+// One block; the row index is kept on the device (see probe64).
 template <typename T>
-__global__ void probe_shift(const KShift* __restrict__ kp, const T* __restrict__ A, int64_t N,
-                            const int64_t* __restrict__ nodes, int nprobe, double* __restrict__ out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nprobe) return;
-    double r, ux, uy, uz;
-    macro_f64(*kp, A, N, nodes[p], r, ux, uy, uz);
-    out[p] = r;
+__global__ __launch_bounds__(1024) void probe_shift(const KShift* __restrict__ kp, const T* __restrict__ A, int64_t N,
+                                                    const int64_t* __restrict__ nodes, int nprobe,
+                                                    double* __restrict__ out, int32_t* __restrict__ slot,
+                                                    int32_t cap) {
+    const int32_t row = *slot;
+    for (int p = threadIdx.x; p < nprobe; p += blockDim.x) {
+        double r, ux, uy, uz;
+        macro_f64(*kp, A, N, nodes[p], r, ux, uy, uz);
+        if (row < cap) out[(int64_t)row * nprobe + p] = r;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *slot = row + 1;
 }
 
 hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, int64_t N, const int64_t* nodes, int nprobe,
-                              double* out, hipStream_t s) {
+                              double* out, int32_t* slot, int32_t cap, hipStream_t s) {
     if (nprobe <= 0) return hipSuccess;
-    dim3 grid((nprobe + 63) / 64), block(64);
+    dim3 grid(1), block(nprobe >= 1024 ? 1024 : ((nprobe + 63) / 64) * 64);
     if (prec == ACP_PREC_F32_SHIFTED)
-        hipLaunchKernelGGL(probe_shift<float>, grid, block, 0, s, k, (const float*)A, N, nodes, nprobe, out);
+        hipLaunchKernelGGL(probe_shift<float>, grid, block, 0, s, k, (const float*)A, N, nodes, nprobe, out, slot,
+                           cap);
     else
-        hipLaunchKernelGGL(probe_shift<_Float16>, grid, block, 0, s, k, (const _Float16*)A, N, nodes, nprobe, out);
+        hipLaunchKernelGGL(probe_shift<_Float16>, grid, block, 0, s, k, (const _Float16*)A, N, nodes, nprobe, out,
+                           slot, cap);
     return hipGetLastError();
 }
 

@@ -53,8 +53,13 @@ hipError_t launch_init_random_shift(int prec, int order, const KShift* k, void* 
                                     uint64_t seed, double rho_amp, double u_amp, hipStream_t s);
 // density (SI, float64) at probe nodes and per-block mass partial sums (float64)
 hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, int64_t N, const int64_t* nodes, int nprobe,
-                              double* out, hipStream_t s);
+                              double* out, int32_t* slot, int32_t cap, hipStream_t s);
 hipError_t launch_mass_shift(int prec, const KShift* k, const void* A, int64_t N, double* partial, int nblocks,
                              hipStream_t s);
+
+// Equilibrium state from a density band rho_rows[(j - j0) * nx + i] (u = 0)
+// into both arrays of any tier (k64_aux.hip, float64 arithmetic).
+hipError_t launch_init_rows(int prec, int order, const K64* k, const KShift* ks, const double* rho_rows, int j0,
+                            int rows, const Grid& g, void* A, void* B, hipStream_t s);
 
 }  // namespace acp

@@ -1,0 +1,180 @@
+"""Serving loop: many acoustic-pulse cases resident on one GPU (acp_server_*).
+
+The reference runs one case per process (main.cpp:37-271).  Here a native
+server (csrc/server.cpp) keeps cases resident, admits them by HBM footprint
+and steps them concurrently on separate streams in captured-graph chunks.
+This module builds request mixes, drives a server and reports cases/hour and
+latency percentiles (arrival to result).
+
+Request mix (one draw per request, in this order, from numpy's PCG64 with the
+given seed): inter-arrival time (exponential, unit mean, scaled by 1/rate),
+nx and ny uniform in [128, 1024], a refinement flag with probability 1/2 and a
+coupling scheme (overlap or direct), the collision operator uniform over
+BGK/MRT-GS/MRT-RM/RR/HRR, the pulse amplitude log-uniform in [1e-4, 1e-2], the
+half-width uniform in [4, 20] nodes, the step count uniform in [1000, 8000]
+(times `step_scale`), and the tier (float64 or float32-shifted, 1/2 each).
+Grids are nx x ny x 3 with LODI on all faces, tau = 0.5 + 3e-4, N_eq = 3; the
+pulse is centred and the probe sits at (3/4 nx, ny/2, 1).  This engine has one
+grid level: a request that asks for refinement runs on its coarse grid alone,
+and the flag is only counted.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _capi
+from ._capi import INIT, Request, Result, ServerDesc, ServerStats, check, lib
+
+OPS = ["SRT", "MRT-GS", "MRT-RM", "RR", "HRR"]
+TAU = 0.5 + 3e-4
+
+
+def make_requests(n=256, seed=1, rate=None, step_scale=1.0, nxy=(128, 1024), steps=(1000, 8000), precision=None):
+    """The serving workload as a list of dicts (arrival times in seconds when
+    `rate` is given in cases/s, else all zero)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    t = 0.0
+    for i in range(n):
+        gap = rng.exponential(1.0)
+        nx = int(rng.integers(nxy[0], nxy[1] + 1))
+        ny = int(rng.integers(nxy[0], nxy[1] + 1))
+        refined = bool(rng.random() < 0.5)
+        scheme = "overlap" if rng.random() < 0.5 else "direct"
+        op = OPS[int(rng.integers(0, 5))]
+        amp = float(10.0 ** rng.uniform(-4.0, -2.0))
+        hw = float(rng.uniform(4.0, 20.0))
+        nsteps = int(rng.integers(steps[0], steps[1] + 1))
+        prec = "F64" if rng.random() < 0.5 else "F32-SHIFTED"
+        if rate:
+            t += gap / rate
+        nsteps = max(2, int(round(nsteps * step_scale)))
+        out.append({"tag": i, "arrival": t if rate else 0.0, "nx": nx, "ny": ny, "nz": 3, "op": op, "amp": amp,
+                    "halfwidth": hw, "steps": nsteps, "precision": precision or prec, "refined": refined,
+                    "scheme": scheme})
+    return out
+
+
+def request_desc(r):
+    from . import make_desc
+    return make_desc(r["nx"], r["ny"], r["nz"], op=r["op"], order=3, boundary="LODI", mrt_mode="RELAXED",
+                     precision=r["precision"], tau=TAU, sigma=0.98)
+
+
+def probe_point(r):
+    return (3 * r["nx"] // 4, r["ny"] // 2, r["nz"] // 2)
+
+
+class Server:
+    """acp_server_* handle: submit requests, poll results, read statistics."""
+
+    def __init__(self, hbm_budget_bytes, max_active=8, graph_chunk=64, launch_steps=16, chunks_ahead=4,
+                 admit_threads=4):
+        L = lib()
+        d = ServerDesc(int(hbm_budget_bytes), int(max_active), int(graph_chunk), int(launch_steps),
+                       int(chunks_ahead), int(admit_threads))
+        h = ctypes.c_void_p()
+        check(L.acp_server_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        self._probe_bufs = {}
+
+    def submit(self, r, arrival=None):
+        q = Request()
+        q.desc = request_desc(r)
+        q.steps = int(r["steps"])
+        q.init_kind = INIT["PULSE"]
+        q.amp, q.x0, q.y0, q.halfwidth = r["amp"], r["nx"] / 2.0, r["ny"] / 2.0, r["halfwidth"]
+        pi, pj, pk = probe_point(r)
+        q.probe[0], q.probe[1], q.probe[2] = pi, pj, pk
+        buf = np.zeros(max(q.steps, 1))
+        self._probe_bufs[int(r["tag"])] = buf
+        q.probe_out = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        q.arrival = float(r["arrival"] if arrival is None else arrival)
+        q.tag = int(r["tag"])
+        check(lib().acp_server_submit(self.h, ctypes.byref(q)))
+
+    def poll(self, max_results=64, wait_ms=0):
+        out = (Result * max_results)()
+        n = ctypes.c_int32()
+        check(lib().acp_server_poll(self.h, out, max_results, int(wait_ms), ctypes.byref(n)))
+        res = []
+        for k in range(n.value):
+            x = out[k]
+            d = {f: getattr(x, f) for f, _ in Result._fields_}
+            d["error"] = x.error.decode(errors="replace")
+            d["probe"] = self._probe_bufs.pop(int(x.tag), None)
+            res.append(d)
+        return res
+
+    def stats(self):
+        st = ServerStats()
+        check(lib().acp_server_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def clock(self):
+        return lib().acp_server_clock(self.h)
+
+    def close(self):
+        if self.h:
+            lib().acp_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def run_mix(reqs, hbm_budget_bytes, max_active=8, graph_chunk=64, launch_steps=16, chunks_ahead=4,
+            admit_threads=4, timeout_s=3600.0):
+    """Offer `reqs` (arrival times relative to the start) to a fresh server and
+    collect every result.  Returns (results sorted by tag, summary dict)."""
+    with Server(hbm_budget_bytes, max_active, graph_chunk, launch_steps, chunks_ahead, admit_threads) as srv:
+        for r in reqs:
+            srv.submit(r)
+        results = []
+        t_end = time.monotonic() + timeout_s
+        while len(results) < len(reqs):
+            results += srv.poll(64, wait_ms=200)
+            if time.monotonic() > t_end:
+                raise TimeoutError(f"{len(results)} of {len(reqs)} cases finished within {timeout_s} s")
+        stats = srv.stats()
+    results.sort(key=lambda d: d["tag"])
+    return results, summarize(results, stats)
+
+
+def summarize(results, stats=None):
+    ok = [r for r in results if r["status"] == 0]
+    lat = np.array([r["t_done"] - r["t_arrival"] for r in ok]) if ok else np.zeros(1)
+    wait = np.array([r["t_admit"] - r["t_arrival"] for r in ok]) if ok else np.zeros(1)
+    t0 = min(r["t_arrival"] for r in results) if results else 0.0
+    t1 = max(r["t_done"] for r in results) if results else 0.0
+    span = max(t1 - t0, 1e-9)
+    node_steps = sum(r["steps"] * (r["device_bytes"] > 0) for r in ok)
+    out = {
+        "cases": len(results), "failed": len(results) - len(ok), "makespan_s": span,
+        "cases_per_hour": len(ok) / span * 3600.0,
+        "latency_p50_s": float(np.percentile(lat, 50)), "latency_p99_s": float(np.percentile(lat, 99)),
+        "latency_mean_s": float(lat.mean()), "queue_wait_p50_s": float(np.percentile(wait, 50)),
+        "queue_wait_p99_s": float(np.percentile(wait, 99)), "steps_total": int(node_steps),
+    }
+    if stats:
+        out.update({"peak_active": stats["peak_active"], "peak_hbm_bytes": stats["peak_bytes"],
+                    "budget_bytes": stats["budget_bytes"]})
+    return out
+
+
+def run_alone(r, graph_chunk=0):
+    """One request on its own case (no server): final checksum, mass and probe series."""
+    from . import Case
+    with Case(request_desc(r)) as c:
+        c.init_pulse(r["amp"], r["nx"] / 2.0, r["ny"] / 2.0, r["halfwidth"])
+        c.set_probes([probe_point(r)], capacity=max(r["steps"], 1))
+        if graph_chunk:
+            c.enable_graphs(graph_chunk)
+        c.step(r["steps"])
+        c.sync()
+        probe = c.read_probes()[:, 0].copy()
+        return {"checksum": c.checksum(), "mass": c.mass(), "probe": probe}

@@ -106,6 +106,11 @@ void acp_case_desc_default(acp_case_desc* out);
  * uniform-level case.  `stream` is a hipStream_t (NULL: the library creates one). */
 int acp_case_create(const acp_case_desc* desc, void* stream, acp_case** out);
 void acp_case_destroy(acp_case* c);
+/* The same inside a caller-owned device region (256-byte aligned, at least
+ * acp_case_footprint bytes): the case allocates nothing and frees nothing, so
+ * creating and destroying it never waits on other streams.  No counterpart in
+ * the reference; for servers that keep many cases resident. */
+int acp_case_create_in(const acp_case_desc* desc, void* stream, void* region, int64_t region_bytes, acp_case** out);
 int acp_case_info_get(const acp_case* c, acp_case_info* out);
 
 /* Lattice::initialization(Control&) (Lattice.cpp:1397-1493): the reference's
@@ -126,6 +131,24 @@ int acp_case_set_populations(acp_case* c, const double* f);
 int acp_case_step(acp_case* c, int32_t n);
 /* Block until the case stream is idle. */
 int acp_case_sync(acp_case* c);
+
+/* No counterpart in the reference (the reference launches nothing): run every
+ * full chunk of `chunk` steps inside acp_case_step as one replay of a captured
+ * HIP graph instead of one launch sequence per step.  `chunk` is even (the
+ * arrays end where they started) or 0 to switch graphs off.  Results are
+ * identical to plain stepping; probes keep recording every step. */
+int acp_case_enable_graphs(acp_case* c, int32_t chunk);
+
+/* No counterpart in the reference: device bytes a case of `desc` holds, with
+ * `nprobe` probes of `probe_capacity` steps, for admission by HBM footprint.
+ * Every buffer counts in multiples of 256 bytes.  Validates `desc` as
+ * acp_case_create does; allocates nothing. */
+int acp_case_footprint(const acp_case_desc* desc, int32_t nprobe, int32_t probe_capacity, int64_t* bytes);
+
+/* No counterpart in the reference: order-independent 64-bit fingerprint of the
+ * current population array in its storage format (bit patterns mixed with the
+ * element index and summed modulo 2^64).  Equal states give equal values. */
+int acp_case_checksum(acp_case* c, uint64_t* out);
 
 /* Lattice::getDistribution(bool&, ...) (Lattice.h:119) for the whole current
  * array: host float64 [19][N]. */
@@ -151,10 +174,81 @@ int acp_write_vtk_host(const acp_case_desc* desc, const double* fcur, const doub
                        const char* stem);
 
 /* Introspection, no counterpart in the reference: the boundary nodes of an
- * nx*ny*nz LODI case in the order the engine processes them (phase by phase;
- * within a phase in the reference's pass order, Lattice.cpp:1582-1624) and the
- * phase of each.  Returns the node count; writes at most `cap` entries. */
+ * nx*ny*nz LODI case in the order the engine processes them (phase by phase,
+ * phases derived from the reference's pass order, Lattice.cpp:1582-1624;
+ * within a phase by ascending node index) and the phase of each.  Returns the node count; writes at most `cap` entries. */
 int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t* nodes, int32_t* phases, int64_t cap);
+
+/* ------------------------------------------------------------------ serving
+ * No counterpart in the reference, which runs one case per process
+ * (main.cpp:37-271).  A server keeps many independent cases resident on one
+ * GPU and steps them concurrently:
+ *   - requests are admitted in arrival order once they have arrived, one of
+ *     `max_active` stream slots is free and their footprint
+ *     (acp_case_footprint) fits a free range of one device arena of
+ *     `hbm_budget_bytes` allocated when the server starts;
+ *   - each resident case runs on its own stream in chunks of `graph_chunk`
+ *     steps replayed from captured HIP graphs (or `launch_steps` plain steps
+ *     when graph_chunk is 0), with at most `chunks_ahead` chunks queued;
+ *   - a finished case posts an acp_result and gives back its range and slot.
+ * Results are those of the same case run alone: cases share no state. */
+enum { ACP_INIT_REFERENCE = 0, ACP_INIT_PULSE = 1, ACP_INIT_RANDOM = 2 };
+
+typedef struct acp_server_desc {
+    int64_t hbm_budget_bytes;   /* device arena for resident cases */
+    int32_t max_active;         /* resident cases (one stream each) */
+    int32_t graph_chunk;        /* steps per captured graph replay (even), 0: plain launches */
+    int32_t launch_steps;       /* steps per acp_case_step call when graph_chunk is 0 */
+    int32_t chunks_ahead;       /* chunks queued per case before the driver waits on it */
+    int32_t admit_threads;      /* host threads creating and initialising admitted cases (0: 4) */
+} acp_server_desc;
+
+typedef struct acp_request {
+    acp_case_desc desc;
+    int32_t steps;
+    int32_t init_kind;          /* ACP_INIT_*: acp_case_init_reference / _pulse / _random */
+    double amp, x0, y0, halfwidth;      /* pulse parameters (acp_case_init_pulse) */
+    uint64_t seed;
+    double rho_amp, u_amp;              /* random field (acp_case_init_random) */
+    int32_t probe[3];           /* probe node (i, j, k); probe[0] < 0: none */
+    double* probe_out;          /* host buffer for `steps` densities, or NULL; must stay valid */
+    double arrival;             /* seconds on the server clock; not admitted earlier */
+    uint64_t tag;               /* caller's identifier, returned in the result */
+} acp_request;
+
+typedef struct acp_result {
+    uint64_t tag;
+    int32_t status;             /* ACP_OK or the error of the failing call */
+    int32_t steps;              /* steps launched */
+    int64_t device_bytes;       /* footprint the case held */
+    double t_arrival, t_submit, t_admit, t_start, t_done;  /* server clock, seconds */
+    double mass;                /* acp_case_mass of the final state */
+    uint64_t checksum;          /* acp_case_checksum of the final state */
+    char error[128];
+} acp_result;
+
+typedef struct acp_server_stats {
+    int64_t submitted, completed, queued;
+    int32_t active, peak_active;
+    int64_t bytes_in_use, peak_bytes, budget_bytes;
+    double clock;
+} acp_server_stats;
+
+typedef struct acp_server acp_server;
+
+/* Start a server on the current device (allocates the arena and streams and
+ * starts its admission and driver threads). */
+int acp_server_create(const acp_server_desc* desc, acp_server** out);
+/* Queue a request (copied); thread-safe.  Fails at once when the case is
+ * invalid or larger than the whole budget. */
+int acp_server_submit(acp_server* s, const acp_request* r);
+/* Move up to `max` finished results to `out`, waiting up to `wait_ms` for one. */
+int acp_server_poll(acp_server* s, acp_result* out, int32_t max, int32_t wait_ms, int32_t* n);
+int acp_server_get_stats(acp_server* s, acp_server_stats* out);
+/* Seconds since acp_server_create (the clock of arrival and result times). */
+double acp_server_clock(acp_server* s);
+/* Finish every submitted request, then stop the threads and free the arena. */
+void acp_server_destroy(acp_server* s);
 
 /* Measurement helper, no counterpart in the reference: the device-copy
  * ceiling for the engine's memory layout.  Copies `planes` (1 or 19) float64

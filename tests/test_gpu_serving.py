@@ -1,0 +1,111 @@
+"""Captured-graph stepping and the serving loop on the GPU.
+
+Every comparison here is exact: a case gives bit-identical populations (via
+the order-independent checksum and, for small cases, the full arrays), probe
+series and mass whether it is stepped with plain launches, with graph replays,
+alone, or inside the server among other cases.
+"""
+import numpy as np
+import pytest
+
+import acpulse_vclbm_amd as acp
+from acpulse_vclbm_amd import _capi, serve
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("spec", [
+    dict(op="HRR", precision="F64", transverse=True, cubic=True),
+    dict(op="SRT", precision="F32-SHIFTED"),
+    dict(op="MRT-RM", precision="F16-SHIFTED", boundary="PERIODIC", mrt_mode="RELAXED"),
+], ids=["hrr-f64-cubic", "srt-f32", "mrtrm-f16-periodic"])
+def test_footprint_is_what_a_case_allocates(gpu, spec):
+    d = acp.make_desc(40, 36, 5, tau=0.6, **spec)
+    with acp.Case(d) as c:
+        assert c.info().device_bytes == acp.footprint(d)
+        c.set_probes([(3, 4, 2), (20, 18, 2)], capacity=77)
+        assert c.info().device_bytes == acp.footprint(d, 2, 77)
+
+
+@pytest.mark.parametrize("spec", [
+    dict(op="HRR", precision="F64", transverse=True),
+    dict(op="RR", precision="F64", cubic=True),
+    dict(op="SRT", precision="F32-SHIFTED"),
+    dict(op="MRT-GS", precision="F64", boundary="PERIODIC", mrt_mode="RELAXED"),
+], ids=["hrr-f64", "rr-cubic-f64", "srt-f32", "mrtgs-periodic"])
+def test_graph_replays_match_plain_steps(gpu, spec):
+    """Chunk of 4 steps: every replay (both ping-pong parities, odd remainders in
+    between, eager work and synchronisation between replays) must leave the case
+    in exactly the state plain stepping gives."""
+    d = acp.make_desc(30, 26, 3, tau=0.53, **spec)
+    with acp.Case(d) as a, acp.Case(d) as b:
+        for c in (a, b):
+            c.init_pulse(1e-3, 15.0, 13.0, 4.0)
+            c.set_probes([(22, 13, 1), (5, 5, 1)], capacity=200)
+        b.enable_graphs(4)
+        plan = [4, 4, 1, 4, 8, 3, 4, 12, 1, 6]
+        for n in plan:
+            a.step(n)
+            b.step(n)
+            a.sync()
+            b.sync()
+            assert a.checksum() == b.checksum()
+            assert a.mass() == b.mass()
+        assert np.array_equal(a.populations(), b.populations())
+        pa, pb = a.read_probes(), b.read_probes()
+        assert pa.shape == (sum(plan), 2)
+        assert np.array_equal(pa, pb)
+
+
+def test_graph_chunk_argument_checks(gpu):
+    d = acp.make_desc(10, 10, 3, tau=0.6)
+    with acp.Case(d) as c:
+        for bad in (-2, 1, 3):
+            with pytest.raises(_capi.AcpError):
+                c.enable_graphs(bad)
+        c.enable_graphs(0)
+
+
+def _small_mix(n, seed):
+    reqs = serve.make_requests(n, seed=seed, nxy=(24, 72), steps=(20, 90))
+    for i, r in enumerate(reqs):
+        r["arrival"] = 0.002 * i
+    return reqs
+
+
+def test_server_results_equal_alone(gpu):
+    reqs = _small_mix(12, seed=5)
+    alone = [serve.run_alone(r) for r in reqs]
+    results, summary = serve.run_mix(reqs, hbm_budget_bytes=1 << 30, max_active=4, graph_chunk=8, chunks_ahead=2,
+                                     timeout_s=120)
+    assert summary["failed"] == 0, [(r["tag"], r["status"], r["error"]) for r in results if r["status"]]
+    assert len(results) == len(reqs)
+    for r, a, q in zip(results, alone, reqs):
+        assert r["status"] == 0, r["error"]
+        assert r["steps"] == q["steps"]
+        assert r["checksum"] == a["checksum"], q
+        assert r["mass"] == a["mass"], q
+        assert np.array_equal(r["probe"], a["probe"]), q
+        assert r["t_arrival"] <= r["t_admit"] + 1e-9 <= r["t_done"] + 2e-9
+    assert summary["peak_active"] <= 4
+    assert summary["latency_p99_s"] >= summary["latency_p50_s"] > 0
+
+
+def test_server_admits_by_footprint(gpu):
+    reqs = _small_mix(8, seed=9)
+    fps = [acp.footprint(serve.request_desc(r), 1, r["steps"]) for r in reqs]
+    budget = (2 * max(fps) + 255) // 256 * 256  # room for at most two of the largest at once
+    alone = [serve.run_alone(r) for r in reqs]
+    results, summary = serve.run_mix(reqs, hbm_budget_bytes=budget, max_active=8, graph_chunk=0, launch_steps=5,
+                                     chunks_ahead=3, timeout_s=120)
+    assert summary["failed"] == 0, [(r["tag"], r["status"], r["error"]) for r in results if r["status"]]
+    assert summary["peak_hbm_bytes"] <= budget
+    for r, a in zip(results, alone):
+        assert r["checksum"] == a["checksum"]
+        assert np.array_equal(r["probe"], a["probe"])
+    # a case larger than the whole budget is refused at submission
+    with serve.Server(budget, max_active=2) as srv:
+        big = dict(reqs[0], nx=400, ny=400, tag=99)
+        with pytest.raises(_capi.AcpError) as e:
+            srv.submit(big)
+        assert e.value.code == _capi.ACP_E_OOM

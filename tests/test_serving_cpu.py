@@ -1,0 +1,82 @@
+"""Serving loop, host side: request mix, footprint accounting (no GPU needed)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import acpulse_vclbm_amd as acp
+from acpulse_vclbm_amd import _capi, serve
+
+
+def test_request_mix_is_reproducible_and_in_range():
+    a = serve.make_requests(256, seed=1, rate=2.0)
+    b = serve.make_requests(256, seed=1, rate=2.0)
+    assert a == b
+    assert serve.make_requests(256, seed=2, rate=2.0) != a
+    nx = np.array([r["nx"] for r in a])
+    st = np.array([r["steps"] for r in a])
+    amp = np.array([r["amp"] for r in a])
+    hw = np.array([r["halfwidth"] for r in a])
+    arr = np.array([r["arrival"] for r in a])
+    assert nx.min() >= 128 and nx.max() <= 1024
+    assert st.min() >= 1000 and st.max() <= 8000
+    assert amp.min() >= 1e-4 and amp.max() <= 1e-2
+    assert hw.min() >= 4 and hw.max() <= 20
+    assert np.all(np.diff(arr) >= 0)
+    # Poisson arrivals at rate 2/s: mean gap 0.5 s (256 samples: within 20 %)
+    assert 0.4 < np.diff(arr).mean() < 0.6
+    assert {r["op"] for r in a} == set(serve.OPS)
+    assert {r["precision"] for r in a} == {"F64", "F32-SHIFTED"}
+    f = np.mean([r["refined"] for r in a])
+    assert 0.35 < f < 0.65
+
+
+def test_rate_only_scales_arrivals():
+    a = serve.make_requests(50, seed=1, rate=1.0)
+    b = serve.make_requests(50, seed=1, rate=4.0)
+    for x, y in zip(a, b):
+        assert x["nx"] == y["nx"] and x["op"] == y["op"] and x["steps"] == y["steps"]
+        assert y["arrival"] == pytest.approx(x["arrival"] / 4.0)
+
+
+def _fp(d, nprobe=0, cap=0):
+    return acp.footprint(d, nprobe, cap)
+
+
+@pytest.mark.parametrize("n", [(3, 3, 3), (7, 5, 3), (40, 36, 3), (17, 9, 12)])
+def test_footprint_lodi_term_is_the_schedule(n):
+    nx, ny, nz = n
+    d_lodi = acp.make_desc(nx, ny, nz, op="SRT", tau=0.6)
+    d_per = acp.make_desc(nx, ny, nz, op="SRT", tau=0.6, boundary="PERIODIC")
+    count = len(acp.lodi_schedule(nx, ny, nz)[0])
+    assert count == nx * ny * nz - (nx - 2) * (ny - 2) * (nz - 2)
+    diff = _fp(d_lodi) - _fp(d_per)
+    assert diff == (count * 64 + 255) // 256 * 256
+
+
+def test_footprint_tiers_and_probes():
+    n = (64, 48, 3)
+    f64 = _fp(acp.make_desc(*n, op="SRT", tau=0.6, boundary="PERIODIC"))
+    f32 = _fp(acp.make_desc(*n, op="SRT", tau=0.6, boundary="PERIODIC", precision="F32-SHIFTED"))
+    N = n[0] * n[1] * n[2]
+    r = lambda b: (b + 255) // 256 * 256
+    # float64: constants + 2 arrays of 19 float64 planes; float32-shifted: both constant
+    # blocks (one 256-byte unit each) + 2 arrays of 19 float32 planes; the staging and
+    # reduction buffers are common to both tiers
+    assert f64 - f32 == 2 * r(19 * N * 8) - 2 * r(19 * N * 4) - 256
+    base = acp.make_desc(*n, op="HRR", tau=0.6)
+    assert _fp(base, 1, 100) - _fp(base) == 256 + r(800) + 256
+    assert _fp(base, 3, 1000) - _fp(base) == r(24) + r(24000) + 256
+    cub = acp.make_desc(*n, op="HRR", tau=0.6, cubic=True)
+    assert _fp(cub) - _fp(base) == r(19 * N * 8)
+
+
+def test_footprint_rejects_what_create_rejects():
+    d = acp.make_desc(16, 16, 3, op="SRT", tau=0.6)
+    d.max_level = 1
+    with pytest.raises(_capi.AcpError) as e:
+        acp.footprint(d)
+    assert e.value.code == _capi.ACP_E_UNSUPPORTED
+    d = acp.make_desc(16, 16, 3, op="SRT", tau=0.6, precision="F32-SHIFTED", cubic=True)
+    with pytest.raises(_capi.AcpError):
+        acp.footprint(d)
