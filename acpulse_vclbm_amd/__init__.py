@@ -96,6 +96,13 @@ class Case:
     def init_pulse(self, amp, x0, y0, b):
         check(lib().acp_case_init_pulse(self.h, amp, x0, y0, b))
 
+    def init_density(self, rho):
+        """Equilibrium state feq_N(rho, 0) from a density field of N values (SI)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        if rho.size != self.info().nodes:
+            raise ValueError("density field needs one value per node")
+        check(lib().acp_case_init_density(self.h, dptr(rho)))
+
     def init_random(self, seed, rho_amp, u_amp):
         check(lib().acp_case_init_random(self.h, seed, rho_amp, u_amp))
 

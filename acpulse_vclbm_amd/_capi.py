@@ -111,6 +111,7 @@ def lib():
         "acp_case_init_pulse": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl]),
         "acp_case_init_random": (ctypes.c_int, [vp, ctypes.c_uint64, dbl, dbl]),
         "acp_case_set_populations": (ctypes.c_int, [vp, dp]),
+        "acp_case_init_density": (ctypes.c_int, [vp, dp]),
         "acp_case_step": (ctypes.c_int, [vp, i32]),
         "acp_case_sync": (ctypes.c_int, [vp]),
         "acp_case_get_populations": (ctypes.c_int, [vp, dp]),

@@ -1,0 +1,104 @@
+"""Precision-tier ablation: float32- and float16-shifted storage against float64.
+
+The float64 tier reproduces the reference solver bit for bit, so it is the
+reference of every error below.  HRR collision (sigma 0.98), N_eq = 3,
+tau = 0.5 + 3e-4, SI parameters of the reference's case file (rho0 1.17621,
+c_s 347.3 m/s, spacing 0.01 m), float16 storage scale 1e3.
+
+Part A, field and probe (the 200 x 200 x 3 periodic pulse case): Gaussian
+density pulse rho0 (1 + A exp(-ln2 r^2 / 10^2)) centred at (100, 100), u = 0,
+A in {1e-4, 1e-3, 1e-2}, 500 steps, density probe at (150, 100, 1) recorded
+every step (float64 reduction on the device in every tier):
+  field_max, field_rms  max / RMS over the grid of |rho - rho_f64| / (A rho0)
+                        after the last step;
+  probe_l2              ||p - p_f64||_2 / ||p_f64 - rho0||_2 over the series;
+  probe_peak            |max (p - rho0) - max (p_f64 - rho0)| / max (p_f64 - rho0).
+
+Part B, reflection of the LODI boundary with transverse terms (K1 = 1): a
+spherical pulse (half-width 4 nodes) at the centre of a 96^3 box, probe 24
+nodes towards +x, 200 steps.  The same pulse in a 192^3 box, whose faces are
+too far away for a reflection to reach the probe within 200 steps, gives the
+reflection-free signal p_free of the same tier;
+R = max_t |p(t) - p_free(t)| / max_t |p_free(t) - rho0| and
+R_rel = |R - R_f64| / R_f64.
+
+Why the two parts: on a quasi-2D grid (nz = 3) with LODI on the z faces, two
+of the three planes are boundary nodes whose characteristic update has no z
+gradient to act on, so they hold the initial density and an acoustic pulse
+does not propagate; the reference itself runs three-dimensional boxes.
+"""
+import numpy as np
+
+TIERS = ("F64", "F32-SHIFTED", "F16-SHIFTED")
+AMPS = (1e-4, 1e-3, 1e-2)
+TAU = 0.5 + 3e-4
+
+
+def _desc(acp, nx, ny, nz, precision, boundary, transverse=False):
+    return acp.make_desc(nx, ny, nz, op="HRR", order=3, transverse=transverse, boundary=boundary,
+                         precision=precision, tau=TAU, sigma=0.98, f16_scale=1000.0)
+
+
+def pulse_2d(acp, amp, precision, steps=500):
+    """(probe series, final density, rho0) of the 200 x 200 x 3 periodic pulse."""
+    d = _desc(acp, 200, 200, 3, precision, "PERIODIC")
+    with acp.Case(d) as c:
+        c.init_pulse(amp, 100.0, 100.0, 10.0)
+        c.set_probes([(150, 100, 1)], capacity=steps)
+        c.step(steps)
+        c.sync()
+        probe = c.read_probes()[:, 0].copy()
+        rho, _ = c.macros()
+        return probe, rho, d.rho0
+
+
+def spherical_probe(acp, n, amp, precision, steps=200):
+    """Probe series 24 nodes from the centre of an n^3 LODI box (transverse terms)."""
+    d = _desc(acp, n, n, n, precision, "LODI", transverse=True)
+    c0 = n // 2
+    rho0 = d.rho0
+    with acp.Case(d) as c:
+        # spherical pulse: rho0 (1 + A exp(-ln2 r^2 / 4^2)), r measured in 3D
+        x = np.arange(n) - c0
+        r2 = x[None, None, :] ** 2 + x[None, :, None] ** 2 + x[:, None, None] ** 2
+        rho = rho0 * (1.0 + amp * np.exp(-np.log(2.0) * r2 / 16.0)).reshape(-1)
+        c.init_density(rho)
+        c.set_probes([(c0 + 24, c0, c0)], capacity=steps)
+        c.step(steps)
+        c.sync()
+        return c.read_probes()[:, 0].copy(), rho0
+
+
+def reflection(p, p_free, rho0):
+    return float(np.max(np.abs(p - p_free)) / np.max(np.abs(p_free - rho0)))
+
+
+def run(acp, amps=AMPS, tiers=TIERS):
+    out = {"field_probe": {}, "reflection": {}}
+    for amp in amps:
+        res = {t: pulse_2d(acp, amp, t) for t in tiers}
+        p64, rho64, rho0 = res["F64"]
+        sig = p64 - rho0
+        row = {}
+        for t in tiers:
+            if t == "F64":
+                continue
+            p, rho, _ = res[t]
+            row[t] = {
+                "field_max": float(np.max(np.abs(rho - rho64)) / (amp * rho0)),
+                "field_rms": float(np.sqrt(np.mean((rho - rho64) ** 2)) / (amp * rho0)),
+                "probe_l2": float(np.linalg.norm(p - p64) / np.linalg.norm(sig)),
+                "probe_peak": float(abs(np.max(p - rho0) - np.max(sig)) / np.max(sig)),
+            }
+        out["field_probe"][f"{amp:g}"] = row
+    amp = 1e-3
+    rr = {}
+    for t in tiers:
+        p, rho0 = spherical_probe(acp, 96, amp, t)
+        pf, _ = spherical_probe(acp, 192, amp, t)
+        rr[t] = reflection(p, pf, rho0)
+    out["reflection"] = {"amplitude": amp, "R_f64": rr["F64"]}
+    for t in tiers:
+        if t != "F64":
+            out["reflection"][t] = {"R": rr[t], "R_rel": abs(rr[t] - rr["F64"]) / rr["F64"]}
+    return out

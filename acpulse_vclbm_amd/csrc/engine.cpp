@@ -646,6 +646,23 @@ extern "C" int acp_case_init_pulse(acp_case* c, double amp, double x0, double y0
     });
 }
 
+extern "C" int acp_case_init_density(acp_case* c, const double* rho) {
+    if (int rc = check_case(c)) return rc;
+    if (!rho) return fail(ACP_E_ARG, "null density field");
+    void* A = shifted(c) ? c->SA : (void*)c->A;
+    void* B = shifted(c) ? c->SB : (void*)c->B;
+    for (int64_t l0 = 0; l0 < c->g.N; l0 += c->stage_elems) {
+        const int64_t n = std::min(c->stage_elems, c->g.N - l0);
+        // the upload waits on the stream, so the previous range's kernel is done with the stage
+        HIPCHK(upload(c, c->stage, rho + l0, sizeof(double) * n));
+        HIPCHK(acp::launch_init_nodes(c->precision, c->d.order, c->dk, c->dks, c->stage, l0, n, c->g, A, B,
+                                      c->stream));
+    }
+    if (c->psi) HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(double) * 19 * c->g.N, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return restart(c);
+}
+
 extern "C" int acp_case_init_random(acp_case* c, uint64_t seed, double rho_amp, double u_amp) {
     if (int rc = check_case(c)) return rc;
     hipError_t e;

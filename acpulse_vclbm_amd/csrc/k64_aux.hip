@@ -264,6 +264,43 @@ __global__ __launch_bounds__(256) void init_rows(const K64* __restrict__ kp, con
     });
 }
 
+// The same for a contiguous node range [l0, l0 + n) with one density per node.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(256) void init_nodes(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
+                                                  const double* __restrict__ rho, int64_t l0, int64_t n,
+                                                  T* __restrict__ A, T* __restrict__ B, int64_t N) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t l = l0 + t;
+    double eq[19];
+    feq64<ORDER>(*kp, rho[t], 0., 0., 0., eq);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        T v;
+        if constexpr (sizeof(T) == 8) v = eq[q];
+        else v = (T)((double)ksp->s * (eq[q] / ksp->rho0 - weight(q)));
+        A[q * N + l] = v;
+        B[q * N + l] = v;
+    });
+}
+
+hipError_t launch_init_nodes(int prec, int order, const K64* k, const KShift* ks, const double* rho, int64_t l0,
+                             int64_t n, const Grid& g, void* A, void* B, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+#define IN(T, O) hipLaunchKernelGGL((init_nodes<T, O>), grid, block, 0, s, k, ks, rho, l0, n, (T*)A, (T*)B, g.N)
+#define INO(T)                     \
+    if (order == 2) IN(T, 2);      \
+    else if (order == 3) IN(T, 3); \
+    else IN(T, 4)
+    if (prec == ACP_PREC_F64) INO(double);
+    else if (prec == ACP_PREC_F32_SHIFTED) INO(float);
+    else INO(_Float16);
+#undef INO
+#undef IN
+    return hipGetLastError();
+}
+
 hipError_t launch_init_rows(int prec, int order, const K64* k, const KShift* ks, const double* rho_rows, int j0,
                             int rows, const Grid& g, void* A, void* B, hipStream_t s) {
     const int64_t n = (int64_t)rows * g.nx * g.nz;

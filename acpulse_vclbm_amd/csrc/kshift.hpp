@@ -61,5 +61,8 @@ hipError_t launch_mass_shift(int prec, const KShift* k, const void* A, int64_t N
 // into both arrays of any tier (k64_aux.hip, float64 arithmetic).
 hipError_t launch_init_rows(int prec, int order, const K64* k, const KShift* ks, const double* rho_rows, int j0,
                             int rows, const Grid& g, void* A, void* B, hipStream_t s);
+// The same from one density per node of the range [l0, l0 + n).
+hipError_t launch_init_nodes(int prec, int order, const K64* k, const KShift* ks, const double* rho, int64_t l0,
+                             int64_t n, const Grid& g, void* A, void* B, hipStream_t s);
 
 }  // namespace acp

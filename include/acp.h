@@ -119,6 +119,10 @@ int acp_case_init_reference(acp_case* c);
 /* Same initialization with a parameterised pulse in node units:
  * rho = rho0 (1 + amp exp(-ln2 ((i-x0)^2 + (j-y0)^2) / b^2)), u = 0. */
 int acp_case_init_pulse(acp_case* c, double amp, double x0, double y0, double b);
+/* Equilibrium state feq_N(rho, 0) from a host density field rho[N] (kg/m^3,
+ * node order i + nx*(j + ny*k)), in both arrays; no counterpart in the
+ * reference, whose initial field is hard-coded. */
+int acp_case_init_density(acp_case* c, const double* rho);
 /* Random fluctuation field on the device: rho = rho0 (1 + rho_amp n0),
  * u_a = u_amp c_s sqrt(3) n_a with n ~ N(0,1) from Philox-4x32-10 (key = seed,
  * counter = node index) and Box-Muller; populations feq_N(rho, u). */

@@ -1,0 +1,58 @@
+"""Density-field initial condition and the precision-tier ablation targets (GPU).
+
+Tolerances: the float32-shifted tier must keep the reflection coefficient of
+the LODI boundary within 1 % of float64 (the tier's stated target) and the
+probe series within 1e-4 (L2, relative to the signal); float16-shifted within
+1 % on the reflection coefficient and 5e-3 on the probe series.  The measured
+values are about 5e-6 / 4e-6 (float32) and 9e-4 / 5e-4 (float16), so the
+bounds leave a margin of 10x or more.
+"""
+import numpy as np
+import pytest
+
+import acpulse_vclbm_amd as acp
+import oracle_lib as ol
+from acpulse_vclbm_amd import ablation
+from make_golden import CS, DX, RHO0
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("order", [2, 3, 4])
+def test_init_density_is_the_oracle_equilibrium(gpu, order):
+    nx, ny, nz = 23, 17, 6
+    p = ol.make_params(nx, ny, nz, op="SRT", order=order, tau=0.6, cs=CS, spacing=DX, rho0=RHO0)
+    rng = np.random.default_rng(order)
+    rho = RHO0 * (1.0 + 1e-2 * rng.standard_normal(nx * ny * nz))
+    want = ol.equilibrium_field(p, rho, np.zeros((3, nx * ny * nz)))
+    d = acp.make_desc(nx, ny, nz, op="SRT", order=order, tau=0.6, cs=CS, spacing=DX, rho0=RHO0)
+    with acp.Case(d) as c:
+        c.init_density(rho)
+        assert np.array_equal(c.populations(), want)
+
+
+def test_init_density_in_shifted_tier_matches_float64(gpu):
+    nx, ny, nz = 20, 12, 5
+    rng = np.random.default_rng(7)
+    rho = RHO0 * (1.0 + 1e-3 * rng.standard_normal(nx * ny * nz))
+    out = {}
+    for prec in ("F64", "F32-SHIFTED"):
+        d = acp.make_desc(nx, ny, nz, op="SRT", order=3, tau=0.6, precision=prec)
+        with acp.Case(d) as c:
+            c.init_density(rho)
+            out[prec] = c.macros()[0]
+    assert np.max(np.abs(out["F32-SHIFTED"] - out["F64"])) / RHO0 < 1e-7
+
+
+def test_precision_ablation_targets(gpu):
+    r = ablation.run(acp, amps=(1e-3,))
+    fp = r["field_probe"]["0.001"]
+    assert fp["F32-SHIFTED"]["probe_l2"] < 1e-4
+    assert fp["F32-SHIFTED"]["field_max"] < 1e-4
+    assert fp["F16-SHIFTED"]["probe_l2"] < 5e-3
+    assert fp["F16-SHIFTED"]["field_max"] < 2e-3
+    rf = r["reflection"]
+    # the boundary reflects measurably, and far less than a rigid wall would
+    assert 1e-3 < rf["R_f64"] < 0.2
+    assert rf["F32-SHIFTED"]["R_rel"] < 0.01
+    assert rf["F16-SHIFTED"]["R_rel"] < 0.01
