@@ -14,7 +14,7 @@ import numpy as np
 from . import _capi
 from ._capi import (ACP_OK, AcpError, BOUNDARY, CaseDesc, CaseInfo, MRT_MODE, OP, PREC, check, dptr, lib)
 
-__all__ = ["Case", "Control", "Lattice", "CaseDesc", "AcpError", "make_desc", "lib", "lodi_schedule"]
+__all__ = ["Case", "Control", "Lattice", "Refined", "CaseDesc", "AcpError", "make_desc", "lib", "lodi_schedule"]
 
 
 def make_desc(nx, ny, nz, op="SRT", order=3, cubic=False, transverse=False, boundary="LODI",
@@ -158,6 +158,68 @@ class Case:
     def write_vtk(self, stem):
         """Lattice::writeResultsVTK (Lattice.cpp:2212-2243, 4094-4349): <stem>.vtk."""
         check(lib().acp_case_write_vtk(self.h, str(stem).encode()))
+
+
+class _LevelCase(Case):
+    """A level of a Refined case: the acp_case API on a handle the refined case owns."""
+
+    def __init__(self, h, desc):
+        self.desc = desc
+        self.h = h
+        self.nx, self.ny, self.nz = desc.nx, desc.ny, desc.nz
+        self.N = self.nx * self.ny * self.nz
+
+    def close(self):
+        self.h = ctypes.c_void_p()
+
+
+class Refined:
+    """Two-level case (acp_refined_*): periodic coarse level `desc`, fine level
+    over coarse x in [x0, x1] at half the spacing.  level(0) / level(1) give the
+    levels' cases for probes, read-back and output; step with Refined.step."""
+
+    def __init__(self, desc, x0, x1, stream=None):
+        self.h = ctypes.c_void_p()
+        st = ctypes.c_void_p(int(stream)) if stream else None
+        check(lib().acp_refined_create(ctypes.byref(desc), int(x0), int(x1), st, ctypes.byref(self.h)))
+        fd = CaseDesc.from_buffer_copy(desc)
+        fd.nx, fd.ny, fd.nz = 2 * (x1 - x0) + 1, 2 * desc.ny, 2 * desc.nz
+        fd.spacing = desc.spacing / 2.0
+        fd.boundary = BOUNDARY["OPEN-X"]
+        self.x0, self.x1 = int(x0), int(x1)
+        self._levels = []
+        for lv, d in ((0, desc), (1, fd)):
+            hc = ctypes.c_void_p()
+            check(lib().acp_refined_level(self.h, lv, ctypes.byref(hc)))
+            self._levels.append(_LevelCase(hc, d))
+
+    def level(self, i):
+        return self._levels[i]
+
+    def init_pulse(self, amp, x0, y0, b):
+        check(lib().acp_refined_init_pulse(self.h, amp, x0, y0, b))
+
+    def step(self, n=1):
+        check(lib().acp_refined_step(self.h, int(n)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            for lv in self._levels:
+                lv.close()
+            lib().acp_refined_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def lodi_schedule(nx, ny, nz):

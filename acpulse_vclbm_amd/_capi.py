@@ -17,7 +17,7 @@ HEADER_PATH = os.path.join(os.path.dirname(HERE), "include", "acp.h")
 ACP_OK, ACP_E_ARG, ACP_E_CONFIG, ACP_E_OOM, ACP_E_HIP, ACP_E_STATE, ACP_E_UNSUPPORTED, ACP_E_IO = range(8)
 OP = {"SRT": 0, "MRT-GS": 1, "MRT-RM": 2, "RR": 3, "HRR": 4}
 PREC = {"F64": 0, "F32-SHIFTED": 1, "F16-SHIFTED": 2}
-BOUNDARY = {"LODI": 0, "PERIODIC": 1}
+BOUNDARY = {"LODI": 0, "PERIODIC": 1, "OPEN-X": 2}
 MRT_MODE = {"REFERENCE": 0, "RELAXED": 1}
 
 
@@ -127,6 +127,11 @@ def lib():
         "acp_case_footprint": (ctypes.c_int, [D, i32, i32, ctypes.POINTER(i64)]),
         "acp_case_create_in": (ctypes.c_int, [D, vp, vp, i64, ctypes.POINTER(vp)]),
         "acp_case_checksum": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "acp_refined_create": (ctypes.c_int, [D, i32, i32, vp, ctypes.POINTER(vp)]),
+        "acp_refined_destroy": (None, [vp]),
+        "acp_refined_level": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
+        "acp_refined_init_pulse": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl]),
+        "acp_refined_step": (ctypes.c_int, [vp, i32]),
         "acp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(vp)]),
         "acp_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request)]),
         "acp_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result), i32, i32, ctypes.POINTER(i32)]),

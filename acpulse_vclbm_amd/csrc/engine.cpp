@@ -452,7 +452,10 @@ int validate_desc(const acp_case_desc& d) {
     if (d.order < 2 || d.order > 4) return fail(ACP_E_CONFIG, "Wrong order of equilibrium!");
     if (d.op < ACP_OP_SRT || d.op > ACP_OP_HRR) return fail(ACP_E_ARG, "unknown collision operator");
     if (d.precision < ACP_PREC_F64 || d.precision > ACP_PREC_F16_SHIFTED) return fail(ACP_E_ARG, "unknown precision");
-    if (d.boundary != ACP_BC_LODI && d.boundary != ACP_BC_PERIODIC) return fail(ACP_E_ARG, "unknown boundary");
+    if (d.boundary != ACP_BC_LODI && d.boundary != ACP_BC_PERIODIC && d.boundary != ACP_BC_OPEN_X)
+        return fail(ACP_E_ARG, "unknown boundary");
+    if (d.boundary == ACP_BC_OPEN_X && (d.op == ACP_OP_HRR || d.cubic))
+        return fail(ACP_E_UNSUPPORTED, "open x faces: HRR and the cubic-Mach correction need neighbours beyond them");
     if (d.precision != ACP_PREC_F64 && d.cubic)
         return fail(ACP_E_UNSUPPORTED, "the cubic-Mach correction is available in the float64 tier only");
     if (!(d.cs > 0) || !(d.spacing > 0) || !(d.nu > 0) || !(d.rho0 > 0))
@@ -479,7 +482,7 @@ int64_t footprint_bytes(const acp_case_desc& d, int32_t nprobe, int32_t cap) {
         if (d.op == ACP_OP_HRR) b += round_alloc(4 * N * 4);
     }
     b += round_alloc(std::min<int64_t>(N, kStageElems) * 8);
-    if (d.boundary != ACP_BC_PERIODIC)
+    if (d.boundary == ACP_BC_LODI)
         b += round_alloc((N - (int64_t)(d.nx - 2) * (d.ny - 2) * (d.nz - 2)) * (int64_t)sizeof(acp::LodiNode));
     b += round_alloc(std::min<int64_t>(1024, (N + 255) / 256) * 8);
     if (nprobe > 0)
@@ -523,9 +526,9 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
     c->arena = (char*)region;
     c->arena_size = region_bytes;
     c->d = d;
-    c->g = acp::Grid{d.nx, d.ny, d.nz, (int64_t)d.nx * d.ny * d.nz};
+    c->g = acp::Grid{d.nx, d.ny, d.nz, (int64_t)d.nx * d.ny * d.nz, d.boundary == ACP_BC_OPEN_X ? 1 : 0};
     c->precision = d.precision;
-    c->periodic = d.boundary == ACP_BC_PERIODIC;
+    c->periodic = d.boundary != ACP_BC_LODI;
     switch (d.op) {
         case ACP_OP_SRT: c->bulk_op = acp::OP_SRT; break;
         case ACP_OP_RR: c->bulk_op = acp::OP_RR; break;
@@ -761,6 +764,10 @@ int step_graph_chunk(acp_case* c) {
 }
 
 }  // namespace
+
+namespace acp {
+int case_step_launch(acp_case* c, int32_t n) { return step_eager(c, n); }
+}  // namespace acp
 
 extern "C" int acp_case_step(acp_case* c, int32_t n) {
     if (int rc = check_case(c)) return rc;

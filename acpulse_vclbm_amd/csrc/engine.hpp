@@ -36,6 +36,10 @@ struct LodiSchedule {
 };
 LodiSchedule build_lodi_schedule(int nx, int ny, int nz);
 
+// Launch n plain steps of a case on its stream (no graphs, no checks); the
+// current array is c->A (c->SA) afterwards and the one read, c->B (c->SB).
+int case_step_launch(acp_case* c, int32_t n);
+
 // VTK output of one level from host float64 arrays (current array, the other
 // array for the Non-Reflecting boundary quirk, cubic-Mach psi or null).
 int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, const double* fprev, const double* psi,

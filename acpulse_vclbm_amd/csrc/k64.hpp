@@ -21,9 +21,13 @@ struct LodiNode {
     int32_t dir, corner;
 };
 
+// xopen (periodic grids only): the x faces are not wrapped; their nodes
+// collide like interior nodes and links leaving the grid are dropped (the
+// fine level of a refined case, whose x faces the coarse level sets).
 struct Grid {
     int nx, ny, nz;
     int64_t N;
+    int xopen = 0;
 };
 
 hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64* k, const double* A, double* B,

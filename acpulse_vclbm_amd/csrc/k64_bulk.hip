@@ -22,7 +22,7 @@ template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
-                                                     int64_t N) {
+                                                     int64_t N, int xopen) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -97,6 +97,13 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         constexpr int q = decltype(qc)::value;
         if constexpr (!PERIODIC) {
             if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
+        } else {
+            if constexpr (EX[q] > 0) {
+                if (xopen && i == nx - 1) return;
+            }
+            if constexpr (EX[q] < 0) {
+                if (xopen && i == 0) return;
+            }
         }
         I d = l;
         if constexpr (EX[q] > 0) d += oxp;
@@ -117,10 +124,10 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     if (narrow_planes(g.N, sizeof(double)))
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, true>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.N);
+                           g.ny, g.nz, g.N, g.xopen);
     else
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, false>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.N);
+                           g.ny, g.nz, g.N, g.xopen);
     return hipGetLastError();
 }
 

@@ -28,7 +28,7 @@ __device__ __forceinline__ void load_dev(const T* __restrict__ F, int64_t N, int
 template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
-    int ny, int nz, int64_t N) {
+    int ny, int nz, int64_t N, int xopen) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,6 +196,13 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         constexpr int q = decltype(qc)::value;
         if constexpr (!PERIODIC) {
             if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
+        } else {
+            if constexpr (EX[q] > 0) {
+                if (xopen && i == nx - 1) return;
+            }
+            if constexpr (EX[q] < 0) {
+                if (xopen && i == 0) return;
+            }
         }
         I dd = l;
         if constexpr (EX[q] > 0) dd += oxp;
@@ -218,7 +225,7 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     const bool narrow = narrow_planes(g.N, sizeof(float) > sizeof(T) ? sizeof(float) : sizeof(T));
 #define BS(PER, NAR)                                                                                             \
     hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
-                       g.ny, g.nz, g.N)
+                       g.ny, g.nz, g.N, g.xopen)
     if (periodic) {
         if (narrow) BS(true, true);
         else BS(true, false);

@@ -1,0 +1,286 @@
+// Two-level vertex-centred grid refinement (ratio 2), float64.
+//
+// The reference's shipped case file selects the standard coupling
+// (Coupling-Approach STD: Filippova & Hanel 1998, doi 10.1006/jcph.1998.6089)
+// with one coarse overlap cell, every fine link replaced from the coarse level
+// (Coarse-to-Fine fAll), every coarse link replaced from the fine level
+// (Fine-to-Coarse cAll), no fine-to-coarse filter (F2C-Filter NOFI) and
+// population-based scaling (Scaling-Style SRTSTYLE)
+// (BoundaryConditions.txt:52-90).  Its coupling arithmetic lives in
+// GridCoupling.cpp, which the reference does not contain; the declarations
+// (GridCoupling.h:28-46: interpolTimeScalingCoarseToFine, interpolSpaceMidNode,
+// interpolSpaceCenterNode, scalingFineToCoarse) and the nested stepping of
+// Lattice.cpp:4354-5770 fix the sequence, and the published scheme the rest:
+//
+//   one coarse step:  coarse collide + stream            t -> t+1
+//                     fine collide + stream              t -> t+1/2
+//                     coarse-to-fine at t+1/2            (time-interpolated)
+//                     fine collide + stream              t+1/2 -> t+1
+//                     coarse-to-fine at t+1
+//                     fine-to-coarse at t+1
+//
+// Geometry: the coarse level is a periodic nx x ny x nz grid; the fine level
+// covers coarse x in [x0, x1] (every y, z) at half the spacing, i.e.
+// 2 (x1 - x0) + 1 by 2 ny by 2 nz nodes, periodic in y and z, with open x
+// faces (ACP_BC_OPEN_X).  Coarse node (I, J, K) coincides with fine node
+// (2 (I - x0), 2 J, 2 K).
+//
+// Coarse-to-fine on the fine x faces (all 19 links): coincident nodes take
+// the coarse populations of the interface column, linearly interpolated in
+// time (f = 0.5 (f(t) + f(t+1)) at t+1/2), with the non-equilibrium part
+// scaled:  f_fine = feq(rho, u) + a (f - feq(rho, u)),  a = tau_f / (2 tau_c);
+// the other face nodes average their coincident neighbours along y and z
+// (mid nodes: 2, centre nodes: 4).
+// Fine-to-coarse on every coarse node strictly inside the fine region (the
+// overlap cell x0 + 1, x1 - 1 and, so that the whole coarse field stays a
+// valid solution, the nodes beyond it):  f_coarse = feq + (1 / a) (f_fine - feq)
+// of the coincident fine node.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <initializer_list>
+
+#include "acp.h"
+#include "engine.hpp"
+
+namespace acp {
+
+namespace {
+
+// density and velocity (SI) and feq of one node's populations
+template <int ORDER>
+__device__ __forceinline__ void rescale(const K64& k, double f[19], double a) {
+    double rho, ux, uy, uz, eq[19];
+    moments64(k, f, rho, ux, uy, uz);
+    feq64<ORDER>(k, rho, ux, uy, uz, eq);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = eq[q] + a * (f[q] - eq[q]);
+}
+
+// Coincident fine face nodes from the coarse interface column ci (and the
+// fine face column fi): one thread per coarse (J, K).
+template <int ORDER>
+__global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp, const double* __restrict__ Cprev,
+                                                      const double* __restrict__ Ccur, int half, double a,
+                                                      int cnx, int cny, int cnz, int ci, double* __restrict__ F,
+                                                      int fnx, int fny, int fi) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cny * cnz) return;
+    const int J = t % cny, K = t / cny;
+    const int64_t CN = (int64_t)cnx * cny * cnz;
+    const int64_t FN = (int64_t)fnx * fny * (2 * cnz);
+    const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
+    const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    double f[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q)
+        f[q] = half ? 0.5 * (Cprev[q * CN + lc] + Ccur[q * CN + lc]) : Ccur[q * CN + lc];
+    rescale<ORDER>(*kp, f, a);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) F[q * FN + lf] = f[q];
+}
+
+// The other fine face nodes (odd j or odd k) from their coincident neighbours.
+__global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int fnx, int fny, int fnz, int fi) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fny * fnz) return;
+    const int j = t % fny, k = t / fny;
+    if (!(j & 1) && !(k & 1)) return;
+    const int64_t FN = (int64_t)fnx * fny * fnz;
+    auto at = [&](int jj, int kk) {
+        jj = (jj + fny) % fny;
+        kk = (kk + fnz) % fnz;
+        return fi + (int64_t)fnx * (jj + (int64_t)fny * kk);
+    };
+    const int64_t l = at(j, k);
+    if ((j & 1) && (k & 1)) {
+        const int64_t a = at(j - 1, k - 1), b = at(j + 1, k - 1), c = at(j - 1, k + 1), d = at(j + 1, k + 1);
+#pragma unroll
+        for (int q = 0; q < 19; ++q)
+            F[q * FN + l] = ((F[q * FN + a] + F[q * FN + b]) + (F[q * FN + c] + F[q * FN + d])) * 0.25;
+    } else {
+        const int64_t a = (j & 1) ? at(j - 1, k) : at(j, k - 1);
+        const int64_t b = (j & 1) ? at(j + 1, k) : at(j, k + 1);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) F[q * FN + l] = (F[q * FN + a] + F[q * FN + b]) * 0.5;
+    }
+}
+
+// Coarse nodes with x in [x0 + 1, x1 - 1] from their coincident fine nodes.
+template <int ORDER>
+__global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
+                                           int fny, int fnz, double ainv, double* __restrict__ C, int cnx, int cny,
+                                           int cnz, int x0, int width) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)width * cny * cnz) return;
+    const int di = (int)(t % width);
+    const int64_t r = t / width;
+    const int J = (int)(r % cny), K = (int)(r / cny);
+    const int I = (x0 + 1 + di) % cnx;
+    const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * fnz;
+    const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
+    const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    double f[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = F[q * FN + lf];
+    rescale<ORDER>(*kp, f, ainv);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) C[q * CN + lc] = f[q];
+}
+
+}  // namespace
+
+}  // namespace acp
+
+struct acp_refined {
+    acp_case* coarse = nullptr;
+    acp_case* fine = nullptr;
+    int x0 = 0, x1 = 0;
+    double a = 1.0;  // tau_f / (2 tau_c)
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+};
+
+namespace {
+
+using acp::fail;
+
+hipError_t launch_c2f(acp_refined* r, int half) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int order = c->d.order;
+    const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
+    const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
+    const int faces[2][2] = {{r->x0 % cnx, 0}, {r->x1 % cnx, fnx - 1}};
+    for (const auto& fc : faces) {
+        const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
+#define C2F(O)                                                                                                   \
+    hipLaunchKernelGGL((acp::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, cny, cnz, \
+                       fc[0], f->A, fnx, fny, fc[1])
+        if (order == 2) C2F(2);
+        else if (order == 3) C2F(3);
+        else C2F(4);
+#undef C2F
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(acp::c2f_between, dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream, f->A, fnx,
+                           fny, fnz, fc[1]);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_f2c(acp_refined* r) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int width = r->x1 - r->x0 - 1;
+    if (width <= 0) return hipSuccess;
+    const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+#define F2C(O)                                                                                                  \
+    hipLaunchKernelGGL((acp::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, c->A, \
+                       c->g.nx, c->g.ny, c->g.nz, r->x0, width)
+    if (c->d.order == 2) F2C(2);
+    else if (c->d.order == 3) F2C(3);
+    else F2C(4);
+#undef F2C
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream,
+                                  acp_refined** out) {
+    if (!coarse || !out) return fail(ACP_E_ARG, "acp_refined_create: null argument");
+    *out = nullptr;
+    const acp_case_desc& d = *coarse;
+    if (d.boundary != ACP_BC_PERIODIC) return fail(ACP_E_UNSUPPORTED, "refined cases need a periodic coarse level");
+    if (d.precision != ACP_PREC_F64) return fail(ACP_E_UNSUPPORTED, "refined cases run in the float64 tier");
+    if (d.op == ACP_OP_HRR || d.cubic)
+        return fail(ACP_E_UNSUPPORTED, "refined cases: HRR and the cubic-Mach correction are not coupled");
+    if (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx || x1 - x0 >= d.nx - 1)
+        return fail(ACP_E_ARG, "fine region [x0, x1] must satisfy 1 <= x0, x0 + 2 <= x1 <= nx, x1 - x0 < nx - 1");
+    acp_case_desc fd = d;
+    fd.nx = 2 * (x1 - x0) + 1;
+    fd.ny = 2 * d.ny;
+    fd.nz = 2 * d.nz;
+    fd.spacing = d.spacing / 2.;
+    fd.domain[0] = (fd.nx - 0.5) * fd.spacing;
+    fd.domain[1] = (fd.ny - 0.5) * fd.spacing;
+    fd.domain[2] = (fd.nz - 0.5) * fd.spacing;
+    fd.boundary = ACP_BC_OPEN_X;
+    acp_refined* r = new acp_refined();
+    r->x0 = x0;
+    r->x1 = x1;
+    if (stream) {
+        r->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete r;
+            return acp::hip_fail(e, "hipStreamCreate");
+        }
+        r->own_stream = true;
+    }
+    int rc = acp_case_create(&d, r->stream, &r->coarse);
+    if (rc == ACP_OK) rc = acp_case_create(&fd, r->stream, &r->fine);
+    if (rc != ACP_OK) {
+        acp_case_destroy(r->coarse);
+        acp_case_destroy(r->fine);
+        if (r->own_stream) (void)hipStreamDestroy(r->stream);
+        delete r;
+        return rc;
+    }
+    // tau = 1 / omega of each level (Control.cpp:54 with each level's time step)
+    r->a = (1. / r->fine->hk.omega) / (2. * (1. / r->coarse->hk.omega));
+    *out = r;
+    acp::clear_error();
+    return ACP_OK;
+}
+
+extern "C" void acp_refined_destroy(acp_refined* r) {
+    if (!r) return;
+    acp_case_destroy(r->fine);
+    acp_case_destroy(r->coarse);
+    if (r->own_stream && r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+}
+
+extern "C" int acp_refined_level(acp_refined* r, int32_t level, acp_case** out) {
+    if (!r || !out || (level != 0 && level != 1)) return fail(ACP_E_ARG, "acp_refined_level: level 0 or 1");
+    *out = level == 0 ? r->coarse : r->fine;
+    return ACP_OK;
+}
+
+extern "C" int acp_refined_init_pulse(acp_refined* r, double amp, double x0, double y0, double b) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_init_pulse: null argument");
+    int rc = acp_case_init_pulse(r->coarse, amp, x0, y0, b);
+    // the same field at the fine nodes: x = x0_region + i/2, y = j/2 in coarse node units
+    if (rc == ACP_OK) rc = acp_case_init_pulse(r->fine, amp, 2. * (x0 - r->x0), 2. * y0, 2. * b);
+    return rc;
+}
+
+extern "C" int acp_refined_step(acp_refined* r, int32_t n) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_step: null argument");
+    if (n < 0) return fail(ACP_E_ARG, "negative step count");
+    if (!r->coarse->initialized || !r->fine->initialized) return fail(ACP_E_STATE, "refined case not initialised");
+    for (acp_case* c : {r->coarse, r->fine})
+        if (c->nprobe > 0 && c->probe_count + (c == r->fine ? 2 : 1) * n > c->probe_cap)
+            return fail(ACP_E_STATE, "probe buffer full: read the probes first");
+    for (int32_t s = 0; s < n; ++s) {
+        int rc = acp::case_step_launch(r->coarse, 1);
+        if (rc == ACP_OK) rc = acp::case_step_launch(r->fine, 1);
+        if (rc != ACP_OK) return rc;
+        hipError_t e = launch_c2f(r, 1);
+        if (e == hipSuccess) {
+            rc = acp::case_step_launch(r->fine, 1);
+            if (rc != ACP_OK) return rc;
+            e = launch_c2f(r, 0);
+        }
+        if (e == hipSuccess) e = launch_f2c(r);
+        if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_step launch");
+    }
+    return ACP_OK;
+}

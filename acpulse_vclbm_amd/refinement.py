@@ -1,0 +1,66 @@
+"""Interface reflection of the two-level refinement (acp_refined_*).
+
+Case (the refined acoustic-pulse configuration): coarse 256 x 128 x 3
+periodic grid, RR collision, N_eq = 3, tau = 0.5 + 3e-4, SI parameters of the
+reference's case file; fine level (ratio 2) over coarse x in [128, 256];
+Gaussian pulse rho0 (1 + A exp(-ln2 r^2 / 8^2)) at (64, 64), A = 1e-3;
+density probe at (96, 64, 1) on the coarse level.
+
+The periodic domain has two interfaces.  The direct wave reaches the probe
+after ~55 steps (32 nodes at c_s = 1/sqrt(3)), the echo of the x = 128
+interface after ~166 steps (96 nodes), the echo of the other interface
+(x = 256 = 0) not before ~277 steps (160 nodes), so the window is 260 coarse
+steps.  Reference signals: the same pulse on the uniform coarse grid (same
+resolution as the probe's level: the difference is the interface's effect)
+and on a uniform grid at the fine resolution (512 x 256 x 6, probe at
+(192, 128, 2)).  R = max_t |p - p_ref| / max_t |p_ref - rho0|.
+"""
+import time
+
+import numpy as np
+
+TAU = 0.5 + 3e-4
+
+
+def interface_reflection(acp, op="RR", steps=260, amp=1e-3):
+    d = acp.make_desc(256, 128, 3, op=op, order=3, boundary="PERIODIC", mrt_mode="RELAXED", tau=TAU)
+    # refined run
+    with acp.Refined(d, 128, 256) as r:
+        r.init_pulse(amp, 64.0, 64.0, 8.0)
+        c = r.level(0)
+        c.set_probes([(96, 64, 1)], capacity=steps)
+        r.level(0).sync()
+        t0 = time.perf_counter()
+        r.step(steps)
+        c.sync()
+        dt = time.perf_counter() - t0
+        p_ref = c.read_probes()[:, 0].copy()
+        fine_nodes = r.level(1).N
+    rho0 = d.rho0
+    # uniform coarse
+    with acp.Case(d) as u:
+        u.init_pulse(amp, 64.0, 64.0, 8.0)
+        u.set_probes([(96, 64, 1)], capacity=steps)
+        u.step(steps)
+        p_uc = u.read_probes()[:, 0].copy()
+    # uniform fine: same viscosity at half the spacing, two steps per coarse step
+    df = acp.make_desc(512, 256, 6, op=op, order=3, boundary="PERIODIC", mrt_mode="RELAXED", nu=d.nu,
+                       spacing=d.spacing / 2.0)
+    with acp.Case(df) as u:
+        u.init_pulse(amp, 128.0, 128.0, 16.0)
+        u.set_probes([(192, 128, 2)], capacity=2 * steps)
+        u.step(2 * steps)
+        p_uf = u.read_probes()[1::2, 0].copy()
+    sig_c = np.max(np.abs(p_uc - rho0))
+    sig_f = np.max(np.abs(p_uf - rho0))
+    node_updates = 256 * 128 * 3 + 2 * fine_nodes
+    return {
+        "operator": op, "steps": steps, "amplitude": amp,
+        "R_vs_uniform_coarse": float(np.max(np.abs(p_ref - p_uc)) / sig_c),
+        "R_vs_uniform_fine": float(np.max(np.abs(p_ref - p_uf)) / sig_f),
+        "uniform_coarse_vs_fine": float(np.max(np.abs(p_uc - p_uf)) / sig_f),
+        "early_max_diff": float(np.max(np.abs(p_ref[:100] - p_uc[:100])) / sig_c),
+        "ms_per_coarse_step": dt / steps * 1e3,
+        "node_updates_per_coarse_step": node_updates,
+        "mlups": node_updates * steps / dt / 1e6,
+    }

@@ -40,8 +40,12 @@ enum { ACP_OP_SRT = 0, ACP_OP_MRT_GS = 1, ACP_OP_MRT_RM = 2, ACP_OP_RR = 3, ACP_
 /* Storage / arithmetic tier.  F64 reproduces the reference arithmetic; the shifted
  * tiers store f_i - w_i rho0 (scaled) and collide in float32, in lattice units. */
 enum { ACP_PREC_F64 = 0, ACP_PREC_F32_SHIFTED = 1, ACP_PREC_F16_SHIFTED = 2 };
-/* Domain boundary: the reference applies LODI on all six faces (Lattice.cpp:1582-1624). */
-enum { ACP_BC_LODI = 0, ACP_BC_PERIODIC = 1 };
+/* Domain boundary: the reference applies LODI on all six faces (Lattice.cpp:1582-1624).
+ * PERIODIC wraps all three axes.  OPEN_X wraps y and z; the x faces are left
+ * open: their nodes collide like interior nodes and links leaving the grid are
+ * dropped, so something else must set them (the fine level of a refined case,
+ * acp_refined_*); not available with HRR or the cubic-Mach correction. */
+enum { ACP_BC_LODI = 0, ACP_BC_PERIODIC = 1, ACP_BC_OPEN_X = 2 };
 /* MRT operators: the reference computes the relaxed populations and discards them
  * (Node.cpp:2603-2604, 2833-2834), i.e. MRT steps are pure streaming + LODI. */
 enum { ACP_MRT_AS_REFERENCE = 0, ACP_MRT_RELAXED = 1 };
@@ -182,6 +186,30 @@ int acp_write_vtk_host(const acp_case_desc* desc, const double* fcur, const doub
  * phases derived from the reference's pass order, Lattice.cpp:1582-1624;
  * within a phase by ascending node index) and the phase of each.  Returns the node count; writes at most `cap` entries. */
 int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t* nodes, int32_t* phases, int64_t cap);
+
+/* ------------------------------------------------------------------ two-level refinement
+ * Lattice::solve(bool&, bool&) with MaxLevel 1 (Lattice.cpp:2185-2209, nested
+ * stepping Lattice.cpp:4354-15051) for the coupling the reference's case file
+ * selects (Coupling-Approach STD, Overlap 1, Coarse-to-Fine fAll,
+ * Fine-to-Coarse cAll, F2C-Filter NOFI, Scaling-Style SRTSTYLE;
+ * BoundaryConditions.txt:52-106), whose arithmetic (GridCoupling.cpp) the
+ * reference does not contain; restated from the published scheme.  Geometry
+ * differs from the reference's fine shell (Lattice.cpp:38-126): the coarse
+ * level is a periodic grid and the fine level covers coarse x in [x0, x1]
+ * over every y and z at half the spacing and half the time step, with
+ * ratio-2 vertex-centred coupling on its two x faces.  Float64 tier; SRT, RR
+ * (no cubic correction), MRT-GS and MRT-RM. */
+typedef struct acp_refined acp_refined;
+int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, acp_refined** out);
+void acp_refined_destroy(acp_refined* r);
+/* The level's case (0 coarse, 1 fine) for probes, read-back and output; step
+ * it only through acp_refined_step.  Probes record each level's state after
+ * its own step, before the coupling writes into it. */
+int acp_refined_level(acp_refined* r, int32_t level, acp_case** out);
+/* acp_case_init_pulse on both levels with the same field (coarse node units). */
+int acp_refined_init_pulse(acp_refined* r, double amp, double x0, double y0, double b);
+/* n coarse steps (2n fine steps). */
+int acp_refined_step(acp_refined* r, int32_t n);
 
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference, which runs one case per process

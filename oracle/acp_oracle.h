@@ -28,7 +28,8 @@ typedef struct acpo_params {
     int32_t order;      /* Order-Of-Equilibrium 2|3|4 */
     int32_t cubic;      /* Cubic-Mach-Correction yes=1 */
     int32_t transverse; /* Transverse yes=1 */
-    int32_t periodic;   /* 0: LODI on all six faces (the reference), 1: fully periodic (new) */
+    int32_t periodic;   /* 0: LODI on all six faces (the reference), 1: fully periodic (new),
+                           2: periodic y, z with open x faces (fine level of a refined case) */
     int32_t mrt_mode;   /* 0: as the reference (result discarded -> identity), 1: relaxation applied */
     int32_t nthreads;
     double rho0, nu, cs, spacing, mach, k1, sigma;
@@ -101,6 +102,15 @@ void acpo_equilibrium_field(const acpo_params* p, int64_t N, const double* rho, 
  * u = 0, populations feq_N (the initialization of Lattice.cpp:1457-1475 with a new profile). */
 void acpo_pulse_ic(const acpo_params* p, double amp, double x0, double y0, double b, double* f);
 
+/* Two-level refinement (ratio 2; coarse periodic, fine over coarse x in [x0, x1]).
+ * Restates the published STD coupling; the reference's coupling source is absent,
+ * so this part is PARITY UNPINNED against the reference (see acp_oracle.c). */
+typedef struct acpo_refined acpo_refined;
+acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1);
+void acpo_refined_destroy(acpo_refined* r);
+acpo_case* acpo_refined_level(acpo_refined* r, int level);
+double acpo_refined_scale(const acpo_refined* r);
+void acpo_refined_step(acpo_refined* r, int n);
 #ifdef __cplusplus
 }
 #endif

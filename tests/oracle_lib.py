@@ -87,6 +87,14 @@ def lib():
         L.acpo_reference_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, dp]
         L.acpo_equilibrium_field.argtypes = [P, ctypes.c_int64, dp, dp, dp]
         L.acpo_pulse_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+        L.acpo_refined_create.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.acpo_refined_create.restype = ctypes.c_void_p
+        L.acpo_refined_destroy.argtypes = [ctypes.c_void_p]
+        L.acpo_refined_level.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.acpo_refined_level.restype = ctypes.c_void_p
+        L.acpo_refined_scale.argtypes = [ctypes.c_void_p]
+        L.acpo_refined_scale.restype = ctypes.c_double
+        L.acpo_refined_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -165,6 +173,44 @@ class OracleCase:
         lib().acpo_lodi_schedule(self.h, node.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                  ph.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
         return node, ph
+
+
+class _Level(OracleCase):
+    """A level of an OracleRefined (owned by it)."""
+
+    def __init__(self, h, p):
+        self.p = p
+        self.N = p.nx * p.ny * p.nz
+        self.h = h
+
+    def __del__(self):
+        pass
+
+
+class OracleRefined:
+    """Two-level oracle run (coarse periodic p, fine over coarse x in [x0, x1])."""
+
+    def __init__(self, p, x0, x1):
+        self.h = lib().acpo_refined_create(ctypes.byref(p), x0, x1)
+        if not self.h:
+            raise ValueError("oracle rejected the refined case")
+        fp = Params.from_buffer_copy(p)
+        fp.nx, fp.ny, fp.nz = 2 * (x1 - x0) + 1, 2 * p.ny, 2 * p.nz
+        fp.spacing = p.spacing / 2.0
+        fp.periodic = 2
+        self.levels = [_Level(lib().acpo_refined_level(self.h, 0), p), _Level(lib().acpo_refined_level(self.h, 1), fp)]
+        self.x0, self.x1 = x0, x1
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().acpo_refined_destroy(self.h)
+            self.h = None
+
+    def scale(self):
+        return lib().acpo_refined_scale(self.h)
+
+    def step(self, n):
+        lib().acpo_refined_step(self.h, n)
 
 
 def feq(p, order, rho, u):

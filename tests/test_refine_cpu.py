@@ -1,0 +1,94 @@
+"""Two-level refinement: invariants of the oracle's restatement (CPU).
+
+The reference's coupling source (GridCoupling.cpp) is absent, so the
+refinement part of the oracle cannot be checked against the reference; these
+tests pin it by what the scheme fixes: the non-equilibrium scale factor, exact
+preservation of uniform states through every coupling operation, the open-x
+push rule and agreement of a refined run with a uniform fine run.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+
+W = np.array([1 / 18] * 6 + [1 / 36] * 12 + [1 / 3])
+
+
+def refined(op="SRT", tau=0.6, nx=24, ny=10, nz=3, x0=8, x1=16):
+    p = ol.make_params(nx, ny, nz, op=op, periodic=True, tau=tau, mrt_mode=1, nthreads=4)
+    return ol.OracleRefined(p, x0, x1), p
+
+
+@pytest.mark.parametrize("tau", [0.5003, 0.6, 1.0])
+def test_scale_factor_is_tau_ratio(tau):
+    r, _ = refined(tau=tau)
+    tau_f = 2.0 * tau - 0.5  # same viscosity at half the spacing and time step
+    assert r.scale() == pytest.approx(tau_f / (2.0 * tau), rel=1e-12)
+
+
+@pytest.mark.parametrize("op,moving", [("SRT", True), ("RR", True), ("SRT", False), ("RR", False),
+                                       ("MRT-RM", False), ("MRT-GS", False)])
+def test_uniform_state_stays_uniform(op, moving):
+    """A uniform equilibrium is a fixed point of the step on each level and of
+    the coupling (time interpolation of equal values, rescaling of a zero
+    non-equilibrium part).  Moving equilibria only for the Hermite-equilibrium
+    operators: the raw-moment and Gram-Schmidt MRT relax towards their own
+    equilibria, of which a moving feq_3 is not one."""
+    r, p = refined(op)
+    u = np.array([0.02 * p.cs, -0.01 * p.cs, 0.0]) if moving else np.zeros(3)
+    for lv in r.levels:
+        N = lv.N
+        rho = np.full(N, p.rho0)
+        lv.set_state(ol.equilibrium_field(lv.p, rho, np.repeat(u[:, None], N, axis=1)))
+    r.step(6)
+    ref = r.levels[0].state()[:, 0]
+    for lv in r.levels:
+        f = lv.state()
+        assert np.max(np.abs(f - ref[:, None])) <= 1e-14 * np.max(np.abs(ref))
+
+
+def test_open_x_drops_exactly_the_links_leaving_in_x():
+    p = ol.make_params(6, 5, 4, op="SRT", periodic=2, tau=0.6, nthreads=1)
+    c = ol.OracleCase(p)
+    rows = c.geometry()
+    E = [(1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 0, 1), (1, 0, -1), (0, 1, 1),
+         (0, 1, -1), (-1, 0, 1), (-1, 0, -1), (0, -1, 1), (0, -1, -1), (-1, 1, 0), (1, -1, 0), (1, 1, 0), (-1, -1, 0)]
+    for l in range(c.N):
+        i, j, k = l % 6, (l // 6) % 5, l // 30
+        assert rows[l, 26] == 0  # every node collides
+        for q, (ex, ey, ez) in enumerate(E):
+            if (i == 5 and ex > 0) or (i == 0 and ex < 0):
+                assert rows[l, q] == -1
+            else:
+                assert rows[l, q] == (i + ex) + 6 * (((j + ey) % 5) + 5 * ((k + ez) % 4))
+
+
+def test_interface_reflection_is_small():
+    """A pulse crossing the coarse-to-fine interface, seen by a coarse-level probe
+    in front of it, against the same pulse on the uniform coarse grid: identical
+    physics until the interface's echo can arrive (pulse at x = 16, probe at
+    x = 24, interface at x = 32: the echo travels 24 nodes, about 42 steps at
+    c_s = 1/sqrt(3)); afterwards the difference is the reflection, a few per
+    cent of the pulse for this ratio-2 interface."""
+    nx, ny, x0, x1, steps = 64, 32, 32, 64, 60
+    p = ol.make_params(nx, ny, 3, op="SRT", periodic=True, tau=0.55, nthreads=8)
+    r = ol.OracleRefined(p, x0, x1)
+    amp, cx, cy, b = 1e-3, 16.0, 16.0, 4.0
+    r.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    fp = r.levels[1].p
+    r.levels[1].set_state(ol.pulse_ic(fp, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    u = ol.OracleCase(p)
+    u.set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    lc = 24 + nx * (16 + ny * 1)
+    pr, pu = [], []
+    for _ in range(steps):
+        r.step(1)
+        pr.append(r.levels[0].state()[:, lc].sum())
+        pu.append(u.step(1, probes=[(24, 16, 1)])[0, 0])
+    pr, pu = np.array(pr), np.array(pu)
+    sig = np.max(np.abs(pu - p.rho0))
+    assert sig > 0.05 * amp * p.rho0  # the pulse reaches the probe
+    early = np.max(np.abs(pr[:25] - pu[:25])) / sig
+    late = np.max(np.abs(pr - pu)) / sig
+    assert early < 1e-3
+    assert late < 0.05
