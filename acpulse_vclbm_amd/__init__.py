@@ -199,6 +199,10 @@ class Refined:
     def init_pulse(self, amp, x0, y0, b):
         check(lib().acp_refined_init_pulse(self.h, amp, x0, y0, b))
 
+    def set_coupling_precision(self, precision):
+        """Shifted tiers: coupling arithmetic in "F64" (default) or "F32-SHIFTED"."""
+        check(lib().acp_refined_set_coupling_precision(self.h, PREC[precision]))
+
     def step(self, n=1):
         check(lib().acp_refined_step(self.h, int(n)))
 

@@ -132,6 +132,7 @@ def lib():
         "acp_refined_level": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
         "acp_refined_init_pulse": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl]),
         "acp_refined_step": (ctypes.c_int, [vp, i32]),
+        "acp_refined_set_coupling_precision": (ctypes.c_int, [vp, i32]),
         "acp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(vp)]),
         "acp_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request)]),
         "acp_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result), i32, i32, ctypes.POINTER(i32)]),

@@ -43,6 +43,7 @@
 
 #include "acp.h"
 #include "engine.hpp"
+#include "physf.hpp"
 
 namespace acp {
 
@@ -129,6 +130,124 @@ __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const dou
     for (int q = 0; q < 19; ++q) C[q * CN + lc] = f[q];
 }
 
+// ---- shifted tiers: stored g = s (f / rho0 - w) ---------------------------
+// The coupling arithmetic runs in float64 on SI populations (F64 = true; the
+// stored values converted exactly as the population upload and download
+// convert them) or in float32 on lattice-unit deviations (the tier's own
+// moments and equilibrium).
+template <typename T>
+__device__ __forceinline__ double g_to_f(const KShift& ks, T g, int q) {
+    return ks.rho0 * (weight(q) + (double)g / (double)ks.s);
+}
+template <typename T>
+__device__ __forceinline__ T f_to_g(const KShift& ks, double f, int q) {
+    return (T)((double)ks.s * (f / ks.rho0 - weight(q)));
+}
+
+template <typename T, int ORDER, bool F64>
+__device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
+                                            int64_t CN, int64_t lc, int half, double a, T* F, int64_t FN,
+                                            int64_t lf) {
+    if constexpr (F64) {
+        double f[19];
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const double fc = g_to_f(ks, Ccur[q * CN + lc], q);
+            f[q] = half ? 0.5 * (g_to_f(ks, Cprev[q * CN + lc], q) + fc) : fc;
+        });
+        rescale<ORDER>(k, f, a);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            F[q * FN + lf] = f_to_g<T>(ks, f[q], q);
+        });
+    } else {
+        float d[19], eq[19], dr, ux, uy, uz;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const float dc = (float)Ccur[q * CN + lc] * ks.inv_s;
+            d[q] = half ? 0.5f * ((float)Cprev[q * CN + lc] * ks.inv_s + dc) : dc;
+        });
+        moments_dev(d, dr, ux, uy, uz);
+        feq_dev<ORDER>(dr, ux, uy, uz, eq);
+        const float af = (float)a;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            F[q * FN + lf] = (T)((eq[q] + af * (d[q] - eq[q])) * ks.s);
+        });
+    }
+}
+
+template <typename T, int ORDER, bool F64>
+__global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
+                                                        const T* __restrict__ Cprev, const T* __restrict__ Ccur,
+                                                        int half, double a, int cnx, int cny, int cnz, int ci,
+                                                        T* __restrict__ F, int fnx, int fny, int fi) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cny * cnz) return;
+    const int J = t % cny, K = t / cny;
+    const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * (2 * cnz);
+    const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
+    const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    couple_node<T, ORDER, F64>(*kp, *ksp, Cprev, Ccur, CN, lc, half, a, F, FN, lf);
+}
+
+template <typename T, bool F64>
+__global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ ksp, T* __restrict__ F, int fnx,
+                                                     int fny, int fnz, int fi) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= fny * fnz) return;
+    const int j = t % fny, k = t / fny;
+    if (!(j & 1) && !(k & 1)) return;
+    const KShift& ks = *ksp;
+    const int64_t FN = (int64_t)fnx * fny * fnz;
+    auto at = [&](int jj, int kk) {
+        jj = (jj + fny) % fny;
+        kk = (kk + fnz) % fnz;
+        return fi + (int64_t)fnx * (jj + (int64_t)fny * kk);
+    };
+    const int64_t l = at(j, k);
+    int64_t nb[4];
+    int n;
+    if ((j & 1) && (k & 1)) {
+        nb[0] = at(j - 1, k - 1), nb[1] = at(j + 1, k - 1), nb[2] = at(j - 1, k + 1), nb[3] = at(j + 1, k + 1);
+        n = 4;
+    } else {
+        nb[0] = (j & 1) ? at(j - 1, k) : at(j, k - 1);
+        nb[1] = (j & 1) ? at(j + 1, k) : at(j, k + 1);
+        n = 2;
+    }
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (F64) {
+            double v = n == 4 ? ((g_to_f(ks, F[q * FN + nb[0]], q) + g_to_f(ks, F[q * FN + nb[1]], q)) +
+                                 (g_to_f(ks, F[q * FN + nb[2]], q) + g_to_f(ks, F[q * FN + nb[3]], q))) * 0.25
+                              : (g_to_f(ks, F[q * FN + nb[0]], q) + g_to_f(ks, F[q * FN + nb[1]], q)) * 0.5;
+            F[q * FN + l] = f_to_g<T>(ks, v, q);
+        } else {
+            float v = n == 4 ? (((float)F[q * FN + nb[0]] + (float)F[q * FN + nb[1]]) +
+                                ((float)F[q * FN + nb[2]] + (float)F[q * FN + nb[3]])) * 0.25f
+                             : ((float)F[q * FN + nb[0]] + (float)F[q * FN + nb[1]]) * 0.5f;
+            F[q * FN + l] = (T)v;
+        }
+    });
+}
+
+template <typename T, int ORDER, bool F64>
+__global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
+                                             const T* __restrict__ F, int fnx, int fny, int fnz, double ainv,
+                                             T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)width * cny * cnz) return;
+    const int di = (int)(t % width);
+    const int64_t r = t / width;
+    const int J = (int)(r % cny), K = (int)(r / cny);
+    const int I = (x0 + 1 + di) % cnx;
+    const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * fnz;
+    const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
+    const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    couple_node<T, ORDER, F64>(*kp, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
+}
+
 }  // namespace
 
 }  // namespace acp
@@ -137,7 +256,8 @@ struct acp_refined {
     acp_case* coarse = nullptr;
     acp_case* fine = nullptr;
     int x0 = 0, x1 = 0;
-    double a = 1.0;  // tau_f / (2 tau_c)
+    double a = 1.0;        // tau_f / (2 tau_c)
+    bool f64_coupling = true;  // shifted tiers: coupling arithmetic in float64 (else float32)
     hipStream_t stream = nullptr;
     bool own_stream = false;
 };
@@ -146,9 +266,56 @@ namespace {
 
 using acp::fail;
 
+template <typename T, bool F64>
+hipError_t launch_c2f_s(acp_refined* r, int half) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
+    const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
+    const int faces[2][2] = {{r->x0 % cnx, 0}, {r->x1 % cnx, fnx - 1}};
+    for (const auto& fc : faces) {
+        const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
+#define C2FS(O)                                                                                                  \
+    hipLaunchKernelGGL((acp::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks, (const T*)c->SB,    \
+                       (const T*)c->SA, half, r->a, cnx, cny, cnz, fc[0], (T*)f->SA, fnx, fny, fc[1])
+        if (c->d.order == 2) C2FS(2);
+        else if (c->d.order == 3) C2FS(3);
+        else C2FS(4);
+#undef C2FS
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((acp::c2f_between_s<T, F64>), dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream,
+                           f->dks, (T*)f->SA, fnx, fny, fnz, fc[1]);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+template <typename T, bool F64>
+hipError_t launch_f2c_s(acp_refined* r) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int width = r->x1 - r->x0 - 1;
+    const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+#define F2CS(O)                                                                                                  \
+    hipLaunchKernelGGL((acp::f2c_s<T, O, F64>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, f->g.ny, \
+                       f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width)
+    if (c->d.order == 2) F2CS(2);
+    else if (c->d.order == 3) F2CS(3);
+    else F2CS(4);
+#undef F2CS
+    return hipGetLastError();
+}
+
 hipError_t launch_c2f(acp_refined* r, int half) {
     acp_case* c = r->coarse;
     acp_case* f = r->fine;
+    if (c->precision == ACP_PREC_F32_SHIFTED)
+        return r->f64_coupling ? launch_c2f_s<float, true>(r, half) : launch_c2f_s<float, false>(r, half);
+    if (c->precision == ACP_PREC_F16_SHIFTED)
+        return r->f64_coupling ? launch_c2f_s<_Float16, true>(r, half) : launch_c2f_s<_Float16, false>(r, half);
     const int order = c->d.order;
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
     const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
@@ -176,6 +343,10 @@ hipError_t launch_f2c(acp_refined* r) {
     acp_case* c = r->coarse;
     acp_case* f = r->fine;
     const int width = r->x1 - r->x0 - 1;
+    if (width > 0 && c->precision == ACP_PREC_F32_SHIFTED)
+        return r->f64_coupling ? launch_f2c_s<float, true>(r) : launch_f2c_s<float, false>(r);
+    if (width > 0 && c->precision == ACP_PREC_F16_SHIFTED)
+        return r->f64_coupling ? launch_f2c_s<_Float16, true>(r) : launch_f2c_s<_Float16, false>(r);
     if (width <= 0) return hipSuccess;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
@@ -197,7 +368,6 @@ extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32
     *out = nullptr;
     const acp_case_desc& d = *coarse;
     if (d.boundary != ACP_BC_PERIODIC) return fail(ACP_E_UNSUPPORTED, "refined cases need a periodic coarse level");
-    if (d.precision != ACP_PREC_F64) return fail(ACP_E_UNSUPPORTED, "refined cases run in the float64 tier");
     if (d.op == ACP_OP_HRR || d.cubic)
         return fail(ACP_E_UNSUPPORTED, "refined cases: HRR and the cubic-Mach correction are not coupled");
     if (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx || x1 - x0 >= d.nx - 1)
@@ -246,6 +416,16 @@ extern "C" void acp_refined_destroy(acp_refined* r) {
     acp_case_destroy(r->coarse);
     if (r->own_stream && r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
+}
+
+extern "C" int acp_refined_set_coupling_precision(acp_refined* r, int32_t precision) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_set_coupling_precision: null argument");
+    if (precision != ACP_PREC_F64 && precision != ACP_PREC_F32_SHIFTED)
+        return fail(ACP_E_ARG, "coupling precision: ACP_PREC_F64 or ACP_PREC_F32_SHIFTED");
+    if (r->coarse->precision == ACP_PREC_F64 && precision != ACP_PREC_F64)
+        return fail(ACP_E_UNSUPPORTED, "a float64 refined case couples in float64");
+    r->f64_coupling = precision == ACP_PREC_F64;
+    return ACP_OK;
 }
 
 extern "C" int acp_refined_level(acp_refined* r, int32_t level, acp_case** out) {

@@ -22,14 +22,18 @@ import numpy as np
 TAU = 0.5 + 3e-4
 
 
-def interface_reflection(acp, op="RR", steps=260, amp=1e-3):
-    d = acp.make_desc(256, 128, 3, op=op, order=3, boundary="PERIODIC", mrt_mode="RELAXED", tau=TAU)
-    # refined run
+def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", coupling="F64", with_fine=True):
+    """Reflection of the refined pulse case for one tier; `coupling` selects the
+    coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED")."""
+    kw = dict(op=op, order=3, boundary="PERIODIC", mrt_mode="RELAXED", precision=precision)
+    d = acp.make_desc(256, 128, 3, tau=TAU, **kw)
     with acp.Refined(d, 128, 256) as r:
+        if precision != "F64":
+            r.set_coupling_precision(coupling)
         r.init_pulse(amp, 64.0, 64.0, 8.0)
         c = r.level(0)
         c.set_probes([(96, 64, 1)], capacity=steps)
-        r.level(0).sync()
+        c.sync()
         t0 = time.perf_counter()
         r.step(steps)
         c.sync()
@@ -37,30 +41,52 @@ def interface_reflection(acp, op="RR", steps=260, amp=1e-3):
         p_ref = c.read_probes()[:, 0].copy()
         fine_nodes = r.level(1).N
     rho0 = d.rho0
-    # uniform coarse
     with acp.Case(d) as u:
         u.init_pulse(amp, 64.0, 64.0, 8.0)
         u.set_probes([(96, 64, 1)], capacity=steps)
         u.step(steps)
         p_uc = u.read_probes()[:, 0].copy()
-    # uniform fine: same viscosity at half the spacing, two steps per coarse step
-    df = acp.make_desc(512, 256, 6, op=op, order=3, boundary="PERIODIC", mrt_mode="RELAXED", nu=d.nu,
-                       spacing=d.spacing / 2.0)
-    with acp.Case(df) as u:
-        u.init_pulse(amp, 128.0, 128.0, 16.0)
-        u.set_probes([(192, 128, 2)], capacity=2 * steps)
-        u.step(2 * steps)
-        p_uf = u.read_probes()[1::2, 0].copy()
     sig_c = np.max(np.abs(p_uc - rho0))
-    sig_f = np.max(np.abs(p_uf - rho0))
     node_updates = 256 * 128 * 3 + 2 * fine_nodes
-    return {
-        "operator": op, "steps": steps, "amplitude": amp,
+    out = {
+        "operator": op, "precision": precision, "coupling": coupling if precision != "F64" else "F64",
+        "steps": steps, "amplitude": amp,
         "R_vs_uniform_coarse": float(np.max(np.abs(p_ref - p_uc)) / sig_c),
-        "R_vs_uniform_fine": float(np.max(np.abs(p_ref - p_uf)) / sig_f),
-        "uniform_coarse_vs_fine": float(np.max(np.abs(p_uc - p_uf)) / sig_f),
         "early_max_diff": float(np.max(np.abs(p_ref[:100] - p_uc[:100])) / sig_c),
         "ms_per_coarse_step": dt / steps * 1e3,
         "node_updates_per_coarse_step": node_updates,
         "mlups": node_updates * steps / dt / 1e6,
+        "probe": p_ref,
     }
+    if with_fine:
+        # uniform grid at the fine resolution: same viscosity at half the spacing, two steps per coarse step
+        df = acp.make_desc(512, 256, 6, nu=d.nu, spacing=d.spacing / 2.0, **kw)
+        with acp.Case(df) as u:
+            u.init_pulse(amp, 128.0, 128.0, 16.0)
+            u.set_probes([(192, 128, 2)], capacity=2 * steps)
+            u.step(2 * steps)
+            p_uf = u.read_probes()[1::2, 0].copy()
+        sig_f = np.max(np.abs(p_uf - rho0))
+        out["R_vs_uniform_fine"] = float(np.max(np.abs(p_ref - p_uf)) / sig_f)
+        out["uniform_coarse_vs_fine"] = float(np.max(np.abs(p_uc - p_uf)) / sig_f)
+    return out
+
+
+def restriction_ablation(acp, op="RR"):
+    """Refined pulse case in the shifted tiers with the coupling arithmetic in
+    float64 and in float32, against the float64 refined run: probe-series L2
+    error and the relative error of the interface reflection coefficient."""
+    ref = interface_reflection(acp, op, with_fine=False)
+    p64, r64 = ref["probe"], ref["R_vs_uniform_coarse"]
+    rho0 = acp.make_desc(8, 8, 8, tau=0.6).rho0
+    sig = np.linalg.norm(p64 - rho0)
+    out = {"operator": op, "R_f64": r64}
+    for prec in ("F32-SHIFTED", "F16-SHIFTED"):
+        for coupling in ("F64", "F32-SHIFTED"):
+            x = interface_reflection(acp, op, precision=prec, coupling=coupling, with_fine=False)
+            out[f"{prec} coupling {coupling}"] = {
+                "probe_l2": float(np.linalg.norm(x["probe"] - p64) / sig),
+                "R": x["R_vs_uniform_coarse"],
+                "R_rel": abs(x["R_vs_uniform_coarse"] - r64) / r64,
+            }
+    return out

@@ -197,8 +197,9 @@ int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t* nodes, in
  * differs from the reference's fine shell (Lattice.cpp:38-126): the coarse
  * level is a periodic grid and the fine level covers coarse x in [x0, x1]
  * over every y and z at half the spacing and half the time step, with
- * ratio-2 vertex-centred coupling on its two x faces.  Float64 tier; SRT, RR
- * (no cubic correction), MRT-GS and MRT-RM. */
+ * ratio-2 vertex-centred coupling on its two x faces.  SRT, RR (no cubic
+ * correction), MRT-GS and MRT-RM, every storage tier (both levels use the
+ * coarse description's tier). */
 typedef struct acp_refined acp_refined;
 int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, acp_refined** out);
 void acp_refined_destroy(acp_refined* r);
@@ -210,6 +211,10 @@ int acp_refined_level(acp_refined* r, int32_t level, acp_case** out);
 int acp_refined_init_pulse(acp_refined* r, double amp, double x0, double y0, double b);
 /* n coarse steps (2n fine steps). */
 int acp_refined_step(acp_refined* r, int32_t n);
+/* Shifted tiers: run the coupling arithmetic (time interpolation, rescaling,
+ * restriction) in float64 on SI populations (ACP_PREC_F64, the default) or in
+ * float32 on lattice-unit deviations (ACP_PREC_F32_SHIFTED). */
+int acp_refined_set_coupling_precision(acp_refined* r, int32_t precision);
 
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference, which runs one case per process
