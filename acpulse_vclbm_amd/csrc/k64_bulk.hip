@@ -22,7 +22,7 @@ template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
-                                                     int64_t N, int xopen) {
+                                                     int64_t N, int xopen, int row_lds) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -93,6 +93,36 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         }
     }
 
+    // Periodic rows held by one workgroup (row_lds, uniform per block, no thread
+    // left early): the x-shifted links are realigned through LDS so that every
+    // store of a wave covers whole cache lines; position i of the destination
+    // row takes the value of node i - e_x, and with open x faces a position
+    // whose source lies outside the grid is not written.
+    if constexpr (PERIODIC) {
+        if (row_lds) {
+            __shared__ double row[kBulkBlock];
+            static_for<0, 18>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                I base = l - (I)i;
+                if constexpr (EY[q] > 0) base += oyp;
+                if constexpr (EY[q] < 0) base += oym;
+                if constexpr (EZ[q] > 0) base += ozp;
+                if constexpr (EZ[q] < 0) base += ozm;
+                if constexpr (EX[q] == 0) {
+                    P::at(B, q, (I)(base + (I)i), N) = f[q];
+                } else {
+                    __syncthreads();
+                    row[(i + EX[q] + nx) % nx] = f[q];
+                    __syncthreads();
+                    const int src = i - EX[q];
+                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), N) = row[i];
+                }
+            });
+            P::at(B, 18, l, N) = f[18];
+            return;
+        }
+    }
+
     static_for<0, 18>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (!PERIODIC) {
@@ -122,12 +152,13 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
                      hipStream_t s) {
     const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
+    const int row_lds = PERIODIC && g.nx == bx;
     if (narrow_planes(g.N, sizeof(double)))
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, true>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.N, g.xopen);
+                           g.ny, g.nz, g.N, g.xopen, row_lds);
     else
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, false>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.N, g.xopen);
+                           g.ny, g.nz, g.N, g.xopen, row_lds);
     return hipGetLastError();
 }
 

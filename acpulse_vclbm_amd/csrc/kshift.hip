@@ -28,7 +28,7 @@ __device__ __forceinline__ void load_dev(const T* __restrict__ F, int64_t N, int
 template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
-    int ny, int nz, int64_t N, int xopen) {
+    int ny, int nz, int64_t N, int xopen, int row_lds) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -192,6 +192,33 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             });
         }
     }
+    // periodic rows held by one workgroup: x-shifted links realigned through
+    // LDS (see bulk64)
+    if constexpr (PERIODIC) {
+        if (row_lds) {
+            __shared__ T row[kBulkBlock];
+            static_for<0, 18>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                I base = l - (I)i;
+                if constexpr (EY[q] > 0) base += oyp;
+                if constexpr (EY[q] < 0) base += oym;
+                if constexpr (EZ[q] > 0) base += ozp;
+                if constexpr (EZ[q] < 0) base += ozm;
+                if constexpr (EX[q] == 0) {
+                    P::at(B, q, (I)(base + (I)i), N) = (T)g[q];
+                } else {
+                    __syncthreads();
+                    row[(i + EX[q] + nx) % nx] = (T)g[q];
+                    __syncthreads();
+                    const int src = i - EX[q];
+                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), N) = row[i];
+                }
+            });
+            P::at(B, 18, l, N) = (T)g[18];
+            return;
+        }
+    }
+
     static_for<0, 18>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (!PERIODIC) {
@@ -223,9 +250,10 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     // the macro array M is float32 like a float32 population plane
     const bool narrow = narrow_planes(g.N, sizeof(float) > sizeof(T) ? sizeof(float) : sizeof(T));
+    const int row_lds = periodic && g.nx == bx;
 #define BS(PER, NAR)                                                                                             \
     hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
-                       g.ny, g.nz, g.N, g.xopen)
+                       g.ny, g.nz, g.N, g.xopen, row_lds)
     if (periodic) {
         if (narrow) BS(true, true);
         else BS(true, false);

@@ -158,6 +158,49 @@ def test_new_modes_match_oracle_bitexact(gpu, op, order, bnd, mrt):
     assert np.array_equal(f, o.state()), f"max |df| = {np.abs(f - o.state()).max():.3e}"
 
 
+ROW_LDS = [("SRT", 3, 64, 6, 5), ("RR", 4, 64, 6, 5), ("HRR", 3, 128, 4, 4), ("MRT-RM", 3, 64, 7, 3),
+           ("MRT-GS", 3, 128, 3, 4), ("HRR-cubic", 3, 64, 5, 4)]
+
+
+@pytest.mark.parametrize("op,order,nx,ny,nz", ROW_LDS, ids=[f"{a}-nx{c}" for a, b, c, d, e in ROW_LDS])
+def test_periodic_full_row_workgroups_match_oracle_bitexact(gpu, op, order, nx, ny, nz):
+    """Periodic grids whose rows fit one workgroup (nx a multiple of 64 up to
+    512) take the path that realigns x-shifted links through LDS before the
+    store; the result must be the same bits as the oracle's push."""
+    cubic = op.endswith("-cubic")
+    op = op.replace("-cubic", "")
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=True, mrt_mode=1, tau=0.56, cubic=cubic)
+    f0 = _random_f(p, 5)
+    with acp.Case(acp.make_desc(nx, ny, nz, op=op, order=order, boundary="PERIODIC", mrt_mode="RELAXED", tau=0.56,
+                                cubic=cubic)) as c:
+        c.set_populations(f0)
+        c.step(9)
+        f = c.populations()
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(9)
+    assert np.array_equal(f, o.state()), f"max |df| = {np.abs(f - o.state()).max():.3e}"
+
+
+@pytest.mark.parametrize("prec,tol", [("F32-SHIFTED", 2e-4), ("F16-SHIFTED", 2e-2)])
+def test_periodic_full_row_workgroups_shifted_tiers(gpu, prec, tol):
+    """The same LDS path in the shifted tiers, against the float64 oracle
+    (pulse of amplitude 1e-3, 40 steps; tolerance relative to the pulse)."""
+    nx, ny, nz = 64, 48, 3
+    p = ol.make_params(nx, ny, nz, op="SRT", periodic=True, tau=0.56)
+    f0 = ol.pulse_ic(p, 1e-3, 20.0, 24.0, 5.0)
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(40)
+    rho64 = o.state().sum(axis=0)
+    with acp.Case(acp.make_desc(nx, ny, nz, op="SRT", boundary="PERIODIC", tau=0.56, precision=prec)) as c:
+        c.init_pulse(1e-3, 20.0, 24.0, 5.0)
+        c.step(40)
+        rho, _ = c.macros()
+    dev = rho64 - p.rho0
+    assert np.max(np.abs(rho - rho64)) <= tol * np.max(np.abs(dev))
+
+
 def test_periodic_mass_and_translation(gpu):
     """Periodic BGK: exact shift equivariance, mass conserved to round-off."""
     nx, ny, nz = 16, 12, 10
