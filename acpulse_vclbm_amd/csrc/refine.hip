@@ -47,7 +47,7 @@
 
 namespace acp {
 
-namespace {
+namespace refine {
 
 // density and velocity (SI) and feq of one node's populations
 template <int ORDER>
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const K
     couple_node<T, ORDER, F64>(*kp, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
 }
 
-}  // namespace
+}  // namespace refine
 
 }  // namespace acp
 
@@ -276,7 +276,7 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
     for (const auto& fc : faces) {
         const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
 #define C2FS(O)                                                                                                  \
-    hipLaunchKernelGGL((acp::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks, (const T*)c->SB,    \
+    hipLaunchKernelGGL((acp::refine::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks, (const T*)c->SB,    \
                        (const T*)c->SA, half, r->a, cnx, cny, cnz, fc[0], (T*)f->SA, fnx, fny, fc[1])
         if (c->d.order == 2) C2FS(2);
         else if (c->d.order == 3) C2FS(3);
@@ -284,7 +284,7 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
 #undef C2FS
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((acp::c2f_between_s<T, F64>), dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream,
+        hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream,
                            f->dks, (T*)f->SA, fnx, fny, fnz, fc[1]);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -300,7 +300,7 @@ hipError_t launch_f2c_s(acp_refined* r) {
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define F2CS(O)                                                                                                  \
-    hipLaunchKernelGGL((acp::f2c_s<T, O, F64>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, f->g.ny, \
+    hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, f->g.ny, \
                        f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width)
     if (c->d.order == 2) F2CS(2);
     else if (c->d.order == 3) F2CS(3);
@@ -323,7 +323,7 @@ hipError_t launch_c2f(acp_refined* r, int half) {
     for (const auto& fc : faces) {
         const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
 #define C2F(O)                                                                                                   \
-    hipLaunchKernelGGL((acp::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, cny, cnz, \
+    hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, cny, cnz, \
                        fc[0], f->A, fnx, fny, fc[1])
         if (order == 2) C2F(2);
         else if (order == 3) C2F(3);
@@ -331,7 +331,7 @@ hipError_t launch_c2f(acp_refined* r, int half) {
 #undef C2F
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(acp::c2f_between, dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream, f->A, fnx,
+        hipLaunchKernelGGL(acp::refine::c2f_between, dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream, f->A, fnx,
                            fny, fnz, fc[1]);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -351,7 +351,7 @@ hipError_t launch_f2c(acp_refined* r) {
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define F2C(O)                                                                                                  \
-    hipLaunchKernelGGL((acp::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, c->A, \
+    hipLaunchKernelGGL((acp::refine::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, c->A, \
                        c->g.nx, c->g.ny, c->g.nz, r->x0, width)
     if (c->d.order == 2) F2C(2);
     else if (c->d.order == 3) F2C(3);

@@ -90,3 +90,24 @@ def restriction_ablation(acp, op="RR"):
                 "R_rel": abs(x["R_vs_uniform_coarse"] - r64) / r64,
             }
     return out
+
+
+def main():
+    import argparse
+    import json
+
+    import acpulse_vclbm_amd as acp
+
+    ap = argparse.ArgumentParser(description="interface reflection of the refined acoustic-pulse case")
+    ap.add_argument("--op", default="RR")
+    ap.add_argument("--precision", default="F64")
+    ap.add_argument("--coupling", default="F64")
+    ap.add_argument("--steps", type=int, default=260)
+    a = ap.parse_args()
+    r = interface_reflection(acp, op=a.op, steps=a.steps, precision=a.precision, coupling=a.coupling)
+    r.pop("probe", None)
+    print(json.dumps(r))
+
+
+if __name__ == "__main__":
+    main()

@@ -178,3 +178,24 @@ def run_alone(r, graph_chunk=0):
         c.sync()
         probe = c.read_probes()[:, 0].copy()
         return {"checksum": c.checksum(), "mass": c.mass(), "probe": probe}
+
+
+def main():
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="run the serving request mix on one GPU")
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--step-scale", type=float, default=0.25)
+    ap.add_argument("--rate", type=float, default=0.0, help="Poisson arrivals per second; 0: all at once")
+    ap.add_argument("--max-active", type=int, default=16)
+    ap.add_argument("--graph-chunk", type=int, default=64)
+    ap.add_argument("--budget-gib", type=float, default=64.0)
+    a = ap.parse_args()
+    reqs = make_requests(a.requests, seed=1, rate=a.rate or None, step_scale=a.step_scale)
+    _, s = run_mix(reqs, int(a.budget_gib * (1 << 30)), max_active=a.max_active, graph_chunk=a.graph_chunk)
+    print(json.dumps(s))
+
+
+if __name__ == "__main__":
+    main()

@@ -17,8 +17,9 @@ Serving (reported beside, in "serving"): the acoustic-pulse request mix of
 acpulse_vclbm_amd.serve (nx, ny in [128, 1024], nz = 3, LODI, five collision
 operators, float64 / float32-shifted), 48 requests with a quarter of the step
 counts by default (--serve-requests 256 --serve-step-scale 1 for the full mix).
-Sequential float64 capacity is measured first; Poisson arrivals (seed 1) at
-1.2x that capacity are then offered to the concurrent server and to the
+Sequential float64 capacity and the concurrent server's capacity are measured
+with every request arriving at once; Poisson arrivals (seed 1) at 1.2x the
+sequential capacity are then offered to the concurrent server and to the
 sequential float64 server; cases/hour and p50/p99 latency (arrival to result).
 
 Precision ablation (in "precision_ablation"): acpulse_vclbm_amd.ablation, the
@@ -92,9 +93,10 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
     """Serving workload: `n_req` acoustic-pulse requests of the request mix
     (seed 1; steps scaled by `step_scale`).  First the same requests, all in
     float64, run one at a time with one launch sequence per step (sequential
-    capacity); then Poisson arrivals at 1.2x that capacity are offered to the
-    concurrent server (mixed tiers, captured-graph chunks) and, for
-    comparison, to the sequential float64 server."""
+    capacity) and, mixed as requested, all at once to the concurrent server
+    (its capacity); then Poisson arrivals at 1.2x the sequential capacity are
+    offered to the concurrent server (mixed tiers, captured-graph chunks) and,
+    for comparison, to the sequential float64 server."""
     from acpulse_vclbm_amd import serve
 
     budget = 64 << 30
@@ -102,6 +104,7 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
     seq_all = [dict(r, precision="F64") for r in base]
     _, cap = serve.run_mix(seq_all, budget, max_active=1, graph_chunk=0, launch_steps=1, chunks_ahead=64,
                            admit_threads=1)
+    _, srv_cap = serve.run_mix(base, budget, max_active=max_active, graph_chunk=graph_chunk, chunks_ahead=4)
     rate = 1.2 * cap["cases_per_hour"] / 3600.0
     reqs = serve.make_requests(n_req, seed=1, rate=rate, step_scale=step_scale)
     res, mix = serve.run_mix(reqs, budget, max_active=max_active, graph_chunk=graph_chunk, chunks_ahead=4)
@@ -112,6 +115,7 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
     return {
         "requests": n_req, "step_scale": step_scale, "offered_rate_per_hour": round(rate * 3600.0, 1),
         "sequential_f64_capacity_per_hour": round(cap["cases_per_hour"], 1),
+        "server_capacity_per_hour": round(srv_cap["cases_per_hour"], 1),
         "server": {**{k: (round(v, 4) if isinstance(v, float) else v) for k, v in mix.items() if k in keep},
                    "peak_active": mix["peak_active"], "max_active": max_active, "graph_chunk": graph_chunk},
         "sequential_f64_same_arrivals": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in seq.items()
@@ -211,9 +215,11 @@ def main():
     srv = None
     if not a.no_serving:
         srv = serving(acp, a.serve_requests, a.serve_step_scale, a.serve_max_active, a.serve_graph_chunk)
-        print(f"# serving: {srv['server']['cases_per_hour']:.0f} cases/h (sequential float64: "
-              f"{srv['sequential_f64_capacity_per_hour']:.0f}), p50 {srv['server']['latency_p50_s']:.2f} s, "
-              f"p99 {srv['server']['latency_p99_s']:.2f} s", file=sys.stderr, flush=True)
+        print(f"# serving: capacity {srv['server_capacity_per_hour']:.0f} cases/h (sequential float64: "
+              f"{srv['sequential_f64_capacity_per_hour']:.0f}); at 1.2x the sequential capacity p50 "
+              f"{srv['server']['latency_p50_s']:.2f} s, p99 {srv['server']['latency_p99_s']:.2f} s (sequential: "
+              f"{srv['sequential_f64_same_arrivals']['latency_p50_s']:.2f} / "
+              f"{srv['sequential_f64_same_arrivals']['latency_p99_s']:.2f} s)", file=sys.stderr, flush=True)
     abl = None
     if not a.no_ablation:
         from acpulse_vclbm_amd import ablation
