@@ -24,6 +24,144 @@ __device__ __forceinline__ void load_dev(const T* __restrict__ F, int64_t N, int
     });
 }
 
+// Collision of one node in lattice units on its stored deviations g (in place):
+// the operators of the float64 tier rewritten for deviations.  o[6] are the
+// +x, +y, -x, -y, +z, -z neighbour offsets (HRR reads their macros from M).
+template <int OP, int ORDER, bool NARROW>
+__device__ __forceinline__ void collide_shift(const KShift& k, float g[19], const float* __restrict__ M,
+                                              typename Plane<NARROW>::idx l,
+                                              const typename Plane<NARROW>::idx o[6], int64_t N) {
+    using P = Plane<NARROW>;
+    using I = typename P::idx;
+    float d[19];
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        d[q] = g[q] * k.inv_s;
+    });
+    float dr, ux, uy, uz;
+    moments_dev(d, dr, ux, uy, uz);
+    float eq[19];
+    if constexpr (OP == OP_MRT_RM) rm_feq_dev(dr, ux, uy, uz, eq);
+    else if constexpr (OP != OP_MRT_GS) feq_dev<ORDER>(dr, ux, uy, uz, eq);
+    const float om1 = 1.f - k.omega;
+    if constexpr (OP == OP_SRT) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            d[q] = eq[q] + om1 * (d[q] - eq[q]);
+        });
+    } else if constexpr (OP == OP_RR || OP == OP_HRR) {
+        float fneq[19], a2[6], f1[19];
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            fneq[q] = d[q] - eq[q];
+        });
+        pi1_dev(fneq, a2);
+        if constexpr (OP == OP_HRR) {
+            // FD strain of the neighbours' pre-collision velocity (Node.cpp:1750-1889), dx = 1
+                        float v[6][3];
+#pragma unroll
+            for (int m = 0; m < 6; ++m)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) v[m][a] = P::at(M, a + 1, (I)(l + o[m]), N);
+            const float S[6] = {0.5f * (v[0][0] - v[2][0]), 0.5f * (v[1][1] - v[3][1]),
+                                0.5f * (v[4][2] - v[5][2]),
+                                0.5f * (0.5f * (v[0][1] - v[2][1]) + 0.5f * (v[1][0] - v[3][0])),
+                                0.5f * (0.5f * (v[0][2] - v[2][2]) + 0.5f * (v[4][0] - v[5][0])),
+                                0.5f * (0.5f * (v[1][2] - v[3][2]) + 0.5f * (v[4][1] - v[5][1]))};
+            const float c = (1.f - k.sigma) * k.hrr_pre * (1.f + dr);
+#pragma unroll
+            for (int a = 0; a < 6; ++a) a2[a] = k.sigma * a2[a] + c * S[a];
+        }
+        offeq_dev<ORDER>(a2, ux, uy, uz, f1);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            d[q] = eq[q] + om1 * f1[q];
+        });
+    } else if constexpr (OP == OP_MRT_RM) {
+        // feq_RM + (1-omega) M^-1 (non-equilibrium of N_xz, N_yz, Pi_xy, Pi_xz, Pi_yz);
+        // every other raw moment relaxes with rate 1 (Node.cpp:2573-2600)
+        float n5 = 0.f, n6 = 0.f, nxy = 0.f, nxz = 0.f, nyz = 0.f;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const float f = d[q] - eq[q];
+            constexpr int c5 = EX[q] * EX[q] - EZ[q] * EZ[q], c6 = EY[q] * EY[q] - EZ[q] * EZ[q];
+            if constexpr (c5 > 0) n5 += f;
+            if constexpr (c5 < 0) n5 -= f;
+            if constexpr (c6 > 0) n6 += f;
+            if constexpr (c6 < 0) n6 -= f;
+            if constexpr (EX[q] * EY[q] > 0) nxy += f;
+            if constexpr (EX[q] * EY[q] < 0) nxy -= f;
+            if constexpr (EX[q] * EZ[q] > 0) nxz += f;
+            if constexpr (EX[q] * EZ[q] < 0) nxz -= f;
+            if constexpr (EY[q] * EZ[q] > 0) nyz += f;
+            if constexpr (EY[q] * EZ[q] < 0) nyz -= f;
+        });
+        const float t3 = 1.f / 3.f, t6 = 1.f / 6.f;
+        const float ax = om1 * (t3 * n5 - t6 * n6), ay = om1 * (t3 * n6 - t6 * n5), az = -om1 * t6 * (n5 + n6);
+        const float dxy = 0.25f * om1 * nxy, dxz = 0.25f * om1 * nxz, dyz = 0.25f * om1 * nyz;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            float add = 0.f;
+            if constexpr (EX[q] != 0 && EY[q] == 0 && EZ[q] == 0) add = ax;
+            if constexpr (EY[q] != 0 && EX[q] == 0 && EZ[q] == 0) add = ay;
+            if constexpr (EZ[q] != 0 && EX[q] == 0 && EY[q] == 0) add = az;
+            if constexpr (EX[q] * EY[q] != 0) add = (float)(EX[q] * EY[q]) * dxy;
+            if constexpr (EX[q] * EZ[q] != 0) add = (float)(EX[q] * EZ[q]) * dxz;
+            if constexpr (EY[q] * EZ[q] != 0) add = (float)(EY[q] * EZ[q]) * dyz;
+            d[q] = eq[q] + add;
+        });
+    } else if constexpr (OP == OP_MRT_GS) {
+        // Gram-Schmidt moments of the deviations (the rest state has only m0 = 1),
+        // relaxation (Node.cpp:2798-2835) and inverse transform f += M^-1 dm
+        float m[19];
+        static_for<0, 19>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            float s = 0.f;
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                constexpr int a = gsf_coef(r, q);
+                if constexpr (a != 0) s += (float)a * d[q];
+            });
+            m[r] = s;
+        });
+        const float rho = 1.f + m[0];
+        const float ir = 1.f / rho, jx = m[3], jy = m[5], jz = m[7];
+        const float xx = jx * jx, yy = jy * jy, zz = jz * jz;
+        float meq[19] = {0.f};
+        meq[1] = (xx + yy + zz) * ir;
+        meq[9] = (2.f * xx - yy - zz) * ir;
+        meq[11] = (yy - zz) * ir;
+        meq[13] = jx * jy * ir;
+        meq[14] = jy * jz * ir;
+        meq[15] = jx * jz * ir;
+        float dm[19];
+        static_for<0, 19>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (r == 0 || r == 3 || r == 5 || r == 7) dm[r] = 0.f;
+            else if constexpr (r == 1 || r == 9 || r == 11 || r == 13 || r == 14 || r == 15)
+                dm[r] = k.omega * (meq[r] - m[r]);
+            else dm[r] = meq[r] - m[r];
+        });
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            float s = 0.f;
+            static_for<0, 19>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                constexpr int a = gsf_coef(r, q);
+                if constexpr (a != 0) {
+                    constexpr float coef = wf(q) * (float)a / gsf_norm(r);
+                    s += coef * dm[r];
+                }
+            });
+            d[q] += s;
+        });
+    }
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        g[q] = d[q] * k.s;
+    });
+}
+
 // ------------------------------------------------------------------ bulk collide + push
 template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
@@ -62,134 +200,8 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     }
     if constexpr (OP != OP_STREAM) {
         if (!boundary) {
-            float d[19];
-            static_for<0, 19>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                d[q] = g[q] * k.inv_s;
-            });
-            float dr, ux, uy, uz;
-            moments_dev(d, dr, ux, uy, uz);
-            float eq[19];
-            if constexpr (OP == OP_MRT_RM) rm_feq_dev(dr, ux, uy, uz, eq);
-            else if constexpr (OP != OP_MRT_GS) feq_dev<ORDER>(dr, ux, uy, uz, eq);
-            const float om1 = 1.f - k.omega;
-            if constexpr (OP == OP_SRT) {
-                static_for<0, 19>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    d[q] = eq[q] + om1 * (d[q] - eq[q]);
-                });
-            } else if constexpr (OP == OP_RR || OP == OP_HRR) {
-                float fneq[19], a2[6], f1[19];
-                static_for<0, 19>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    fneq[q] = d[q] - eq[q];
-                });
-                pi1_dev(fneq, a2);
-                if constexpr (OP == OP_HRR) {
-                    // FD strain of the neighbours' pre-collision velocity (Node.cpp:1750-1889), dx = 1
-                    const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-                    float v[6][3];
-#pragma unroll
-                    for (int m = 0; m < 6; ++m)
-#pragma unroll
-                        for (int a = 0; a < 3; ++a) v[m][a] = P::at(M, a + 1, (I)(l + o[m]), N);
-                    const float S[6] = {0.5f * (v[0][0] - v[2][0]), 0.5f * (v[1][1] - v[3][1]),
-                                        0.5f * (v[4][2] - v[5][2]),
-                                        0.5f * (0.5f * (v[0][1] - v[2][1]) + 0.5f * (v[1][0] - v[3][0])),
-                                        0.5f * (0.5f * (v[0][2] - v[2][2]) + 0.5f * (v[4][0] - v[5][0])),
-                                        0.5f * (0.5f * (v[1][2] - v[3][2]) + 0.5f * (v[4][1] - v[5][1]))};
-                    const float c = (1.f - k.sigma) * k.hrr_pre * (1.f + dr);
-#pragma unroll
-                    for (int a = 0; a < 6; ++a) a2[a] = k.sigma * a2[a] + c * S[a];
-                }
-                offeq_dev<ORDER>(a2, ux, uy, uz, f1);
-                static_for<0, 19>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    d[q] = eq[q] + om1 * f1[q];
-                });
-            } else if constexpr (OP == OP_MRT_RM) {
-                // feq_RM + (1-omega) M^-1 (non-equilibrium of N_xz, N_yz, Pi_xy, Pi_xz, Pi_yz);
-                // every other raw moment relaxes with rate 1 (Node.cpp:2573-2600)
-                float n5 = 0.f, n6 = 0.f, nxy = 0.f, nxz = 0.f, nyz = 0.f;
-                static_for<0, 19>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    const float f = d[q] - eq[q];
-                    constexpr int c5 = EX[q] * EX[q] - EZ[q] * EZ[q], c6 = EY[q] * EY[q] - EZ[q] * EZ[q];
-                    if constexpr (c5 > 0) n5 += f;
-                    if constexpr (c5 < 0) n5 -= f;
-                    if constexpr (c6 > 0) n6 += f;
-                    if constexpr (c6 < 0) n6 -= f;
-                    if constexpr (EX[q] * EY[q] > 0) nxy += f;
-                    if constexpr (EX[q] * EY[q] < 0) nxy -= f;
-                    if constexpr (EX[q] * EZ[q] > 0) nxz += f;
-                    if constexpr (EX[q] * EZ[q] < 0) nxz -= f;
-                    if constexpr (EY[q] * EZ[q] > 0) nyz += f;
-                    if constexpr (EY[q] * EZ[q] < 0) nyz -= f;
-                });
-                const float t3 = 1.f / 3.f, t6 = 1.f / 6.f;
-                const float ax = om1 * (t3 * n5 - t6 * n6), ay = om1 * (t3 * n6 - t6 * n5), az = -om1 * t6 * (n5 + n6);
-                const float dxy = 0.25f * om1 * nxy, dxz = 0.25f * om1 * nxz, dyz = 0.25f * om1 * nyz;
-                static_for<0, 19>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    float add = 0.f;
-                    if constexpr (EX[q] != 0 && EY[q] == 0 && EZ[q] == 0) add = ax;
-                    if constexpr (EY[q] != 0 && EX[q] == 0 && EZ[q] == 0) add = ay;
-                    if constexpr (EZ[q] != 0 && EX[q] == 0 && EY[q] == 0) add = az;
-                    if constexpr (EX[q] * EY[q] != 0) add = (float)(EX[q] * EY[q]) * dxy;
-                    if constexpr (EX[q] * EZ[q] != 0) add = (float)(EX[q] * EZ[q]) * dxz;
-                    if constexpr (EY[q] * EZ[q] != 0) add = (float)(EY[q] * EZ[q]) * dyz;
-                    d[q] = eq[q] + add;
-                });
-            } else if constexpr (OP == OP_MRT_GS) {
-                // Gram-Schmidt moments of the deviations (the rest state has only m0 = 1),
-                // relaxation (Node.cpp:2798-2835) and inverse transform f += M^-1 dm
-                float m[19];
-                static_for<0, 19>([&](auto rc) {
-                    constexpr int r = decltype(rc)::value;
-                    float s = 0.f;
-                    static_for<0, 19>([&](auto qc) {
-                        constexpr int q = decltype(qc)::value;
-                        constexpr int a = gsf_coef(r, q);
-                        if constexpr (a != 0) s += (float)a * d[q];
-                    });
-                    m[r] = s;
-                });
-                const float rho = 1.f + m[0];
-                const float ir = 1.f / rho, jx = m[3], jy = m[5], jz = m[7];
-                const float xx = jx * jx, yy = jy * jy, zz = jz * jz;
-                float meq[19] = {0.f};
-                meq[1] = (xx + yy + zz) * ir;
-                meq[9] = (2.f * xx - yy - zz) * ir;
-                meq[11] = (yy - zz) * ir;
-                meq[13] = jx * jy * ir;
-                meq[14] = jy * jz * ir;
-                meq[15] = jx * jz * ir;
-                float dm[19];
-                static_for<0, 19>([&](auto rc) {
-                    constexpr int r = decltype(rc)::value;
-                    if constexpr (r == 0 || r == 3 || r == 5 || r == 7) dm[r] = 0.f;
-                    else if constexpr (r == 1 || r == 9 || r == 11 || r == 13 || r == 14 || r == 15)
-                        dm[r] = k.omega * (meq[r] - m[r]);
-                    else dm[r] = meq[r] - m[r];
-                });
-                static_for<0, 19>([&](auto qc) {
-                    constexpr int q = decltype(qc)::value;
-                    float s = 0.f;
-                    static_for<0, 19>([&](auto rc) {
-                        constexpr int r = decltype(rc)::value;
-                        constexpr int a = gsf_coef(r, q);
-                        if constexpr (a != 0) {
-                            constexpr float coef = wf(q) * (float)a / gsf_norm(r);
-                            s += coef * dm[r];
-                        }
-                    });
-                    d[q] += s;
-                });
-            }
-            static_for<0, 19>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                g[q] = d[q] * k.s;
-            });
+            const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+            collide_shift<OP, ORDER, NARROW>(k, g, M, l, o, N);
         }
     }
     // periodic rows held by one workgroup: x-shifted links realigned through
@@ -243,6 +255,73 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     P::at(B, 18, l, N) = (T)g[18];
 }
 
+// Periodic rows of nx = 2 * blockDim.x nodes (nx a multiple of 128, no open
+// faces): two nodes per thread, so that each load and store of a wave moves
+// 64 pairs of consecutive populations; each node collides exactly as in
+// bulk_shift and the x-shifted links are realigned through LDS before the
+// pair stores.
+template <typename T>
+struct alignas(2 * sizeof(T)) Pair2 {
+    T x, y;
+};
+
+template <typename T, int OP, int ORDER, bool NARROW>
+__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift2(
+    const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
+    int ny, int nz, int64_t N) {
+    using P = Plane<NARROW>;
+    using I = typename P::idx;
+    const int t = threadIdx.x;
+    const int j = blockIdx.y, kz = blockIdx.z;
+    const I sxy = (I)nx * (I)ny;
+    const I rs = (I)nx * (I)j + sxy * (I)kz;
+    const KShift& k = *kp;
+    float g[2][19];
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const Pair2<T> v = *reinterpret_cast<const Pair2<T>*>(&P::at(A, q, (I)(rs + (I)(2 * t)), N));
+        g[0][q] = (float)v.x;
+        g[1][q] = (float)v.y;
+    });
+    I oyp = (I)nx, oym = (I)0 - (I)nx, ozp = sxy, ozm = (I)0 - sxy;
+    if (j == ny - 1) oyp = (I)0 - (I)nx * (I)(ny - 1);
+    if (j == 0) oym = (I)nx * (I)(ny - 1);
+    if (kz == nz - 1) ozp = (I)0 - sxy * (I)(nz - 1);
+    if (kz == 0) ozm = sxy * (I)(nz - 1);
+    if constexpr (OP != OP_STREAM) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int i = 2 * t + u;
+            const I oxp = i == nx - 1 ? (I)0 - (I)(nx - 1) : (I)1;
+            const I oxm = i == 0 ? (I)(nx - 1) : (I)-1;
+            const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+            collide_shift<OP, ORDER, NARROW>(k, g[u], M, (I)(rs + (I)i), o, N);
+        }
+    }
+    __shared__ T row[2 * kBulkBlock];
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        I base = rs;
+        if constexpr (EY[q] > 0) base += oyp;
+        if constexpr (EY[q] < 0) base += oym;
+        if constexpr (EZ[q] > 0) base += ozp;
+        if constexpr (EZ[q] < 0) base += ozm;
+        Pair2<T> v;
+        if constexpr (EX[q] == 0) {
+            v.x = (T)g[0][q];
+            v.y = (T)g[1][q];
+        } else {
+            __syncthreads();
+            row[(2 * t + EX[q] + nx) % nx] = (T)g[0][q];
+            row[(2 * t + 1 + EX[q] + nx) % nx] = (T)g[1][q];
+            __syncthreads();
+            v.x = row[2 * t];
+            v.y = row[2 * t + 1];
+        }
+        *reinterpret_cast<Pair2<T>*>(&P::at(B, q, (I)(base + (I)(2 * t)), N)) = v;
+    });
+}
+
 template <typename T, int OP, int ORDER>
 static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B, const float* M, const Grid& g,
                           hipStream_t s) {
@@ -250,6 +329,16 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     // the macro array M is float32 like a float32 population plane
     const bool narrow = narrow_planes(g.N, sizeof(float) > sizeof(T) ? sizeof(float) : sizeof(T));
+    if (periodic && !g.xopen && g.nx % 128 == 0 && g.nx / 2 <= kBulkBlock) {
+        const dim3 b2(g.nx / 2, 1, 1), g2(1, g.ny, g.nz);
+        if (narrow)
+            hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, true>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny,
+                               g.nz, g.N);
+        else
+            hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, false>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx,
+                               g.ny, g.nz, g.N);
+        return hipGetLastError();
+    }
     const int row_lds = periodic && g.nx == bx;
 #define BS(PER, NAR)                                                                                             \
     hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \

@@ -201,6 +201,57 @@ def test_periodic_full_row_workgroups_shifted_tiers(gpu, prec, tol):
     assert np.max(np.abs(rho - rho64)) <= tol * np.max(np.abs(dev))
 
 
+PAIR_ROWS = [("SRT", 3), ("RR", 4), ("HRR", 3), ("MRT-RM", 3), ("MRT-GS", 3)]
+
+
+@pytest.mark.parametrize("prec,tol", [("F32-SHIFTED", 2e-4), ("F16-SHIFTED", 2e-2)])
+@pytest.mark.parametrize("op,order", PAIR_ROWS, ids=[a for a, _ in PAIR_ROWS])
+def test_periodic_pair_rows_shifted_tiers(gpu, prec, tol, op, order):
+    """Periodic rows of 128 to 1024 nodes (multiples of 128) take the shifted
+    kernel with two nodes per thread.  Against the float64 oracle on a pulse
+    off the grid's centre (amplitude 1e-3, 40 steps), with the same tolerances
+    as the one-node-per-thread kernels, for every collision operator."""
+    nx, ny, nz = 128, 20, 3
+    mrt = "RELAXED" if op.startswith("MRT") else "REFERENCE"
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=True, mrt_mode=int(op.startswith("MRT")), tau=0.56,
+                       sigma=0.98)
+    f0 = ol.pulse_ic(p, 1e-3, 41.0, 9.0, 5.0)
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(40)
+    rho64, _ = ol.macros(p, o.state())
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, boundary="PERIODIC", mrt_mode=mrt, tau=0.56, sigma=0.98,
+                      precision=prec, f16_scale=1000.0)
+    with acp.Case(d) as c:
+        c.set_populations(f0)
+        c.step(40)
+        rho, _ = c.macros()
+    dev = rho64 - p.rho0
+    err = np.max(np.abs(rho - rho64)) / np.max(np.abs(dev))
+    print(f"{prec} {op}: relative density error {err:.3e}")
+    assert err <= tol
+
+
+@pytest.mark.parametrize("prec", ["F32-SHIFTED", "F16-SHIFTED"])
+def test_periodic_pair_rows_wrap_exactly(gpu, prec):
+    """A state of period 128 in x on a 256-wide periodic grid stays the
+    128-wide grid's state tiled twice, to the bit: the realignment of x-links
+    through LDS wraps at the row's end exactly like the neighbour offsets."""
+    ny, nz = 10, 4
+    p = ol.make_params(128, ny, nz, op="RR", order=3, periodic=True, tau=0.56)
+    f0 = _random_f(p, 17).reshape(19, nz, ny, 128)
+    out = {}
+    for nx in (128, 256):
+        f = np.concatenate([f0] * (nx // 128), axis=3).reshape(19, -1)
+        with acp.Case(acp.make_desc(nx, ny, nz, op="RR", order=3, boundary="PERIODIC", tau=0.56,
+                                    precision=prec, f16_scale=1000.0)) as c:
+            c.set_populations(f)
+            c.step(12)
+            out[nx] = c.populations().reshape(19, nz, ny, nx)
+    assert np.array_equal(out[256][..., :128], out[128])
+    assert np.array_equal(out[256][..., 128:], out[128])
+
+
 def test_periodic_mass_and_translation(gpu):
     """Periodic BGK: exact shift equivariance, mass conserved to round-off."""
     nx, ny, nz = 16, 12, 10
