@@ -12,7 +12,7 @@ import math
 import numpy as np
 
 from . import _capi
-from ._capi import (ACP_OK, AcpError, BOUNDARY, CaseDesc, CaseInfo, MRT_MODE, OP, PREC, check, dptr, lib)
+from ._capi import (ACP_OK, AcpError, BOUNDARY, F2C_FILTER, CaseDesc, CaseInfo, MRT_MODE, OP, PREC, check, dptr, lib)
 
 __all__ = ["Case", "Control", "Lattice", "Refined", "CaseDesc", "AcpError", "make_desc", "lib", "lodi_schedule"]
 
@@ -202,6 +202,16 @@ class Refined:
     def set_coupling_precision(self, precision):
         """Shifted tiers: coupling arithmetic in "F64" (default) or "F32-SHIFTED"."""
         check(lib().acp_refined_set_coupling_precision(self.h, PREC[precision]))
+
+    def set_filter(self, name):
+        """Fine-to-coarse filter (F2C-Filter): "NOFI" (default) or "LAG"."""
+        if name not in F2C_FILTER:
+            raise AcpError(_capi.ACP_E_UNSUPPORTED, f"fine-to-coarse filter {name!r}: NOFI or LAG")
+        check(lib().acp_refined_set_filter(self.h, F2C_FILTER[name]))
+
+    def restrict(self):
+        """The fine-to-coarse restriction alone, on the levels' current states."""
+        check(lib().acp_refined_restrict(self.h))
 
     def step(self, n=1):
         check(lib().acp_refined_step(self.h, int(n)))

@@ -18,6 +18,7 @@ ACP_OK, ACP_E_ARG, ACP_E_CONFIG, ACP_E_OOM, ACP_E_HIP, ACP_E_STATE, ACP_E_UNSUPP
 OP = {"SRT": 0, "MRT-GS": 1, "MRT-RM": 2, "RR": 3, "HRR": 4}
 PREC = {"F64": 0, "F32-SHIFTED": 1, "F16-SHIFTED": 2}
 BOUNDARY = {"LODI": 0, "PERIODIC": 1, "OPEN-X": 2}
+F2C_FILTER = {"NOFI": 0, "LAG": 1}
 MRT_MODE = {"REFERENCE": 0, "RELAXED": 1}
 
 
@@ -133,6 +134,8 @@ def lib():
         "acp_refined_init_pulse": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl]),
         "acp_refined_step": (ctypes.c_int, [vp, i32]),
         "acp_refined_set_coupling_precision": (ctypes.c_int, [vp, i32]),
+        "acp_refined_set_filter": (ctypes.c_int, [vp, i32]),
+        "acp_refined_restrict": (ctypes.c_int, [vp]),
         "acp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(vp)]),
         "acp_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request)]),
         "acp_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result), i32, i32, ctypes.POINTER(i32)]),

@@ -108,8 +108,44 @@ __global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int f
     }
 }
 
-// Coarse nodes with x in [x0 + 1, x1 - 1] from their coincident fine nodes.
+// F2C-Filter LAG (Lagrava et al. 2012, doi 10.1016/j.jcp.2012.03.015; the
+// reference calls it at Lattice.cpp:5190-5199): the coarse node receives the
+// coincident fine node's equilibrium plus (1/a) times the mean non-equilibrium
+// of that node and its 18 D3Q19 neighbours, each against its own equilibrium,
+// summed in the order: the node, then directions 0..17.  (fx, fy, fz) is the
+// coincident fine node; its x neighbours exist (fx in [2, fnx - 3]), y and z
+// wrap.
 template <int ORDER>
+__device__ __forceinline__ void lag_mean(const K64& k, const double* __restrict__ F, int fnx, int fny, int fnz,
+                                         int fx, int fy, int fz, double ainv, double v[19]) {
+    const int64_t FN = (int64_t)fnx * fny * fnz;
+    double rho, ux, uy, uz, eq[19], acc[19];
+    const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) v[q] = F[q * FN + l0];
+    moments64(k, v, rho, ux, uy, uz);
+    feq64<ORDER>(k, rho, ux, uy, uz, eq);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) acc[q] = v[q] - eq[q];
+    static_for<0, 18>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int y = (fy + EY[j] + fny) % fny, z = (fz + EZ[j] + fnz) % fnz;
+        const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
+        double g[19], e[19], r, x1, x2, x3;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) g[q] = F[q * FN + l];
+        moments64(k, g, r, x1, x2, x3);
+        feq64<ORDER>(k, r, x1, x2, x3, e);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
+    });
+#pragma unroll
+    for (int q = 0; q < 19; ++q) v[q] = eq[q] + ainv * (acc[q] / 19.);
+}
+
+// Coarse nodes with x in [x0 + 1, x1 - 1] from their coincident fine nodes
+// (F2C-Filter NOFI, or LAG).
+template <int ORDER, bool LAG>
 __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                            int fny, int fnz, double ainv, double* __restrict__ C, int cnx, int cny,
                                            int cnz, int x0, int width) {
@@ -121,11 +157,15 @@ __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const dou
     const int I = (x0 + 1 + di) % cnx;
     const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * fnz;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
-    const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
     double f[19];
+    if constexpr (LAG) {
+        lag_mean<ORDER>(*kp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, f);
+    } else {
+        const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
 #pragma unroll
-    for (int q = 0; q < 19; ++q) f[q] = F[q * FN + lf];
-    rescale<ORDER>(*kp, f, ainv);
+        for (int q = 0; q < 19; ++q) f[q] = F[q * FN + lf];
+        rescale<ORDER>(*kp, f, ainv);
+    }
 #pragma unroll
     for (int q = 0; q < 19; ++q) C[q * CN + lc] = f[q];
 }
@@ -173,6 +213,74 @@ __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, cons
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             F[q * FN + lf] = (T)((eq[q] + af * (d[q] - eq[q])) * ks.s);
+        });
+    }
+}
+
+// LAG restriction in the shifted tiers: the same mean as lag_mean, in float64
+// on SI populations (F64) or in float32 on lattice-unit deviations.
+template <typename T, int ORDER, bool F64>
+__device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
+                                           int fnz, int fx, int fy, int fz, double ainv, T* __restrict__ C,
+                                           int64_t CN, int64_t lc) {
+    const int64_t FN = (int64_t)fnx * fny * fnz;
+    const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
+    if constexpr (F64) {
+        double v[19], eq[19], acc[19], rho, ux, uy, uz;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            v[q] = g_to_f(ks, F[q * FN + l0], q);
+        });
+        moments64(k, v, rho, ux, uy, uz);
+        feq64<ORDER>(k, rho, ux, uy, uz, eq);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) acc[q] = v[q] - eq[q];
+        static_for<0, 18>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int y = (fy + EY[j] + fny) % fny, z = (fz + EZ[j] + fnz) % fnz;
+            const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
+            double g[19], e[19], r, x1, x2, x3;
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                g[q] = g_to_f(ks, F[q * FN + l], q);
+            });
+            moments64(k, g, r, x1, x2, x3);
+            feq64<ORDER>(k, r, x1, x2, x3, e);
+#pragma unroll
+            for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
+        });
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            C[q * CN + lc] = f_to_g<T>(ks, eq[q] + ainv * (acc[q] / 19.), q);
+        });
+    } else {
+        float d[19], eq[19], acc[19], dr, ux, uy, uz;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            d[q] = (float)F[q * FN + l0] * ks.inv_s;
+        });
+        moments_dev(d, dr, ux, uy, uz);
+        feq_dev<ORDER>(dr, ux, uy, uz, eq);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) acc[q] = d[q] - eq[q];
+        static_for<0, 18>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int y = (fy + EY[j] + fny) % fny, z = (fz + EZ[j] + fnz) % fnz;
+            const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
+            float g[19], e[19], r, x1, x2, x3;
+            static_for<0, 19>([&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                g[q] = (float)F[q * FN + l] * ks.inv_s;
+            });
+            moments_dev(g, r, x1, x2, x3);
+            feq_dev<ORDER>(r, x1, x2, x3, e);
+#pragma unroll
+            for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
+        });
+        const float af = (float)ainv;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            C[q * CN + lc] = (T)((eq[q] + af * (acc[q] * (1.f / 19.f))) * ks.s);
         });
     }
 }
@@ -232,7 +340,7 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
     });
 }
 
-template <typename T, int ORDER, bool F64>
+template <typename T, int ORDER, bool F64, bool LAG>
 __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                              const T* __restrict__ F, int fnx, int fny, int fnz, double ainv,
                                              T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width) {
@@ -244,8 +352,12 @@ __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const K
     const int I = (x0 + 1 + di) % cnx;
     const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * fnz;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
-    const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
-    couple_node<T, ORDER, F64>(*kp, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
+    if constexpr (LAG) {
+        lag_node_s<T, ORDER, F64>(*kp, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc);
+    } else {
+        const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+        couple_node<T, ORDER, F64>(*kp, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
+    }
 }
 
 }  // namespace refine
@@ -258,6 +370,7 @@ struct acp_refined {
     int x0 = 0, x1 = 0;
     double a = 1.0;        // tau_f / (2 tau_c)
     bool f64_coupling = true;  // shifted tiers: coupling arithmetic in float64 (else float32)
+    bool lag = false;          // fine-to-coarse filter LAG (else NOFI)
     hipStream_t stream = nullptr;
     bool own_stream = false;
 };
@@ -299,12 +412,18 @@ hipError_t launch_f2c_s(acp_refined* r) {
     const int width = r->x1 - r->x0 - 1;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-#define F2CS(O)                                                                                                  \
-    hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, f->g.ny, \
-                       f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width)
-    if (c->d.order == 2) F2CS(2);
-    else if (c->d.order == 3) F2CS(3);
-    else F2CS(4);
+#define F2CS(O, L)                                                                                               \
+    hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64, L>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, \
+                       f->g.ny, f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width)
+    if (r->lag) {
+        if (c->d.order == 2) F2CS(2, true);
+        else if (c->d.order == 3) F2CS(3, true);
+        else F2CS(4, true);
+    } else {
+        if (c->d.order == 2) F2CS(2, false);
+        else if (c->d.order == 3) F2CS(3, false);
+        else F2CS(4, false);
+    }
 #undef F2CS
     return hipGetLastError();
 }
@@ -350,12 +469,18 @@ hipError_t launch_f2c(acp_refined* r) {
     if (width <= 0) return hipSuccess;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-#define F2C(O)                                                                                                  \
-    hipLaunchKernelGGL((acp::refine::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, c->A, \
-                       c->g.nx, c->g.ny, c->g.nz, r->x0, width)
-    if (c->d.order == 2) F2C(2);
-    else if (c->d.order == 3) F2C(3);
-    else F2C(4);
+#define F2C(O, L)                                                                                               \
+    hipLaunchKernelGGL((acp::refine::f2c<O, L>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, \
+                       c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width)
+    if (r->lag) {
+        if (c->d.order == 2) F2C(2, true);
+        else if (c->d.order == 3) F2C(3, true);
+        else F2C(4, true);
+    } else {
+        if (c->d.order == 2) F2C(2, false);
+        else if (c->d.order == 3) F2C(3, false);
+        else F2C(4, false);
+    }
 #undef F2C
     return hipGetLastError();
 }
@@ -425,6 +550,22 @@ extern "C" int acp_refined_set_coupling_precision(acp_refined* r, int32_t precis
     if (r->coarse->precision == ACP_PREC_F64 && precision != ACP_PREC_F64)
         return fail(ACP_E_UNSUPPORTED, "a float64 refined case couples in float64");
     r->f64_coupling = precision == ACP_PREC_F64;
+    return ACP_OK;
+}
+
+extern "C" int acp_refined_set_filter(acp_refined* r, int32_t filter) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_set_filter: null argument");
+    if (filter != ACP_F2C_NOFI && filter != ACP_F2C_LAG)
+        return fail(ACP_E_UNSUPPORTED, "fine-to-coarse filter: ACP_F2C_NOFI or ACP_F2C_LAG");
+    r->lag = filter == ACP_F2C_LAG;
+    return ACP_OK;
+}
+
+extern "C" int acp_refined_restrict(acp_refined* r) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_restrict: null argument");
+    if (!r->coarse->initialized || !r->fine->initialized) return fail(ACP_E_STATE, "refined case not initialised");
+    const hipError_t e = launch_f2c(r);
+    if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_restrict launch");
     return ACP_OK;
 }
 

@@ -22,12 +22,15 @@ import numpy as np
 TAU = 0.5 + 3e-4
 
 
-def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", coupling="F64", with_fine=True):
+def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", coupling="F64", with_fine=True,
+                         f2c_filter="NOFI"):
     """Reflection of the refined pulse case for one tier; `coupling` selects the
-    coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED")."""
+    coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED") and
+    `f2c_filter` the fine-to-coarse filter ("NOFI" or "LAG")."""
     kw = dict(op=op, order=3, boundary="PERIODIC", mrt_mode="RELAXED", precision=precision)
     d = acp.make_desc(256, 128, 3, tau=TAU, **kw)
     with acp.Refined(d, 128, 256) as r:
+        r.set_filter(f2c_filter)
         if precision != "F64":
             r.set_coupling_precision(coupling)
         r.init_pulse(amp, 64.0, 64.0, 8.0)
@@ -50,6 +53,7 @@ def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", cou
     node_updates = 256 * 128 * 3 + 2 * fine_nodes
     out = {
         "operator": op, "precision": precision, "coupling": coupling if precision != "F64" else "F64",
+        "f2c_filter": f2c_filter,
         "steps": steps, "amplitude": amp,
         "R_vs_uniform_coarse": float(np.max(np.abs(p_ref - p_uc)) / sig_c),
         "early_max_diff": float(np.max(np.abs(p_ref[:100] - p_uc[:100])) / sig_c),
@@ -72,18 +76,19 @@ def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", cou
     return out
 
 
-def restriction_ablation(acp, op="RR"):
+def restriction_ablation(acp, op="RR", f2c_filter="NOFI"):
     """Refined pulse case in the shifted tiers with the coupling arithmetic in
     float64 and in float32, against the float64 refined run: probe-series L2
     error and the relative error of the interface reflection coefficient."""
-    ref = interface_reflection(acp, op, with_fine=False)
+    ref = interface_reflection(acp, op, with_fine=False, f2c_filter=f2c_filter)
     p64, r64 = ref["probe"], ref["R_vs_uniform_coarse"]
     rho0 = acp.make_desc(8, 8, 8, tau=0.6).rho0
     sig = np.linalg.norm(p64 - rho0)
-    out = {"operator": op, "R_f64": r64}
+    out = {"operator": op, "f2c_filter": f2c_filter, "R_f64": r64}
     for prec in ("F32-SHIFTED", "F16-SHIFTED"):
         for coupling in ("F64", "F32-SHIFTED"):
-            x = interface_reflection(acp, op, precision=prec, coupling=coupling, with_fine=False)
+            x = interface_reflection(acp, op, precision=prec, coupling=coupling, with_fine=False,
+                                     f2c_filter=f2c_filter)
             out[f"{prec} coupling {coupling}"] = {
                 "probe_l2": float(np.linalg.norm(x["probe"] - p64) / sig),
                 "R": x["R_vs_uniform_coarse"],
@@ -103,8 +108,10 @@ def main():
     ap.add_argument("--precision", default="F64")
     ap.add_argument("--coupling", default="F64")
     ap.add_argument("--steps", type=int, default=260)
+    ap.add_argument("--f2c-filter", default="NOFI", choices=["NOFI", "LAG"])
     a = ap.parse_args()
-    r = interface_reflection(acp, op=a.op, steps=a.steps, precision=a.precision, coupling=a.coupling)
+    r = interface_reflection(acp, op=a.op, steps=a.steps, precision=a.precision, coupling=a.coupling,
+                             f2c_filter=a.f2c_filter)
     r.pop("probe", None)
     print(json.dumps(r))
 

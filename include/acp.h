@@ -215,6 +215,16 @@ int acp_refined_step(acp_refined* r, int32_t n);
  * restriction) in float64 on SI populations (ACP_PREC_F64, the default) or in
  * float32 on lattice-unit deviations (ACP_PREC_F32_SHIFTED). */
 int acp_refined_set_coupling_precision(acp_refined* r, int32_t precision);
+/* F2C-Filter (BoundaryConditions.txt:69; selected at Lattice.cpp:5179-5211):
+ * NOFI hands each coarse node its coincident fine node's non-equilibrium, LAG
+ * the mean non-equilibrium of that fine node and its 18 D3Q19 neighbours
+ * (Lagrava et al. 2012).  The anisotropic filter TOU is not available. */
+enum { ACP_F2C_NOFI = 0, ACP_F2C_LAG = 1 };
+int acp_refined_set_filter(acp_refined* r, int32_t filter);
+/* The fine-to-coarse restriction alone, on the levels' current states (the
+ * last part of acp_refined_step; for tests and for a caller that edits a
+ * level's state). */
+int acp_refined_restrict(acp_refined* r);
 
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference, which runs one case per process

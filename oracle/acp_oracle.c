@@ -996,6 +996,7 @@ struct acpo_refined {
     acpo_case* f;
     int x0, x1;
     double a;
+    int filter; /* 0 NOFI, 1 LAG */
 };
 
 static void rescale19(const acpo_consts* k, int order, double f[19], double a) {
@@ -1075,23 +1076,62 @@ static void refined_c2f(acpo_refined* r, int half) {
     }
 }
 
+/* F2C-Filter LAG (Lagrava et al. 2012, doi 10.1016/j.jcp.2012.03.015, cited
+ * at README.md:18 as the "simple arithmetic average of fine non-equilibria";
+ * called at Lattice.cpp:5190-5199): the non-equilibrium part handed to the
+ * coarse node is the mean over the coincident fine node and its 18 D3Q19
+ * neighbours, each taken against its own equilibrium; the equilibrium part is
+ * the coincident node's.  Summation: the node itself, then directions 0..17. */
+static void lag19(const acpo_consts* k, int order, const acpo_case* f, int fx, int fy, int fz, double ainv,
+                  double v[19]) {
+    const int fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
+    double rho, u[3], eq[19], acc[19], g[19], e[19];
+    const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
+    for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + l0];
+    acpo_moments(k, v, NULL, &rho, u);
+    acpo_feq(k, order, rho, u, eq);
+    for (int q = 0; q < 19; ++q) acc[q] = v[q] - eq[q];
+    for (int j = 0; j < 18; ++j) {
+        const int x = fx + E[j][0], y = (fy + E[j][1] + fny) % fny, z = (fz + E[j][2] + fnz) % fnz;
+        const int64_t l = x + (int64_t)fnx * (y + (int64_t)fny * z);
+        for (int q = 0; q < 19; ++q) g[q] = f->A[(size_t)q * f->N + l];
+        acpo_moments(k, g, NULL, &rho, u);
+        acpo_feq(k, order, rho, u, e);
+        for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
+    }
+    for (int q = 0; q < 19; ++q) v[q] = eq[q] + ainv * (acc[q] / 19.);
+}
+
 static void refined_f2c(acpo_refined* r) {
     acpo_case* c = r->c;
     acpo_case* f = r->f;
     const int cnx = c->p.nx, cny = c->p.ny, cnz = c->p.nz;
     const int fnx = f->p.nx, fny = f->p.ny;
     const double ainv = 1.0 / r->a;
+#pragma omp parallel for collapse(2) schedule(static) num_threads(c->p.nthreads > 0 ? c->p.nthreads : 1)
     for (int I = r->x0 + 1; I <= r->x1 - 1; ++I)
         for (int K = 0; K < cnz; ++K)
             for (int J = 0; J < cny; ++J) {
                 const int64_t lc = (I % cnx) + (int64_t)cnx * (J + (int64_t)cny * K);
                 const int64_t lf = 2 * (I - r->x0) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
                 double v[19];
-                for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + lf];
-                rescale19(&c->k, c->p.order, v, ainv);
+                if (r->filter == 1) {
+                    lag19(&c->k, c->p.order, f, 2 * (I - r->x0), 2 * J, 2 * K, ainv, v);
+                } else {
+                    for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + lf];
+                    rescale19(&c->k, c->p.order, v, ainv);
+                }
                 for (int q = 0; q < 19; ++q) c->A[(size_t)q * c->N + lc] = v[q];
             }
 }
+
+int acpo_refined_set_filter(acpo_refined* r, int filter) {
+    if (filter != 0 && filter != 1) return -1;
+    r->filter = filter;
+    return 0;
+}
+
+void acpo_refined_restrict(acpo_refined* r) { refined_f2c(r); }
 
 /* n coarse steps: coarse step; fine step; c2f at t+1/2; fine step; c2f at t+1; f2c. */
 void acpo_refined_step(acpo_refined* r, int n) {

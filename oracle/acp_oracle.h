@@ -111,6 +111,10 @@ void acpo_refined_destroy(acpo_refined* r);
 acpo_case* acpo_refined_level(acpo_refined* r, int level);
 double acpo_refined_scale(const acpo_refined* r);
 void acpo_refined_step(acpo_refined* r, int n);
+/* fine-to-coarse filter: 0 NOFI (default), 1 LAG; -1 for any other value */
+int acpo_refined_set_filter(acpo_refined* r, int filter);
+/* the fine-to-coarse restriction alone, on the levels' current A arrays */
+void acpo_refined_restrict(acpo_refined* r);
 #ifdef __cplusplus
 }
 #endif

@@ -95,6 +95,8 @@ def lib():
         L.acpo_refined_scale.argtypes = [ctypes.c_void_p]
         L.acpo_refined_scale.restype = ctypes.c_double
         L.acpo_refined_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.acpo_refined_set_filter.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.acpo_refined_restrict.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -211,6 +213,15 @@ class OracleRefined:
 
     def step(self, n):
         lib().acpo_refined_step(self.h, n)
+
+    def set_filter(self, name):
+        """Fine-to-coarse filter: "NOFI" or "LAG"."""
+        if lib().acpo_refined_set_filter(self.h, {"NOFI": 0, "LAG": 1}[name]) != 0:
+            raise ValueError(name)
+
+    def restrict(self):
+        """The fine-to-coarse restriction alone, on the levels' current states."""
+        lib().acpo_refined_restrict(self.h)
 
 
 def feq(p, order, rho, u):

@@ -31,6 +31,69 @@ def test_refined_matches_oracle_bitexact(gpu, op, mrt):
                 assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
 
 
+@pytest.mark.parametrize("op,order,mrt", [("SRT", 3, 0), ("RR", 4, 0), ("MRT-RM", 3, 1), ("SRT", 2, 0)])
+def test_refined_lag_matches_oracle_bitexact(gpu, op, order, mrt):
+    """F2C-Filter LAG: the GPU's refined run is the oracle's to the bit."""
+    nx, ny, nz, x0, x1 = 24, 10, 3, 8, 16
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=True, tau=0.6, mrt_mode=mrt, cs=CS, spacing=DX,
+                       rho0=RHO0)
+    o = ol.OracleRefined(p, x0, x1)
+    o.set_filter("LAG")
+    amp, cx, cy, b = 1e-3, 6.0, 5.0, 2.5
+    o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, tau=0.6, boundary="PERIODIC",
+                      mrt_mode="RELAXED" if mrt else "REFERENCE")
+    with acp.Refined(d, x0, x1) as r:
+        r.set_filter("LAG")
+        r.init_pulse(amp, cx, cy, b)
+        for n in (1, 5):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+
+
+def test_refined_restriction_alone_matches_oracle(gpu):
+    """The restriction on its own (acp_refined_restrict) over a random fine
+    state, both filters, against the oracle bit for bit; and the LAG - NOFI
+    difference of a quadratic non-equilibrium profile is 10/19 of its
+    curvature (the 19-point mean)."""
+    nx, ny, nz, x0, x1 = 20, 8, 3, 6, 14
+    p = ol.make_params(nx, ny, nz, op="SRT", periodic=True, tau=0.6, cs=CS, spacing=DX, rho0=RHO0)
+    o = ol.OracleRefined(p, x0, x1)
+    fp = o.levels[1].p
+    rng = np.random.default_rng(3)
+    W = np.array([1 / 18] * 6 + [1 / 36] * 12 + [1 / 3])
+    FN = fp.nx * fp.ny * fp.nz
+    ff = RHO0 * W[:, None] * (1.0 + 1e-3 * rng.standard_normal((19, FN)))
+    fc = RHO0 * W[:, None] * (1.0 + 1e-3 * rng.standard_normal((19, nx * ny * nz)))
+    d = acp.make_desc(nx, ny, nz, op="SRT", tau=0.6, boundary="PERIODIC")
+    out = {}
+    with acp.Refined(d, x0, x1) as r:
+        for filt in ("NOFI", "LAG"):
+            o.set_filter(filt)
+            r.set_filter(filt)
+            o.levels[0].set_state(fc)
+            o.levels[1].set_state(ff)
+            o.restrict()
+            r.level(0).set_populations(fc)
+            r.level(1).set_populations(ff)
+            r.restrict()
+            out[filt] = r.level(0).populations()
+            assert np.array_equal(out[filt], o.levels[0].state()), filt
+    assert not np.array_equal(out["NOFI"], out["LAG"])
+
+
+def test_refined_lag_shifted_tiers_follow_float64(gpu):
+    """LAG in the shifted tiers: the probe series of the refined pulse case
+    within 1e-4 (float32, float64 coupling) and 1e-3 (float32 coupling) of
+    the float64 refined run, L2 relative to the signal."""
+    a = refinement.restriction_ablation(acp, f2c_filter="LAG")
+    assert a["F32-SHIFTED coupling F64"]["probe_l2"] < 1e-4
+    assert a["F32-SHIFTED coupling F32-SHIFTED"]["probe_l2"] < 1e-3
+
+
 def test_refined_rejects_unsupported():
     d = acp.make_desc(24, 10, 3, op="HRR", tau=0.6, boundary="PERIODIC")
     with pytest.raises(acp.AcpError):
@@ -41,6 +104,11 @@ def test_refined_rejects_unsupported():
     d = acp.make_desc(24, 10, 3, op="SRT", tau=0.6, boundary="PERIODIC")
     with pytest.raises(acp.AcpError):
         acp.Refined(d, 0, 16)
+    with acp.Refined(d, 8, 16) as r:
+        with pytest.raises(acp.AcpError):
+            r.set_filter("TOU")
+        with pytest.raises(acp.AcpError):
+            acp.check(acp.lib().acp_refined_set_filter(r.h, 2))
 
 
 def test_interface_reflection_of_the_refined_pulse_case(gpu):

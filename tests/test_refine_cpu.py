@@ -26,15 +26,17 @@ def test_scale_factor_is_tau_ratio(tau):
     assert r.scale() == pytest.approx(tau_f / (2.0 * tau), rel=1e-12)
 
 
-@pytest.mark.parametrize("op,moving", [("SRT", True), ("RR", True), ("SRT", False), ("RR", False),
-                                       ("MRT-RM", False), ("MRT-GS", False)])
-def test_uniform_state_stays_uniform(op, moving):
+@pytest.mark.parametrize("op,moving,filt", [("SRT", True, "NOFI"), ("RR", True, "NOFI"), ("SRT", False, "NOFI"),
+                                            ("RR", False, "NOFI"), ("MRT-RM", False, "NOFI"),
+                                            ("MRT-GS", False, "NOFI"), ("SRT", True, "LAG"), ("RR", False, "LAG")])
+def test_uniform_state_stays_uniform(op, moving, filt):
     """A uniform equilibrium is a fixed point of the step on each level and of
     the coupling (time interpolation of equal values, rescaling of a zero
     non-equilibrium part).  Moving equilibria only for the Hermite-equilibrium
     operators: the raw-moment and Gram-Schmidt MRT relax towards their own
     equilibria, of which a moving feq_3 is not one."""
     r, p = refined(op)
+    r.set_filter(filt)
     u = np.array([0.02 * p.cs, -0.01 * p.cs, 0.0]) if moving else np.zeros(3)
     for lv in r.levels:
         N = lv.N
@@ -92,3 +94,73 @@ def test_interface_reflection_is_small():
     late = np.max(np.abs(pr - pu)) / sig
     assert early < 1e-3
     assert late < 0.05
+
+
+def _restricted(filt, fine_state, p, x0, x1):
+    r = ol.OracleRefined(p, x0, x1)
+    r.set_filter(filt)
+    r.levels[0].set_state(np.zeros((19, p.nx * p.ny * p.nz)))
+    r.levels[1].set_state(fine_state)
+    r.restrict()
+    return r.levels[0].state().reshape(19, p.nz, p.ny, p.nx), r.scale()
+
+
+@pytest.mark.parametrize("axis", ["x", "y"])
+def test_lag_is_the_19_point_mean_of_the_nonequilibrium(axis):
+    """LAG hands the coarse node the arithmetic mean of the fine
+    non-equilibria over the coincident node and its 18 D3Q19 neighbours.  For
+    a non-equilibrium part h * (alpha + beta s^2) that varies quadratically
+    along one axis s (fine node units; h carries no mass or momentum, so every
+    node has the same density, velocity and equilibrium) the mean is
+    h * (alpha + beta (s^2 + 10/19)): 10 of the 19 links have a non-zero
+    component along s.  NOFI takes the coincident node's value, so
+    LAG - NOFI = (1/a) beta (10/19) h exactly in real arithmetic (a 27-point
+    mean would give 18/27)."""
+    nx, ny, nz, x0, x1 = 20, 8, 3, 6, 14
+    p = ol.make_params(nx, ny, nz, op="SRT", periodic=True, tau=0.6, nthreads=4)
+    fp = ol.make_params(2 * (x1 - x0) + 1, 2 * ny, 2 * nz, op="SRT", periodic=2, tau=0.6, nthreads=4)
+    E = np.array([(1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 0, 1), (1, 0, -1),
+                  (0, 1, 1), (0, 1, -1), (-1, 0, 1), (-1, 0, -1), (0, -1, 1), (0, -1, -1), (-1, 1, 0), (1, -1, 0),
+                  (1, 1, 0), (-1, -1, 0), (0, 0, 0)])
+    h = p.rho0 * W * (E[:, 0] ** 2 - E[:, 1] ** 2) * 1e-3
+    fnx, fny, fnz = fp.nx, fp.ny, fp.nz
+    k, j, i = np.meshgrid(np.arange(fnz), np.arange(fny), np.arange(fnx), indexing="ij")
+    s = (i if axis == "x" else j).astype(float)
+    alpha, beta = 0.3, 0.01
+    feq0 = ol.equilibrium_field(fp, np.full(1, p.rho0), np.zeros((3, 1)))[:, 0]
+    f = feq0[:, None] + h[:, None] * (alpha + beta * s.reshape(-1)[None, :] ** 2)
+    lag, a = _restricted("LAG", f, p, x0, x1)
+    nofi, _ = _restricted("NOFI", f, p, x0, x1)
+    want = h / a * beta * 10.0 / 19.0
+    # coarse nodes that were restricted; along y keep clear of the periodic wrap
+    Js = range(1, ny - 1) if axis == "y" else range(ny)
+    for I in range(x0 + 1, x1):
+        for J in Js:
+            for K in range(nz):
+                d = lag[:, K, J, I] - nofi[:, K, J, I]
+                assert np.max(np.abs(d - want)) <= 1e-9 * np.max(np.abs(want)), (I, J, K)
+    # coarse nodes outside the fine region are left alone
+    assert not np.any(lag[:, :, :, : x0 + 1]) and not np.any(lag[:, :, :, x1:])
+
+
+def test_lag_interface_reflection_is_small():
+    """The interface-reflection check of the NOFI coupling, with LAG."""
+    nx, ny, x0, x1, steps = 64, 32, 32, 64, 60
+    p = ol.make_params(nx, ny, 3, op="SRT", periodic=True, tau=0.55, nthreads=8)
+    r = ol.OracleRefined(p, x0, x1)
+    r.set_filter("LAG")
+    amp, cx, cy, b = 1e-3, 16.0, 16.0, 4.0
+    r.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    r.levels[1].set_state(ol.pulse_ic(r.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    u = ol.OracleCase(p)
+    u.set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    lc = 24 + nx * (16 + ny * 1)
+    pr, pu = [], []
+    for _ in range(steps):
+        r.step(1)
+        pr.append(r.levels[0].state()[:, lc].sum())
+        pu.append(u.step(1, probes=[(24, 16, 1)])[0, 0])
+    pr, pu = np.array(pr), np.array(pu)
+    sig = np.max(np.abs(pu - p.rho0))
+    assert np.max(np.abs(pr[:25] - pu[:25])) / sig < 1e-3
+    assert np.max(np.abs(pr - pu)) / sig < 0.05
