@@ -213,6 +213,14 @@ class Refined:
         """The fine-to-coarse restriction alone, on the levels' current states."""
         check(lib().acp_refined_restrict(self.h))
 
+    def enable_graphs(self, chunk):
+        """Replay `chunk` coarse steps (even) from captured HIP graphs; 0 turns it off."""
+        check(lib().acp_refined_enable_graphs(self.h, int(chunk)))
+
+    def checksum(self):
+        """Order-independent checksum of both levels' states (coarse + fine, mod 2^64)."""
+        return (self._levels[0].checksum() + self._levels[1].checksum()) % (1 << 64)
+
     def step(self, n=1):
         check(lib().acp_refined_step(self.h, int(n)))
 
@@ -251,6 +259,15 @@ def footprint(desc, nprobe=0, probe_capacity=0):
     """Device bytes a case of `desc` holds with `nprobe` probes of `probe_capacity` steps (acp_case_footprint)."""
     b = ctypes.c_int64()
     check(lib().acp_case_footprint(ctypes.byref(desc), int(nprobe), int(probe_capacity), ctypes.byref(b)))
+    return b.value
+
+
+def refined_footprint(desc, x0, x1, nprobe=0, probe_capacity=0):
+    """Device bytes of a refined case (both levels; probes on the coarse level),
+    acp_refined_footprint."""
+    b = ctypes.c_int64()
+    check(lib().acp_refined_footprint(ctypes.byref(desc), int(x0), int(x1), int(nprobe), int(probe_capacity),
+                                      ctypes.byref(b)))
     return b.value
 
 

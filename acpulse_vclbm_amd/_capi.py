@@ -58,6 +58,7 @@ class Request(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("rho_amp", ctypes.c_double), ("u_amp", ctypes.c_double),
         ("probe", ctypes.c_int32 * 3), ("probe_out", ctypes.POINTER(ctypes.c_double)),
         ("arrival", ctypes.c_double), ("tag", ctypes.c_uint64),
+        ("refine", ctypes.c_int32 * 2), ("f2c_filter", ctypes.c_int32),
     ]
 
 
@@ -136,6 +137,9 @@ def lib():
         "acp_refined_set_coupling_precision": (ctypes.c_int, [vp, i32]),
         "acp_refined_set_filter": (ctypes.c_int, [vp, i32]),
         "acp_refined_restrict": (ctypes.c_int, [vp]),
+        "acp_refined_footprint": (ctypes.c_int, [D, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int64)]),
+        "acp_refined_create_in": (ctypes.c_int, [D, i32, i32, vp, vp, ctypes.c_int64, ctypes.POINTER(vp)]),
+        "acp_refined_enable_graphs": (ctypes.c_int, [vp, i32]),
         "acp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(vp)]),
         "acp_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request)]),
         "acp_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result), i32, i32, ctypes.POINTER(i32)]),

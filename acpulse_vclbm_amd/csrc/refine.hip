@@ -373,6 +373,8 @@ struct acp_refined {
     bool lag = false;          // fine-to-coarse filter LAG (else NOFI)
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int32_t graph_chunk = 0;              // coarse steps per captured graph (0: plain launches)
+    hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // by coarse + 2 * fine parity
 };
 
 namespace {
@@ -487,17 +489,16 @@ hipError_t launch_f2c(acp_refined* r) {
 
 }  // namespace
 
-extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream,
-                                  acp_refined** out) {
-    if (!coarse || !out) return fail(ACP_E_ARG, "acp_refined_create: null argument");
-    *out = nullptr;
-    const acp_case_desc& d = *coarse;
+namespace {
+
+// The coarse description's checks and the fine level's description.
+int fine_desc(const acp_case_desc& d, int32_t x0, int32_t x1, acp_case_desc& fd) {
     if (d.boundary != ACP_BC_PERIODIC) return fail(ACP_E_UNSUPPORTED, "refined cases need a periodic coarse level");
     if (d.op == ACP_OP_HRR || d.cubic)
         return fail(ACP_E_UNSUPPORTED, "refined cases: HRR and the cubic-Mach correction are not coupled");
     if (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx || x1 - x0 >= d.nx - 1)
         return fail(ACP_E_ARG, "fine region [x0, x1] must satisfy 1 <= x0, x0 + 2 <= x1 <= nx, x1 - x0 < nx - 1");
-    acp_case_desc fd = d;
+    fd = d;
     fd.nx = 2 * (x1 - x0) + 1;
     fd.ny = 2 * d.ny;
     fd.nz = 2 * d.nz;
@@ -506,6 +507,37 @@ extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32
     fd.domain[1] = (fd.ny - 0.5) * fd.spacing;
     fd.domain[2] = (fd.nz - 0.5) * fd.spacing;
     fd.boundary = ACP_BC_OPEN_X;
+    return ACP_OK;
+}
+
+void drop_graphs(acp_refined* r) {
+    for (hipGraphExec_t& g : r->gexec)
+        if (g) {
+            (void)hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+int parity(const acp_case* c) {
+    const void* cur = c->precision == ACP_PREC_F64 ? (const void*)c->A : c->SA;
+    return cur == c->arr0 ? 0 : 1;
+}
+
+int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, char* region, int64_t bytes,
+                   acp_refined** out) {
+    if (!coarse || !out) return fail(ACP_E_ARG, "acp_refined_create: null argument");
+    *out = nullptr;
+    const acp_case_desc& d = *coarse;
+    acp_case_desc fd;
+    if (int rc = fine_desc(d, x0, x1, fd)) return rc;
+    int64_t cb = 0, fb = 0;
+    if (region) {
+        if (!stream) return fail(ACP_E_ARG, "acp_refined_create_in: a placed refined case needs the caller's stream");
+        if (int rc = acp_case_footprint(&d, 0, 0, &cb)) return rc;
+        if (int rc = acp_case_footprint(&fd, 0, 0, &fb)) return rc;
+        if (fb % 256 != 0 || cb + fb > bytes)
+            return fail(ACP_E_ARG, "acp_refined_create_in: region smaller than acp_refined_footprint");
+    }
     acp_refined* r = new acp_refined();
     r->x0 = x0;
     r->x1 = x1;
@@ -519,8 +551,17 @@ extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32
         }
         r->own_stream = true;
     }
-    int rc = acp_case_create(&d, r->stream, &r->coarse);
-    if (rc == ACP_OK) rc = acp_case_create(&fd, r->stream, &r->fine);
+    int rc;
+    if (region) {
+        // the fine level first (its footprint is a multiple of 256 bytes, so the
+        // coarse part stays aligned); the coarse level takes the rest, where
+        // probes set on it later find the room acp_refined_footprint counted
+        rc = acp_case_create_in(&fd, r->stream, region, fb, &r->fine);
+        if (rc == ACP_OK) rc = acp_case_create_in(&d, r->stream, region + fb, bytes - fb, &r->coarse);
+    } else {
+        rc = acp_case_create(&d, r->stream, &r->coarse);
+        if (rc == ACP_OK) rc = acp_case_create(&fd, r->stream, &r->fine);
+    }
     if (rc != ACP_OK) {
         acp_case_destroy(r->coarse);
         acp_case_destroy(r->fine);
@@ -535,8 +576,105 @@ extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32
     return ACP_OK;
 }
 
+// One refined coarse step as plain launches on the case's stream.
+int step_once(acp_refined* r) {
+    int rc = acp::case_step_launch(r->coarse, 1);
+    if (rc == ACP_OK) rc = acp::case_step_launch(r->fine, 1);
+    if (rc != ACP_OK) return rc;
+    hipError_t e = launch_c2f(r, 1);
+    if (e == hipSuccess) {
+        rc = acp::case_step_launch(r->fine, 1);
+        if (rc != ACP_OK) return rc;
+        e = launch_c2f(r, 0);
+    }
+    if (e == hipSuccess) e = launch_f2c(r);
+    if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_step launch");
+    return ACP_OK;
+}
+
+// graph_chunk coarse steps (an even count: both levels end on the arrays they
+// started from) replayed from a graph captured on first use for each parity.
+int step_graph_chunk(acp_refined* r) {
+    const int p = parity(r->coarse) + 2 * parity(r->fine);
+    acp_case* lv[2] = {r->coarse, r->fine};
+    if (!r->gexec[p]) {
+        int32_t steps0[2], count0[2];
+        for (int i = 0; i < 2; ++i) {
+            steps0[i] = lv[i]->steps;
+            count0[i] = lv[i]->probe_count;
+        }
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(r->stream, hipStreamCaptureModeRelaxed);
+        if (e != hipSuccess) return acp::hip_fail(e, "refined graph capture");
+        int rc = ACP_OK;
+        for (int32_t s = 0; s < r->graph_chunk && rc == ACP_OK; ++s) rc = step_once(r);
+        e = hipStreamEndCapture(r->stream, &g);
+        for (int i = 0; i < 2; ++i) {
+            lv[i]->steps = steps0[i];
+            lv[i]->probe_count = count0[i];
+        }
+        if (rc != ACP_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&r->gexec[p], g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            r->gexec[p] = nullptr;
+            return acp::hip_fail(e, "refined graph capture");
+        }
+    }
+    const hipError_t e = hipGraphLaunch(r->gexec[p], r->stream);
+    if (e != hipSuccess) return acp::hip_fail(e, "refined graph launch");
+    for (int i = 0; i < 2; ++i) {
+        const int32_t k = (i == 0 ? 1 : 2) * r->graph_chunk;
+        lv[i]->steps += k;
+        if (lv[i]->nprobe > 0) lv[i]->probe_count += k;
+    }
+    return ACP_OK;
+}
+
+}  // namespace
+
+extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream,
+                                  acp_refined** out) {
+    return refined_create(coarse, x0, x1, stream, nullptr, 0, out);
+}
+
+extern "C" int acp_refined_footprint(const acp_case_desc* coarse, int32_t x0, int32_t x1, int32_t nprobe,
+                                     int32_t probe_capacity, int64_t* bytes) {
+    if (!coarse || !bytes) return fail(ACP_E_ARG, "acp_refined_footprint: null argument");
+    acp_case_desc fd;
+    if (int rc = fine_desc(*coarse, x0, x1, fd)) return rc;
+    int64_t cb = 0, fb = 0;
+    if (int rc = acp_case_footprint(coarse, nprobe, probe_capacity, &cb)) return rc;
+    if (int rc = acp_case_footprint(&fd, 0, 0, &fb)) return rc;
+    *bytes = fb + cb;
+    return ACP_OK;
+}
+
+extern "C" int acp_refined_create_in(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream,
+                                     void* region, int64_t region_bytes, acp_refined** out) {
+    if (!region || region_bytes <= 0 || ((uintptr_t)region % 256) != 0)
+        return fail(ACP_E_ARG, "acp_refined_create_in: region must be non-null, 256-byte aligned and non-empty");
+    return refined_create(coarse, x0, x1, stream, (char*)region, region_bytes, out);
+}
+
+extern "C" int acp_refined_enable_graphs(acp_refined* r, int32_t chunk) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_enable_graphs: null argument");
+    if (chunk < 0 || chunk == 1 || (chunk % 2) != 0)
+        return fail(ACP_E_ARG, "graph chunk must be 0 (off) or an even number of coarse steps");
+    const hipError_t e = hipStreamSynchronize(r->stream);
+    if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_enable_graphs");
+    drop_graphs(r);
+    r->graph_chunk = chunk;
+    return ACP_OK;
+}
+
 extern "C" void acp_refined_destroy(acp_refined* r) {
     if (!r) return;
+    if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
+    drop_graphs(r);
     acp_case_destroy(r->fine);
     acp_case_destroy(r->coarse);
     if (r->own_stream && r->stream) (void)hipStreamDestroy(r->stream);
@@ -549,6 +687,8 @@ extern "C" int acp_refined_set_coupling_precision(acp_refined* r, int32_t precis
         return fail(ACP_E_ARG, "coupling precision: ACP_PREC_F64 or ACP_PREC_F32_SHIFTED");
     if (r->coarse->precision == ACP_PREC_F64 && precision != ACP_PREC_F64)
         return fail(ACP_E_UNSUPPORTED, "a float64 refined case couples in float64");
+    if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
+    drop_graphs(r);
     r->f64_coupling = precision == ACP_PREC_F64;
     return ACP_OK;
 }
@@ -557,6 +697,8 @@ extern "C" int acp_refined_set_filter(acp_refined* r, int32_t filter) {
     if (!r) return fail(ACP_E_ARG, "acp_refined_set_filter: null argument");
     if (filter != ACP_F2C_NOFI && filter != ACP_F2C_LAG)
         return fail(ACP_E_UNSUPPORTED, "fine-to-coarse filter: ACP_F2C_NOFI or ACP_F2C_LAG");
+    if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
+    drop_graphs(r);
     r->lag = filter == ACP_F2C_LAG;
     return ACP_OK;
 }
@@ -590,18 +732,11 @@ extern "C" int acp_refined_step(acp_refined* r, int32_t n) {
     for (acp_case* c : {r->coarse, r->fine})
         if (c->nprobe > 0 && c->probe_count + (c == r->fine ? 2 : 1) * n > c->probe_cap)
             return fail(ACP_E_STATE, "probe buffer full: read the probes first");
-    for (int32_t s = 0; s < n; ++s) {
-        int rc = acp::case_step_launch(r->coarse, 1);
-        if (rc == ACP_OK) rc = acp::case_step_launch(r->fine, 1);
-        if (rc != ACP_OK) return rc;
-        hipError_t e = launch_c2f(r, 1);
-        if (e == hipSuccess) {
-            rc = acp::case_step_launch(r->fine, 1);
-            if (rc != ACP_OK) return rc;
-            e = launch_c2f(r, 0);
-        }
-        if (e == hipSuccess) e = launch_f2c(r);
-        if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_step launch");
+    while (r->graph_chunk > 0 && n >= r->graph_chunk) {
+        if (int rc = step_graph_chunk(r)) return rc;
+        n -= r->graph_chunk;
     }
+    for (int32_t s = 0; s < n; ++s)
+        if (int rc = step_once(r)) return rc;
     return ACP_OK;
 }

@@ -74,7 +74,8 @@ struct Job {
     int64_t bytes = 0;   // footprint
     int64_t off = -1;    // arena offset
     int slot = -1;
-    acp_case* c = nullptr;
+    acp_case* c = nullptr;      // the case, or the coarse level of r
+    acp_refined* r = nullptr;   // two-level case
     int32_t launched = 0;
     std::deque<hipEvent_t> inflight;
     double t_submit = 0, t_admit = 0, t_start = 0;
@@ -140,6 +141,43 @@ void post_result(acp_server* s, Job& j, double t_done, double mass, uint64_t che
     s->cv_admit.notify_all();
 }
 
+int init_case(acp_case* c, const acp_request& q);
+
+bool two_level(const acp_request& q) { return q.refine[1] > q.refine[0]; }
+
+void destroy_job_case(Job& j) {
+    if (j.r) acp_refined_destroy(j.r);
+    else if (j.c) acp_case_destroy(j.c);
+    j.r = nullptr;
+    j.c = nullptr;
+}
+
+// Create and initialise the job's case inside its arena range.
+int create_job_case(acp_server* s, Job& j) {
+    const acp_request& q = j.req;
+    int rc;
+    if (two_level(q)) {
+        if (q.init_kind != ACP_INIT_PULSE)
+            return acp::fail(ACP_E_UNSUPPORTED, "two-level requests start from a pulse (ACP_INIT_PULSE)");
+        rc = acp_refined_create_in(&q.desc, q.refine[0], q.refine[1], s->streams[j.slot], s->arena + j.off, j.bytes,
+                                   &j.r);
+        if (rc == ACP_OK) rc = acp_refined_level(j.r, 0, &j.c);
+        if (rc == ACP_OK) rc = acp_refined_set_filter(j.r, q.f2c_filter);
+        if (rc == ACP_OK) rc = acp_refined_init_pulse(j.r, q.amp, q.x0, q.y0, q.halfwidth);
+    } else {
+        rc = acp_case_create_in(&q.desc, s->streams[j.slot], s->arena + j.off, j.bytes, &j.c);
+        if (rc == ACP_OK) rc = init_case(j.c, q);
+    }
+    if (rc == ACP_OK && q.probe[0] >= 0) {
+        const acp_case_desc& d = q.desc;
+        const int64_t node = (int64_t)q.probe[0] + (int64_t)d.nx * ((int64_t)q.probe[1] + (int64_t)d.ny * q.probe[2]);
+        rc = acp_case_set_probes(j.c, &node, 1, std::max(q.steps, 1));
+    }
+    if (rc == ACP_OK && s->cfg.graph_chunk > 0)
+        rc = j.r ? acp_refined_enable_graphs(j.r, s->cfg.graph_chunk) : acp_case_enable_graphs(j.c, s->cfg.graph_chunk);
+    return rc;
+}
+
 int init_case(acp_case* c, const acp_request& q) {
     switch (q.init_kind) {
         case ACP_INIT_REFERENCE: return acp_case_init_reference(c);
@@ -189,20 +227,11 @@ void admit_loop(acp_server* s) {
         j.t_admit = s->now();
         lk.unlock();
 
-        int rc = acp_case_create_in(&j.req.desc, s->streams[j.slot], s->arena + j.off, j.bytes, &j.c);
-        if (rc == ACP_OK) rc = init_case(j.c, j.req);
-        if (rc == ACP_OK && j.req.probe[0] >= 0) {
-            const acp_case_desc& d = j.req.desc;
-            const int64_t node = (int64_t)j.req.probe[0] + (int64_t)d.nx * ((int64_t)j.req.probe[1] +
-                                                                              (int64_t)d.ny * j.req.probe[2]);
-            rc = acp_case_set_probes(j.c, &node, 1, std::max(j.req.steps, 1));
-        }
-        if (rc == ACP_OK && s->cfg.graph_chunk > 0) rc = acp_case_enable_graphs(j.c, s->cfg.graph_chunk);
+        const int rc = create_job_case(s, j);
         if (rc != ACP_OK) {
             j.status = rc;
             j.error = acp_last_error();
-            if (j.c) acp_case_destroy(j.c);
-            j.c = nullptr;
+            destroy_job_case(j);
             lk.lock();
             --s->in_admission;
             lk.unlock();
@@ -238,13 +267,19 @@ void finish_job(acp_server* s, Job& j) {
     }
     if (rc == ACP_OK) rc = acp_case_mass(j.c, &mass);
     if (rc == ACP_OK) rc = acp_case_checksum(j.c, &cs);
+    if (rc == ACP_OK && j.r) {
+        acp_case* fine = nullptr;
+        uint64_t fcs = 0;
+        rc = acp_refined_level(j.r, 1, &fine);
+        if (rc == ACP_OK) rc = acp_case_checksum(fine, &fcs);
+        cs += fcs;
+    }
     if (rc != ACP_OK && j.status == ACP_OK) {
         j.status = rc;
         j.error = acp_last_error();
     }
     const double t_done = s->now();
-    acp_case_destroy(j.c);
-    j.c = nullptr;
+    destroy_job_case(j);
     {
         std::lock_guard<std::mutex> lk(s->mu);
         --s->running;
@@ -290,7 +325,7 @@ void drive_loop(acp_server* s) {
             while (j.status == ACP_OK && j.launched < j.req.steps && (int32_t)j.inflight.size() < ahead) {
                 if (j.launched == 0) j.t_start = s->now();
                 const int32_t k = std::min(chunk, j.req.steps - j.launched);
-                int rc = acp_case_step(j.c, k);
+                int rc = j.r ? acp_refined_step(j.r, k) : acp_case_step(j.c, k);
                 hipEvent_t ev = nullptr;
                 if (rc == ACP_OK) {
                     if (!pool.empty()) {
@@ -377,7 +412,16 @@ extern "C" int acp_server_submit(acp_server* s, const acp_request* r) {
                              r->probe[2] >= d.nz))
         return acp::fail(ACP_E_ARG, "acp_server_submit: probe outside the grid");
     int64_t bytes = 0;
-    if (int rc = acp_case_footprint(&d, r->probe[0] >= 0 ? 1 : 0, std::max(r->steps, 1), &bytes)) return rc;
+    const int32_t np = r->probe[0] >= 0 ? 1 : 0, cap = std::max(r->steps, 1);
+    if (two_level(*r)) {
+        if (r->init_kind != ACP_INIT_PULSE)
+            return acp::fail(ACP_E_UNSUPPORTED, "acp_server_submit: two-level requests start from a pulse");
+        if (r->f2c_filter != ACP_F2C_NOFI && r->f2c_filter != ACP_F2C_LAG)
+            return acp::fail(ACP_E_UNSUPPORTED, "acp_server_submit: fine-to-coarse filter NOFI or LAG");
+        if (int rc = acp_refined_footprint(&d, r->refine[0], r->refine[1], np, cap, &bytes)) return rc;
+    } else if (int rc = acp_case_footprint(&d, np, cap, &bytes)) {
+        return rc;
+    }
     if (bytes > s->cfg.hbm_budget_bytes)
         return acp::fail(ACP_E_OOM, "acp_server_submit: case needs " + std::to_string(bytes) +
                                         " bytes, more than the server's HBM budget");

@@ -13,10 +13,18 @@ coupling scheme (overlap or direct), the collision operator uniform over
 BGK/MRT-GS/MRT-RM/RR/HRR, the pulse amplitude log-uniform in [1e-4, 1e-2], the
 half-width uniform in [4, 20] nodes, the step count uniform in [1000, 8000]
 (times `step_scale`), and the tier (float64 or float32-shifted, 1/2 each).
-Grids are nx x ny x 3 with LODI on all faces, tau = 0.5 + 3e-4, N_eq = 3; the
-pulse is centred and the probe sits at (3/4 nx, ny/2, 1).  This engine has one
-grid level: a request that asks for refinement runs on its coarse grid alone,
-and the flag is only counted.
+Grids are nx x ny x 3, tau = 0.5 + 3e-4, N_eq = 3, the probe at
+(3/4 nx, ny/2, 1).
+
+  * One-level requests: LODI on all faces, the pulse at the centre.
+  * Refined requests (acp_refined_*): a periodic coarse grid with a fine level
+    (ratio 2) over x in [nx/2, nx] (every coarse node of that half has fine
+    nodes), the pulse at (nx/4, ny/2) in the coarse half, the probe read on the
+    coarse level inside the fine region; standard coupling with the F2C filter
+    NOFI.  The direct coupling is not available: requests that draw it run
+    with the standard coupling and are counted as such.  HRR is not coupled
+    across levels: refined HRR requests run on their one-level grid and are
+    counted.
 """
 import ctypes
 import time
@@ -56,14 +64,45 @@ def make_requests(n=256, seed=1, rate=None, step_scale=1.0, nxy=(128, 1024), ste
     return out
 
 
+def two_level(r):
+    """Whether the request runs as a refined case (HRR is not coupled across levels)."""
+    return bool(r.get("refined")) and r["op"] != "HRR"
+
+
+def fine_region(r):
+    return (r["nx"] // 2, r["nx"])
+
+
 def request_desc(r):
     from . import make_desc
-    return make_desc(r["nx"], r["ny"], r["nz"], op=r["op"], order=3, boundary="LODI", mrt_mode="RELAXED",
+    return make_desc(r["nx"], r["ny"], r["nz"], op=r["op"], order=3,
+                     boundary="PERIODIC" if two_level(r) else "LODI", mrt_mode="RELAXED",
                      precision=r["precision"], tau=TAU, sigma=0.98)
+
+
+def pulse_centre(r):
+    return (r["nx"] / 4.0 if two_level(r) else r["nx"] / 2.0, r["ny"] / 2.0)
 
 
 def probe_point(r):
     return (3 * r["nx"] // 4, r["ny"] // 2, r["nz"] // 2)
+
+
+def footprint(r):
+    """Device bytes the server reserves for request `r` (one probe of `steps` densities)."""
+    from . import footprint as case_footprint, refined_footprint
+    cap = max(int(r["steps"]), 1)
+    if two_level(r):
+        return refined_footprint(request_desc(r), *fine_region(r), 1, cap)
+    return case_footprint(request_desc(r), 1, cap)
+
+
+def mix_counts(reqs):
+    """How the refinement flags of a request list were served."""
+    return {"requests": len(reqs), "refined_flag": sum(bool(r["refined"]) for r in reqs),
+            "two_level": sum(two_level(r) for r in reqs),
+            "refined_hrr_one_level": sum(bool(r["refined"]) and r["op"] == "HRR" for r in reqs),
+            "direct_run_as_overlap": sum(two_level(r) and r["scheme"] == "direct" for r in reqs)}
 
 
 class Server:
@@ -84,7 +123,11 @@ class Server:
         q.desc = request_desc(r)
         q.steps = int(r["steps"])
         q.init_kind = INIT["PULSE"]
-        q.amp, q.x0, q.y0, q.halfwidth = r["amp"], r["nx"] / 2.0, r["ny"] / 2.0, r["halfwidth"]
+        cx, cy = pulse_centre(r)
+        q.amp, q.x0, q.y0, q.halfwidth = r["amp"], cx, cy, r["halfwidth"]
+        if two_level(r):
+            q.refine[0], q.refine[1] = fine_region(r)
+            q.f2c_filter = _capi.F2C_FILTER["NOFI"]
         pi, pj, pk = probe_point(r)
         q.probe[0], q.probe[1], q.probe[2] = pi, pj, pk
         buf = np.zeros(max(q.steps, 1))
@@ -168,9 +211,22 @@ def summarize(results, stats=None):
 
 def run_alone(r, graph_chunk=0):
     """One request on its own case (no server): final checksum, mass and probe series."""
-    from . import Case
+    from . import Case, Refined
+    cx, cy = pulse_centre(r)
+    if two_level(r):
+        with Refined(request_desc(r), *fine_region(r)) as rc:
+            rc.set_filter("NOFI")
+            rc.init_pulse(r["amp"], cx, cy, r["halfwidth"])
+            c = rc.level(0)
+            c.set_probes([probe_point(r)], capacity=max(r["steps"], 1))
+            if graph_chunk:
+                rc.enable_graphs(graph_chunk)
+            rc.step(r["steps"])
+            c.sync()
+            probe = c.read_probes()[:, 0].copy()
+            return {"checksum": rc.checksum(), "mass": c.mass(), "probe": probe}
     with Case(request_desc(r)) as c:
-        c.init_pulse(r["amp"], r["nx"] / 2.0, r["ny"] / 2.0, r["halfwidth"])
+        c.init_pulse(r["amp"], cx, cy, r["halfwidth"])
         c.set_probes([probe_point(r)], capacity=max(r["steps"], 1))
         if graph_chunk:
             c.enable_graphs(graph_chunk)

@@ -14,8 +14,10 @@ algorithmic bytes of the bulk kernel, 2*19*8 = 304 B (float64) and 2*19*4 =
 no boundary pass on a periodic grid the step is one kernel launch.
 
 Serving (reported beside, in "serving"): the acoustic-pulse request mix of
-acpulse_vclbm_amd.serve (nx, ny in [128, 1024], nz = 3, LODI, five collision
-operators, float64 / float32-shifted), 48 requests with a quarter of the step
+acpulse_vclbm_amd.serve (nx, ny in [128, 1024], nz = 3, five collision
+operators, float64 / float32-shifted, half of them flagged for a 2:1 refined
+region: one-level requests on LODI grids, refined ones as two-level cases on a
+periodic coarse grid), 48 requests with a quarter of the step
 counts by default (--serve-requests 256 --serve-step-scale 1 for the full mix).
 Sequential float64 capacity and the concurrent server's capacity are measured
 with every request arriving at once; Poisson arrivals (seed 1) at 1.2x the
@@ -89,6 +91,16 @@ def copy_ceiling(acp, torch, n, reps=5):
     return {"planes19_gbs": nbytes / (ms19 * 1e-3) / 1e9, "planes1_gbs": nbytes / (ms1 * 1e-3) / 1e9}
 
 
+def node_updates_per_step(serve, r):
+    """Node updates of one (coarse) step of a request: the grid, or for a
+    refined request the coarse grid plus two steps of the fine level."""
+    n = r["nx"] * r["ny"] * r["nz"]
+    if serve.two_level(r):
+        x0, x1 = serve.fine_region(r)
+        n += 2 * (2 * (x1 - x0) + 1) * (2 * r["ny"]) * (2 * r["nz"])
+    return n
+
+
 def serving(acp, n_req, step_scale, max_active, graph_chunk):
     """Serving workload: `n_req` acoustic-pulse requests of the request mix
     (seed 1; steps scaled by `step_scale`).  First the same requests, all in
@@ -120,8 +132,8 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
                    "peak_active": mix["peak_active"], "max_active": max_active, "graph_chunk": graph_chunk},
         "sequential_f64_same_arrivals": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in seq.items()
                                          if k in keep},
-        "refined_requests_run_uniform": sum(r["refined"] for r in reqs),
-        "node_steps": sum(r["nx"] * r["ny"] * r["nz"] * r["steps"] for r in reqs),
+        "mix": serve.mix_counts(reqs),
+        "node_steps": sum(node_updates_per_step(serve, r) * r["steps"] for r in reqs),
     }
 
 

@@ -225,6 +225,18 @@ int acp_refined_set_filter(acp_refined* r, int32_t filter);
  * last part of acp_refined_step; for tests and for a caller that edits a
  * level's state). */
 int acp_refined_restrict(acp_refined* r);
+/* Placement and graphs, as acp_case_footprint / acp_case_create_in /
+ * acp_case_enable_graphs for a single case (no counterpart in the reference):
+ * the bytes a refined case needs with `nprobe` probes of `probe_capacity`
+ * coarse steps on its coarse level; creation inside a caller's 256-byte
+ * aligned device region of at least that size, on the caller's stream (fine
+ * level first, coarse level after it); and captured graphs of `chunk` coarse
+ * steps (even; 0 turns them off). */
+int acp_refined_footprint(const acp_case_desc* coarse, int32_t x0, int32_t x1, int32_t nprobe, int32_t probe_capacity,
+                          int64_t* bytes);
+int acp_refined_create_in(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, void* region,
+                          int64_t region_bytes, acp_refined** out);
+int acp_refined_enable_graphs(acp_refined* r, int32_t chunk);
 
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference, which runs one case per process
@@ -261,6 +273,9 @@ typedef struct acp_request {
     double* probe_out;          /* host buffer for `steps` densities, or NULL; must stay valid */
     double arrival;             /* seconds on the server clock; not admitted earlier */
     uint64_t tag;               /* caller's identifier, returned in the result */
+    int32_t refine[2];          /* fine region [x0, x1] of a two-level case (acp_refined_*; periodic
+                                   coarse level, pulse initial condition); refine[1] <= refine[0]: one level */
+    int32_t f2c_filter;         /* ACP_F2C_* of a two-level case */
 } acp_request;
 
 typedef struct acp_result {
@@ -269,8 +284,8 @@ typedef struct acp_result {
     int32_t steps;              /* steps launched */
     int64_t device_bytes;       /* footprint the case held */
     double t_arrival, t_submit, t_admit, t_start, t_done;  /* server clock, seconds */
-    double mass;                /* acp_case_mass of the final state */
-    uint64_t checksum;          /* acp_case_checksum of the final state */
+    double mass;                /* acp_case_mass of the final state (coarse level of a two-level case) */
+    uint64_t checksum;          /* acp_case_checksum of the final state (two levels: coarse + fine, mod 2^64) */
     char error[128];
 } acp_result;
 

@@ -57,6 +57,38 @@ def test_graph_replays_match_plain_steps(gpu, spec):
         assert np.array_equal(pa, pb)
 
 
+@pytest.mark.parametrize("spec", [dict(op="RR", precision="F64"), dict(op="SRT", precision="F32-SHIFTED"),
+                                  dict(op="MRT-RM", precision="F64", mrt_mode="RELAXED", lag=True)],
+                         ids=["rr-f64", "srt-f32", "mrtrm-f64-lag"])
+def test_refined_graph_replays_match_plain_steps(gpu, spec):
+    """Refined cases replayed from captured graphs of 4 coarse steps (both
+    levels' parities, odd remainders, synchronisation between replays) against
+    plain refined steps: both levels bit-identical, probe series equal; and the
+    refined footprint is what the two levels allocate."""
+    spec = dict(spec)
+    lag = spec.pop("lag", False)
+    d = acp.make_desc(32, 12, 3, tau=0.56, boundary="PERIODIC", **spec)
+    with acp.Refined(d, 16, 32) as a, acp.Refined(d, 16, 32) as b:
+        for r in (a, b):
+            if lag:
+                r.set_filter("LAG")
+            r.init_pulse(1e-3, 8.0, 6.0, 3.0)
+            r.level(0).set_probes([(24, 6, 1)], capacity=100)
+        assert (a.level(0).info().device_bytes + a.level(1).info().device_bytes ==
+                acp.refined_footprint(d, 16, 32, 1, 100))
+        b.enable_graphs(4)
+        plan = [4, 1, 4, 8, 3, 12]
+        for n in plan:
+            a.step(n)
+            b.step(n)
+            a.level(0).sync()
+            b.level(0).sync()
+            assert a.checksum() == b.checksum()
+        for lv in (0, 1):
+            assert np.array_equal(a.level(lv).populations(), b.level(lv).populations())
+        assert np.array_equal(a.level(0).read_probes(), b.level(0).read_probes())
+
+
 def test_graph_chunk_argument_checks(gpu):
     d = acp.make_desc(10, 10, 3, tau=0.6)
     with acp.Case(d) as c:
@@ -74,7 +106,10 @@ def _small_mix(n, seed):
 
 
 def test_server_results_equal_alone(gpu):
+    """The mix holds one-level and two-level requests (seed 5: refined requests
+    among them); each result equals the same request run alone."""
     reqs = _small_mix(12, seed=5)
+    assert 0 < sum(serve.two_level(r) for r in reqs) < len(reqs)
     alone = [serve.run_alone(r) for r in reqs]
     results, summary = serve.run_mix(reqs, hbm_budget_bytes=1 << 30, max_active=4, graph_chunk=8, chunks_ahead=2,
                                      timeout_s=120)
@@ -93,7 +128,7 @@ def test_server_results_equal_alone(gpu):
 
 def test_server_admits_by_footprint(gpu):
     reqs = _small_mix(8, seed=9)
-    fps = [acp.footprint(serve.request_desc(r), 1, r["steps"]) for r in reqs]
+    fps = [serve.footprint(r) for r in reqs]
     budget = (2 * max(fps) + 255) // 256 * 256  # room for at most two of the largest at once
     alone = [serve.run_alone(r) for r in reqs]
     results, summary = serve.run_mix(reqs, hbm_budget_bytes=budget, max_active=8, graph_chunk=0, launch_steps=5,

@@ -80,3 +80,38 @@ def test_footprint_rejects_what_create_rejects():
     d = acp.make_desc(16, 16, 3, op="SRT", tau=0.6, precision="F32-SHIFTED", cubic=True)
     with pytest.raises(_capi.AcpError):
         acp.footprint(d)
+
+
+def test_refined_footprint_is_both_levels():
+    """A refined case reserves its fine level (half spacing, 2(x1-x0)+1 x 2ny x
+    2nz, open x faces) plus its coarse level with the probes."""
+    d = acp.make_desc(64, 40, 3, op="RR", tau=0.56, boundary="PERIODIC")
+    fd = acp.make_desc(65, 80, 6, op="RR", nu=d.nu, spacing=d.spacing / 2, boundary="OPEN-X")
+    assert acp.refined_footprint(d, 32, 64, 1, 500) == _fp(fd) + _fp(d, 1, 500)
+    for bad in ((0, 32), (10, 11), (1, 64), (40, 65)):
+        with pytest.raises(_capi.AcpError):
+            acp.refined_footprint(d, *bad)
+    with pytest.raises(_capi.AcpError) as e:
+        acp.refined_footprint(acp.make_desc(64, 40, 3, op="RR", tau=0.56), 32, 64)
+    assert e.value.code == _capi.ACP_E_UNSUPPORTED
+
+
+def test_refined_requests_in_the_mix():
+    """Half of the requests are flagged for refinement; those not running HRR
+    become two-level cases on a periodic coarse grid, with the fine level over
+    the upper half in x and the pulse in the lower half."""
+    reqs = serve.make_requests(256, seed=1)
+    c = serve.mix_counts(reqs)
+    assert c["refined_flag"] == c["two_level"] + c["refined_hrr_one_level"]
+    assert 0.25 < c["two_level"] / len(reqs) < 0.55
+    for r in reqs:
+        d = serve.request_desc(r)
+        if serve.two_level(r):
+            assert d.boundary == _capi.BOUNDARY["PERIODIC"]
+            x0, x1 = serve.fine_region(r)
+            assert (x0, x1) == (r["nx"] // 2, r["nx"])
+            assert serve.pulse_centre(r)[0] < x0 <= serve.probe_point(r)[0]
+            assert serve.footprint(r) == acp.refined_footprint(d, x0, x1, 1, r["steps"])
+        else:
+            assert d.boundary == _capi.BOUNDARY["LODI"]
+            assert serve.footprint(r) == _fp(d, 1, r["steps"])
