@@ -271,12 +271,14 @@ def refined_footprint(desc, x0, x1, nprobe=0, probe_capacity=0):
     return b.value
 
 
-def copy_probe(planes, plane_elems, reps=5, stream=None):
+def copy_probe(planes, plane_elems, reps=5, stream=None, block_nodes=0):
     """Mean milliseconds of one device copy of `planes` (1 or 19) float64
-    planes of `plane_elems` elements in the engine's layout (acp_copy_probe)."""
+    planes of `plane_elems` elements, whole planes or (19 planes) in blocks of
+    `block_nodes` nodes (acp_copy_probe)."""
     L = _capi.lib()
     ms = ctypes.c_double()
-    _capi.check(L.acp_copy_probe(int(planes), int(plane_elems), int(reps), stream, ctypes.byref(ms)))
+    _capi.check(L.acp_copy_probe(int(planes), int(plane_elems), int(block_nodes), int(reps), stream,
+                                 ctypes.byref(ms)))
     return ms.value
 
 

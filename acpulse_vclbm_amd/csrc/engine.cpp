@@ -157,6 +157,7 @@ void kshift_init(KShift& k, const acp_case_desc& d) {
     k.inv_len[2] = 1.0f / (float)d.nz;
     k.rho0 = d.rho0;
     k.c_si = k64.c;
+    k.k6 = (float)(1. / (6. * k64.c * k64.c));
 }
 
 LodiSchedule build_lodi_schedule(int nx, int ny, int nz) {
@@ -376,24 +377,15 @@ int restart(acp_case* c) {
     return ACP_OK;
 }
 
-// Upload host float64 populations [19][N] (SI units) into both arrays.
+// Upload host float64 populations [19][N] (SI units) into both arrays: one
+// q-plane segment at a time through the staging buffer, placed in the block
+// layout (and converted for the shifted tiers) on the device.
 int upload_f64(acp_case* c, const double* f) {
     const int64_t N = c->g.N;
-    if (!shifted(c)) {
-        for (int q = 0; q < 19; ++q) {
-            HIPCHK(hipMemcpyAsync(c->A + (size_t)q * N, f + (size_t)q * N, sizeof(double) * N, hipMemcpyHostToDevice,
-                                  c->stream));
-            HIPCHK(hipMemcpyAsync(c->B + (size_t)q * N, c->A + (size_t)q * N, sizeof(double) * N,
-                                  hipMemcpyDeviceToDevice, c->stream));
-        }
-        if (c->psi) HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(double) * 19 * N, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        return ACP_OK;
-    }
-    // staged conversion, one q-plane segment at a time
     const int64_t chunk = c->stage_elems;
     double* stage = c->stage;
-    const size_t eb = elem_bytes(c->precision);
+    void* A = shifted(c) ? c->SA : (void*)c->A;
+    void* B = shifted(c) ? c->SB : (void*)c->B;
     int rc = ACP_OK;
     for (int q = 0; q < 19 && rc == ACP_OK; ++q)
         for (int64_t l0 = 0; l0 < N && rc == ACP_OK; l0 += chunk) {
@@ -401,10 +393,33 @@ int upload_f64(acp_case* c, const double* f) {
             hipError_t e = hipMemcpyAsync(stage, f + (size_t)q * N + l0, sizeof(double) * n, hipMemcpyHostToDevice,
                                           c->stream);
             if (e == hipSuccess)
-                e = acp::launch_to_store(c->precision, c->dks, stage, (char*)c->SA + ((size_t)q * N + l0) * eb,
-                                         (char*)c->SB + ((size_t)q * N + l0) * eb, n, q, c->stream);
+                e = acp::launch_put_plane(c->precision, c->dks, stage, A, B, c->g.L, q, l0, n, c->stream);
             if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
             if (e != hipSuccess) rc = acp::hip_fail(e, "population upload");
+        }
+    if (rc == ACP_OK && c->psi) {
+        HIPCHK(hipMemsetAsync(c->psi, 0, sizeof(double) * 19 * N, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    return rc;
+}
+
+// Download one population array of the case (A when `current`, else B) into
+// host float64 [19][N] (SI units).
+int download_array(acp_case* c, bool current, double* f) {
+    const int64_t N = c->g.N;
+    const int64_t chunk = c->stage_elems;
+    double* stage = c->stage;
+    const void* src = shifted(c) ? (current ? c->SA : c->SB) : (const void*)(current ? c->A : c->B);
+    int rc = ACP_OK;
+    for (int q = 0; q < 19 && rc == ACP_OK; ++q)
+        for (int64_t l0 = 0; l0 < N && rc == ACP_OK; l0 += chunk) {
+            const int64_t n = std::min(chunk, N - l0);
+            hipError_t e = acp::launch_get_plane(c->precision, c->dks, src, stage, c->g.L, q, l0, n, c->stream);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(f + (size_t)q * N + l0, stage, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+            if (e != hipSuccess) rc = acp::hip_fail(e, "population download");
         }
     return rc;
 }
@@ -442,8 +457,14 @@ int init_rho_field(acp_case* c, RhoFn rho_of) {
 
 namespace {
 
-// float64 staging buffer of the shifted tiers' conversions: 1 Mi elements (8 MiB)
+// float64 staging buffer of the host transfers: 1 Mi elements (8 MiB)
 constexpr int64_t kStageElems = (int64_t)1 << 20;
+
+// Nodes per population-layout block by tier (plane.hpp).  Measured with the
+// collide + push kernels on the 512^3 periodic case: blocks of 8192 nodes run
+// 9-12 % faster than whole planes in float64 and 3-7 % in float32; float16
+// ran faster with whole planes.
+int64_t layout_block_nodes(int precision) { return precision == ACP_PREC_F16_SHIFTED ? 0 : 8192; }
 
 int validate_desc(const acp_case_desc& d) {
     if (d.max_level != 0)
@@ -469,16 +490,17 @@ int validate_desc(const acp_case_desc& d) {
 int64_t footprint_bytes(const acp_case_desc& d, int32_t nprobe, int32_t cap) {
     using acp::round_alloc;
     const int64_t N = (int64_t)d.nx * d.ny * d.nz;
+    const int64_t E = acp::make_layout(N, layout_block_nodes(d.precision)).elems;
     const bool need_macro = d.op == ACP_OP_HRR || (d.op == ACP_OP_RR && d.cubic);
     const bool need_psi = d.cubic && (d.op == ACP_OP_RR || d.op == ACP_OP_HRR);
     int64_t b = round_alloc(sizeof(acp::K64));
     if (d.precision == ACP_PREC_F64) {
-        b += 2 * round_alloc(19 * N * 8);
+        b += 2 * round_alloc(E * 8);
         if (need_psi) b += round_alloc(19 * N * 8);
         if (need_macro) b += round_alloc(4 * N * 8);
     } else {
         const int64_t eb = d.precision == ACP_PREC_F32_SHIFTED ? 4 : 2;
-        b += round_alloc(sizeof(acp::KShift)) + 2 * round_alloc(19 * N * eb);
+        b += round_alloc(sizeof(acp::KShift)) + 2 * round_alloc(E * eb);
         if (d.op == ACP_OP_HRR) b += round_alloc(4 * N * 4);
     }
     b += round_alloc(std::min<int64_t>(N, kStageElems) * 8);
@@ -527,6 +549,7 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
     c->arena_size = region_bytes;
     c->d = d;
     c->g = acp::Grid{d.nx, d.ny, d.nz, (int64_t)d.nx * d.ny * d.nz, d.boundary == ACP_BC_OPEN_X ? 1 : 0};
+    c->g.L = acp::make_layout(c->g.N, layout_block_nodes(d.precision));
     c->precision = d.precision;
     c->periodic = d.boundary != ACP_BC_LODI;
     switch (d.op) {
@@ -558,8 +581,8 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
         return code;
     };
     if (!shifted(c)) {
-        if ((rc = dalloc(c, &c->dk, 1)) || (rc = dalloc(c, &c->A, (size_t)19 * N)) ||
-            (rc = dalloc(c, &c->B, (size_t)19 * N)))
+        if ((rc = dalloc(c, &c->dk, 1)) || (rc = dalloc(c, &c->A, (size_t)c->g.L.elems)) ||
+            (rc = dalloc(c, &c->B, (size_t)c->g.L.elems)))
             return bail(rc);
         if (c->need_psi && (rc = dalloc(c, &c->psi, (size_t)19 * N))) return bail(rc);
         if (c->need_macro && (rc = dalloc(c, &c->M, (size_t)4 * N))) return bail(rc);
@@ -568,8 +591,8 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
     } else {
         const size_t eb = elem_bytes(c->precision);
         char *a = nullptr, *b = nullptr;
-        if ((rc = dalloc(c, &c->dks, 1)) || (rc = dalloc(c, &a, (size_t)19 * N * eb)) ||
-            (rc = dalloc(c, &b, (size_t)19 * N * eb)))
+        if ((rc = dalloc(c, &c->dks, 1)) || (rc = dalloc(c, &a, (size_t)c->g.L.elems * eb)) ||
+            (rc = dalloc(c, &b, (size_t)c->g.L.elems * eb)))
             return bail(rc);
         c->SA = a;
         c->SB = b;
@@ -592,6 +615,15 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
     if ((rc = dalloc(c, &c->partial, (size_t)c->partial_blocks))) return bail(rc);
     c->stage_elems = std::min<int64_t>(N, kStageElems);
     if ((rc = dalloc(c, &c->stage, (size_t)c->stage_elems))) return bail(rc);
+    // the padding of the last layout block is never written: zero it once, so
+    // that whole-array reductions (the checksum) see fixed values there
+    if (c->g.L.elems > 19 * N) {
+        const size_t bytes = (size_t)c->g.L.elems * (shifted(c) ? elem_bytes(c->precision) : sizeof(double));
+        hipError_t e = hipMemsetAsync(shifted(c) ? c->SA : (void*)c->A, 0, bytes, c->stream);
+        if (e == hipSuccess) e = hipMemsetAsync(shifted(c) ? c->SB : (void*)c->B, 0, bytes, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) return bail(acp::hip_fail(e, "population arrays"));
+    }
     c->probe_mark = c->arena_used;
     c->arr0 = shifted(c) ? c->SA : (const void*)c->A;
     *out = c;
@@ -694,7 +726,6 @@ namespace {
 
 // Launch n steps on the case stream, one kernel sequence per step.
 int step_eager(acp_case* c, int32_t n) {
-    const int64_t N = c->g.N;
     const int nph = c->phase_start.empty() ? 0 : (int)c->phase_start.size() - 1;
     for (int32_t s = 0; s < n; ++s) {
         hipError_t e = hipSuccess;
@@ -705,10 +736,10 @@ int step_eager(acp_case* c, int32_t n) {
                                        c->psi, c->M, c->g, c->stream);
             for (int p = 0; p < nph && e == hipSuccess; ++p)
                 e = acp::launch_lodi64(c->d.order, c->d.cubic != 0, c->d.transverse != 0, c->dk, c->A, c->B, c->psi,
-                                       c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], N,
+                                       c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], c->g.L,
                                        c->stream);
             if (e == hipSuccess && c->nprobe > 0)
-                e = acp::launch_probe64(c->B, N, c->probe_nodes, c->nprobe, c->probe_buf, c->probe_slot,
+                e = acp::launch_probe64(c->B, c->g.L, c->probe_nodes, c->nprobe, c->probe_buf, c->probe_slot,
                                         c->probe_cap, c->stream);
             std::swap(c->A, c->B);
         } else {
@@ -718,11 +749,11 @@ int step_eager(acp_case* c, int32_t n) {
                                            c->SM, c->g, c->stream);
             for (int p = 0; p < nph && e == hipSuccess; ++p)
                 e = acp::launch_lodi_shift(c->precision, c->d.order, c->d.transverse != 0, c->dks, c->SA, c->SB,
-                                           c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], N,
-                                           c->stream);
+                                           c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p],
+                                           c->g.L, c->stream);
             if (e == hipSuccess && c->nprobe > 0)
-                e = acp::launch_probe_shift(c->precision, c->dks, c->SB, N, c->probe_nodes, c->nprobe, c->probe_buf,
-                                            c->probe_slot, c->probe_cap, c->stream);
+                e = acp::launch_probe_shift(c->precision, c->dks, c->SB, c->g.L, c->probe_nodes, c->nprobe,
+                                            c->probe_buf, c->probe_slot, c->probe_cap, c->stream);
             std::swap(c->SA, c->SB);
         }
         if (e != hipSuccess) return acp::hip_fail(e, "acp_case_step launch");
@@ -798,7 +829,7 @@ extern "C" int acp_case_checksum(acp_case* c, uint64_t* out) {
     const void* a = shifted(c) ? c->SA : (const void*)c->A;
     const int eb = shifted(c) ? (int)elem_bytes(c->precision) : 8;
     unsigned long long* part = reinterpret_cast<unsigned long long*>(c->partial);
-    HIPCHK(acp::launch_checksum(a, eb, 19 * c->g.N, part, c->partial_blocks, c->stream));
+    HIPCHK(acp::launch_checksum(a, eb, c->g.L.elems, part, c->partial_blocks, c->stream));
     std::vector<unsigned long long> h(c->partial_blocks);
     HIPCHK(hipMemcpyAsync(h.data(), part, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -817,27 +848,7 @@ extern "C" int acp_case_sync(acp_case* c) {
 extern "C" int acp_case_get_populations(acp_case* c, double* f) {
     if (int rc = check_case(c)) return rc;
     if (!f) return fail(ACP_E_ARG, "null population array");
-    const int64_t N = c->g.N;
-    if (!shifted(c)) {
-        HIPCHK(hipMemcpyAsync(f, c->A, sizeof(double) * 19 * N, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        return ACP_OK;
-    }
-    const int64_t chunk = c->stage_elems;
-    double* stage = c->stage;
-    const size_t eb = elem_bytes(c->precision);
-    int rc = ACP_OK;
-    for (int q = 0; q < 19 && rc == ACP_OK; ++q)
-        for (int64_t l0 = 0; l0 < N && rc == ACP_OK; l0 += chunk) {
-            const int64_t n = std::min(chunk, N - l0);
-            hipError_t e = acp::launch_from_store(c->precision, c->dks, (const char*)c->SA + ((size_t)q * N + l0) * eb,
-                                                  stage, n, q, c->stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(f + (size_t)q * N + l0, stage, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) rc = acp::hip_fail(e, "population download");
-        }
-    return rc;
+    return download_array(c, true, f);
 }
 
 extern "C" int acp_case_get_macros(acp_case* c, double* rho, double* u) {
@@ -859,9 +870,9 @@ extern "C" int acp_case_get_macros(acp_case* c, double* rho, double* u) {
 extern "C" int acp_case_mass(acp_case* c, double* mass) {
     if (int rc = check_case(c)) return rc;
     if (!mass) return fail(ACP_E_ARG, "null output");
-    hipError_t e = shifted(c) ? acp::launch_mass_shift(c->precision, c->dks, c->SA, c->g.N, c->partial,
+    hipError_t e = shifted(c) ? acp::launch_mass_shift(c->precision, c->dks, c->SA, c->g.L, c->partial,
                                                        c->partial_blocks, c->stream)
-                              : acp::launch_mass64(c->A, c->g.N, c->partial, c->partial_blocks, c->stream);
+                              : acp::launch_mass64(c->A, c->g.L, c->partial, c->partial_blocks, c->stream);
     if (e != hipSuccess) return acp::hip_fail(e, "acp_case_mass");
     std::vector<double> h((size_t)c->partial_blocks);
     HIPCHK(hipMemcpyAsync(h.data(), c->partial, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
@@ -942,33 +953,6 @@ extern "C" int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t
             }
     return n;
 }
-
-namespace {
-int download_array(acp_case* c, bool current, double* f) {
-    const int64_t N = c->g.N;
-    if (!shifted(c)) {
-        HIPCHK(hipMemcpyAsync(f, current ? c->A : c->B, sizeof(double) * 19 * N, hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(hipStreamSynchronize(c->stream));
-        return ACP_OK;
-    }
-    const int64_t chunk = c->stage_elems;
-    double* stage = c->stage;
-    const size_t eb = elem_bytes(c->precision);
-    const char* src = (const char*)(current ? c->SA : c->SB);
-    int rc = ACP_OK;
-    for (int q = 0; q < 19 && rc == ACP_OK; ++q)
-        for (int64_t l0 = 0; l0 < N && rc == ACP_OK; l0 += chunk) {
-            const int64_t n = std::min(chunk, N - l0);
-            hipError_t e = acp::launch_from_store(c->precision, c->dks, src + ((size_t)q * N + l0) * eb, stage, n, q,
-                                                  c->stream);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(f + (size_t)q * N + l0, stage, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-            if (e != hipSuccess) rc = acp::hip_fail(e, "population download");
-        }
-    return rc;
-}
-}  // namespace
 
 extern "C" int acp_case_write_vtk(acp_case* c, const char* stem) {
     if (int rc = check_case(c)) return rc;

@@ -6,8 +6,11 @@
 #include <cstdint>
 
 #include "phys64.hpp"
+#include "plane.hpp"
 
 namespace acp {
+
+struct KShift;
 
 // bulk-kernel operator codes
 enum BulkOp : int { OP_SRT = 0, OP_MRT_GS = 1, OP_MRT_RM = 2, OP_RR = 3, OP_HRR = 4, OP_STREAM = 5 };
@@ -24,10 +27,12 @@ struct LodiNode {
 // xopen (periodic grids only): the x faces are not wrapped; their nodes
 // collide like interior nodes and links leaving the grid are dropped (the
 // fine level of a refined case, whose x faces the coarse level sets).
+// L: layout of the population arrays (plane.hpp).
 struct Grid {
     int nx, ny, nz;
     int64_t N;
     int xopen = 0;
+    PopLayout L;
 };
 
 hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64* k, const double* A, double* B,
@@ -35,14 +40,21 @@ hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64
 hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Grid& g, bool periodic,
                           hipStream_t s);
 hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, const double* A, double* B,
-                         const double* psi, const LodiNode* nodes, int64_t count, int64_t N, hipStream_t s);
+                         const double* psi, const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s);
 hipError_t launch_init_random64(int order, const K64* k, double* A, double* B, const Grid& g, uint64_t seed,
                                double rho0, double rho_amp, double u_amp, hipStream_t s);
-hipError_t launch_probe64(const double* A, int64_t N, const int64_t* nodes, int nprobe, double* out,
+hipError_t launch_probe64(const double* A, const PopLayout& L, const int64_t* nodes, int nprobe, double* out,
                           int32_t* slot, int32_t cap, hipStream_t s);
 hipError_t launch_checksum(const void* a, int elem_bytes, int64_t n, unsigned long long* partial, int nblocks,
                            hipStream_t s);
-hipError_t launch_mass64(const double* A, int64_t N, double* partial, int nblocks, hipStream_t s);
+hipError_t launch_mass64(const double* A, const PopLayout& L, double* partial, int nblocks, hipStream_t s);
+// one population plane q of the node range [l0, l0 + n) between a contiguous
+// float64 buffer and the arrays of a case of any tier (converted to and from
+// the shifted storage for those tiers; dB may be null)
+hipError_t launch_put_plane(int prec, const KShift* k, const double* src, void* dA, void* dB, const PopLayout& L,
+                            int q, int64_t l0, int64_t n, hipStream_t s);
+hipError_t launch_get_plane(int prec, const KShift* k, const void* src, double* dst, const PopLayout& L, int q,
+                            int64_t l0, int64_t n, hipStream_t s);
 hipError_t launch_macros_out64(const K64* k, const double* A, const Grid& g, double* rho, double* u,
                                hipStream_t s);
 

@@ -5,7 +5,15 @@
 
 namespace acp {
 
-__device__ __forceinline__ void load19(const double* __restrict__ F, int64_t N, int64_t l, double f[19]) {
+__device__ __forceinline__ void load19(const double* __restrict__ F, const PopLayout& L, int64_t l, double f[19]) {
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = F[pidx(L, q, l)];
+    });
+}
+
+// 19 planes of N values (the cubic-Mach terms psi)
+__device__ __forceinline__ void load19_soa(const double* __restrict__ F, int64_t N, int64_t l, double f[19]) {
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         f[q] = F[q * N + l];
@@ -20,7 +28,7 @@ __device__ __forceinline__ void load19(const double* __restrict__ F, int64_t N, 
 template <int ORDER, bool CUBIC>
 __global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const double* __restrict__ A,
                                               double* __restrict__ B, const double* __restrict__ psi,
-                                              const LodiNode* __restrict__ nodes, int64_t count, int64_t N,
+                                              const LodiNode* __restrict__ nodes, int64_t count, const PopLayout PL,
                                               int transverse) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= count) return;
@@ -28,18 +36,18 @@ __global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const 
     const LodiNode ln = nodes[s];
     double f[19], ps[19];
     double rb, vb[3], r1, v1[3], r2, v2[3];
-    load19(A, N, ln.node, f);
+    load19(A, PL, ln.node, f);
     moments64(k, f, rb, vb[0], vb[1], vb[2]);
-    load19(B, N, ln.n1, f);
+    load19(B, PL, ln.n1, f);
     if constexpr (CUBIC) {
-        load19(psi, N, ln.n1, ps);
+        load19_soa(psi, PL.N, ln.n1, ps);
         moments64_psi(k, f, ps, r1, v1[0], v1[1], v1[2]);
     } else {
         moments64(k, f, r1, v1[0], v1[1], v1[2]);
     }
-    load19(B, N, ln.n2, f);
+    load19(B, PL, ln.n2, f);
     if constexpr (CUBIC) {
-        load19(psi, N, ln.n2, ps);
+        load19_soa(psi, PL.N, ln.n2, ps);
         moments64_psi(k, f, ps, r2, v2[0], v2[1], v2[2]);
     } else {
         moments64(k, f, r2, v2[0], v2[1], v2[2]);
@@ -55,7 +63,7 @@ __global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const 
         double rt[4], vt[4][3];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            load19(A, N, ln.t[t], f);
+            load19(A, PL, ln.t[t], f);
             moments64(k, f, rt[t], vt[t][0], vt[t][1], vt[t][2]);
         }
         dT[0] = (rt[0] - rt[1]) / (2. * ls);
@@ -115,17 +123,17 @@ __global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const 
     feq64<ORDER>(k, rho, u[0], u[1], u[2], eq);
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        B[q * N + ln.node] = eq[q];
+        B[pidx(PL, q, ln.node)] = eq[q];
     });
 }
 
 hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, const double* A, double* B,
-                         const double* psi, const LodiNode* nodes, int64_t count, int64_t N, hipStream_t s) {
+                         const double* psi, const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     const int bs = 256;
     dim3 grid((unsigned)((count + bs - 1) / bs));
     const int tr = transverse ? 1 : 0;
-#define L64(O, C) hipLaunchKernelGGL((lodi64<O, C>), grid, dim3(bs), 0, s, k, A, B, psi, nodes, count, N, tr)
+#define L64(O, C) hipLaunchKernelGGL((lodi64<O, C>), grid, dim3(bs), 0, s, k, A, B, psi, nodes, count, L, tr)
     if (cubic) {
         if (order == 2) L64(2, true);
         else if (order == 3) L64(3, true);
@@ -143,11 +151,12 @@ hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, c
 // Pre-collision density and velocity of every node (calcDensityPreCol /
 // calcVelocityPreCol, Node.cpp:32-51, 121-154) for the HRR strain stencil.
 __global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                               double* __restrict__ M, int64_t N) {
+                                               double* __restrict__ M, const PopLayout L) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = L.N;
     if (l >= N) return;
     double f[19], r, ux, uy, uz;
-    load19(A, N, l, f);
+    load19(A, L, l, f);
     moments64(*kp, f, r, ux, uy, uz);
     M[l] = r;
     M[N + l] = ux;
@@ -157,17 +166,18 @@ __global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const
 
 hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Grid& g, bool, hipStream_t s) {
     const int bs = 256;
-    hipLaunchKernelGGL(macro64, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, k, A, macro, g.N);
+    hipLaunchKernelGGL(macro64, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, k, A, macro, g.L);
     return hipGetLastError();
 }
 
 // macros in the output layout rho[N], u[3][N]
 __global__ __launch_bounds__(256) void macros_out64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                                    double* __restrict__ rho, double* __restrict__ u, int64_t N) {
+                                                    double* __restrict__ rho, double* __restrict__ u, const PopLayout L) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = L.N;
     if (l >= N) return;
     double f[19], r, ux, uy, uz;
-    load19(A, N, l, f);
+    load19(A, L, l, f);
     moments64(*kp, f, r, ux, uy, uz);
     rho[l] = r;
     u[l] = ux;
@@ -178,7 +188,7 @@ __global__ __launch_bounds__(256) void macros_out64(const K64* __restrict__ kp, 
 hipError_t launch_macros_out64(const K64* k, const double* A, const Grid& g, double* rho, double* u,
                                hipStream_t s) {
     const int bs = 256;
-    hipLaunchKernelGGL(macros_out64, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, k, A, rho, u, g.N);
+    hipLaunchKernelGGL(macros_out64, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, k, A, rho, u, g.L);
     return hipGetLastError();
 }
 
@@ -203,10 +213,10 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 
 template <int ORDER>
 __global__ __launch_bounds__(256) void init_random64(const K64* __restrict__ kp, double* __restrict__ A,
-                                                     double* __restrict__ B, int64_t N, uint64_t seed, double rho0,
+                                                     double* __restrict__ B, const PopLayout L, uint64_t seed, double rho0,
                                                      double rho_amp, double u_amp) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= N) return;
+    if (l >= L.N) return;
     uint32_t c[4] = {(uint32_t)l, (uint32_t)((uint64_t)l >> 32), 0u, 0u};
     philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     double uu[4];
@@ -223,8 +233,8 @@ __global__ __launch_bounds__(256) void init_random64(const K64* __restrict__ kp,
     feq64<ORDER>(k, rho, us * n1, us * n2, us * n3, eq);
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        A[q * N + l] = eq[q];
-        B[q * N + l] = eq[q];
+        A[pidx(L, q, l)] = eq[q];
+        B[pidx(L, q, l)] = eq[q];
     });
 }
 
@@ -232,9 +242,9 @@ hipError_t launch_init_random64(int order, const K64* k, double* A, double* B, c
                                double rho0, double rho_amp, double u_amp, hipStream_t s) {
     const int bs = 256;
     dim3 grid((unsigned)((g.N + bs - 1) / bs));
-    if (order == 2) hipLaunchKernelGGL(init_random64<2>, grid, dim3(bs), 0, s, k, A, B, g.N, seed, rho0, rho_amp, u_amp);
-    else if (order == 3) hipLaunchKernelGGL(init_random64<3>, grid, dim3(bs), 0, s, k, A, B, g.N, seed, rho0, rho_amp, u_amp);
-    else hipLaunchKernelGGL(init_random64<4>, grid, dim3(bs), 0, s, k, A, B, g.N, seed, rho0, rho_amp, u_amp);
+    if (order == 2) hipLaunchKernelGGL(init_random64<2>, grid, dim3(bs), 0, s, k, A, B, g.L, seed, rho0, rho_amp, u_amp);
+    else if (order == 3) hipLaunchKernelGGL(init_random64<3>, grid, dim3(bs), 0, s, k, A, B, g.L, seed, rho0, rho_amp, u_amp);
+    else hipLaunchKernelGGL(init_random64<4>, grid, dim3(bs), 0, s, k, A, B, g.L, seed, rho0, rho_amp, u_amp);
     return hipGetLastError();
 }
 
@@ -246,7 +256,7 @@ hipError_t launch_init_random64(int order, const K64* k, double* A, double* B, c
 template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void init_rows(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                                  const double* __restrict__ rho_rows, int j0, int rows, int nx, int ny,
-                                                 int nz, T* __restrict__ A, T* __restrict__ B, int64_t N) {
+                                                 int nz, T* __restrict__ A, T* __restrict__ B, const PopLayout L) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t band = (int64_t)rows * nx;
     if (t >= band * nz) return;
@@ -259,8 +269,8 @@ __global__ __launch_bounds__(256) void init_rows(const K64* __restrict__ kp, con
         T v;
         if constexpr (sizeof(T) == 8) v = eq[q];
         else v = (T)((double)ksp->s * (eq[q] / ksp->rho0 - weight(q)));
-        A[q * N + l] = v;
-        B[q * N + l] = v;
+        A[pidx(L, q, l)] = v;
+        B[pidx(L, q, l)] = v;
     });
 }
 
@@ -268,7 +278,7 @@ __global__ __launch_bounds__(256) void init_rows(const K64* __restrict__ kp, con
 template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void init_nodes(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                                   const double* __restrict__ rho, int64_t l0, int64_t n,
-                                                  T* __restrict__ A, T* __restrict__ B, int64_t N) {
+                                                  T* __restrict__ A, T* __restrict__ B, const PopLayout L) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t l = l0 + t;
@@ -279,8 +289,8 @@ __global__ __launch_bounds__(256) void init_nodes(const K64* __restrict__ kp, co
         T v;
         if constexpr (sizeof(T) == 8) v = eq[q];
         else v = (T)((double)ksp->s * (eq[q] / ksp->rho0 - weight(q)));
-        A[q * N + l] = v;
-        B[q * N + l] = v;
+        A[pidx(L, q, l)] = v;
+        B[pidx(L, q, l)] = v;
     });
 }
 
@@ -288,7 +298,7 @@ hipError_t launch_init_nodes(int prec, int order, const K64* k, const KShift* ks
                              int64_t n, const Grid& g, void* A, void* B, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const dim3 grid((unsigned)((n + 255) / 256)), block(256);
-#define IN(T, O) hipLaunchKernelGGL((init_nodes<T, O>), grid, block, 0, s, k, ks, rho, l0, n, (T*)A, (T*)B, g.N)
+#define IN(T, O) hipLaunchKernelGGL((init_nodes<T, O>), grid, block, 0, s, k, ks, rho, l0, n, (T*)A, (T*)B, g.L)
 #define INO(T)                     \
     if (order == 2) IN(T, 2);      \
     else if (order == 3) IN(T, 3); \
@@ -308,7 +318,7 @@ hipError_t launch_init_rows(int prec, int order, const K64* k, const KShift* ks,
     const dim3 grid((unsigned)((n + 255) / 256)), block(256);
 #define IR(T, O)                                                                                                  \
     hipLaunchKernelGGL((init_rows<T, O>), grid, block, 0, s, k, ks, rho_rows, j0, rows, g.nx, g.ny, g.nz, (T*)A, \
-                       (T*)B, g.N)
+                       (T*)B, g.L)
 #define IRO(T)                     \
     if (order == 2) IR(T, 2);      \
     else if (order == 3) IR(T, 3); \
@@ -326,7 +336,7 @@ hipError_t launch_init_rows(int prec, int order, const K64* k, const KShift* ks,
 // `slot` of the series (if below the capacity), and thread 0 advances the
 // counter after the barrier.  Keeping the row index on the device lets a
 // captured graph of several steps write a new row on every replay.
-__global__ __launch_bounds__(1024) void probe64(const double* __restrict__ A, int64_t N,
+__global__ __launch_bounds__(1024) void probe64(const double* __restrict__ A, const PopLayout L,
                                                 const int64_t* __restrict__ nodes, int nprobe,
                                                 double* __restrict__ out, int32_t* __restrict__ slot, int32_t cap) {
     const int32_t row = *slot;
@@ -334,18 +344,18 @@ __global__ __launch_bounds__(1024) void probe64(const double* __restrict__ A, in
         const int64_t l = nodes[p];
         double r = 0.;
 #pragma unroll
-        for (int q = 0; q < 19; ++q) r += A[q * N + l];
+        for (int q = 0; q < 19; ++q) r += A[pidx(L, q, l)];
         if (row < cap) out[(int64_t)row * nprobe + p] = r;
     }
     __syncthreads();
     if (threadIdx.x == 0) *slot = row + 1;
 }
 
-hipError_t launch_probe64(const double* A, int64_t N, const int64_t* nodes, int nprobe, double* out,
+hipError_t launch_probe64(const double* A, const PopLayout& L, const int64_t* nodes, int nprobe, double* out,
                           int32_t* slot, int32_t cap, hipStream_t s) {
     if (nprobe <= 0) return hipSuccess;
     const int bs = nprobe >= 1024 ? 1024 : ((nprobe + 63) / 64) * 64;
-    hipLaunchKernelGGL(probe64, dim3(1), dim3(bs), 0, s, A, N, nodes, nprobe, out, slot, cap);
+    hipLaunchKernelGGL(probe64, dim3(1), dim3(bs), 0, s, A, L, nodes, nprobe, out, slot, cap);
     return hipGetLastError();
 }
 
@@ -394,13 +404,14 @@ hipError_t launch_checksum(const void* a, int elem_bytes, int64_t n, unsigned lo
 }
 
 // per-block partial sums of sum_q f_q in a fixed order (deterministic)
-__global__ __launch_bounds__(256) void mass64(const double* __restrict__ A, int64_t N, double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void mass64(const double* __restrict__ A, const PopLayout L,
+                                              double* __restrict__ partial) {
     __shared__ double sh[256];
     double acc = 0.;
-    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < N; l += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < L.N; l += (int64_t)gridDim.x * blockDim.x) {
         double r = 0.;
 #pragma unroll
-        for (int q = 0; q < 19; ++q) r += A[q * N + l];
+        for (int q = 0; q < 19; ++q) r += A[pidx(L, q, l)];
         acc += r;
     }
     sh[threadIdx.x] = acc;
@@ -412,8 +423,64 @@ __global__ __launch_bounds__(256) void mass64(const double* __restrict__ A, int6
     if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
-hipError_t launch_mass64(const double* A, int64_t N, double* partial, int nblocks, hipStream_t s) {
-    hipLaunchKernelGGL(mass64, dim3(nblocks), dim3(256), 0, s, A, N, partial);
+hipError_t launch_mass64(const double* A, const PopLayout& L, double* partial, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL(mass64, dim3(nblocks), dim3(256), 0, s, A, L, partial);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ host transfers
+// Population plane q of nodes [l0, l0 + n) between a contiguous float64 buffer
+// (SI values) and a case's arrays: float64 copied, the shifted tiers converted
+// as (T)(s * (f / rho0 - w_q)) and back as rho0 * (w_q + g / s).
+template <typename T>
+__global__ __launch_bounds__(256) void put_plane(const KShift* __restrict__ kp, const double* __restrict__ src,
+                                                 T* __restrict__ dA, T* __restrict__ dB, const PopLayout L, int q,
+                                                 int64_t l0, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    T v;
+    if constexpr (sizeof(T) == 8) v = src[t];
+    else v = (T)((double)kp->s * (src[t] / kp->rho0 - weight(q)));
+    const int64_t e = pidx(L, q, l0 + t);
+    dA[e] = v;
+    if (dB) dB[e] = v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void get_plane(const KShift* __restrict__ kp, const T* __restrict__ src,
+                                                 double* __restrict__ dst, const PopLayout L, int q, int64_t l0,
+                                                 int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const T v = src[pidx(L, q, l0 + t)];
+    if constexpr (sizeof(T) == 8) dst[t] = v;
+    else dst[t] = kp->rho0 * (weight(q) + (double)v / (double)kp->s);
+}
+
+hipError_t launch_put_plane(int prec, const KShift* k, const double* src, void* dA, void* dB, const PopLayout& L,
+                            int q, int64_t l0, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (prec == ACP_PREC_F64)
+        hipLaunchKernelGGL(put_plane<double>, grid, block, 0, s, k, src, (double*)dA, (double*)dB, L, q, l0, n);
+    else if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(put_plane<float>, grid, block, 0, s, k, src, (float*)dA, (float*)dB, L, q, l0, n);
+    else
+        hipLaunchKernelGGL(put_plane<_Float16>, grid, block, 0, s, k, src, (_Float16*)dA, (_Float16*)dB, L, q, l0,
+                           n);
+    return hipGetLastError();
+}
+
+hipError_t launch_get_plane(int prec, const KShift* k, const void* src, double* dst, const PopLayout& L, int q,
+                            int64_t l0, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (prec == ACP_PREC_F64)
+        hipLaunchKernelGGL(get_plane<double>, grid, block, 0, s, k, (const double*)src, dst, L, q, l0, n);
+    else if (prec == ACP_PREC_F32_SHIFTED)
+        hipLaunchKernelGGL(get_plane<float>, grid, block, 0, s, k, (const float*)src, dst, L, q, l0, n);
+    else
+        hipLaunchKernelGGL(get_plane<_Float16>, grid, block, 0, s, k, (const _Float16*)src, dst, L, q, l0, n);
     return hipGetLastError();
 }
 

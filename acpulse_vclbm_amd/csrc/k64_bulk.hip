@@ -4,6 +4,7 @@
 // that every cache line of the destination arrays is written by a single
 // workgroup: the x-shifted stores of a row stay inside that row.  A (time t) is
 // read once, B (time t+1) written once: 2*19*8 = 304 bytes per node and step.
+// Populations are addressed through the case's block layout (plane.hpp).
 //
 // Semantics follow Lattice::timeStepping* (Lattice.cpp:1524-1580): interior
 // nodes collide in place and push all 18 moving populations to their neighbours
@@ -22,7 +23,7 @@ template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
-                                                     int64_t N, int xopen, int row_lds) {
+                                                     const PopLayout L, int xopen, int row_lds) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     double f[19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = P::at(A, q, l, N);
+        f[q] = P::at(A, q, l, L);
     });
 
     int bi = 0, bj = 0, bk = 0;
@@ -66,7 +67,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             if constexpr (CUBIC) {
                 static_for<0, 19>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    ps[q] = P::at(psi, q, l, N);
+                    ps[q] = P::soa(psi, q, l, L.N);
                 });
             }
             double nb[6][4];
@@ -75,14 +76,14 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
 #pragma unroll
                 for (int m = 0; m < 6; ++m)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) nb[m][v] = P::at(M, v, (I)(l + o[m]), N);
+                    for (int v = 0; v < 4; ++v) nb[m][v] = P::soa(M, v, (I)(l + o[m]), L.N);
             }
             collide_rr64<ORDER, OP == OP_HRR, CUBIC>(k, f, ps, nb);
             if constexpr (OP == OP_HRR || CUBIC) {
                 if (psi != nullptr) {
                     static_for<0, 19>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        P::at(psi, q, l, N) = ps[q];
+                        P::soa(psi, q, l, L.N) = ps[q];
                     });
                 }
             }
@@ -109,16 +110,16 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                 if constexpr (EZ[q] > 0) base += ozp;
                 if constexpr (EZ[q] < 0) base += ozm;
                 if constexpr (EX[q] == 0) {
-                    P::at(B, q, (I)(base + (I)i), N) = f[q];
+                    P::at(B, q, (I)(base + (I)i), L) = f[q];
                 } else {
                     __syncthreads();
                     row[(i + EX[q] + nx) % nx] = f[q];
                     __syncthreads();
                     const int src = i - EX[q];
-                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), N) = row[i];
+                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), L) = row[i];
                 }
             });
-            P::at(B, 18, l, N) = f[18];
+            P::at(B, 18, l, L) = f[18];
             return;
         }
     }
@@ -142,9 +143,9 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if constexpr (EY[q] < 0) d += oym;
         if constexpr (EZ[q] > 0) d += ozp;
         if constexpr (EZ[q] < 0) d += ozm;
-        P::at(B, q, d, N) = f[q];
+        P::at(B, q, d, L) = f[q];
     });
-    P::at(B, 18, l, N) = f[18];
+    P::at(B, 18, l, L) = f[18];
 }
 
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC>
@@ -153,12 +154,12 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
     const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     const int row_lds = PERIODIC && g.nx == bx;
-    if (narrow_planes(g.N, sizeof(double)))
+    if (narrow_planes(g.L, sizeof(double), sizeof(double)))
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, true>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.N, g.xopen, row_lds);
+                           g.ny, g.nz, g.L, g.xopen, row_lds);
     else
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, false>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.N, g.xopen, row_lds);
+                           g.ny, g.nz, g.L, g.xopen, row_lds);
     return hipGetLastError();
 }
 

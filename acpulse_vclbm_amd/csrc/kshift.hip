@@ -17,10 +17,11 @@ __device__ __forceinline__ float ld(const T* __restrict__ p) {
 }
 
 template <typename T>
-__device__ __forceinline__ void load_dev(const T* __restrict__ F, int64_t N, int64_t l, float inv_s, float d[19]) {
+__device__ __forceinline__ void load_dev(const T* __restrict__ F, const PopLayout& L, int64_t l, float inv_s,
+                                         float d[19]) {
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        d[q] = (float)F[q * N + l] * inv_s;
+        d[q] = (float)F[pidx(L, q, l)] * inv_s;
     });
 }
 
@@ -42,7 +43,7 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
     moments_dev(d, dr, ux, uy, uz);
     float eq[19];
     if constexpr (OP == OP_MRT_RM) rm_feq_dev(dr, ux, uy, uz, eq);
-    else if constexpr (OP != OP_MRT_GS) feq_dev<ORDER>(dr, ux, uy, uz, eq);
+    else if constexpr (OP != OP_MRT_GS) feq_dev<ORDER>(dr, ux, uy, uz, eq, k.k6);
     const float om1 = 1.f - k.omega;
     if constexpr (OP == OP_SRT) {
         static_for<0, 19>([&](auto qc) {
@@ -62,7 +63,7 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
 #pragma unroll
             for (int m = 0; m < 6; ++m)
 #pragma unroll
-                for (int a = 0; a < 3; ++a) v[m][a] = P::at(M, a + 1, (I)(l + o[m]), N);
+                for (int a = 0; a < 3; ++a) v[m][a] = P::soa(M, a + 1, (I)(l + o[m]), N);
             const float S[6] = {0.5f * (v[0][0] - v[2][0]), 0.5f * (v[1][1] - v[3][1]),
                                 0.5f * (v[4][2] - v[5][2]),
                                 0.5f * (0.5f * (v[0][1] - v[2][1]) + 0.5f * (v[1][0] - v[3][0])),
@@ -72,7 +73,7 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
 #pragma unroll
             for (int a = 0; a < 6; ++a) a2[a] = k.sigma * a2[a] + c * S[a];
         }
-        offeq_dev<ORDER>(a2, ux, uy, uz, f1);
+        offeq_dev<ORDER>(a2, ux, uy, uz, f1, k.k6);
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             d[q] = eq[q] + om1 * f1[q];
@@ -166,7 +167,7 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
 template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
-    int ny, int nz, int64_t N, int xopen, int row_lds) {
+    int ny, int nz, const PopLayout L, int xopen, int row_lds) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     float g[19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        g[q] = (float)P::at(A, q, l, N);
+        g[q] = (float)P::at(A, q, l, L);
     });
     int bi = 0, bj = 0, bk = 0;
     if constexpr (!PERIODIC) {
@@ -201,7 +202,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     if constexpr (OP != OP_STREAM) {
         if (!boundary) {
             const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-            collide_shift<OP, ORDER, NARROW>(k, g, M, l, o, N);
+            collide_shift<OP, ORDER, NARROW>(k, g, M, l, o, L.N);
         }
     }
     // periodic rows held by one workgroup: x-shifted links realigned through
@@ -217,16 +218,16 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                 if constexpr (EZ[q] > 0) base += ozp;
                 if constexpr (EZ[q] < 0) base += ozm;
                 if constexpr (EX[q] == 0) {
-                    P::at(B, q, (I)(base + (I)i), N) = (T)g[q];
+                    P::at(B, q, (I)(base + (I)i), L) = (T)g[q];
                 } else {
                     __syncthreads();
                     row[(i + EX[q] + nx) % nx] = (T)g[q];
                     __syncthreads();
                     const int src = i - EX[q];
-                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), N) = row[i];
+                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), L) = row[i];
                 }
             });
-            P::at(B, 18, l, N) = (T)g[18];
+            P::at(B, 18, l, L) = (T)g[18];
             return;
         }
     }
@@ -250,9 +251,9 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if constexpr (EY[q] < 0) dd += oym;
         if constexpr (EZ[q] > 0) dd += ozp;
         if constexpr (EZ[q] < 0) dd += ozm;
-        P::at(B, q, dd, N) = (T)g[q];
+        P::at(B, q, dd, L) = (T)g[q];
     });
-    P::at(B, 18, l, N) = (T)g[18];
+    P::at(B, 18, l, L) = (T)g[18];
 }
 
 // Periodic rows of nx = 2 * blockDim.x nodes (nx a multiple of 128, no open
@@ -268,7 +269,7 @@ struct alignas(2 * sizeof(T)) Pair2 {
 template <typename T, int OP, int ORDER, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift2(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
-    int ny, int nz, int64_t N) {
+    int ny, int nz, const PopLayout L) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int t = threadIdx.x;
@@ -279,7 +280,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     float g[2][19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const Pair2<T> v = *reinterpret_cast<const Pair2<T>*>(&P::at(A, q, (I)(rs + (I)(2 * t)), N));
+        const Pair2<T> v = *reinterpret_cast<const Pair2<T>*>(&P::at(A, q, (I)(rs + (I)(2 * t)), L));
         g[0][q] = (float)v.x;
         g[1][q] = (float)v.y;
     });
@@ -295,7 +296,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             const I oxp = i == nx - 1 ? (I)0 - (I)(nx - 1) : (I)1;
             const I oxm = i == 0 ? (I)(nx - 1) : (I)-1;
             const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-            collide_shift<OP, ORDER, NARROW>(k, g[u], M, (I)(rs + (I)i), o, N);
+            collide_shift<OP, ORDER, NARROW>(k, g[u], M, (I)(rs + (I)i), o, L.N);
         }
     }
     __shared__ T row[2 * kBulkBlock];
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             v.x = row[2 * t];
             v.y = row[2 * t + 1];
         }
-        *reinterpret_cast<Pair2<T>*>(&P::at(B, q, (I)(base + (I)(2 * t)), N)) = v;
+        *reinterpret_cast<Pair2<T>*>(&P::at(B, q, (I)(base + (I)(2 * t)), L)) = v;
     });
 }
 
@@ -328,21 +329,21 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     // the macro array M is float32 like a float32 population plane
-    const bool narrow = narrow_planes(g.N, sizeof(float) > sizeof(T) ? sizeof(float) : sizeof(T));
+    const bool narrow = narrow_planes(g.L, sizeof(T), sizeof(float));
     if (periodic && !g.xopen && g.nx % 128 == 0 && g.nx / 2 <= kBulkBlock) {
         const dim3 b2(g.nx / 2, 1, 1), g2(1, g.ny, g.nz);
         if (narrow)
             hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, true>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny,
-                               g.nz, g.N);
+                               g.nz, g.L);
         else
             hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, false>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx,
-                               g.ny, g.nz, g.N);
+                               g.ny, g.nz, g.L);
         return hipGetLastError();
     }
     const int row_lds = periodic && g.nx == bx;
 #define BS(PER, NAR)                                                                                             \
     hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
-                       g.ny, g.nz, g.N, g.xopen, row_lds)
+                       g.ny, g.nz, g.L, g.xopen, row_lds)
     if (periodic) {
         if (narrow) BS(true, true);
         else BS(true, false);
@@ -385,11 +386,12 @@ hipError_t launch_bulk_shift(int prec, int op, int order, bool periodic, const K
 // ------------------------------------------------------------------ HRR pre-pass
 template <typename T>
 __global__ __launch_bounds__(256) void macro_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
-                                                   float* __restrict__ M, int64_t N) {
+                                                   float* __restrict__ M, const PopLayout L) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = L.N;
     if (l >= N) return;
     float d[19], dr, ux, uy, uz;
-    load_dev(A, N, l, kp->inv_s, d);
+    load_dev(A, L, l, kp->inv_s, d);
     moments_dev(d, dr, ux, uy, uz);
     M[l] = dr;
     M[N + l] = ux;
@@ -400,9 +402,9 @@ __global__ __launch_bounds__(256) void macro_shift(const KShift* __restrict__ kp
 hipError_t launch_macro_shift(int prec, const KShift* k, const void* A, float* macro, const Grid& g, hipStream_t s) {
     dim3 grid((unsigned)((g.N + 255) / 256));
     if (prec == ACP_PREC_F32_SHIFTED)
-        hipLaunchKernelGGL(macro_shift<float>, grid, dim3(256), 0, s, k, (const float*)A, macro, g.N);
+        hipLaunchKernelGGL(macro_shift<float>, grid, dim3(256), 0, s, k, (const float*)A, macro, g.L);
     else
-        hipLaunchKernelGGL(macro_shift<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)A, macro, g.N);
+        hipLaunchKernelGGL(macro_shift<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)A, macro, g.L);
     return hipGetLastError();
 }
 
@@ -412,18 +414,18 @@ hipError_t launch_macro_shift(int prec, const KShift* k, const void* A, float* m
 template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
                                                   T* __restrict__ B, const LodiNode* __restrict__ nodes, int64_t count,
-                                                  int64_t N, int transverse) {
+                                                  const PopLayout L, int transverse) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= count) return;
     const KShift& k = *kp;
     const LodiNode ln = nodes[s];
     float d[19];
     float rb, vb[3], r1, v1[3], r2, v2[3];
-    load_dev(A, N, ln.node, k.inv_s, d);
+    load_dev(A, L, ln.node, k.inv_s, d);
     moments_dev(d, rb, vb[0], vb[1], vb[2]);
-    load_dev(B, N, ln.n1, k.inv_s, d);
+    load_dev(B, L, ln.n1, k.inv_s, d);
     moments_dev(d, r1, v1[0], v1[1], v1[2]);
-    load_dev(B, N, ln.n2, k.inv_s, d);
+    load_dev(B, L, ln.n2, k.inv_s, d);
     moments_dev(d, r2, v2[0], v2[1], v2[2]);
     float dr = (4.f * r1 - r2 - 3.f * rb) * -0.5f;
     float vg[3];
@@ -435,7 +437,7 @@ __global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp,
         float rt[4], vt[4][3];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-            load_dev(A, N, ln.t[t], k.inv_s, d);
+            load_dev(A, L, ln.t[t], k.inv_s, d);
             moments_dev(d, rt[t], vt[t][0], vt[t][1], vt[t][2]);
         }
         dT[0] = 0.5f * (rt[0] - rt[1]);
@@ -485,19 +487,19 @@ __global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp,
     u[p1] = svb[1] - L3 + T3;
     u[p2] = svb[2] - L4 + T4;
     float eq[19];
-    feq_dev<ORDER>(dr_new, u[0], u[1], u[2], eq);
+    feq_dev<ORDER>(dr_new, u[0], u[1], u[2], eq, k.k6);
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        B[q * N + ln.node] = (T)(eq[q] * k.s);
+        B[pidx(L, q, ln.node)] = (T)(eq[q] * k.s);
     });
 }
 
 hipError_t launch_lodi_shift(int prec, int order, bool transverse, const KShift* k, const void* A, void* B,
-                             const LodiNode* nodes, int64_t count, int64_t N, hipStream_t s) {
+                             const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s) {
     if (count <= 0) return hipSuccess;
     dim3 grid((unsigned)((count + 255) / 256)), block(256);
     const int tr = transverse ? 1 : 0;
-#define LS(T, O) hipLaunchKernelGGL((lodi_shift<T, O>), grid, block, 0, s, k, (const T*)A, (T*)B, nodes, count, N, tr)
+#define LS(T, O) hipLaunchKernelGGL((lodi_shift<T, O>), grid, block, 0, s, k, (const T*)A, (T*)B, nodes, count, L, tr)
     if (prec == ACP_PREC_F32_SHIFTED) {
         if (order == 2) LS(float, 2);
         else if (order == 3) LS(float, 3);
@@ -511,54 +513,14 @@ hipError_t launch_lodi_shift(int prec, int order, bool transverse, const KShift*
     return hipGetLastError();
 }
 
-// ------------------------------------------------------------------ conversions
-template <typename T>
-__global__ void to_store(const KShift* __restrict__ kp, const double* __restrict__ src, T* __restrict__ dA,
-                         T* __restrict__ dB, int64_t n, double w) {
-    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
-    const T v = (T)((double)kp->s * (src[l] / kp->rho0 - w));
-    dA[l] = v;
-    if (dB) dB[l] = v;
-}
-
-template <typename T>
-__global__ void from_store(const KShift* __restrict__ kp, const T* __restrict__ src, double* __restrict__ dst,
-                           int64_t n, double w) {
-    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n) return;
-    dst[l] = kp->rho0 * (w + (double)src[l] / (double)kp->s);
-}
-
-hipError_t launch_to_store(int prec, const KShift* k, const double* src, void* dA, void* dB, int64_t n, int q,
-                           hipStream_t s) {
-    dim3 grid((unsigned)((n + 255) / 256));
-    if (prec == ACP_PREC_F32_SHIFTED)
-        hipLaunchKernelGGL(to_store<float>, grid, dim3(256), 0, s, k, src, (float*)dA, (float*)dB, n, weight(q));
-    else
-        hipLaunchKernelGGL(to_store<_Float16>, grid, dim3(256), 0, s, k, src, (_Float16*)dA, (_Float16*)dB, n,
-                           weight(q));
-    return hipGetLastError();
-}
-
-hipError_t launch_from_store(int prec, const KShift* k, const void* src, double* dst, int64_t n, int q,
-                             hipStream_t s) {
-    dim3 grid((unsigned)((n + 255) / 256));
-    if (prec == ACP_PREC_F32_SHIFTED)
-        hipLaunchKernelGGL(from_store<float>, grid, dim3(256), 0, s, k, (const float*)src, dst, n, weight(q));
-    else
-        hipLaunchKernelGGL(from_store<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)src, dst, n, weight(q));
-    return hipGetLastError();
-}
-
 // density and velocity in float64 (SI) from the stored deviations
 template <typename T>
-__device__ __forceinline__ void macro_f64(const KShift& k, const T* __restrict__ A, int64_t N, int64_t l, double& rho,
+__device__ __forceinline__ void macro_f64(const KShift& k, const T* __restrict__ A, const PopLayout& L, int64_t l, double& rho,
                                           double& ux, double& uy, double& uz) {
     double r = 0., jx = 0., jy = 0., jz = 0.;
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const double v = (double)A[q * N + l];
+        const double v = (double)A[pidx(L, q, l)];
         r += v;
         if constexpr (EX[q] != 0) jx += EX[q] * v;
         if constexpr (EY[q] != 0) jy += EY[q] * v;
@@ -574,11 +536,13 @@ __device__ __forceinline__ void macro_f64(const KShift& k, const T* __restrict__
 
 template <typename T>
 __global__ __launch_bounds__(256) void macros_out_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
-                                                        double* __restrict__ rho, double* __restrict__ u, int64_t N) {
+                                                        double* __restrict__ rho, double* __restrict__ u,
+                                                        const PopLayout L) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t N = L.N;
     if (l >= N) return;
     double r, ux, uy, uz;
-    macro_f64(*kp, A, N, l, r, ux, uy, uz);
+    macro_f64(*kp, A, L, l, r, ux, uy, uz);
     rho[l] = r;
     u[l] = ux;
     u[N + l] = uy;
@@ -589,9 +553,9 @@ hipError_t launch_macros_out_shift(int prec, const KShift* k, const void* A, con
                                    hipStream_t s) {
     dim3 grid((unsigned)((g.N + 255) / 256));
     if (prec == ACP_PREC_F32_SHIFTED)
-        hipLaunchKernelGGL(macros_out_shift<float>, grid, dim3(256), 0, s, k, (const float*)A, rho, u, g.N);
+        hipLaunchKernelGGL(macros_out_shift<float>, grid, dim3(256), 0, s, k, (const float*)A, rho, u, g.L);
     else
-        hipLaunchKernelGGL(macros_out_shift<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)A, rho, u, g.N);
+        hipLaunchKernelGGL(macros_out_shift<_Float16>, grid, dim3(256), 0, s, k, (const _Float16*)A, rho, u, g.L);
     return hipGetLastError();
 }
 
@@ -613,10 +577,10 @@ __device__ __forceinline__ void philox(uint32_t c[4], uint32_t k0, uint32_t k1) 
 
 template <typename T, int ORDER>
 __global__ __launch_bounds__(256) void init_random_shift(const KShift* __restrict__ kp, T* __restrict__ A,
-                                                         T* __restrict__ B, int64_t N, uint64_t seed, double rho_amp,
-                                                         double u_amp) {
+                                                         T* __restrict__ B, const PopLayout L, uint64_t seed,
+                                                         double rho_amp, double u_amp) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= N) return;
+    if (l >= L.N) return;
     uint32_t c[4] = {(uint32_t)l, (uint32_t)((uint64_t)l >> 32), 0u, 0u};
     philox(c, (uint32_t)seed, (uint32_t)(seed >> 32));
     double uu[4];
@@ -627,19 +591,19 @@ __global__ __launch_bounds__(256) void init_random_shift(const KShift* __restric
     const double n0 = r0 * cos(two_pi * uu[1]), n1 = r0 * sin(two_pi * uu[1]);
     const double n2 = r1 * cos(two_pi * uu[3]), n3 = r1 * sin(two_pi * uu[3]);
     float eq[19];
-    feq_dev<ORDER>((float)(rho_amp * n0), (float)(u_amp * n1), (float)(u_amp * n2), (float)(u_amp * n3), eq);
+    feq_dev<ORDER>((float)(rho_amp * n0), (float)(u_amp * n1), (float)(u_amp * n2), (float)(u_amp * n3), eq, kp->k6);
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const T v = (T)(eq[q] * kp->s);
-        A[q * N + l] = v;
-        B[q * N + l] = v;
+        A[pidx(L, q, l)] = v;
+        B[pidx(L, q, l)] = v;
     });
 }
 
 hipError_t launch_init_random_shift(int prec, int order, const KShift* k, void* A, void* B, const Grid& g,
                                     uint64_t seed, double rho_amp, double u_amp, hipStream_t s) {
     dim3 grid((unsigned)((g.N + 255) / 256)), block(256);
-#define IR(T, O) hipLaunchKernelGGL((init_random_shift<T, O>), grid, block, 0, s, k, (T*)A, (T*)B, g.N, seed, rho_amp, u_amp)
+#define IR(T, O) hipLaunchKernelGGL((init_random_shift<T, O>), grid, block, 0, s, k, (T*)A, (T*)B, g.L, seed, rho_amp, u_amp)
     if (prec == ACP_PREC_F32_SHIFTED) {
         if (order == 2) IR(float, 2);
         else if (order == 3) IR(float, 3);
@@ -656,42 +620,42 @@ hipError_t launch_init_random_shift(int prec, int order, const KShift* k, void* 
 // ------------------------------------------------------------------ probes and mass (float64)
 // One block; the row index is kept on the device (see probe64).
 template <typename T>
-__global__ __launch_bounds__(1024) void probe_shift(const KShift* __restrict__ kp, const T* __restrict__ A, int64_t N,
+__global__ __launch_bounds__(1024) void probe_shift(const KShift* __restrict__ kp, const T* __restrict__ A, const PopLayout L,
                                                     const int64_t* __restrict__ nodes, int nprobe,
                                                     double* __restrict__ out, int32_t* __restrict__ slot,
                                                     int32_t cap) {
     const int32_t row = *slot;
     for (int p = threadIdx.x; p < nprobe; p += blockDim.x) {
         double r, ux, uy, uz;
-        macro_f64(*kp, A, N, nodes[p], r, ux, uy, uz);
+        macro_f64(*kp, A, L, nodes[p], r, ux, uy, uz);
         if (row < cap) out[(int64_t)row * nprobe + p] = r;
     }
     __syncthreads();
     if (threadIdx.x == 0) *slot = row + 1;
 }
 
-hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, int64_t N, const int64_t* nodes, int nprobe,
-                              double* out, int32_t* slot, int32_t cap, hipStream_t s) {
+hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, const PopLayout& L, const int64_t* nodes,
+                              int nprobe, double* out, int32_t* slot, int32_t cap, hipStream_t s) {
     if (nprobe <= 0) return hipSuccess;
     dim3 grid(1), block(nprobe >= 1024 ? 1024 : ((nprobe + 63) / 64) * 64);
     if (prec == ACP_PREC_F32_SHIFTED)
-        hipLaunchKernelGGL(probe_shift<float>, grid, block, 0, s, k, (const float*)A, N, nodes, nprobe, out, slot,
+        hipLaunchKernelGGL(probe_shift<float>, grid, block, 0, s, k, (const float*)A, L, nodes, nprobe, out, slot,
                            cap);
     else
-        hipLaunchKernelGGL(probe_shift<_Float16>, grid, block, 0, s, k, (const _Float16*)A, N, nodes, nprobe, out,
+        hipLaunchKernelGGL(probe_shift<_Float16>, grid, block, 0, s, k, (const _Float16*)A, L, nodes, nprobe, out,
                            slot, cap);
     return hipGetLastError();
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void mass_shift(const KShift* __restrict__ kp, const T* __restrict__ A, int64_t N,
-                                                  double* __restrict__ partial) {
+__global__ __launch_bounds__(256) void mass_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                  const PopLayout L, double* __restrict__ partial) {
     __shared__ double sh[256];
     double acc = 0.;
-    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < N; l += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; l < L.N; l += (int64_t)gridDim.x * blockDim.x) {
         double r = 0.;
 #pragma unroll
-        for (int q = 0; q < 19; ++q) r += (double)A[q * N + l];
+        for (int q = 0; q < 19; ++q) r += (double)A[pidx(L, q, l)];
         acc += r;
     }
     sh[threadIdx.x] = acc;
@@ -707,12 +671,12 @@ __global__ __launch_bounds__(256) void mass_shift(const KShift* __restrict__ kp,
     }
 }
 
-hipError_t launch_mass_shift(int prec, const KShift* k, const void* A, int64_t N, double* partial, int nblocks,
-                             hipStream_t s) {
+hipError_t launch_mass_shift(int prec, const KShift* k, const void* A, const PopLayout& L, double* partial,
+                             int nblocks, hipStream_t s) {
     if (prec == ACP_PREC_F32_SHIFTED)
-        hipLaunchKernelGGL(mass_shift<float>, dim3(nblocks), dim3(256), 0, s, k, (const float*)A, N, partial);
+        hipLaunchKernelGGL(mass_shift<float>, dim3(nblocks), dim3(256), 0, s, k, (const float*)A, L, partial);
     else
-        hipLaunchKernelGGL(mass_shift<_Float16>, dim3(nblocks), dim3(256), 0, s, k, (const _Float16*)A, N, partial);
+        hipLaunchKernelGGL(mass_shift<_Float16>, dim3(nblocks), dim3(256), 0, s, k, (const _Float16*)A, L, partial);
     return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@ struct KShift {
     float hrr_pre;     // -(2 cs^2 / omega): (1-sigma) * this * rho * S enters a2
     float k1, ma, ma2; // LODI relaxation and Mach number
     float inv_len[3];  // 1 / (nodes along x, y, z): cs/L of the LODI relaxation uses L in nodes
+    float k6;          // 1 / (6 c^2), c = sqrt(3) cs in m/s: the fourth-order terms' mixing factor (physf.hpp)
     double rho0;       // SI density scale for conversions
     double c_si;       // molecular velocity cs*sqrt(3) for SI velocity output
 };
@@ -39,23 +40,17 @@ hipError_t launch_bulk_shift(int prec, int op, int order, bool periodic, const K
                              const float* macro, const Grid& g, hipStream_t s);
 hipError_t launch_macro_shift(int prec, const KShift* k, const void* A, float* macro, const Grid& g, hipStream_t s);
 hipError_t launch_lodi_shift(int prec, int order, bool transverse, const KShift* k, const void* A, void* B,
-                             const LodiNode* nodes, int64_t count, int64_t N, hipStream_t s);
-// conversions of one q-plane segment of n values between float64 SI populations and
-// the stored tier (dB may be null)
-hipError_t launch_to_store(int prec, const KShift* k, const double* src, void* dA, void* dB, int64_t n, int q,
-                           hipStream_t s);
-hipError_t launch_from_store(int prec, const KShift* k, const void* src, double* dst, int64_t n, int q,
-                             hipStream_t s);
+                             const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s);
 // density and velocity (SI, float64) of every node: rho [N], u [3][N]
 hipError_t launch_macros_out_shift(int prec, const KShift* k, const void* A, const Grid& g, double* rho, double* u,
                                    hipStream_t s);
 hipError_t launch_init_random_shift(int prec, int order, const KShift* k, void* A, void* B, const Grid& g,
                                     uint64_t seed, double rho_amp, double u_amp, hipStream_t s);
 // density (SI, float64) at probe nodes and per-block mass partial sums (float64)
-hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, int64_t N, const int64_t* nodes, int nprobe,
-                              double* out, int32_t* slot, int32_t cap, hipStream_t s);
-hipError_t launch_mass_shift(int prec, const KShift* k, const void* A, int64_t N, double* partial, int nblocks,
-                             hipStream_t s);
+hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, const PopLayout& L, const int64_t* nodes,
+                              int nprobe, double* out, int32_t* slot, int32_t cap, hipStream_t s);
+hipError_t launch_mass_shift(int prec, const KShift* k, const void* A, const PopLayout& L, double* partial,
+                             int nblocks, hipStream_t s);
 
 // Equilibrium state from a density band rho_rows[(j - j0) * nx + i] (u = 0)
 // into both arrays of any tier (k64_aux.hip, float64 arithmetic).

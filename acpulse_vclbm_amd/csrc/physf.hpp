@@ -35,9 +35,19 @@ __device__ __forceinline__ void moments_dev(const float d[19], float& dr, float&
     uz = jz * ir;
 }
 
+// The fourth-order terms of the reference (Node.cpp:1032-1045 for the
+// equilibrium, 1343-1357 for the off-equilibrium coefficients, and the
+// Hermite values of each link) are written in SI units as, e.g.,
+// H_xxyy + (1/6) H_zz: a
+// fourth-order and a second-order Hermite polynomial of the molecular velocity
+// added with a factor 1/6 that carries no units.  In lattice units (velocities
+// divided by c = sqrt(3) c_s) the same expression reads H_xxyy + k6 H_zz with
+// k6 = 1 / (6 c^2), c in m/s; the velocity polynomials A, B, C and the
+// off-equilibrium coefficients mix orders the same way.  k6 is that factor.
+
 // feq_q - w_q for the order-N Gauss-Hermite equilibrium (Node.cpp:739-1132).
 template <int ORDER>
-__device__ __forceinline__ void feq_dev(float dr, float ux, float uy, float uz, float out[19]) {
+__device__ __forceinline__ void feq_dev(float dr, float ux, float uy, float uz, float out[19], float k6) {
     const float rho = 1.f + dr;
     const float xx = ux * ux, yy = uy * uy, zz = uz * uz;
     const float u2 = xx + yy + zz;
@@ -45,9 +55,9 @@ __device__ __forceinline__ void feq_dev(float dr, float ux, float uy, float uz, 
     const float a3[6] = {XXY + YZZ, XXZ + YYZ, XYY + XZZ, YZZ - XXY, YYZ - XXZ, XZZ - XYY};
     float A = 0.f, B = 0.f, C = 0.f;
     if constexpr (ORDER >= 4) {
-        A = uy * XXY + (1.f / 6.f) * zz;
-        B = uz * XXZ + (1.f / 6.f) * yy + (2.f / 7.f) * A;
-        C = uy * YZZ + (1.f / 6.f) * xx + (2.f / 7.f) * A + (2.f / 5.f) * B;
+        A = uy * XXY + k6 * zz;
+        B = uz * XXZ + k6 * yy + (2.f / 7.f) * A;
+        C = uy * YZZ + k6 * xx + (2.f / 7.f) * A + (2.f / 5.f) * B;
     }
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -66,9 +76,9 @@ __device__ __forceinline__ void feq_dev(float dr, float ux, float uy, float uz, 
         if constexpr (ORDER >= 4) {
             constexpr float hx = EX[q], hy = EY[q], hz = EZ[q];
             constexpr float hxx = hx * hx - 1.f / 3.f, hyy = hy * hy - 1.f / 3.f, hzz = hz * hz - 1.f / 3.f;
-            constexpr float a4 = hxx * hyy + (1.f / 6.f) * hzz;
-            constexpr float b4 = hxx * hzz + (1.f / 6.f) * hyy + (2.f / 7.f) * a4;
-            constexpr float c4 = hyy * hzz + (1.f / 6.f) * hxx + (2.f / 7.f) * a4 + (2.f / 5.f) * b4;
+            const float a4 = hxx * hyy + k6 * hzz;
+            const float b4 = hxx * hzz + k6 * hyy + (2.f / 7.f) * a4;
+            const float c4 = hyy * hzz + k6 * hxx + (2.f / 7.f) * a4 + (2.f / 5.f) * b4;
             const float s4 = (8.f / 7.f) * a4 * A + (56.f / 45.f) * b4 * B + (40.f / 27.f) * c4 * C;
             p += 20.25f * s4;
         }
@@ -105,7 +115,7 @@ __device__ __forceinline__ void pi1_dev(const float fneq[19], float pi[6]) {
 // recursive regularised off-equilibrium from second-order coefficients
 // (Node.cpp:1136-1427, 1430-1747), lattice units
 template <int ORDER>
-__device__ __forceinline__ void offeq_dev(const float a2[6], float ux, float uy, float uz, float out[19]) {
+__device__ __forceinline__ void offeq_dev(const float a2[6], float ux, float uy, float uz, float out[19], float k6) {
     const float XX = a2[0], YY = a2[1], ZZ = a2[2], XY = a2[3], XZ = a2[4], YZ = a2[5];
     float b3[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if constexpr (ORDER >= 3) {
@@ -119,9 +129,9 @@ __device__ __forceinline__ void offeq_dev(const float a2[6], float ux, float uy,
         const float fXXYY = uy * uy * XX + ux * ux * YY + 4.f * ux * uy * XY;
         const float fXXZZ = uz * uz * XX + ux * ux * ZZ + 4.f * ux * uz * XZ;
         const float fYYZZ = uy * uy * ZZ + uz * uz * YY + 4.f * uy * uz * YZ;
-        A = fXXYY + (1.f / 6.f) * ZZ;
-        B = fXXZZ + (1.f / 6.f) * YY + (2.f / 7.f) * A;
-        C = fYYZZ + (1.f / 6.f) * XX + (2.f / 7.f) * A + (2.f / 5.f) * B;
+        A = fXXYY + k6 * ZZ;
+        B = fXXZZ + k6 * YY + (2.f / 7.f) * A;
+        C = fYYZZ + k6 * XX + (2.f / 7.f) * A + (2.f / 5.f) * B;
     }
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
@@ -136,9 +146,9 @@ __device__ __forceinline__ void offeq_dev(const float a2[6], float ux, float uy,
                          H[5] * b3[5]);
         }
         if constexpr (ORDER >= 4) {
-            constexpr float a4 = hxx * hyy + (1.f / 6.f) * hzz;
-            constexpr float b4 = hxx * hzz + (1.f / 6.f) * hyy + (2.f / 7.f) * a4;
-            constexpr float c4 = hyy * hzz + (1.f / 6.f) * hxx + (2.f / 7.f) * a4 + (2.f / 5.f) * b4;
+            const float a4 = hxx * hyy + k6 * hzz;
+            const float b4 = hxx * hzz + k6 * hyy + (2.f / 7.f) * a4;
+            const float c4 = hyy * hzz + k6 * hxx + (2.f / 7.f) * a4 + (2.f / 5.f) * b4;
             x += 20.25f * ((8.f / 7.f) * a4 * A + (56.f / 45.f) * b4 * B + (40.f / 27.f) * c4 * C);
         }
         out[q] = wf(q) * x;

@@ -1,5 +1,16 @@
-// Addressing of the structure-of-arrays population planes (plane q of array F
-// starts at F + q*N) and the compile-time shape of the bulk kernels.
+// Addressing of the population arrays and the compile-time shape of the bulk
+// kernels.
+//
+// Population layout.  The 19 populations of a node live in blocks of B
+// consecutive nodes: block b holds, plane after plane, population q of nodes
+// [bB, (b+1)B), so element (q, l) sits at
+//     (l >> log2 B) * 19B + q * B + (l & (B - 1)).
+// One block spanning the whole grid is the plain structure of arrays (plane q
+// starts at q*N).  Blocks of 8192 nodes are 64 KiB per plane segment in float64.
+// They keep the node update's 19 read and 19 write streams inside a few
+// contiguous regions of the array, which the memory system serves measurably
+// faster than 19 streams a whole plane (N elements) apart. The last block is padded
+// to B nodes; the padding is zero and never written.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -20,8 +31,46 @@ namespace acp {
 
 constexpr int kBulkBlock = ACP_BULK_BLOCK;
 
-// NARROW: a plane spans at most 4 GiB, so every access is a uniform (scalar)
-// plane base plus a 32-bit byte offset, which global loads and stores take
+struct PopLayout {
+    int64_t N = 0;      // nodes
+    int64_t mask = -1;  // nodes per block - 1; all bits for a single block
+    int64_t plane = 0;  // distance between populations q and q + 1 of a node
+    int64_t block = 0;  // distance between consecutive blocks (0: a single block)
+    int64_t elems = 0;  // elements of one population array, padding included
+    int32_t shift = 0;  // log2(nodes per block)
+};
+
+__host__ __device__ __forceinline__ int64_t pidx(const PopLayout& L, int q, int64_t l) {
+    return (l >> L.shift) * L.block + (int64_t)q * L.plane + (l & L.mask);
+}
+
+// Layout of N nodes in blocks of `block_nodes` (a power of two; 0, or a grid
+// of at most one block, gives the plain structure of arrays).
+inline PopLayout make_layout(int64_t N, int64_t block_nodes) {
+    PopLayout L;
+    L.N = N;
+    if (block_nodes <= 0 || N <= block_nodes) {
+        L.mask = -1;
+        L.plane = N;
+        L.block = 0;
+        L.shift = 0;
+        L.elems = 19 * N;
+        return L;
+    }
+    int s = 0;
+    while (((int64_t)1 << s) < block_nodes) ++s;
+    L.shift = s;
+    L.mask = ((int64_t)1 << s) - 1;
+    L.plane = (int64_t)1 << s;
+    L.block = 19 * L.plane;
+    L.elems = ((N + L.mask) >> s) * L.block;
+    return L;
+}
+
+// NARROW: the per-node part of every population offset (the node's block and
+// its position in the block, or its index in a whole plane) and every macro
+// plane stay below 4 GiB, so each access is a uniform base (array plus plane
+// offset) and a 32-bit byte offset, which global loads and stores take
 // directly with an SGPR base; this keeps 64-bit address arithmetic out of the
 // vector registers.  Otherwise 64-bit element indices.
 template <bool NARROW>
@@ -30,10 +79,19 @@ struct Plane;
 template <>
 struct Plane<true> {
     using idx = uint32_t;
+    // population q of node l in layout L: the plane offset q * plane is uniform
+    // (64-bit, scalar), the node's block and position in it a 32-bit offset
     template <typename T>
-    static __device__ __forceinline__ T& at(T* base, int q, idx l, int64_t N) {
+    static __device__ __forceinline__ T& at(T* base, int q, idx l, const PopLayout& L) {
         using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
-        return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(base + q * N) + (uint32_t)(l * (uint32_t)sizeof(T)));
+        const uint32_t e = (l >> L.shift) * (uint32_t)L.block + (l & (uint32_t)L.mask);
+        return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(base + q * L.plane) + e * (uint32_t)sizeof(T));
+    }
+    // plane v of a structure-of-arrays buffer with planes of N elements
+    template <typename T>
+    static __device__ __forceinline__ T& soa(T* base, int v, idx l, int64_t N) {
+        using Byte = std::conditional_t<std::is_const_v<T>, const char, char>;
+        return *reinterpret_cast<T*>(reinterpret_cast<Byte*>(base + v * N) + (uint32_t)(l * (uint32_t)sizeof(T)));
     }
 };
 
@@ -41,12 +99,22 @@ template <>
 struct Plane<false> {
     using idx = int64_t;
     template <typename T>
-    static __device__ __forceinline__ T& at(T* base, int q, idx l, int64_t N) {
-        return base[q * N + l];
+    static __device__ __forceinline__ T& at(T* base, int q, idx l, const PopLayout& L) {
+        return base[pidx(L, q, l)];
+    }
+    template <typename T>
+    static __device__ __forceinline__ T& soa(T* base, int v, idx l, int64_t N) {
+        return base[v * N + l];
     }
 };
 
-// true when the narrow form can address every plane of N elements of `bytes`
-inline bool narrow_planes(int64_t N, int bytes) { return N * (int64_t)bytes <= ((int64_t)1 << 32); }
+// true when the narrow form addresses population arrays of `pop_bytes` per
+// element and macro planes of `plane_bytes` per element: the per-node part of
+// every offset (the node's block and position, or its index in a whole plane)
+// stays below 4 GiB
+inline bool narrow_planes(const PopLayout& L, int pop_bytes, int plane_bytes) {
+    const int64_t node_part = L.block ? (((L.N - 1) >> L.shift) * L.block + L.mask + 1) : L.N;
+    return node_part * (int64_t)pop_bytes <= ((int64_t)1 << 32) && L.N * (int64_t)plane_bytes <= ((int64_t)1 << 32);
+}
 
 }  // namespace acp

@@ -1,4 +1,4 @@
-// Device-copy ceiling for the structure-of-arrays layout (acp_copy_probe).
+// Device-copy ceilings for the population layouts (acp_copy_probe).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -20,6 +20,19 @@ __global__ __launch_bounds__(512) void copy_planes(const double* __restrict__ a,
     for (int s = 0; s < P; ++s) b[s * n + i] = v[s];
 }
 
+// 19 populations of one node per thread in the block layout (plane.hpp):
+// loads first, then stores
+__global__ __launch_bounds__(512) void copy_blocks(const double* __restrict__ a, double* __restrict__ b,
+                                                   const PopLayout L) {
+    const int64_t i = (int64_t)blockIdx.x * 512 + threadIdx.x;
+    if (i >= L.N) return;
+    double v[19];
+#pragma unroll
+    for (int s = 0; s < 19; ++s) v[s] = a[pidx(L, s, i)];
+#pragma unroll
+    for (int s = 0; s < 19; ++s) b[pidx(L, s, i)] = v[s];
+}
+
 __global__ __launch_bounds__(512) void fill_ramp(double* __restrict__ a, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * 512 + threadIdx.x;
     if (i < n) a[i] = (double)(i & 1023);
@@ -27,11 +40,14 @@ __global__ __launch_bounds__(512) void fill_ramp(double* __restrict__ a, int64_t
 
 }  // namespace acp
 
-extern "C" int acp_copy_probe(int32_t planes, int64_t plane_elems, int32_t reps, void* stream, double* ms_per_copy) {
+extern "C" int acp_copy_probe(int32_t planes, int64_t plane_elems, int64_t block_nodes, int32_t reps, void* stream,
+                              double* ms_per_copy) {
     acp::clear_error();
-    if ((planes != 1 && planes != 19) || plane_elems <= 0 || reps <= 0 || !ms_per_copy)
-        return acp::fail(ACP_E_ARG, "acp_copy_probe: planes must be 1 or 19, sizes positive");
-    const int64_t total = planes * plane_elems;
+    if ((planes != 1 && planes != 19) || plane_elems <= 0 || reps <= 0 || !ms_per_copy || block_nodes < 0 ||
+        (block_nodes & (block_nodes - 1)) != 0 || (block_nodes > 0 && planes != 19))
+        return acp::fail(ACP_E_ARG, "acp_copy_probe: planes 1 or 19, sizes positive, blocks a power of two (19 planes)");
+    const acp::PopLayout L = acp::make_layout(plane_elems, block_nodes);
+    const int64_t total = planes == 19 ? L.elems : plane_elems;
     const unsigned blocks = (unsigned)((plane_elems + 511) / 512);
     if ((plane_elems + 511) / 512 > 0x7fffffff) return acp::fail(ACP_E_ARG, "acp_copy_probe: plane too large");
     hipStream_t s = (hipStream_t)stream;
@@ -50,6 +66,8 @@ extern "C" int acp_copy_probe(int32_t planes, int64_t plane_elems, int32_t reps,
     auto launch = [&]() {
         if (planes == 1)
             hipLaunchKernelGGL(acp::copy_planes<1>, dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
+        else if (L.block)
+            hipLaunchKernelGGL(acp::copy_blocks, dim3(blocks), dim3(512), 0, s, a, b, L);
         else
             hipLaunchKernelGGL(acp::copy_planes<19>, dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
         return hipGetLastError();

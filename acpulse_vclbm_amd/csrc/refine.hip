@@ -65,30 +65,29 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp, const double* __restrict__ Cprev,
                                                       const double* __restrict__ Ccur, int half, double a,
                                                       int cnx, int cny, int cnz, int ci, double* __restrict__ F,
-                                                      int fnx, int fny, int fi) {
+                                                      int fnx, int fny, int fi, const PopLayout CN,
+                                                      const PopLayout FN) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cny * cnz) return;
     const int J = t % cny, K = t / cny;
-    const int64_t CN = (int64_t)cnx * cny * cnz;
-    const int64_t FN = (int64_t)fnx * fny * (2 * cnz);
     const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
     const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
     double f[19];
 #pragma unroll
     for (int q = 0; q < 19; ++q)
-        f[q] = half ? 0.5 * (Cprev[q * CN + lc] + Ccur[q * CN + lc]) : Ccur[q * CN + lc];
+        f[q] = half ? 0.5 * (Cprev[pidx(CN, q, lc)] + Ccur[pidx(CN, q, lc)]) : Ccur[pidx(CN, q, lc)];
     rescale<ORDER>(*kp, f, a);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) F[q * FN + lf] = f[q];
+    for (int q = 0; q < 19; ++q) F[pidx(FN, q, lf)] = f[q];
 }
 
 // The other fine face nodes (odd j or odd k) from their coincident neighbours.
-__global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int fnx, int fny, int fnz, int fi) {
+__global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int fnx, int fny, int fnz, int fi,
+                                                   const PopLayout FN) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= fny * fnz) return;
     const int j = t % fny, k = t / fny;
     if (!(j & 1) && !(k & 1)) return;
-    const int64_t FN = (int64_t)fnx * fny * fnz;
     auto at = [&](int jj, int kk) {
         jj = (jj + fny) % fny;
         kk = (kk + fnz) % fnz;
@@ -99,12 +98,12 @@ __global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int f
         const int64_t a = at(j - 1, k - 1), b = at(j + 1, k - 1), c = at(j - 1, k + 1), d = at(j + 1, k + 1);
 #pragma unroll
         for (int q = 0; q < 19; ++q)
-            F[q * FN + l] = ((F[q * FN + a] + F[q * FN + b]) + (F[q * FN + c] + F[q * FN + d])) * 0.25;
+            F[pidx(FN, q, l)] = ((F[pidx(FN, q, a)] + F[pidx(FN, q, b)]) + (F[pidx(FN, q, c)] + F[pidx(FN, q, d)])) * 0.25;
     } else {
         const int64_t a = (j & 1) ? at(j - 1, k) : at(j, k - 1);
         const int64_t b = (j & 1) ? at(j + 1, k) : at(j, k + 1);
 #pragma unroll
-        for (int q = 0; q < 19; ++q) F[q * FN + l] = (F[q * FN + a] + F[q * FN + b]) * 0.5;
+        for (int q = 0; q < 19; ++q) F[pidx(FN, q, l)] = (F[pidx(FN, q, a)] + F[pidx(FN, q, b)]) * 0.5;
     }
 }
 
@@ -117,12 +116,11 @@ __global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int f
 // wrap.
 template <int ORDER>
 __device__ __forceinline__ void lag_mean(const K64& k, const double* __restrict__ F, int fnx, int fny, int fnz,
-                                         int fx, int fy, int fz, double ainv, double v[19]) {
-    const int64_t FN = (int64_t)fnx * fny * fnz;
+                                         int fx, int fy, int fz, double ainv, double v[19], const PopLayout& FN) {
     double rho, ux, uy, uz, eq[19], acc[19];
     const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) v[q] = F[q * FN + l0];
+    for (int q = 0; q < 19; ++q) v[q] = F[pidx(FN, q, l0)];
     moments64(k, v, rho, ux, uy, uz);
     feq64<ORDER>(k, rho, ux, uy, uz, eq);
 #pragma unroll
@@ -133,7 +131,7 @@ __device__ __forceinline__ void lag_mean(const K64& k, const double* __restrict_
         const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
         double g[19], e[19], r, x1, x2, x3;
 #pragma unroll
-        for (int q = 0; q < 19; ++q) g[q] = F[q * FN + l];
+        for (int q = 0; q < 19; ++q) g[q] = F[pidx(FN, q, l)];
         moments64(k, g, r, x1, x2, x3);
         feq64<ORDER>(k, r, x1, x2, x3, e);
 #pragma unroll
@@ -148,26 +146,25 @@ __device__ __forceinline__ void lag_mean(const K64& k, const double* __restrict_
 template <int ORDER, bool LAG>
 __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                            int fny, int fnz, double ainv, double* __restrict__ C, int cnx, int cny,
-                                           int cnz, int x0, int width) {
+                                           int cnz, int x0, int width, const PopLayout CN, const PopLayout FN) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)width * cny * cnz) return;
     const int di = (int)(t % width);
     const int64_t r = t / width;
     const int J = (int)(r % cny), K = (int)(r / cny);
     const int I = (x0 + 1 + di) % cnx;
-    const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * fnz;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
     double f[19];
     if constexpr (LAG) {
-        lag_mean<ORDER>(*kp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, f);
+        lag_mean<ORDER>(*kp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, f, FN);
     } else {
         const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
 #pragma unroll
-        for (int q = 0; q < 19; ++q) f[q] = F[q * FN + lf];
+        for (int q = 0; q < 19; ++q) f[q] = F[pidx(FN, q, lf)];
         rescale<ORDER>(*kp, f, ainv);
     }
 #pragma unroll
-    for (int q = 0; q < 19; ++q) C[q * CN + lc] = f[q];
+    for (int q = 0; q < 19; ++q) C[pidx(CN, q, lc)] = f[q];
 }
 
 // ---- shifted tiers: stored g = s (f / rho0 - w) ---------------------------
@@ -186,33 +183,34 @@ __device__ __forceinline__ T f_to_g(const KShift& ks, double f, int q) {
 
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
-                                            int64_t CN, int64_t lc, int half, double a, T* F, int64_t FN,
+                                            const PopLayout& CN, int64_t lc, int half, double a, T* F,
+                                            const PopLayout& FN,
                                             int64_t lf) {
     if constexpr (F64) {
         double f[19];
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const double fc = g_to_f(ks, Ccur[q * CN + lc], q);
-            f[q] = half ? 0.5 * (g_to_f(ks, Cprev[q * CN + lc], q) + fc) : fc;
+            const double fc = g_to_f(ks, Ccur[pidx(CN, q, lc)], q);
+            f[q] = half ? 0.5 * (g_to_f(ks, Cprev[pidx(CN, q, lc)], q) + fc) : fc;
         });
         rescale<ORDER>(k, f, a);
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            F[q * FN + lf] = f_to_g<T>(ks, f[q], q);
+            F[pidx(FN, q, lf)] = f_to_g<T>(ks, f[q], q);
         });
     } else {
         float d[19], eq[19], dr, ux, uy, uz;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const float dc = (float)Ccur[q * CN + lc] * ks.inv_s;
-            d[q] = half ? 0.5f * ((float)Cprev[q * CN + lc] * ks.inv_s + dc) : dc;
+            const float dc = (float)Ccur[pidx(CN, q, lc)] * ks.inv_s;
+            d[q] = half ? 0.5f * ((float)Cprev[pidx(CN, q, lc)] * ks.inv_s + dc) : dc;
         });
         moments_dev(d, dr, ux, uy, uz);
-        feq_dev<ORDER>(dr, ux, uy, uz, eq);
+        feq_dev<ORDER>(dr, ux, uy, uz, eq, ks.k6);
         const float af = (float)a;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            F[q * FN + lf] = (T)((eq[q] + af * (d[q] - eq[q])) * ks.s);
+            F[pidx(FN, q, lf)] = (T)((eq[q] + af * (d[q] - eq[q])) * ks.s);
         });
     }
 }
@@ -222,14 +220,13 @@ __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, cons
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
                                            int fnz, int fx, int fy, int fz, double ainv, T* __restrict__ C,
-                                           int64_t CN, int64_t lc) {
-    const int64_t FN = (int64_t)fnx * fny * fnz;
+                                           const PopLayout& CN, int64_t lc, const PopLayout& FN) {
     const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
     if constexpr (F64) {
         double v[19], eq[19], acc[19], rho, ux, uy, uz;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            v[q] = g_to_f(ks, F[q * FN + l0], q);
+            v[q] = g_to_f(ks, F[pidx(FN, q, l0)], q);
         });
         moments64(k, v, rho, ux, uy, uz);
         feq64<ORDER>(k, rho, ux, uy, uz, eq);
@@ -242,7 +239,7 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
             double g[19], e[19], r, x1, x2, x3;
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                g[q] = g_to_f(ks, F[q * FN + l], q);
+                g[q] = g_to_f(ks, F[pidx(FN, q, l)], q);
             });
             moments64(k, g, r, x1, x2, x3);
             feq64<ORDER>(k, r, x1, x2, x3, e);
@@ -251,16 +248,16 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
         });
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            C[q * CN + lc] = f_to_g<T>(ks, eq[q] + ainv * (acc[q] / 19.), q);
+            C[pidx(CN, q, lc)] = f_to_g<T>(ks, eq[q] + ainv * (acc[q] / 19.), q);
         });
     } else {
         float d[19], eq[19], acc[19], dr, ux, uy, uz;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            d[q] = (float)F[q * FN + l0] * ks.inv_s;
+            d[q] = (float)F[pidx(FN, q, l0)] * ks.inv_s;
         });
         moments_dev(d, dr, ux, uy, uz);
-        feq_dev<ORDER>(dr, ux, uy, uz, eq);
+        feq_dev<ORDER>(dr, ux, uy, uz, eq, ks.k6);
 #pragma unroll
         for (int q = 0; q < 19; ++q) acc[q] = d[q] - eq[q];
         static_for<0, 18>([&](auto jc) {
@@ -270,17 +267,17 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
             float g[19], e[19], r, x1, x2, x3;
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                g[q] = (float)F[q * FN + l] * ks.inv_s;
+                g[q] = (float)F[pidx(FN, q, l)] * ks.inv_s;
             });
             moments_dev(g, r, x1, x2, x3);
-            feq_dev<ORDER>(r, x1, x2, x3, e);
+            feq_dev<ORDER>(r, x1, x2, x3, e, ks.k6);
 #pragma unroll
             for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
         });
         const float af = (float)ainv;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            C[q * CN + lc] = (T)((eq[q] + af * (acc[q] * (1.f / 19.f))) * ks.s);
+            C[pidx(CN, q, lc)] = (T)((eq[q] + af * (acc[q] * (1.f / 19.f))) * ks.s);
         });
     }
 }
@@ -289,11 +286,11 @@ template <typename T, int ORDER, bool F64>
 __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                                         const T* __restrict__ Cprev, const T* __restrict__ Ccur,
                                                         int half, double a, int cnx, int cny, int cnz, int ci,
-                                                        T* __restrict__ F, int fnx, int fny, int fi) {
+                                                        T* __restrict__ F, int fnx, int fny, int fi,
+                                                        const PopLayout CN, const PopLayout FN) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= cny * cnz) return;
     const int J = t % cny, K = t / cny;
-    const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * (2 * cnz);
     const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
     const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
     couple_node<T, ORDER, F64>(*kp, *ksp, Cprev, Ccur, CN, lc, half, a, F, FN, lf);
@@ -301,13 +298,12 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
 
 template <typename T, bool F64>
 __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ ksp, T* __restrict__ F, int fnx,
-                                                     int fny, int fnz, int fi) {
+                                                     int fny, int fnz, int fi, const PopLayout FN) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= fny * fnz) return;
     const int j = t % fny, k = t / fny;
     if (!(j & 1) && !(k & 1)) return;
     const KShift& ks = *ksp;
-    const int64_t FN = (int64_t)fnx * fny * fnz;
     auto at = [&](int jj, int kk) {
         jj = (jj + fny) % fny;
         kk = (kk + fnz) % fnz;
@@ -327,15 +323,15 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (F64) {
-            double v = n == 4 ? ((g_to_f(ks, F[q * FN + nb[0]], q) + g_to_f(ks, F[q * FN + nb[1]], q)) +
-                                 (g_to_f(ks, F[q * FN + nb[2]], q) + g_to_f(ks, F[q * FN + nb[3]], q))) * 0.25
-                              : (g_to_f(ks, F[q * FN + nb[0]], q) + g_to_f(ks, F[q * FN + nb[1]], q)) * 0.5;
-            F[q * FN + l] = f_to_g<T>(ks, v, q);
+            double v = n == 4 ? ((g_to_f(ks, F[pidx(FN, q, nb[0])], q) + g_to_f(ks, F[pidx(FN, q, nb[1])], q)) +
+                                 (g_to_f(ks, F[pidx(FN, q, nb[2])], q) + g_to_f(ks, F[pidx(FN, q, nb[3])], q))) * 0.25
+                              : (g_to_f(ks, F[pidx(FN, q, nb[0])], q) + g_to_f(ks, F[pidx(FN, q, nb[1])], q)) * 0.5;
+            F[pidx(FN, q, l)] = f_to_g<T>(ks, v, q);
         } else {
-            float v = n == 4 ? (((float)F[q * FN + nb[0]] + (float)F[q * FN + nb[1]]) +
-                                ((float)F[q * FN + nb[2]] + (float)F[q * FN + nb[3]])) * 0.25f
-                             : ((float)F[q * FN + nb[0]] + (float)F[q * FN + nb[1]]) * 0.5f;
-            F[q * FN + l] = (T)v;
+            float v = n == 4 ? (((float)F[pidx(FN, q, nb[0])] + (float)F[pidx(FN, q, nb[1])]) +
+                                ((float)F[pidx(FN, q, nb[2])] + (float)F[pidx(FN, q, nb[3])])) * 0.25f
+                             : ((float)F[pidx(FN, q, nb[0])] + (float)F[pidx(FN, q, nb[1])]) * 0.5f;
+            F[pidx(FN, q, l)] = (T)v;
         }
     });
 }
@@ -343,17 +339,17 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
 template <typename T, int ORDER, bool F64, bool LAG>
 __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                              const T* __restrict__ F, int fnx, int fny, int fnz, double ainv,
-                                             T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width) {
+                                             T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width,
+                                             const PopLayout CN, const PopLayout FN) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)width * cny * cnz) return;
     const int di = (int)(t % width);
     const int64_t r = t / width;
     const int J = (int)(r % cny), K = (int)(r / cny);
     const int I = (x0 + 1 + di) % cnx;
-    const int64_t CN = (int64_t)cnx * cny * cnz, FN = (int64_t)fnx * fny * fnz;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
     if constexpr (LAG) {
-        lag_node_s<T, ORDER, F64>(*kp, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc);
+        lag_node_s<T, ORDER, F64>(*kp, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN);
     } else {
         const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
         couple_node<T, ORDER, F64>(*kp, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
@@ -392,7 +388,7 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
         const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
 #define C2FS(O)                                                                                                  \
     hipLaunchKernelGGL((acp::refine::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks, (const T*)c->SB,    \
-                       (const T*)c->SA, half, r->a, cnx, cny, cnz, fc[0], (T*)f->SA, fnx, fny, fc[1])
+                       (const T*)c->SA, half, r->a, cnx, cny, cnz, fc[0], (T*)f->SA, fnx, fny, fc[1], c->g.L, f->g.L)
         if (c->d.order == 2) C2FS(2);
         else if (c->d.order == 3) C2FS(3);
         else C2FS(4);
@@ -400,7 +396,7 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream,
-                           f->dks, (T*)f->SA, fnx, fny, fnz, fc[1]);
+                           f->dks, (T*)f->SA, fnx, fny, fnz, fc[1], f->g.L);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -414,9 +410,9 @@ hipError_t launch_f2c_s(acp_refined* r) {
     const int width = r->x1 - r->x0 - 1;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-#define F2CS(O, L)                                                                                               \
-    hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64, L>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, \
-                       f->g.ny, f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width)
+#define F2CS(O, LG)                                                                                              \
+    hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64, LG>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, \
+                       f->g.ny, f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
     if (r->lag) {
         if (c->d.order == 2) F2CS(2, true);
         else if (c->d.order == 3) F2CS(3, true);
@@ -445,7 +441,7 @@ hipError_t launch_c2f(acp_refined* r, int half) {
         const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
 #define C2F(O)                                                                                                   \
     hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, cny, cnz, \
-                       fc[0], f->A, fnx, fny, fc[1])
+                       fc[0], f->A, fnx, fny, fc[1], c->g.L, f->g.L)
         if (order == 2) C2F(2);
         else if (order == 3) C2F(3);
         else C2F(4);
@@ -453,7 +449,7 @@ hipError_t launch_c2f(acp_refined* r, int half) {
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(acp::refine::c2f_between, dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream, f->A, fnx,
-                           fny, fnz, fc[1]);
+                           fny, fnz, fc[1], f->g.L);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -471,9 +467,9 @@ hipError_t launch_f2c(acp_refined* r) {
     if (width <= 0) return hipSuccess;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-#define F2C(O, L)                                                                                               \
-    hipLaunchKernelGGL((acp::refine::f2c<O, L>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, \
-                       c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width)
+#define F2C(O, LG)                                                                                              \
+    hipLaunchKernelGGL((acp::refine::f2c<O, LG>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, \
+                       c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
     if (r->lag) {
         if (c->d.order == 2) F2C(2, true);
         else if (c->d.order == 3) F2C(3, true);

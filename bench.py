@@ -79,16 +79,22 @@ def run_tier(acp, torch, precision, n, steps, warmup, seed=0):
     }
 
 
+LAYOUT_BLOCK_NODES = 8192  # float64 / float32 population layout blocks (csrc/engine.cpp)
+
+
 def copy_ceiling(acp, torch, n, reps=5):
-    """Device-copy bandwidth for the same bytes as one float64 step: 19 planes
-    of n^3 float64 copied one element of every plane per thread (the bulk
-    kernel's access pattern), and one plane of 19 n^3 elements."""
+    """Device-copy bandwidth for the same bytes as one float64 step: the 19
+    populations of n^3 nodes copied one node per thread in the engine's block
+    layout (the bulk kernel's access pattern without the neighbour shift), the
+    same as 19 whole planes, and one plane of 19 n^3 elements."""
     stream = torch.cuda.Stream()
     nodes = n ** 3
     nbytes = 2 * 19 * 8 * nodes
+    msb = acp.copy_probe(19, nodes, reps, stream.cuda_stream, block_nodes=LAYOUT_BLOCK_NODES)
     ms19 = acp.copy_probe(19, nodes, reps, stream.cuda_stream)
     ms1 = acp.copy_probe(1, 19 * nodes, reps, stream.cuda_stream)
-    return {"planes19_gbs": nbytes / (ms19 * 1e-3) / 1e9, "planes1_gbs": nbytes / (ms1 * 1e-3) / 1e9}
+    return {"layout_gbs": nbytes / (msb * 1e-3) / 1e9, "planes19_gbs": nbytes / (ms19 * 1e-3) / 1e9,
+            "planes1_gbs": nbytes / (ms1 * 1e-3) / 1e9}
 
 
 def node_updates_per_step(serve, r):
@@ -222,7 +228,8 @@ def main():
     r = res[main_t]
     ceil = None if a.no_copy_ceiling else copy_ceiling(acp, torch, a.n)
     if ceil is not None:
-        print(f"# copy ceiling: 19 planes {ceil['planes19_gbs']:.0f} GB/s, 1 plane {ceil['planes1_gbs']:.0f} GB/s",
+        print(f"# copy ceiling: engine layout (blocks of {LAYOUT_BLOCK_NODES} nodes) {ceil['layout_gbs']:.0f} GB/s, "
+              f"19 whole planes {ceil['planes19_gbs']:.0f} GB/s, 1 plane {ceil['planes1_gbs']:.0f} GB/s",
               file=sys.stderr, flush=True)
     srv = None
     if not a.no_serving:
@@ -278,7 +285,9 @@ def main():
                      "traffic": traffic_from_profiles(f"bulk64_mrtrm_periodic_{a.n}.txt")},
         "tiers": {t: {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res[t].items()} for t in res},
         "copy_ceiling": None if ceil is None else {
+            "layout_gbs": round(ceil["layout_gbs"], 1), "layout_block_nodes": LAYOUT_BLOCK_NODES,
             "planes19_gbs": round(ceil["planes19_gbs"], 1), "planes1_gbs": round(ceil["planes1_gbs"], 1),
+            "frac_of_layout": round(r["achieved_gbs"] / ceil["layout_gbs"], 4),
             "frac_of_planes19": round(r["achieved_gbs"] / ceil["planes19_gbs"], 4),
             "frac_of_planes1": round(r["achieved_gbs"] / ceil["planes1_gbs"], 4)},
         "serving": srv,

@@ -313,12 +313,16 @@ double acp_server_clock(acp_server* s);
 void acp_server_destroy(acp_server* s);
 
 /* Measurement helper, no counterpart in the reference: the device-copy
- * ceiling for the engine's memory layout.  Copies `planes` (1 or 19) float64
- * planes of `plane_elems` elements each, one element of every plane per thread
- * (the access pattern of the bulk kernel without the neighbour shift), `reps`
- * times on `stream` after one untimed copy; writes the mean milliseconds per
- * copy.  Allocates and frees its own two buffers. */
-int acp_copy_probe(int32_t planes, int64_t plane_elems, int32_t reps, void* stream, double* ms_per_copy);
+ * ceiling for a population layout.  Copies `planes` (1 or 19) float64 planes
+ * of `plane_elems` elements each, one element of every plane per thread (the
+ * access pattern of the bulk kernel without the neighbour shift); with 19
+ * planes and `block_nodes` > 0 (a power of two) the planes are laid out in
+ * blocks of that many nodes as the engine lays out its float64 and float32
+ * populations, with 0 as whole planes.  `reps` copies on `stream` after one
+ * untimed copy; writes the mean milliseconds per copy.  Allocates and frees
+ * its own two buffers. */
+int acp_copy_probe(int32_t planes, int64_t plane_elems, int64_t block_nodes, int32_t reps, void* stream,
+                   double* ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -126,3 +126,12 @@ def test_null_arguments_do_not_crash():
     assert L.acp_case_step(None, 1) == _capi.ACP_E_ARG
     assert L.acp_case_info_get(None, None) == _capi.ACP_E_ARG
     L.acp_case_destroy(None)
+
+
+@pytest.mark.parametrize("planes,elems,block", [(2, 100, 0), (19, 0, 0), (19, 100, 3), (19, 100, -8), (1, 100, 64)])
+def test_copy_probe_rejects_bad_arguments(planes, elems, block):
+    """Argument checks run before any device call: planes 1 or 19, positive
+    sizes, blocks a power of two and only with 19 planes."""
+    ms = ctypes.c_double()
+    rc = acp.lib().acp_copy_probe(planes, elems, block, 1, None, ctypes.byref(ms))
+    assert rc == _capi.ACP_E_ARG

@@ -54,16 +54,28 @@ def test_footprint_lodi_term_is_the_schedule(n):
     assert diff == (count * 64 + 255) // 256 * 256
 
 
-def test_footprint_tiers_and_probes():
-    n = (64, 48, 3)
+def _elems(N, precision):
+    """Elements of one population array: blocks of 8192 nodes (19 planes each),
+    the last one padded, for float64 and float32; whole planes for float16 and
+    for grids of at most one block."""
+    B = 8192
+    if precision == "F16-SHIFTED" or N <= B:
+        return 19 * N
+    return -(-N // B) * 19 * B
+
+
+@pytest.mark.parametrize("n", [(64, 48, 3), (16, 16, 16), (40, 40, 9)])
+def test_footprint_tiers_and_probes(n):
     f64 = _fp(acp.make_desc(*n, op="SRT", tau=0.6, boundary="PERIODIC"))
     f32 = _fp(acp.make_desc(*n, op="SRT", tau=0.6, boundary="PERIODIC", precision="F32-SHIFTED"))
+    f16 = _fp(acp.make_desc(*n, op="SRT", tau=0.6, boundary="PERIODIC", precision="F16-SHIFTED"))
     N = n[0] * n[1] * n[2]
     r = lambda b: (b + 255) // 256 * 256
-    # float64: constants + 2 arrays of 19 float64 planes; float32-shifted: both constant
-    # blocks (one 256-byte unit each) + 2 arrays of 19 float32 planes; the staging and
-    # reduction buffers are common to both tiers
-    assert f64 - f32 == 2 * r(19 * N * 8) - 2 * r(19 * N * 4) - 256
+    # float64: constants + 2 population arrays of float64; float32-shifted: both constant
+    # blocks (one 256-byte unit each) + 2 arrays of float32; the staging and reduction
+    # buffers are common to every tier
+    assert f64 - f32 == 2 * r(_elems(N, "F64") * 8) - 2 * r(_elems(N, "F32-SHIFTED") * 4) - 256
+    assert f32 - f16 == 2 * r(_elems(N, "F32-SHIFTED") * 4) - 2 * r(_elems(N, "F16-SHIFTED") * 2)
     base = acp.make_desc(*n, op="HRR", tau=0.6)
     assert _fp(base, 1, 100) - _fp(base) == 256 + r(800) + 256
     assert _fp(base, 3, 1000) - _fp(base) == r(24) + r(24000) + 256
