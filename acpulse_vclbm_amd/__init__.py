@@ -255,6 +255,23 @@ def lodi_schedule(nx, ny, nz):
     return nodes, ph
 
 
+LODI_RECORD = np.dtype([("node", np.int64), ("n1", np.int64), ("n2", np.int64), ("t", np.int64, 4),
+                        ("dir", np.int32), ("corner", np.int32)])
+
+
+def lodi_launches(nx, ny, nz):
+    """(records, launch_start) of the LODI pass as the kernels receive them
+    (acp_lodi_launches); records is a structured array of LODI_RECORD."""
+    L = lib()
+    nl = ctypes.c_int32()
+    n = L.acp_lodi_launches(nx, ny, nz, None, 0, None, 0, ctypes.byref(nl))
+    rec = np.zeros(n, dtype=LODI_RECORD)
+    starts = np.empty(nl.value + 1, dtype=np.int64)
+    L.acp_lodi_launches(nx, ny, nz, rec.ctypes.data_as(ctypes.c_void_p), n,
+                        starts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), nl.value + 1, ctypes.byref(nl))
+    return rec, starts
+
+
 def footprint(desc, nprobe=0, probe_capacity=0):
     """Device bytes a case of `desc` holds with `nprobe` probes of `probe_capacity` steps (acp_case_footprint)."""
     b = ctypes.c_int64()

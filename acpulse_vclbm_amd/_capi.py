@@ -124,6 +124,7 @@ def lib():
         "acp_case_write_vtk": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "acp_write_vtk_host": (ctypes.c_int, [D, dp, dp, dp, ctypes.c_char_p]),
         "acp_lodi_schedule": (ctypes.c_int64, [i32, i32, i32, ctypes.POINTER(i64), ctypes.POINTER(i32), i64]),
+        "acp_lodi_launches": (ctypes.c_int64, [i32, i32, i32, vp, i64, ctypes.POINTER(i64), i32, ctypes.POINTER(i32)]),
         "acp_copy_probe": (ctypes.c_int, [i32, i64, i64, i32, vp, dp]),
         "acp_case_enable_graphs": (ctypes.c_int, [vp, i32]),
         "acp_case_footprint": (ctypes.c_int, [D, i32, i32, ctypes.POINTER(i64)]),

@@ -3,6 +3,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <unordered_map>
@@ -951,6 +952,24 @@ extern "C" int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t
                 if (nodes) nodes[t] = s.nodes[i].node;
                 if (phases) phases[t] = (int32_t)p + 1;
             }
+    return n;
+}
+
+static_assert(sizeof(acp_lodi_record) == sizeof(acp::LodiNode) &&
+                  offsetof(acp_lodi_record, corner) == offsetof(acp::LodiNode, corner) &&
+                  offsetof(acp_lodi_record, dir) == offsetof(acp::LodiNode, dir),
+              "acp_lodi_record mirrors the kernels' record");
+
+extern "C" int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* records, int64_t cap,
+                                     int64_t* launch_start, int32_t launch_cap, int32_t* nlaunch) {
+    if (nx < 3 || ny < 3 || nz < 3) return 0;
+    acp::LodiSchedule s = acp::build_lodi_schedule(nx, ny, nz);
+    const int64_t n = (int64_t)s.nodes.size();
+    if (records && cap > 0) std::memcpy(records, s.nodes.data(), sizeof(acp_lodi_record) * (size_t)std::min(n, cap));
+    const int32_t nl = (int32_t)s.phase_start.size() - 1;
+    for (int32_t p = 0; p <= nl && p < launch_cap; ++p)
+        if (launch_start) launch_start[p] = s.phase_start[p];
+    if (nlaunch) *nlaunch = nl;
     return n;
 }
 

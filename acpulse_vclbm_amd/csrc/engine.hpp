@@ -30,9 +30,12 @@ void kshift_init(KShift& k, const acp_case_desc& d);
 // (x faces, z faces, y faces; Lattice.cpp:1582-1624) grouped into phases so that
 // a node reading another boundary node's B entries sees them exactly as the
 // reference's sequential loops do.
+// LODI work of one step: records [phase_start[p], phase_start[p+1]) form
+// launch p, the nodes of dependency phase p + 1 of the reference's sequential
+// pass order.
 struct LodiSchedule {
-    std::vector<LodiNode> nodes;      // sorted by phase
-    std::vector<int64_t> phase_start; // [nphase + 1]
+    std::vector<LodiNode> nodes;
+    std::vector<int64_t> phase_start;  // [nphase + 1]
 };
 LodiSchedule build_lodi_schedule(int nx, int ny, int nz);
 
@@ -71,7 +74,7 @@ struct acp_case {
     int32_t* probe_slot = nullptr;  // device row counter of the probe series
     int32_t nprobe = 0, probe_cap = 0, probe_count = 0;
     int64_t probe_bytes = 0;
-    // staging buffer of the shifted tiers' float64 conversions
+    // staging buffer of the host transfers
     double* stage = nullptr;
     int64_t stage_elems = 0;
     // caller-owned device region the case lives in (acp_case_create_in), or null

@@ -23,6 +23,7 @@ struct LodiNode {
     int64_t node, n1, n2, t[4];
     int32_t dir, corner;
 };
+static_assert(sizeof(LodiNode) == 64, "one LODI record per 64 bytes");
 
 // xopen (periodic grids only): the x faces are not wrapped; their nodes
 // collide like interior nodes and links leaving the grid are dropped (the
@@ -39,6 +40,11 @@ hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64
                          double* psi, const double* macro, const Grid& g, hipStream_t s);
 hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Grid& g, bool periodic,
                           hipStream_t s);
+// Workgroup size of a LODI phase: a phase of fewer than 16 Ki nodes (the x
+// faces of a quasi-2D grid, edges, corners) fills few CUs with 256-thread
+// workgroups; 64-thread workgroups spread it over four times as many.
+inline int lodi_block(int64_t count) { return count >= ((int64_t)1 << 14) ? 256 : 64; }
+
 hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, const double* A, double* B,
                          const double* psi, const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s);
 hipError_t launch_init_random64(int order, const K64* k, double* A, double* B, const Grid& g, uint64_t seed,

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const 
 hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, const double* A, double* B,
                          const double* psi, const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    const int bs = 256;
+    const int bs = lodi_block(count);
     dim3 grid((unsigned)((count + bs - 1) / bs));
     const int tr = transverse ? 1 : 0;
 #define L64(O, C) hipLaunchKernelGGL((lodi64<O, C>), grid, dim3(bs), 0, s, k, A, B, psi, nodes, count, L, tr)

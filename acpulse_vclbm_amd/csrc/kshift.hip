@@ -497,7 +497,8 @@ __global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp,
 hipError_t launch_lodi_shift(int prec, int order, bool transverse, const KShift* k, const void* A, void* B,
                              const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s) {
     if (count <= 0) return hipSuccess;
-    dim3 grid((unsigned)((count + 255) / 256)), block(256);
+    const int bs = lodi_block(count);
+    dim3 grid((unsigned)((count + bs - 1) / bs)), block(bs);
     const int tr = transverse ? 1 : 0;
 #define LS(T, O) hipLaunchKernelGGL((lodi_shift<T, O>), grid, block, 0, s, k, (const T*)A, (T*)B, nodes, count, L, tr)
     if (prec == ACP_PREC_F32_SHIFTED) {

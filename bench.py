@@ -24,6 +24,11 @@ with every request arriving at once; Poisson arrivals (seed 1) at 1.2x the
 sequential capacity are then offered to the concurrent server and to the
 sequential float64 server; cases/hour and p50/p99 latency (arrival to result).
 
+LODI path (in "lodi"): three grids with LODI faces (the reference's case-file
+box 151 x 151 x 21 with HRR, the quasi-2D 1024 x 1024 x 3 serving grid and a
+256^3 cube, float64): whole-grid GLUPS and the step time against the same grid
+with periodic faces.
+
 Precision ablation (in "precision_ablation"): acpulse_vclbm_amd.ablation, the
 200 x 200 x 3 periodic HRR pulse at three amplitudes (field and probe errors of
 the shifted tiers against float64) and the reflection coefficient of the LODI
@@ -95,6 +100,53 @@ def copy_ceiling(acp, torch, n, reps=5):
     ms1 = acp.copy_probe(1, 19 * nodes, reps, stream.cuda_stream)
     return {"layout_gbs": nbytes / (msb * 1e-3) / 1e9, "planes19_gbs": nbytes / (ms19 * 1e-3) / 1e9,
             "planes1_gbs": nbytes / (ms1 * 1e-3) / 1e9}
+
+
+LODI_CASES = [
+    ("reference case-file box, HRR sigma 0.98", (151, 151, 21), "HRR"),
+    ("quasi-2D serving grid, SRT", (1024, 1024, 3), "SRT"),
+    ("cube, SRT", (256, 256, 256), "SRT"),
+]
+
+
+def lodi_path(acp, torch, steps=40, warmup=5):
+    """The LODI boundary path (float64, N_eq 3, tau 0.5003, classical LODI on
+    every face): each grid is timed with LODI faces and, for the bulk-kernel
+    share, as the same grid with periodic faces (HIP events on the case
+    stream).  whole-grid GLUPS counts every node; the boundary-node rate is
+    the LODI step's extra time over the periodic step per boundary node."""
+    out = []
+    for name, (nx, ny, nz), op in LODI_CASES:
+        ms = {}
+        for bnd in ("LODI", "PERIODIC"):
+            stream = torch.cuda.Stream()
+            d = acp.make_desc(nx, ny, nz, op=op, order=3, boundary=bnd, tau=0.5003, sigma=0.98)
+            case = acp.Case(d, stream=stream.cuda_stream)
+            try:
+                case.init_pulse(1e-3, nx / 2.0, ny / 2.0, 8.0)
+                case.step(warmup)
+                case.sync()
+                t0 = torch.cuda.Event(enable_timing=True)
+                t1 = torch.cuda.Event(enable_timing=True)
+                t0.record(stream)
+                case.step(steps)
+                t1.record(stream)
+                t1.synchronize()
+                ms[bnd] = t0.elapsed_time(t1) / steps
+            finally:
+                case.close()
+                del case
+                torch.cuda.synchronize()
+        nodes = nx * ny * nz
+        boundary = nodes - (nx - 2) * (ny - 2) * (nz - 2)
+        extra = max(ms["LODI"] - ms["PERIODIC"], 1e-9)
+        out.append({"case": name, "grid": [nx, ny, nz], "op": op, "nodes": nodes, "boundary_nodes": boundary,
+                    "ms_per_step_lodi": round(ms["LODI"], 4), "ms_per_step_periodic": round(ms["PERIODIC"], 4),
+                    "glups_whole_grid": round(nodes / (ms["LODI"] * 1e-3) / 1e9, 3),
+                    "glups_periodic_same_grid": round(nodes / (ms["PERIODIC"] * 1e-3) / 1e9, 3),
+                    "lodi_share_of_step": round(extra / ms["LODI"], 3),
+                    "boundary_mlups_extra": round(boundary / (extra * 1e-3) / 1e6, 1)})
+    return out
 
 
 def node_updates_per_step(serve, r):
@@ -193,6 +245,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--no-refinement", action="store_true")
+    ap.add_argument("--no-lodi", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48, help="256 for the full request mix")
     ap.add_argument("--serve-step-scale", type=float, default=0.25, help="1.0 for the full step counts")
     ap.add_argument("--serve-max-active", type=int, default=16)
@@ -263,6 +316,13 @@ def main():
             print(f"# refinement (256x128x3 + fine half, RR, F2C {filt}): interface echo "
                   f"{x['R_vs_uniform_coarse']:.2e} of the pulse, {x['ms_per_coarse_step']:.3f} ms per coarse step",
                   file=sys.stderr, flush=True)
+    lodi = None
+    if not a.no_lodi:
+        lodi = lodi_path(acp, torch)
+        for x in lodi:
+            print(f"# LODI {x['case']} {x['grid'][0]}x{x['grid'][1]}x{x['grid'][2]}: {x['glups_whole_grid']:.2f} "
+                  f"GLUPS whole grid ({x['ms_per_step_lodi']:.3f} ms/step; periodic {x['ms_per_step_periodic']:.3f}"
+                  f" ms), LODI share {100 * x['lodi_share_of_step']:.0f} %", file=sys.stderr, flush=True)
     cpu = None if a.no_cpu_baseline else cpu_baseline()
     out = {
         "metric": "GLUPS & %HBM roofline, FP64 vs FP32-shifted; p50/p99 case latency",
@@ -293,6 +353,7 @@ def main():
         "serving": srv,
         "precision_ablation": abl,
         "refinement": ref,
+        "lodi": lodi,
         "cpu_baseline": cpu,
     }
     if "F32-SHIFTED" in res and "F64" in res:

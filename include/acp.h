@@ -182,10 +182,23 @@ int acp_write_vtk_host(const acp_case_desc* desc, const double* fcur, const doub
                        const char* stem);
 
 /* Introspection, no counterpart in the reference: the boundary nodes of an
- * nx*ny*nz LODI case in the order the engine processes them (phase by phase,
- * phases derived from the reference's pass order, Lattice.cpp:1582-1624;
- * within a phase by ascending node index) and the phase of each.  Returns the node count; writes at most `cap` entries. */
+ * nx*ny*nz LODI case by dependency phase (phases derived from the reference's
+ * pass order, Lattice.cpp:1582-1624; within a phase by ascending node index)
+ * and the phase of each.  Returns the node count; writes at most `cap` entries. */
 int64_t acp_lodi_schedule(int32_t nx, int32_t ny, int32_t nz, int64_t* nodes, int32_t* phases, int64_t cap);
+/* The same nodes as the LODI kernels receive them, one record each: the node,
+ * its normal and second-normal neighbours (whose B entries it reads), four
+ * tangent neighbours (-1: none), inward normal direction and corner flag as
+ * Lattice::setConnectivity assigns them (Lattice.cpp:500-1392).
+ * `launch_start` receives the first record of every launch (nlaunch + 1
+ * entries).  Returns the record count; writes at most `cap` records and
+ * `launch_cap` launch starts; *nlaunch receives the launch count. */
+typedef struct acp_lodi_record {
+    int64_t node, n1, n2, t[4];
+    int32_t dir, corner;
+} acp_lodi_record;
+int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* records, int64_t cap,
+                          int64_t* launch_start, int32_t launch_cap, int32_t* nlaunch);
 
 /* ------------------------------------------------------------------ two-level refinement
  * Lattice::solve(bool&, bool&) with MaxLevel 1 (Lattice.cpp:2185-2209, nested

@@ -135,3 +135,59 @@ def test_copy_probe_rejects_bad_arguments(planes, elems, block):
     ms = ctypes.c_double()
     rc = acp.lib().acp_copy_probe(planes, elems, block, 1, None, ctypes.byref(ms))
     assert rc == _capi.ACP_E_ARG
+
+
+def _pass_position(nx, ny, nz):
+    """Position of every boundary node in the reference's sequential LODI pass
+    (Lattice.cpp:1582-1624: x faces by (j, k), then z faces by (i, j), then y
+    faces by (i, k), each face pair low then high)."""
+    pos, t = {}, 0
+    lin = lambda i, j, k: i + nx * (j + ny * k)
+    for j in range(ny):
+        for k in range(nz):
+            for i in (0, nx - 1):
+                pos[lin(i, j, k)] = t
+                t += 1
+    for i in range(1, nx - 1):
+        for j in range(ny):
+            for k in (0, nz - 1):
+                pos[lin(i, j, k)] = t
+                t += 1
+    for i in range(1, nx - 1):
+        for k in range(1, nz - 1):
+            for j in (0, ny - 1):
+                pos[lin(i, j, k)] = t
+                t += 1
+    return pos
+
+
+@pytest.mark.parametrize("dims", [(3, 3, 3), (5, 4, 3), (7, 5, 3), (9, 7, 3), (4, 5, 4), (8, 3, 6), (7, 6, 5),
+                                  (33, 17, 3), (12, 9, 4), (40, 30, 3)])
+def test_lodi_launch_order_keeps_the_reference_reads(dims):
+    """Every LODI update reads the B entries of its normal neighbours.  In the
+    reference's sequential pass a read sees a boundary node's new entries when
+    that node comes earlier in the pass, its old ones otherwise.  The engine
+    runs launches one after another and the records of a launch in parallel.
+    Each read must see the same version as in the reference and never race
+    with a write in the same launch; each boundary node is updated exactly
+    once."""
+    nx, ny, nz = dims
+    rec, starts = acp.lodi_launches(*dims)
+    n = len(rec)
+    assert n == nx * ny * nz - (nx - 2) * (ny - 2) * (nz - 2)
+    pos = _pass_position(nx, ny, nz)
+    assert sorted(pos) == sorted(rec["node"].tolist())
+    assert starts[0] == 0 and starts[-1] == n and np.all(np.diff(starts) > 0)
+    launch = {}  # node -> launch
+    for p in range(len(starts) - 1):
+        for r in range(int(starts[p]), int(starts[p + 1])):
+            assert rec["node"][r] not in launch
+            launch[int(rec["node"][r])] = p
+    assert len(launch) == n
+    for r in range(n):
+        a = int(rec["node"][r])
+        for x in (int(rec["n1"][r]), int(rec["n2"][r])):
+            if x not in pos:
+                continue  # an interior node: its B entries are final before the LODI pass
+            assert launch[x] != launch[a], f"node {a} reads node {x} written in the same launch"
+            assert (launch[x] < launch[a]) == (pos[x] < pos[a]), (a, x)
