@@ -476,8 +476,8 @@ int validate_desc(const acp_case_desc& d) {
     if (d.precision < ACP_PREC_F64 || d.precision > ACP_PREC_F16_SHIFTED) return fail(ACP_E_ARG, "unknown precision");
     if (d.boundary != ACP_BC_LODI && d.boundary != ACP_BC_PERIODIC && d.boundary != ACP_BC_OPEN_X)
         return fail(ACP_E_ARG, "unknown boundary");
-    if (d.boundary == ACP_BC_OPEN_X && (d.op == ACP_OP_HRR || d.cubic))
-        return fail(ACP_E_UNSUPPORTED, "open x faces: HRR and the cubic-Mach correction need neighbours beyond them");
+    if (d.boundary == ACP_BC_OPEN_X && d.cubic)
+        return fail(ACP_E_UNSUPPORTED, "open x faces: the cubic-Mach terms are not carried across them");
     if (d.precision != ACP_PREC_F64 && d.cubic)
         return fail(ACP_E_UNSUPPORTED, "the cubic-Mach correction is available in the float64 tier only");
     if (!(d.cs > 0) || !(d.spacing > 0) || !(d.nu > 0) || !(d.rho0 > 0))

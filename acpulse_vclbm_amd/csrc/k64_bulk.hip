@@ -77,6 +77,19 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                 for (int m = 0; m < 6; ++m)
 #pragma unroll
                     for (int v = 0; v < 4; ++v) nb[m][v] = P::soa(M, v, (I)(l + o[m]), L.N);
+                if constexpr (PERIODIC) {
+                    // open x face: the missing neighbour's macros by quadratic
+                    // extrapolation, u(-1) = 3 u(0) - 3 u(1) + u(2), which makes the
+                    // central difference the second-order one-sided one
+                    if (xopen && (i == 0 || i == nx - 1)) {
+                        const int m = i == 0 ? 2 : 0;
+                        const I s1 = i == 0 ? (I)1 : (I)-1;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            nb[m][v] = (3. * P::soa(M, v, l, L.N) - 3. * P::soa(M, v, (I)(l + s1), L.N)) +
+                                       P::soa(M, v, (I)(l + s1 + s1), L.N);
+                    }
+                }
             }
             collide_rr64<ORDER, OP == OP_HRR, CUBIC>(k, f, ps, nb);
             if constexpr (OP == OP_HRR || CUBIC) {

@@ -31,7 +31,7 @@ __device__ __forceinline__ void load_dev(const T* __restrict__ F, const PopLayou
 template <int OP, int ORDER, bool NARROW>
 __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], const float* __restrict__ M,
                                               typename Plane<NARROW>::idx l,
-                                              const typename Plane<NARROW>::idx o[6], int64_t N) {
+                                              const typename Plane<NARROW>::idx o[6], int64_t N, int ghost) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     float d[19];
@@ -64,6 +64,16 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
             for (int m = 0; m < 6; ++m)
 #pragma unroll
                 for (int a = 0; a < 3; ++a) v[m][a] = P::soa(M, a + 1, (I)(l + o[m]), N);
+            if (ghost) {
+                // open x face (ghost = +1: no +x neighbour, -1: no -x neighbour):
+                // quadratic extrapolation of the missing neighbour's velocity
+                const int m = ghost > 0 ? 0 : 2;
+                const I s1 = ghost > 0 ? (I)-1 : (I)1;
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    v[m][a] = (3.f * P::soa(M, a + 1, l, N) - 3.f * P::soa(M, a + 1, (I)(l + s1), N)) +
+                              P::soa(M, a + 1, (I)(l + s1 + s1), N);
+            }
             const float S[6] = {0.5f * (v[0][0] - v[2][0]), 0.5f * (v[1][1] - v[3][1]),
                                 0.5f * (v[4][2] - v[5][2]),
                                 0.5f * (0.5f * (v[0][1] - v[2][1]) + 0.5f * (v[1][0] - v[3][0])),
@@ -202,7 +212,8 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     if constexpr (OP != OP_STREAM) {
         if (!boundary) {
             const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-            collide_shift<OP, ORDER, NARROW>(k, g, M, l, o, L.N);
+            const int ghost = (PERIODIC && xopen) ? (i == nx - 1 ? 1 : (i == 0 ? -1 : 0)) : 0;
+            collide_shift<OP, ORDER, NARROW>(k, g, M, l, o, L.N, ghost);
         }
     }
     // periodic rows held by one workgroup: x-shifted links realigned through
@@ -296,7 +307,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             const I oxp = i == nx - 1 ? (I)0 - (I)(nx - 1) : (I)1;
             const I oxm = i == 0 ? (I)(nx - 1) : (I)-1;
             const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-            collide_shift<OP, ORDER, NARROW>(k, g[u], M, (I)(rs + (I)i), o, L.N);
+            collide_shift<OP, ORDER, NARROW>(k, g[u], M, (I)(rs + (I)i), o, L.N, 0);
         }
     }
     __shared__ T row[2 * kBulkBlock];

@@ -490,8 +490,7 @@ namespace {
 // The coarse description's checks and the fine level's description.
 int fine_desc(const acp_case_desc& d, int32_t x0, int32_t x1, acp_case_desc& fd) {
     if (d.boundary != ACP_BC_PERIODIC) return fail(ACP_E_UNSUPPORTED, "refined cases need a periodic coarse level");
-    if (d.op == ACP_OP_HRR || d.cubic)
-        return fail(ACP_E_UNSUPPORTED, "refined cases: HRR and the cubic-Mach correction are not coupled");
+    if (d.cubic) return fail(ACP_E_UNSUPPORTED, "refined cases: the cubic-Mach terms are not coupled");
     if (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx || x1 - x0 >= d.nx - 1)
         return fail(ACP_E_ARG, "fine region [x0, x1] must satisfy 1 <= x0, x0 + 2 <= x1 <= nx, x1 - x0 < nx - 1");
     fd = d;

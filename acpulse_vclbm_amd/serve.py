@@ -22,9 +22,7 @@ Grids are nx x ny x 3, tau = 0.5 + 3e-4, N_eq = 3, the probe at
     nodes), the pulse at (nx/4, ny/2) in the coarse half, the probe read on the
     coarse level inside the fine region; standard coupling with the F2C filter
     NOFI.  The direct coupling is not available: requests that draw it run
-    with the standard coupling and are counted as such.  HRR is not coupled
-    across levels: refined HRR requests run on their one-level grid and are
-    counted.
+    with the standard coupling and are counted as such.
 """
 import ctypes
 import time
@@ -65,8 +63,8 @@ def make_requests(n=256, seed=1, rate=None, step_scale=1.0, nxy=(128, 1024), ste
 
 
 def two_level(r):
-    """Whether the request runs as a refined case (HRR is not coupled across levels)."""
-    return bool(r.get("refined")) and r["op"] != "HRR"
+    """Whether the request runs as a refined case."""
+    return bool(r.get("refined"))
 
 
 def fine_region(r):
@@ -101,7 +99,6 @@ def mix_counts(reqs):
     """How the refinement flags of a request list were served."""
     return {"requests": len(reqs), "refined_flag": sum(bool(r["refined"]) for r in reqs),
             "two_level": sum(two_level(r) for r in reqs),
-            "refined_hrr_one_level": sum(bool(r["refined"]) and r["op"] == "HRR" for r in reqs),
             "direct_run_as_overlap": sum(two_level(r) and r["scheme"] == "direct" for r in reqs)}
 
 

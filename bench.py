@@ -309,11 +309,11 @@ def main():
     if not a.no_refinement:
         from acpulse_vclbm_amd import refinement
         ref = {}
-        for filt in ("NOFI", "LAG"):
-            x = refinement.interface_reflection(acp, f2c_filter=filt)
+        for op, filt in (("RR", "NOFI"), ("RR", "LAG"), ("HRR", "NOFI")):
+            x = refinement.interface_reflection(acp, op=op, f2c_filter=filt)
             x.pop("probe", None)
-            ref[filt] = x
-            print(f"# refinement (256x128x3 + fine half, RR, F2C {filt}): interface echo "
+            ref[f"{op} {filt}"] = x
+            print(f"# refinement (256x128x3 + fine half, {op}, F2C {filt}): interface echo "
                   f"{x['R_vs_uniform_coarse']:.2e} of the pulse, {x['ms_per_coarse_step']:.3f} ms per coarse step",
                   file=sys.stderr, flush=True)
     lodi = None

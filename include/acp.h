@@ -44,7 +44,10 @@ enum { ACP_PREC_F64 = 0, ACP_PREC_F32_SHIFTED = 1, ACP_PREC_F16_SHIFTED = 2 };
  * PERIODIC wraps all three axes.  OPEN_X wraps y and z; the x faces are left
  * open: their nodes collide like interior nodes and links leaving the grid are
  * dropped, so something else must set them (the fine level of a refined case,
- * acp_refined_*); not available with HRR or the cubic-Mach correction. */
+ * acp_refined_*).  HRR's strain stencil takes the missing neighbour beyond an
+ * open face by quadratic extrapolation (u(-1) = 3 u(0) - 3 u(1) + u(2), the
+ * second-order one-sided difference); not available with the cubic-Mach
+ * correction. */
 enum { ACP_BC_LODI = 0, ACP_BC_PERIODIC = 1, ACP_BC_OPEN_X = 2 };
 /* MRT operators: the reference computes the relaxed populations and discards them
  * (Node.cpp:2603-2604, 2833-2834), i.e. MRT steps are pure streaming + LODI. */
@@ -210,9 +213,10 @@ int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* r
  * differs from the reference's fine shell (Lattice.cpp:38-126): the coarse
  * level is a periodic grid and the fine level covers coarse x in [x0, x1]
  * over every y and z at half the spacing and half the time step, with
- * ratio-2 vertex-centred coupling on its two x faces.  SRT, RR (no cubic
- * correction), MRT-GS and MRT-RM, every storage tier (both levels use the
- * coarse description's tier). */
+ * ratio-2 vertex-centred coupling on its two x faces.  Every operator
+ * without the cubic-Mach correction (HRR's strain stencil extrapolates at the
+ * fine level's open faces, see ACP_BC_OPEN_X), every storage tier (both
+ * levels use the coarse description's tier). */
 typedef struct acp_refined acp_refined;
 int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, acp_refined** out);
 void acp_refined_destroy(acp_refined* r);

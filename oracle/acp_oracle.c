@@ -862,6 +862,19 @@ static void one_step(acpo_case* c) {
                     if (need_nb) {
                         for (int m = 0; m < 6; ++m) {
                             int a = ii + E[m][0], b = jj + E[m][1], d = kk + E[m][2];
+                            if (p->periodic == 2 && (a < 0 || a >= nx)) {
+                                /* open x face: the missing neighbour's macros by quadratic
+                                 * extrapolation, u(-1) = 3 u(0) - 3 u(1) + u(2), which turns
+                                 * the central difference into the second-order one-sided one */
+                                const int s1 = a < 0 ? 1 : -1;
+                                const int64_t o0 = lin(c, ii, jj, kk), o1 = lin(c, ii + s1, jj, kk);
+                                const int64_t o2 = lin(c, ii + 2 * s1, jj, kk);
+                                for (int v = 0; v < 4; ++v) {
+                                    const double* M = c->macro + (size_t)v * N;
+                                    nb[m][v] = (3. * M[o0] - 3. * M[o1]) + M[o2];
+                                }
+                                continue;
+                            }
                             if (p->periodic) { a = (a + nx) % nx; b = (b + ny) % ny; d = (d + nz) % nz; }
                             const int64_t o = lin(c, a, b, d);
                             for (int v = 0; v < 4; ++v) nb[m][v] = c->macro[(size_t)v * N + o];

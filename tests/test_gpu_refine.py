@@ -11,7 +11,7 @@ from make_golden import CS, DX, RHO0
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("op,mrt", [("SRT", 0), ("RR", 0), ("MRT-RM", 1), ("MRT-GS", 1)])
+@pytest.mark.parametrize("op,mrt", [("SRT", 0), ("RR", 0), ("MRT-RM", 1), ("MRT-GS", 1), ("HRR", 0)])
 def test_refined_matches_oracle_bitexact(gpu, op, mrt):
     nx, ny, nz, x0, x1 = 24, 10, 3, 8, 16
     p = ol.make_params(nx, ny, nz, op=op, periodic=True, tau=0.6, mrt_mode=mrt, cs=CS, spacing=DX, rho0=RHO0)
@@ -94,10 +94,42 @@ def test_refined_lag_shifted_tiers_follow_float64(gpu):
     assert a["F32-SHIFTED coupling F32-SHIFTED"]["probe_l2"] < 1e-3
 
 
+@pytest.mark.parametrize("order", [3, 4])
+def test_open_x_hrr_matches_oracle_bitexact(gpu, order):
+    """HRR with open x faces (the fine level's boundary kind): the strain
+    stencil's extrapolated neighbour at the faces, bit for bit against the
+    oracle, and the float32-shifted tier within 2e-5 of the largest density
+    deviation."""
+    nx, ny, nz = 12, 8, 5
+    p = ol.make_params(nx, ny, nz, op="HRR", order=order, periodic=2, tau=0.56, sigma=0.7, cs=CS, spacing=DX,
+                       rho0=RHO0)
+    rng = np.random.default_rng(order)
+    N = nx * ny * nz
+    rho = RHO0 * (1.0 + 0.01 * rng.uniform(-1, 1, N))
+    u = 0.05 * CS * rng.uniform(-1, 1, (3, N))
+    f0 = ol.equilibrium_field(p, rho, u)
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(6)
+    d = acp.make_desc(nx, ny, nz, op="HRR", order=order, tau=0.56, sigma=0.7, boundary="OPEN-X")
+    with acp.Case(d) as c:
+        c.set_populations(f0)
+        c.step(6)
+        assert np.array_equal(c.populations(), o.state())
+    d32 = acp.make_desc(nx, ny, nz, op="HRR", order=order, tau=0.56, sigma=0.7, boundary="OPEN-X",
+                        precision="F32-SHIFTED")
+    with acp.Case(d32) as c:
+        c.set_populations(f0)
+        c.step(6)
+        rho32, _ = c.macros()
+    r64 = o.state().sum(axis=0)
+    assert np.max(np.abs(rho32 - r64)) <= 2e-5 * np.max(np.abs(r64 - RHO0))
+
+
 def test_refined_rejects_unsupported():
-    d = acp.make_desc(24, 10, 3, op="HRR", tau=0.6, boundary="PERIODIC")
+    d = acp.make_desc(24, 10, 3, op="HRR", cubic=True, tau=0.6, boundary="PERIODIC")
     with pytest.raises(acp.AcpError):
-        acp.Refined(d, 8, 16)
+        acp.Refined(d, 8, 16)  # the cubic-Mach terms are not coupled
     d = acp.make_desc(24, 10, 3, op="SRT", tau=0.6)
     with pytest.raises(acp.AcpError):
         acp.Refined(d, 8, 16)  # LODI coarse level

@@ -28,7 +28,8 @@ def test_scale_factor_is_tau_ratio(tau):
 
 @pytest.mark.parametrize("op,moving,filt", [("SRT", True, "NOFI"), ("RR", True, "NOFI"), ("SRT", False, "NOFI"),
                                             ("RR", False, "NOFI"), ("MRT-RM", False, "NOFI"),
-                                            ("MRT-GS", False, "NOFI"), ("SRT", True, "LAG"), ("RR", False, "LAG")])
+                                            ("MRT-GS", False, "NOFI"), ("SRT", True, "LAG"), ("RR", False, "LAG"),
+                                            ("HRR", True, "NOFI"), ("HRR", False, "LAG")])
 def test_uniform_state_stays_uniform(op, moving, filt):
     """A uniform equilibrium is a fixed point of the step on each level and of
     the coupling (time interpolation of equal values, rescaling of a zero
@@ -164,3 +165,40 @@ def test_lag_interface_reflection_is_small():
     sig = np.max(np.abs(pu - p.rho0))
     assert np.max(np.abs(pr[:25] - pu[:25])) / sig < 1e-3
     assert np.max(np.abs(pr - pu)) / sig < 0.05
+
+
+def test_open_x_hrr_strain_is_second_order_one_sided():
+    """HRR on a grid with open x faces (the fine level of a refined case): the
+    FD strain at a face node takes the missing neighbour's macros by quadratic
+    extrapolation, which makes the central difference the second-order
+    one-sided one, exact for a velocity linear in x.  So a grid whose face
+    lies at x = 0 evolves like a wider grid in which the same nodes are
+    interior: after one HRR step (sigma 0.5, the strain weighs half) the nodes
+    that received pushes from the face node agree to rounding.  A zero-gradient
+    ghost (u(-1) = u(0)) would halve the face strain and miss by orders of
+    magnitude."""
+    ny, nz, shift = 4, 3, 2
+    kw = dict(op="HRR", order=3, periodic=2, tau=0.6, sigma=0.5, nthreads=1)
+    pa = ol.make_params(6, ny, nz, **kw)
+    pb = ol.make_params(6 + 2 * shift, ny, nz, **kw)
+    G = 0.02 * pa.cs / 6.0  # u_x = G * x (x in nodes)
+
+    def state(p, x0):
+        N = p.nx * p.ny * p.nz
+        i = np.arange(N) % p.nx
+        u = np.zeros((3, N))
+        u[0] = G * (i - x0)
+        return ol.equilibrium_field(p, np.full(N, p.rho0), u)
+
+    a, b = ol.OracleCase(pa), ol.OracleCase(pb)
+    a.set_state(state(pa, 0))
+    b.set_state(state(pb, shift))
+    a.step(1)
+    b.step(1)
+    fa = a.state().reshape(19, nz, ny, pa.nx)
+    fb = b.state().reshape(19, nz, ny, pb.nx)
+    # nodes 1..4 of the narrow grid: every push they receive comes from nodes
+    # whose state and strain match the wide grid's nodes 3..6
+    da = fa[..., 1:5]
+    db = fb[..., 1 + shift:5 + shift]
+    assert np.max(np.abs(da - db)) <= 1e-12 * np.max(np.abs(db))

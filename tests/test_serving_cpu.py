@@ -109,13 +109,13 @@ def test_refined_footprint_is_both_levels():
 
 
 def test_refined_requests_in_the_mix():
-    """Half of the requests are flagged for refinement; those not running HRR
-    become two-level cases on a periodic coarse grid, with the fine level over
-    the upper half in x and the pulse in the lower half."""
+    """Half of the requests are flagged for refinement; they become two-level
+    cases on a periodic coarse grid, with the fine level over the upper half in
+    x and the pulse in the lower half."""
     reqs = serve.make_requests(256, seed=1)
     c = serve.mix_counts(reqs)
-    assert c["refined_flag"] == c["two_level"] + c["refined_hrr_one_level"]
-    assert 0.25 < c["two_level"] / len(reqs) < 0.55
+    assert c["refined_flag"] == c["two_level"]
+    assert 0.35 < c["two_level"] / len(reqs) < 0.65
     for r in reqs:
         d = serve.request_desc(r)
         if serve.two_level(r):
