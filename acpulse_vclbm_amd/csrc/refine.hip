@@ -40,6 +40,7 @@
 #include <algorithm>
 #include <cstring>
 #include <initializer_list>
+#include <type_traits>
 
 #include "acp.h"
 #include "engine.hpp"
@@ -59,35 +60,51 @@ __device__ __forceinline__ void rescale(const K64& k, double f[19], double a) {
     for (int q = 0; q < 19; ++q) f[q] = eq[q] + a * (f[q] - eq[q]);
 }
 
-// Coincident fine face nodes from the coarse interface column ci (and the
-// fine face column fi): one thread per coarse (J, K).
+// The populations a coincident fine face node takes from coarse node lc:
+// time-interpolated (half: at t + 1/2) and rescaled.
 template <int ORDER>
-__global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp, const double* __restrict__ Cprev,
-                                                      const double* __restrict__ Ccur, int half, double a,
-                                                      int cnx, int cny, int cnz, int ci, double* __restrict__ F,
-                                                      int fnx, int fny, int fi, const PopLayout CN,
-                                                      const PopLayout FN) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cny * cnz) return;
-    const int J = t % cny, K = t / cny;
-    const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
-    const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
-    double f[19];
+__device__ __forceinline__ void c2f_value(const K64& k, const double* __restrict__ Cprev,
+                                          const double* __restrict__ Ccur, const PopLayout& CN, int64_t lc, int half,
+                                          double a, double f[19]) {
 #pragma unroll
     for (int q = 0; q < 19; ++q)
         f[q] = half ? 0.5 * (Cprev[pidx(CN, q, lc)] + Ccur[pidx(CN, q, lc)]) : Ccur[pidx(CN, q, lc)];
-    rescale<ORDER>(*kp, f, a);
+    rescale<ORDER>(k, f, a);
+}
+
+// Coarse-to-fine on both fine x faces, in two launches.  Face 0 is fine
+// column 0 under coarse column ci0, face 1 fine column fnx - 1 under ci1.
+// First the coincident nodes (even j and k, one thread per coarse (J, K) of
+// either face) take their coarse node's value; then the other face nodes
+// average their 2 (mid nodes) or 4 (centre nodes) coincident neighbours.
+template <int ORDER>
+__global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp, const double* __restrict__ Cprev,
+                                                      const double* __restrict__ Ccur, int half, double a, int cnx,
+                                                      int cny, int cnz, int ci0, int ci1, double* __restrict__ F,
+                                                      int fnx, int fny, const PopLayout CN, const PopLayout FN) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = cny * cnz;
+    if (t >= 2 * per) return;
+    const int face = t / per, r = t - face * per;
+    const int J = r % cny, K = r / cny;
+    const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0;
+    const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
+    const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    double f[19];
+    c2f_value<ORDER>(*kp, Cprev, Ccur, CN, lc, half, a, f);
 #pragma unroll
     for (int q = 0; q < 19; ++q) F[pidx(FN, q, lf)] = f[q];
 }
 
-// The other fine face nodes (odd j or odd k) from their coincident neighbours.
-__global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int fnx, int fny, int fnz, int fi,
+__global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int fnx, int fny, int fnz,
                                                    const PopLayout FN) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= fny * fnz) return;
-    const int j = t % fny, k = t / fny;
+    const int per = fny * fnz;
+    if (t >= 2 * per) return;
+    const int face = t / per, r = t - face * per;
+    const int j = r % fny, k = r / fny;
     if (!(j & 1) && !(k & 1)) return;
+    const int fi = face ? fnx - 1 : 0;
     auto at = [&](int jj, int kk) {
         jj = (jj + fny) % fny;
         kk = (kk + fnz) % fnz;
@@ -98,7 +115,8 @@ __global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int f
         const int64_t a = at(j - 1, k - 1), b = at(j + 1, k - 1), c = at(j - 1, k + 1), d = at(j + 1, k + 1);
 #pragma unroll
         for (int q = 0; q < 19; ++q)
-            F[pidx(FN, q, l)] = ((F[pidx(FN, q, a)] + F[pidx(FN, q, b)]) + (F[pidx(FN, q, c)] + F[pidx(FN, q, d)])) * 0.25;
+            F[pidx(FN, q, l)] =
+                ((F[pidx(FN, q, a)] + F[pidx(FN, q, b)]) + (F[pidx(FN, q, c)] + F[pidx(FN, q, d)])) * 0.25;
     } else {
         const int64_t a = (j & 1) ? at(j - 1, k) : at(j, k - 1);
         const int64_t b = (j & 1) ? at(j + 1, k) : at(j, k + 1);
@@ -181,11 +199,12 @@ __device__ __forceinline__ T f_to_g(const KShift& ks, double f, int q) {
     return (T)((double)ks.s * (f / ks.rho0 - weight(q)));
 }
 
+// The stored values (tier T) a node receives from node lc of another level:
+// time-interpolated (half) and rescaled by a, in float64 on SI populations
+// (F64) or in float32 on lattice-unit deviations.
 template <typename T, int ORDER, bool F64>
-__device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
-                                            const PopLayout& CN, int64_t lc, int half, double a, T* F,
-                                            const PopLayout& FN,
-                                            int64_t lf) {
+__device__ __forceinline__ void couple_value(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
+                                             const PopLayout& CN, int64_t lc, int half, double a, T out[19]) {
     if constexpr (F64) {
         double f[19];
         static_for<0, 19>([&](auto qc) {
@@ -196,7 +215,7 @@ __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, cons
         rescale<ORDER>(k, f, a);
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            F[pidx(FN, q, lf)] = f_to_g<T>(ks, f[q], q);
+            out[q] = f_to_g<T>(ks, f[q], q);
         });
     } else {
         float d[19], eq[19], dr, ux, uy, uz;
@@ -210,9 +229,19 @@ __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, cons
         const float af = (float)a;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            F[pidx(FN, q, lf)] = (T)((eq[q] + af * (d[q] - eq[q])) * ks.s);
+            out[q] = (T)((eq[q] + af * (d[q] - eq[q])) * ks.s);
         });
     }
+}
+
+template <typename T, int ORDER, bool F64>
+__device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
+                                            const PopLayout& CN, int64_t lc, int half, double a, T* F,
+                                            const PopLayout& FN, int64_t lf) {
+    T v[19];
+    couple_value<T, ORDER, F64>(k, ks, Cprev, Ccur, CN, lc, half, a, v);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) F[pidx(FN, q, lf)] = v[q];
 }
 
 // LAG restriction in the shifted tiers: the same mean as lag_mean, in float64
@@ -282,15 +311,21 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
     }
 }
 
+// The coarse-to-fine kernels of the shifted tiers (see c2f_coincident and
+// c2f_between); the averages are formed from the coincident values as
+// stored, in float64 (F64 coupling) or float32.
 template <typename T, int ORDER, bool F64>
 __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                                         const T* __restrict__ Cprev, const T* __restrict__ Ccur,
-                                                        int half, double a, int cnx, int cny, int cnz, int ci,
-                                                        T* __restrict__ F, int fnx, int fny, int fi,
+                                                        int half, double a, int cnx, int cny, int cnz, int ci0,
+                                                        int ci1, T* __restrict__ F, int fnx, int fny,
                                                         const PopLayout CN, const PopLayout FN) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= cny * cnz) return;
-    const int J = t % cny, K = t / cny;
+    const int per = cny * cnz;
+    if (t >= 2 * per) return;
+    const int face = t / per, r = t - face * per;
+    const int J = r % cny, K = r / cny;
+    const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0;
     const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
     const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
     couple_node<T, ORDER, F64>(*kp, *ksp, Cprev, Ccur, CN, lc, half, a, F, FN, lf);
@@ -298,11 +333,14 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
 
 template <typename T, bool F64>
 __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ ksp, T* __restrict__ F, int fnx,
-                                                     int fny, int fnz, int fi, const PopLayout FN) {
+                                                     int fny, int fnz, const PopLayout FN) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= fny * fnz) return;
-    const int j = t % fny, k = t / fny;
+    const int per = fny * fnz;
+    if (t >= 2 * per) return;
+    const int face = t / per, r = t - face * per;
+    const int j = r % fny, k = r / fny;
     if (!(j & 1) && !(k & 1)) return;
+    const int fi = face ? fnx - 1 : 0;
     const KShift& ks = *ksp;
     auto at = [&](int jj, int kk) {
         jj = (jj + fny) % fny;
@@ -377,30 +415,29 @@ namespace {
 
 using acp::fail;
 
+// Coarse-to-fine on both fine faces: coincident nodes, then the others.
+// These are a few thousand latency-bound threads each: 64-thread workgroups
+// spread them over more CUs.
 template <typename T, bool F64>
 hipError_t launch_c2f_s(acp_refined* r, int half) {
     acp_case* c = r->coarse;
     acp_case* f = r->fine;
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
     const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
-    const int faces[2][2] = {{r->x0 % cnx, 0}, {r->x1 % cnx, fnx - 1}};
-    for (const auto& fc : faces) {
-        const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
+    const dim3 b(64), g1((unsigned)((2 * cny * cnz + 63) / 64)), g2((unsigned)((2 * fny * fnz + 63) / 64));
 #define C2FS(O)                                                                                                  \
-    hipLaunchKernelGGL((acp::refine::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks, (const T*)c->SB,    \
-                       (const T*)c->SA, half, r->a, cnx, cny, cnz, fc[0], (T*)f->SA, fnx, fny, fc[1], c->g.L, f->g.L)
-        if (c->d.order == 2) C2FS(2);
-        else if (c->d.order == 3) C2FS(3);
-        else C2FS(4);
+    hipLaunchKernelGGL((acp::refine::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks,            \
+                       (const T*)c->SB, (const T*)c->SA, half, r->a, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx,     \
+                       (T*)f->SA, fnx, fny, c->g.L, f->g.L)
+    if (c->d.order == 2) C2FS(2);
+    else if (c->d.order == 3) C2FS(3);
+    else C2FS(4);
 #undef C2FS
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream,
-                           f->dks, (T*)f->SA, fnx, fny, fnz, fc[1], f->g.L);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), g2, b, 0, r->stream, f->dks, (T*)f->SA, fnx, fny, fnz,
+                       f->g.L);
+    return hipGetLastError();
 }
 
 template <typename T, bool F64>
@@ -433,27 +470,20 @@ hipError_t launch_c2f(acp_refined* r, int half) {
         return r->f64_coupling ? launch_c2f_s<float, true>(r, half) : launch_c2f_s<float, false>(r, half);
     if (c->precision == ACP_PREC_F16_SHIFTED)
         return r->f64_coupling ? launch_c2f_s<_Float16, true>(r, half) : launch_c2f_s<_Float16, false>(r, half);
-    const int order = c->d.order;
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
     const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
-    const int faces[2][2] = {{r->x0 % cnx, 0}, {r->x1 % cnx, fnx - 1}};
-    for (const auto& fc : faces) {
-        const dim3 g1((unsigned)((cny * cnz + 255) / 256)), b(256);
-#define C2F(O)                                                                                                   \
-    hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, cny, cnz, \
-                       fc[0], f->A, fnx, fny, fc[1], c->g.L, f->g.L)
-        if (order == 2) C2F(2);
-        else if (order == 3) C2F(3);
-        else C2F(4);
+    const dim3 b(64), g1((unsigned)((2 * cny * cnz + 63) / 64)), g2((unsigned)((2 * fny * fnz + 63) / 64));
+#define C2F(O)                                                                                                    \
+    hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, \
+                       cny, cnz, r->x0 % cnx, r->x1 % cnx, f->A, fnx, fny, c->g.L, f->g.L)
+    if (c->d.order == 2) C2F(2);
+    else if (c->d.order == 3) C2F(3);
+    else C2F(4);
 #undef C2F
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(acp::refine::c2f_between, dim3((unsigned)((fny * fnz + 255) / 256)), b, 0, r->stream, f->A, fnx,
-                           fny, fnz, fc[1], f->g.L);
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-    }
-    return hipSuccess;
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(acp::refine::c2f_between, g2, b, 0, r->stream, f->A, fnx, fny, fnz, f->g.L);
+    return hipGetLastError();
 }
 
 hipError_t launch_f2c(acp_refined* r) {
