@@ -174,8 +174,10 @@ class _LevelCase(Case):
 
 
 class Refined:
-    """Two-level case (acp_refined_*): periodic coarse level `desc`, fine level
-    over coarse x in [x0, x1] at half the spacing.  level(0) / level(1) give the
+    """Two-level case (acp_refined_*): coarse level `desc` (PERIODIC or LODI-X
+    boundary), fine level over coarse x in [x0, x1] at half the spacing; with a
+    LODI-X coarse level and x1 = nx - 1 the fine level's high-x face is a LODI
+    face and only the face at x0 is coupled.  level(0) / level(1) give the
     levels' cases for probes, read-back and output; step with Refined.step."""
 
     def __init__(self, desc, x0, x1, stream=None):
@@ -185,7 +187,8 @@ class Refined:
         fd = CaseDesc.from_buffer_copy(desc)
         fd.nx, fd.ny, fd.nz = 2 * (x1 - x0) + 1, 2 * desc.ny, 2 * desc.nz
         fd.spacing = desc.spacing / 2.0
-        fd.boundary = BOUNDARY["OPEN-X"]
+        lodi_end = desc.boundary == BOUNDARY["LODI-X"] and x1 == desc.nx - 1
+        fd.boundary = BOUNDARY["OPEN-LODI-X" if lodi_end else "OPEN-X"]
         self.x0, self.x1 = int(x0), int(x1)
         self._levels = []
         for lv, d in ((0, desc), (1, fd)):

@@ -199,6 +199,7 @@ extern "C" int acp_control_read(const char* path, acp_case_desc* out) {
     if (r.str("Boundary", s, false)) {
         if (s == "LODI") d.boundary = ACP_BC_LODI;
         else if (s == "PERIODIC") d.boundary = ACP_BC_PERIODIC;
+        else if (s == "LODI-X") d.boundary = ACP_BC_LODI_X;
         else return acp::fail(ACP_E_CONFIG, "unknown Boundary \"" + s + "\"");
     }
     if (r.str("MRT-Mode", s, false)) {

@@ -296,6 +296,36 @@ LodiSchedule build_lodi_schedule(int nx, int ny, int nz) {
     return out;
 }
 
+LodiSchedule build_lodi_schedule_x(int nx, int ny, int nz, int faces) {
+    const int im = nx - 1;
+    auto lin = [&](int i, int j, int k) { return (int64_t)i + (int64_t)nx * ((int64_t)j + (int64_t)ny * k); };
+    // x-face records as for the reference's x faces (Lattice.cpp:826-908),
+    // tangent neighbours wrapped in y and z.  With nx >= 4 the normal
+    // neighbours are interior nodes, so no record reads another's B entries
+    // and the faces form one phase, emitted in linear node order.
+    LodiSchedule out;
+    out.nodes.reserve((size_t)((faces & 1) + ((faces >> 1) & 1)) * ny * nz);
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int side = 0; side < 2; ++side) {
+                if (!(faces & (1 << side))) continue;
+                const int i = side ? im : 0, s = side ? -1 : 1;  // s: inward x direction
+                LodiNode n;
+                n.node = lin(i, j, k);
+                n.n1 = lin(i + s, j, k);
+                n.n2 = lin(i + 2 * s, j, k);
+                n.t[0] = lin(i, (j + 1) % ny, k);
+                n.t[1] = lin(i, (j + ny - 1) % ny, k);
+                n.t[2] = lin(i, j, (k + 1) % nz);
+                n.t[3] = lin(i, j, (k + nz - 1) % nz);
+                n.dir = side ? 2 : 0;
+                n.corner = 0;
+                out.nodes.push_back(n);
+            }
+    out.phase_start = {0, (int64_t)out.nodes.size()};
+    return out;
+}
+
 }  // namespace acp
 
 using acp::fail;
@@ -467,6 +497,11 @@ constexpr int64_t kStageElems = (int64_t)1 << 20;
 // ran faster with whole planes.
 int64_t layout_block_nodes(int precision) { return precision == ACP_PREC_F16_SHIFTED ? 0 : 8192; }
 
+// The LODI x faces of a boundary kind: bit 0 the face i = 0, bit 1 i = nx - 1.
+int x_lodi_faces(int32_t boundary) {
+    return boundary == ACP_BC_LODI_X ? 3 : (boundary == ACP_BC_OPEN_LODI_X ? 2 : 0);
+}
+
 int validate_desc(const acp_case_desc& d) {
     if (d.max_level != 0)
         return fail(ACP_E_UNSUPPORTED, "MaxLevel > 0: two-level refinement is not part of this engine");
@@ -474,10 +509,11 @@ int validate_desc(const acp_case_desc& d) {
     if (d.order < 2 || d.order > 4) return fail(ACP_E_CONFIG, "Wrong order of equilibrium!");
     if (d.op < ACP_OP_SRT || d.op > ACP_OP_HRR) return fail(ACP_E_ARG, "unknown collision operator");
     if (d.precision < ACP_PREC_F64 || d.precision > ACP_PREC_F16_SHIFTED) return fail(ACP_E_ARG, "unknown precision");
-    if (d.boundary != ACP_BC_LODI && d.boundary != ACP_BC_PERIODIC && d.boundary != ACP_BC_OPEN_X)
-        return fail(ACP_E_ARG, "unknown boundary");
-    if (d.boundary == ACP_BC_OPEN_X && d.cubic)
+    if (d.boundary < ACP_BC_LODI || d.boundary > ACP_BC_OPEN_LODI_X) return fail(ACP_E_ARG, "unknown boundary");
+    if ((d.boundary == ACP_BC_OPEN_X || d.boundary == ACP_BC_OPEN_LODI_X) && d.cubic)
         return fail(ACP_E_UNSUPPORTED, "open x faces: the cubic-Mach terms are not carried across them");
+    if ((d.boundary == ACP_BC_LODI_X || d.boundary == ACP_BC_OPEN_LODI_X) && d.nx < 4)
+        return fail(ACP_E_ARG, "LODI x faces need at least 4 nodes in x");
     if (d.precision != ACP_PREC_F64 && d.cubic)
         return fail(ACP_E_UNSUPPORTED, "the cubic-Mach correction is available in the float64 tier only");
     if (!(d.cs > 0) || !(d.spacing > 0) || !(d.nu > 0) || !(d.rho0 > 0))
@@ -507,6 +543,9 @@ int64_t footprint_bytes(const acp_case_desc& d, int32_t nprobe, int32_t cap) {
     b += round_alloc(std::min<int64_t>(N, kStageElems) * 8);
     if (d.boundary == ACP_BC_LODI)
         b += round_alloc((N - (int64_t)(d.nx - 2) * (d.ny - 2) * (d.nz - 2)) * (int64_t)sizeof(acp::LodiNode));
+    else if (x_lodi_faces(d.boundary))
+        b += round_alloc((int64_t)(x_lodi_faces(d.boundary) == 3 ? 2 : 1) * d.ny * d.nz *
+                         (int64_t)sizeof(acp::LodiNode));
     b += round_alloc(std::min<int64_t>(1024, (N + 255) / 256) * 8);
     if (nprobe > 0)
         b += round_alloc((int64_t)nprobe * 8) + round_alloc((int64_t)nprobe * std::max(cap, 1) * 8) + round_alloc(4);
@@ -549,8 +588,10 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
     c->arena = (char*)region;
     c->arena_size = region_bytes;
     c->d = d;
-    c->g = acp::Grid{d.nx, d.ny, d.nz, (int64_t)d.nx * d.ny * d.nz, d.boundary == ACP_BC_OPEN_X ? 1 : 0};
+    c->g = acp::Grid{d.nx, d.ny, d.nz, (int64_t)d.nx * d.ny * d.nz,
+                     d.boundary == ACP_BC_OPEN_X ? 3 : (d.boundary == ACP_BC_OPEN_LODI_X ? 1 : 0)};
     c->g.L = acp::make_layout(c->g.N, layout_block_nodes(d.precision));
+    c->g.xlodi = x_lodi_faces(d.boundary);
     c->precision = d.precision;
     c->periodic = d.boundary != ACP_BC_LODI;
     switch (d.op) {
@@ -605,8 +646,9 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
         e = upload(c, c->dk, &c->hk, sizeof(acp::K64));
         if (e != hipSuccess) return bail(acp::hip_fail(e, "constants upload"));
     }
-    if (!c->periodic) {
-        acp::LodiSchedule sch = acp::build_lodi_schedule(d.nx, d.ny, d.nz);
+    if (!c->periodic || c->g.xlodi) {
+        acp::LodiSchedule sch = c->periodic ? acp::build_lodi_schedule_x(d.nx, d.ny, d.nz, c->g.xlodi)
+                                            : acp::build_lodi_schedule(d.nx, d.ny, d.nz);
         c->phase_start = sch.phase_start;
         if ((rc = dalloc(c, &c->lodi, sch.nodes.size()))) return bail(rc);
         hipError_t e = upload(c, c->lodi, sch.nodes.data(), sizeof(acp::LodiNode) * sch.nodes.size());

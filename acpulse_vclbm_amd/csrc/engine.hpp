@@ -38,6 +38,9 @@ struct LodiSchedule {
     std::vector<int64_t> phase_start;  // [nphase + 1]
 };
 LodiSchedule build_lodi_schedule(int nx, int ny, int nz);
+// The LODI x faces of a grid wrapped in y and z (faces: bit 0 the face i = 0,
+// bit 1 the face i = nx - 1; nx >= 4).
+LodiSchedule build_lodi_schedule_x(int nx, int ny, int nz, int faces);
 
 // Launch n plain steps of a case on its stream (no graphs, no checks); the
 // current array is c->A (c->SA) afterwards and the one read, c->B (c->SB).

@@ -25,15 +25,20 @@ struct LodiNode {
 };
 static_assert(sizeof(LodiNode) == 64, "one LODI record per 64 bytes");
 
-// xopen (periodic grids only): the x faces are not wrapped; their nodes
-// collide like interior nodes and links leaving the grid are dropped (the
-// fine level of a refined case, whose x faces the coarse level sets).
+// For grids wrapped in y and z (the kernels' PERIODIC form) each x face is
+// wrapped, open or LODI; bit 0 of xopen / xlodi is the face i = 0, bit 1 the
+// face i = nx - 1 (both flags 0: x wraps too).  An open face is not wrapped:
+// its nodes collide like interior nodes and links leaving the grid are
+// dropped (the faces of a refined case's fine level that the coarse level
+// sets).  A LODI face holds boundary nodes as in the reference: they do not
+// collide, push only their inward x links, and the LODI pass updates them.
 // L: layout of the population arrays (plane.hpp).
 struct Grid {
     int nx, ny, nz;
     int64_t N;
     int xopen = 0;
     PopLayout L;
+    int xlodi = 0;
 };
 
 hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64* k, const double* A, double* B,

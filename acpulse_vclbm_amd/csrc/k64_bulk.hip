@@ -23,7 +23,7 @@ template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
-                                                     const PopLayout L, int xopen, int row_lds) {
+                                                     const PopLayout L, int xopen, int row_lds, int xlodi) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -44,6 +44,8 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         bi = i == 0 ? 1 : (i == nx - 1 ? -1 : 0);
         bj = j == 0 ? 1 : (j == ny - 1 ? -1 : 0);
         bk = kz == 0 ? 1 : (kz == nz - 1 ? -1 : 0);
+    } else {
+        bi = (i == 0 && (xlodi & 1)) ? 1 : ((i == nx - 1 && (xlodi & 2)) ? -1 : 0);
     }
     const bool boundary = (bi | bj | bk) != 0;
 
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                     // open x face: the missing neighbour's macros by quadratic
                     // extrapolation, u(-1) = 3 u(0) - 3 u(1) + u(2), which makes the
                     // central difference the second-order one-sided one
-                    if (xopen && (i == 0 || i == nx - 1)) {
+                    if (((xopen & 1) && i == 0) || ((xopen & 2) && i == nx - 1)) {
                         const int m = i == 0 ? 2 : 0;
                         const I s1 = i == 0 ? (I)1 : (I)-1;
 #pragma unroll
@@ -110,8 +112,9 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     // Periodic rows held by one workgroup (row_lds, uniform per block, no thread
     // left early): the x-shifted links are realigned through LDS so that every
     // store of a wave covers whole cache lines; position i of the destination
-    // row takes the value of node i - e_x, and with open x faces a position
-    // whose source lies outside the grid is not written.
+    // row takes the value of node i - e_x, and a position whose source would
+    // push across an open x face is not written.  Rows with LODI x faces take
+    // the direct stores below (their face nodes push only inward links).
     if constexpr (PERIODIC) {
         if (row_lds) {
             __shared__ double row[kBulkBlock];
@@ -129,7 +132,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                     row[(i + EX[q] + nx) % nx] = f[q];
                     __syncthreads();
                     const int src = i - EX[q];
-                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), L) = row[i];
+                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1)))) P::at(B, q, (I)(base + (I)i), L) = row[i];
                 }
             });
             P::at(B, 18, l, L) = f[18];
@@ -142,11 +145,12 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if constexpr (!PERIODIC) {
             if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
         } else {
+            if (boundary && EX[q] != bi) return;  // LODI x face: inward x links only
             if constexpr (EX[q] > 0) {
-                if (xopen && i == nx - 1) return;
+                if ((xopen & 2) && i == nx - 1) return;
             }
             if constexpr (EX[q] < 0) {
-                if (xopen && i == 0) return;
+                if ((xopen & 1) && i == 0) return;
             }
         }
         I d = l;
@@ -166,13 +170,13 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
                      hipStream_t s) {
     const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
-    const int row_lds = PERIODIC && g.nx == bx;
+    const int row_lds = PERIODIC && g.nx == bx && !g.xlodi;
     if (narrow_planes(g.L, sizeof(double), sizeof(double)))
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, true>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.L, g.xopen, row_lds);
+                           g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi);
     else
         hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, false>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.L, g.xopen, row_lds);
+                           g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi);
     return hipGetLastError();
 }
 

@@ -177,7 +177,7 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
 template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
-    int ny, int nz, const PopLayout L, int xopen, int row_lds) {
+    int ny, int nz, const PopLayout L, int xopen, int row_lds, int xlodi) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -196,6 +196,8 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         bi = i == 0 ? 1 : (i == nx - 1 ? -1 : 0);
         bj = j == 0 ? 1 : (j == ny - 1 ? -1 : 0);
         bk = kz == 0 ? 1 : (kz == nz - 1 ? -1 : 0);
+    } else {
+        bi = (i == 0 && (xlodi & 1)) ? 1 : ((i == nx - 1 && (xlodi & 2)) ? -1 : 0);
     }
     const bool boundary = (bi | bj | bk) != 0;
     // neighbour offsets; unsigned wrap-around in the narrow form is exact because
@@ -212,7 +214,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     if constexpr (OP != OP_STREAM) {
         if (!boundary) {
             const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-            const int ghost = (PERIODIC && xopen) ? (i == nx - 1 ? 1 : (i == 0 ? -1 : 0)) : 0;
+            const int ghost = PERIODIC ? ((i == nx - 1 && (xopen & 2)) ? 1 : ((i == 0 && (xopen & 1)) ? -1 : 0)) : 0;
             collide_shift<OP, ORDER, NARROW>(k, g, M, l, o, L.N, ghost);
         }
     }
@@ -235,7 +237,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                     row[(i + EX[q] + nx) % nx] = (T)g[q];
                     __syncthreads();
                     const int src = i - EX[q];
-                    if (!xopen || (src >= 0 && src < nx)) P::at(B, q, (I)(base + (I)i), L) = row[i];
+                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1)))) P::at(B, q, (I)(base + (I)i), L) = row[i];
                 }
             });
             P::at(B, 18, l, L) = (T)g[18];
@@ -248,11 +250,12 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if constexpr (!PERIODIC) {
             if (boundary && !boundary_pushes(q, bi, bj, bk)) return;
         } else {
+            if (boundary && EX[q] != bi) return;  // LODI x face: inward x links only
             if constexpr (EX[q] > 0) {
-                if (xopen && i == nx - 1) return;
+                if ((xopen & 2) && i == nx - 1) return;
             }
             if constexpr (EX[q] < 0) {
-                if (xopen && i == 0) return;
+                if ((xopen & 1) && i == 0) return;
             }
         }
         I dd = l;
@@ -341,7 +344,7 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     // the macro array M is float32 like a float32 population plane
     const bool narrow = narrow_planes(g.L, sizeof(T), sizeof(float));
-    if (periodic && !g.xopen && g.nx % 128 == 0 && g.nx / 2 <= kBulkBlock) {
+    if (periodic && !g.xopen && !g.xlodi && g.nx % 128 == 0 && g.nx / 2 <= kBulkBlock) {
         const dim3 b2(g.nx / 2, 1, 1), g2(1, g.ny, g.nz);
         if (narrow)
             hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, true>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny,
@@ -351,10 +354,10 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
                                g.ny, g.nz, g.L);
         return hipGetLastError();
     }
-    const int row_lds = periodic && g.nx == bx;
+    const int row_lds = periodic && g.nx == bx && !g.xlodi;
 #define BS(PER, NAR)                                                                                             \
     hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
-                       g.ny, g.nz, g.L, g.xopen, row_lds)
+                       g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi)
     if (periodic) {
         if (narrow) BS(true, true);
         else BS(true, false);

@@ -19,13 +19,15 @@
 //                     coarse-to-fine at t+1
 //                     fine-to-coarse at t+1
 //
-// Geometry: the coarse level is a periodic nx x ny x nz grid; the fine level
-// covers coarse x in [x0, x1] (every y, z) at half the spacing, i.e.
-// 2 (x1 - x0) + 1 by 2 ny by 2 nz nodes, periodic in y and z, with open x
-// faces (ACP_BC_OPEN_X).  Coarse node (I, J, K) coincides with fine node
-// (2 (I - x0), 2 J, 2 K).
+// Geometry: the coarse level is an nx x ny x nz grid, periodic or with LODI x
+// faces (ACP_BC_LODI_X); the fine level covers coarse x in [x0, x1] (every
+// y, z) at half the spacing, i.e. 2 (x1 - x0) + 1 by 2 ny by 2 nz nodes,
+// periodic in y and z, with open x faces (ACP_BC_OPEN_X), or, when the region
+// reaches the high-x face of a LODI_X coarse level, an open face at x0 and a
+// LODI face at the other end (ACP_BC_OPEN_LODI_X).  Coarse node (I, J, K)
+// coincides with fine node (2 (I - x0), 2 J, 2 K).
 //
-// Coarse-to-fine on the fine x faces (all 19 links): coincident nodes take
+// Coarse-to-fine on the open fine x faces (all 19 links): coincident nodes take
 // the coarse populations of the interface column, linearly interpolated in
 // time (f = 0.5 (f(t) + f(t+1)) at t+1/2), with the non-equilibrium part
 // scaled:  f_fine = feq(rho, u) + a (f - feq(rho, u)),  a = tau_f / (2 tau_c);
@@ -34,7 +36,9 @@
 // Fine-to-coarse on every coarse node strictly inside the fine region (the
 // overlap cell x0 + 1, x1 - 1 and, so that the whole coarse field stays a
 // valid solution, the nodes beyond it):  f_coarse = feq + (1 / a) (f_fine - feq)
-// of the coincident fine node.
+// of the coincident fine node.  When the fine region reaches the coarse
+// level's high-x LODI face, the coarse node on that face keeps its own LODI
+// update (the fine level's LODI face lies under it).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -72,8 +76,9 @@ __device__ __forceinline__ void c2f_value(const K64& k, const double* __restrict
     rescale<ORDER>(k, f, a);
 }
 
-// Coarse-to-fine on both fine x faces, in two launches.  Face 0 is fine
-// column 0 under coarse column ci0, face 1 fine column fnx - 1 under ci1.
+// Coarse-to-fine on the coupled fine x faces (nfaces: 2 both, 1 the face at
+// x0 only), in two launches.  Face 0 is fine column 0 under coarse column
+// ci0, face 1 fine column fnx - 1 under ci1.
 // First the coincident nodes (even j and k, one thread per coarse (J, K) of
 // either face) take their coarse node's value; then the other face nodes
 // average their 2 (mid nodes) or 4 (centre nodes) coincident neighbours.
@@ -81,10 +86,11 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp, const double* __restrict__ Cprev,
                                                       const double* __restrict__ Ccur, int half, double a, int cnx,
                                                       int cny, int cnz, int ci0, int ci1, double* __restrict__ F,
-                                                      int fnx, int fny, const PopLayout CN, const PopLayout FN) {
+                                                      int fnx, int fny, const PopLayout CN, const PopLayout FN,
+                                                      int nfaces) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = cny * cnz;
-    if (t >= 2 * per) return;
+    if (t >= nfaces * per) return;
     const int face = t / per, r = t - face * per;
     const int J = r % cny, K = r / cny;
     const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0;
@@ -97,10 +103,10 @@ __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp
 }
 
 __global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int fnx, int fny, int fnz,
-                                                   const PopLayout FN) {
+                                                   const PopLayout FN, int nfaces) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = fny * fnz;
-    if (t >= 2 * per) return;
+    if (t >= nfaces * per) return;
     const int face = t / per, r = t - face * per;
     const int j = r % fny, k = r / fny;
     if (!(j & 1) && !(k & 1)) return;
@@ -319,10 +325,10 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
                                                         const T* __restrict__ Cprev, const T* __restrict__ Ccur,
                                                         int half, double a, int cnx, int cny, int cnz, int ci0,
                                                         int ci1, T* __restrict__ F, int fnx, int fny,
-                                                        const PopLayout CN, const PopLayout FN) {
+                                                        const PopLayout CN, const PopLayout FN, int nfaces) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = cny * cnz;
-    if (t >= 2 * per) return;
+    if (t >= nfaces * per) return;
     const int face = t / per, r = t - face * per;
     const int J = r % cny, K = r / cny;
     const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0;
@@ -333,10 +339,10 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
 
 template <typename T, bool F64>
 __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ ksp, T* __restrict__ F, int fnx,
-                                                     int fny, int fnz, const PopLayout FN) {
+                                                     int fny, int fnz, const PopLayout FN, int nfaces) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = fny * fnz;
-    if (t >= 2 * per) return;
+    if (t >= nfaces * per) return;
     const int face = t / per, r = t - face * per;
     const int j = r % fny, k = r / fny;
     if (!(j & 1) && !(k & 1)) return;
@@ -415,20 +421,25 @@ namespace {
 
 using acp::fail;
 
-// Coarse-to-fine on both fine faces: coincident nodes, then the others.
-// These are a few thousand latency-bound threads each: 64-thread workgroups
-// spread them over more CUs.
+// The fine faces the coarse level sets: both, or only the one at x0 when the
+// fine level's high-x face is a LODI face.
+int coupled_faces(const acp_refined* r) { return (r->fine->g.xlodi & 2) ? 1 : 2; }
+
+// Coarse-to-fine on the coupled fine faces: coincident nodes, then the
+// others.  These are a few thousand latency-bound threads each: 64-thread
+// workgroups spread them over more CUs.
 template <typename T, bool F64>
 hipError_t launch_c2f_s(acp_refined* r, int half) {
     acp_case* c = r->coarse;
     acp_case* f = r->fine;
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
     const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
-    const dim3 b(64), g1((unsigned)((2 * cny * cnz + 63) / 64)), g2((unsigned)((2 * fny * fnz + 63) / 64));
+    const int nf = coupled_faces(r);
+    const dim3 b(64), g1((unsigned)((nf * cny * cnz + 63) / 64)), g2((unsigned)((nf * fny * fnz + 63) / 64));
 #define C2FS(O)                                                                                                  \
     hipLaunchKernelGGL((acp::refine::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks,            \
                        (const T*)c->SB, (const T*)c->SA, half, r->a, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx,     \
-                       (T*)f->SA, fnx, fny, c->g.L, f->g.L)
+                       (T*)f->SA, fnx, fny, c->g.L, f->g.L, nf)
     if (c->d.order == 2) C2FS(2);
     else if (c->d.order == 3) C2FS(3);
     else C2FS(4);
@@ -436,7 +447,7 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), g2, b, 0, r->stream, f->dks, (T*)f->SA, fnx, fny, fnz,
-                       f->g.L);
+                       f->g.L, nf);
     return hipGetLastError();
 }
 
@@ -472,17 +483,18 @@ hipError_t launch_c2f(acp_refined* r, int half) {
         return r->f64_coupling ? launch_c2f_s<_Float16, true>(r, half) : launch_c2f_s<_Float16, false>(r, half);
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
     const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
-    const dim3 b(64), g1((unsigned)((2 * cny * cnz + 63) / 64)), g2((unsigned)((2 * fny * fnz + 63) / 64));
+    const int nf = coupled_faces(r);
+    const dim3 b(64), g1((unsigned)((nf * cny * cnz + 63) / 64)), g2((unsigned)((nf * fny * fnz + 63) / 64));
 #define C2F(O)                                                                                                    \
     hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, \
-                       cny, cnz, r->x0 % cnx, r->x1 % cnx, f->A, fnx, fny, c->g.L, f->g.L)
+                       cny, cnz, r->x0 % cnx, r->x1 % cnx, f->A, fnx, fny, c->g.L, f->g.L, nf)
     if (c->d.order == 2) C2F(2);
     else if (c->d.order == 3) C2F(3);
     else C2F(4);
 #undef C2F
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(acp::refine::c2f_between, g2, b, 0, r->stream, f->A, fnx, fny, fnz, f->g.L);
+    hipLaunchKernelGGL(acp::refine::c2f_between, g2, b, 0, r->stream, f->A, fnx, fny, fnz, f->g.L, nf);
     return hipGetLastError();
 }
 
@@ -519,10 +531,13 @@ namespace {
 
 // The coarse description's checks and the fine level's description.
 int fine_desc(const acp_case_desc& d, int32_t x0, int32_t x1, acp_case_desc& fd) {
-    if (d.boundary != ACP_BC_PERIODIC) return fail(ACP_E_UNSUPPORTED, "refined cases need a periodic coarse level");
+    if (d.boundary != ACP_BC_PERIODIC && d.boundary != ACP_BC_LODI_X)
+        return fail(ACP_E_UNSUPPORTED, "refined cases need a periodic or LODI_X coarse level");
     if (d.cubic) return fail(ACP_E_UNSUPPORTED, "refined cases: the cubic-Mach terms are not coupled");
-    if (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx || x1 - x0 >= d.nx - 1)
+    if (d.boundary == ACP_BC_PERIODIC && (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx || x1 - x0 >= d.nx - 1))
         return fail(ACP_E_ARG, "fine region [x0, x1] must satisfy 1 <= x0, x0 + 2 <= x1 <= nx, x1 - x0 < nx - 1");
+    if (d.boundary == ACP_BC_LODI_X && (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx - 1))
+        return fail(ACP_E_ARG, "fine region [x0, x1] of a LODI_X coarse level must satisfy 1 <= x0, x0 + 2 <= x1 <= nx - 1");
     fd = d;
     fd.nx = 2 * (x1 - x0) + 1;
     fd.ny = 2 * d.ny;
@@ -531,7 +546,7 @@ int fine_desc(const acp_case_desc& d, int32_t x0, int32_t x1, acp_case_desc& fd)
     fd.domain[0] = (fd.nx - 0.5) * fd.spacing;
     fd.domain[1] = (fd.ny - 0.5) * fd.spacing;
     fd.domain[2] = (fd.nz - 0.5) * fd.spacing;
-    fd.boundary = ACP_BC_OPEN_X;
+    fd.boundary = (d.boundary == ACP_BC_LODI_X && x1 == d.nx - 1) ? ACP_BC_OPEN_LODI_X : ACP_BC_OPEN_X;
     return ACP_OK;
 }
 
@@ -593,6 +608,29 @@ int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* st
         if (r->own_stream) (void)hipStreamDestroy(r->stream);
         delete r;
         return rc;
+    }
+    // A fine LODI face relaxes towards rho0 over the level-0 domain length,
+    // with its own spacing and time step (Node.cpp:237-238, 311: every level's
+    // LODIpressureOutlet takes getNumberOfNodesX() * getSpacing()); the shifted
+    // tiers hold that length in the fine level's lattice units.
+    if (r->fine->g.xlodi) {
+        acp_case* f = r->fine;
+        const int n[3] = {d.nx, d.ny, d.nz};
+        for (int a = 0; a < 3; ++a) {
+            f->hk.dom_len[a] = n[a] * d.spacing;
+            f->hks.inv_len[a] = 1.0f / (float)(2 * n[a]);
+        }
+        hipError_t e = hipMemcpyAsync(f->dk, &f->hk, sizeof(acp::K64), hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess && f->dks)
+            e = hipMemcpyAsync(f->dks, &f->hks, sizeof(acp::KShift), hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess) {
+            acp_case_destroy(r->coarse);
+            acp_case_destroy(r->fine);
+            if (r->own_stream) (void)hipStreamDestroy(r->stream);
+            delete r;
+            return acp::hip_fail(e, "fine-level constants");
+        }
     }
     // tau = 1 / omega of each level (Control.cpp:54 with each level's time step)
     r->a = (1. / r->fine->hk.omega) / (2. * (1. / r->coarse->hk.omega));

@@ -7,7 +7,9 @@
 //
 // With Non-Reflecting yes the reference prints boundary nodes with i > 0 from
 // the other population array (Lattice.cpp:4213-4217, 4257-4261, 4324-4330);
-// `fprev` is that array.
+// `fprev` is that array.  The boundary nodes are the LODI nodes of the
+// boundary kind: every face node with LODI on all faces, the LODI x faces'
+// nodes of a grid wrapped in y and z, none of a periodic grid.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -93,7 +95,13 @@ int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, con
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
                 const int64_t l = i + (int64_t)nx * (j + (int64_t)ny * kz);
-                const bool bnd = i == 0 || j == 0 || kz == 0 || i == nx - 1 || j == ny - 1 || kz == nz - 1;
+                bool bnd;
+                if (d.boundary == ACP_BC_LODI)
+                    bnd = i == 0 || j == 0 || kz == 0 || i == nx - 1 || j == ny - 1 || kz == nz - 1;
+                else if (d.boundary == ACP_BC_LODI_X)
+                    bnd = i == 0 || i == nx - 1;
+                else
+                    bnd = d.boundary == ACP_BC_OPEN_LODI_X && i == nx - 1;
                 const double* F = (d.nonreflecting && i > 0 && bnd) ? fprev : fcur;
                 double f[19], ps[19];
                 for (int q = 0; q < 19; ++q) f[q] = F[(size_t)q * N + l];

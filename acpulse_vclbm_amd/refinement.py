@@ -1,19 +1,31 @@
 """Interface reflection of the two-level refinement (acp_refined_*).
 
-Case (the refined acoustic-pulse configuration): coarse 256 x 128 x 3
-periodic grid, RR collision, N_eq = 3, tau = 0.5 + 3e-4, SI parameters of the
-reference's case file; fine level (ratio 2) over coarse x in [128, 256];
-Gaussian pulse rho0 (1 + A exp(-ln2 r^2 / 8^2)) at (64, 64), A = 1e-3;
-density probe at (96, 64, 1) on the coarse level.
+Case (the refined acoustic-pulse configuration): coarse 256 x 128 x 3 grid
+with LODI non-reflecting x faces (classical, K1 = 1) and periodic y and z, RR
+collision, N_eq = 3, tau = 0.5 + 3e-4, SI parameters of the reference's case
+file; fine level (ratio 2) over the right half, coarse x in [128, 255], which
+reaches the high-x face, so the fine level ends on a LODI face there and is
+coupled only at the interface x = 128; Gaussian pulse
+rho0 (1 + A exp(-ln2 r^2 / 8^2)) at (64, 64), A = 1e-3; density probe at
+(96, 64, 1) on the coarse level; 2000 coarse steps.
 
-The periodic domain has two interfaces.  The direct wave reaches the probe
-after ~55 steps (32 nodes at c_s = 1/sqrt(3)), the echo of the x = 128
-interface after ~166 steps (96 nodes), the echo of the other interface
-(x = 256 = 0) not before ~277 steps (160 nodes), so the window is 260 coarse
-steps.  Reference signals: the same pulse on the uniform coarse grid (same
-resolution as the probe's level: the difference is the interface's effect)
-and on a uniform grid at the fine resolution (512 x 256 x 6, probe at
-(192, 128, 2)).  R = max_t |p - p_ref| / max_t |p_ref - rho0|.
+Arrivals at the probe (c_s = 1/sqrt(3) nodes per step, half-width 8 nodes):
+the direct wave after ~55 steps (32 nodes), the interface's echo after ~166
+(96 nodes), the low-x face's echo after ~277 (160 nodes; the same face in the
+refined and the uniform run), the high-x face's echo not before ~580 (350
+nodes, less the pulse's width; the refined run meets a fine-level LODI face
+there, the uniform run a coarse one).  R is taken over the first 560 steps,
+where the refined and the uniform coarse run differ only by what the
+interface does; `R_all_steps` over the whole run adds the difference of the
+two high-x faces.  Reference signals: the same pulse on the uniform coarse
+grid (same resolution as the probe's level: the difference is the
+interface's effect) and on a uniform grid at the fine resolution
+(511 x 256 x 6 with LODI x faces, probe at (192, 128, 2)).
+R = max_t |p - p_ref| / max_t |p_ref - rho0|.
+
+`geometry="periodic"` selects the earlier fully periodic form of the case:
+fine level over [128, 256] on a periodic coarse grid, two interfaces, 260
+steps (the window before the second interface's echo).
 """
 import time
 
@@ -21,15 +33,25 @@ import numpy as np
 
 TAU = 0.5 + 3e-4
 
+# coarse boundary, fine region end, steps, R window, uniform fine grid (nx, boundary)
+GEOMETRY = {
+    "duct": dict(boundary="LODI-X", x1=255, steps=2000, window=560, fine_nx=511),
+    "periodic": dict(boundary="PERIODIC", x1=256, steps=260, window=260, fine_nx=512),
+}
 
-def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", coupling="F64", with_fine=True,
-                         f2c_filter="NOFI"):
+
+def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", coupling="F64", with_fine=True,
+                         f2c_filter="NOFI", geometry="duct"):
     """Reflection of the refined pulse case for one tier; `coupling` selects the
-    coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED") and
-    `f2c_filter` the fine-to-coarse filter ("NOFI" or "LAG")."""
-    kw = dict(op=op, order=3, boundary="PERIODIC", mrt_mode="RELAXED", precision=precision)
+    coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED"),
+    `f2c_filter` the fine-to-coarse filter ("NOFI" or "LAG") and `geometry`
+    the coarse level's x faces ("duct": LODI, "periodic")."""
+    g = GEOMETRY[geometry]
+    steps = g["steps"] if steps is None else steps
+    win = min(g["window"], steps)
+    kw = dict(op=op, order=3, boundary=g["boundary"], mrt_mode="RELAXED", precision=precision)
     d = acp.make_desc(256, 128, 3, tau=TAU, **kw)
-    with acp.Refined(d, 128, 256) as r:
+    with acp.Refined(d, 128, g["x1"]) as r:
         r.set_filter(f2c_filter)
         if precision != "F64":
             r.set_coupling_precision(coupling)
@@ -49,13 +71,15 @@ def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", cou
         u.set_probes([(96, 64, 1)], capacity=steps)
         u.step(steps)
         p_uc = u.read_probes()[:, 0].copy()
-    sig_c = np.max(np.abs(p_uc - rho0))
+    sig_c = np.max(np.abs(p_uc[:win] - rho0))
     node_updates = 256 * 128 * 3 + 2 * fine_nodes
     out = {
+        "geometry": geometry, "coarse_boundary": g["boundary"], "fine_region": [128, g["x1"]],
         "operator": op, "precision": precision, "coupling": coupling if precision != "F64" else "F64",
         "f2c_filter": f2c_filter,
-        "steps": steps, "amplitude": amp,
-        "R_vs_uniform_coarse": float(np.max(np.abs(p_ref - p_uc)) / sig_c),
+        "steps": steps, "window": win, "amplitude": amp,
+        "R_vs_uniform_coarse": float(np.max(np.abs(p_ref[:win] - p_uc[:win])) / sig_c),
+        "R_all_steps": float(np.max(np.abs(p_ref - p_uc)) / sig_c),
         "early_max_diff": float(np.max(np.abs(p_ref[:100] - p_uc[:100])) / sig_c),
         "ms_per_coarse_step": dt / steps * 1e3,
         "node_updates_per_coarse_step": node_updates,
@@ -64,15 +88,15 @@ def interface_reflection(acp, op="RR", steps=260, amp=1e-3, precision="F64", cou
     }
     if with_fine:
         # uniform grid at the fine resolution: same viscosity at half the spacing, two steps per coarse step
-        df = acp.make_desc(512, 256, 6, nu=d.nu, spacing=d.spacing / 2.0, **kw)
+        df = acp.make_desc(g["fine_nx"], 256, 6, nu=d.nu, spacing=d.spacing / 2.0, **kw)
         with acp.Case(df) as u:
             u.init_pulse(amp, 128.0, 128.0, 16.0)
-            u.set_probes([(192, 128, 2)], capacity=2 * steps)
-            u.step(2 * steps)
+            u.set_probes([(192, 128, 2)], capacity=2 * win)
+            u.step(2 * win)
             p_uf = u.read_probes()[1::2, 0].copy()
         sig_f = np.max(np.abs(p_uf - rho0))
-        out["R_vs_uniform_fine"] = float(np.max(np.abs(p_ref - p_uf)) / sig_f)
-        out["uniform_coarse_vs_fine"] = float(np.max(np.abs(p_uc - p_uf)) / sig_f)
+        out["R_vs_uniform_fine"] = float(np.max(np.abs(p_ref[:win] - p_uf)) / sig_f)
+        out["uniform_coarse_vs_fine"] = float(np.max(np.abs(p_uc[:win] - p_uf)) / sig_f)
     return out
 
 
@@ -107,11 +131,12 @@ def main():
     ap.add_argument("--op", default="RR")
     ap.add_argument("--precision", default="F64")
     ap.add_argument("--coupling", default="F64")
-    ap.add_argument("--steps", type=int, default=260)
+    ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--f2c-filter", default="NOFI", choices=["NOFI", "LAG"])
+    ap.add_argument("--geometry", default="duct", choices=sorted(GEOMETRY))
     a = ap.parse_args()
     r = interface_reflection(acp, op=a.op, steps=a.steps, precision=a.precision, coupling=a.coupling,
-                             f2c_filter=a.f2c_filter)
+                             f2c_filter=a.f2c_filter, geometry=a.geometry)
     r.pop("probe", None)
     print(json.dumps(r))
 

@@ -41,14 +41,25 @@ enum { ACP_OP_SRT = 0, ACP_OP_MRT_GS = 1, ACP_OP_MRT_RM = 2, ACP_OP_RR = 3, ACP_
  * tiers store f_i - w_i rho0 (scaled) and collide in float32, in lattice units. */
 enum { ACP_PREC_F64 = 0, ACP_PREC_F32_SHIFTED = 1, ACP_PREC_F16_SHIFTED = 2 };
 /* Domain boundary: the reference applies LODI on all six faces (Lattice.cpp:1582-1624).
- * PERIODIC wraps all three axes.  OPEN_X wraps y and z; the x faces are left
- * open: their nodes collide like interior nodes and links leaving the grid are
- * dropped, so something else must set them (the fine level of a refined case,
- * acp_refined_*).  HRR's strain stencil takes the missing neighbour beyond an
- * open face by quadratic extrapolation (u(-1) = 3 u(0) - 3 u(1) + u(2), the
- * second-order one-sided difference); not available with the cubic-Mach
+ * PERIODIC wraps all three axes.  The other kinds wrap y and z and give each x
+ * face its own rule:
+ *   LODI_X       LODI on both x faces, as the reference treats its x faces
+ *                (Lattice.cpp:1586-1593, Node.cpp:229-504; no edges or
+ *                corners, so every face node has its four tangent neighbours,
+ *                wrapped in y and z); at least 4 nodes in x.  A duct with
+ *                non-reflecting ends: the coarse level of the refinement
+ *                measurement case.
+ *   OPEN_X       both x faces open: their nodes collide like interior nodes and
+ *                links leaving the grid are dropped, so something else must set
+ *                them (the fine level of a refined case, acp_refined_*).
+ *   OPEN_LODI_X  face x = 0 open, face x = nx - 1 LODI: the fine level of a
+ *                refined case whose region reaches the high-x face of a
+ *                LODI_X coarse level (acp_refined_create makes it).
+ * HRR's strain stencil takes the missing neighbour beyond an open face by
+ * quadratic extrapolation (u(-1) = 3 u(0) - 3 u(1) + u(2), the second-order
+ * one-sided difference); open faces are not available with the cubic-Mach
  * correction. */
-enum { ACP_BC_LODI = 0, ACP_BC_PERIODIC = 1, ACP_BC_OPEN_X = 2 };
+enum { ACP_BC_LODI = 0, ACP_BC_PERIODIC = 1, ACP_BC_OPEN_X = 2, ACP_BC_LODI_X = 3, ACP_BC_OPEN_LODI_X = 4 };
 /* MRT operators: the reference computes the relaxed populations and discards them
  * (Node.cpp:2603-2604, 2833-2834), i.e. MRT steps are pure streaming + LODI. */
 enum { ACP_MRT_AS_REFERENCE = 0, ACP_MRT_RELAXED = 1 };
@@ -211,12 +222,22 @@ int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* r
  * BoundaryConditions.txt:52-106), whose arithmetic (GridCoupling.cpp) the
  * reference does not contain; restated from the published scheme.  Geometry
  * differs from the reference's fine shell (Lattice.cpp:38-126): the coarse
- * level is a periodic grid and the fine level covers coarse x in [x0, x1]
- * over every y and z at half the spacing and half the time step, with
- * ratio-2 vertex-centred coupling on its two x faces.  Every operator
- * without the cubic-Mach correction (HRR's strain stencil extrapolates at the
- * fine level's open faces, see ACP_BC_OPEN_X), every storage tier (both
- * levels use the coarse description's tier). */
+ * level is an ACP_BC_PERIODIC or ACP_BC_LODI_X grid and the fine level covers
+ * coarse x in [x0, x1] over every y and z at half the spacing and half the
+ * time step, with ratio-2 vertex-centred coupling on its x faces:
+ *   periodic coarse level: 1 <= x0, x0 + 2 <= x1 <= nx, x1 - x0 < nx - 1
+ *     (x1 = nx is the wrapped column 0); both fine faces are coupled;
+ *   LODI_X coarse level: 1 <= x0, x0 + 2 <= x1 <= nx - 1.  With x1 < nx - 1
+ *     both fine faces are coupled; with x1 = nx - 1 the fine region reaches
+ *     the high-x face, the fine level's face there is a LODI face
+ *     (ACP_BC_OPEN_LODI_X) and only the face at x0 is coupled.
+ * Every coarse node strictly between x0 and x1 is restricted from the fine
+ * level.  A fine LODI face uses the level-0 domain length and its own spacing
+ * and time step, as the reference's LODIpressureOutlet does on every level
+ * (Node.cpp:237-311).  Every operator without the cubic-Mach correction
+ * (HRR's strain stencil extrapolates at the fine level's open faces, see
+ * ACP_BC_OPEN_X), every storage tier (both levels use the coarse
+ * description's tier). */
 typedef struct acp_refined acp_refined;
 int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, acp_refined** out);
 void acp_refined_destroy(acp_refined* r);
@@ -290,8 +311,9 @@ typedef struct acp_request {
     double* probe_out;          /* host buffer for `steps` densities, or NULL; must stay valid */
     double arrival;             /* seconds on the server clock; not admitted earlier */
     uint64_t tag;               /* caller's identifier, returned in the result */
-    int32_t refine[2];          /* fine region [x0, x1] of a two-level case (acp_refined_*; periodic
-                                   coarse level, pulse initial condition); refine[1] <= refine[0]: one level */
+    int32_t refine[2];          /* fine region [x0, x1] of a two-level case (acp_refined_*; periodic or
+                                   LODI_X coarse level, pulse initial condition); refine[1] <= refine[0]:
+                                   one level */
     int32_t f2c_filter;         /* ACP_F2C_* of a two-level case */
 } acp_request;
 

@@ -604,14 +604,31 @@ static int is_boundary(const acpo_params* p, int i, int j, int k) {
     return i == 0 || j == 0 || k == 0 || i == p->nx - 1 || j == p->ny - 1 || k == p->nz - 1;
 }
 
+/* Modes 3 and 4: the LODI x faces (bit 0: i = 0, bit 1: i = nx - 1); their
+ * nodes are boundary nodes like the reference's x-face nodes. */
+static int lodi_x_faces(const acpo_params* p) { return p->periodic == 3 ? 3 : (p->periodic == 4 ? 2 : 0); }
+static int is_lodi_x(const acpo_params* p, int i) {
+    const int f = lodi_x_faces(p);
+    return ((f & 1) && i == 0) || ((f & 2) && i == p->nx - 1);
+}
+
 /* Push mask (Lattice.cpp:244-471 interior; 500-1392 boundary): interior nodes push all 18
  * links; a boundary node pushes exactly the links that point inward in every dimension in
  * which it lies on the boundary. */
 static uint32_t push_mask(const acpo_params* p, int i, int j, int k) {
-    if (p->periodic == 2) {  /* open x faces: links leaving the grid in x are dropped */
+    if (p->periodic == 2 || p->periodic == 4) {  /* open x faces: links leaving the grid in x are dropped */
+        const int hi_open = p->periodic == 2;
         uint32_t m = 0x7FFFFu;
         for (int q = 0; q < 18; ++q)
-            if ((i == p->nx - 1 && E[q][0] > 0) || (i == 0 && E[q][0] < 0)) m &= ~(1u << q);
+            if ((hi_open && i == p->nx - 1 && E[q][0] > 0) || (i == 0 && E[q][0] < 0)) m &= ~(1u << q);
+        if (!is_lodi_x(p, i)) return m;
+    }
+    if (lodi_x_faces(p)) {  /* LODI x face: the inward x links and the rest population */
+        if (!is_lodi_x(p, i)) return 0x7FFFFu;
+        const int b = i == 0 ? 1 : -1;
+        uint32_t m = 1u << 18;
+        for (int q = 0; q < 18; ++q)
+            if (E[q][0] == b) m |= 1u << q;
         return m;
     }
     if (p->periodic || !is_boundary(p, i, j, k)) return 0x7FFFFu;
@@ -630,6 +647,7 @@ static uint32_t push_mask(const acpo_params* p, int i, int j, int k) {
 acpo_case* acpo_create(const acpo_params* p) {
     if (p->nx < 3 || p->ny < 3 || p->nz < 3) return NULL;
     if (p->order < 2 || p->order > 4) return NULL;
+    if (p->periodic < 0 || p->periodic > 4 || (lodi_x_faces(p) && p->nx < 4)) return NULL;
     acpo_case* c = (acpo_case*)calloc(1, sizeof(acpo_case));
     c->p = *p;
     if (acpo_consts_init(&c->k, p) != 0) { free(c); return NULL; }
@@ -646,11 +664,12 @@ acpo_case* acpo_create(const acpo_params* p) {
             for (int ii = 0; ii < p->nx; ++ii) {
                 const int64_t l = lin(c, ii, jj, kk);
                 c->mask[l] = push_mask(p, ii, jj, kk);
-                c->interior[l] = (uint8_t)(p->periodic || !is_boundary(p, ii, jj, kk));
+                c->interior[l] = (uint8_t)(p->periodic ? !is_lodi_x(p, ii) : !is_boundary(p, ii, jj, kk));
             }
-    if (!p->periodic) {
+    if (!p->periodic || lodi_x_faces(p)) {
         /* boundary nodes in the order of Lattice::timeSteppingSRT's LODI loops
-         * (Lattice.cpp:1582-1624): x faces, then z faces, then y faces */
+         * (Lattice.cpp:1582-1624): x faces, then z faces, then y faces; modes 3
+         * and 4 have only (some of) the x faces, whose tangent neighbours wrap */
         const int im = p->nx - 1, jm = p->ny - 1, km = p->nz - 1;
         int64_t cap = 2 * ((int64_t)p->ny * p->nz + (int64_t)p->nx * p->ny + (int64_t)p->nx * p->nz);
         c->l_node = (int64_t*)calloc(cap, sizeof(int64_t));
@@ -673,13 +692,37 @@ acpo_case* acpo_create(const acpo_params* p) {
         c->l_corner[n] = g.corner;                                                                \
         ++n;                                                                                      \
     } while (0)
-        for (int j = 0; j <= jm; ++j)
-            for (int k = 0; k <= km; ++k) { ADD(0, j, k); ADD(im, j, k); }
-        for (int i = 1; i <= im - 1; ++i)
-            for (int j = 0; j <= jm; ++j) { ADD(i, j, 0); ADD(i, j, km); }
-        for (int i = 1; i <= im - 1; ++i)
-            for (int k = 1; k <= km - 1; ++k) { ADD(i, 0, k); ADD(i, jm, k); }
+#define ADDX(I, J, K)                                                                              \
+    do {                                                                                           \
+        const int s_ = (I) == 0 ? 1 : -1;                                                          \
+        c->l_node[n] = lin(c, (I), (J), (K));                                                      \
+        c->l_n1[n] = lin(c, (I) + s_, (J), (K));                                                   \
+        c->l_n2[n] = lin(c, (I) + 2 * s_, (J), (K));                                               \
+        c->l_t[0][n] = lin(c, (I), ((J) + 1) % p->ny, (K));                                        \
+        c->l_t[1][n] = lin(c, (I), ((J) + p->ny - 1) % p->ny, (K));                                \
+        c->l_t[2][n] = lin(c, (I), (J), ((K) + 1) % p->nz);                                        \
+        c->l_t[3][n] = lin(c, (I), (J), ((K) + p->nz - 1) % p->nz);                                \
+        c->l_dir[n] = (I) == 0 ? 0 : 2;                                                            \
+        c->l_corner[n] = 0;                                                                        \
+        ++n;                                                                                       \
+    } while (0)
+        if (lodi_x_faces(p)) {
+            const int f = lodi_x_faces(p);
+            for (int j = 0; j <= jm; ++j)
+                for (int k = 0; k <= km; ++k) {
+                    if (f & 1) ADDX(0, j, k);
+                    if (f & 2) ADDX(im, j, k);
+                }
+        } else {
+            for (int j = 0; j <= jm; ++j)
+                for (int k = 0; k <= km; ++k) { ADD(0, j, k); ADD(im, j, k); }
+            for (int i = 1; i <= im - 1; ++i)
+                for (int j = 0; j <= jm; ++j) { ADD(i, j, 0); ADD(i, j, km); }
+            for (int i = 1; i <= im - 1; ++i)
+                for (int k = 1; k <= km - 1; ++k) { ADD(i, 0, k); ADD(i, jm, k); }
+        }
 #undef ADD
+#undef ADDX
         c->nl = n;
         /* Phases: a node that reads the B array of a boundary node handled earlier must
          * run after it, one that reads a boundary node handled later must run before it. */
@@ -862,7 +905,7 @@ static void one_step(acpo_case* c) {
                     if (need_nb) {
                         for (int m = 0; m < 6; ++m) {
                             int a = ii + E[m][0], b = jj + E[m][1], d = kk + E[m][2];
-                            if (p->periodic == 2 && (a < 0 || a >= nx)) {
+                            if ((p->periodic == 2 && (a < 0 || a >= nx)) || (p->periodic == 4 && a < 0)) {
                                 /* open x face: the missing neighbour's macros by quadratic
                                  * extrapolation, u(-1) = 3 u(0) - 3 u(1) + u(2), which turns
                                  * the central difference into the second-order one-sided one */
@@ -896,7 +939,7 @@ static void one_step(acpo_case* c) {
     /* LODI passes (Lattice.cpp:1582-1624).  Nodes of one phase neither read nor
      * write each other's B entries, so each phase runs in parallel; phases run in
      * order, which reproduces the reference's sequential pass order. */
-    if (!p->periodic)
+    if (c->nl > 0)
         for (int32_t ph = 1; ph <= c->max_phase; ++ph) {
             const int64_t b = c->ph_start[ph - 1], e = c->ph_start[ph];
 #pragma omp parallel for schedule(static) num_threads(p->nthreads > 0 ? p->nthreads : 1)
@@ -1001,8 +1044,11 @@ void acpo_equilibrium_field(const acpo_params* p, int64_t N, const double* rho, 
  * stepping (Lattice.cpp:4354-5770) and is PARITY UNPINNED against the
  * reference; tests pin it by invariants (uniform states, scaling round trips)
  * and by the interface reflection against a uniform fine run.
- * Coarse: periodic nx*ny*nz; fine: coarse x in [x0, x1] at half spacing,
- * 2(x1-x0)+1 x 2ny x 2nz, periodic in y and z, open x faces.
+ * Coarse: nx*ny*nz, periodic (mode 1) or with LODI x faces (mode 3); fine:
+ * coarse x in [x0, x1] at half spacing, 2(x1-x0)+1 x 2ny x 2nz, periodic in
+ * y and z, open x faces (mode 2), or, when a mode-3 coarse level's region
+ * reaches x1 = nx - 1, an open face at x0 and a LODI face at the other end
+ * (mode 4), where only the face at x0 is coupled.
  * ------------------------------------------------------------------------ */
 struct acpo_refined {
     acpo_case* c;
@@ -1020,16 +1066,27 @@ static void rescale19(const acpo_consts* k, int order, double f[19], double a) {
 }
 
 acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
-    if (!coarse->periodic || x0 < 1 || x1 <= x0 + 1 || x1 > coarse->nx || x1 - x0 >= coarse->nx - 1) return NULL;
+    if (coarse->periodic == 1) {
+        if (x0 < 1 || x1 <= x0 + 1 || x1 > coarse->nx || x1 - x0 >= coarse->nx - 1) return NULL;
+    } else if (coarse->periodic == 3) {
+        if (x0 < 1 || x1 <= x0 + 1 || x1 > coarse->nx - 1) return NULL;
+    } else {
+        return NULL;
+    }
     acpo_params fp = *coarse;
     fp.nx = 2 * (x1 - x0) + 1;
     fp.ny = 2 * coarse->ny;
     fp.nz = 2 * coarse->nz;
     fp.spacing = coarse->spacing / 2.;
-    fp.periodic = 2;
+    fp.periodic = (coarse->periodic == 3 && x1 == coarse->nx - 1) ? 4 : 2;
     acpo_refined* r = (acpo_refined*)calloc(1, sizeof(acpo_refined));
     r->c = acpo_create(coarse);
     r->f = acpo_create(&fp);
+    /* a fine LODI face uses the level-0 domain length (Node.cpp:237, 311:
+     * getNumberOfNodesX() * getSpacing() on every level) */
+    r->f->k.dom_len[0] = coarse->nx * coarse->spacing;
+    r->f->k.dom_len[1] = coarse->ny * coarse->spacing;
+    r->f->k.dom_len[2] = coarse->nz * coarse->spacing;
     r->x0 = x0;
     r->x1 = x1;
     r->a = (1. / r->f->k.omega) / (2. * (1. / r->c->k.omega));
@@ -1053,7 +1110,8 @@ static void refined_c2f(acpo_refined* r, int half) {
     const int fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
     const int64_t CN = c->N, FN = f->N;
     const int faces[2][2] = {{r->x0 % cnx, 0}, {r->x1 % cnx, fnx - 1}};
-    for (int s = 0; s < 2; ++s) {
+    const int nfaces = f->p.periodic == 4 ? 1 : 2; /* a fine LODI face is not coupled */
+    for (int s = 0; s < nfaces; ++s) {
         const int ci = faces[s][0], fi = faces[s][1];
         for (int K = 0; K < cnz; ++K)
             for (int J = 0; J < cny; ++J) {

@@ -29,7 +29,10 @@ typedef struct acpo_params {
     int32_t cubic;      /* Cubic-Mach-Correction yes=1 */
     int32_t transverse; /* Transverse yes=1 */
     int32_t periodic;   /* 0: LODI on all six faces (the reference), 1: fully periodic (new),
-                           2: periodic y, z with open x faces (fine level of a refined case) */
+                           2: periodic y, z with open x faces (fine level of a refined case),
+                           3: periodic y, z with LODI on both x faces (nx >= 4),
+                           4: periodic y, z, open face x = 0, LODI face x = nx - 1 (fine level
+                              of a refined case reaching the high-x face of a mode-3 coarse level) */
     int32_t mrt_mode;   /* 0: as the reference (result discarded -> identity), 1: relaxation applied */
     int32_t nthreads;
     double rho0, nu, cs, spacing, mach, k1, sigma;
@@ -102,7 +105,8 @@ void acpo_equilibrium_field(const acpo_params* p, int64_t N, const double* rho, 
  * u = 0, populations feq_N (the initialization of Lattice.cpp:1457-1475 with a new profile). */
 void acpo_pulse_ic(const acpo_params* p, double amp, double x0, double y0, double b, double* f);
 
-/* Two-level refinement (ratio 2; coarse periodic, fine over coarse x in [x0, x1]).
+/* Two-level refinement (ratio 2; coarse periodic or with LODI x faces, fine over
+ * coarse x in [x0, x1]; NULL for a region the coarse mode does not allow).
  * Restates the published STD coupling; the reference's coupling source is absent,
  * so this part is PARITY UNPINNED against the reference (see acp_oracle.c). */
 typedef struct acpo_refined acpo_refined;

@@ -190,7 +190,8 @@ class _Level(OracleCase):
 
 
 class OracleRefined:
-    """Two-level oracle run (coarse periodic p, fine over coarse x in [x0, x1])."""
+    """Two-level oracle run (coarse p periodic or with LODI x faces, fine over
+    coarse x in [x0, x1])."""
 
     def __init__(self, p, x0, x1):
         self.h = lib().acpo_refined_create(ctypes.byref(p), x0, x1)
@@ -199,7 +200,7 @@ class OracleRefined:
         fp = Params.from_buffer_copy(p)
         fp.nx, fp.ny, fp.nz = 2 * (x1 - x0) + 1, 2 * p.ny, 2 * p.nz
         fp.spacing = p.spacing / 2.0
-        fp.periodic = 2
+        fp.periodic = 4 if (p.periodic == 3 and x1 == p.nx - 1) else 2
         self.levels = [_Level(lib().acpo_refined_level(self.h, 0), p), _Level(lib().acpo_refined_level(self.h, 1), fp)]
         self.x0, self.x1 = x0, x1
 

@@ -243,6 +243,72 @@ def test_block_layout_grids_match_oracle_bitexact(gpu, op, order, bnd, n, tr):
     assert np.max(np.abs(rho - r64)) <= 2e-4 * np.max(np.abs(r64 - p.rho0))
 
 
+LODI_X = [("SRT", 2, "LODI-X", False), ("HRR", 3, "LODI-X", False), ("RR", 4, "LODI-X", True),
+          ("MRT-RM", 3, "LODI-X", False), ("MRT-GS", 3, "LODI-X", False), ("HRR", 4, "LODI-X", True),
+          ("SRT", 3, "OPEN-LODI-X", False), ("HRR", 3, "OPEN-LODI-X", False)]
+
+
+@pytest.mark.parametrize("n", [(16, 12, 10), (40, 24, 12)], ids=["one-block", "two-blocks"])
+@pytest.mark.parametrize("op,order,bnd,cubic", LODI_X,
+                         ids=[f"{a}-o{b}-{c}" + ("-cubic" if d else "") for a, b, c, d in LODI_X])
+def test_lodi_x_faces_match_oracle_bitexact(gpu, op, order, bnd, cubic, n):
+    """LODI x faces on grids wrapped in y and z (the oracle's modes 3 and 4,
+    which the CPU suite pins to the reference's all-face LODI by causality):
+    9 steps with transverse terms, the state, probes on both x faces and the
+    mass bit for bit / to 1e-13 against the oracle."""
+    nx, ny, nz = n
+    mode = {"LODI-X": 3, "OPEN-LODI-X": 4}[bnd]
+    relaxed = op.startswith("MRT")
+    p = ol.make_params(nx, ny, nz, op=op, order=order, cubic=cubic, periodic=mode, mrt_mode=int(relaxed), tau=0.56,
+                       transverse=True)
+    f0 = _random_f(p, 31)
+    probes = [(0, 1, 2), (nx - 1, ny - 1, nz - 1), (nx // 2, 0, 0)]
+    print("case", op, order, bnd, cubic, n, flush=True)
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    po = o.step(9, probes)
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, cubic=cubic, boundary=bnd,
+                      mrt_mode="RELAXED" if relaxed else "REFERENCE", tau=0.56, transverse=True)
+    with acp.Case(d) as c:
+        c.set_populations(f0)
+        c.set_probes(probes, 9)
+        c.step(9)
+        f = c.populations()
+        assert np.array_equal(f, o.state()), f"max |df| = {np.abs(f - o.state()).max():.3e}"
+        assert np.array_equal(c.read_probes(), po)
+        m64 = c.mass()
+        info = c.info()
+    assert m64 == pytest.approx(o.state().sum(), rel=1e-13)
+    assert info.lodi_nodes == (2 if mode == 3 else 1) * ny * nz and info.lodi_launches == 1
+
+
+@pytest.mark.parametrize("prec,tol", [("F32-SHIFTED", 2e-4), ("F16-SHIFTED", 2e-2)])
+@pytest.mark.parametrize("bnd,nx", [("LODI-X", 128), ("LODI-X", 50), ("OPEN-LODI-X", 50)])
+@pytest.mark.parametrize("op", ["SRT", "HRR"])
+def test_lodi_x_faces_shifted_tiers(gpu, prec, tol, bnd, nx, op):
+    """The shifted tiers with LODI x faces against the float64 oracle: a pulse
+    (amplitude 1e-3) next to the high-x face, 40 steps, density within the
+    tolerances of the other shifted-tier tests (of the largest density
+    deviation).  nx = 128 is a row length the two-nodes-per-thread kernel
+    takes on fully periodic grids; with LODI faces the one-node kernel runs."""
+    ny, nz = 20, 3
+    mode = {"LODI-X": 3, "OPEN-LODI-X": 4}[bnd]
+    p = ol.make_params(nx, ny, nz, op=op, order=3, periodic=mode, tau=0.56, sigma=0.98, transverse=True)
+    f0 = ol.pulse_ic(p, 1e-3, nx - 9.0, 9.0, 5.0)
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(40)
+    rho64, _ = ol.macros(p, o.state())
+    d = acp.make_desc(nx, ny, nz, op=op, order=3, boundary=bnd, tau=0.56, sigma=0.98, transverse=True,
+                      precision=prec, f16_scale=1000.0)
+    with acp.Case(d) as c:
+        c.set_populations(f0)
+        c.step(40)
+        rho, _ = c.macros()
+    dev = rho64 - p.rho0
+    assert np.max(np.abs(rho - rho64)) <= tol * np.max(np.abs(dev))
+
+
 ALL_ORDERS = [("SRT", 2), ("SRT", 3), ("SRT", 4), ("RR", 2), ("RR", 3), ("RR", 4), ("HRR", 3), ("HRR", 4),
               ("MRT-RM", 3), ("MRT-GS", 3)]
 

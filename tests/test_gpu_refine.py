@@ -54,6 +54,61 @@ def test_refined_lag_matches_oracle_bitexact(gpu, op, order, mrt):
                 assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
 
 
+@pytest.mark.parametrize("x1", [17, 23], ids=["inside", "to-lodi-face"])
+@pytest.mark.parametrize("op,order,mrt,filt", [("SRT", 3, 0, "NOFI"), ("RR", 3, 0, "LAG"), ("HRR", 3, 0, "NOFI"),
+                                               ("MRT-RM", 3, 1, "NOFI"), ("RR", 4, 0, "NOFI")])
+def test_refined_lodi_x_matches_oracle_bitexact(gpu, op, order, mrt, filt, x1):
+    """A coarse level with LODI x faces (y, z periodic), fine region inside it
+    (both fine faces coupled) or reaching its high-x face (the fine level ends
+    on a LODI face relaxing over the level-0 domain length, one coupled
+    face): both levels bit for bit against the oracle, pulse crossing the
+    interface and reaching the faces."""
+    nx, ny, nz, x0 = 24, 10, 3, 8
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=3, tau=0.6, mrt_mode=mrt, cs=CS, spacing=DX,
+                       rho0=RHO0, transverse=True)
+    o = ol.OracleRefined(p, x0, x1)
+    o.set_filter(filt)
+    amp, cx, cy, b = 1e-3, 12.0, 5.0, 2.5
+    o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, tau=0.6, boundary="LODI-X", transverse=True,
+                      mrt_mode="RELAXED" if mrt else "REFERENCE")
+    print("case", op, order, filt, x1, flush=True)
+    with acp.Refined(d, x0, x1) as r:
+        r.set_filter(filt)
+        r.init_pulse(amp, cx, cy, b)
+        for lv in (0, 1):
+            assert np.array_equal(r.level(lv).populations(), o.levels[lv].state())
+        for n in (1, 4, 11):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+
+
+@pytest.mark.parametrize("prec,tol", [("F32-SHIFTED", 1e-4), ("F16-SHIFTED", 2e-2)])
+def test_refined_lodi_x_shifted_tiers_follow_float64(gpu, prec, tol):
+    """The refined case ending on a fine LODI face in the shifted tiers (float64
+    coupling) against the float64 oracle: coarse density within `tol` of the
+    largest density deviation after 30 coarse steps."""
+    nx, ny, nz, x0, x1 = 24, 10, 3, 8, 23
+    p = ol.make_params(nx, ny, nz, op="HRR", order=3, periodic=3, tau=0.6, cs=CS, spacing=DX, rho0=RHO0,
+                       transverse=True)
+    o = ol.OracleRefined(p, x0, x1)
+    amp, cx, cy, b = 1e-3, 14.0, 5.0, 2.5
+    o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    o.step(30)
+    r64 = o.levels[0].state().sum(axis=0)
+    d = acp.make_desc(nx, ny, nz, op="HRR", order=3, tau=0.6, boundary="LODI-X", transverse=True, precision=prec,
+                      f16_scale=1000.0)
+    with acp.Refined(d, x0, x1) as r:
+        r.init_pulse(amp, cx, cy, b)
+        r.step(30)
+        rho, _ = r.level(0).macros()
+    assert np.max(np.abs(rho - r64)) <= tol * np.max(np.abs(r64 - RHO0))
+
+
 def test_refined_restriction_alone_matches_oracle(gpu):
     """The restriction on its own (acp_refined_restrict) over a random fine
     state, both filters, against the oracle bit for bit; and the LAG - NOFI
@@ -133,6 +188,10 @@ def test_refined_rejects_unsupported():
     d = acp.make_desc(24, 10, 3, op="SRT", tau=0.6)
     with pytest.raises(acp.AcpError):
         acp.Refined(d, 8, 16)  # LODI coarse level
+    d = acp.make_desc(24, 10, 3, op="SRT", tau=0.6, boundary="LODI-X")
+    for x0, x1 in [(0, 16), (8, 24), (8, 9)]:
+        with pytest.raises(acp.AcpError):
+            acp.Refined(d, x0, x1)  # regions on or beyond the LODI faces
     d = acp.make_desc(24, 10, 3, op="SRT", tau=0.6, boundary="PERIODIC")
     with pytest.raises(acp.AcpError):
         acp.Refined(d, 0, 16)
