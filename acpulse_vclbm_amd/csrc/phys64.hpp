@@ -33,6 +33,34 @@ struct K64 {
     double gs_pre[19][19];           // inverse Gram-Schmidt transform (Node.cpp:2878-2897)
 };
 
+// Copy of the case constants in LDS, for kernels of a few thousand threads
+// (boundary and coupling passes).  Such a kernel is latency-bound, and its
+// equilibria read the Hermite tables through a chain of dependent
+// scalar-cache loads; read from LDS they cost one load latency for the whole
+// table.  The first wave copies; every thread of the workgroup must call this
+// (it holds a barrier) before any of them returns.
+__device__ __forceinline__ const K64& stage_k64(const K64* __restrict__ kp, K64& sk) {
+    constexpr int n = (int)(sizeof(K64) / sizeof(double));
+    static_assert(sizeof(K64) % sizeof(double) == 0, "K64 holds doubles only");
+    const double* src = reinterpret_cast<const double*>(kp);
+    double* dst = reinterpret_cast<double*>(&sk);
+    if (threadIdx.x < 64) {
+        double v[(n + 63) / 64];
+#pragma unroll
+        for (int c = 0; c < (n + 63) / 64; ++c) {
+            const int i = c * 64 + (int)threadIdx.x;
+            v[c] = i < n ? src[i] : 0.;
+        }
+#pragma unroll
+        for (int c = 0; c < (n + 63) / 64; ++c) {
+            const int i = c * 64 + (int)threadIdx.x;
+            if (i < n) dst[i] = v[c];
+        }
+    }
+    __syncthreads();
+    return sk;
+}
+
 // ---------------------------------------------------------------- moments (a3)
 // Node.cpp:11-30 and 85-118
 __host__ __device__ __forceinline__ void moments64(const K64& k, const double f[19], double& rho, double& ux,

@@ -66,13 +66,23 @@ __device__ __forceinline__ void rescale(const K64& k, double f[19], double a) {
 
 // The populations a coincident fine face node takes from coarse node lc:
 // time-interpolated (half: at t + 1/2) and rescaled.
+// Both time levels are loaded unconditionally, all 38 loads ahead of the
+// arithmetic, so that one memory latency covers them.
 template <int ORDER>
 __device__ __forceinline__ void c2f_value(const K64& k, const double* __restrict__ Cprev,
                                           const double* __restrict__ Ccur, const PopLayout& CN, int64_t lc, int half,
                                           double a, double f[19]) {
+    const int64_t o = (lc >> CN.shift) * CN.block + (lc & CN.mask);
+    double prev[19];
 #pragma unroll
-    for (int q = 0; q < 19; ++q)
-        f[q] = half ? 0.5 * (Cprev[pidx(CN, q, lc)] + Ccur[pidx(CN, q, lc)]) : Ccur[pidx(CN, q, lc)];
+    for (int q = 0; q < 19; ++q) {
+        f[q] = Ccur[o + q * CN.plane];
+        prev[q] = Cprev[o + q * CN.plane];
+    }
+    if (half) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) f[q] = 0.5 * (prev[q] + f[q]);
+    }
     rescale<ORDER>(k, f, a);
 }
 
@@ -88,6 +98,8 @@ __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp
                                                       int cny, int cnz, int ci0, int ci1, double* __restrict__ F,
                                                       int fnx, int fny, const PopLayout CN, const PopLayout FN,
                                                       int nfaces) {
+    __shared__ K64 sk;
+    const K64& k = stage_k64(kp, sk);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = cny * cnz;
     if (t >= nfaces * per) return;
@@ -97,13 +109,17 @@ __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp
     const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
     const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
     double f[19];
-    c2f_value<ORDER>(*kp, Cprev, Ccur, CN, lc, half, a, f);
+    c2f_value<ORDER>(k, Cprev, Ccur, CN, lc, half, a, f);
 #pragma unroll
     for (int q = 0; q < 19; ++q) F[pidx(FN, q, lf)] = f[q];
 }
 
-__global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int fnx, int fny, int fnz,
-                                                   const PopLayout FN, int nfaces) {
+// c2f_between: Fc and F are the same array; the kernel reads only coincident nodes (even j
+// and k) through Fc and writes only the others through F, so no element is
+// both read and written and both pointers may be restrict-qualified, which
+// lets every load of a thread be issued before its first store.
+__global__ __launch_bounds__(256) void c2f_between(const double* __restrict__ Fc, double* __restrict__ F, int fnx,
+                                                   int fny, int fnz, const PopLayout FN, int nfaces) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = fny * fnz;
     if (t >= nfaces * per) return;
@@ -122,12 +138,12 @@ __global__ __launch_bounds__(256) void c2f_between(double* __restrict__ F, int f
 #pragma unroll
         for (int q = 0; q < 19; ++q)
             F[pidx(FN, q, l)] =
-                ((F[pidx(FN, q, a)] + F[pidx(FN, q, b)]) + (F[pidx(FN, q, c)] + F[pidx(FN, q, d)])) * 0.25;
+                ((Fc[pidx(FN, q, a)] + Fc[pidx(FN, q, b)]) + (Fc[pidx(FN, q, c)] + Fc[pidx(FN, q, d)])) * 0.25;
     } else {
         const int64_t a = (j & 1) ? at(j - 1, k) : at(j, k - 1);
         const int64_t b = (j & 1) ? at(j + 1, k) : at(j, k + 1);
 #pragma unroll
-        for (int q = 0; q < 19; ++q) F[pidx(FN, q, l)] = (F[pidx(FN, q, a)] + F[pidx(FN, q, b)]) * 0.5;
+        for (int q = 0; q < 19; ++q) F[pidx(FN, q, l)] = (Fc[pidx(FN, q, a)] + Fc[pidx(FN, q, b)]) * 0.5;
     }
 }
 
@@ -171,6 +187,8 @@ template <int ORDER, bool LAG>
 __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                            int fny, int fnz, double ainv, double* __restrict__ C, int cnx, int cny,
                                            int cnz, int x0, int width, const PopLayout CN, const PopLayout FN) {
+    __shared__ K64 sk;
+    const K64& k = stage_k64(kp, sk);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)width * cny * cnz) return;
     const int di = (int)(t % width);
@@ -180,12 +198,12 @@ __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const dou
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
     double f[19];
     if constexpr (LAG) {
-        lag_mean<ORDER>(*kp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, f, FN);
+        lag_mean<ORDER>(k, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, f, FN);
     } else {
         const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
 #pragma unroll
         for (int q = 0; q < 19; ++q) f[q] = F[pidx(FN, q, lf)];
-        rescale<ORDER>(*kp, f, ainv);
+        rescale<ORDER>(k, f, ainv);
     }
 #pragma unroll
     for (int q = 0; q < 19; ++q) C[pidx(CN, q, lc)] = f[q];
@@ -208,15 +226,33 @@ __device__ __forceinline__ T f_to_g(const KShift& ks, double f, int q) {
 // The stored values (tier T) a node receives from node lc of another level:
 // time-interpolated (half) and rescaled by a, in float64 on SI populations
 // (F64) or in float32 on lattice-unit deviations.
+// The 19 stored values of node l (any storage type), loaded before any of
+// them is used.
+template <typename T>
+__device__ __forceinline__ void load_raw19(const T* __restrict__ A, const PopLayout& L, int64_t l, T out[19]) {
+    const int64_t o = (l >> L.shift) * L.block + (l & L.mask);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) out[q] = A[o + q * L.plane];
+}
+
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void couple_value(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
                                              const PopLayout& CN, int64_t lc, int half, double a, T out[19]) {
+    // every load ahead of the arithmetic (one memory latency for all of them)
+    T cur[19], prev[19];
+    load_raw19(Ccur, CN, lc, cur);
+    if (half) {
+        load_raw19(Cprev, CN, lc, prev);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) prev[q] = cur[q];
+    }
     if constexpr (F64) {
         double f[19];
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const double fc = g_to_f(ks, Ccur[pidx(CN, q, lc)], q);
-            f[q] = half ? 0.5 * (g_to_f(ks, Cprev[pidx(CN, q, lc)], q) + fc) : fc;
+            const double fc = g_to_f(ks, cur[q], q);
+            f[q] = half ? 0.5 * (g_to_f(ks, prev[q], q) + fc) : fc;
         });
         rescale<ORDER>(k, f, a);
         static_for<0, 19>([&](auto qc) {
@@ -227,8 +263,8 @@ __device__ __forceinline__ void couple_value(const K64& k, const KShift& ks, con
         float d[19], eq[19], dr, ux, uy, uz;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            const float dc = (float)Ccur[pidx(CN, q, lc)] * ks.inv_s;
-            d[q] = half ? 0.5f * ((float)Cprev[pidx(CN, q, lc)] * ks.inv_s + dc) : dc;
+            const float dc = (float)cur[q] * ks.inv_s;
+            d[q] = half ? 0.5f * ((float)prev[q] * ks.inv_s + dc) : dc;
         });
         moments_dev(d, dr, ux, uy, uz);
         feq_dev<ORDER>(dr, ux, uy, uz, eq, ks.k6);
@@ -257,11 +293,13 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
                                            int fnz, int fx, int fy, int fz, double ainv, T* __restrict__ C,
                                            const PopLayout& CN, int64_t lc, const PopLayout& FN) {
     const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
+    T raw[19];
+    load_raw19(F, FN, l0, raw);
     if constexpr (F64) {
         double v[19], eq[19], acc[19], rho, ux, uy, uz;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            v[q] = g_to_f(ks, F[pidx(FN, q, l0)], q);
+            v[q] = g_to_f(ks, raw[q], q);
         });
         moments64(k, v, rho, ux, uy, uz);
         feq64<ORDER>(k, rho, ux, uy, uz, eq);
@@ -272,9 +310,11 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
             const int y = (fy + EY[j] + fny) % fny, z = (fz + EZ[j] + fnz) % fnz;
             const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
             double g[19], e[19], r, x1, x2, x3;
+            T rn[19];
+            load_raw19(F, FN, l, rn);
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                g[q] = g_to_f(ks, F[pidx(FN, q, l)], q);
+                g[q] = g_to_f(ks, rn[q], q);
             });
             moments64(k, g, r, x1, x2, x3);
             feq64<ORDER>(k, r, x1, x2, x3, e);
@@ -289,7 +329,7 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
         float d[19], eq[19], acc[19], dr, ux, uy, uz;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            d[q] = (float)F[pidx(FN, q, l0)] * ks.inv_s;
+            d[q] = (float)raw[q] * ks.inv_s;
         });
         moments_dev(d, dr, ux, uy, uz);
         feq_dev<ORDER>(dr, ux, uy, uz, eq, ks.k6);
@@ -300,9 +340,11 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
             const int y = (fy + EY[j] + fny) % fny, z = (fz + EZ[j] + fnz) % fnz;
             const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
             float g[19], e[19], r, x1, x2, x3;
+            T rn[19];
+            load_raw19(F, FN, l, rn);
             static_for<0, 19>([&](auto qc) {
                 constexpr int q = decltype(qc)::value;
-                g[q] = (float)F[pidx(FN, q, l)] * ks.inv_s;
+                g[q] = (float)rn[q] * ks.inv_s;
             });
             moments_dev(g, r, x1, x2, x3);
             feq_dev<ORDER>(r, x1, x2, x3, e, ks.k6);
@@ -326,6 +368,8 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
                                                         int half, double a, int cnx, int cny, int cnz, int ci0,
                                                         int ci1, T* __restrict__ F, int fnx, int fny,
                                                         const PopLayout CN, const PopLayout FN, int nfaces) {
+    __shared__ K64 sk;
+    const K64& k = F64 ? stage_k64(kp, sk) : *kp;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = cny * cnz;
     if (t >= nfaces * per) return;
@@ -334,12 +378,13 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
     const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0;
     const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
     const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
-    couple_node<T, ORDER, F64>(*kp, *ksp, Cprev, Ccur, CN, lc, half, a, F, FN, lf);
+    couple_node<T, ORDER, F64>(k, *ksp, Cprev, Ccur, CN, lc, half, a, F, FN, lf);
 }
 
 template <typename T, bool F64>
-__global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ ksp, T* __restrict__ F, int fnx,
-                                                     int fny, int fnz, const PopLayout FN, int nfaces) {
+__global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ ksp, const T* __restrict__ Fc,
+                                                     T* __restrict__ F, int fnx, int fny, int fnz, const PopLayout FN,
+                                                     int nfaces) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = fny * fnz;
     if (t >= nfaces * per) return;
@@ -364,17 +409,23 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
         nb[1] = (j & 1) ? at(j + 1, k) : at(j, k + 1);
         n = 2;
     }
+    T r0[19], r1[19], r2[19], r3[19];
+    load_raw19(Fc, FN, nb[0], r0);
+    load_raw19(Fc, FN, nb[1], r1);
+    if (n == 4) {
+        load_raw19(Fc, FN, nb[2], r2);
+        load_raw19(Fc, FN, nb[3], r3);
+    }
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (F64) {
-            double v = n == 4 ? ((g_to_f(ks, F[pidx(FN, q, nb[0])], q) + g_to_f(ks, F[pidx(FN, q, nb[1])], q)) +
-                                 (g_to_f(ks, F[pidx(FN, q, nb[2])], q) + g_to_f(ks, F[pidx(FN, q, nb[3])], q))) * 0.25
-                              : (g_to_f(ks, F[pidx(FN, q, nb[0])], q) + g_to_f(ks, F[pidx(FN, q, nb[1])], q)) * 0.5;
+            double v = n == 4 ? ((g_to_f(ks, r0[q], q) + g_to_f(ks, r1[q], q)) +
+                                 (g_to_f(ks, r2[q], q) + g_to_f(ks, r3[q], q))) * 0.25
+                              : (g_to_f(ks, r0[q], q) + g_to_f(ks, r1[q], q)) * 0.5;
             F[pidx(FN, q, l)] = f_to_g<T>(ks, v, q);
         } else {
-            float v = n == 4 ? (((float)F[pidx(FN, q, nb[0])] + (float)F[pidx(FN, q, nb[1])]) +
-                                ((float)F[pidx(FN, q, nb[2])] + (float)F[pidx(FN, q, nb[3])])) * 0.25f
-                             : ((float)F[pidx(FN, q, nb[0])] + (float)F[pidx(FN, q, nb[1])]) * 0.5f;
+            float v = n == 4 ? (((float)r0[q] + (float)r1[q]) + ((float)r2[q] + (float)r3[q])) * 0.25f
+                             : ((float)r0[q] + (float)r1[q]) * 0.5f;
             F[pidx(FN, q, l)] = (T)v;
         }
     });
@@ -385,6 +436,8 @@ __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const K
                                              const T* __restrict__ F, int fnx, int fny, int fnz, double ainv,
                                              T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width,
                                              const PopLayout CN, const PopLayout FN) {
+    __shared__ K64 sk;
+    const K64& k = F64 ? stage_k64(kp, sk) : *kp;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)width * cny * cnz) return;
     const int di = (int)(t % width);
@@ -393,10 +446,10 @@ __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const K
     const int I = (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
     if constexpr (LAG) {
-        lag_node_s<T, ORDER, F64>(*kp, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN);
+        lag_node_s<T, ORDER, F64>(k, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN);
     } else {
         const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
-        couple_node<T, ORDER, F64>(*kp, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
+        couple_node<T, ORDER, F64>(k, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
     }
 }
 
@@ -446,8 +499,8 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
 #undef C2FS
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), g2, b, 0, r->stream, f->dks, (T*)f->SA, fnx, fny, fnz,
-                       f->g.L, nf);
+    hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), g2, b, 0, r->stream, f->dks, (const T*)f->SA,
+                       (T*)f->SA, fnx, fny, fnz, f->g.L, nf);
     return hipGetLastError();
 }
 
@@ -494,7 +547,8 @@ hipError_t launch_c2f(acp_refined* r, int half) {
 #undef C2F
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(acp::refine::c2f_between, g2, b, 0, r->stream, f->A, fnx, fny, fnz, f->g.L, nf);
+    hipLaunchKernelGGL(acp::refine::c2f_between, g2, b, 0, r->stream, (const double*)f->A, f->A, fnx, fny, fnz,
+                       f->g.L, nf);
     return hipGetLastError();
 }
 

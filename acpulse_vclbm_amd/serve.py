@@ -17,12 +17,13 @@ Grids are nx x ny x 3, tau = 0.5 + 3e-4, N_eq = 3, the probe at
 (3/4 nx, ny/2, 1).
 
   * One-level requests: LODI on all faces, the pulse at the centre.
-  * Refined requests (acp_refined_*): a periodic coarse grid with a fine level
-    (ratio 2) over x in [nx/2, nx] (every coarse node of that half has fine
-    nodes), the pulse at (nx/4, ny/2) in the coarse half, the probe read on the
-    coarse level inside the fine region; standard coupling with the F2C filter
-    NOFI.  The direct coupling is not available: requests that draw it run
-    with the standard coupling and are counted as such.
+  * Refined requests (acp_refined_*): the refined acoustic-pulse geometry, a
+    coarse grid with LODI x faces, periodic in y and z, and a fine level
+    (ratio 2) over x in [nx/2, nx - 1], which ends on a LODI face at the
+    high-x end; the pulse at (nx/4, ny/2) in the coarse half, the probe read
+    on the coarse level inside the fine region; standard coupling with the
+    F2C filter NOFI.  The direct coupling is not available: requests that draw
+    it run with the standard coupling and are counted as such.
 """
 import ctypes
 import time
@@ -68,13 +69,13 @@ def two_level(r):
 
 
 def fine_region(r):
-    return (r["nx"] // 2, r["nx"])
+    return (r["nx"] // 2, r["nx"] - 1)
 
 
 def request_desc(r):
     from . import make_desc
     return make_desc(r["nx"], r["ny"], r["nz"], op=r["op"], order=3,
-                     boundary="PERIODIC" if two_level(r) else "LODI", mrt_mode="RELAXED",
+                     boundary="LODI-X" if two_level(r) else "LODI", mrt_mode="RELAXED",
                      precision=r["precision"], tau=TAU, sigma=0.98)
 
 

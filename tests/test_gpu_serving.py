@@ -58,24 +58,30 @@ def test_graph_replays_match_plain_steps(gpu, spec):
 
 
 @pytest.mark.parametrize("spec", [dict(op="RR", precision="F64"), dict(op="SRT", precision="F32-SHIFTED"),
-                                  dict(op="MRT-RM", precision="F64", mrt_mode="RELAXED", lag=True)],
-                         ids=["rr-f64", "srt-f32", "mrtrm-f64-lag"])
+                                  dict(op="MRT-RM", precision="F64", mrt_mode="RELAXED", lag=True),
+                                  dict(op="HRR", precision="F64", duct=True),
+                                  dict(op="RR", precision="F32-SHIFTED", duct=True)],
+                         ids=["rr-f64", "srt-f32", "mrtrm-f64-lag", "hrr-f64-duct", "rr-f32-duct"])
 def test_refined_graph_replays_match_plain_steps(gpu, spec):
     """Refined cases replayed from captured graphs of 4 coarse steps (both
     levels' parities, odd remainders, synchronisation between replays) against
     plain refined steps: both levels bit-identical, probe series equal; and the
-    refined footprint is what the two levels allocate."""
+    refined footprint is what the two levels allocate.  Periodic coarse grids
+    with the fine level over [16, 32], and ducts (LODI x faces) with the fine
+    level over [16, 31], ending on a LODI face."""
     spec = dict(spec)
     lag = spec.pop("lag", False)
-    d = acp.make_desc(32, 12, 3, tau=0.56, boundary="PERIODIC", **spec)
-    with acp.Refined(d, 16, 32) as a, acp.Refined(d, 16, 32) as b:
+    duct = spec.pop("duct", False)
+    d = acp.make_desc(32, 12, 3, tau=0.56, boundary="LODI-X" if duct else "PERIODIC", **spec)
+    x1 = 31 if duct else 32
+    with acp.Refined(d, 16, x1) as a, acp.Refined(d, 16, x1) as b:
         for r in (a, b):
             if lag:
                 r.set_filter("LAG")
             r.init_pulse(1e-3, 8.0, 6.0, 3.0)
             r.level(0).set_probes([(24, 6, 1)], capacity=100)
         assert (a.level(0).info().device_bytes + a.level(1).info().device_bytes ==
-                acp.refined_footprint(d, 16, 32, 1, 100))
+                acp.refined_footprint(d, 16, x1, 1, 100))
         b.enable_graphs(4)
         plan = [4, 1, 4, 8, 3, 12]
         for n in plan:

@@ -106,12 +106,17 @@ def test_refined_footprint_is_both_levels():
     with pytest.raises(_capi.AcpError) as e:
         acp.refined_footprint(acp.make_desc(64, 40, 3, op="RR", tau=0.56), 32, 64)
     assert e.value.code == _capi.ACP_E_UNSUPPORTED
+    # LODI x faces: a region reaching the high-x face gives a fine level whose
+    # far face is a LODI face (one record per node of that face)
+    dl = acp.make_desc(64, 40, 3, op="RR", tau=0.56, boundary="LODI-X")
+    fl = acp.make_desc(63, 80, 6, op="RR", nu=d.nu, spacing=d.spacing / 2, boundary="OPEN-LODI-X")
+    assert acp.refined_footprint(dl, 32, 63, 1, 500) == _fp(fl) + _fp(dl, 1, 500)
 
 
 def test_refined_requests_in_the_mix():
     """Half of the requests are flagged for refinement; they become two-level
-    cases on a periodic coarse grid, with the fine level over the upper half in
-    x and the pulse in the lower half."""
+    cases on a coarse grid with LODI x faces, with the fine level over the
+    upper half in x up to the high-x face and the pulse in the lower half."""
     reqs = serve.make_requests(256, seed=1)
     c = serve.mix_counts(reqs)
     assert c["refined_flag"] == c["two_level"]
@@ -119,9 +124,9 @@ def test_refined_requests_in_the_mix():
     for r in reqs:
         d = serve.request_desc(r)
         if serve.two_level(r):
-            assert d.boundary == _capi.BOUNDARY["PERIODIC"]
+            assert d.boundary == _capi.BOUNDARY["LODI-X"]
             x0, x1 = serve.fine_region(r)
-            assert (x0, x1) == (r["nx"] // 2, r["nx"])
+            assert (x0, x1) == (r["nx"] // 2, r["nx"] - 1)
             assert serve.pulse_centre(r)[0] < x0 <= serve.probe_point(r)[0]
             assert serve.footprint(r) == acp.refined_footprint(d, x0, x1, 1, r["steps"])
         else:
