@@ -25,18 +25,14 @@ __device__ __forceinline__ void load19_soa(const double* __restrict__ F, int64_t
 // boundary node: own state from A (time t), normal neighbours from B (time t+1,
 // plus their cubic-Mach terms), tangent neighbours from A; writes feq_N of the
 // characteristic update into the node's B entries.
-// STAGE: the case constants are read from a copy in LDS (stage_k64), which
-// pays in the small launches of 64-thread workgroups (x faces, edges,
-// corners, the faces of quasi-2D grids) and costs a little in the large ones.
-template <int ORDER, bool CUBIC, bool STAGE>
+template <int ORDER, bool CUBIC>
 __global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const double* __restrict__ A,
                                               double* __restrict__ B, const double* __restrict__ psi,
                                               const LodiNode* __restrict__ nodes, int64_t count, const PopLayout PL,
                                               int transverse) {
-    __shared__ K64 sk;
-    const K64& k = STAGE ? stage_k64(kp, sk) : *kp;
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= count) return;
+    const K64& k = *kp;
     const LodiNode ln = nodes[s];
     double f[19], ps[19];
     double rb, vb[3], r1, v1[3], r2, v2[3];
@@ -137,13 +133,7 @@ hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, c
     const int bs = lodi_block(count);
     dim3 grid((unsigned)((count + bs - 1) / bs));
     const int tr = transverse ? 1 : 0;
-#define L64(O, C)                                                                                         \
-    do {                                                                                                  \
-        if (bs == 64)                                                                                     \
-            hipLaunchKernelGGL((lodi64<O, C, true>), grid, dim3(bs), 0, s, k, A, B, psi, nodes, count, L, tr);  \
-        else                                                                                              \
-            hipLaunchKernelGGL((lodi64<O, C, false>), grid, dim3(bs), 0, s, k, A, B, psi, nodes, count, L, tr); \
-    } while (0)
+#define L64(O, C) hipLaunchKernelGGL((lodi64<O, C>), grid, dim3(bs), 0, s, k, A, B, psi, nodes, count, L, tr)
     if (cubic) {
         if (order == 2) L64(2, true);
         else if (order == 3) L64(3, true);

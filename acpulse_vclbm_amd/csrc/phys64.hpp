@@ -33,8 +33,9 @@ struct K64 {
     double gs_pre[19][19];           // inverse Gram-Schmidt transform (Node.cpp:2878-2897)
 };
 
-// Copy of the case constants in LDS, for kernels of a few thousand threads
-// (boundary and coupling passes).  Such a kernel is latency-bound, and its
+// Copy of the case constants in LDS, for kernels of a few hundred to a few
+// tens of thousands of threads (the coupling passes of a refined case).  Such
+// a kernel is latency-bound, and its
 // equilibria read the Hermite tables through a chain of dependent
 // scalar-cache loads; read from LDS they cost one load latency for the whole
 // table.  The first wave copies; every thread of the workgroup must call this
