@@ -121,6 +121,12 @@ class Case:
         check(lib().acp_case_get_populations(self.h, dptr(f)))
         return f
 
+    def previous_populations(self):
+        """The other array of the ping-pong pair (the one the last step read)."""
+        f = np.empty((19, self.N))
+        check(lib().acp_case_get_previous_populations(self.h, dptr(f)))
+        return f
+
     def macros(self):
         rho = np.empty(self.N)
         u = np.empty((3, self.N))
@@ -425,9 +431,9 @@ class Lattice:
         self._alt = not self._alt
 
     def getDistribution(self, alternating, dir, level, i, j, k):
+        if dir >= 19:
+            return 0.0
         if level != 0:
             raise AcpError(_capi.ACP_E_UNSUPPORTED, "single level only")
-        if bool(alternating) != self._alt:
-            raise AcpError(_capi.ACP_E_STATE, "only the current array is kept on the device")
-        f = self.case.populations()
+        f = self.case.populations() if bool(alternating) == self._alt else self.case.previous_populations()
         return float(f[dir, self.case.node(i, j, k)])

@@ -105,6 +105,10 @@ def lib():
     sig = {
         "acp_last_error": (ctypes.c_char_p, []),
         "acp_control_read": (ctypes.c_int, [ctypes.c_char_p, D]),
+        "acp_control_read_value": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, dp]),
+        "acp_control_read_string": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64]),
+        "acp_control_read_filename": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, i64]),
+        "acp_control_read_vector": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, dp]),
         "acp_case_desc_default": (None, [D]),
         "acp_case_create": (ctypes.c_int, [D, vp, ctypes.POINTER(vp)]),
         "acp_case_destroy": (None, [vp]),
@@ -117,6 +121,7 @@ def lib():
         "acp_case_step": (ctypes.c_int, [vp, i32]),
         "acp_case_sync": (ctypes.c_int, [vp]),
         "acp_case_get_populations": (ctypes.c_int, [vp, dp]),
+        "acp_case_get_previous_populations": (ctypes.c_int, [vp, dp]),
         "acp_case_get_macros": (ctypes.c_int, [vp, dp, dp]),
         "acp_case_mass": (ctypes.c_int, [vp, dp]),
         "acp_case_set_probes": (ctypes.c_int, [vp, ctypes.POINTER(i64), i32, i32]),

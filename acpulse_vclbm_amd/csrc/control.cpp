@@ -212,3 +212,55 @@ extern "C" int acp_control_read(const char* path, acp_case_desc* out) {
     acp::clear_error();
     return ACP_OK;
 }
+
+// Control::readValue / readString / readFileName / readVector
+// (Control.cpp:135-273) for any keyword: the first line whose first token is
+// the keyword; a missing file or keyword is ACP_E_CONFIG (the reference prints
+// the same message and exits).
+extern "C" int acp_control_read_value(const char* path, const char* keyword, double* out) {
+    if (!path || !keyword || !out) return acp::fail(ACP_E_ARG, "acp_control_read_value: null argument");
+    Reader r{path, {}};
+    if (!r.value(keyword, *out)) return acp::fail(ACP_E_CONFIG, r.err);
+    acp::clear_error();
+    return ACP_OK;
+}
+
+extern "C" int acp_control_read_string(const char* path, const char* keyword, char* out, int64_t cap) {
+    if (!path || !keyword || !out || cap <= 0) return acp::fail(ACP_E_ARG, "acp_control_read_string: bad argument");
+    Reader r{path, {}};
+    std::string s;
+    if (!r.str(keyword, s)) return acp::fail(ACP_E_CONFIG, r.err);
+    if ((int64_t)s.size() >= cap) return acp::fail(ACP_E_ARG, "acp_control_read_string: output buffer too small");
+    copy_str(out, (size_t)cap, s);
+    acp::clear_error();
+    return ACP_OK;
+}
+
+extern "C" int acp_control_read_filename(const char* path, const char* keyword, char* out, int64_t cap) {
+    if (!path || !keyword || !out || cap <= 0) return acp::fail(ACP_E_ARG, "acp_control_read_filename: bad argument");
+    Reader r{path, {}};
+    std::string s;
+    if (!r.rest_of_line(keyword, s)) return acp::fail(ACP_E_CONFIG, r.err);
+    if ((int64_t)s.size() >= cap) return acp::fail(ACP_E_ARG, "acp_control_read_filename: output buffer too small");
+    copy_str(out, (size_t)cap, s);
+    acp::clear_error();
+    return ACP_OK;
+}
+
+extern "C" int acp_control_read_vector(const char* path, const char* keyword, double* out3) {
+    if (!path || !keyword || !out3) return acp::fail(ACP_E_ARG, "acp_control_read_vector: null argument");
+    Reader r{path, {}};
+    std::string line;
+    std::istringstream rest;
+    if (!r.line_for(keyword, line, rest)) {
+        if (r.err.empty()) r.err = std::string("No Value for keyword \"") + keyword + "\" found in " + path;
+        return acp::fail(ACP_E_CONFIG, r.err);
+    }
+    double x = 0., y = 0., z = 0.;
+    rest >> x >> y >> z;
+    out3[0] = x;
+    out3[1] = y;
+    out3[2] = z;
+    acp::clear_error();
+    return ACP_OK;
+}

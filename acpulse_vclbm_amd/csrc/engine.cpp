@@ -894,6 +894,12 @@ extern "C" int acp_case_get_populations(acp_case* c, double* f) {
     return download_array(c, true, f);
 }
 
+extern "C" int acp_case_get_previous_populations(acp_case* c, double* f) {
+    if (int rc = check_case(c)) return rc;
+    if (!f) return fail(ACP_E_ARG, "null population array");
+    return download_array(c, false, f);
+}
+
 extern "C" int acp_case_get_macros(acp_case* c, double* rho, double* u) {
     if (int rc = check_case(c)) return rc;
     if (!rho || !u) return fail(ACP_E_ARG, "null output array");

@@ -116,6 +116,16 @@ const char* acp_last_error(void);
  * Also reads the optional keys this engine adds (Precision, Boundary, MRT-Mode,
  * FP16-Scale); unknown keys are ignored as in the reference. */
 int acp_control_read(const char* path, acp_case_desc* out);
+/* Control::readValue / readString / readFileName / readVector
+ * (Control.cpp:135-273) for any keyword of a case file: the first line whose
+ * first token is `keyword`; the number, the next token, the rest of the line,
+ * or three numbers.  A missing file or keyword is ACP_E_CONFIG with the
+ * reference's message (the reference exits); a string longer than cap - 1
+ * bytes is ACP_E_ARG. */
+int acp_control_read_value(const char* path, const char* keyword, double* out);
+int acp_control_read_string(const char* path, const char* keyword, char* out, int64_t cap);
+int acp_control_read_filename(const char* path, const char* keyword, char* out, int64_t cap);
+int acp_control_read_vector(const char* path, const char* keyword, double* out3);
 
 /* Defaults of the reference's BoundaryConditions.txt for every field. */
 void acp_case_desc_default(acp_case_desc* out);
@@ -175,6 +185,13 @@ int acp_case_checksum(acp_case* c, uint64_t* out);
 /* Lattice::getDistribution(bool&, ...) (Lattice.h:119) for the whole current
  * array: host float64 [19][N]. */
 int acp_case_get_populations(acp_case* c, double* f);
+/* The same for the other array of the ping-pong pair (the one the last step
+ * read; after initialisation or acp_case_set_populations both hold the same
+ * state): Lattice::getDistribution with the flag of the previous step.  Its
+ * interior nodes hold their state before the last step's collision; the
+ * reference collides in place (Node.cpp:511-557), so its array holds their
+ * post-collision values.  Boundary nodes, which do not collide, agree. */
+int acp_case_get_previous_populations(acp_case* c, double* f);
 /* Density and velocity of every node from the current array (Node.cpp:11-30,
  * 85-118, without cubic-Mach terms): rho [N], u [3][N], float64. */
 int acp_case_get_macros(acp_case* c, double* rho, double* u);

@@ -516,3 +516,34 @@ def test_acp_run_driver_end_to_end(gpu, tmp_path):
     acp._capi.check(acp.lib().acp_write_vtk_host(d, acp._capi.dptr(cur), acp._capi.dptr(prev), None,
                                                  str(tmp_path / "oracle").encode()))
     assert open(tmp_path / "run_6.vtk", "rb").read() == open(tmp_path / "oracle.vtk", "rb").read()
+
+
+def test_lattice_interface_both_arrays(gpu, tmp_path):
+    """The Control / Lattice interface (the reference's main() loop, Lattice.h:102-120):
+    initialization, solve with the caller's ping-pong flag, and getDistribution from
+    either array, against the oracle's current and previous arrays bit for bit; a
+    flag naming the previous array is refused by solve."""
+    from make_golden import bc_text
+    nx, ny, nz, steps = 12, 10, 5, 5
+    (tmp_path / "bc.txt").write_text(bc_text(nx, ny, nz, op="RR", order=3, tau=0.56, spacing=0.02, steps=steps,
+                                             transverse=True))
+    bc = acp.Control(str(tmp_path / "bc.txt"))
+    lat = acp.Lattice(bc)
+    lat.initialization(bc)
+    alt = True
+    for _ in range(steps):
+        lat.solve(alt)
+        alt = not alt
+    with pytest.raises(acp.AcpError):
+        lat.solve(not alt)
+    p = ol.make_params(nx, ny, nz, op="RR", order=3, tau=0.56, spacing=0.02, transverse=True)
+    o = ol.OracleCase(p)
+    o.set_state(ol.reference_ic(p, (nx - 0.5) * 0.02, (ny - 0.5) * 0.02))
+    o.step(steps)
+    cur, prev = o.state(), o.prev()
+    for (q, i, j, k) in [(0, 0, 0, 0), (5, 3, 4, 2), (18, 11, 9, 4), (13, 6, 0, 3)]:
+        l = i + nx * (j + ny * k)
+        assert lat.getDistribution(alt, q, 0, i, j, k) == cur[q, l]
+        assert lat.getDistribution(not alt, q, 0, i, j, k) == prev[q, l]
+    assert lat.getDistribution(alt, 19, 0, 0, 0, 0) == 0.0
+    assert np.array_equal(lat.case.previous_populations(), prev)
