@@ -3,7 +3,8 @@
 // One thread per node, one workgroup per grid row (x is the fastest index), so
 // that every cache line of the destination arrays is written by a single
 // workgroup: the x-shifted stores of a row stay inside that row.  A (time t) is
-// read once, B (time t+1) written once: 2*19*8 = 304 bytes per node and step.
+// read once, B (time t+1) written once: 2*19*8 = 304 bytes per node and step,
+// with the non-temporal hint when the arrays exceed the cache (NT, plane.hpp).
 // Populations are addressed through the case's block layout (plane.hpp).
 //
 // Semantics follow Lattice::timeStepping* (Lattice.cpp:1524-1580): interior
@@ -19,7 +20,7 @@
 
 namespace acp {
 
-template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW>
+template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
@@ -36,7 +37,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     double f[19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        f[q] = P::at(A, q, l, L);
+        f[q] = ld_pop<NT>(&P::at(A, q, l, L));
     });
 
     int bi = 0, bj = 0, bk = 0;
@@ -69,7 +70,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             if constexpr (CUBIC) {
                 static_for<0, 19>([&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    ps[q] = P::soa(psi, q, l, L.N);
+                    ps[q] = ld_pop<NT>(&P::soa(psi, q, l, L.N));
                 });
             }
             double nb[6][4];
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                 if (psi != nullptr) {
                     static_for<0, 19>([&](auto qc) {
                         constexpr int q = decltype(qc)::value;
-                        P::soa(psi, q, l, L.N) = ps[q];
+                        st_pop<NT>(&P::soa(psi, q, l, L.N), ps[q]);
                     });
                 }
             }
@@ -126,16 +127,17 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                 if constexpr (EZ[q] > 0) base += ozp;
                 if constexpr (EZ[q] < 0) base += ozm;
                 if constexpr (EX[q] == 0) {
-                    P::at(B, q, (I)(base + (I)i), L) = f[q];
+                    st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), f[q]);
                 } else {
                     __syncthreads();
                     row[(i + EX[q] + nx) % nx] = f[q];
                     __syncthreads();
                     const int src = i - EX[q];
-                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1)))) P::at(B, q, (I)(base + (I)i), L) = row[i];
+                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1))))
+                        st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), row[i]);
                 }
             });
-            P::at(B, 18, l, L) = f[18];
+            st_pop<NT>(&P::at(B, 18, l, L), f[18]);
             return;
         }
     }
@@ -160,9 +162,9 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if constexpr (EY[q] < 0) d += oym;
         if constexpr (EZ[q] > 0) d += ozp;
         if constexpr (EZ[q] < 0) d += ozm;
-        P::at(B, q, d, L) = f[q];
+        st_pop<NT>(&P::at(B, q, d, L), f[q]);
     });
-    P::at(B, 18, l, L) = f[18];
+    st_pop<NT>(&P::at(B, 18, l, L), f[18]);
 }
 
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC>
@@ -171,12 +173,13 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
     const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     const int row_lds = PERIODIC && g.nx == bx && !g.xlodi;
-    if (narrow_planes(g.L, sizeof(double), sizeof(double)))
-        hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, true>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi);
-    else
-        hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, false>), grid, block, 0, s, k, A, B, psi, M, g.nx,
-                           g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi);
+#define B64(NAR, NT)                                                                                          \
+    hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, NAR, NT>), grid, block, 0, s, k, A, B, psi, M, g.nx, \
+                       g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi)
+    if (!narrow_planes(g.L, sizeof(double), sizeof(double))) B64(false, true);
+    else if (stream_populations(g.L, sizeof(double))) B64(true, true);
+    else B64(true, false);
+#undef B64
     return hipGetLastError();
 }
 

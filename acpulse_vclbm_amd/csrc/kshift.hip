@@ -174,7 +174,7 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
 }
 
 // ------------------------------------------------------------------ bulk collide + push
-template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW>
+template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW, bool NT>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
     int ny, int nz, const PopLayout L, int xopen, int row_lds, int xlodi) {
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     float g[19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        g[q] = (float)P::at(A, q, l, L);
+        g[q] = (float)ld_pop<NT>(&P::at(A, q, l, L));
     });
     int bi = 0, bj = 0, bk = 0;
     if constexpr (!PERIODIC) {
@@ -231,16 +231,17 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                 if constexpr (EZ[q] > 0) base += ozp;
                 if constexpr (EZ[q] < 0) base += ozm;
                 if constexpr (EX[q] == 0) {
-                    P::at(B, q, (I)(base + (I)i), L) = (T)g[q];
+                    st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), (T)g[q]);
                 } else {
                     __syncthreads();
                     row[(i + EX[q] + nx) % nx] = (T)g[q];
                     __syncthreads();
                     const int src = i - EX[q];
-                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1)))) P::at(B, q, (I)(base + (I)i), L) = row[i];
+                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1))))
+                        st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), row[i]);
                 }
             });
-            P::at(B, 18, l, L) = (T)g[18];
+            st_pop<NT>(&P::at(B, 18, l, L), (T)g[18]);
             return;
         }
     }
@@ -265,9 +266,9 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if constexpr (EY[q] < 0) dd += oym;
         if constexpr (EZ[q] > 0) dd += ozp;
         if constexpr (EZ[q] < 0) dd += ozm;
-        P::at(B, q, dd, L) = (T)g[q];
+        st_pop<NT>(&P::at(B, q, dd, L), (T)g[q]);
     });
-    P::at(B, 18, l, L) = (T)g[18];
+    st_pop<NT>(&P::at(B, 18, l, L), (T)g[18]);
 }
 
 // Periodic rows of nx = 2 * blockDim.x nodes (nx a multiple of 128, no open
@@ -280,7 +281,29 @@ struct alignas(2 * sizeof(T)) Pair2 {
     T x, y;
 };
 
-template <typename T, int OP, int ORDER, bool NARROW>
+// A pair as a two-element vector, so that it moves with a single access (with
+// the non-temporal hint when NT).
+template <typename T>
+using PairVec = T __attribute__((ext_vector_type(2)));
+
+template <bool NT, typename T>
+__device__ __forceinline__ Pair2<T> ld_pair(const T* p) {
+    const PairVec<T> u = ld_pop<NT>(reinterpret_cast<const PairVec<T>*>(p));
+    Pair2<T> v;
+    v.x = u.x;
+    v.y = u.y;
+    return v;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void st_pair(T* p, const Pair2<T>& v) {
+    PairVec<T> u;
+    u.x = v.x;
+    u.y = v.y;
+    st_pop<NT>(reinterpret_cast<PairVec<T>*>(p), u);
+}
+
+template <typename T, int OP, int ORDER, bool NARROW, bool NT>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift2(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
     int ny, int nz, const PopLayout L) {
@@ -294,7 +317,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     float g[2][19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        const Pair2<T> v = *reinterpret_cast<const Pair2<T>*>(&P::at(A, q, (I)(rs + (I)(2 * t)), L));
+        const Pair2<T> v = ld_pair<NT>(&P::at(A, q, (I)(rs + (I)(2 * t)), L));
         g[0][q] = (float)v.x;
         g[1][q] = (float)v.y;
     });
@@ -333,7 +356,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             v.x = row[2 * t];
             v.y = row[2 * t + 1];
         }
-        *reinterpret_cast<Pair2<T>*>(&P::at(B, q, (I)(base + (I)(2 * t)), L)) = v;
+        st_pair<NT>(&P::at(B, q, (I)(base + (I)(2 * t)), L), v);
     });
 }
 
@@ -344,26 +367,30 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     // the macro array M is float32 like a float32 population plane
     const bool narrow = narrow_planes(g.L, sizeof(T), sizeof(float));
+    const bool nt = stream_populations(g.L, sizeof(T));
     if (periodic && !g.xopen && !g.xlodi && g.nx % 128 == 0 && g.nx / 2 <= kBulkBlock) {
         const dim3 b2(g.nx / 2, 1, 1), g2(1, g.ny, g.nz);
-        if (narrow)
-            hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, true>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny,
-                               g.nz, g.L);
-        else
-            hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, false>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx,
-                               g.ny, g.nz, g.L);
+#define BS2(NAR, NT)                                                                                          \
+    hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, NAR, NT>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny, \
+                       g.nz, g.L)
+        if (!narrow) BS2(false, true);
+        else if (nt) BS2(true, true);
+        else BS2(true, false);
+#undef BS2
         return hipGetLastError();
     }
     const int row_lds = periodic && g.nx == bx && !g.xlodi;
-#define BS(PER, NAR)                                                                                             \
-    hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
+#define BS(PER, NAR, NT)                                                                                             \
+    hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR, NT>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
                        g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi)
     if (periodic) {
-        if (narrow) BS(true, true);
-        else BS(true, false);
+        if (!narrow) BS(true, false, true);
+        else if (nt) BS(true, true, true);
+        else BS(true, true, false);
     } else {
-        if (narrow) BS(false, true);
-        else BS(false, false);
+        if (!narrow) BS(false, false, true);
+        else if (nt) BS(false, true, true);
+        else BS(false, true, false);
     }
 #undef BS
     return hipGetLastError();

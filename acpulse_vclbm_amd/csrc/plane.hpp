@@ -108,6 +108,32 @@ struct Plane<false> {
     }
 };
 
+// Streaming accesses.  A step reads each population of A once and writes each
+// population of B once.  When the two arrays are larger than the 256 MB
+// last-level cache (Infinity Cache) they cannot stay in it from one kernel to
+// the next, and the bulk kernels issue these accesses with the non-temporal
+// hint (NT = true): on a copy of one float64 step of a 512^3 grid in this
+// layout (19 planes in blocks of 8192 nodes, one node per thread, 512-thread
+// workgroups) it reaches 6.18 TB/s against 5.72 TB/s without (6.01 TB/s with
+// the hint on stores only).  Smaller cases keep plain accesses: there the hint
+// evicts what the LODI pass and the next step would find in the cache (a
+// 151 x 151 x 21 HRR case, 145 MB of populations, took 12 % longer per step
+// with it).  Arrays whose offsets need 64-bit indices are always far larger.
+constexpr int64_t kCacheBytes = (int64_t)256 << 20;
+inline bool stream_populations(const PopLayout& L, int elem_bytes) {
+    return 2 * L.elems * (int64_t)elem_bytes > kCacheBytes;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_pop(const T* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_pop(T* p, T v) {
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
 // true when the narrow form addresses population arrays of `pop_bytes` per
 // element and macro planes of `plane_bytes` per element: the per-node part of
 // every offset (the node's block and position, or its index in a whole plane)

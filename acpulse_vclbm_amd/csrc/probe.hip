@@ -8,29 +8,33 @@
 
 namespace acp {
 
+// The copies use the bulk kernels' access rule: the non-temporal hint when the
+// two arrays exceed the last-level cache (NT, plane.hpp).
+
 // one element of each of P planes per thread, loads first, then stores
-template <int P>
+template <int P, bool NT>
 __global__ __launch_bounds__(512) void copy_planes(const double* __restrict__ a, double* __restrict__ b, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * 512 + threadIdx.x;
     if (i >= n) return;
     double v[P];
 #pragma unroll
-    for (int s = 0; s < P; ++s) v[s] = a[s * n + i];
+    for (int s = 0; s < P; ++s) v[s] = ld_pop<NT>(a + s * n + i);
 #pragma unroll
-    for (int s = 0; s < P; ++s) b[s * n + i] = v[s];
+    for (int s = 0; s < P; ++s) st_pop<NT>(b + s * n + i, v[s]);
 }
 
 // 19 populations of one node per thread in the block layout (plane.hpp):
 // loads first, then stores
+template <bool NT>
 __global__ __launch_bounds__(512) void copy_blocks(const double* __restrict__ a, double* __restrict__ b,
                                                    const PopLayout L) {
     const int64_t i = (int64_t)blockIdx.x * 512 + threadIdx.x;
     if (i >= L.N) return;
     double v[19];
 #pragma unroll
-    for (int s = 0; s < 19; ++s) v[s] = a[pidx(L, s, i)];
+    for (int s = 0; s < 19; ++s) v[s] = ld_pop<NT>(a + pidx(L, s, i));
 #pragma unroll
-    for (int s = 0; s < 19; ++s) b[pidx(L, s, i)] = v[s];
+    for (int s = 0; s < 19; ++s) st_pop<NT>(b + pidx(L, s, i), v[s]);
 }
 
 __global__ __launch_bounds__(512) void fill_ramp(double* __restrict__ a, int64_t n) {
@@ -63,13 +67,18 @@ extern "C" int acp_copy_probe(int32_t planes, int64_t plane_elems, int64_t block
         hipLaunchKernelGGL(acp::fill_ramp, dim3((unsigned)((total + 511) / 512)), dim3(512), 0, s, a, total);
         e = hipGetLastError();
     }
+    const bool nt = 2 * total * (int64_t)sizeof(double) > acp::kCacheBytes;
     auto launch = [&]() {
-        if (planes == 1)
-            hipLaunchKernelGGL(acp::copy_planes<1>, dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
-        else if (L.block)
-            hipLaunchKernelGGL(acp::copy_blocks, dim3(blocks), dim3(512), 0, s, a, b, L);
-        else
-            hipLaunchKernelGGL(acp::copy_planes<19>, dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
+        if (planes == 1) {
+            if (nt) hipLaunchKernelGGL((acp::copy_planes<1, true>), dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
+            else hipLaunchKernelGGL((acp::copy_planes<1, false>), dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
+        } else if (L.block) {
+            if (nt) hipLaunchKernelGGL(acp::copy_blocks<true>, dim3(blocks), dim3(512), 0, s, a, b, L);
+            else hipLaunchKernelGGL(acp::copy_blocks<false>, dim3(blocks), dim3(512), 0, s, a, b, L);
+        } else {
+            if (nt) hipLaunchKernelGGL((acp::copy_planes<19, true>), dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
+            else hipLaunchKernelGGL((acp::copy_planes<19, false>), dim3(blocks), dim3(512), 0, s, a, b, plane_elems);
+        }
         return hipGetLastError();
     };
     if (e == hipSuccess) e = launch();

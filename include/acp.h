@@ -374,9 +374,10 @@ void acp_server_destroy(acp_server* s);
  * access pattern of the bulk kernel without the neighbour shift); with 19
  * planes and `block_nodes` > 0 (a power of two) the planes are laid out in
  * blocks of that many nodes as the engine lays out its float64 and float32
- * populations, with 0 as whole planes.  `reps` copies on `stream` after one
- * untimed copy; writes the mean milliseconds per copy.  Allocates and frees
- * its own two buffers. */
+ * populations, with 0 as whole planes.  Like the bulk kernels, the copy
+ * carries the non-temporal hint when its two buffers exceed the 256 MB
+ * last-level cache.  `reps` copies on `stream` after one untimed copy; writes
+ * the mean milliseconds per copy.  Allocates and frees its own two buffers. */
 int acp_copy_probe(int32_t planes, int64_t plane_elems, int64_t block_nodes, int32_t reps, void* stream,
                    double* ms_per_copy);
 
