@@ -28,17 +28,22 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nx) return;
+    // a row held by one workgroup (row_lds) keeps the threads beyond it for the
+    // LDS exchange of its stores; they load, compute and store nothing
+    const bool active = i < nx;
+    if (!active && !row_lds) return;
     const int j = blockIdx.y, kz = blockIdx.z;
     const I sxy = (I)nx * (I)ny;
     const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
     const K64& k = *kp;
 
     double f[19];
-    static_for<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        f[q] = ld_pop<NT>(&P::at(A, q, l, L));
-    });
+    if (active) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = ld_pop<NT>(&P::at(A, q, l, L));
+        });
+    }
 
     int bi = 0, bj = 0, bk = 0;
     if constexpr (!PERIODIC) {
@@ -62,7 +67,7 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if (kz == 0) ozm = sxy * (I)(nz - 1);
     }
 
-    if (!boundary) {
+    if (active && !boundary) {
         if constexpr (OP == OP_SRT) {
             collide_srt64<ORDER, false>(k, f, nullptr);
         } else if constexpr (OP == OP_RR || OP == OP_HRR) {
@@ -110,36 +115,34 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         }
     }
 
-    // Periodic rows held by one workgroup (row_lds, uniform per block, no thread
-    // left early): the x-shifted links are realigned through LDS so that every
-    // store of a wave covers whole cache lines; position i of the destination
-    // row takes the value of node i - e_x, and a position whose source would
-    // push across an open x face is not written.  Rows with LODI x faces take
-    // the direct stores below (their face nodes push only inward links).
-    if constexpr (PERIODIC) {
-        if (row_lds) {
-            __shared__ double row[kBulkBlock];
-            static_for<0, 18>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                I base = l - (I)i;
-                if constexpr (EY[q] > 0) base += oyp;
-                if constexpr (EY[q] < 0) base += oym;
-                if constexpr (EZ[q] > 0) base += ozp;
-                if constexpr (EZ[q] < 0) base += ozm;
-                if constexpr (EX[q] == 0) {
+    // Rows held by one workgroup (row_lds, uniform per block, no thread left
+    // early): the x-shifted links are realigned through LDS so that every store
+    // of a wave covers whole cache lines.  Position i of the destination row
+    // takes the value of node i - e_x, and is written only if that node pushes
+    // the link (row_source_pushes: boundary nodes, open and LODI x faces);
+    // positions nothing pushes keep their contents, as with direct stores.
+    if (row_lds) {
+        __shared__ double row[kBulkBlock];
+        static_for<0, 18>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            I base = l - (I)i;
+            if constexpr (EY[q] > 0) base += oyp;
+            if constexpr (EY[q] < 0) base += oym;
+            if constexpr (EZ[q] > 0) base += ozp;
+            if constexpr (EZ[q] < 0) base += ozm;
+            if constexpr (EX[q] == 0) {
+                if (active && row_source_pushes<PERIODIC>(q, i, nx, bj, bk, xopen, xlodi))
                     st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), f[q]);
-                } else {
-                    __syncthreads();
-                    row[(i + EX[q] + nx) % nx] = f[q];
-                    __syncthreads();
-                    const int src = i - EX[q];
-                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1))))
-                        st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), row[i]);
-                }
-            });
-            st_pop<NT>(&P::at(B, 18, l, L), f[18]);
-            return;
-        }
+            } else {
+                __syncthreads();
+                if (active) row[(i + EX[q] + nx) % nx] = f[q];
+                __syncthreads();
+                if (active && row_source_pushes<PERIODIC>(q, i - EX[q], nx, bj, bk, xopen, xlodi))
+                    st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), row[i]);
+            }
+        });
+        if (active) st_pop<NT>(&P::at(B, 18, l, L), f[18]);
+        return;
     }
 
     static_for<0, 18>([&](auto qc) {
@@ -172,7 +175,14 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
                      hipStream_t s) {
     const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
-    const int row_lds = PERIODIC && g.nx == bx && !g.xlodi;
+    // the whole row in one workgroup of whole waves: rows of 64 to 512 nodes (a
+    // multiple of 64), whose realigned stores cover whole cache lines.  With
+    // boundary nodes in the rows (LODI, open x faces) only on grids of at least
+    // 4096 rows: on fewer the kernel is latency-bound and the exchange's
+    // barriers cost more than the aligned stores gain (measured: a 256 x 128 x 3
+    // duct 17.3 against 20.7 us per step, a 256^3 LODI cube 1.18 against 1.32 ms).
+    const bool fully_periodic = PERIODIC && !g.xopen && !g.xlodi;
+    const int row_lds = g.nx == bx && (fully_periodic || (int64_t)g.ny * g.nz >= 4096);
 #define B64(NAR, NT)                                                                                          \
     hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, NAR, NT>), grid, block, 0, s, k, A, B, psi, M, g.nx, \
                        g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi)

@@ -181,16 +181,21 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nx) return;
+    // threads beyond a row held by one workgroup only take part in its LDS
+    // exchange (see bulk64)
+    const bool active = i < nx;
+    if (!active && !row_lds) return;
     const int j = blockIdx.y, kz = blockIdx.z;
     const I sxy = (I)nx * (I)ny;
     const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
     const KShift& k = *kp;
     float g[19];
-    static_for<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        g[q] = (float)ld_pop<NT>(&P::at(A, q, l, L));
-    });
+    if (active) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            g[q] = (float)ld_pop<NT>(&P::at(A, q, l, L));
+        });
+    }
     int bi = 0, bj = 0, bk = 0;
     if constexpr (!PERIODIC) {
         bi = i == 0 ? 1 : (i == nx - 1 ? -1 : 0);
@@ -212,38 +217,36 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if (kz == 0) ozm = sxy * (I)(nz - 1);
     }
     if constexpr (OP != OP_STREAM) {
-        if (!boundary) {
+        if (active && !boundary) {
             const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
             const int ghost = PERIODIC ? ((i == nx - 1 && (xopen & 2)) ? 1 : ((i == 0 && (xopen & 1)) ? -1 : 0)) : 0;
             collide_shift<OP, ORDER, NARROW>(k, g, M, l, o, L.N, ghost);
         }
     }
-    // periodic rows held by one workgroup: x-shifted links realigned through
-    // LDS (see bulk64)
-    if constexpr (PERIODIC) {
-        if (row_lds) {
-            __shared__ T row[kBulkBlock];
-            static_for<0, 18>([&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                I base = l - (I)i;
-                if constexpr (EY[q] > 0) base += oyp;
-                if constexpr (EY[q] < 0) base += oym;
-                if constexpr (EZ[q] > 0) base += ozp;
-                if constexpr (EZ[q] < 0) base += ozm;
-                if constexpr (EX[q] == 0) {
+    // rows held by one workgroup: x-shifted links realigned through LDS, each
+    // position written only if its source node pushes the link (see bulk64)
+    if (row_lds) {
+        __shared__ T row[kBulkBlock];
+        static_for<0, 18>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            I base = l - (I)i;
+            if constexpr (EY[q] > 0) base += oyp;
+            if constexpr (EY[q] < 0) base += oym;
+            if constexpr (EZ[q] > 0) base += ozp;
+            if constexpr (EZ[q] < 0) base += ozm;
+            if constexpr (EX[q] == 0) {
+                if (active && row_source_pushes<PERIODIC>(q, i, nx, bj, bk, xopen, xlodi))
                     st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), (T)g[q]);
-                } else {
-                    __syncthreads();
-                    row[(i + EX[q] + nx) % nx] = (T)g[q];
-                    __syncthreads();
-                    const int src = i - EX[q];
-                    if (!((src < 0 && (xopen & 2)) || (src >= nx && (xopen & 1))))
-                        st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), row[i]);
-                }
-            });
-            st_pop<NT>(&P::at(B, 18, l, L), (T)g[18]);
-            return;
-        }
+            } else {
+                __syncthreads();
+                if (active) row[(i + EX[q] + nx) % nx] = (T)g[q];
+                __syncthreads();
+                if (active && row_source_pushes<PERIODIC>(q, i - EX[q], nx, bj, bk, xopen, xlodi))
+                    st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), row[i]);
+            }
+        });
+        if (active) st_pop<NT>(&P::at(B, 18, l, L), (T)g[18]);
+        return;
     }
 
     static_for<0, 18>([&](auto qc) {
@@ -379,7 +382,14 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
 #undef BS2
         return hipGetLastError();
     }
-    const int row_lds = periodic && g.nx == bx && !g.xlodi;
+    // the whole row in one workgroup of whole waves: rows of 64 to 512 nodes (a
+    // multiple of 64), whose realigned stores cover whole cache lines.  With
+    // boundary nodes in the rows (LODI, open x faces) only on grids of at least
+    // 4096 rows: on fewer the kernel is latency-bound and the exchange's
+    // barriers cost more than the aligned stores gain (measured: a 256 x 128 x 3
+    // duct 17.3 against 20.7 us per step, a 256^3 LODI cube 1.18 against 1.32 ms).
+    const bool fully_periodic = periodic && !g.xopen && !g.xlodi;
+    const int row_lds = g.nx == bx && (fully_periodic || (int64_t)g.ny * g.nz >= 4096);
 #define BS(PER, NAR, NT)                                                                                             \
     hipLaunchKernelGGL((bulk_shift<T, OP, ORDER, PER, NAR, NT>), grid, block, 0, s, k, (const T*)A, (T*)B, M, g.nx, \
                        g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi)

@@ -31,4 +31,25 @@ __host__ __device__ inline bool boundary_pushes(int q, int bi, int bj, int bk) {
     return (bi == 0 || EX[q] == bi) && (bj == 0 || EY[q] == bj) && (bk == 0 || EZ[q] == bk);
 }
 
+// Whether the node at x index s of a grid row pushes link q, for the bulk
+// kernels' row-wise stores.  (bj, bk) is the row's y/z boundary class (0 on
+// grids wrapped in y and z).  s may lie one node beyond the row, where the
+// source is the node across the x wrap: none without wrap (LODI on all faces),
+// none across an open or a LODI x face (xopen / xlodi: bit 0 the face s = 0,
+// bit 1 the face s = nx - 1).  Inside the row the push rule above applies, and
+// on a grid wrapped in y and z a LODI x-face node pushes its inward x links.
+template <bool WRAPPED_YZ>
+__host__ __device__ inline bool row_source_pushes(int q, int s, int nx, int bj, int bk, int xopen, int xlodi) {
+    if (s < 0 || s >= nx) {
+        if (!WRAPPED_YZ) return false;
+        return ((xopen | xlodi) & (s < 0 ? 2 : 1)) == 0;
+    }
+    if (WRAPPED_YZ) {
+        const int bi = (s == 0 && (xlodi & 1)) ? 1 : ((s == nx - 1 && (xlodi & 2)) ? -1 : 0);
+        return bi == 0 || EX[q] == bi;
+    }
+    const int bi = s == 0 ? 1 : (s == nx - 1 ? -1 : 0);
+    return (bi | bj | bk) == 0 || boundary_pushes(q, bi, bj, bk);
+}
+
 }  // namespace acp
