@@ -17,7 +17,7 @@ Serving (reported beside, in "serving"): the acoustic-pulse request mix of
 acpulse_vclbm_amd.serve (nx, ny in [128, 1024], nz = 3, five collision
 operators, float64 / float32-shifted, half of them flagged for a 2:1 refined
 region: one-level requests on LODI grids, refined ones as two-level cases on a
-periodic coarse grid), 48 requests with a quarter of the step
+coarse duct with LODI x faces), 48 requests with a quarter of the step
 counts by default (--serve-requests 256 --serve-step-scale 1 for the full mix).
 Sequential float64 capacity and the concurrent server's capacity are measured
 with every request arriving at once; Poisson arrivals (seed 1) at 1.2x the
