@@ -5,6 +5,8 @@
 // same push rule, the same LODI pass order and phases) but read/write 4 or 2
 // bytes per population: 2*19*4 = 152 and 2*19*2 = 76 bytes per node and step.
 // Global reductions (mass, probes) and conversions to SI are done in float64.
+#include <algorithm>
+
 #include "kshift.hpp"
 #include "physf.hpp"
 #include "plane.hpp"
@@ -275,9 +277,10 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
 }
 
 // Periodic rows of nx = 2 * blockDim.x nodes (nx a multiple of 128, no open
-// faces): two nodes per thread, so that each load and store of a wave moves
-// 64 pairs of consecutive populations; each node collides exactly as in
-// bulk_shift and the x-shifted links are realigned through LDS before the
+// faces), blockDim.y consecutive rows per workgroup: two nodes per thread, so
+// that each load and store of a wave moves 64 pairs of consecutive
+// populations; each node collides exactly as in bulk_shift and the x-shifted
+// links are realigned through LDS (one buffer row per grid row) before the
 // pair stores.
 template <typename T>
 struct alignas(2 * sizeof(T)) Pair2 {
@@ -306,40 +309,50 @@ __device__ __forceinline__ void st_pair(T* p, const Pair2<T>& v) {
     st_pop<NT>(reinterpret_cast<PairVec<T>*>(p), u);
 }
 
-template <typename T, int OP, int ORDER, bool NARROW, bool NT>
+// ROWS: blockDim.y rows per workgroup (else one, blockDim.y = 1, compiled
+// without the checks for rows beyond the grid).
+template <typename T, int OP, int ORDER, bool NARROW, bool NT, bool ROWS>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift2(
     const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
     int ny, int nz, const PopLayout L) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const int t = threadIdx.x;
-    const int j = blockIdx.y, kz = blockIdx.z;
+    // blockDim.y consecutive rows per workgroup; a row beyond the grid only
+    // takes part in the LDS exchange
+    const int j = ROWS ? (int)(blockIdx.y * blockDim.y + threadIdx.y) : (int)blockIdx.y, kz = blockIdx.z;
+    const bool active = !ROWS || j < ny;
     const I sxy = (I)nx * (I)ny;
     const I rs = (I)nx * (I)j + sxy * (I)kz;
     const KShift& k = *kp;
     float g[2][19];
-    static_for<0, 19>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        const Pair2<T> v = ld_pair<NT>(&P::at(A, q, (I)(rs + (I)(2 * t)), L));
-        g[0][q] = (float)v.x;
-        g[1][q] = (float)v.y;
-    });
+    if (active) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const Pair2<T> v = ld_pair<NT>(&P::at(A, q, (I)(rs + (I)(2 * t)), L));
+            g[0][q] = (float)v.x;
+            g[1][q] = (float)v.y;
+        });
+    }
     I oyp = (I)nx, oym = (I)0 - (I)nx, ozp = sxy, ozm = (I)0 - sxy;
     if (j == ny - 1) oyp = (I)0 - (I)nx * (I)(ny - 1);
     if (j == 0) oym = (I)nx * (I)(ny - 1);
     if (kz == nz - 1) ozp = (I)0 - sxy * (I)(nz - 1);
     if (kz == 0) ozm = sxy * (I)(nz - 1);
     if constexpr (OP != OP_STREAM) {
+        if (active) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int i = 2 * t + u;
-            const I oxp = i == nx - 1 ? (I)0 - (I)(nx - 1) : (I)1;
-            const I oxm = i == 0 ? (I)(nx - 1) : (I)-1;
-            const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-            collide_shift<OP, ORDER, NARROW>(k, g[u], M, (I)(rs + (I)i), o, L.N, 0);
+            for (int u = 0; u < 2; ++u) {
+                const int i = 2 * t + u;
+                const I oxp = i == nx - 1 ? (I)0 - (I)(nx - 1) : (I)1;
+                const I oxm = i == 0 ? (I)(nx - 1) : (I)-1;
+                const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+                collide_shift<OP, ORDER, NARROW>(k, g[u], M, (I)(rs + (I)i), o, L.N, 0);
+            }
         }
     }
-    __shared__ T row[2 * kBulkBlock];
+    __shared__ T rows[2 * kBulkBlock];
+    T* row = ROWS ? rows + threadIdx.y * nx : rows;
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         I base = rs;
@@ -353,13 +366,15 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             v.y = (T)g[1][q];
         } else {
             __syncthreads();
-            row[(2 * t + EX[q] + nx) % nx] = (T)g[0][q];
-            row[(2 * t + 1 + EX[q] + nx) % nx] = (T)g[1][q];
+            if (active) {
+                row[(2 * t + EX[q] + nx) % nx] = (T)g[0][q];
+                row[(2 * t + 1 + EX[q] + nx) % nx] = (T)g[1][q];
+            }
             __syncthreads();
             v.x = row[2 * t];
             v.y = row[2 * t + 1];
         }
-        st_pair<NT>(&P::at(B, q, (I)(base + (I)(2 * t)), L), v);
+        if (active) st_pair<NT>(&P::at(B, q, (I)(base + (I)(2 * t)), L), v);
     });
 }
 
@@ -372,9 +387,14 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     const bool narrow = narrow_planes(g.L, sizeof(T), sizeof(float));
     const bool nt = stream_populations(g.L, sizeof(T));
     if (periodic && !g.xopen && !g.xlodi && g.nx % 128 == 0 && g.nx / 2 <= kBulkBlock) {
-        const dim3 b2(g.nx / 2, 1, 1), g2(1, g.ny, g.nz);
-#define BS2(NAR, NT)                                                                                          \
-    hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, NAR, NT>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny, \
+        // float32 without HRR: as many whole rows per workgroup as fit
+        // kBulkBlock threads (512-node rows: 36.2 -> 38.9 GLUPS on 512^3); HRR
+        // and float16 keep one row (two rows: HRR 22.6 -> 21.7, float16 75 -> 66)
+        constexpr bool rows = sizeof(T) == 4 && OP != OP_HRR;
+        const int ry = rows ? std::max(1, kBulkBlock / (g.nx / 2)) : 1;
+        const dim3 b2(g.nx / 2, ry, 1), g2(1, (g.ny + ry - 1) / ry, g.nz);
+#define BS2(NAR, NT)                                                                                                \
+    hipLaunchKernelGGL((bulk_shift2<T, OP, ORDER, NAR, NT, rows>), g2, b2, 0, s, k, (const T*)A, (T*)B, M, g.nx, g.ny, \
                        g.nz, g.L)
         if (!narrow) BS2(false, true);
         else if (nt) BS2(true, true);
