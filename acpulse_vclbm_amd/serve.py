@@ -106,7 +106,7 @@ def mix_counts(reqs):
 class Server:
     """acp_server_* handle: submit requests, poll results, read statistics."""
 
-    def __init__(self, hbm_budget_bytes, max_active=8, graph_chunk=64, launch_steps=16, chunks_ahead=4,
+    def __init__(self, hbm_budget_bytes, max_active=8, graph_chunk=16, launch_steps=16, chunks_ahead=4,
                  admit_threads=4):
         L = lib()
         d = ServerDesc(int(hbm_budget_bytes), int(max_active), int(graph_chunk), int(launch_steps),
@@ -168,7 +168,7 @@ class Server:
         self.close()
 
 
-def run_mix(reqs, hbm_budget_bytes, max_active=8, graph_chunk=64, launch_steps=16, chunks_ahead=4,
+def run_mix(reqs, hbm_budget_bytes, max_active=8, graph_chunk=16, launch_steps=16, chunks_ahead=4,
             admit_threads=4, timeout_s=3600.0):
     """Offer `reqs` (arrival times relative to the start) to a fresh server and
     collect every result.  Returns (results sorted by tag, summary dict)."""
@@ -243,7 +243,7 @@ def main():
     ap.add_argument("--step-scale", type=float, default=0.25)
     ap.add_argument("--rate", type=float, default=0.0, help="Poisson arrivals per second; 0: all at once")
     ap.add_argument("--max-active", type=int, default=16)
-    ap.add_argument("--graph-chunk", type=int, default=64)
+    ap.add_argument("--graph-chunk", type=int, default=16)
     ap.add_argument("--budget-gib", type=float, default=64.0)
     a = ap.parse_args()
     reqs = make_requests(a.requests, seed=1, rate=a.rate or None, step_scale=a.step_scale)

@@ -249,7 +249,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=48, help="256 for the full request mix")
     ap.add_argument("--serve-step-scale", type=float, default=0.25, help="1.0 for the full step counts")
     ap.add_argument("--serve-max-active", type=int, default=16)
-    ap.add_argument("--serve-graph-chunk", type=int, default=64)
+    ap.add_argument("--serve-graph-chunk", type=int, default=16)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 4 GiB single-plane device copies (counter calibration) and exit")
     a = ap.parse_args()
