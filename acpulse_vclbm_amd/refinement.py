@@ -40,21 +40,16 @@ GEOMETRY = {
 }
 
 
-def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", coupling="F64", with_fine=True,
-                         f2c_filter="NOFI", geometry="duct"):
-    """Reflection of the refined pulse case for one tier; `coupling` selects the
-    coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED"),
-    `f2c_filter` the fine-to-coarse filter ("NOFI" or "LAG") and `geometry`
-    the coarse level's x faces ("duct": LODI, "periodic")."""
-    g = GEOMETRY[geometry]
-    steps = g["steps"] if steps is None else steps
-    win = min(g["window"], steps)
-    kw = dict(op=op, order=3, boundary=g["boundary"], mrt_mode="RELAXED", precision=precision)
-    d = acp.make_desc(256, 128, 3, tau=TAU, **kw)
-    with acp.Refined(d, 128, g["x1"]) as r:
+def _refined_run(acp, d, x1, f2c_filter, precision, coupling, amp, steps, graph_chunk=0):
+    """One refined pulse run: (probe series at (96, 64, 1), seconds of the
+    stepping, fine-level node count).  With `graph_chunk` > 0 the coarse steps
+    are replayed from captured graphs of that many steps."""
+    with acp.Refined(d, 128, x1) as r:
         r.set_filter(f2c_filter)
         if precision != "F64":
             r.set_coupling_precision(coupling)
+        if graph_chunk:
+            r.enable_graphs(graph_chunk)
         r.init_pulse(amp, 64.0, 64.0, 8.0)
         c = r.level(0)
         c.set_probes([(96, 64, 1)], capacity=steps)
@@ -63,8 +58,24 @@ def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", co
         r.step(steps)
         c.sync()
         dt = time.perf_counter() - t0
-        p_ref = c.read_probes()[:, 0].copy()
-        fine_nodes = r.level(1).N
+        return c.read_probes()[:, 0].copy(), dt, r.level(1).N
+
+
+def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", coupling="F64", with_fine=True,
+                         f2c_filter="NOFI", geometry="duct", graph_chunk=0):
+    """Reflection of the refined pulse case for one tier; `coupling` selects the
+    coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED"),
+    `f2c_filter` the fine-to-coarse filter ("NOFI" or "LAG") and `geometry`
+    the coarse level's x faces ("duct": LODI, "periodic").  With `graph_chunk`
+    > 0 the run is repeated with the coarse steps replayed from captured graphs
+    of that many steps; its timing is reported beside the plain launches' and
+    its probe series must equal theirs bit for bit."""
+    g = GEOMETRY[geometry]
+    steps = g["steps"] if steps is None else steps
+    win = min(g["window"], steps)
+    kw = dict(op=op, order=3, boundary=g["boundary"], mrt_mode="RELAXED", precision=precision)
+    d = acp.make_desc(256, 128, 3, tau=TAU, **kw)
+    p_ref, dt, fine_nodes = _refined_run(acp, d, g["x1"], f2c_filter, precision, coupling, amp, steps)
     rho0 = d.rho0
     with acp.Case(d) as u:
         u.init_pulse(amp, 64.0, 64.0, 8.0)
@@ -86,6 +97,13 @@ def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", co
         "mlups": node_updates * steps / dt / 1e6,
         "probe": p_ref,
     }
+    if graph_chunk:
+        p_g, dt_g, _ = _refined_run(acp, d, g["x1"], f2c_filter, precision, coupling, amp, steps, graph_chunk)
+        if not np.array_equal(p_g, p_ref):
+            raise RuntimeError("refined graph replays differ from plain launches")
+        out["graph_chunk"] = graph_chunk
+        out["ms_per_coarse_step_graph"] = dt_g / steps * 1e3
+        out["mlups_graph"] = node_updates * steps / dt_g / 1e6
     if with_fine:
         # uniform grid at the fine resolution: same viscosity at half the spacing, two steps per coarse step
         df = acp.make_desc(g["fine_nx"], 256, 6, nu=d.nu, spacing=d.spacing / 2.0, **kw)
@@ -134,9 +152,10 @@ def main():
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--f2c-filter", default="NOFI", choices=["NOFI", "LAG"])
     ap.add_argument("--geometry", default="duct", choices=sorted(GEOMETRY))
+    ap.add_argument("--graph-chunk", type=int, default=0, help="also time graph replays of this many coarse steps (even; 0: no)")
     a = ap.parse_args()
     r = interface_reflection(acp, op=a.op, steps=a.steps, precision=a.precision, coupling=a.coupling,
-                             f2c_filter=a.f2c_filter, geometry=a.geometry)
+                             f2c_filter=a.f2c_filter, geometry=a.geometry, graph_chunk=a.graph_chunk)
     r.pop("probe", None)
     print(json.dumps(r))
 

@@ -29,6 +29,13 @@ box 151 x 151 x 21 with HRR, the quasi-2D 1024 x 1024 x 3 serving grid and a
 256^3 cube, float64): whole-grid GLUPS and the step time against the same grid
 with periodic faces.
 
+Refinement (in "refinement"): acpulse_vclbm_amd.refinement, the refined
+acoustic-pulse duct (coarse 256 x 128 x 3 with LODI x faces, the fine level over
+its right half) with RR/NOFI, RR/LAG and HRR/NOFI: the interface echo and the
+time per coarse step with plain launches and replayed from captured graphs of
+--refine-graph-chunk coarse steps (whose probe series must equal the plain
+run's bit for bit).
+
 Precision ablation (in "precision_ablation"): acpulse_vclbm_amd.ablation, the
 200 x 200 x 3 periodic HRR pulse at three amplitudes (field and probe errors of
 the shifted tiers against float64) and the reflection coefficient of the LODI
@@ -250,6 +257,8 @@ def main():
     ap.add_argument("--serve-step-scale", type=float, default=0.25, help="1.0 for the full step counts")
     ap.add_argument("--serve-max-active", type=int, default=16)
     ap.add_argument("--serve-graph-chunk", type=int, default=16)
+    ap.add_argument("--refine-graph-chunk", type=int, default=16,
+                    help="coarse steps per graph replay in the refined-step timing (even; 0: plain launches only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run three 4 GiB single-plane device copies (counter calibration) and exit")
     a = ap.parse_args()
@@ -310,12 +319,14 @@ def main():
         from acpulse_vclbm_amd import refinement
         ref = {}
         for op, filt in (("RR", "NOFI"), ("RR", "LAG"), ("HRR", "NOFI")):
-            x = refinement.interface_reflection(acp, op=op, f2c_filter=filt)
+            x = refinement.interface_reflection(acp, op=op, f2c_filter=filt, graph_chunk=a.refine_graph_chunk)
             x.pop("probe", None)
             ref[f"{op} {filt}"] = x
+            g = (f" ({x['ms_per_coarse_step_graph']:.3f} ms replayed from graphs of {x['graph_chunk']} steps)"
+                 if "graph_chunk" in x else "")
             print(f"# refinement (256x128x3 + fine half, {op}, F2C {filt}): interface echo "
-                  f"{x['R_vs_uniform_coarse']:.2e} of the pulse, {x['ms_per_coarse_step']:.3f} ms per coarse step",
-                  file=sys.stderr, flush=True)
+                  f"{x['R_vs_uniform_coarse']:.2e} of the pulse, {x['ms_per_coarse_step']:.3f} ms per coarse step"
+                  f"{g}", file=sys.stderr, flush=True)
     lodi = None
     if not a.no_lodi:
         lodi = lodi_path(acp, torch)
