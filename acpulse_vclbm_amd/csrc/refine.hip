@@ -151,39 +151,72 @@ __global__ __launch_bounds__(256) void c2f_between(const double* __restrict__ Fc
 // reference calls it at Lattice.cpp:5190-5199): the coarse node receives the
 // coincident fine node's equilibrium plus (1/a) times the mean non-equilibrium
 // of that node and its 18 D3Q19 neighbours, each against its own equilibrium,
-// summed in the order: the node, then directions 0..17.  (fx, fy, fz) is the
-// coincident fine node; its x neighbours exist (fx in [2, fnx - 3]), y and z
-// wrap.
+// summed in the order: the node, then directions 0..17.  The coincident fine
+// node of coarse node (x0 + 1 + di, J, K) is (2 (1 + di), 2 J, 2 K); its x
+// neighbours exist (x in [2, fnx - 3]), y and z wrap.
+// The 19 non-equilibria of a coarse node's mean are spread over 19 lanes:
+// lane s evaluates fine node s of the mean (s = 0 the coincident node, s = 1 + j
+// its neighbour in direction j) into LDS, and after the barrier lane q sums
+// direction q over s = 0, 1, ..., 18 in that order.  (One thread per coarse
+// node chaining the 19 equilibria ran at 256 VGPRs and under one wave per SIMD.)
+constexpr int kLagNodes = 13;  // coarse nodes per 256-thread workgroup (247 lanes)
+
 template <int ORDER>
-__device__ __forceinline__ void lag_mean(const K64& k, const double* __restrict__ F, int fnx, int fny, int fnz,
-                                         int fx, int fy, int fz, double ainv, double v[19], const PopLayout& FN) {
-    double rho, ux, uy, uz, eq[19], acc[19];
-    const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) v[q] = F[pidx(FN, q, l0)];
-    moments64(k, v, rho, ux, uy, uz);
-    feq64<ORDER>(k, rho, ux, uy, uz, eq);
-#pragma unroll
-    for (int q = 0; q < 19; ++q) acc[q] = v[q] - eq[q];
-    static_for<0, 18>([&](auto jc) {
-        constexpr int j = decltype(jc)::value;
-        const int y = (fy + EY[j] + fny) % fny, z = (fz + EZ[j] + fnz) % fnz;
-        const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
-        double g[19], e[19], r, x1, x2, x3;
+__global__ __launch_bounds__(256) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
+                                               int fny, int fnz, double ainv, double* __restrict__ C, int cnx,
+                                               int cny, int cnz, int x0, int width, const PopLayout CN,
+                                               const PopLayout FN) {
+    __shared__ K64 sk;
+    __shared__ double neq[kLagNodes][19][19];  // [node][fine node s][direction]
+    __shared__ double eq0[kLagNodes][19];      // the coincident node's equilibrium
+    const K64& k = stage_k64(kp, sk);
+    const int node = (int)threadIdx.x / 19, s = (int)threadIdx.x % 19;
+    const int64_t t = (int64_t)blockIdx.x * kLagNodes + node;
+    const bool active = node < kLagNodes && t < (int64_t)width * cny * cnz;
+    int di = 0, J = 0, K = 0;
+    if (active) {
+        di = (int)(t % width);
+        const int64_t r = t / width;
+        J = (int)(r % cny);
+        K = (int)(r / cny);
+        int dx = 0, dy = 0, dz = 0;
+        static_for<0, 18>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            if (s == j + 1) {
+                dx = EX[j];
+                dy = EY[j];
+                dz = EZ[j];
+            }
+        });
+        const int fx = 2 * (1 + di), fy = 2 * J, fz = 2 * K;
+        const int y = (fy + dy + fny) % fny, z = (fz + dz + fnz) % fnz;
+        const int64_t l = fx + dx + (int64_t)fnx * (y + (int64_t)fny * z);
+        double g[19], e[19], rho, ux, uy, uz;
 #pragma unroll
         for (int q = 0; q < 19; ++q) g[q] = F[pidx(FN, q, l)];
-        moments64(k, g, r, x1, x2, x3);
-        feq64<ORDER>(k, r, x1, x2, x3, e);
+        moments64(k, g, rho, ux, uy, uz);
+        feq64<ORDER>(k, rho, ux, uy, uz, e);
 #pragma unroll
-        for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
-    });
+        for (int q = 0; q < 19; ++q) neq[node][s][q] = g[q] - e[q];
+        if (s == 0) {
 #pragma unroll
-    for (int q = 0; q < 19; ++q) v[q] = eq[q] + ainv * (acc[q] / 19.);
+            for (int q = 0; q < 19; ++q) eq0[node][q] = e[q];
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+    const int q = s;
+    double acc = neq[node][0][q];
+#pragma unroll
+    for (int m = 1; m < 19; ++m) acc += neq[node][m][q];
+    const int I = (x0 + 1 + di) % cnx;
+    const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
+    C[pidx(CN, q, lc)] = eq0[node][q] + ainv * (acc / 19.);
 }
 
 // Coarse nodes with x in [x0 + 1, x1 - 1] from their coincident fine nodes
-// (F2C-Filter NOFI, or LAG).
-template <int ORDER, bool LAG>
+// (F2C-Filter NOFI; LAG is f2c_lag).
+template <int ORDER>
 __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                            int fny, int fnz, double ainv, double* __restrict__ C, int cnx, int cny,
                                            int cnz, int x0, int width, const PopLayout CN, const PopLayout FN) {
@@ -197,14 +230,10 @@ __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const dou
     const int I = (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
     double f[19];
-    if constexpr (LAG) {
-        lag_mean<ORDER>(k, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, f, FN);
-    } else {
-        const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
 #pragma unroll
-        for (int q = 0; q < 19; ++q) f[q] = F[pidx(FN, q, lf)];
-        rescale<ORDER>(k, f, ainv);
-    }
+    for (int q = 0; q < 19; ++q) f[q] = F[pidx(FN, q, lf)];
+    rescale<ORDER>(k, f, ainv);
 #pragma unroll
     for (int q = 0; q < 19; ++q) C[pidx(CN, q, lc)] = f[q];
 }
@@ -286,7 +315,7 @@ __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, cons
     for (int q = 0; q < 19; ++q) F[pidx(FN, q, lf)] = v[q];
 }
 
-// LAG restriction in the shifted tiers: the same mean as lag_mean, in float64
+// LAG restriction in the shifted tiers: the same mean as f2c_lag, in float64
 // on SI populations (F64) or in float32 on lattice-unit deviations.
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
@@ -563,19 +592,25 @@ hipError_t launch_f2c(acp_refined* r) {
     if (width <= 0) return hipSuccess;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-#define F2C(O, LG)                                                                                              \
-    hipLaunchKernelGGL((acp::refine::f2c<O, LG>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, \
+#define F2C(O)                                                                                                  \
+    hipLaunchKernelGGL((acp::refine::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, \
                        c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
+#define F2CLAG(O)                                                                                                \
+    hipLaunchKernelGGL((acp::refine::f2c_lag<O>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) /             \
+                                                                   acp::refine::kLagNodes)),                     \
+                       b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, c->A, c->g.nx, c->g.ny, \
+                       c->g.nz, r->x0, width, c->g.L, f->g.L)
     if (r->lag) {
-        if (c->d.order == 2) F2C(2, true);
-        else if (c->d.order == 3) F2C(3, true);
-        else F2C(4, true);
+        if (c->d.order == 2) F2CLAG(2);
+        else if (c->d.order == 3) F2CLAG(3);
+        else F2CLAG(4);
     } else {
-        if (c->d.order == 2) F2C(2, false);
-        else if (c->d.order == 3) F2C(3, false);
-        else F2C(4, false);
+        if (c->d.order == 2) F2C(2);
+        else if (c->d.order == 3) F2C(3);
+        else F2C(4);
     }
 #undef F2C
+#undef F2CLAG
     return hipGetLastError();
 }
 
