@@ -159,17 +159,19 @@ __global__ __launch_bounds__(256) void c2f_between(const double* __restrict__ Fc
 // its neighbour in direction j) into LDS, and after the barrier lane q sums
 // direction q over s = 0, 1, ..., 18 in that order.  (One thread per coarse
 // node chaining the 19 equilibria ran at 256 VGPRs and under one wave per SIMD.)
+// Four waves per SIMD: 39.5 KB of LDS per workgroup lets four workgroups share
+// a CU's 160 KB, so the case constants are read from memory rather than from a
+// copy in LDS, and registers are held to 128.
 constexpr int kLagNodes = 13;  // coarse nodes per 256-thread workgroup (247 lanes)
 
 template <int ORDER>
-__global__ __launch_bounds__(256) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
+__global__ __launch_bounds__(256, 4) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                                int fny, int fnz, double ainv, double* __restrict__ C, int cnx,
                                                int cny, int cnz, int x0, int width, const PopLayout CN,
                                                const PopLayout FN) {
-    __shared__ K64 sk;
     __shared__ double neq[kLagNodes][19][19];  // [node][fine node s][direction]
     __shared__ double eq0[kLagNodes][19];      // the coincident node's equilibrium
-    const K64& k = stage_k64(kp, sk);
+    const K64& k = *kp;
     const int node = (int)threadIdx.x / 19, s = (int)threadIdx.x % 19;
     const int64_t t = (int64_t)blockIdx.x * kLagNodes + node;
     const bool active = node < kLagNodes && t < (int64_t)width * cny * cnz;
