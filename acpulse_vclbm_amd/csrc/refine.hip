@@ -161,11 +161,12 @@ __global__ __launch_bounds__(256) void c2f_between(const double* __restrict__ Fc
 // node chaining the 19 equilibria ran at 256 VGPRs and under one wave per SIMD.)
 // Four waves per SIMD: 39.5 KB of LDS per workgroup lets four workgroups share
 // a CU's 160 KB, so the case constants are read from memory rather than from a
-// copy in LDS, and registers are held to 128.
+// copy in LDS, and registers are held to 128 (orders 2 and 3; order 4 needs
+// 130 and runs at three waves, where four would spill).
 constexpr int kLagNodes = 13;  // coarse nodes per 256-thread workgroup (247 lanes)
 
 template <int ORDER>
-__global__ __launch_bounds__(256, 4) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ? 4 : 3))) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                                int fny, int fnz, double ainv, double* __restrict__ C, int cnx,
                                                int cny, int cnz, int x0, int width, const PopLayout CN,
                                                const PopLayout FN) {
