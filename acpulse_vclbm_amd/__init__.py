@@ -155,6 +155,13 @@ class Case:
         """Run every full chunk of `chunk` (even) steps as one captured-graph replay; 0 switches off."""
         check(lib().acp_case_enable_graphs(self.h, int(chunk)))
 
+    BULK_VARIANTS = {"auto": 0, "wide_nt": 1, "narrow_nt": 2, "narrow_plain": 3}
+
+    def set_bulk_variant(self, name):
+        """Force the bulk kernels' addressing and streaming form ("auto",
+        "wide_nt", "narrow_nt", "narrow_plain"); results are identical in every form."""
+        check(lib().acp_case_set_bulk_variant(self.h, self.BULK_VARIANTS[name]))
+
     def checksum(self):
         """Order-independent 64-bit fingerprint of the current population array."""
         v = ctypes.c_uint64()

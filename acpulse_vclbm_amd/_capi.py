@@ -135,6 +135,7 @@ def lib():
         "acp_case_footprint": (ctypes.c_int, [D, i32, i32, ctypes.POINTER(i64)]),
         "acp_case_create_in": (ctypes.c_int, [D, vp, vp, i64, ctypes.POINTER(vp)]),
         "acp_case_checksum": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
+        "acp_case_set_bulk_variant": (ctypes.c_int, [vp, ctypes.c_int32]),
         "acp_refined_create": (ctypes.c_int, [D, i32, i32, vp, ctypes.POINTER(vp)]),
         "acp_refined_destroy": (None, [vp]),
         "acp_refined_level": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),

@@ -866,6 +866,21 @@ extern "C" int acp_case_enable_graphs(acp_case* c, int32_t chunk) {
     return ACP_OK;
 }
 
+extern "C" int acp_case_set_bulk_variant(acp_case* c, int32_t variant) {
+    if (int rc = check_case(c)) return rc;
+    if (variant < acp::BULK_AUTO || variant > acp::BULK_NARROW_PLAIN)
+        return fail(ACP_E_ARG, "unknown bulk-kernel variant");
+    const int eb = shifted(c) ? (int)elem_bytes(c->precision) : 8;
+    if ((variant == acp::BULK_NARROW_NT || variant == acp::BULK_NARROW_PLAIN) &&
+        !acp::narrow_planes(c->g.L, eb, shifted(c) ? 4 : 8))
+        return fail(ACP_E_ARG, "the arrays of this case need 64-bit offsets");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);  // captured graphs hold the old kernels
+    c->g.variant = variant;
+    ++c->graph_epoch;
+    return ACP_OK;
+}
+
 extern "C" int acp_case_checksum(acp_case* c, uint64_t* out) {
     if (int rc = check_case(c)) return rc;
     if (!out) return fail(ACP_E_ARG, "null checksum output");
@@ -941,6 +956,7 @@ extern "C" int acp_case_set_probes(acp_case* c, const int64_t* nodes, int32_t np
         if (nodes[p] < 0 || nodes[p] >= c->g.N) return fail(ACP_E_ARG, "probe node out of range");
     HIPCHK(hipStreamSynchronize(c->stream));
     drop_graphs(c);  // captured graphs hold the old buffers
+    ++c->graph_epoch;
     dfree(c, c->probe_nodes);
     dfree(c, c->probe_buf);
     dfree(c, c->probe_slot);

@@ -93,5 +93,9 @@ struct acp_case {
     bool own_stream = false;
     bool initialized = false;
     int32_t steps = 0;
+    // bumped whenever something a captured graph holds changes (probe buffers,
+    // kernel form): graphs that other objects captured over this case are
+    // stale once it differs from the value they recorded
+    uint64_t graph_epoch = 0;
     int64_t device_bytes = 0;
 };

@@ -39,6 +39,7 @@ struct Grid {
     int xopen = 0;
     PopLayout L;
     int xlodi = 0;
+    int variant = BULK_AUTO;  // addressing and streaming form of the bulk kernels (plane.hpp)
 };
 
 hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64* k, const double* A, double* B,

@@ -186,8 +186,9 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
 #define B64(NAR, NT)                                                                                          \
     hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, NAR, NT>), grid, block, 0, s, k, A, B, psi, M, g.nx, \
                        g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi)
-    if (!narrow_planes(g.L, sizeof(double), sizeof(double))) B64(false, true);
-    else if (stream_populations(g.L, sizeof(double))) B64(true, true);
+    const BulkForm form = bulk_form(g.L, sizeof(double), sizeof(double), g.variant);
+    if (!form.narrow) B64(false, true);
+    else if (form.nt) B64(true, true);
     else B64(true, false);
 #undef B64
     return hipGetLastError();

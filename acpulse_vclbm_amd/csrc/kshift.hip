@@ -384,8 +384,8 @@ static hipError_t bulk_go(bool periodic, const KShift* k, const void* A, void* B
     const int bx = g.nx >= kBulkBlock ? kBulkBlock : ((g.nx + 63) / 64) * 64;
     dim3 block(bx, 1, 1), grid((g.nx + bx - 1) / bx, g.ny, g.nz);
     // the macro array M is float32 like a float32 population plane
-    const bool narrow = narrow_planes(g.L, sizeof(T), sizeof(float));
-    const bool nt = stream_populations(g.L, sizeof(T));
+    const BulkForm form = bulk_form(g.L, sizeof(T), sizeof(float), g.variant);
+    const bool narrow = form.narrow, nt = form.nt;
     if (periodic && !g.xopen && !g.xlodi && g.nx % 128 == 0 && g.nx / 2 <= kBulkBlock) {
         // float32 without HRR: as many whole rows per workgroup as fit
         // kBulkBlock threads (512-node rows: 36.2 -> 38.9 GLUPS on 512^3); HRR

@@ -143,4 +143,24 @@ inline bool narrow_planes(const PopLayout& L, int pop_bytes, int plane_bytes) {
     return node_part * (int64_t)pop_bytes <= ((int64_t)1 << 32) && L.N * (int64_t)plane_bytes <= ((int64_t)1 << 32);
 }
 
+// Addressing and streaming form of a bulk-kernel launch.  By default
+// (BULK_AUTO) it follows the arrays' size: 64-bit offsets when the narrow
+// form cannot address them, the non-temporal hint when they exceed the
+// last-level cache.  Tests force the forms that only large grids take onto
+// small grids, so that every instantiation the large cases run is compared
+// with the oracle; a narrow form is never forced onto arrays it cannot address.
+enum BulkVariant : int { BULK_AUTO = 0, BULK_WIDE_NT = 1, BULK_NARROW_NT = 2, BULK_NARROW_PLAIN = 3 };
+struct BulkForm {
+    bool narrow, nt;
+};
+inline BulkForm bulk_form(const PopLayout& L, int pop_bytes, int plane_bytes, int variant) {
+    const bool narrow = narrow_planes(L, pop_bytes, plane_bytes);
+    switch (variant) {
+        case BULK_WIDE_NT: return {false, true};
+        case BULK_NARROW_NT: return {narrow, true};
+        case BULK_NARROW_PLAIN: return {narrow, !narrow};
+        default: return {narrow, !narrow || stream_populations(L, pop_bytes)};
+    }
+}
+
 }  // namespace acp

@@ -500,6 +500,7 @@ struct acp_refined {
     bool own_stream = false;
     int32_t graph_chunk = 0;              // coarse steps per captured graph (0: plain launches)
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // by coarse + 2 * fine parity
+    uint64_t epoch[2] = {0, 0};  // the levels' graph epochs the captured graphs hold
 };
 
 namespace {
@@ -887,6 +888,17 @@ extern "C" int acp_refined_step(acp_refined* r, int32_t n) {
     for (acp_case* c : {r->coarse, r->fine})
         if (c->nprobe > 0 && c->probe_count + (c == r->fine ? 2 : 1) * n > c->probe_cap)
             return fail(ACP_E_STATE, "probe buffer full: read the probes first");
+    // a level whose probes (or kernel form) changed since the graphs were
+    // captured: the graphs hold its old buffers, so capture them anew
+    if (r->coarse->graph_epoch != r->epoch[0] || r->fine->graph_epoch != r->epoch[1]) {
+        if (r->graph_chunk > 0) {
+            const hipError_t e = hipStreamSynchronize(r->stream);
+            if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_step");
+        }
+        drop_graphs(r);
+        r->epoch[0] = r->coarse->graph_epoch;
+        r->epoch[1] = r->fine->graph_epoch;
+    }
     while (r->graph_chunk > 0 && n >= r->graph_chunk) {
         if (int rc = step_graph_chunk(r)) return rc;
         n -= r->graph_chunk;

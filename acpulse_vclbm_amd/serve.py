@@ -128,12 +128,17 @@ class Server:
             q.f2c_filter = _capi.F2C_FILTER["NOFI"]
         pi, pj, pk = probe_point(r)
         q.probe[0], q.probe[1], q.probe[2] = pi, pj, pk
+        tag = int(r["tag"])
+        # the native server writes the probe series into this buffer when the
+        # case finishes, so it must stay alive and unshared until then
+        if tag in self._probe_bufs:
+            raise ValueError(f"request tag {tag} is already in flight")
         buf = np.zeros(max(q.steps, 1))
-        self._probe_bufs[int(r["tag"])] = buf
         q.probe_out = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         q.arrival = float(r["arrival"] if arrival is None else arrival)
-        q.tag = int(r["tag"])
+        q.tag = tag
         check(lib().acp_server_submit(self.h, ctypes.byref(q)))
+        self._probe_bufs[tag] = buf
 
     def poll(self, max_results=64, wait_ms=0):
         out = (Result * max_results)()

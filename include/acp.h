@@ -171,6 +171,18 @@ int acp_case_sync(acp_case* c);
  * identical to plain stepping; probes keep recording every step. */
 int acp_case_enable_graphs(acp_case* c, int32_t chunk);
 
+/* No counterpart in the reference: the addressing and streaming form of the
+ * bulk (collide + push) kernels.  ACP_BULK_AUTO (the default) follows the
+ * arrays' size: 32-bit offsets from a uniform base while the per-node part of
+ * every offset stays below 4 GiB, 64-bit offsets beyond; non-temporal loads
+ * and stores once the two population arrays exceed the 256 MB last-level
+ * cache.  The other values force one form on any grid, so that tests can
+ * compare the forms only large grids take (the 512^3 float64 grid runs
+ * ACP_BULK_WIDE_NT) with the oracle on small grids; results are identical in
+ * every form.  A 32-bit form on arrays that need 64-bit offsets is ACP_E_ARG. */
+enum { ACP_BULK_AUTO = 0, ACP_BULK_WIDE_NT = 1, ACP_BULK_NARROW_NT = 2, ACP_BULK_NARROW_PLAIN = 3 };
+int acp_case_set_bulk_variant(acp_case* c, int32_t variant);
+
 /* No counterpart in the reference: device bytes a case of `desc` holds, with
  * `nprobe` probes of `probe_capacity` steps, for admission by HBM footprint.
  * Every buffer counts in multiples of 256 bytes.  Validates `desc` as

@@ -1204,10 +1204,17 @@ int acpo_refined_set_filter(acpo_refined* r, int filter) {
 
 void acpo_refined_restrict(acpo_refined* r) { refined_f2c(r); }
 
-/* n coarse steps: coarse step; fine step; c2f at t+1/2; fine step; c2f at t+1; f2c. */
-void acpo_refined_step(acpo_refined* r, int n) {
+/* n coarse steps: coarse step; fine step; c2f at t+1/2; fine step; c2f at t+1; f2c.
+ * Probes (coarse node indices) record sum_q f_q of the coarse level after its
+ * own step, before the coupling writes into it. */
+void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out) {
     for (int s = 0; s < n; ++s) {
         one_step(r->c);
+        for (int i = 0; i < nprobe; ++i) {
+            double v = 0.;
+            for (int q = 0; q < 19; ++q) v += r->c->A[(size_t)q * r->c->N + probe_idx[i]];
+            probe_out[(size_t)s * nprobe + i] = v;
+        }
         one_step(r->f);
         refined_c2f(r, 1);
         one_step(r->f);
@@ -1215,3 +1222,5 @@ void acpo_refined_step(acpo_refined* r, int n) {
         refined_f2c(r);
     }
 }
+
+void acpo_refined_step(acpo_refined* r, int n) { acpo_refined_step_probes(r, n, NULL, 0, NULL); }

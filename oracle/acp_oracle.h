@@ -115,6 +115,9 @@ void acpo_refined_destroy(acpo_refined* r);
 acpo_case* acpo_refined_level(acpo_refined* r, int level);
 double acpo_refined_scale(const acpo_refined* r);
 void acpo_refined_step(acpo_refined* r, int n);
+/* the same, recording sum_q f_q at coarse nodes probe_idx after each coarse
+ * step, before the coupling writes into the coarse level */
+void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out);
 /* fine-to-coarse filter: 0 NOFI (default), 1 LAG; -1 for any other value */
 int acpo_refined_set_filter(acpo_refined* r, int filter);
 /* the fine-to-coarse restriction alone, on the levels' current A arrays */

@@ -95,6 +95,7 @@ def lib():
         L.acpo_refined_scale.argtypes = [ctypes.c_void_p]
         L.acpo_refined_scale.restype = ctypes.c_double
         L.acpo_refined_step.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.acpo_refined_step_probes.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_int, dp]
         L.acpo_refined_set_filter.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.acpo_refined_restrict.argtypes = [ctypes.c_void_p]
         _lib = L
@@ -212,8 +213,18 @@ class OracleRefined:
     def scale(self):
         return lib().acpo_refined_scale(self.h)
 
-    def step(self, n):
-        lib().acpo_refined_step(self.h, n)
+    def step(self, n, probes=()):
+        """n coarse steps; probes (coarse nodes) record the coarse density after
+        the coarse level's own step, before the coupling writes into it."""
+        if not probes:
+            lib().acpo_refined_step(self.h, n)
+            return None
+        p = self.levels[0].p
+        idx = np.array([i + p.nx * (j + p.ny * k) for (i, j, k) in probes], dtype=np.int64)
+        out = np.zeros(n * len(probes))
+        lib().acpo_refined_step_probes(self.h, n, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), len(probes),
+                                       dptr(out))
+        return out.reshape(n, len(probes))
 
     def set_filter(self, name):
         """Fine-to-coarse filter: "NOFI" or "LAG"."""

@@ -95,6 +95,28 @@ def test_refined_graph_replays_match_plain_steps(gpu, spec):
         assert np.array_equal(a.level(0).read_probes(), b.level(0).read_probes())
 
 
+def test_refined_graphs_follow_new_probes(gpu):
+    """Probes set on a level after graph replays: the refined case captures
+    its graphs anew, so the next replays record into the new probe buffer
+    (different node, different capacity) exactly as plain steps do."""
+    d = acp.make_desc(32, 12, 3, tau=0.56, boundary="LODI-X", op="RR")
+    with acp.Refined(d, 16, 31) as a, acp.Refined(d, 16, 31) as b:
+        for r in (a, b):
+            r.init_pulse(1e-3, 8.0, 6.0, 3.0)
+            r.level(0).set_probes([(24, 6, 1)], capacity=16)
+        b.enable_graphs(4)
+        for r in (a, b):
+            r.step(8)
+        assert np.array_equal(a.level(0).read_probes(), b.level(0).read_probes())
+        for r in (a, b):
+            r.level(0).set_probes([(20, 3, 1), (12, 6, 1)], capacity=40)
+            r.step(12)
+        pa, pb = a.level(0).read_probes(), b.level(0).read_probes()
+        assert pa.shape == (12, 2)
+        assert np.array_equal(pa, pb)
+        assert a.checksum() == b.checksum()
+
+
 def test_graph_chunk_argument_checks(gpu):
     d = acp.make_desc(10, 10, 3, tau=0.6)
     with acp.Case(d) as c:
@@ -150,3 +172,82 @@ def test_server_admits_by_footprint(gpu):
         with pytest.raises(_capi.AcpError) as e:
             srv.submit(big)
         assert e.value.code == _capi.ACP_E_OOM
+
+
+def _mix64(x):
+    x = x ^ (x >> np.uint64(30))
+    x = x * np.uint64(0xbf58476d1ce4e5b9)
+    x = x ^ (x >> np.uint64(27))
+    x = x * np.uint64(0x94d049bb133111eb)
+    return x ^ (x >> np.uint64(31))
+
+
+def layout_checksum64(f):
+    """acp_case_checksum of a float64 state [19][N] computed on the host: the
+    engine lays float64 populations out in blocks of 8192 nodes (element
+    (q, l) at (l >> 13) 19 8192 + 8192 q + (l & 8191), the last block padded
+    with zeros; a grid of one block holds whole planes) and sums
+    mix64(bits ^ index * 0x9e3779b97f4a7c15) over every element mod 2^64."""
+    N = f.shape[1]
+    if N <= 8192:
+        flat = f.reshape(-1)
+    else:
+        nb = (N + 8191) // 8192
+        pad = np.zeros((19, nb * 8192))
+        pad[:, :N] = f
+        flat = pad.reshape(19, nb, 8192).transpose(1, 0, 2).reshape(-1)
+    with np.errstate(over="ignore"):
+        bits = flat.view(np.uint64)
+        idx = np.arange(flat.size, dtype=np.uint64) * np.uint64(0x9e3779b97f4a7c15)
+        return int(_mix64(bits ^ idx).sum(dtype=np.uint64))
+
+
+def _oracle_request(r):
+    """The request run by the oracle: final float64 state(s) and probe series."""
+    import oracle_lib as ol
+    from make_golden import CS, DX, RHO0
+    cx, cy = serve.pulse_centre(r)
+    probe = [serve.probe_point(r)]
+    two = serve.two_level(r)
+    p = ol.make_params(r["nx"], r["ny"], r["nz"], op=r["op"], order=3, periodic=3 if two else 0,
+                       mrt_mode=1, tau=serve.TAU, sigma=0.98, cs=CS, spacing=DX, rho0=RHO0)
+    if two:
+        x0, x1 = serve.fine_region(r)
+        o = ol.OracleRefined(p, x0, x1)
+        o.set_filter("NOFI")
+        o.levels[0].set_state(ol.pulse_ic(p, r["amp"], cx, cy, r["halfwidth"]))
+        o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, r["amp"], 2 * (cx - x0), 2 * cy, 2 * r["halfwidth"]))
+        series = o.step(r["steps"], probe)[:, 0]
+        return [o.levels[0].state(), o.levels[1].state()], series
+    o = ol.OracleCase(p)
+    o.set_state(ol.pulse_ic(p, r["amp"], cx, cy, r["halfwidth"]))
+    series = o.step(r["steps"], probe)[:, 0]
+    return [o.state()], series
+
+
+def test_served_results_match_the_oracle(gpu):
+    """Every case of a served mix (one- and two-level, float64 and
+    float32-shifted) against the oracle's own run of the same request: float64
+    cases give the oracle's probe series to the bit and its final state to the
+    bit (the served checksum equals the checksum of the oracle's state, summed
+    over both levels of a two-level case) and its mass to 1e-13; float32-shifted
+    cases follow the oracle's probe series within 2e-4 of the pulse amplitude."""
+    reqs = _small_mix(12, seed=5)
+    assert any(serve.two_level(r) for r in reqs) and any(not serve.two_level(r) for r in reqs)
+    assert {r["precision"] for r in reqs} == {"F64", "F32-SHIFTED"}
+    results, summary = serve.run_mix(reqs, hbm_budget_bytes=1 << 30, max_active=4, graph_chunk=8, chunks_ahead=2,
+                                     timeout_s=120)
+    assert summary["failed"] == 0
+    for res, q in zip(results, reqs):
+        states, series = _oracle_request(q)
+        print("case", q["tag"], q["op"], q["precision"], "two-level" if serve.two_level(q) else "one-level",
+              flush=True)
+        if q["precision"] == "F64":
+            assert np.array_equal(res["probe"], series), q
+            csum = sum(layout_checksum64(s) for s in states) % (1 << 64)
+            assert res["checksum"] == csum, q
+            assert res["mass"] == pytest.approx(states[0].sum(), rel=1e-13, abs=0), q
+        else:
+            rho0 = 1.17621
+            err = np.max(np.abs(res["probe"] - series)) / (q["amp"] * rho0)
+            assert err <= 2e-4, (q, err)

@@ -132,3 +132,19 @@ def test_refined_requests_in_the_mix():
         else:
             assert d.boundary == _capi.BOUNDARY["LODI"]
             assert serve.footprint(r) == _fp(d, 1, r["steps"])
+
+
+def test_server_refuses_a_tag_in_flight():
+    """The native server writes a finished case's probe series into the
+    buffer registered under its tag: a second request with a tag still in
+    flight is refused before anything reaches the server, and the first
+    request's buffer stays registered."""
+    srv = serve.Server.__new__(serve.Server)
+    srv.h = None
+    first = np.zeros(8)
+    srv._probe_bufs = {5: first}
+    req = serve.make_requests(1, seed=1)[0]
+    req["tag"] = 5
+    with pytest.raises(ValueError):
+        srv.submit(req)
+    assert srv._probe_bufs[5] is first
