@@ -55,10 +55,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def run_tier(acp, torch, precision, n, steps, warmup, seed=0):
+def run_tier(acp, torch, precision, n, steps, warmup, seed=0, op="MRT-RM", order=3):
     stream = torch.cuda.Stream()
-    d = acp.make_desc(n, n, n, op="MRT-RM", order=3, boundary="PERIODIC", mrt_mode="RELAXED",
-                      precision=precision, tau=0.503)
+    d = acp.make_desc(n, n, n, op=op, order=order, boundary="PERIODIC", mrt_mode="RELAXED",
+                      precision=precision, tau=0.503, sigma=0.98)
     case = acp.Case(d, stream=stream.cuda_stream)
     try:
         case.init_random(seed, 1e-3, 1e-3)
@@ -83,7 +83,7 @@ def run_tier(acp, torch, precision, n, steps, warmup, seed=0):
     ms_step = ms / steps
     nodes = n ** 3
     glups = nodes / (ms_step * 1e-3) / 1e9
-    bpn = 304 if precision == "F64" else (152 if precision == "F32-SHIFTED" else 76)
+    bpn = info.bytes_per_node_step
     achieved = nodes * bpn / (ms_step * 1e-3) / 1e9
     return {
         "glups": glups, "ms_per_step": ms_step, "bytes_per_node_step": bpn, "achieved_gbs": achieved,
@@ -182,6 +182,9 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
     _, cap = serve.run_mix(seq_all, budget, max_active=1, graph_chunk=0, launch_steps=1, chunks_ahead=64,
                            admit_threads=1)
     _, srv_cap = serve.run_mix(base, budget, max_active=max_active, graph_chunk=graph_chunk, chunks_ahead=4)
+    # the same requests all in float64 through the concurrent server: the
+    # concurrency gain alone, without the float32-shifted tier's smaller steps
+    _, srv_cap64 = serve.run_mix(seq_all, budget, max_active=max_active, graph_chunk=graph_chunk, chunks_ahead=4)
     rate = 1.2 * cap["cases_per_hour"] / 3600.0
     reqs = serve.make_requests(n_req, seed=1, rate=rate, step_scale=step_scale)
     res, mix = serve.run_mix(reqs, budget, max_active=max_active, graph_chunk=graph_chunk, chunks_ahead=4)
@@ -189,10 +192,14 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
     _, seq = serve.run_mix(seq_reqs, budget, max_active=1, graph_chunk=0, launch_steps=1, chunks_ahead=64,
                            admit_threads=1)
     keep = ("cases_per_hour", "latency_p50_s", "latency_p99_s", "makespan_s", "failed")
+    full = n_req == 256 and step_scale == 1.0
     return {
+        "workload": ("the full request mix: 256 requests, 1000-8000 steps" if full else
+                     f"REDUCED request mix: the first {n_req} of the seed-1 requests, step counts x {step_scale}"),
         "requests": n_req, "step_scale": step_scale, "offered_rate_per_hour": round(rate * 3600.0, 1),
         "sequential_f64_capacity_per_hour": round(cap["cases_per_hour"], 1),
         "server_capacity_per_hour": round(srv_cap["cases_per_hour"], 1),
+        "server_capacity_all_f64_per_hour": round(srv_cap64["cases_per_hour"], 1),
         "server": {**{k: (round(v, 4) if isinstance(v, float) else v) for k, v in mix.items() if k in keep},
                    "peak_active": mix["peak_active"], "max_active": max_active, "graph_chunk": graph_chunk},
         "sequential_f64_same_arrivals": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in seq.items()
@@ -212,14 +219,63 @@ def traffic_from_profiles(name):
     return float(m.group(1)) if m else None
 
 
+def host_cpu():
+    """The host's processor: model name, logical CPUs of the machine, the CPUs
+    this process may run on, and the CPU quota of its control group (cores;
+    None when unlimited)."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": affinity, "cgroup_quota_cores": quota}
+
+
+def reference_measured():
+    """The reference solver's own speed, measured when its fixtures were made
+    (its unmodified sources built with g++ -O3 -fopenmp, 8 OpenMP threads on
+    the 8-core Xeon of the development container; the time of its solve()
+    calls): the raw-moment MRT case on 48^3 with LODI faces, the workload
+    closest to this benchmark's, and the config-1R quasi-2D cases."""
+    import numpy as np
+    out = []
+    for name in ("cfg3r_mrtrm", "cfg1r_srt", "cfg1r_hrr_tr"):
+        path = os.path.join(ROOT, "tests", "golden", f"big_{name}.npz")
+        if not os.path.exists(path):
+            continue
+        d = np.load(path, allow_pickle=False)
+        spec = json.loads(str(d["spec"]))
+        nodes = spec["nx"] * spec["ny"] * spec["nz"]
+        steps = int(d["nsteps"])
+        secs = float(d["ref_seconds"])
+        out.append({"case": f"{spec['nx']}x{spec['ny']}x{spec['nz']} {spec['op']}, LODI faces, {steps} steps",
+                    "value": round(nodes * steps / secs / 1e9, 6), "unit": "GLUPS", "cores": int(d["ref_threads"]),
+                    "cpu": "Intel Xeon @ 2.70 GHz (8 cores, development container)", "kind": "reference"})
+    return out
+
+
 def cpu_baseline(sample_n=96, budget_s=12.0):
-    """The CPU oracle (C, OpenMP) on a bounded sample of the same workload."""
+    """The CPU oracle (C, OpenMP) on a bounded sample of the same workload, on
+    every CPU this process may use (capped by its control group's quota)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import numpy as np
     import oracle_lib as ol
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    threads = max(1, min(threads, 16))
+    hc = host_cpu()
+    threads = hc["affinity"]
+    if hc["cgroup_quota_cores"]:
+        threads = min(threads, max(1, int(hc["cgroup_quota_cores"])))
     p = ol.make_params(sample_n, sample_n, sample_n, op="MRT-RM", order=3, periodic=True, mrt_mode=1, tau=0.503,
                        nthreads=threads, cs=1.0 / math.sqrt(3.0), spacing=1.0, rho0=1.0)
     N = sample_n ** 3
@@ -237,7 +293,8 @@ def cpu_baseline(sample_n=96, budget_s=12.0):
         steps += 2
     glups = N * steps / t / 1e9
     return {"value": glups, "unit": "GLUPS", "cores": threads, "kind": "port",
-            "sample": f"{sample_n}^3 periodic MRT-RM relaxed, float64 oracle, {steps} steps in {t:.1f} s"}
+            "sample": f"{sample_n}^3 periodic MRT-RM relaxed, float64 oracle, {steps} steps in {t:.1f} s",
+            "host": hc, "reference_measured": reference_measured()}
 
 
 def main():
@@ -246,6 +303,9 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--op", default="MRT-RM",
+                    help="collision operator of the timed tiers (the measured workload is MRT-RM; others for profiles)")
+    ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tiers", default="F64,F32-SHIFTED")
     ap.add_argument("--no-copy-ceiling", action="store_true")
@@ -283,7 +343,7 @@ def main():
     tiers = a.tiers.split(",")
     res = {}
     for t in tiers:
-        res[t] = run_tier(acp, torch, t, a.n, a.steps, a.warmup)
+        res[t] = run_tier(acp, torch, t, a.n, a.steps, a.warmup, op=a.op, order=a.order)
         print(f"# {t}: {res[t]['glups']:.2f} GLUPS, {res[t]['ms_per_step']:.3f} ms/step, "
               f"{100 * res[t]['frac']:.1f}% of HBM peak", file=sys.stderr, flush=True)
     main_t = "F64" if "F64" in res else tiers[0]
