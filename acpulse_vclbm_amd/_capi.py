@@ -123,6 +123,7 @@ def lib():
         "acp_case_get_populations": (ctypes.c_int, [vp, dp]),
         "acp_case_get_previous_populations": (ctypes.c_int, [vp, dp]),
         "acp_case_get_macros": (ctypes.c_int, [vp, dp, dp]),
+        "acp_case_get_macros_noalloc": (ctypes.c_int, [vp, dp, dp]),
         "acp_case_mass": (ctypes.c_int, [vp, dp]),
         "acp_case_set_probes": (ctypes.c_int, [vp, ctypes.POINTER(i64), i32, i32]),
         "acp_case_read_probes": (ctypes.c_int, [vp, dp, ctypes.POINTER(i32)]),

@@ -207,6 +207,14 @@ int acp_case_get_previous_populations(acp_case* c, double* f);
 /* Density and velocity of every node from the current array (Node.cpp:11-30,
  * 85-118, without cubic-Mach terms): rho [N], u [3][N], float64. */
 int acp_case_get_macros(acp_case* c, double* rho, double* u);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): acp_case_get_macros with the same results, staged in a
+ * fixed device buffer of the library instead of one allocated per call.  The
+ * buffer holds 4 194 304 nodes (128 MiB, held by each GPU the library runs on
+ * until the process ends); a larger case is ACP_E_ARG.  Calls that overlap in
+ * time on one GPU share that buffer and race: serialise them. */
+int acp_case_get_macros_noalloc(acp_case* c, double* rho, double* u);
 /* Total mass sum over all nodes of sum_q f_q, reduced in float64 on the device. */
 int acp_case_mass(acp_case* c, double* mass);
 /* Probes: nodes whose density (float64) is recorded on the device after every
