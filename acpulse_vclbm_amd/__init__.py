@@ -215,6 +215,10 @@ class Refined:
     def init_pulse(self, amp, x0, y0, b):
         check(lib().acp_refined_init_pulse(self.h, amp, x0, y0, b))
 
+    def init_reference(self):
+        """Lattice::initialization (Lattice.cpp:1397-1493) on both levels."""
+        check(lib().acp_refined_init_reference(self.h))
+
     def set_coupling_precision(self, precision):
         """Shifted tiers: coupling arithmetic in "F64" (default) or "F32-SHIFTED"."""
         check(lib().acp_refined_set_coupling_precision(self.h, PREC[precision]))

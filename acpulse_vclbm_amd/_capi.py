@@ -124,6 +124,7 @@ def lib():
         "acp_case_get_previous_populations": (ctypes.c_int, [vp, dp]),
         "acp_case_get_macros": (ctypes.c_int, [vp, dp, dp]),
         "acp_case_get_macros_noalloc": (ctypes.c_int, [vp, dp, dp]),
+        "acp_case_get_psi": (ctypes.c_int, [vp, dp]),
         "acp_case_mass": (ctypes.c_int, [vp, dp]),
         "acp_case_set_probes": (ctypes.c_int, [vp, ctypes.POINTER(i64), i32, i32]),
         "acp_case_read_probes": (ctypes.c_int, [vp, dp, ctypes.POINTER(i32)]),
@@ -141,6 +142,7 @@ def lib():
         "acp_refined_destroy": (None, [vp]),
         "acp_refined_level": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
         "acp_refined_init_pulse": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl]),
+        "acp_refined_init_reference": (ctypes.c_int, [vp]),
         "acp_refined_step": (ctypes.c_int, [vp, i32]),
         "acp_refined_set_coupling_precision": (ctypes.c_int, [vp, i32]),
         "acp_refined_set_filter": (ctypes.c_int, [vp, i32]),

@@ -701,17 +701,29 @@ extern "C" int acp_case_info_get(const acp_case* c, acp_case_info* out) {
     return ACP_OK;
 }
 
-extern "C" int acp_case_init_reference(acp_case* c) {
+namespace acp {
+// Lattice::initialization (Lattice.cpp:1397-1493) on one level of a lattice:
+// the reference's hard-coded pulse (Lattice.cpp:1415-1433) at the level's node
+// coordinates (createLattice, Lattice.cpp:33-35): x = (i + i0) * (spacing /
+// 2^level), y = j * (spacing / 2^level), with the level-0 spacing, density
+// and domain size of d0; i0 is the level's first node in the level's own
+// index space of the whole domain.
+int init_reference_level(acp_case* c, int level, int i0, const acp_case_desc& d0) {
     if (int rc = check_case(c)) return rc;
-    // Lattice.cpp:1415-1433 with node coordinates of createLattice (Lattice.cpp:33-35)
     const double Rc = 0.06, A = 0.01, xc = -8. * Rc;
-    const double dens = c->d.rho0, sp = c->d.spacing, lx = c->d.domain[0], ly = c->d.domain[1];
+    const double dens = d0.rho0, sp = d0.spacing, lx = d0.domain[0], ly = d0.domain[1];
     return init_rho_field(c, [&](int i, int j, int) {
-        const double x = i * (sp / std::pow(2., 0)), y = j * (sp / std::pow(2., 0));
+        const double x = (i + i0) * (sp / std::pow(2., level)), y = j * (sp / std::pow(2., level));
         const double xn = x - lx / 2., yn = y - ly / 2.;
         const double xr = xn - xc;
         return dens * (1 + A * std::exp(-(xr * xr + yn * yn) / (2 * Rc * Rc)));
     });
+}
+}  // namespace acp
+
+extern "C" int acp_case_init_reference(acp_case* c) {
+    if (int rc = check_case(c)) return rc;
+    return acp::init_reference_level(c, 0, 0, c->d);
 }
 
 extern "C" int acp_case_init_pulse(acp_case* c, double amp, double x0, double y0, double b) {
@@ -928,6 +940,19 @@ extern "C" int acp_case_get_macros(acp_case* c, double* rho, double* u) {
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     (void)hipFree(buf);
     if (e != hipSuccess) return acp::hip_fail(e, "acp_case_get_macros");
+    return ACP_OK;
+}
+
+extern "C" int acp_case_get_psi(acp_case* c, double* psi) {
+    if (int rc = check_case(c)) return rc;
+    if (!psi) return fail(ACP_E_ARG, "null output array");
+    const size_t n = (size_t)19 * c->g.N;
+    if (!c->psi) {
+        std::fill(psi, psi + n, 0.);
+        return ACP_OK;
+    }
+    HIPCHK(hipMemcpyAsync(psi, c->psi, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     return ACP_OK;
 }
 

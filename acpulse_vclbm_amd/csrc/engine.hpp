@@ -42,6 +42,10 @@ LodiSchedule build_lodi_schedule(int nx, int ny, int nz);
 // bit 1 the face i = nx - 1; nx >= 4).
 LodiSchedule build_lodi_schedule_x(int nx, int ny, int nz, int faces);
 
+// The reference's initial pulse on one level of a lattice (acp_case_init_reference
+// for level 0; refined cases for their fine level).
+int init_reference_level(acp_case* c, int level, int i0, const acp_case_desc& d0);
+
 // Launch n plain steps of a case on its stream (no graphs, no checks); the
 // current array is c->A (c->SA) afterwards and the one read, c->B (c->SB).
 int case_step_launch(acp_case* c, int32_t n);

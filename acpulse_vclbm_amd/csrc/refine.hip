@@ -881,6 +881,14 @@ extern "C" int acp_refined_init_pulse(acp_refined* r, double amp, double x0, dou
     return rc;
 }
 
+extern "C" int acp_refined_init_reference(acp_refined* r) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_init_reference: null argument");
+    int rc = acp::init_reference_level(r->coarse, 0, 0, r->coarse->d);
+    // the fine level's first node is coarse node x0, fine index 2 x0 of the domain
+    if (rc == ACP_OK) rc = acp::init_reference_level(r->fine, 1, 2 * r->x0, r->coarse->d);
+    return rc;
+}
+
 extern "C" int acp_refined_step(acp_refined* r, int32_t n) {
     if (!r) return fail(ACP_E_ARG, "acp_refined_step: null argument");
     if (n < 0) return fail(ACP_E_ARG, "negative step count");

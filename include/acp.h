@@ -215,6 +215,9 @@ int acp_case_get_macros(acp_case* c, double* rho, double* u);
  * until the process ends); a larger case is ACP_E_ARG.  Calls that overlap in
  * time on one GPU share that buffer and race: serialise them. */
 int acp_case_get_macros_noalloc(acp_case* c, double* rho, double* u);
+/* Node::getCubicMachCorrection (Node.h:52) for every node: the cubic-Mach
+ * terms psi [19][N] the last step computed (zeros for a case without them). */
+int acp_case_get_psi(acp_case* c, double* psi);
 /* Total mass sum over all nodes of sum_q f_q, reduced in float64 on the device. */
 int acp_case_mass(acp_case* c, double* mass);
 /* Probes: nodes whose density (float64) is recorded on the device after every
@@ -284,6 +287,11 @@ void acp_refined_destroy(acp_refined* r);
 int acp_refined_level(acp_refined* r, int32_t level, acp_case** out);
 /* acp_case_init_pulse on both levels with the same field (coarse node units). */
 int acp_refined_init_pulse(acp_refined* r, double amp, double x0, double y0, double b);
+/* Lattice::initialization (Lattice.cpp:1397-1493) on both levels: the
+ * reference's hard-coded pulse evaluated at each level's own node
+ * coordinates (fine node i of the region lies at fine index 2 x0 + i of the
+ * domain, spacing / 2), feq_N(rho, 0) in both arrays. */
+int acp_refined_init_reference(acp_refined* r);
 /* n coarse steps (2n fine steps). */
 int acp_refined_step(acp_refined* r, int32_t n);
 /* Shifted tiers: run the coupling arithmetic (time interpolation, rescaling,

@@ -1004,6 +1004,31 @@ void acpo_reference_ic(const acpo_params* p, double domx, double domy, double* f
             }
 }
 
+/* Lattice.cpp:1397-1493 on level `level` of a lattice: the pulse at the level's
+ * node coordinates x = (i + i0) * (spacing0 / 2^level), y = j * (spacing0 /
+ * 2^level) (createLattice, Lattice.cpp:33-35), with the level-0 spacing and
+ * domain size; p describes the level's own grid. */
+void acpo_reference_ic_level(const acpo_params* p, int level, int i0, double spacing0, double domx, double domy,
+                             double* f) {
+    acpo_consts k;
+    acpo_consts_init(&k, p);
+    const int64_t N = (int64_t)p->nx * p->ny * p->nz;
+    const double Rc = 0.06, A = 0.01, xc = -8. * Rc;
+    for (int kk = 0; kk < p->nz; ++kk)
+        for (int jj = 0; jj < p->ny; ++jj)
+            for (int ii = 0; ii < p->nx; ++ii) {
+                const double x = (ii + i0) * (spacing0 / pow(2., level)), y = jj * (spacing0 / pow(2., level));
+                const double xn = x - domx / 2., yn = y - domy / 2.;
+                const double xr = xn - xc;
+                const double rho = p->rho0 * (1 + A * exp(-(xr * xr + yn * yn) / (2 * Rc * Rc)));
+                const double u[3] = {0., 0., 0.};
+                double eq[19];
+                acpo_feq(&k, p->order, rho, u, eq);
+                const int64_t l = (int64_t)ii + (int64_t)p->nx * ((int64_t)jj + (int64_t)p->ny * kk);
+                for (int q = 0; q < 19; ++q) f[(size_t)q * N + l] = eq[q];
+            }
+}
+
 void acpo_pulse_ic(const acpo_params* p, double amp, double x0, double y0, double b, double* f) {
     acpo_consts k;
     acpo_consts_init(&k, p);

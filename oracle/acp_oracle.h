@@ -99,6 +99,10 @@ const acpo_consts* acpo_case_consts(const acpo_case* c);
 
 /* Reference initial condition (Lattice.cpp:1397-1493): feq_N of the hard-coded pulse. */
 void acpo_reference_ic(const acpo_params* p, double domx, double domy, double* f);
+/* The same on level `level` (first node at level index i0 of the whole domain,
+ * level-0 spacing spacing0); p is the level's own grid. */
+void acpo_reference_ic_level(const acpo_params* p, int level, int i0, double spacing0, double domx, double domy,
+                             double* f);
 /* feq_N(rho[l], u[.][l]) for N nodes: rho [N], u [3][N] -> f [19][N] */
 void acpo_equilibrium_field(const acpo_params* p, int64_t N, const double* rho, const double* u, double* f);
 /* Parameterised pulse in node units: rho = rho0 (1 + amp exp(-ln2 ((i-x0)^2 + (j-y0)^2) / b^2)),

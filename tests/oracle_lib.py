@@ -85,6 +85,8 @@ def lib():
         L.acpo_lodi_count.restype = ctypes.c_int64
         L.acpo_lodi_schedule.argtypes = [ctypes.c_void_p, ip, ctypes.POINTER(ctypes.c_int32)]
         L.acpo_reference_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, dp]
+        L.acpo_reference_ic_level.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_double, dp]
         L.acpo_equilibrium_field.argtypes = [P, ctypes.c_int64, dp, dp, dp]
         L.acpo_pulse_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         L.acpo_refined_create.argtypes = [P, ctypes.c_int, ctypes.c_int]
@@ -263,6 +265,12 @@ def pulse_ic(p, amp, x0, y0, b):
 def reference_ic(p, domx, domy):
     f = np.empty((19, p.nx * p.ny * p.nz))
     lib().acpo_reference_ic(ctypes.byref(p), domx, domy, dptr(f))
+    return f
+
+
+def reference_ic_level(p, level, i0, spacing0, domx, domy):
+    f = np.empty((19, p.nx * p.ny * p.nz))
+    lib().acpo_reference_ic_level(ctypes.byref(p), level, i0, spacing0, domx, domy, dptr(f))
     return f
 
 

@@ -142,3 +142,17 @@ def test_keyword_readers_through_the_c_abi(tmp_path):
     assert L.acp_control_read_filename(str(f).encode(), b"Path", buf, 64) == 0 and buf.value == b"/a b/c d "
     assert L.acp_control_read_string(str(f).encode(), b"Name", buf, 3) == acp._capi.ACP_E_ARG
     assert L.acp_control_read_value(str(f).encode(), b"Beta", ctypes.byref(v)) == acp._capi.ACP_E_CONFIG
+
+
+def test_reference_two_level_geometry_is_refused_with_its_reason(tmp_path):
+    """The reference's own MaxLevel 1 geometry (a fine shell around a coarse
+    box, Lattice.cpp:38-126, selected by its keys alone) is not built: the
+    Lattice constructor exits with status 1 and names the geometry that is
+    available instead, before it touches a device."""
+    exe = os.path.join(ROOT, "tests", "_bin", "lattice_check")
+    assert os.path.exists(exe), "tests/_bin/lattice_check is not built (__graft_entry__.build())"
+    f = tmp_path / "bc.txt"
+    f.write_text(bc_text(20, 20, 7, op="HRR").replace("MaxLevel 0", "MaxLevel 1"))
+    r = subprocess.run([exe, str(f), "create"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "fine shell" in r.stderr and "Refined-Region" in r.stderr and "Boundary LODI-X" in r.stderr
