@@ -234,6 +234,16 @@ void acpo_offeq(const acpo_consts* k, int order, const double a2[6], const doubl
 /* Node.cpp:1750-1889.  nb_macro[m] = pre-collision (rho,ux,uy,uz) of neighbour m in
  * directions 0..5 (+x,+y,-x,-y,+z,-z).  S = {xx,yy,zz,xy,xz,yz}. */
 void acpo_strain_psi(const acpo_consts* k, const double nb[6][4], double S[6], double psi[19]) {
+    double d[6];
+    for (int m = 0; m < 6; ++m) d[m] = k->spacing / pow(2., 0);
+    acpo_strain_psi_dist(k, nb, d, S, psi);
+}
+
+/* The arithmetic shared by Node.cpp:1750-1889 and the partner functions of
+ * Node.cpp:1892-2490: central differences of velocity and of the ten cubic
+ * terms over the six axis neighbours at distanceToNeighbor dist[m]. */
+void acpo_strain_psi_dist(const acpo_consts* k, const double nb[6][4], const double dist[6], double S[6],
+                          double psi[19]) {
     double P[10][6];
     double d[6];
     for (int m = 0; m < 6; ++m) {
@@ -248,7 +258,7 @@ void acpo_strain_psi(const acpo_consts* k, const double nb[6][4], double S[6], d
         P[7][m] = 0.5 * r * vy * vx * vx; /* YZZ */
         P[8][m] = r * vz * vz * vz;        /* ZZZ */
         P[9][m] = r * vx * vy * vz;        /* XYZ */
-        d[m] = k->spacing / pow(2., 0);
+        d[m] = dist[m];
     }
     const double dx = d[0] + d[2], dy = d[1] + d[3], dz = d[4] + d[5];
     enum { XXX, XXY, XXZ, XYY, YYY, YYZ, XZZ, YZZ, ZZZ, XYZ };
@@ -276,6 +286,98 @@ void acpo_strain_psi(const acpo_consts* k, const double nb[6][4], double S[6], d
     S[3] = 0.5 * ((nb[0][2] - nb[2][2]) / dx + (nb[1][1] - nb[3][1]) / dy);
     S[4] = 0.5 * ((nb[0][3] - nb[2][3]) / dx + (nb[4][1] - nb[5][1]) / dz);
     S[5] = 0.5 * ((nb[1][3] - nb[3][3]) / dy + (nb[4][2] - nb[5][2]) / dz);
+}
+
+/* ------------------------------------------------- strain-rate partners
+ * The strain rate and cubic-Mach terms that the two-level step computes at
+ * interface nodes from nodes of both levels (Node.cpp:1892-2490).  Each takes
+ * the populations of the nodes the reference function reads, per axis
+ * direction m (+x, +y, -x, -y, +z, -z):
+ *   nb[m]  the axis neighbour's populations in its level's current array,
+ *   pre[m] its pre-collision copy,
+ *   oth[m] the populations of the node on the other level the function
+ *          reaches for direction m instead (a partner or a second neighbour).
+ * Densities and velocities are calcDensity / calcVelocity of those arrays
+ * (Node.cpp:11-154, without cubic-Mach terms). */
+
+static void macros4(const acpo_consts* k, const double f[19], double m4[4]) {
+    double u[3];
+    acpo_moments(k, f, NULL, &m4[0], u);
+    m4[1] = u[0]; m4[2] = u[1]; m4[3] = u[2];
+}
+
+/* linear time interpolation of density and velocity (Node.cpp:2305-2315, 1992-1995) */
+static void macros_mid(const acpo_consts* k, const double past[19], const double fut[19], double m4[4]) {
+    double a[4], b[4];
+    macros4(k, past, a);
+    macros4(k, fut, b);
+    m4[0] = 0.5 * (a[0] + b[0]);
+    for (int c = 1; c < 4; ++c) m4[c] = 0.5 * (a[c] + b[c]);
+}
+
+/* Node.cpp:2114-2240 (CoarsePartners, t + 1) on a level-0 node: a neighbour
+ * that is a grid-coupling node (gc[m]) is read through its fine partner
+ * (oth[m], level-1 array), any other from its own level-0 array (nb[m]);
+ * distances are the level-0 spacing. */
+void acpo_strain_coarse_partner(const acpo_consts* k, double spacing0, const int gc[6], const double nb[6][19],
+                                const double oth[6][19], double S[6], double psi[19]) {
+    double m4[6][4], d[6];
+    for (int m = 0; m < 6; ++m) {
+        macros4(k, gc[m] ? oth[m] : nb[m], m4[m]);
+        d[m] = spacing0;
+    }
+    acpo_strain_psi_dist(k, m4, d, S, psi);
+}
+
+/* Node.cpp:2244-2380 (HalfCoarsePartners, t + 1/2): grid-coupling neighbours
+ * through their fine partner as above (its state at t + 1/2), the others as
+ * the mean of their pre-collision (t) and current (t + 1) states. */
+void acpo_strain_half_coarse_partner(const acpo_consts* k, double spacing0, const int gc[6], const double pre[6][19],
+                                     const double nb[6][19], const double oth[6][19], double S[6], double psi[19]) {
+    double m4[6][4], d[6];
+    for (int m = 0; m < 6; ++m) {
+        if (gc[m]) macros4(k, oth[m], m4[m]);
+        else macros_mid(k, pre[m], nb[m], m4[m]);
+        d[m] = spacing0;
+    }
+    acpo_strain_psi_dist(k, m4, d, S, psi);
+}
+
+/* Node.cpp:2383-2490 (FinePartners) on a level-1 node: its six fine
+ * neighbours' current states at the level-1 spacing. */
+void acpo_strain_fine_partner(const acpo_consts* k, double spacing1, const double nb[6][19], double S[6],
+                              double psi[19]) {
+    double m4[6][4], d[6];
+    for (int m = 0; m < 6; ++m) {
+        macros4(k, nb[m], m4[m]);
+        d[m] = spacing1;
+    }
+    acpo_strain_psi_dist(k, m4, d, S, psi);
+}
+
+/* Node.cpp:1892-2111 (DirectCoupling).  Level 0 (spacing = level-0 spacing):
+ * a missing neighbour (missing[m]) is read through the partner's second fine
+ * neighbour along m (oth[m], level-1 array), an existing one from its level-0
+ * array; every distance is the level-0 spacing.  Level 1 (spacing = level-1
+ * spacing): a missing neighbour is the coarse partner's neighbour along m,
+ * the mean of its pre-collision and level-0 states (pre[m], nb[m]); an
+ * existing one is read two fine nodes away (oth[m], level-1 array); every
+ * distance is twice the level-1 spacing, a coarse stencil. */
+void acpo_strain_direct_coupling(const acpo_consts* k, int level, double spacing, const int missing[6],
+                                 const double pre[6][19], const double nb[6][19], const double oth[6][19],
+                                 double S[6], double psi[19]) {
+    double m4[6][4], d[6];
+    for (int m = 0; m < 6; ++m) {
+        if (level == 0) {
+            macros4(k, missing[m] ? oth[m] : nb[m], m4[m]);
+            d[m] = spacing;
+        } else {
+            if (missing[m]) macros_mid(k, pre[m], nb[m], m4[m]);
+            else macros4(k, oth[m], m4[m]);
+            d[m] = 2. * spacing;
+        }
+    }
+    acpo_strain_psi_dist(k, m4, d, S, psi);
 }
 
 /* ------------------------------------------------------------- MRT pieces */

@@ -67,6 +67,19 @@ void acpo_moments(const acpo_consts* k, const double f[19], const double* psi, d
 void acpo_pi1(const acpo_consts* k, const double f[19], const double feq[19], const double* psi, double pi[6]);
 void acpo_offeq(const acpo_consts* k, int order, const double a2[6], const double u[3], double out[19]);
 void acpo_strain_psi(const acpo_consts* k, const double nb_macro[6][4], double S[6], double psi[19]);
+void acpo_strain_psi_dist(const acpo_consts* k, const double nb_macro[6][4], const double dist[6], double S[6],
+                          double psi[19]);
+/* strain-rate partner functions of the two-level step (Node.cpp:1892-2490);
+ * see acp_oracle.c for the arrays each reads */
+void acpo_strain_coarse_partner(const acpo_consts* k, double spacing0, const int gc[6], const double nb[6][19],
+                                const double oth[6][19], double S[6], double psi[19]);
+void acpo_strain_half_coarse_partner(const acpo_consts* k, double spacing0, const int gc[6], const double pre[6][19],
+                                     const double nb[6][19], const double oth[6][19], double S[6], double psi[19]);
+void acpo_strain_fine_partner(const acpo_consts* k, double spacing1, const double nb[6][19], double S[6],
+                              double psi[19]);
+void acpo_strain_direct_coupling(const acpo_consts* k, int level, double spacing, const int missing[6],
+                                 const double pre[6][19], const double nb[6][19], const double oth[6][19],
+                                 double S[6], double psi[19]);
 void acpo_rm_moments(const acpo_consts* k, const double f[19], double m[19]);
 void acpo_rm_feq4(const acpo_consts* k, double rho, const double u[3], double out[19]);
 void acpo_rm_inverse(const acpo_consts* k, const double m[19], double f[19]);

@@ -19,6 +19,9 @@
 //        (Lattice.cpp:237-1394) through the public Node getters.
 //   kat  <bc.txt> <outdir> <nsamples> <seed>
 //        Per-function known-answer vectors from standalone FluidNode objects.
+//   partner <bc.txt> <outdir> <nsamples> <seed>
+//        Known answers of the strain-rate partner functions of the two-level
+//        step (Node.cpp:1892-2490) on standalone node clusters.
 //   vtk  <bc.txt> <nsteps> <outfile-stem> [--ic <file>]
 //        Runs nsteps and writes the reference's own VTK file.
 //
@@ -32,6 +35,7 @@
 #include <cstring>
 #include <cmath>
 #include <new>
+#include <memory>
 #include <chrono>
 #include <iostream>
 #include <fstream>
@@ -365,13 +369,174 @@ static int mode_kat(int argc, char** argv) {
     return 0;
 }
 
+// Known answers of the strain-rate partner functions of the two-level step
+// (Node.cpp:1892-2490) on standalone node clusters (heap nodes, whose unset
+// neighbour pointers the zero-filling operator new leaves NULL): a centre node, its six
+// axis neighbours, their partners on the other level and, for the direct
+// coupling, the second neighbours along each axis.  Every node holds random
+// populations in both arrays and in its pre-collision copy, so that the
+// array each function reads is pinned.  The level-0 flag is true and the
+// level-1 flag false.
+static int mode_partner(int argc, char** argv) {
+    if (argc < 6) die("usage: partner <bc.txt> <outdir> <nsamples> <seed>");
+    Control bc(argv[2]);
+    std::string out = argv[3];
+    int ns = std::atoi(argv[4]);
+    Rng rng{(uint64_t)std::strtoull(argv[5], nullptr, 10)};
+    const double rho0 = bc.getDensity(), cs = bc.getSpeedOfSound();
+    double* w = bc.getWeightFactors();
+    bool alt0 = true, alt1 = false;
+    std::vector<double> i_now, i_other, i_pre, i_fine, i_fine_other, i_mask;
+    std::vector<double> o_full_s, o_full_psi, o_half_s, o_half_psi, o_fine_s, o_fine_psi;
+    std::vector<double> o_dc0_s, o_dc0_psi, o_dc0p_s, o_dc0p_psi, o_dc1_s, o_dc1_psi;
+    auto pops = [&](double* f) {
+        double rn = rho0 * (1.0 + 0.01 * rng.next());
+        double un[3] = {0.05 * cs * rng.next(), 0.05 * cs * rng.next(), 0.05 * cs * rng.next()};
+        FluidNode t(0, 0, 0, 0, 0, 0, 0);
+        t.calcEquilibrium3_GHbasis(f, bc, rn, un);
+        for (int q = 0; q < 19; ++q) f[q] += rho0 * w[q] * 0.01 * rng.next();
+    };
+    auto put = [&](Node& n, bool which, const double* f) {
+        for (int q = 0; q < 19; ++q) { int qq = q; n.setDistribution(which, qq, f[q]); }
+    };
+    auto put_pre = [&](Node& n, const double* f) {
+        for (int q = 0; q < 19; ++q) { int qq = q; double v = f[q]; n.setDistributionPreCol(qq, v); }
+    };
+    auto strain = [&](Node& n, int which, std::vector<double>& s) {
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b)
+                s.push_back(which == 0 ? n.getStrainRateTensorFD(a, b)
+                                       : (which == 1 ? n.getStrainRateTensorFDhalf(a, b) : n.getStrainRateTensorFDfull(a, b)));
+    };
+    auto psi = [&](Node& n, int which, std::vector<double>& s) {
+        for (int q = 0; q < 19; ++q) {
+            int qq = q;
+            s.push_back(which == 0 ? n.getCubicMachCorrection(qq)
+                                   : (which == 1 ? n.getCubicMachCorrectionHalf(qq) : n.getCubicMachCorrectionFull(qq)));
+        }
+    };
+    for (int s = 0; s < ns; ++s) {
+        // per axis neighbour m: its populations in the level-0 array (now), the
+        // other array, its pre-collision copy, and the populations of the node
+        // on the other level it leads to (level-1 array, and the other array)
+        double now[6][19], other[6][19], pre[6][19], fine[6][19], fine_other[6][19];
+        int mask[6];
+        for (int m = 0; m < 6; ++m) {
+            pops(now[m]); pops(other[m]); pops(pre[m]); pops(fine[m]); pops(fine_other[m]);
+            mask[m] = rng.next() > 0.0 ? 1 : 0;
+            for (int q = 0; q < 19; ++q) {
+                i_now.push_back(now[m][q]); i_other.push_back(other[m][q]); i_pre.push_back(pre[m][q]);
+                i_fine.push_back(fine[m][q]); i_fine_other.push_back(fine_other[m][q]);
+            }
+            i_mask.push_back(mask[m]);
+        }
+        // coarse partner: neighbours on level 0, grid-coupling ones (mask) lead
+        // to their fine partner (Node.cpp:2114-2240, 2244-2380)
+        {
+            std::unique_ptr<FluidNode> cu(new FluidNode(0, 0, 0, 5, 5, 5, 0));
+            FluidNode& c = *cu;
+            std::vector<FluidNode> nb, fp;
+            nb.reserve(6); fp.reserve(6);
+            for (int m = 0; m < 6; ++m) { nb.emplace_back(0, 0, 0, 5, 5, 5, 0); fp.emplace_back(0, 0, 0, 10, 10, 10, 1); }
+            for (int m = 0; m < 6; ++m) {
+                put(nb[m], alt0, now[m]); put(nb[m], !alt0, other[m]); put_pre(nb[m], pre[m]);
+                put(fp[m], alt1, fine[m]); put(fp[m], !alt1, fine_other[m]);
+                if (mask[m]) { nb[m].tag_ = "gridcoupling"; Node* pp = &fp[m]; nb[m].setPartner(pp); }
+                int dm = m; Node* np = &nb[m]; c.setNeighbor(dm, np);
+            }
+            c.calcStrainRateTensorFDandCubicMachCorrectionCoarsePartners(alt0, alt1, bc);
+            strain(c, 2, o_full_s); psi(c, 2, o_full_psi);
+            c.calcStrainRateTensorFDandCubicMachCorrectionHalfCoarsePartners(alt0, alt1, bc);
+            strain(c, 1, o_half_s); psi(c, 1, o_half_psi);
+        }
+        // fine partner: a level-1 node and its six fine neighbours (Node.cpp:2383-2490)
+        {
+            std::unique_ptr<FluidNode> cu(new FluidNode(0, 0, 0, 10, 10, 10, 1));
+            FluidNode& c = *cu;
+            std::vector<FluidNode> nb;
+            nb.reserve(6);
+            for (int m = 0; m < 6; ++m) nb.emplace_back(0, 0, 0, 10, 10, 10, 1);
+            for (int m = 0; m < 6; ++m) {
+                put(nb[m], alt1, fine[m]); put(nb[m], !alt1, fine_other[m]);
+                int dm = m; Node* np = &nb[m]; c.setNeighbor(dm, np);
+            }
+            c.calcStrainRateTensorFDandCubicMachCorrectionFinePartners(alt1, bc);
+            strain(c, 0, o_fine_s); psi(c, 0, o_fine_psi);
+        }
+        // direct coupling, level 0: a missing neighbour (mask) is replaced by
+        // the partner's second fine neighbour along the axis (Node.cpp:1929-1953)
+        {
+            std::unique_ptr<FluidNode> cu(new FluidNode(0, 0, 0, 5, 5, 5, 0)), pu(new FluidNode(0, 0, 0, 10, 10, 10, 1));
+            FluidNode &c = *cu, &p = *pu;
+            std::vector<FluidNode> nb, a1, a2;
+            nb.reserve(6); a1.reserve(6); a2.reserve(6);
+            for (int m = 0; m < 6; ++m) {
+                nb.emplace_back(0, 0, 0, 5, 5, 5, 0); a1.emplace_back(0, 0, 0, 10, 10, 10, 1); a2.emplace_back(0, 0, 0, 10, 10, 10, 1);
+            }
+            Node* pp = &p;
+            c.setPartner(pp);
+            for (int m = 0; m < 6; ++m) {
+                int dm = m;
+                put(nb[m], alt0, now[m]); put(nb[m], !alt0, other[m]); put_pre(nb[m], pre[m]);
+                put(a1[m], alt1, other[m]); put(a1[m], !alt1, pre[m]);
+                put(a2[m], alt1, fine[m]); put(a2[m], !alt1, fine_other[m]);
+                Node* x1 = &a1[m]; p.setNeighbor(dm, x1);
+                Node* x2 = &a2[m]; a1[m].setNeighbor(dm, x2);
+                if (!mask[m]) { Node* np = &nb[m]; c.setNeighbor(dm, np); }
+            }
+            c.calcStrainRateTensorFDandCubicMachCorrectionDirectCoupling(alt0, alt1, bc);
+            strain(c, 0, o_dc0_s); psi(c, 0, o_dc0_psi);
+            strain(p, 0, o_dc0p_s); psi(p, 0, o_dc0p_psi);
+        }
+        // direct coupling, level 1: a missing neighbour (mask) is replaced by the
+        // coarse partner's neighbour, averaged over its pre-collision and level-0
+        // states; an existing one by the second fine neighbour (Node.cpp:1976-2033)
+        {
+            std::unique_ptr<FluidNode> cu(new FluidNode(0, 0, 0, 10, 10, 10, 1)), qu(new FluidNode(0, 0, 0, 5, 5, 5, 0));
+            FluidNode &c = *cu, &q = *qu;
+            std::vector<FluidNode> cn, a1, a2;
+            cn.reserve(6); a1.reserve(6); a2.reserve(6);
+            for (int m = 0; m < 6; ++m) {
+                cn.emplace_back(0, 0, 0, 5, 5, 5, 0); a1.emplace_back(0, 0, 0, 10, 10, 10, 1); a2.emplace_back(0, 0, 0, 10, 10, 10, 1);
+            }
+            Node* qp = &q;
+            c.setPartner(qp);
+            for (int m = 0; m < 6; ++m) {
+                int dm = m;
+                put(cn[m], alt0, now[m]); put(cn[m], !alt0, other[m]); put_pre(cn[m], pre[m]);
+                Node* cp = &cn[m]; q.setNeighbor(dm, cp);
+                put(a1[m], alt1, other[m]); put(a1[m], !alt1, pre[m]);
+                put(a2[m], alt1, fine[m]); put(a2[m], !alt1, fine_other[m]);
+                Node* x2 = &a2[m]; a1[m].setNeighbor(dm, x2);
+                if (!mask[m]) { Node* x1 = &a1[m]; c.setNeighbor(dm, x1); }
+            }
+            c.calcStrainRateTensorFDandCubicMachCorrectionDirectCoupling(alt0, alt1, bc);
+            strain(c, 0, o_dc1_s); psi(c, 0, o_dc1_psi);
+        }
+    }
+    write_f64(out + "/in_now.bin", i_now);
+    write_f64(out + "/in_other.bin", i_other);
+    write_f64(out + "/in_pre.bin", i_pre);
+    write_f64(out + "/in_fine.bin", i_fine);
+    write_f64(out + "/in_fine_other.bin", i_fine_other);
+    write_f64(out + "/in_mask.bin", i_mask);
+    write_f64(out + "/full_s.bin", o_full_s); write_f64(out + "/full_psi.bin", o_full_psi);
+    write_f64(out + "/half_s.bin", o_half_s); write_f64(out + "/half_psi.bin", o_half_psi);
+    write_f64(out + "/fine_s.bin", o_fine_s); write_f64(out + "/fine_psi.bin", o_fine_psi);
+    write_f64(out + "/dc0_s.bin", o_dc0_s); write_f64(out + "/dc0_psi.bin", o_dc0_psi);
+    write_f64(out + "/dc0p_s.bin", o_dc0p_s); write_f64(out + "/dc0p_psi.bin", o_dc0p_psi);
+    write_f64(out + "/dc1_s.bin", o_dc1_s); write_f64(out + "/dc1_psi.bin", o_dc1_psi);
+    return 0;
+}
+
 int main(int argc, char** argv) {
-    if (argc < 2) die("usage: ref_driver run|geom|kat|vtk ...");
+    if (argc < 2) die("usage: ref_driver run|geom|kat|partner|vtk ...");
     std::string m = argv[1];
     if (m == "run") return mode_run(argc, argv, false);
     if (m == "vtk") return mode_run(argc, argv, true);
     if (m == "geom") return mode_geom(argc, argv);
     if (m == "kat") return mode_kat(argc, argv);
+    if (m == "partner") return mode_partner(argc, argv);
     die("unknown mode " + m);
     return 2;
 }

@@ -6,6 +6,8 @@ reference's own sources) on small cases and stores its outputs:
 
   geom_<nx>x<ny>x<nz>.npz   connectivity of Lattice::setConnectivity
   kat_o<N>.npz              per-function known-answer vectors
+  kat_partner.npz           known answers of the two-level strain-rate partner
+                            functions (Node.cpp:1892-2490)
   step_<name>.npz           initial populations, final populations and probe
                             series after whole uniform steps
   big_<name>.npz            larger runs: probe series, final density plane and
@@ -156,16 +158,34 @@ def make_vtk_case(name, spec, nsteps, seed, tmp, outdir, nonreflecting):
                         nonreflecting=int(nonreflecting), vtk=np.frombuffer(txt, dtype=np.uint8))
 
 
+def make_partner_kat(tmp, outdir, nsamples=48, seed=77):
+    """Known answers of the strain-rate partner functions of the two-level
+    step (Node.cpp:1892-2490) from the reference driver's `partner` mode."""
+    d = os.path.join(tmp, "partner")
+    os.makedirs(d)
+    spec = dict(nx=8, ny=8, nz=8, op="HRR", order=3, tau=0.57)
+    with open(os.path.join(d, "bc.txt"), "w") as fh:
+        fh.write(bc_text(8, 8, 8, op="HRR", order=3, tau=0.57))
+    run_driver(["partner", "bc.txt", ".", str(nsamples), str(seed)], d)
+    arrs = {n[:-4]: np.fromfile(os.path.join(d, n)) for n in os.listdir(d) if n.endswith(".bin")}
+    np.savez_compressed(os.path.join(outdir, "kat_partner.npz"), spec=spec_json(spec), nsamples=nsamples, **arrs)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=HERE)
     ap.add_argument("--skip-big", action="store_true")
     ap.add_argument("--only-vtk", action="store_true")
+    ap.add_argument("--only-partner", action="store_true")
     a = ap.parse_args()
     if not os.path.exists(DRIVER):
         sys.exit("build the reference driver first: make -C oracle ref")
     out = a.out
     with tempfile.TemporaryDirectory() as tmp:
+        make_partner_kat(tmp, out)
+        print("partner kat", flush=True)
+        if a.only_partner:
+            return
         make_vtk_case("srt_q2d_nonrefl", dict(nx=9, ny=7, nz=3, op="SRT", order=3), 7, 101, tmp, out, True)
         make_vtk_case("hrr_3d_refl", dict(nx=7, ny=6, nz=5, op="HRR", order=3, transverse=True), 5, 102, tmp, out,
                       False)
