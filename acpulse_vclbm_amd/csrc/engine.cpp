@@ -379,6 +379,10 @@ void drop_graphs(acp_case* c) {
 
 void release(acp_case* c) {
     drop_graphs(c);
+    if (!c->arena)
+        for (void* p : c->extra)
+            if (p) (void)hipFree(p);
+    c->extra.clear();
     void* ptrs[] = {c->dk, c->A, c->B, c->psi, c->M, c->dks, c->SA, c->SB, c->SM, c->lodi, c->probe_nodes,
                     c->probe_buf, c->probe_slot, c->partial, c->stage};
     if (!c->arena)
@@ -789,6 +793,8 @@ int step_eager(acp_case* c, int32_t n) {
             if (e == hipSuccess)
                 e = acp::launch_bulk64(c->bulk_op, c->d.order, c->d.cubic != 0, c->periodic, c->dk, c->A, c->B,
                                        c->psi, c->M, c->g, c->stream);
+            for (int x = 0; x < c->nfix && e == hipSuccess; ++x)
+                e = acp::launch_fix_hrr64(c->d.order, c->dk, c->A, c->B, c->g, c->fix_col[x], c->fix_S[x], c->stream);
             for (int p = 0; p < nph && e == hipSuccess; ++p)
                 e = acp::launch_lodi64(c->d.order, c->d.cubic != 0, c->d.transverse != 0, c->dk, c->A, c->B, c->psi,
                                        c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], c->g.L,
@@ -802,6 +808,9 @@ int step_eager(acp_case* c, int32_t n) {
             if (e == hipSuccess)
                 e = acp::launch_bulk_shift(c->precision, c->bulk_op, c->d.order, c->periodic, c->dks, c->SA, c->SB,
                                            c->SM, c->g, c->stream);
+            for (int x = 0; x < c->nfix && e == hipSuccess; ++x)
+                e = acp::launch_fix_hrr_shift(c->precision, c->d.order, c->dks, c->SA, c->SB, c->g, c->fix_col[x],
+                                              c->fix_S[x], c->hk.dt, c->stream);
             for (int p = 0; p < nph && e == hipSuccess; ++p)
                 e = acp::launch_lodi_shift(c->precision, c->d.order, c->d.transverse != 0, c->dks, c->SA, c->SB,
                                            c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p],
@@ -853,6 +862,16 @@ int step_graph_chunk(acp_case* c) {
 
 namespace acp {
 int case_step_launch(acp_case* c, int32_t n) { return step_eager(c, n); }
+
+int case_alloc_extra(acp_case* c, int64_t bytes, void** p) {
+    char* q = nullptr;
+    if (int rc = dalloc(c, &q, (size_t)bytes)) return rc;
+    c->extra.push_back(q);
+    // probes set later go after this buffer in a placed case
+    c->probe_mark = c->arena_used;
+    *p = q;
+    return ACP_OK;
+}
 }  // namespace acp
 
 extern "C" int acp_case_step(acp_case* c, int32_t n) {

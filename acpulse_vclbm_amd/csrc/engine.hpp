@@ -46,6 +46,19 @@ LodiSchedule build_lodi_schedule_x(int nx, int ny, int nz, int faces);
 // for level 0; refined cases for their fine level).
 int init_reference_level(acp_case* c, int level, int i0, const acp_case_desc& d0);
 
+// The nodes of x column `col` collide with a strain rate handed over by the
+// other level of a two-level step (S: float64 planes [6][ny nz], SI units)
+// and push as the bulk kernel does; launched right after the bulk kernel
+// (refine.hip, kshift.hip).
+hipError_t launch_fix_hrr64(int order, const K64* k, const double* A, double* B, const Grid& g, int col,
+                            const double* S, hipStream_t s);
+hipError_t launch_fix_hrr_shift(int prec, int order, const KShift* k, const void* A, void* B, const Grid& g, int col,
+                                const double* S, double dt, hipStream_t s);
+
+// Device memory a two-level case adds to one of its levels (counted in the
+// level's footprint, placed in its region, freed with it).
+int case_alloc_extra(acp_case* c, int64_t bytes, void** p);
+
 // Launch n plain steps of a case on its stream (no graphs, no checks); the
 // current array is c->A (c->SA) afterwards and the one read, c->B (c->SB).
 int case_step_launch(acp_case* c, int32_t n);
@@ -101,5 +114,11 @@ struct acp_case {
     // kernel form): graphs that other objects captured over this case are
     // stale once it differs from the value they recorded
     uint64_t graph_epoch = 0;
+    // x columns whose nodes collide with a strain rate handed over by the other
+    // level of a two-level HRR case (fix_S: float64 planes [6][ny nz], SI)
+    int32_t nfix = 0;
+    int32_t fix_col[2] = {0, 0};
+    const double* fix_S[2] = {nullptr, nullptr};
+    std::vector<void*> extra;  // buffers of case_alloc_extra (freed with the case)
     int64_t device_bytes = 0;
 };

@@ -175,6 +175,91 @@ __device__ __forceinline__ void collide_shift(const KShift& k, float g[19], cons
     });
 }
 
+// HRR with a given strain rate S (lattice units of the level: 1 / time
+// step): the collision of a grid-coupling node of a two-level step, which
+// keeps the strain rate handed to it (Node.cpp:1795-1798); otherwise the HRR
+// branch of collide_shift.
+template <int ORDER>
+__device__ __forceinline__ void collide_shift_hrr_strain(const KShift& k, float g[19], const float S[6]) {
+    float d[19];
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        d[q] = g[q] * k.inv_s;
+    });
+    float dr, ux, uy, uz;
+    moments_dev(d, dr, ux, uy, uz);
+    float eq[19], fneq[19], a2[6], f1[19];
+    feq_dev<ORDER>(dr, ux, uy, uz, eq, k.k6);
+    const float om1 = 1.f - k.omega;
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        fneq[q] = d[q] - eq[q];
+    });
+    pi1_dev(fneq, a2);
+    const float c = (1.f - k.sigma) * k.hrr_pre * (1.f + dr);
+#pragma unroll
+    for (int a = 0; a < 6; ++a) a2[a] = k.sigma * a2[a] + c * S[a];
+    offeq_dev<ORDER>(a2, ux, uy, uz, f1, k.k6);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        g[q] = (eq[q] + om1 * f1[q]) * k.s;
+    });
+}
+
+// The nodes of x column `col` of a shifted level collide with the strain rate
+// S_SI (float64 planes [6][ny nz], 1/s) times the level's time step and push
+// as the bulk kernel pushes them (y, z wrapped; x wrapped on a level periodic
+// in x, else links leaving the grid dropped), overwriting what the bulk
+// kernel pushed from these nodes.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(64) void fix_hrr_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                    T* __restrict__ B, int nx, int ny, int nz, const PopLayout L,
+                                                    int col, int xwrap, const double* __restrict__ S, double dt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = ny * nz;
+    if (t >= per) return;
+    const int j = t % ny, kz = t / ny;
+    const int64_t l = col + (int64_t)nx * (j + (int64_t)ny * kz);
+    float g[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) g[q] = (float)A[pidx(L, q, l)];
+    float s6[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) s6[v] = (float)(S[v * per + t] * dt);
+    collide_shift_hrr_strain<ORDER>(*kp, g, s6);
+    static_for<0, 18>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        int a = col + EX[q];
+        if (xwrap) a = (a + nx) % nx;
+        else if (a < 0 || a >= nx) return;
+        const int b = (j + EY[q] + ny) % ny, d = (kz + EZ[q] + nz) % nz;
+        B[pidx(L, q, a + (int64_t)nx * (b + (int64_t)ny * d))] = (T)g[q];
+    });
+    B[pidx(L, 18, l)] = (T)g[18];
+}
+
+hipError_t launch_fix_hrr_shift(int prec, int order, const KShift* k, const void* A, void* B, const Grid& g, int col,
+                                const double* S, double dt, hipStream_t s) {
+    const int per = g.ny * g.nz;
+    const dim3 grid((unsigned)((per + 63) / 64)), block(64);
+    const int xwrap = !g.xopen && !g.xlodi;
+#define FIXS(T, O)                                                                                              \
+    hipLaunchKernelGGL((fix_hrr_shift<T, O>), grid, block, 0, s, k, (const T*)A, (T*)B, g.nx, g.ny, g.nz, g.L, \
+                       col, xwrap, S, dt)
+#define FIXT(T)                  \
+    if (order == 2) FIXS(T, 2);  \
+    else if (order == 3) FIXS(T, 3); \
+    else FIXS(T, 4)
+    if (prec == ACP_PREC_F32_SHIFTED) {
+        FIXT(float);
+    } else {
+        FIXT(_Float16);
+    }
+#undef FIXT
+#undef FIXS
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------ bulk collide + push
 template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW, bool NT>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(

@@ -287,19 +287,15 @@ __host__ __device__ __forceinline__ void collide_srt64(const K64& k, double f[19
     });
 }
 
-// RR (Node.cpp:560-650) and HRR (Node.cpp:653-736).  psi is the node's cubic-Mach
-// state, updated in place; nb holds the neighbours' pre-collision macros.
+// The part of RR / HRR after the strain rate (Node.cpp:560-650, 653-736):
+// Pi1, the hybrid mix with the strain rate S (xx, yy, zz, xy, xz, yz), the
+// regularised off-equilibrium and the relaxation, on the pre-collision
+// equilibrium eq and velocity of the node.
 template <int ORDER, bool HYBRID, bool CUBIC>
-__host__ __device__ __forceinline__ void collide_rr64(const K64& k, double f[19], double psi[19],
-                                                       const double nb[6][4]) {
-    double rho, ux, uy, uz, eq[19], a2[6], f1[19];
-    if constexpr (CUBIC) moments64_psi(k, f, psi, rho, ux, uy, uz);
-    else moments64(k, f, rho, ux, uy, uz);
-    feq64<ORDER>(k, rho, ux, uy, uz, eq);
-    double S[6] = {0., 0., 0., 0., 0., 0.};
-    // the reference always evaluates psi for HRR (Node.cpp:709); it only enters the
-    // result with the cubic correction, so it is skipped otherwise
-    if constexpr (HYBRID || CUBIC) strain_psi64<CUBIC>(k, nb, S, psi);
+__host__ __device__ __forceinline__ void rr64_relax(const K64& k, double f[19], const double psi[19],
+                                                    const double S[6], const double eq[19], double ux, double uy,
+                                                    double uz) {
+    double a2[6], f1[19];
     pi1_64<CUBIC>(k, f, eq, psi, a2);
     if constexpr (HYBRID) {
         double dens = 0.;
@@ -324,6 +320,34 @@ __host__ __device__ __forceinline__ void collide_rr64(const K64& k, double f[19]
         if constexpr (CUBIC) f[q] = eq[q] + (1 - om) * f1[q] + k.half_dt * psi[q];
         else f[q] = eq[q] + (1 - om) * f1[q];
     });
+}
+
+// RR (Node.cpp:560-650) and HRR (Node.cpp:653-736).  psi is the node's cubic-Mach
+// state, updated in place; nb holds the neighbours' pre-collision macros.
+template <int ORDER, bool HYBRID, bool CUBIC>
+__host__ __device__ __forceinline__ void collide_rr64(const K64& k, double f[19], double psi[19],
+                                                       const double nb[6][4]) {
+    double rho, ux, uy, uz, eq[19];
+    if constexpr (CUBIC) moments64_psi(k, f, psi, rho, ux, uy, uz);
+    else moments64(k, f, rho, ux, uy, uz);
+    feq64<ORDER>(k, rho, ux, uy, uz, eq);
+    double S[6] = {0., 0., 0., 0., 0., 0.};
+    // the reference always evaluates psi for HRR (Node.cpp:709); it only enters the
+    // result with the cubic correction, so it is skipped otherwise
+    if constexpr (HYBRID || CUBIC) strain_psi64<CUBIC>(k, nb, S, psi);
+    rr64_relax<ORDER, HYBRID, CUBIC>(k, f, psi, S, eq, ux, uy, uz);
+}
+
+// HRR without the cubic-Mach terms with a given strain rate S: interface
+// nodes of a two-level step collide with the strain rate their partners hand
+// them (a grid-coupling node keeps the strain rate it received, Node.cpp:
+// 1795-1798).
+template <int ORDER>
+__host__ __device__ __forceinline__ void collide_hrr64_strain(const K64& k, double f[19], const double S[6]) {
+    double rho, ux, uy, uz, eq[19];
+    moments64(k, f, rho, ux, uy, uz);
+    feq64<ORDER>(k, rho, ux, uy, uz, eq);
+    rr64_relax<ORDER, true, false>(k, f, nullptr, S, eq, ux, uy, uz);
 }
 
 // raw moments (Node.cpp:2693-2762)

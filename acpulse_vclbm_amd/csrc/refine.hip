@@ -39,6 +39,11 @@
 // of the coincident fine node.  When the fine region reaches the coarse
 // level's high-x LODI face, the coarse node on that face keeps its own LODI
 // update (the fine level's LODI face lies under it).
+// HRR hands strain rates across the interface as the reference's partner
+// functions compute them (Node.cpp:1892-2490, called at Lattice.cpp:12142,
+// 12884 and 13211): the fine face nodes collide with their coarse partners'
+// strain rate (at t and at t + 1/2), the coarse grid-coupling column (x0 + 1,
+// x1 - 1) with its fine partners', instead of their own stencils.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -485,7 +490,182 @@ __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const K
     }
 }
 
+// ---- HRR strain-rate transfer between the levels (Lattice.cpp:12142,
+// 12884, 13211; Node.cpp:1892-2490).  A grid-coupling node collides with the
+// strain rate handed to it (Node.cpp:1795-1798): the fine face nodes with
+// their coarse partners' (coincident nodes their partner's, mid nodes the
+// mean of two, centre nodes of four, as the populations' coarse-to-fine), the
+// coarse grid-coupling column (one coarse cell inside the fine region) with
+// its fine partner's.  Strain rates are float64 in SI units, planes of six
+// components over the column's (J, K) or the face's (j, k), in every tier
+// (the shifted tiers' populations are read as SI populations).
+
+// density and velocity (SI) of node l of a level, any storage type
+template <typename T>
+__device__ __forceinline__ void macros_si(const K64& k, const KShift* ks, const T* __restrict__ A, const PopLayout& L,
+                                          int64_t l, double m4[4]) {
+    T raw[19];
+    load_raw19(A, L, l, raw);
+    double f[19];
+    if constexpr (std::is_same_v<T, double>) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) f[q] = raw[q];
+    } else {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = g_to_f(*ks, raw[q], q);
+        });
+    }
+    moments64(k, f, m4[0], m4[1], m4[2], m4[3]);
+}
+
+// The coarse partners of the fine face nodes: coarse column ci0 (face 0) and
+// ci1 (face 1), their grid-coupling neighbour (+x for face 0, -x for face 1)
+// read through its fine partner (column fi0 / fi1 of the fine level, array F).
+// HALF: the other neighbours as the mean of their states at t (Cprev) and
+// t + 1 (Ccur), Node.cpp:2244-2380; else their current states (Ccur),
+// Node.cpp:2114-2240.  Distances are the coarse spacing (k.dist).
+template <typename T, bool HALF>
+__global__ __launch_bounds__(64) void partner_coarse(const K64* __restrict__ kcp, const KShift* __restrict__ kcs,
+                                                     const KShift* __restrict__ kfs, const T* __restrict__ Cprev,
+                                                     const T* __restrict__ Ccur, const PopLayout CL, int cnx, int cny,
+                                                     int cnz, int ci0, int ci1, const T* __restrict__ F,
+                                                     const PopLayout FL, int fnx, int fny, int fi0, int fi1,
+                                                     int nfaces, double* __restrict__ Sout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = cny * cnz;
+    if (t >= nfaces * per) return;
+    const int face = t / per, r = t - face * per;
+    const int J = r % cny, K = r / cny;
+    const int ci = face ? ci1 : ci0, gcm = face ? 2 : 0, fi = face ? fi1 : fi0;
+    const K64& k = *kcp;
+    double nb[6][4];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        if (m == gcm) {
+            const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+            macros_si(k, kfs, F, FL, lf, nb[m]);
+            continue;
+        }
+        const int a = (ci + EX[m] + cnx) % cnx, b = (J + EY[m] + cny) % cny, d = (K + EZ[m] + cnz) % cnz;
+        const int64_t lc = a + (int64_t)cnx * (b + (int64_t)cny * d);
+        if constexpr (HALF) {
+            double p4[4], c4[4];
+            macros_si(k, kcs, Cprev, CL, lc, p4);
+            macros_si(k, kcs, Ccur, CL, lc, c4);
+            nb[m][0] = 0.5 * (p4[0] + c4[0]);
+#pragma unroll
+            for (int v = 1; v < 4; ++v) nb[m][v] = 0.5 * (p4[v] + c4[v]);
+        } else {
+            macros_si(k, kcs, Ccur, CL, lc, nb[m]);
+        }
+    }
+    double S[6];
+    strain_psi64<false>(k, nb, S, nullptr);
+    double* o = Sout + (int64_t)face * 6 * per;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
+}
+
+// The fine partners of the coarse grid-coupling columns: fine node (fi, 2J,
+// 2K) from its six fine neighbours' current states at the fine spacing
+// (Node.cpp:2383-2490).
+template <typename T>
+__global__ __launch_bounds__(64) void partner_fine(const K64* __restrict__ kfp, const KShift* __restrict__ kfs,
+                                                   const T* __restrict__ F, const PopLayout FL, int fnx, int fny,
+                                                   int fnz, int fi0, int fi1, int cny, int cnz, int nfaces,
+                                                   double* __restrict__ Sout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = cny * cnz;
+    if (t >= nfaces * per) return;
+    const int face = t / per, r = t - face * per;
+    const int J = r % cny, K = r / cny;
+    const int fi = face ? fi1 : fi0;
+    const K64& k = *kfp;
+    double nb[6][4];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int a = fi + EX[m], b = (2 * J + EY[m] + fny) % fny, d = (2 * K + EZ[m] + fnz) % fnz;
+        macros_si(k, kfs, F, FL, a + (int64_t)fnx * (b + (int64_t)fny * d), nb[m]);
+    }
+    double S[6];
+    strain_psi64<false>(k, nb, S, nullptr);
+    double* o = Sout + (int64_t)face * 6 * per;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
+}
+
+// The fine face nodes' strain rates from their coarse partners' (coarse
+// column planes Scol, per face [6][cny cnz]) into per-face [6][fny fnz].
+__global__ __launch_bounds__(256) void face_strain(const double* __restrict__ Scol, int cny, int cnz, int fny, int fnz,
+                                                   int nfaces, double* __restrict__ Sface) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = fny * fnz, cper = cny * cnz;
+    if (t >= nfaces * per) return;
+    const int face = t / per, r = t - face * per;
+    const int j = r % fny, kk = r / fny;
+    const int J = j / 2, K = kk / 2, J1 = (J + 1) % cny, K1 = (K + 1) % cnz;
+    const double* S = Scol + (int64_t)face * 6 * cper;
+    double* o = Sface + (int64_t)face * 6 * per;
+    const int a = J + cny * K, b = J1 + cny * K, c = J + cny * K1, d = J1 + cny * K1;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) {
+        const double* P = S + v * cper;
+        double x;
+        if (!(j & 1) && !(kk & 1)) x = P[a];
+        else if ((j & 1) && (kk & 1)) x = ((P[a] + P[b]) + (P[c] + P[d])) * 0.25;
+        else if (j & 1) x = (P[a] + P[b]) * 0.5;
+        else x = (P[a] + P[c]) * 0.5;
+        o[v * per + r] = x;
+    }
+}
+
 }  // namespace refine
+
+// The nodes of x column `col` of a float64 level collide with the strain rate
+// S (planes [6][ny nz] over (j, k)) and push as the bulk kernel pushes them:
+// all 18 links, y and z wrapped, x wrapped when the level is periodic in x and
+// otherwise dropped when they leave the grid.  Launched right after the bulk
+// kernel, it overwrites what that kernel pushed from these nodes.
+template <int ORDER>
+__global__ __launch_bounds__(64) void fix_hrr64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                                double* __restrict__ B, int nx, int ny, int nz, const PopLayout L,
+                                                int col, int xwrap, const double* __restrict__ S) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = ny * nz;
+    if (t >= per) return;
+    const int j = t % ny, kz = t / ny;
+    const int64_t l = col + (int64_t)nx * (j + (int64_t)ny * kz);
+    double f[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = A[pidx(L, q, l)];
+    double s6[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) s6[v] = S[v * per + t];
+    collide_hrr64_strain<ORDER>(*kp, f, s6);
+    static_for<0, 18>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        int a = col + EX[q];
+        if (xwrap) a = (a + nx) % nx;
+        else if (a < 0 || a >= nx) return;
+        const int b = (j + EY[q] + ny) % ny, d = (kz + EZ[q] + nz) % nz;
+        B[pidx(L, q, a + (int64_t)nx * (b + (int64_t)ny * d))] = f[q];
+    });
+    B[pidx(L, 18, l)] = f[18];
+}
+
+hipError_t launch_fix_hrr64(int order, const K64* k, const double* A, double* B, const Grid& g, int col,
+                            const double* S, hipStream_t s) {
+    const int per = g.ny * g.nz;
+    const dim3 grid((unsigned)((per + 63) / 64)), block(64);
+    const int xwrap = !g.xopen && !g.xlodi;
+#define FIX(O) hipLaunchKernelGGL((fix_hrr64<O>), grid, block, 0, s, k, A, B, g.nx, g.ny, g.nz, g.L, col, xwrap, S)
+    if (order == 2) FIX(2);
+    else if (order == 3) FIX(3);
+    else FIX(4);
+#undef FIX
+    return hipGetLastError();
+}
 
 }  // namespace acp
 
@@ -501,6 +681,13 @@ struct acp_refined {
     int32_t graph_chunk = 0;              // coarse steps per captured graph (0: plain launches)
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // by coarse + 2 * fine parity
     uint64_t epoch[2] = {0, 0};  // the levels' graph epochs the captured graphs hold
+    // HRR strain-rate transfer (per coupled face, float64 planes of six
+    // components): s_col at the coarse partners of the fine face nodes
+    // [nf][6][cny cnz], s_face at the fine face nodes [nf][6][fny fnz], s_gc
+    // at the coarse grid-coupling columns [nf][6][cny cnz]; null without HRR
+    double* s_col = nullptr;
+    double* s_face = nullptr;
+    double* s_gc = nullptr;
 };
 
 namespace {
@@ -618,6 +805,59 @@ hipError_t launch_f2c(acp_refined* r) {
     return hipGetLastError();
 }
 
+// The strain rates the interface columns collide with.  half = 0 at a coarse
+// step's start (states at t), half = 1 after the coarse step and the first
+// fine step (coarse at t and t + 1, fine at t + 1/2): the coarse partners'
+// strain rates and their interpolation onto the fine faces; at the start also
+// the fine partners' for the coarse grid-coupling columns.
+template <typename T>
+hipError_t launch_transfer_t(acp_refined* r, int half) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const bool shifted = c->precision != ACP_PREC_F64;
+    const T* Ccur = (const T*)(shifted ? c->SA : (const void*)c->A);
+    const T* Cprev = (const T*)(shifted ? c->SB : (const void*)c->B);
+    const T* F = (const T*)(shifted ? f->SA : (const void*)f->A);
+    const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz, fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
+    const int nf = coupled_faces(r);
+    const dim3 b64(64), gc((unsigned)((nf * cny * cnz + 63) / 64));
+    if (half)
+        hipLaunchKernelGGL((acp::refine::partner_coarse<T, true>), gc, b64, 0, r->stream, c->dk, c->dks, f->dks, Cprev,
+                           Ccur, c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, 2, fnx - 3, nf,
+                           r->s_col);
+    else
+        hipLaunchKernelGGL((acp::refine::partner_coarse<T, false>), gc, b64, 0, r->stream, c->dk, c->dks, f->dks, Ccur,
+                           Ccur, c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, 2, fnx - 3, nf,
+                           r->s_col);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(acp::refine::face_strain, dim3((unsigned)((nf * fny * fnz + 255) / 256)), dim3(256), 0, r->stream,
+                       (const double*)r->s_col, cny, cnz, fny, fnz, nf, r->s_face);
+    e = hipGetLastError();
+    if (e != hipSuccess || half) return e;
+    hipLaunchKernelGGL((acp::refine::partner_fine<T>), gc, b64, 0, r->stream, f->dk, f->dks, F, f->g.L, fnx, fny, fnz, 2,
+                       fnx - 3, cny, cnz, nf, r->s_gc);
+    return hipGetLastError();
+}
+
+hipError_t launch_transfer(acp_refined* r, int half) {
+    if (!r->s_col) return hipSuccess;
+    switch (r->coarse->precision) {
+        case ACP_PREC_F64: return launch_transfer_t<double>(r, half);
+        case ACP_PREC_F32_SHIFTED: return launch_transfer_t<float>(r, half);
+        default: return launch_transfer_t<_Float16>(r, half);
+    }
+}
+
+// Bytes of the strain-rate transfer buffers of an HRR refined case (every
+// buffer a multiple of 256 bytes; none for other operators).
+int64_t transfer_bytes(const acp_case_desc& d, const acp_case_desc& fd) {
+    if (d.op != ACP_OP_HRR) return 0;
+    const int64_t nf = fd.boundary == ACP_BC_OPEN_LODI_X ? 1 : 2;
+    const int64_t col = acp::round_alloc(nf * 6 * d.ny * d.nz * 8), face = acp::round_alloc(nf * 6 * fd.ny * fd.nz * 8);
+    return 2 * col + face;
+}
+
 }  // namespace
 
 namespace {
@@ -640,6 +880,9 @@ int fine_desc(const acp_case_desc& d, int32_t x0, int32_t x1, acp_case_desc& fd)
     fd.domain[1] = (fd.ny - 0.5) * fd.spacing;
     fd.domain[2] = (fd.nz - 0.5) * fd.spacing;
     fd.boundary = (d.boundary == ACP_BC_LODI_X && x1 == d.nx - 1) ? ACP_BC_OPEN_LODI_X : ACP_BC_OPEN_X;
+    if (d.op == ACP_OP_HRR && fd.boundary == ACP_BC_OPEN_X && x1 - x0 < 3)
+        return fail(ACP_E_ARG, "HRR refined cases with two coupled faces need x1 - x0 >= 3 (one grid-coupling column "
+                               "inside each face)");
     return ACP_OK;
 }
 
@@ -668,6 +911,7 @@ int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* st
         if (!stream) return fail(ACP_E_ARG, "acp_refined_create_in: a placed refined case needs the caller's stream");
         if (int rc = acp_case_footprint(&d, 0, 0, &cb)) return rc;
         if (int rc = acp_case_footprint(&fd, 0, 0, &fb)) return rc;
+        cb += transfer_bytes(d, fd);
         if (fb % 256 != 0 || cb + fb > bytes)
             return fail(ACP_E_ARG, "acp_refined_create_in: region smaller than acp_refined_footprint");
     }
@@ -725,6 +969,35 @@ int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* st
             return acp::hip_fail(e, "fine-level constants");
         }
     }
+    // HRR: the strain-rate transfer buffers, on the coarse level (counted in
+    // its footprint), and the interface columns that collide with them: the
+    // coarse grid-coupling columns x0 + 1 and x1 - 1, the fine face columns
+    if (d.op == ACP_OP_HRR) {
+        const int nf = coupled_faces(r);
+        const int64_t cp = (int64_t)d.ny * d.nz, fp = (int64_t)fd.ny * fd.nz;
+        void *pc = nullptr, *pg = nullptr, *pf = nullptr;
+        int arc = acp::case_alloc_extra(r->coarse, nf * 6 * cp * 8, &pc);
+        if (arc == ACP_OK) arc = acp::case_alloc_extra(r->coarse, nf * 6 * cp * 8, &pg);
+        if (arc == ACP_OK) arc = acp::case_alloc_extra(r->coarse, nf * 6 * fp * 8, &pf);
+        if (arc != ACP_OK) {
+            acp_case_destroy(r->coarse);
+            acp_case_destroy(r->fine);
+            if (r->own_stream) (void)hipStreamDestroy(r->stream);
+            delete r;
+            return arc;
+        }
+        r->s_col = (double*)pc;
+        r->s_gc = (double*)pg;
+        r->s_face = (double*)pf;
+        r->coarse->nfix = nf;
+        r->fine->nfix = nf;
+        for (int s = 0; s < nf; ++s) {
+            r->coarse->fix_col[s] = s == 0 ? x0 + 1 : (x1 - 1) % d.nx;
+            r->coarse->fix_S[s] = r->s_gc + (int64_t)s * 6 * cp;
+            r->fine->fix_col[s] = s == 0 ? 0 : fd.nx - 1;
+            r->fine->fix_S[s] = r->s_face + (int64_t)s * 6 * fp;
+        }
+    }
     // tau = 1 / omega of each level (Control.cpp:54 with each level's time step)
     r->a = (1. / r->fine->hk.omega) / (2. * (1. / r->coarse->hk.omega));
     *out = r;
@@ -734,10 +1007,13 @@ int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* st
 
 // One refined coarse step as plain launches on the case's stream.
 int step_once(acp_refined* r) {
+    hipError_t e = launch_transfer(r, 0);
+    if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_step launch");
     int rc = acp::case_step_launch(r->coarse, 1);
     if (rc == ACP_OK) rc = acp::case_step_launch(r->fine, 1);
     if (rc != ACP_OK) return rc;
-    hipError_t e = launch_c2f(r, 1);
+    e = launch_transfer(r, 1);
+    if (e == hipSuccess) e = launch_c2f(r, 1);
     if (e == hipSuccess) {
         rc = acp::case_step_launch(r->fine, 1);
         if (rc != ACP_OK) return rc;
@@ -805,7 +1081,7 @@ extern "C" int acp_refined_footprint(const acp_case_desc* coarse, int32_t x0, in
     int64_t cb = 0, fb = 0;
     if (int rc = acp_case_footprint(coarse, nprobe, probe_capacity, &cb)) return rc;
     if (int rc = acp_case_footprint(&fd, 0, 0, &fb)) return rc;
-    *bytes = fb + cb;
+    *bytes = fb + cb + transfer_bytes(*coarse, fd);
     return ACP_OK;
 }
 

@@ -561,6 +561,24 @@ void acpo_rm_inverse(const acpo_consts* k, const double m[19], double f[19]) {
 
 /* ------------------------------------------------------------- collisions */
 
+/* HRR without the cubic-Mach terms with a given strain rate S (xx, yy, zz, xy,
+ * xz, yz): a grid-coupling node keeps the strain rate its partner hands it
+ * (Node.cpp:1795-1798) and collides with it (Node.cpp:653-736). */
+void acpo_collide_hrr_strain(const acpo_consts* k, const acpo_params* p, double f[19], const double S[6]) {
+    const double om = k->omega;
+    double rho, u[3], eq[19], a2[6], f1[19];
+    acpo_moments(k, f, NULL, &rho, u);
+    acpo_feq(k, p->order, rho, u, eq);
+    acpo_pi1(k, f, eq, NULL, a2);
+    double dens, uu[3];
+    acpo_moments(k, f, NULL, &dens, uu); /* Node.cpp:1444 */
+    (void)uu;
+    const double dyn = k->nu * dens;
+    for (int i = 0; i < 6; ++i) a2[i] = k->sigma * a2[i] + (1. - k->sigma) * k->inv_pre * (2. * dyn * S[i]);
+    acpo_offeq(k, p->order, a2, u, f1);
+    for (int q = 0; q < 19; ++q) f[q] = eq[q] + (1 - om) * f1[q];
+}
+
 void acpo_collide(const acpo_consts* k, const acpo_params* p, double f[19], double* psi, const double nb[6][4]) {
     const int cubic = p->cubic;
     const double h = 0.5 * k->dt;
@@ -648,6 +666,11 @@ struct acpo_case {
     int32_t max_phase;
     int64_t* ph_start;  /* [max_phase + 1] offsets into ph_order */
     int64_t* ph_order;  /* LODI slots sorted by phase, reference order within a phase */
+    /* x columns whose interior nodes collide with a strain rate handed over by
+     * the other level of a two-level step (HRR): fix_S[x] holds S[6][ny*nz] */
+    int nfix;
+    int fix_col[2];
+    const double* fix_S[2];
 };
 
 static inline int64_t lin(const acpo_case* c, int i, int j, int k) {
@@ -1025,9 +1048,19 @@ static void one_step(acpo_case* c) {
                             for (int v = 0; v < 4; ++v) nb[m][v] = c->macro[(size_t)v * N + o];
                         }
                     }
-                    for (int q = 0; q < 19; ++q) ps[q] = c->psi[(size_t)q * N + l];
-                    acpo_collide(&c->k, p, f, ps, (const double(*)[4])nb);
-                    for (int q = 0; q < 19; ++q) c->psi[(size_t)q * N + l] = ps[q];
+                    int fx = -1;
+                    for (int x = 0; x < c->nfix; ++x)
+                        if (ii == c->fix_col[x]) fx = x;
+                    if (fx >= 0) {
+                        const int64_t t = jj + (int64_t)ny * kk, n2 = (int64_t)ny * nz;
+                        const double S[6] = {c->fix_S[fx][t], c->fix_S[fx][n2 + t], c->fix_S[fx][2 * n2 + t],
+                                             c->fix_S[fx][3 * n2 + t], c->fix_S[fx][4 * n2 + t], c->fix_S[fx][5 * n2 + t]};
+                        acpo_collide_hrr_strain(&c->k, p, f, S);
+                    } else {
+                        for (int q = 0; q < 19; ++q) ps[q] = c->psi[(size_t)q * N + l];
+                        acpo_collide(&c->k, p, f, ps, (const double(*)[4])nb);
+                        for (int q = 0; q < 19; ++q) c->psi[(size_t)q * N + l] = ps[q];
+                    }
                 }
                 const uint32_t m = c->mask[l];
                 for (int q = 0; q < 18; ++q) {
@@ -1183,6 +1216,12 @@ struct acpo_refined {
     int x0, x1;
     double a;
     int filter; /* 0 NOFI, 1 LAG */
+    /* HRR: strain rates handed between the levels, per coupled face:
+     * s_col at the coarse partners of the fine face nodes, s_face at the fine
+     * face nodes, s_gc at the coarse grid-coupling column (each [6][plane]) */
+    double* s_col[2];
+    double* s_face[2];
+    double* s_gc[2];
 };
 
 static void rescale19(const acpo_consts* k, int order, double f[19], double a) {
@@ -1200,6 +1239,9 @@ acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
     } else {
         return NULL;
     }
+    /* HRR hands strain rates to a grid-coupling column next to each coupled
+     * fine face; with two coupled faces the two columns must differ */
+    if (coarse->op == ACPO_HRR && !(coarse->periodic == 3 && x1 == coarse->nx - 1) && x1 - x0 < 3) return NULL;
     acpo_params fp = *coarse;
     fp.nx = 2 * (x1 - x0) + 1;
     fp.ny = 2 * coarse->ny;
@@ -1217,11 +1259,34 @@ acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
     r->x0 = x0;
     r->x1 = x1;
     r->a = (1. / r->f->k.omega) / (2. * (1. / r->c->k.omega));
+    if (coarse->op == ACPO_HRR) {
+        const int nf = fp.periodic == 4 ? 1 : 2;
+        const int64_t cplane = (int64_t)coarse->ny * coarse->nz, fplane = (int64_t)fp.ny * fp.nz;
+        r->c->nfix = nf;
+        r->f->nfix = nf;
+        for (int s = 0; s < nf; ++s) {
+            r->s_col[s] = (double*)calloc((size_t)6 * cplane, sizeof(double));
+            r->s_gc[s] = (double*)calloc((size_t)6 * cplane, sizeof(double));
+            r->s_face[s] = (double*)calloc((size_t)6 * fplane, sizeof(double));
+            /* the coarse grid-coupling columns x0 + 1 and x1 - 1 (one coarse cell
+             * inside the fine region, Overlap 1), the fine face columns 0 and
+             * nx_f - 1 */
+            r->c->fix_col[s] = s == 0 ? x0 + 1 : (x1 - 1) % coarse->nx;
+            r->c->fix_S[s] = r->s_gc[s];
+            r->f->fix_col[s] = s == 0 ? 0 : fp.nx - 1;
+            r->f->fix_S[s] = r->s_face[s];
+        }
+    }
     return r;
 }
 
 void acpo_refined_destroy(acpo_refined* r) {
     if (!r) return;
+    for (int s = 0; s < 2; ++s) {
+        free(r->s_col[s]);
+        free(r->s_face[s]);
+        free(r->s_gc[s]);
+    }
     acpo_destroy(r->c);
     acpo_destroy(r->f);
     free(r);
@@ -1331,11 +1396,112 @@ int acpo_refined_set_filter(acpo_refined* r, int filter) {
 
 void acpo_refined_restrict(acpo_refined* r) { refined_f2c(r); }
 
+/* ---- HRR strain-rate transfer between the levels.  The reference hands
+ * strain rates across the interface with the partner functions of
+ * Node.cpp:1892-2490 (acpo_strain_*_partner, pinned against it), called at
+ * Lattice.cpp:12142 (t + 1/2), 12884 (fine partners before f2c) and 13211
+ * (t + 1); a grid-coupling node collides with the strain rate it received
+ * (Node.cpp:1795-1798).  The interpolation of the handed strain rates to the
+ * fine face nodes lives in the absent GridCoupling.cpp; it is restated like the
+ * populations' coarse-to-fine: coincident nodes take their coarse partner's
+ * value, mid nodes the mean of two, centre nodes of four. */
+
+static void gather19(const acpo_case* c, const double* A, int i, int j, int k, double out[19]) {
+    const int64_t l = lin(c, i, j, k);
+    for (int q = 0; q < 19; ++q) out[q] = A[(size_t)q * c->N + l];
+}
+
+/* the coarse partners of the fine face nodes (coarse column under each coupled
+ * face): full (half = 0) from the current states at t + 1 (or t at a step's
+ * start), half = 1 at t + 1/2 (Lattice.cpp:12142); Cprev, Ccur the coarse
+ * arrays at t and t + 1, Fcur the fine array */
+static void refined_partner_coarse(acpo_refined* r, int half, const double* Cprev, const double* Ccur,
+                                   const double* Fcur) {
+    const acpo_case* c = r->c;
+    const acpo_case* f = r->f;
+    const int cnx = c->p.nx, cny = c->p.ny, cnz = c->p.nz, fnx = f->p.nx;
+    const int64_t plane = (int64_t)cny * cnz;
+    for (int s = 0; s < c->nfix; ++s) {
+        const int ci = s == 0 ? r->x0 % cnx : r->x1 % cnx;
+        const int gcm = s == 0 ? 0 : 2;          /* the direction of the grid-coupling neighbour */
+        const int fi = s == 0 ? 2 : fnx - 3;     /* its fine partner's column */
+        int gc[6] = {0, 0, 0, 0, 0, 0};
+        gc[gcm] = 1;
+        for (int K = 0; K < cnz; ++K)
+            for (int J = 0; J < cny; ++J) {
+                double nb[6][19], pre[6][19], oth[6][19], S[6], psi[19];
+                for (int m = 0; m < 6; ++m) {
+                    const int a = (ci + E[m][0] + cnx) % cnx, b = (J + E[m][1] + cny) % cny, d = (K + E[m][2] + cnz) % cnz;
+                    gather19(c, Ccur, a, b, d, nb[m]);
+                    gather19(c, Cprev, a, b, d, pre[m]);
+                    if (m == gcm) gather19(f, Fcur, fi, 2 * J, 2 * K, oth[m]);
+                    else memcpy(oth[m], nb[m], sizeof(oth[m]));
+                }
+                if (half) acpo_strain_half_coarse_partner(&c->k, c->p.spacing / pow(2., 0), gc,
+                                                          (const double(*)[19])pre, (const double(*)[19])nb,
+                                                          (const double(*)[19])oth, S, psi);
+                else acpo_strain_coarse_partner(&c->k, c->p.spacing / pow(2., 0), gc, (const double(*)[19])nb,
+                                                (const double(*)[19])oth, S, psi);
+                for (int v = 0; v < 6; ++v) r->s_col[s][v * plane + J + (int64_t)cny * K] = S[v];
+            }
+    }
+}
+
+/* the fine face nodes' strain rate from their coarse partners' */
+static void refined_face_strain(acpo_refined* r) {
+    const acpo_case* c = r->c;
+    const acpo_case* f = r->f;
+    const int cny = c->p.ny, cnz = c->p.nz, fny = f->p.ny, fnz = f->p.nz;
+    const int64_t cp = (int64_t)cny * cnz, fpl = (int64_t)fny * fnz;
+    for (int s = 0; s < f->nfix; ++s)
+        for (int k = 0; k < fnz; ++k)
+            for (int j = 0; j < fny; ++j) {
+                const int J = j / 2, K = k / 2, J1 = (J + 1) % cny, K1 = (K + 1) % cnz;
+                for (int v = 0; v < 6; ++v) {
+                    const double* S = r->s_col[s] + v * cp;
+                    double x;
+                    if (!(j & 1) && !(k & 1)) x = S[J + (int64_t)cny * K];
+                    else if ((j & 1) && (k & 1))
+                        x = ((S[J + (int64_t)cny * K] + S[J1 + (int64_t)cny * K]) +
+                             (S[J + (int64_t)cny * K1] + S[J1 + (int64_t)cny * K1])) * 0.25;
+                    else if (j & 1) x = (S[J + (int64_t)cny * K] + S[J1 + (int64_t)cny * K]) * 0.5;
+                    else x = (S[J + (int64_t)cny * K] + S[J + (int64_t)cny * K1]) * 0.5;
+                    r->s_face[s][v * fpl + j + (int64_t)fny * k] = x;
+                }
+            }
+}
+
+/* the fine partners of the coarse grid-coupling columns (Lattice.cpp:12884):
+ * fine node (2, 2J, 2K) or (nx_f - 3, 2J, 2K) from its fine neighbours */
+static void refined_partner_fine(acpo_refined* r, const double* Fcur) {
+    const acpo_case* c = r->c;
+    const acpo_case* f = r->f;
+    const int cny = c->p.ny, cnz = c->p.nz, fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
+    const int64_t plane = (int64_t)cny * cnz;
+    for (int s = 0; s < c->nfix; ++s) {
+        const int fi = s == 0 ? 2 : fnx - 3;
+        for (int K = 0; K < cnz; ++K)
+            for (int J = 0; J < cny; ++J) {
+                double nb[6][19], S[6], psi[19];
+                for (int m = 0; m < 6; ++m)
+                    gather19(f, Fcur, fi + E[m][0], (2 * J + E[m][1] + fny) % fny, (2 * K + E[m][2] + fnz) % fnz, nb[m]);
+                acpo_strain_fine_partner(&f->k, f->p.spacing / pow(2., 0), (const double(*)[19])nb, S, psi);
+                for (int v = 0; v < 6; ++v) r->s_gc[s][v * plane + J + (int64_t)cny * K] = S[v];
+            }
+    }
+}
+
 /* n coarse steps: coarse step; fine step; c2f at t+1/2; fine step; c2f at t+1; f2c.
  * Probes (coarse node indices) record sum_q f_q of the coarse level after its
  * own step, before the coupling writes into it. */
 void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out) {
     for (int s = 0; s < n; ++s) {
+        if (r->c->nfix) {
+            /* t: the strain rates the interface columns collide with */
+            refined_partner_coarse(r, 0, r->c->A, r->c->A, r->f->A);
+            refined_face_strain(r);
+            refined_partner_fine(r, r->f->A);
+        }
         one_step(r->c);
         for (int i = 0; i < nprobe; ++i) {
             double v = 0.;
@@ -1343,6 +1509,11 @@ void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, 
             probe_out[(size_t)s * nprobe + i] = v;
         }
         one_step(r->f);
+        if (r->c->nfix) {
+            /* t + 1/2: coarse states at t (the array the coarse step read) and t + 1 */
+            refined_partner_coarse(r, 1, r->c->B, r->c->A, r->f->A);
+            refined_face_strain(r);
+        }
         refined_c2f(r, 1);
         one_step(r->f);
         refined_c2f(r, 0);

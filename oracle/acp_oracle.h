@@ -92,6 +92,10 @@ void acpo_gs_inverse(const acpo_consts* k, const double m[19], double f[19]);
  * node's cubic-Mach state (read and updated; may be NULL when unused). */
 void acpo_collide(const acpo_consts* k, const acpo_params* p, double f[19], double* psi,
                   const double nb_macro[6][4]);
+/* HRR (no cubic terms) with a given strain rate S (xx, yy, zz, xy, xz, yz): the
+ * collision of a grid-coupling node, which keeps the strain rate its partner
+ * hands it (Node.cpp:1795-1798) */
+void acpo_collide_hrr_strain(const acpo_consts* k, const acpo_params* p, double f[19], const double S[6]);
 
 /* ---- whole lattice ---- */
 typedef struct acpo_case acpo_case;

@@ -66,15 +66,17 @@ def test_open_x_drops_exactly_the_links_leaving_in_x():
                 assert rows[l, q] == (i + ex) + 6 * (((j + ey) % 5) + 5 * ((k + ez) % 4))
 
 
-def test_interface_reflection_is_small():
+@pytest.mark.parametrize("op", ["SRT", "HRR"])
+def test_interface_reflection_is_small(op):
     """A pulse crossing the coarse-to-fine interface, seen by a coarse-level probe
     in front of it, against the same pulse on the uniform coarse grid: identical
     physics until the interface's echo can arrive (pulse at x = 16, probe at
     x = 24, interface at x = 32: the echo travels 24 nodes, about 42 steps at
     c_s = 1/sqrt(3)); afterwards the difference is the reflection, a few per
-    cent of the pulse for this ratio-2 interface."""
+    cent of the pulse for this ratio-2 interface.  HRR (sigma 0.5, the strain
+    rate weighs half) couples its strain rates through the partner transfer."""
     nx, ny, x0, x1, steps = 64, 32, 32, 64, 60
-    p = ol.make_params(nx, ny, 3, op="SRT", periodic=True, tau=0.55, nthreads=8)
+    p = ol.make_params(nx, ny, 3, op=op, periodic=True, tau=0.55, nthreads=8, sigma=0.5)
     r = ol.OracleRefined(p, x0, x1)
     amp, cx, cy, b = 1e-3, 16.0, 16.0, 4.0
     r.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
@@ -202,3 +204,22 @@ def test_open_x_hrr_strain_is_second_order_one_sided():
     da = fa[..., 1:5]
     db = fb[..., 1 + shift:5 + shift]
     assert np.max(np.abs(da - db)) <= 1e-12 * np.max(np.abs(db))
+
+
+def test_refined_hrr_with_sigma_one_is_refined_rr():
+    """HRR with sigma = 1 gives the strain rate no weight, so the refined HRR
+    step, whose interface columns collide with the strain rates handed over by
+    the other level, equals the refined RR step bit for bit: the handed-over
+    path changes nothing but the strain rate."""
+    for periodic, x0, x1 in ((1, 6, 14), (3, 6, 19)):
+        out = []
+        for op in ("RR", "HRR"):
+            p = ol.make_params(20, 8, 3, op=op, order=3, periodic=periodic, tau=0.56, sigma=1.0, nthreads=4)
+            r = ol.OracleRefined(p, x0, x1)
+            r.levels[0].set_state(ol.pulse_ic(p, 1e-2, 8.0, 4.0, 2.5))
+            fp = r.levels[1].p
+            r.levels[1].set_state(ol.pulse_ic(fp, 1e-2, 2 * (8.0 - x0), 8.0, 5.0))
+            r.step(7)
+            out.append((r.levels[0].state(), r.levels[1].state()))
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
