@@ -1216,6 +1216,8 @@ struct acpo_refined {
     int x0, x1;
     double a;
     int filter; /* 0 NOFI, 1 LAG */
+    int scheme; /* 0 standard coupling (STD), 1 direct coupling (DC) */
+    int c2f, f2c, hang; /* DC link sets: 0 missing, 1 missing and common (MfP, McP, hMfP), 2 all (hAll) */
     /* HRR: strain rates handed between the levels, per coupled face:
      * s_col at the coarse partners of the fine face nodes, s_face at the fine
      * face nodes, s_gc at the coarse grid-coupling column (each [6][plane]) */
@@ -1258,6 +1260,7 @@ acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
     r->f->k.dom_len[2] = coarse->nz * coarse->spacing;
     r->x0 = x0;
     r->x1 = x1;
+    r->c2f = r->f2c = r->hang = 2; /* STD: every link */
     r->a = (1. / r->f->k.omega) / (2. * (1. / r->c->k.omega));
     if (coarse->op == ACPO_HRR) {
         const int nf = fp.periodic == 4 ? 1 : 2;
@@ -1388,6 +1391,170 @@ static void refined_f2c(acpo_refined* r) {
             }
 }
 
+/* ---- direct coupling (Coupling-Approach DC1 / DC2: Astoul et al. 2021,
+ * doi 10.1016/j.jcp.2021.110667, cited at BoundaryConditions.txt:52-53; no
+ * overlap, Overlap 0).  The reference orchestrates it at Lattice.cpp:
+ * 13599-15050 -- coarse step, fine step, interpolTimeDirectCoupling at t + 1/2
+ * on the fine interface nodes at coarse positions (14018-14058), spatial
+ * interpolation of the others (14131-14347), fine step, directCoupling at
+ * t + 1 (14699-14739), spatial interpolation (14828-15016) -- and declares the
+ * arithmetic (GridCoupling.h:28-31) in the absent GridCoupling.cpp; it is
+ * restated here, PARITY UNPINNED like the rest of the coupling.
+ * The coarse interface column (x0, and x1 when that face is coupled) and the
+ * fine face column coincide.  At a coincident pair each level misses the links
+ * that arrive from the other level's side: the fine node those arriving from
+ * outside the fine region (e_x = n, n = +1 at x0, -1 at x1), the coarse node
+ * those arriving from inside it (e_x = -n); both hold the common links
+ * (e_x = 0).  The pair's state is assembled from the links each level holds
+ * -- the coarse node's e_x = n and common links, the fine node's e_x = -n
+ * links -- and gives the equilibrium feq(rho, u) against which handed links
+ * are rescaled: fine links f_f = feq + a (f_c - feq), coarse links
+ * f_c = feq + (1 / a) (f_f - feq), a = tau_f / (2 tau_c).  Coarse-to-Fine Mf
+ * hands the fine node its missing links, MfP also the common ones;
+ * Fine-to-Coarse Mc hands the coarse node its missing links, McP also the
+ * common ones.  At t + 1/2 only the fine node receives, from the coarse
+ * state interpolated linearly in time.  Fine face nodes between coincident
+ * ones take the mean of their two (mid) or four (centre) coincident
+ * neighbours (Hanging-Node-Reconstruction): hMf for the missing links,
+ * hMfP also for the common ones, hAll for every link. */
+static int dc_link(int set, int q, int n) { return set == 2 || E[q][0] == n || (set == 1 && E[q][0] == 0); }
+
+/* Mass and momentum (rho, rho u_x, rho u_y, rho u_z) of the links of set
+ * `set` arriving from the fine side (e_x = -n; with the common links for set
+ * 1), of v; `inside` selects those links, !inside the others. */
+static void dc_sums(const acpo_consts* k, const double v[19], int set, int n, int inside, double out[4]) {
+    double r = 0., x = 0., y = 0., z = 0.;
+    for (int q = 0; q < 19; ++q) {
+        if (dc_link(set, q, -n) != inside) continue;
+        r += v[q];
+        if (E[q][0]) x += k->xi[q][0] * v[q];
+        if (E[q][1]) y += k->xi[q][1] * v[q];
+        if (E[q][2]) z += k->xi[q][2] * v[q];
+    }
+    out[0] = r, out[1] = x, out[2] = y, out[3] = z;
+}
+
+/* The chord matrix of a face: the inverse of I - b J, J the derivative at
+ * u = 0 of the mass and momentum of feq over the fine-side links with respect
+ * to (rho, rho u); the set is symmetric in y and z, so J couples only rho
+ * with rho u_x.  Entries m[0..3] the (rho, rho u_x) block by rows, m[4], m[5]
+ * the y and z entries. */
+void acpo_dc_matrix(const acpo_consts* k, double a, int set, int n, double m[6]) {
+    const double b = 1. - 1. / a;
+    double W = 0., V = 0., Tx = 0., Ty = 0., Tz = 0.;
+    for (int q = 0; q < 19; ++q) {
+        if (!dc_link(set, q, -n)) continue;
+        W += k->w[q];
+        V += k->w[q] * k->xi[q][0];
+        Tx += k->w[q] * (k->xi[q][0] * k->xi[q][0]);
+        Ty += k->w[q] * (k->xi[q][1] * k->xi[q][1]);
+        Tz += k->w[q] * (k->xi[q][2] * k->xi[q][2]);
+    }
+    const double a00 = 1. - b * W, a01 = -(b * V) / k->cs2, a10 = -(b * V), a11 = 1. - (b * Tx) / k->cs2;
+    const double det = a00 * a11 - a01 * a10;
+    m[0] = a11 / det, m[1] = -a01 / det, m[2] = -a10 / det, m[3] = a00 / det;
+    m[4] = 1. / (1. - (b * Ty) / k->cs2);
+    m[5] = 1. / (1. - (b * Tz) / k->cs2);
+}
+
+/* The pair's density and velocity: those of the coarse node completed with
+ * the fine-side links rescaled to the coarse non-equilibrium,
+ *   rho u^(0,1) = sum_{q not in C} cv_q xi_q^(0,1)
+ *               + sum_{q in C} [feq_q + (1 / a) (fv_q - feq_q)] xi_q^(0,1),
+ * an implicit equation in (rho, rho u) through feq, solved by chord steps
+ * X -= M^-1 (X - b S(X) - R), b = 1 - 1 / a, S the mass and momentum of
+ * feq(X) over C, from the moments of the unscaled mix, until a step falls
+ * below ACPO_DC_TOL relative (at most ACPO_DC_MAX_ITERS steps).  The
+ * nonlinear part of S is O(u^2), so each step gains a factor O(|u| / c_s). */
+void acpo_dc_state(const acpo_consts* k, int order, double a, const double m[6], int set, int n, const double cv[19],
+                   const double fv[19], double e[19]) {
+    const double ainv = 1. / a, b = 1. - ainv;
+    double g[19], X[4], Rc[4], Rf[4], R[4], S[4], u[3];
+    for (int q = 0; q < 19; ++q) g[q] = dc_link(set, q, -n) ? fv[q] : cv[q];
+    dc_sums(k, g, set, n, 1, Rf);
+    dc_sums(k, g, set, n, 0, Rc);
+    for (int i = 0; i < 4; ++i) X[i] = Rc[i] + Rf[i], R[i] = Rc[i] + ainv * Rf[i];
+    for (int it = 0; it < ACPO_DC_MAX_ITERS; ++it) {
+        for (int i = 0; i < 3; ++i) u[i] = X[1 + i] / X[0];
+        acpo_feq(k, order, X[0], u, e);
+        dc_sums(k, e, set, n, 1, S);
+        double F[4];
+        for (int i = 0; i < 4; ++i) F[i] = (X[i] - b * S[i]) - R[i];
+        const double d0 = m[0] * F[0] + m[1] * F[1], d1 = m[2] * F[0] + m[3] * F[1];
+        const double d2 = m[4] * F[2], d3 = m[5] * F[3];
+        X[0] -= d0, X[1] -= d1, X[2] -= d2, X[3] -= d3;
+        /* converged: the step is below four units in the last place of rho
+         * (of rho c for the momentum) */
+        if (fabs(d0) <= ACPO_DC_TOL * fabs(X[0]) && ((fabs(d1) + fabs(d2)) + fabs(d3)) <= ACPO_DC_TOL * (k->c * fabs(X[0])))
+            break;
+    }
+    for (int i = 0; i < 3; ++i) u[i] = X[1 + i] / X[0];
+    acpo_feq(k, order, X[0], u, e);
+}
+
+static void refined_dc(acpo_refined* r, int half) {
+    acpo_case* c = r->c;
+    acpo_case* f = r->f;
+    const int cnx = c->p.nx, cny = c->p.ny, cnz = c->p.nz;
+    const int fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
+    const int64_t CN = c->N, FN = f->N;
+    const int nfaces = f->p.periodic == 4 ? 1 : 2;
+    for (int s = 0; s < nfaces; ++s) {
+        const int ci = s == 0 ? r->x0 % cnx : r->x1 % cnx, fi = s == 0 ? 0 : fnx - 1, n = s == 0 ? 1 : -1;
+        double m[6];
+        acpo_dc_matrix(&c->k, r->a, r->f2c, n, m);
+        for (int K = 0; K < cnz; ++K)
+            for (int J = 0; J < cny; ++J) {
+                const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
+                const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+                double cv[19], fv[19], e[19];
+                for (int q = 0; q < 19; ++q) {
+                    const double now = c->A[(size_t)q * CN + lc];
+                    cv[q] = half ? 0.5 * (c->B[(size_t)q * CN + lc] + now) : now;
+                    fv[q] = f->A[(size_t)q * FN + lf];
+                }
+                acpo_dc_state(&c->k, c->p.order, r->a, m, r->f2c, n, cv, fv, e);
+                for (int q = 0; q < 19; ++q) {
+                    if (dc_link(r->c2f, q, n)) f->A[(size_t)q * FN + lf] = e[q] + r->a * (cv[q] - e[q]);
+                    if (!half && dc_link(r->f2c, q, -n)) c->A[(size_t)q * CN + lc] = e[q] + (1.0 / r->a) * (fv[q] - e[q]);
+                }
+            }
+        for (int k = 0; k < fnz; ++k)
+            for (int j = 0; j < fny; ++j) {
+                if (!(j & 1) && !(k & 1)) continue;
+#define AT(JJ, KK) (fi + (int64_t)fnx * ((((JJ) + fny) % fny) + (int64_t)fny * (((KK) + fnz) % fnz)))
+                const int64_t l = AT(j, k);
+                for (int q = 0; q < 19; ++q) {
+                    if (!dc_link(r->hang, q, n)) continue;
+                    const double* F = f->A + (size_t)q * FN;
+                    if ((j & 1) && (k & 1))
+                        f->A[(size_t)q * FN + l] = ((F[AT(j - 1, k - 1)] + F[AT(j + 1, k - 1)]) +
+                                                    (F[AT(j - 1, k + 1)] + F[AT(j + 1, k + 1)])) * 0.25;
+                    else if (j & 1) f->A[(size_t)q * FN + l] = (F[AT(j - 1, k)] + F[AT(j + 1, k)]) * 0.5;
+                    else f->A[(size_t)q * FN + l] = (F[AT(j, k - 1)] + F[AT(j, k + 1)]) * 0.5;
+                }
+#undef AT
+            }
+    }
+}
+
+int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int hang) {
+    if (scheme == 0) {
+        if (c2f != 2 || f2c != 2 || hang != 2) return -1; /* STD replaces every link */
+    } else if (scheme == 1) {
+        if (c2f < 0 || c2f > 1 || f2c < 0 || f2c > 1 || hang < 0 || hang > 2) return -1;
+        if (c2f == 1 && f2c == 1) return -1;   /* either MfP-Mc or Mf-McP */
+        if (r->c->p.op == ACPO_HRR) return -1; /* the HRR strain rates follow the standard coupling */
+    } else {
+        return -1;
+    }
+    r->scheme = scheme;
+    r->c2f = c2f;
+    r->f2c = f2c;
+    r->hang = hang;
+    return 0;
+}
+
 int acpo_refined_set_filter(acpo_refined* r, int filter) {
     if (filter != 0 && filter != 1) return -1;
     r->filter = filter;
@@ -1514,10 +1681,17 @@ void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, 
             refined_partner_coarse(r, 1, r->c->B, r->c->A, r->f->A);
             refined_face_strain(r);
         }
-        refined_c2f(r, 1);
-        one_step(r->f);
-        refined_c2f(r, 0);
-        refined_f2c(r);
+        if (r->scheme == 1) {
+            refined_dc(r, 1);
+            one_step(r->f);
+            refined_f2c(r);
+            refined_dc(r, 0);
+        } else {
+            refined_c2f(r, 1);
+            one_step(r->f);
+            refined_c2f(r, 0);
+            refined_f2c(r);
+        }
     }
 }
 

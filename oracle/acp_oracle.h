@@ -141,6 +141,19 @@ void acpo_refined_step(acpo_refined* r, int n);
 void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out);
 /* fine-to-coarse filter: 0 NOFI (default), 1 LAG; -1 for any other value */
 int acpo_refined_set_filter(acpo_refined* r, int filter);
+/* coupling scheme: 0 standard (STD, default: every link, c2f = f2c = hang =
+ * 2), 1 direct coupling (DC); link sets 0 missing (Mf, Mc, hMf), 1 missing and
+ * common (MfP, McP, hMfP), 2 all (hAll, hanging nodes only); -1 for an
+ * invalid combination (MfP with McP) or HRR with DC */
+int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int hang);
+/* direct coupling of one coincident pair on the face with outward fine-region
+ * normal -n (n = +1 at x0): the chord matrix of the face and the pair's
+ * equilibrium (coarse populations cv, fine fv) */
+#define ACPO_DC_MAX_ITERS 16
+#define ACPO_DC_TOL (4. * 2.220446049250313e-16)
+void acpo_dc_matrix(const acpo_consts* k, double a, int set, int n, double m[6]);
+void acpo_dc_state(const acpo_consts* k, int order, double a, const double m[6], int set, int n, const double cv[19],
+                   const double fv[19], double e[19]);
 /* the fine-to-coarse restriction alone, on the levels' current A arrays */
 void acpo_refined_restrict(acpo_refined* r);
 #ifdef __cplusplus

@@ -100,6 +100,7 @@ def lib():
         L.acpo_refined_step_probes.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_int, dp]
         L.acpo_refined_set_filter.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.acpo_refined_restrict.argtypes = [ctypes.c_void_p]
+        L.acpo_refined_set_coupling.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
         _lib = L
     return _lib
 
@@ -192,6 +193,10 @@ class _Level(OracleCase):
         pass
 
 
+# link sets of the coupling keys: 0 missing, 1 missing and common, 2 all
+LINKS = {"Mf": 0, "Mc": 0, "hMf": 0, "MfP": 1, "McP": 1, "hMfP": 1, "fAll": 2, "cAll": 2, "hAll": 2}
+
+
 class OracleRefined:
     """Two-level oracle run (coarse p periodic or with LODI x faces, fine over
     coarse x in [x0, x1])."""
@@ -232,6 +237,15 @@ class OracleRefined:
         """Fine-to-coarse filter: "NOFI" or "LAG"."""
         if lib().acpo_refined_set_filter(self.h, {"NOFI": 0, "LAG": 1}[name]) != 0:
             raise ValueError(name)
+
+    def set_coupling(self, scheme="STD", c2f="fAll", f2c="cAll", hanging="hAll"):
+        """Coupling-Approach (STD, or DC1 / DC2 for the direct coupling) with the
+        case file's link sets (Coarse-to-Fine, Fine-to-Coarse,
+        Hanging-Node-Reconstruction)."""
+        sc = {"STD": 0, "DC": 1, "DC1": 1, "DC2": 1}[scheme]
+        rc = lib().acpo_refined_set_coupling(self.h, sc, LINKS[c2f], LINKS[f2c], LINKS[hanging])
+        if rc != 0:
+            raise ValueError((scheme, c2f, f2c, hanging))
 
     def restrict(self):
         """The fine-to-coarse restriction alone, on the levels' current states."""
