@@ -186,6 +186,12 @@ class _LevelCase(Case):
         self.h = ctypes.c_void_p()
 
 
+# Coupling-Approach and the link sets of Coarse-to-Fine, Fine-to-Coarse and
+# Hanging-Node-Reconstruction (BoundaryConditions.txt:52-86)
+COUPLING = {"STD": 0, "DC1": 1, "DC2": 1}
+LINKS = {"Mf": 0, "Mc": 0, "hMf": 0, "MfP": 1, "McP": 1, "hMfP": 1, "fAll": 2, "cAll": 2, "hAll": 2}
+
+
 class Refined:
     """Two-level case (acp_refined_*): coarse level `desc` (PERIODIC or LODI-X
     boundary), fine level over coarse x in [x0, x1] at half the spacing; with a
@@ -228,6 +234,21 @@ class Refined:
         if name not in F2C_FILTER:
             raise AcpError(_capi.ACP_E_UNSUPPORTED, f"fine-to-coarse filter {name!r}: NOFI or LAG")
         check(lib().acp_refined_set_filter(self.h, F2C_FILTER[name]))
+
+    def set_coupling(self, scheme="STD", c2f=None, f2c=None, hanging=None):
+        """Coupling-Approach "STD" (default) or "DC1" / "DC2" (direct coupling),
+        with the case file's link sets: Coarse-to-Fine "Mf" / "MfP" / "fAll",
+        Fine-to-Coarse "Mc" / "McP" / "cAll", Hanging-Node-Reconstruction
+        "hMf" / "hMfP" / "hAll".  Unset link sets take "fAll" / "cAll" /
+        "hAll" for STD and "Mf" / "Mc" / "hMf" for DC."""
+        if scheme not in COUPLING:
+            raise AcpError(_capi.ACP_E_UNSUPPORTED, f"Coupling-Approach {scheme!r}: STD, DC1 or DC2")
+        dc = COUPLING[scheme] == 1
+        names = (c2f or ("Mf" if dc else "fAll"), f2c or ("Mc" if dc else "cAll"), hanging or ("hMf" if dc else "hAll"))
+        for n, allowed in zip(names, (("Mf", "MfP", "fAll"), ("Mc", "McP", "cAll"), ("hMf", "hMfP", "hAll"))):
+            if n not in allowed:
+                raise AcpError(_capi.ACP_E_ARG, f"link set {n!r}: one of {allowed}")
+        check(lib().acp_refined_set_coupling(self.h, COUPLING[scheme], *(LINKS[n] for n in names)))
 
     def restrict(self):
         """The fine-to-coarse restriction alone, on the levels' current states."""

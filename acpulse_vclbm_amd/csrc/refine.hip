@@ -620,6 +620,232 @@ __global__ __launch_bounds__(256) void face_strain(const double* __restrict__ Sc
     }
 }
 
+// ---- direct coupling (Coupling-Approach DC1 / DC2, Astoul et al. 2021, doi
+// 10.1016/j.jcp.2021.110667; called at Lattice.cpp:14018-14058 at t + 1/2 and
+// 14699-14739 at t + 1, declared at GridCoupling.h:28-31, 45, its arithmetic
+// in the absent GridCoupling.cpp).  The coarse interface column and the fine
+// face column coincide and each level misses the links arriving from the
+// other level's side.  For a coincident pair (coarse populations cv, fine fv)
+// the set C of links the coarse node takes from the fine side (e_x = -n, with
+// the common e_x = 0 links under McP) gives the pair's density and momentum
+// X = (rho, rho u) as those of the coarse node completed with the fine-side
+// links rescaled to the coarse non-equilibrium:
+//   X = sum_{q not in C} cv_q (1, xi_q)
+//     + sum_{q in C} [feq_q(X) + (1/a) (fv_q - feq_q(X))] (1, xi_q),
+// solved by chord steps X -= M^-1 (X - b S(X) - R) with b = 1 - 1/a, S the
+// mass and momentum of feq(X) over C and M^-1 the inverse of the equation's
+// derivative at u = 0 (per face, from the host), starting from the moments of
+// the unscaled mix.  The fine node's links of its set then take
+// feq + a (cv - feq), at t + 1 the coarse node's links of its set
+// feq + (1/a) (fv - feq); the other fine face nodes average their coincident
+// neighbours for the hanging-node link set.
+struct DcFace {
+    double m[6];   // chord matrix: (rho, rho u_x) block by rows, then y, z (SI)
+    float mf[6];   // the same on lattice-unit deviations (float32 coupling)
+    uint32_t fine, coarse, hang;  // link sets, bit q for direction q
+};
+struct DcArgs {
+    DcFace face[2];
+    double a, ainv;
+};
+constexpr int kDcMaxIters = 16;          // float64: steps until converged, at most this many
+constexpr double kDcTol = 4. * 2.220446049250313e-16;
+constexpr int kDcItersF32 = 4;           // float32: a fixed count (each gains a factor O(|u| / c_s))
+
+// mass and momentum over the links of `mask` (inside) or the others
+template <typename R>
+__device__ __forceinline__ void dc_sums(R c, const R v[19], uint32_t mask, bool inside, R out[4]) {
+    R r = 0, x = 0, y = 0, z = 0;
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if ((((mask >> q) & 1u) != 0u) == inside) {
+            r += v[q];
+            if constexpr (EX[q] != 0) x += (EX[q] > 0 ? c : -c) * v[q];
+            if constexpr (EY[q] != 0) y += (EY[q] > 0 ? c : -c) * v[q];
+            if constexpr (EZ[q] != 0) z += (EZ[q] > 0 ? c : -c) * v[q];
+        }
+    });
+    out[0] = r, out[1] = x, out[2] = y, out[3] = z;
+}
+
+template <int ORDER>
+__device__ __forceinline__ void dc_state64(const K64& k, double ainv, const double m[6], uint32_t C,
+                                           const double cv[19], const double fv[19], double e[19]) {
+    const double b = 1. - ainv;
+    double g[19], X[4], Rc[4], Rf[4], R[4], S[4];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) g[q] = ((C >> q) & 1u) ? fv[q] : cv[q];
+    dc_sums<double>(k.c, g, C, true, Rf);
+    dc_sums<double>(k.c, g, C, false, Rc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) X[i] = Rc[i] + Rf[i], R[i] = Rc[i] + ainv * Rf[i];
+    for (int it = 0; it < kDcMaxIters; ++it) {
+        feq64<ORDER>(k, X[0], X[1] / X[0], X[2] / X[0], X[3] / X[0], e);
+        dc_sums<double>(k.c, e, C, true, S);
+        double F[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) F[i] = (X[i] - b * S[i]) - R[i];
+        const double d0 = m[0] * F[0] + m[1] * F[1], d1 = m[2] * F[0] + m[3] * F[1];
+        const double d2 = m[4] * F[2], d3 = m[5] * F[3];
+        X[0] -= d0, X[1] -= d1, X[2] -= d2, X[3] -= d3;
+        if (fabs(d0) <= kDcTol * fabs(X[0]) && ((fabs(d1) + fabs(d2)) + fabs(d3)) <= kDcTol * (k.c * fabs(X[0])))
+            break;
+    }
+    feq64<ORDER>(k, X[0], X[1] / X[0], X[2] / X[0], X[3] / X[0], e);
+}
+
+// the same on lattice-unit deviations d = f / rho0 - w in float32 (X = (d rho, rho u))
+template <int ORDER>
+__device__ __forceinline__ void dc_state32(float k6, float ainv, const float m[6], uint32_t C, const float cv[19],
+                                           const float fv[19], float e[19]) {
+    const float b = 1.f - ainv;
+    float g[19], X[4], Rc[4], Rf[4], R[4], S[4];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) g[q] = ((C >> q) & 1u) ? fv[q] : cv[q];
+    dc_sums<float>(1.f, g, C, true, Rf);
+    dc_sums<float>(1.f, g, C, false, Rc);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) X[i] = Rc[i] + Rf[i], R[i] = Rc[i] + ainv * Rf[i];
+    for (int it = 0; it < kDcItersF32; ++it) {
+        const float ir = 1.f / (1.f + X[0]);
+        feq_dev<ORDER>(X[0], X[1] * ir, X[2] * ir, X[3] * ir, e, k6);
+        dc_sums<float>(1.f, e, C, true, S);
+        float F[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) F[i] = (X[i] - b * S[i]) - R[i];
+        const float d0 = m[0] * F[0] + m[1] * F[1], d1 = m[2] * F[0] + m[3] * F[1];
+        X[0] -= d0, X[1] -= d1, X[2] -= m[4] * F[2], X[3] -= m[5] * F[3];
+    }
+    const float ir = 1.f / (1.f + X[0]);
+    feq_dev<ORDER>(X[0], X[1] * ir, X[2] * ir, X[3] * ir, e, k6);
+}
+
+// One thread per coincident pair of the coupled faces.  MODE 0: float64
+// populations; 1: shifted tier, coupling in float64 on SI populations; 2:
+// shifted tier, coupling in float32 on lattice-unit deviations.  C and F are
+// read and written by the same thread at the same node (not restrict).
+template <typename T, int ORDER, int MODE>
+__global__ __launch_bounds__(64) void dc_coincident(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
+                                                    const T* __restrict__ Cprev, T* C, int half, const DcArgs args,
+                                                    int cnx, int cny, int cnz, int ci0, int ci1, T* F, int fnx,
+                                                    int fny, const PopLayout CN, const PopLayout FN, int nfaces) {
+    __shared__ K64 sk;
+    const K64& k = MODE < 2 ? stage_k64(kp, sk) : *kp;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = cny * cnz;
+    if (t >= nfaces * per) return;
+    const int face = t / per, r = t - face * per;
+    const int J = r % cny, K = r / cny;
+    const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0;
+    const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
+    const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    const uint32_t fm = face ? args.face[1].fine : args.face[0].fine;
+    const uint32_t cm = face ? args.face[1].coarse : args.face[0].coarse;
+    T cur[19], prev[19], fr[19];
+    load_raw19<T>(C, CN, lc, cur);
+    if (half) load_raw19<T>(Cprev, CN, lc, prev);
+    load_raw19<T>(F, FN, lf, fr);
+    if constexpr (MODE < 2) {
+        double m[6], cv[19], fv[19], e[19];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) m[i] = face ? args.face[1].m[i] : args.face[0].m[i];
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (MODE == 0) {
+                cv[q] = half ? 0.5 * ((double)prev[q] + (double)cur[q]) : (double)cur[q];
+                fv[q] = (double)fr[q];
+            } else {
+                const double c1 = g_to_f(*ksp, cur[q], q);
+                cv[q] = half ? 0.5 * (g_to_f(*ksp, prev[q], q) + c1) : c1;
+                fv[q] = g_to_f(*ksp, fr[q], q);
+            }
+        });
+        dc_state64<ORDER>(k, args.ainv, m, cm, cv, fv, e);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if ((fm >> q) & 1u) {
+                const double v = e[q] + args.a * (cv[q] - e[q]);
+                if constexpr (MODE == 0) F[pidx(FN, q, lf)] = (T)v;
+                else F[pidx(FN, q, lf)] = f_to_g<T>(*ksp, v, q);
+            }
+            if (!half && ((cm >> q) & 1u)) {
+                const double v = e[q] + args.ainv * (fv[q] - e[q]);
+                if constexpr (MODE == 0) C[pidx(CN, q, lc)] = (T)v;
+                else C[pidx(CN, q, lc)] = f_to_g<T>(*ksp, v, q);
+            }
+        });
+    } else {
+        const KShift& ks = *ksp;
+        float m[6], cv[19], fv[19], e[19];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) m[i] = face ? args.face[1].mf[i] : args.face[0].mf[i];
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            const float c1 = (float)cur[q] * ks.inv_s;
+            cv[q] = half ? 0.5f * ((float)prev[q] * ks.inv_s + c1) : c1;
+            fv[q] = (float)fr[q] * ks.inv_s;
+        });
+        const float af = (float)args.a, aif = (float)args.ainv;
+        dc_state32<ORDER>(ks.k6, aif, m, cm, cv, fv, e);
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if ((fm >> q) & 1u) F[pidx(FN, q, lf)] = (T)((e[q] + af * (cv[q] - e[q])) * ks.s);
+            if (!half && ((cm >> q) & 1u)) C[pidx(CN, q, lc)] = (T)((e[q] + aif * (fv[q] - e[q])) * ks.s);
+        });
+    }
+}
+
+// The other fine face nodes: the mean of their 2 (mid nodes) or 4 (centre
+// nodes) coincident neighbours for the links of the hanging-node set, formed
+// as c2f_between / c2f_between_s form it.  Reads only coincident nodes, writes
+// only the others.
+template <typename T, int MODE>
+__global__ __launch_bounds__(64) void dc_between(const KShift* __restrict__ ksp, const T* __restrict__ Fc,
+                                                 T* __restrict__ F, int fnx, int fny, int fnz, const PopLayout FN,
+                                                 int nfaces, uint32_t hang0, uint32_t hang1) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = fny * fnz;
+    if (t >= nfaces * per) return;
+    const int face = t / per, r = t - face * per;
+    const int j = r % fny, k = r / fny;
+    if (!(j & 1) && !(k & 1)) return;
+    const int fi = face ? fnx - 1 : 0;
+    const uint32_t hm = face ? hang1 : hang0;
+    auto at = [&](int jj, int kk) {
+        jj = (jj + fny) % fny;
+        kk = (kk + fnz) % fnz;
+        return fi + (int64_t)fnx * (jj + (int64_t)fny * kk);
+    };
+    const int64_t l = at(j, k);
+    const bool centre = (j & 1) && (k & 1);
+    int64_t nb[4];
+    if (centre) {
+        nb[0] = at(j - 1, k - 1), nb[1] = at(j + 1, k - 1), nb[2] = at(j - 1, k + 1), nb[3] = at(j + 1, k + 1);
+    } else {
+        nb[0] = (j & 1) ? at(j - 1, k) : at(j, k - 1);
+        nb[1] = (j & 1) ? at(j + 1, k) : at(j, k + 1);
+        nb[2] = nb[0], nb[3] = nb[1];
+    }
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if (!((hm >> q) & 1u)) return;
+        const T v0 = Fc[pidx(FN, q, nb[0])], v1 = Fc[pidx(FN, q, nb[1])];
+        const T v2 = Fc[pidx(FN, q, nb[2])], v3 = Fc[pidx(FN, q, nb[3])];
+        if constexpr (MODE == 0) {
+            F[pidx(FN, q, l)] = centre ? ((v0 + v1) + (v2 + v3)) * 0.25 : (v0 + v1) * 0.5;
+        } else if constexpr (MODE == 1) {
+            const KShift& ks = *ksp;
+            const double v = centre ? ((g_to_f(ks, v0, q) + g_to_f(ks, v1, q)) + (g_to_f(ks, v2, q) + g_to_f(ks, v3, q))) * 0.25
+                                    : (g_to_f(ks, v0, q) + g_to_f(ks, v1, q)) * 0.5;
+            F[pidx(FN, q, l)] = f_to_g<T>(ks, v, q);
+        } else {
+            const float v = centre ? (((float)v0 + (float)v1) + ((float)v2 + (float)v3)) * 0.25f
+                                   : ((float)v0 + (float)v1) * 0.5f;
+            F[pidx(FN, q, l)] = (T)v;
+        }
+    });
+}
+
 }  // namespace refine
 
 // The nodes of x column `col` of a float64 level collide with the strain rate
@@ -688,6 +914,10 @@ struct acp_refined {
     double* s_col = nullptr;
     double* s_face = nullptr;
     double* s_gc = nullptr;
+    // Coupling-Approach: ACP_COUPLING_STD or ACP_COUPLING_DC with its link
+    // sets (ACP_LINKS_*: Coarse-to-Fine, Fine-to-Coarse, Hanging-Node-Reconstruction)
+    int scheme = ACP_COUPLING_STD;
+    int c2f_set = ACP_LINKS_ALL, f2c_set = ACP_LINKS_ALL, hang_set = ACP_LINKS_ALL;
 };
 
 namespace {
@@ -803,6 +1033,96 @@ hipError_t launch_f2c(acp_refined* r) {
 #undef F2C
 #undef F2CLAG
     return hipGetLastError();
+}
+
+// ---- direct coupling: per-face link sets and chord matrices
+// link q belongs to `set` on the face whose links arriving from outside the
+// fine region have e_x = n
+bool dc_link(int set, int q, int n) {
+    return set == ACP_LINKS_ALL || acp::EX[q] == n || (set == ACP_LINKS_MISSING_COMMON && acp::EX[q] == 0);
+}
+
+// The inverse of I - b J, J the derivative at u = 0 of the mass and momentum
+// of feq over the coarse node's fine-side links with respect to (rho, rho u),
+// for lattice speed c and squared sound speed cs2 (the set is symmetric in y
+// and z, so J couples rho with rho u_x only).
+void dc_matrix(double c, double cs2, double a, int set, int n, double m[6]) {
+    const double b = 1. - 1. / a;
+    double W = 0., V = 0., Tx = 0., Ty = 0., Tz = 0.;
+    for (int q = 0; q < 19; ++q) {
+        if (!dc_link(set, q, -n)) continue;
+        const double w = acp::weight(q);
+        const double x = acp::EX[q] == 0 ? 0. : (acp::EX[q] > 0 ? c : -c);
+        const double y = acp::EY[q] == 0 ? 0. : (acp::EY[q] > 0 ? c : -c);
+        const double z = acp::EZ[q] == 0 ? 0. : (acp::EZ[q] > 0 ? c : -c);
+        W += w;
+        V += w * x;
+        Tx += w * (x * x);
+        Ty += w * (y * y);
+        Tz += w * (z * z);
+    }
+    const double a00 = 1. - b * W, a01 = -(b * V) / cs2, a10 = -(b * V), a11 = 1. - (b * Tx) / cs2;
+    const double det = a00 * a11 - a01 * a10;
+    m[0] = a11 / det, m[1] = -a01 / det, m[2] = -a10 / det, m[3] = a00 / det;
+    m[4] = 1. / (1. - (b * Ty) / cs2);
+    m[5] = 1. / (1. - (b * Tz) / cs2);
+}
+
+acp::refine::DcArgs dc_args(const acp_refined* r) {
+    acp::refine::DcArgs d{};
+    d.a = r->a;
+    d.ainv = 1.0 / r->a;
+    for (int s = 0; s < 2; ++s) {
+        const int n = s == 0 ? 1 : -1;
+        acp::refine::DcFace& f = d.face[s];
+        for (int q = 0; q < 19; ++q) {
+            if (dc_link(r->c2f_set, q, n)) f.fine |= 1u << q;
+            if (dc_link(r->f2c_set, q, -n)) f.coarse |= 1u << q;
+            if (dc_link(r->hang_set, q, n)) f.hang |= 1u << q;
+        }
+        dc_matrix(r->coarse->hk.c, r->coarse->hk.cs2, r->a, r->f2c_set, n, f.m);
+        double ml[6];
+        dc_matrix(1., 1. / 3., r->a, r->f2c_set, n, ml);
+        for (int i = 0; i < 6; ++i) f.mf[i] = (float)ml[i];
+    }
+    return d;
+}
+
+// The direct coupling on the coupled faces: coincident pairs (half: at
+// t + 1/2, only the fine nodes receive), then the hanging fine face nodes.
+template <typename T, int MODE>
+hipError_t launch_dc_t(acp_refined* r, int half) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const bool shifted = c->precision != ACP_PREC_F64;
+    T* Ccur = (T*)(shifted ? c->SA : (void*)c->A);
+    const T* Cprev = (const T*)(shifted ? c->SB : (const void*)c->B);
+    T* F = (T*)(shifted ? f->SA : (void*)f->A);
+    const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz, fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
+    const int nf = coupled_faces(r);
+    const acp::refine::DcArgs args = dc_args(r);
+    const dim3 b(64), g1((unsigned)((nf * cny * cnz + 63) / 64)), g2((unsigned)((nf * fny * fnz + 63) / 64));
+#define DC(O)                                                                                                     \
+    hipLaunchKernelGGL((acp::refine::dc_coincident<T, O, MODE>), g1, b, 0, r->stream, c->dk, c->dks, Cprev, Ccur, half, \
+                       args, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, fnx, fny, c->g.L, f->g.L, nf)
+    if (c->d.order == 2) DC(2);
+    else if (c->d.order == 3) DC(3);
+    else DC(4);
+#undef DC
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((acp::refine::dc_between<T, MODE>), g2, b, 0, r->stream, c->dks, (const T*)F, F, fnx, fny, fnz,
+                       f->g.L, nf, args.face[0].hang, args.face[1].hang);
+    return hipGetLastError();
+}
+
+hipError_t launch_dc(acp_refined* r, int half) {
+    switch (r->coarse->precision) {
+        case ACP_PREC_F64: return launch_dc_t<double, 0>(r, half);
+        case ACP_PREC_F32_SHIFTED:
+            return r->f64_coupling ? launch_dc_t<float, 1>(r, half) : launch_dc_t<float, 2>(r, half);
+        default: return r->f64_coupling ? launch_dc_t<_Float16, 1>(r, half) : launch_dc_t<_Float16, 2>(r, half);
+    }
 }
 
 // The strain rates the interface columns collide with.  half = 0 at a coarse
@@ -1013,6 +1333,21 @@ int step_once(acp_refined* r) {
     if (rc == ACP_OK) rc = acp::case_step_launch(r->fine, 1);
     if (rc != ACP_OK) return rc;
     e = launch_transfer(r, 1);
+    if (r->scheme == ACP_COUPLING_DC) {
+        // direct coupling: the fine nodes at t + 1/2, the second fine step, the
+        // restriction of the coarse nodes inside the fine region, then the pair
+        // exchange at t + 1 (the restriction reads no face node, the exchange
+        // writes only face nodes and the coarse interface columns)
+        if (e == hipSuccess) e = launch_dc(r, 1);
+        if (e == hipSuccess) {
+            rc = acp::case_step_launch(r->fine, 1);
+            if (rc != ACP_OK) return rc;
+            e = launch_f2c(r);
+        }
+        if (e == hipSuccess) e = launch_dc(r, 0);
+        if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_step launch");
+        return ACP_OK;
+    }
     if (e == hipSuccess) e = launch_c2f(r, 1);
     if (e == hipSuccess) {
         rc = acp::case_step_launch(r->fine, 1);
@@ -1132,6 +1467,33 @@ extern "C" int acp_refined_set_filter(acp_refined* r, int32_t filter) {
     if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
     drop_graphs(r);
     r->lag = filter == ACP_F2C_LAG;
+    return ACP_OK;
+}
+
+extern "C" int acp_refined_set_coupling(acp_refined* r, int32_t scheme, int32_t c2f, int32_t f2c, int32_t hanging) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_set_coupling: null argument");
+    auto valid = [](int32_t v) { return v == ACP_LINKS_MISSING || v == ACP_LINKS_MISSING_COMMON || v == ACP_LINKS_ALL; };
+    if (!valid(c2f) || !valid(f2c) || !valid(hanging)) return fail(ACP_E_ARG, "coupling link sets: ACP_LINKS_*");
+    if (scheme == ACP_COUPLING_STD) {
+        if (c2f != ACP_LINKS_ALL || f2c != ACP_LINKS_ALL || hanging != ACP_LINKS_ALL)
+            return fail(ACP_E_UNSUPPORTED, "the standard coupling replaces every link (Coarse-to-Fine fAll, "
+                                           "Fine-to-Coarse cAll, Hanging-Node-Reconstruction hAll)");
+    } else if (scheme == ACP_COUPLING_DC) {
+        if (c2f == ACP_LINKS_ALL || f2c == ACP_LINKS_ALL)
+            return fail(ACP_E_ARG, "direct coupling: Coarse-to-Fine Mf or MfP, Fine-to-Coarse Mc or McP");
+        if (c2f == ACP_LINKS_MISSING_COMMON && f2c == ACP_LINKS_MISSING_COMMON)
+            return fail(ACP_E_ARG, "direct coupling: either MfP-Mc or Mf-McP");
+        if (r->coarse->d.op == ACP_OP_HRR)
+            return fail(ACP_E_UNSUPPORTED, "direct coupling: HRR couples its strain rates with the standard coupling");
+    } else {
+        return fail(ACP_E_ARG, "coupling scheme: ACP_COUPLING_STD or ACP_COUPLING_DC");
+    }
+    if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
+    drop_graphs(r);
+    r->scheme = scheme;
+    r->c2f_set = c2f;
+    r->f2c_set = f2c;
+    r->hang_set = hanging;
     return ACP_OK;
 }
 

@@ -304,6 +304,27 @@ int acp_refined_set_coupling_precision(acp_refined* r, int32_t precision);
  * (Lagrava et al. 2012).  The anisotropic filter TOU is not available. */
 enum { ACP_F2C_NOFI = 0, ACP_F2C_LAG = 1 };
 int acp_refined_set_filter(acp_refined* r, int32_t filter);
+/* Coupling-Approach (BoundaryConditions.txt:52-53, 71-83): the standard
+ * coupling (STD, the default; every link replaced: Coarse-to-Fine fAll,
+ * Fine-to-Coarse cAll, Hanging-Node-Reconstruction hAll) or the direct
+ * coupling of Astoul et al. 2021 (DC1 / DC2; the reference orchestrates it at
+ * Lattice.cpp:13599-15050 with interpolTimeDirectCoupling and directCoupling,
+ * GridCoupling.h:28-31, whose arithmetic it does not contain).  Under DC the
+ * coarse interface columns x0 (and x1) and the fine face columns coincide and
+ * exchange only links: the fine node takes the links arriving from outside
+ * the fine region (ACP_LINKS_MISSING, Mf) or those and the ones in the face
+ * plane (ACP_LINKS_MISSING_COMMON, MfP) from the coarse node, time-
+ * interpolated at t + 1/2; at t + 1 the coarse node takes the links arriving
+ * from inside (Mc) or those and the in-plane ones (McP) from the fine node;
+ * not MfP with McP.  Non-equilibrium parts are rescaled by a = tau_f /
+ * (2 tau_c) about the pair's equilibrium, whose density and velocity are
+ * those of the coarse node completed with the rescaled fine-side links.
+ * Hanging fine face nodes average their coincident neighbours' links of
+ * `hanging` (hMf ACP_LINKS_MISSING, hMfP ACP_LINKS_MISSING_COMMON, hAll
+ * ACP_LINKS_ALL).  Every operator but HRR, every tier. */
+enum { ACP_COUPLING_STD = 0, ACP_COUPLING_DC = 1 };
+enum { ACP_LINKS_MISSING = 0, ACP_LINKS_MISSING_COMMON = 1, ACP_LINKS_ALL = 2 };
+int acp_refined_set_coupling(acp_refined* r, int32_t scheme, int32_t c2f, int32_t f2c, int32_t hanging);
 /* The fine-to-coarse restriction alone, on the levels' current states (the
  * last part of acp_refined_step; for tests and for a caller that edits a
  * level's state). */

@@ -1,0 +1,109 @@
+"""Direct coupling (Coupling-Approach DC1 / DC2) on the GPU: both levels bit for
+bit against the oracle's restatement in float64, the shifted tiers within a
+stated tolerance of it, graph replays equal to plain launches."""
+import numpy as np
+import pytest
+
+import acpulse_vclbm_amd as acp
+import oracle_lib as ol
+from make_golden import CS, DX, RHO0
+
+pytestmark = pytest.mark.gpu
+
+LINKSETS = [("Mf", "Mc", "hMf"), ("MfP", "Mc", "hMfP"), ("Mf", "McP", "hAll")]
+
+
+def _pair(op, order, mrt, links, filt, periodic=True, x0=8, x1=16, nx=24, ny=10, nz=3, cx=6.0, prec="F64"):
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=True if periodic else 3, tau=0.6, mrt_mode=mrt,
+                       cs=CS, spacing=DX, rho0=RHO0, transverse=not periodic)
+    o = ol.OracleRefined(p, x0, x1)
+    o.set_coupling("DC1", *links)
+    o.set_filter(filt)
+    amp, cy, b = 1e-3, 5.0, 2.5
+    o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, tau=0.6, boundary="PERIODIC" if periodic else "LODI-X",
+                      transverse=not periodic, mrt_mode="RELAXED" if mrt else "REFERENCE", precision=prec,
+                      f16_scale=1000.0)
+    r = acp.Refined(d, x0, x1)
+    r.set_coupling("DC1", *links)
+    r.set_filter(filt)
+    r.init_pulse(amp, cx, cy, b)
+    return o, r
+
+
+@pytest.mark.parametrize("links", LINKSETS, ids=["-".join(s) for s in LINKSETS])
+@pytest.mark.parametrize("op,order,mrt,filt", [("SRT", 3, 0, "NOFI"), ("RR", 3, 0, "LAG"), ("RR", 4, 0, "NOFI"),
+                                               ("MRT-RM", 3, 1, "NOFI"), ("MRT-GS", 2, 1, "LAG")])
+def test_direct_coupling_matches_oracle_bitexact(gpu, op, order, mrt, filt, links):
+    print("case", op, order, filt, links, flush=True)
+    o, r = _pair(op, order, mrt, links, filt)
+    with r:
+        for lv in (0, 1):
+            assert np.array_equal(r.level(lv).populations(), o.levels[lv].state())
+        for n in (1, 4, 11):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+
+
+@pytest.mark.parametrize("x1", [17, 23], ids=["inside", "to-lodi-face"])
+def test_direct_coupling_lodi_x_matches_oracle_bitexact(gpu, x1):
+    """LODI x faces on the coarse level; the fine region inside it (two
+    coupled faces) or reaching its high-x face (one)."""
+    print("case lodi-x", x1, flush=True)
+    o, r = _pair("RR", 3, 0, ("Mf", "Mc", "hMf"), "LAG", periodic=False, x1=x1, cx=12.0)
+    with r:
+        for n in (1, 4, 11):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+
+
+@pytest.mark.parametrize("prec,coupling,tol", [("F32-SHIFTED", "F64", 1e-4), ("F32-SHIFTED", "F32-SHIFTED", 1e-3),
+                                               ("F16-SHIFTED", "F64", 2e-2)])
+def test_direct_coupling_shifted_tiers_follow_float64(gpu, prec, coupling, tol):
+    """Coarse density after 30 coarse steps within `tol` of the largest density
+    deviation of the float64 oracle."""
+    print("case", prec, coupling, flush=True)
+    o, r = _pair("RR", 3, 0, ("MfP", "Mc", "hMfP"), "NOFI", prec=prec)
+    o.step(30)
+    r64 = o.levels[0].state().sum(axis=0)
+    with r:
+        r.set_coupling_precision(coupling)
+        r.step(30)
+        rho, _ = r.level(0).macros()
+    assert np.max(np.abs(rho - r64)) <= tol * np.max(np.abs(r64 - RHO0))
+
+
+def test_direct_coupling_graphs_equal_plain_launches(gpu):
+    states = []
+    for chunk in (0, 4):
+        _, r = _pair("RR", 3, 0, ("Mf", "McP", "hMf"), "LAG")
+        with r:
+            r.enable_graphs(chunk)
+            r.step(12)
+            states.append([r.level(lv).populations() for lv in (0, 1)])
+    for lv in (0, 1):
+        assert np.array_equal(states[0][lv], states[1][lv])
+
+
+def test_direct_coupling_refusals(gpu):
+    d = acp.make_desc(24, 10, 3, op="RR", tau=0.6, boundary="PERIODIC")
+    with acp.Refined(d, 8, 16) as r:
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("DC1", "MfP", "McP", "hMf")
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("DC2", "fAll", "Mc", "hMf")
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("STD", "Mf", "cAll", "hAll")
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("DC3")
+        r.set_coupling("DC2", "Mf", "McP", "hMfP")
+        r.set_coupling("STD")
+    h = acp.make_desc(24, 10, 3, op="HRR", tau=0.6, boundary="PERIODIC")
+    with acp.Refined(h, 8, 16) as r:
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("DC1")
