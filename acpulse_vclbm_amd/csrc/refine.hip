@@ -595,6 +595,51 @@ __global__ __launch_bounds__(64) void partner_fine(const K64* __restrict__ kfp, 
     for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
 }
 
+// HRR under the direct coupling at t + 1/2 (Node.cpp:1892-2111 on level 1,
+// called at Lattice.cpp:14064-14108): the fine face nodes at coarse positions,
+// (fi, 2J, 2K), from the fine nodes two fine nodes away along the face and into
+// the fine region (array F, after the exchange at t + 1/2), and for the
+// missing outer neighbour the coarse partner's neighbour (column ci - 1 for
+// face 0, ci + 1 for face 1) with its density and velocity averaged over t
+// (Cprev) and t + 1 (Ccur); every distance twice the fine spacing, the coarse
+// spacing of kcp.  Planes per face [6][cny cnz].
+template <typename T>
+__global__ __launch_bounds__(64) void dc_strain_fine(const K64* __restrict__ kcp, const KShift* __restrict__ kcs,
+                                                     const KShift* __restrict__ kfs, const T* __restrict__ Cprev,
+                                                     const T* __restrict__ Ccur, const PopLayout CL, int cnx, int cny,
+                                                     int cnz, int ci0, int ci1, const T* __restrict__ F,
+                                                     const PopLayout FL, int fnx, int fny, int fnz, int nfaces,
+                                                     double* __restrict__ Sout) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = cny * cnz;
+    if (t >= nfaces * per) return;
+    const int face = t / per, r = t - face * per;
+    const int J = r % cny, K = r / cny;
+    const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0, mm = face ? 0 : 2;
+    const K64& k = *kcp;
+    double nb[6][4];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        if (m == mm) {
+            const int64_t lc = (ci + EX[m] + cnx) % cnx + (int64_t)cnx * (J + (int64_t)cny * K);
+            double p4[4], c4[4];
+            macros_si(k, kcs, Cprev, CL, lc, p4);
+            macros_si(k, kcs, Ccur, CL, lc, c4);
+            nb[m][0] = 0.5 * (p4[0] + c4[0]);
+#pragma unroll
+            for (int v = 1; v < 4; ++v) nb[m][v] = 0.5 * (p4[v] + c4[v]);
+        } else {
+            const int a = fi + 2 * EX[m], b = (2 * J + 2 * EY[m] + fny) % fny, d = (2 * K + 2 * EZ[m] + fnz) % fnz;
+            macros_si(k, kfs, F, FL, a + (int64_t)fnx * (b + (int64_t)fny * d), nb[m]);
+        }
+    }
+    double S[6];
+    strain_psi64<false>(k, nb, S, nullptr);
+    double* o = Sout + (int64_t)face * 6 * per;
+#pragma unroll
+    for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
+}
+
 // The fine face nodes' strain rates from their coarse partners' (coarse
 // column planes Scol, per face [6][cny cnz]) into per-face [6][fny fnz].
 __global__ __launch_bounds__(256) void face_strain(const double* __restrict__ Scol, int cny, int cnz, int fny, int fnz,
@@ -1129,7 +1174,12 @@ hipError_t launch_dc(acp_refined* r, int half) {
 // step's start (states at t), half = 1 after the coarse step and the first
 // fine step (coarse at t and t + 1, fine at t + 1/2): the coarse partners'
 // strain rates and their interpolation onto the fine faces; at the start also
-// the fine partners' for the coarse grid-coupling columns.
+// the fine partners' for the coarse grid-coupling columns (standard coupling;
+// under the direct coupling the coarse interface columns collide with the
+// coarse partners' own, Node.cpp:1892-2111 on level 0 reading the missing
+// inner neighbour through the fine partner's second neighbour as the
+// coarse-partner function does).  Under the direct coupling half = 1 runs
+// after the exchange at t + 1/2 and takes the level-1 stencil.
 template <typename T>
 hipError_t launch_transfer_t(acp_refined* r, int half) {
     acp_case* c = r->coarse;
@@ -1141,6 +1191,18 @@ hipError_t launch_transfer_t(acp_refined* r, int half) {
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz, fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
     const int nf = coupled_faces(r);
     const dim3 b64(64), gc((unsigned)((nf * cny * cnz + 63) / 64));
+    const dim3 gface((unsigned)((nf * fny * fnz + 255) / 256)), b256(256);
+    if (half && r->scheme == ACP_COUPLING_DC) {
+        // the level-1 stencil into the planes s_gc (no grid-coupling column
+        // inside the fine region under the direct coupling), spread over the faces
+        hipLaunchKernelGGL((acp::refine::dc_strain_fine<T>), gc, b64, 0, r->stream, c->dk, c->dks, f->dks, Cprev, Ccur,
+                           c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, fnz, nf, r->s_gc);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(acp::refine::face_strain, gface, b256, 0, r->stream, (const double*)r->s_gc, cny, cnz, fny,
+                           fnz, nf, r->s_face);
+        return hipGetLastError();
+    }
     if (half)
         hipLaunchKernelGGL((acp::refine::partner_coarse<T, true>), gc, b64, 0, r->stream, c->dk, c->dks, f->dks, Cprev,
                            Ccur, c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, 2, fnx - 3, nf,
@@ -1151,10 +1213,12 @@ hipError_t launch_transfer_t(acp_refined* r, int half) {
                            r->s_col);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(acp::refine::face_strain, dim3((unsigned)((nf * fny * fnz + 255) / 256)), dim3(256), 0, r->stream,
-                       (const double*)r->s_col, cny, cnz, fny, fnz, nf, r->s_face);
+    hipLaunchKernelGGL(acp::refine::face_strain, gface, b256, 0, r->stream, (const double*)r->s_col, cny, cnz, fny, fnz,
+                       nf, r->s_face);
     e = hipGetLastError();
-    if (e != hipSuccess || half) return e;
+    // under the direct coupling the coarse interface column collides with the
+    // coarse partners' strain rates themselves (s_col)
+    if (e != hipSuccess || half || r->scheme == ACP_COUPLING_DC) return e;
     hipLaunchKernelGGL((acp::refine::partner_fine<T>), gc, b64, 0, r->stream, f->dk, f->dks, F, f->g.L, fnx, fny, fnz, 2,
                        fnx - 3, cny, cnz, nf, r->s_gc);
     return hipGetLastError();
@@ -1332,13 +1396,14 @@ int step_once(acp_refined* r) {
     int rc = acp::case_step_launch(r->coarse, 1);
     if (rc == ACP_OK) rc = acp::case_step_launch(r->fine, 1);
     if (rc != ACP_OK) return rc;
-    e = launch_transfer(r, 1);
     if (r->scheme == ACP_COUPLING_DC) {
-        // direct coupling: the fine nodes at t + 1/2, the second fine step, the
-        // restriction of the coarse nodes inside the fine region, then the pair
-        // exchange at t + 1 (the restriction reads no face node, the exchange
-        // writes only face nodes and the coarse interface columns)
-        if (e == hipSuccess) e = launch_dc(r, 1);
+        // direct coupling: the fine nodes at t + 1/2 (then the HRR strain
+        // rates of t + 1/2), the second fine step, the restriction of the
+        // coarse nodes inside the fine region, then the pair exchange at t + 1
+        // (the restriction reads no face node, the exchange writes only face
+        // nodes and the coarse interface columns)
+        e = launch_dc(r, 1);
+        if (e == hipSuccess) e = launch_transfer(r, 1);
         if (e == hipSuccess) {
             rc = acp::case_step_launch(r->fine, 1);
             if (rc != ACP_OK) return rc;
@@ -1348,6 +1413,7 @@ int step_once(acp_refined* r) {
         if (e != hipSuccess) return acp::hip_fail(e, "acp_refined_step launch");
         return ACP_OK;
     }
+    e = launch_transfer(r, 1);
     if (e == hipSuccess) e = launch_c2f(r, 1);
     if (e == hipSuccess) {
         rc = acp::case_step_launch(r->fine, 1);
@@ -1483,14 +1549,26 @@ extern "C" int acp_refined_set_coupling(acp_refined* r, int32_t scheme, int32_t 
             return fail(ACP_E_ARG, "direct coupling: Coarse-to-Fine Mf or MfP, Fine-to-Coarse Mc or McP");
         if (c2f == ACP_LINKS_MISSING_COMMON && f2c == ACP_LINKS_MISSING_COMMON)
             return fail(ACP_E_ARG, "direct coupling: either MfP-Mc or Mf-McP");
-        if (r->coarse->d.op == ACP_OP_HRR)
-            return fail(ACP_E_UNSUPPORTED, "direct coupling: HRR couples its strain rates with the standard coupling");
     } else {
         return fail(ACP_E_ARG, "coupling scheme: ACP_COUPLING_STD or ACP_COUPLING_DC");
     }
     if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
     drop_graphs(r);
     r->scheme = scheme;
+    // HRR: the coarse column that collides with handed strain rates is the
+    // grid-coupling column inside the fine region (standard coupling) or the
+    // interface column itself (direct coupling)
+    const int64_t cp = (int64_t)r->coarse->g.ny * r->coarse->g.nz;
+    for (int s = 0; s < r->coarse->nfix; ++s) {
+        const int cnx = r->coarse->g.nx;
+        if (scheme == ACP_COUPLING_STD) {
+            r->coarse->fix_col[s] = s == 0 ? r->x0 + 1 : (r->x1 - 1) % cnx;
+            r->coarse->fix_S[s] = r->s_gc + (int64_t)s * 6 * cp;
+        } else {
+            r->coarse->fix_col[s] = s == 0 ? r->x0 % cnx : r->x1 % cnx;
+            r->coarse->fix_S[s] = r->s_col + (int64_t)s * 6 * cp;
+        }
+    }
     r->c2f_set = c2f;
     r->f2c_set = f2c;
     r->hang_set = hanging;

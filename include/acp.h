@@ -321,7 +321,12 @@ int acp_refined_set_filter(acp_refined* r, int32_t filter);
  * those of the coarse node completed with the rescaled fine-side links.
  * Hanging fine face nodes average their coincident neighbours' links of
  * `hanging` (hMf ACP_LINKS_MISSING, hMfP ACP_LINKS_MISSING_COMMON, hAll
- * ACP_LINKS_ALL).  Every operator but HRR, every tier. */
+ * ACP_LINKS_ALL).  Every operator and tier; HRR hands its strain rates as
+ * the reference's DirectCoupling function computes them (Node.cpp:1892-2111):
+ * the coarse interface column and its fine partners collide with the coarse
+ * stencil that reads the missing inner neighbour through the fine level, the
+ * fine face nodes at t + 1/2 with the coarse-spaced stencil over fine nodes
+ * and the time-averaged coarse neighbour outside. */
 enum { ACP_COUPLING_STD = 0, ACP_COUPLING_DC = 1 };
 enum { ACP_LINKS_MISSING = 0, ACP_LINKS_MISSING_COMMON = 1, ACP_LINKS_ALL = 2 };
 int acp_refined_set_coupling(acp_refined* r, int32_t scheme, int32_t c2f, int32_t f2c, int32_t hanging);

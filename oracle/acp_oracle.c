@@ -1543,12 +1543,24 @@ int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int
         if (c2f != 2 || f2c != 2 || hang != 2) return -1; /* STD replaces every link */
     } else if (scheme == 1) {
         if (c2f < 0 || c2f > 1 || f2c < 0 || f2c > 1 || hang < 0 || hang > 2) return -1;
-        if (c2f == 1 && f2c == 1) return -1;   /* either MfP-Mc or Mf-McP */
-        if (r->c->p.op == ACPO_HRR) return -1; /* the HRR strain rates follow the standard coupling */
+        if (c2f == 1 && f2c == 1) return -1; /* either MfP-Mc or Mf-McP */
     } else {
         return -1;
     }
     r->scheme = scheme;
+    /* HRR: the coarse column that collides with handed strain rates is the
+     * grid-coupling column inside the fine region (STD) or the interface
+     * column itself (DC) */
+    for (int s = 0; s < r->c->nfix; ++s) {
+        const int cnx = r->c->p.nx;
+        if (scheme == 0) {
+            r->c->fix_col[s] = s == 0 ? r->x0 + 1 : (r->x1 - 1) % cnx;
+            r->c->fix_S[s] = r->s_gc[s];
+        } else {
+            r->c->fix_col[s] = s == 0 ? r->x0 % cnx : r->x1 % cnx;
+            r->c->fix_S[s] = r->s_col[s];
+        }
+    }
     r->c2f = c2f;
     r->f2c = f2c;
     r->hang = hang;
@@ -1615,7 +1627,7 @@ static void refined_partner_coarse(acpo_refined* r, int half, const double* Cpre
 }
 
 /* the fine face nodes' strain rate from their coarse partners' */
-static void refined_face_strain(acpo_refined* r) {
+static void refined_face_strain(acpo_refined* r, double* const* src) {
     const acpo_case* c = r->c;
     const acpo_case* f = r->f;
     const int cny = c->p.ny, cnz = c->p.nz, fny = f->p.ny, fnz = f->p.nz;
@@ -1625,7 +1637,7 @@ static void refined_face_strain(acpo_refined* r) {
             for (int j = 0; j < fny; ++j) {
                 const int J = j / 2, K = k / 2, J1 = (J + 1) % cny, K1 = (K + 1) % cnz;
                 for (int v = 0; v < 6; ++v) {
-                    const double* S = r->s_col[s] + v * cp;
+                    const double* S = src[s] + v * cp;
                     double x;
                     if (!(j & 1) && !(k & 1)) x = S[J + (int64_t)cny * K];
                     else if ((j & 1) && (k & 1))
@@ -1658,17 +1670,90 @@ static void refined_partner_fine(acpo_refined* r, const double* Fcur) {
     }
 }
 
-/* n coarse steps: coarse step; fine step; c2f at t+1/2; fine step; c2f at t+1; f2c.
+/* HRR under the direct coupling at t + 1/2 (Lattice.cpp:14064-14108,
+ * Node.cpp:1892-2111 on level 1): the fine face nodes at coarse positions
+ * from the fine nodes two fine nodes away along the face and into the fine
+ * region, and for the missing outer neighbour the coarse partner's neighbour,
+ * its density and velocity averaged over t and t + 1; into the planes s_gc
+ * (the coarse grid-coupling columns have none under the direct coupling). */
+static void refined_dc_strain_fine(acpo_refined* r, const double* Cprev, const double* Ccur, const double* Fcur) {
+    const acpo_case* c = r->c;
+    const acpo_case* f = r->f;
+    const int cnx = c->p.nx, cny = c->p.ny, cnz = c->p.nz, fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
+    const int64_t plane = (int64_t)cny * cnz;
+    for (int s = 0; s < f->nfix; ++s) {
+        const int ci = s == 0 ? r->x0 % cnx : r->x1 % cnx, fi = s == 0 ? 0 : fnx - 1;
+        const int mm = s == 0 ? 2 : 0; /* the missing neighbour: -x at x0, +x at x1 */
+        int missing[6] = {0, 0, 0, 0, 0, 0};
+        missing[mm] = 1;
+        for (int K = 0; K < cnz; ++K)
+            for (int J = 0; J < cny; ++J) {
+                double nb[6][19], pre[6][19], oth[6][19], S[6], psi[19];
+                memset(nb, 0, sizeof(nb));
+                memset(pre, 0, sizeof(pre));
+                memset(oth, 0, sizeof(oth));
+                for (int m = 0; m < 6; ++m) {
+                    if (m == mm) {
+                        const int a = (ci + E[m][0] + cnx) % cnx;
+                        gather19(c, Ccur, a, J, K, nb[m]);
+                        gather19(c, Cprev, a, J, K, pre[m]);
+                    } else {
+                        gather19(f, Fcur, fi + 2 * E[m][0], (2 * J + 2 * E[m][1] + fny) % fny,
+                                 (2 * K + 2 * E[m][2] + fnz) % fnz, oth[m]);
+                    }
+                }
+                acpo_strain_direct_coupling(&f->k, 1, f->p.spacing, missing, (const double(*)[19])pre,
+                                            (const double(*)[19])nb, (const double(*)[19])oth, S, psi);
+                for (int v = 0; v < 6; ++v) r->s_gc[s][v * plane + J + (int64_t)cny * K] = S[v];
+            }
+    }
+}
+
+/* The strain rates the interface columns collide with (HRR): at a coarse
+ * step's start (half = 0) the coarse partners' of the fine face nodes, spread
+ * over the face, and under the standard coupling the fine partners' of the
+ * coarse grid-coupling columns; under the direct coupling the coarse interface
+ * column collides with its own partner strain (Node.cpp:1892-2111 on level 0
+ * reads the missing inner neighbour through the fine partner's second
+ * neighbour, as the coarse-partner function does).  At t + 1/2 (half = 1,
+ * after the coupling at t + 1/2 under the direct coupling, before it under the
+ * standard one) the fine face nodes' from the half-step coarse partners
+ * (standard) or from the level-1 direct-coupling stencil (direct). */
+static void refined_transfer(acpo_refined* r, int half) {
+    if (!r->c->nfix) return;
+    if (!half) {
+        refined_partner_coarse(r, 0, r->c->A, r->c->A, r->f->A);
+        refined_face_strain(r, r->s_col);
+        if (r->scheme == 0) refined_partner_fine(r, r->f->A);
+    } else if (r->scheme == 0) {
+        /* coarse states at t (the array the coarse step read) and t + 1 */
+        refined_partner_coarse(r, 1, r->c->B, r->c->A, r->f->A);
+        refined_face_strain(r, r->s_col);
+    } else {
+        refined_dc_strain_fine(r, r->c->B, r->c->A, r->f->A);
+        refined_face_strain(r, r->s_gc);
+    }
+}
+
+void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_gc, double* s_face) {
+    refined_transfer(r, half);
+    const int nf = r->c->nfix;
+    const int64_t cp = (int64_t)r->c->p.ny * r->c->p.nz, fp = (int64_t)r->f->p.ny * r->f->p.nz;
+    for (int s = 0; s < nf; ++s) {
+        if (s_col) memcpy(s_col + s * 6 * cp, r->s_col[s], sizeof(double) * 6 * cp);
+        if (s_gc) memcpy(s_gc + s * 6 * cp, r->s_gc[s], sizeof(double) * 6 * cp);
+        if (s_face) memcpy(s_face + s * 6 * fp, r->s_face[s], sizeof(double) * 6 * fp);
+    }
+}
+
+/* n coarse steps.  Standard coupling: coarse step; fine step; c2f at t+1/2;
+ * fine step; c2f at t+1; f2c.  Direct coupling: coarse step; fine step; the
+ * pair exchange at t+1/2; fine step; f2c; the pair exchange at t+1.
  * Probes (coarse node indices) record sum_q f_q of the coarse level after its
  * own step, before the coupling writes into it. */
 void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out) {
     for (int s = 0; s < n; ++s) {
-        if (r->c->nfix) {
-            /* t: the strain rates the interface columns collide with */
-            refined_partner_coarse(r, 0, r->c->A, r->c->A, r->f->A);
-            refined_face_strain(r);
-            refined_partner_fine(r, r->f->A);
-        }
+        refined_transfer(r, 0);
         one_step(r->c);
         for (int i = 0; i < nprobe; ++i) {
             double v = 0.;
@@ -1676,17 +1761,14 @@ void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, 
             probe_out[(size_t)s * nprobe + i] = v;
         }
         one_step(r->f);
-        if (r->c->nfix) {
-            /* t + 1/2: coarse states at t (the array the coarse step read) and t + 1 */
-            refined_partner_coarse(r, 1, r->c->B, r->c->A, r->f->A);
-            refined_face_strain(r);
-        }
         if (r->scheme == 1) {
             refined_dc(r, 1);
+            refined_transfer(r, 1);
             one_step(r->f);
             refined_f2c(r);
             refined_dc(r, 0);
         } else {
+            refined_transfer(r, 1);
             refined_c2f(r, 1);
             one_step(r->f);
             refined_c2f(r, 0);

@@ -144,7 +144,7 @@ int acpo_refined_set_filter(acpo_refined* r, int filter);
 /* coupling scheme: 0 standard (STD, default: every link, c2f = f2c = hang =
  * 2), 1 direct coupling (DC); link sets 0 missing (Mf, Mc, hMf), 1 missing and
  * common (MfP, McP, hMfP), 2 all (hAll, hanging nodes only); -1 for an
- * invalid combination (MfP with McP) or HRR with DC */
+ * invalid combination (MfP with McP) */
 int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int hang);
 /* direct coupling of one coincident pair on the face with outward fine-region
  * normal -n (n = +1 at x0): the chord matrix of the face and the pair's
@@ -154,6 +154,11 @@ int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int
 void acpo_dc_matrix(const acpo_consts* k, double a, int set, int n, double m[6]);
 void acpo_dc_state(const acpo_consts* k, int order, double a, const double m[6], int set, int n, const double cv[19],
                    const double fv[19], double e[19]);
+/* HRR: the strain-rate transfer of a step's start (half = 0) or of t + 1/2
+ * (half = 1) alone, on the levels' current arrays (coarse A at t + 1, B at t);
+ * copies the planes out where the pointers are not NULL: s_col and s_gc
+ * [faces][6][cny cnz], s_face [faces][6][fny fnz] */
+void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_gc, double* s_face);
 /* the fine-to-coarse restriction alone, on the levels' current A arrays */
 void acpo_refined_restrict(acpo_refined* r);
 #ifdef __cplusplus

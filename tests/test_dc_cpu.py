@@ -118,7 +118,7 @@ LINKSETS = [("Mf", "Mc", "hMf"), ("MfP", "Mc", "hMfP"), ("Mf", "McP", "hMf"), ("
 
 @pytest.mark.parametrize("links", LINKSETS, ids=["-".join(s) for s in LINKSETS])
 @pytest.mark.parametrize("op,moving", [("SRT", True), ("RR", True), ("RR", False), ("MRT-RM", False),
-                                       ("MRT-GS", False)])
+                                       ("MRT-GS", False), ("HRR", True)])
 def test_uniform_state_stays_uniform(op, moving, links):
     r, p = refined(op)
     r.set_coupling("DC1", *links)
@@ -142,9 +142,6 @@ def test_forbidden_combinations_are_refused():
     with pytest.raises(ValueError):
         r.set_coupling("STD", "Mf", "cAll", "hAll")  # the standard coupling replaces every link
     r.set_coupling("DC2", "Mf", "McP", "hMfP")
-    h, _ = refined("HRR", x1=20)
-    with pytest.raises(ValueError):
-        h.set_coupling("DC1", "Mf", "Mc", "hMf")  # HRR keeps the standard coupling
 
 
 def _pulse(scheme, links=("Mf", "Mc", "hMf"), op="RR"):
@@ -152,7 +149,7 @@ def _pulse(scheme, links=("Mf", "Mc", "hMf"), op="RR"):
     [32, 64]; probe at x = 24.  Returns the refined and the uniform coarse
     probe signals and the coarse level's total mass after each step."""
     nx, ny, x0, x1, steps = 64, 32, 32, 64, 60
-    p = ol.make_params(nx, ny, 3, op=op, periodic=True, tau=0.55, nthreads=8)
+    p = ol.make_params(nx, ny, 3, op=op, periodic=True, tau=0.55, nthreads=8, sigma=0.5)
     r = ol.OracleRefined(p, x0, x1)
     if scheme != "STD":
         r.set_coupling(scheme, *links)
@@ -173,15 +170,16 @@ def _pulse(scheme, links=("Mf", "Mc", "hMf"), op="RR"):
     return np.array(pr), np.array(pu), np.array(mass), excess, p
 
 
-@pytest.mark.parametrize("links", LINKSETS[:3], ids=["-".join(s) for s in LINKSETS[:3]])
-def test_interface_reflection_is_small(links):
+@pytest.mark.parametrize("op,links", [("RR", LINKSETS[0]), ("RR", LINKSETS[1]), ("RR", LINKSETS[2]),
+                                      ("HRR", LINKSETS[0])], ids=["RR-Mf-Mc", "RR-MfP-Mc", "RR-Mf-McP", "HRR-Mf-Mc"])
+def test_interface_reflection_is_small(op, links):
     """As for the standard coupling: identical physics at the probe until the
     interface's echo can arrive (the echo travels 24 nodes, about 42 steps),
     then an echo below half a per cent of the pulse (the standard coupling's
     is 0.27 %); the coarse level's mass (its nodes inside the fine region
     restricted from it) changes by less than 0.25 % of the pulse's excess
-    mass."""
-    pr, pu, mass, excess, p = _pulse("DC1", links)
+    mass.  HRR (sigma 0.5) hands its strain rates across the interface."""
+    pr, pu, mass, excess, p = _pulse("DC1", links, op)
     sig = np.max(np.abs(pu - p.rho0))
     assert sig > 0.05 * 1e-3 * p.rho0
     assert np.max(np.abs(pr[:25] - pu[:25])) / sig < 1e-3
@@ -195,3 +193,47 @@ def test_direct_coupling_is_not_the_standard_one():
     std = _pulse("STD")[0]
     sig = np.max(np.abs(pu - p.rho0))
     assert np.max(np.abs(dc - std)) / sig > 1e-4
+
+
+L.acpo_refined_transfer.argtypes = [ctypes.c_void_p, ctypes.c_int, DP, DP, DP]
+
+
+@pytest.mark.parametrize("scheme", ["STD", "DC1"])
+@pytest.mark.parametrize("half", [0, 1])
+def test_hrr_transfer_is_exact_on_a_linear_shear(scheme, half):
+    """A velocity linear in x (u_y = g x, u_z = h x, uniform density) on both
+    levels, the coarse level equal at t and t + 1: every handed strain rate
+    (the coarse partners', the level-1 direct-coupling stencil's, their
+    spread over the fine faces and the fine partners') is the exact
+    S_xy = g / 2, S_xz = h / 2 (per metre), the others zero.  A neighbour
+    taken one fine node away at the coarse distance, a wrong distance or a
+    missing neighbour read from the wrong side fails it."""
+    nx, ny, nz, x0, x1 = 24, 6, 3, 8, 16
+    p = ol.make_params(nx, ny, nz, op="HRR", periodic=True, tau=0.6, nthreads=2)
+    r = ol.OracleRefined(p, x0, x1)
+    if scheme != "STD":
+        r.set_coupling(scheme, "Mf", "Mc", "hMf")
+    g, h = 2e-3 * p.cs, -1e-3 * p.cs  # velocity change per coarse node
+    for lv, xs in ((0, np.arange(nx, dtype=float)), (1, x0 + 0.5 * np.arange(2 * (x1 - x0) + 1))):
+        lp = r.levels[lv].p
+        X = np.tile(xs, lp.ny * lp.nz) - 12.0
+        u = np.vstack([np.zeros_like(X), g * X, h * X])
+        f = ol.equilibrium_field(lp, np.full(X.size, p.rho0), u)
+        r.levels[lv].set_state(f)  # both arrays: the coarse level equal at t and t + 1
+    cp, fp = ny * nz, 4 * ny * nz
+    col, gc, face = np.full(2 * 6 * cp, np.nan), np.full(2 * 6 * cp, np.nan), np.full(2 * 6 * fp, np.nan)
+    ol.lib().acpo_refined_transfer(r.h, half, ol.dptr(col), ol.dptr(gc), ol.dptr(face))
+    want = np.array([0., 0., 0., 0.5 * g / p.spacing, 0.5 * h / p.spacing, 0.])
+    planes = [face.reshape(2, 6, fp)]
+    if half == 0:
+        planes.append(col.reshape(2, 6, cp))
+        if scheme == "STD":
+            planes.append(gc.reshape(2, 6, cp))
+    elif scheme != "STD":
+        planes.append(gc.reshape(2, 6, cp))
+    else:
+        planes.append(col.reshape(2, 6, cp))
+    for P in planes:
+        for v in range(6):
+            assert np.allclose(P[:, v], want[v], rtol=1e-9, atol=1e-12 * abs(g / p.spacing)), (v, P[:, v].min(),
+                                                                                             P[:, v].max())

@@ -34,7 +34,8 @@ def _pair(op, order, mrt, links, filt, periodic=True, x0=8, x1=16, nx=24, ny=10,
 
 @pytest.mark.parametrize("links", LINKSETS, ids=["-".join(s) for s in LINKSETS])
 @pytest.mark.parametrize("op,order,mrt,filt", [("SRT", 3, 0, "NOFI"), ("RR", 3, 0, "LAG"), ("RR", 4, 0, "NOFI"),
-                                               ("MRT-RM", 3, 1, "NOFI"), ("MRT-GS", 2, 1, "LAG")])
+                                               ("MRT-RM", 3, 1, "NOFI"), ("MRT-GS", 2, 1, "LAG"),
+                                               ("HRR", 3, 0, "NOFI"), ("HRR", 4, 0, "LAG")])
 def test_direct_coupling_matches_oracle_bitexact(gpu, op, order, mrt, filt, links):
     print("case", op, order, filt, links, flush=True)
     o, r = _pair(op, order, mrt, links, filt)
@@ -49,11 +50,12 @@ def test_direct_coupling_matches_oracle_bitexact(gpu, op, order, mrt, filt, link
 
 
 @pytest.mark.parametrize("x1", [17, 23], ids=["inside", "to-lodi-face"])
-def test_direct_coupling_lodi_x_matches_oracle_bitexact(gpu, x1):
+@pytest.mark.parametrize("op", ["RR", "HRR"])
+def test_direct_coupling_lodi_x_matches_oracle_bitexact(gpu, op, x1):
     """LODI x faces on the coarse level; the fine region inside it (two
     coupled faces) or reaching its high-x face (one)."""
-    print("case lodi-x", x1, flush=True)
-    o, r = _pair("RR", 3, 0, ("Mf", "Mc", "hMf"), "LAG", periodic=False, x1=x1, cx=12.0)
+    print("case lodi-x", op, x1, flush=True)
+    o, r = _pair(op, 3, 0, ("Mf", "Mc", "hMf"), "LAG", periodic=False, x1=x1, cx=12.0)
     with r:
         for n in (1, 4, 11):
             r.step(n)
@@ -62,13 +64,15 @@ def test_direct_coupling_lodi_x_matches_oracle_bitexact(gpu, x1):
                 assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
 
 
-@pytest.mark.parametrize("prec,coupling,tol", [("F32-SHIFTED", "F64", 1e-4), ("F32-SHIFTED", "F32-SHIFTED", 1e-3),
-                                               ("F16-SHIFTED", "F64", 2e-2)])
-def test_direct_coupling_shifted_tiers_follow_float64(gpu, prec, coupling, tol):
+@pytest.mark.parametrize("op,prec,coupling,tol", [("RR", "F32-SHIFTED", "F64", 1e-4),
+                                                  ("RR", "F32-SHIFTED", "F32-SHIFTED", 1e-3),
+                                                  ("RR", "F16-SHIFTED", "F64", 2e-2),
+                                                  ("HRR", "F32-SHIFTED", "F64", 1e-4)])
+def test_direct_coupling_shifted_tiers_follow_float64(gpu, op, prec, coupling, tol):
     """Coarse density after 30 coarse steps within `tol` of the largest density
     deviation of the float64 oracle."""
-    print("case", prec, coupling, flush=True)
-    o, r = _pair("RR", 3, 0, ("MfP", "Mc", "hMfP"), "NOFI", prec=prec)
+    print("case", op, prec, coupling, flush=True)
+    o, r = _pair(op, 3, 0, ("MfP", "Mc", "hMfP"), "NOFI", prec=prec)
     o.step(30)
     r64 = o.levels[0].state().sum(axis=0)
     with r:
@@ -78,10 +82,11 @@ def test_direct_coupling_shifted_tiers_follow_float64(gpu, prec, coupling, tol):
     assert np.max(np.abs(rho - r64)) <= tol * np.max(np.abs(r64 - RHO0))
 
 
-def test_direct_coupling_graphs_equal_plain_launches(gpu):
+@pytest.mark.parametrize("op", ["RR", "HRR"])
+def test_direct_coupling_graphs_equal_plain_launches(gpu, op):
     states = []
     for chunk in (0, 4):
-        _, r = _pair("RR", 3, 0, ("Mf", "McP", "hMf"), "LAG")
+        _, r = _pair(op, 3, 0, ("Mf", "McP", "hMf"), "LAG")
         with r:
             r.enable_graphs(chunk)
             r.step(12)
@@ -103,7 +108,19 @@ def test_direct_coupling_refusals(gpu):
             r.set_coupling("DC3")
         r.set_coupling("DC2", "Mf", "McP", "hMfP")
         r.set_coupling("STD")
-    h = acp.make_desc(24, 10, 3, op="HRR", tau=0.6, boundary="PERIODIC")
-    with acp.Refined(h, 8, 16) as r:
-        with pytest.raises(acp.AcpError):
-            r.set_coupling("DC1")
+
+
+def test_hrr_switches_between_couplings_bitexact(gpu):
+    """HRR moves its handed strain rates between the coarse grid-coupling
+    column (standard) and the interface column (direct): a run that switches
+    from one coupling to the other and back equals the oracle's doing the
+    same."""
+    o, r = _pair("HRR", 3, 0, ("Mf", "Mc", "hMf"), "NOFI")
+    with r:
+        for scheme in ("STD", "DC1", "STD", "DC2"):
+            o.set_coupling(scheme, *(("fAll", "cAll", "hAll") if scheme == "STD" else ("Mf", "Mc", "hMf")))
+            r.set_coupling(scheme, *(("fAll", "cAll", "hAll") if scheme == "STD" else ("Mf", "Mc", "hMf")))
+            r.step(3)
+            o.step(3)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (scheme, lv)
