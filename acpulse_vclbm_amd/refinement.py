@@ -3,7 +3,8 @@
 Case (the refined acoustic-pulse configuration): coarse 256 x 128 x 3 grid
 with LODI non-reflecting x faces (classical, K1 = 1) and periodic y and z, RR
 collision, N_eq = 3, tau = 0.5 + 3e-4, SI parameters of the reference's case
-file; fine level (ratio 2) over the right half, coarse x in [128, 255], which
+file, the direct coupling (DC) or the standard one (STD), F2C filter LAG or
+NOFI; fine level (ratio 2) over the right half, coarse x in [128, 255], which
 reaches the high-x face, so the fine level ends on a LODI face there and is
 coupled only at the interface x = 128; Gaussian pulse
 rho0 (1 + A exp(-ln2 r^2 / 8^2)) at (64, 64), A = 1e-3; density probe at
@@ -40,11 +41,12 @@ GEOMETRY = {
 }
 
 
-def _refined_run(acp, d, x1, f2c_filter, precision, coupling, amp, steps, graph_chunk=0):
+def _refined_run(acp, d, x1, f2c_filter, precision, coupling, amp, steps, graph_chunk=0, scheme="STD"):
     """One refined pulse run: (probe series at (96, 64, 1), seconds of the
     stepping, fine-level node count).  With `graph_chunk` > 0 the coarse steps
     are replayed from captured graphs of that many steps."""
     with acp.Refined(d, 128, x1) as r:
+        r.set_coupling(scheme)
         r.set_filter(f2c_filter)
         if precision != "F64":
             r.set_coupling_precision(coupling)
@@ -62,11 +64,13 @@ def _refined_run(acp, d, x1, f2c_filter, precision, coupling, amp, steps, graph_
 
 
 def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", coupling="F64", with_fine=True,
-                         f2c_filter="NOFI", geometry="duct", graph_chunk=0):
+                         f2c_filter="NOFI", geometry="duct", graph_chunk=0, scheme="STD"):
     """Reflection of the refined pulse case for one tier; `coupling` selects the
     coupling arithmetic of the shifted tiers ("F64" or "F32-SHIFTED"),
-    `f2c_filter` the fine-to-coarse filter ("NOFI" or "LAG") and `geometry`
-    the coarse level's x faces ("duct": LODI, "periodic").  With `graph_chunk`
+    `f2c_filter` the fine-to-coarse filter ("NOFI" or "LAG"), `scheme` the
+    Coupling-Approach ("STD", or "DC1" for the direct coupling with the
+    missing links exchanged, Mf / Mc / hMf) and `geometry` the coarse level's
+    x faces ("duct": LODI, "periodic").  With `graph_chunk`
     > 0 the run is repeated with the coarse steps replayed from captured graphs
     of that many steps; its timing is reported beside the plain launches' and
     its probe series must equal theirs bit for bit."""
@@ -75,7 +79,7 @@ def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", co
     win = min(g["window"], steps)
     kw = dict(op=op, order=3, boundary=g["boundary"], mrt_mode="RELAXED", precision=precision)
     d = acp.make_desc(256, 128, 3, tau=TAU, **kw)
-    p_ref, dt, fine_nodes = _refined_run(acp, d, g["x1"], f2c_filter, precision, coupling, amp, steps)
+    p_ref, dt, fine_nodes = _refined_run(acp, d, g["x1"], f2c_filter, precision, coupling, amp, steps, scheme=scheme)
     rho0 = d.rho0
     with acp.Case(d) as u:
         u.init_pulse(amp, 64.0, 64.0, 8.0)
@@ -87,7 +91,7 @@ def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", co
     out = {
         "geometry": geometry, "coarse_boundary": g["boundary"], "fine_region": [128, g["x1"]],
         "operator": op, "precision": precision, "coupling": coupling if precision != "F64" else "F64",
-        "f2c_filter": f2c_filter,
+        "f2c_filter": f2c_filter, "coupling_approach": scheme,
         "steps": steps, "window": win, "amplitude": amp,
         "R_vs_uniform_coarse": float(np.max(np.abs(p_ref[:win] - p_uc[:win])) / sig_c),
         "R_all_steps": float(np.max(np.abs(p_ref - p_uc)) / sig_c),
@@ -98,7 +102,7 @@ def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", co
         "probe": p_ref,
     }
     if graph_chunk:
-        p_g, dt_g, _ = _refined_run(acp, d, g["x1"], f2c_filter, precision, coupling, amp, steps, graph_chunk)
+        p_g, dt_g, _ = _refined_run(acp, d, g["x1"], f2c_filter, precision, coupling, amp, steps, graph_chunk, scheme)
         if not np.array_equal(p_g, p_ref):
             raise RuntimeError("refined graph replays differ from plain launches")
         out["graph_chunk"] = graph_chunk
@@ -151,11 +155,13 @@ def main():
     ap.add_argument("--coupling", default="F64")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--f2c-filter", default="NOFI", choices=["NOFI", "LAG"])
+    ap.add_argument("--scheme", default="STD", choices=["STD", "DC1", "DC2"])
     ap.add_argument("--geometry", default="duct", choices=sorted(GEOMETRY))
     ap.add_argument("--graph-chunk", type=int, default=0, help="also time graph replays of this many coarse steps (even; 0: no)")
     a = ap.parse_args()
     r = interface_reflection(acp, op=a.op, steps=a.steps, precision=a.precision, coupling=a.coupling,
-                             f2c_filter=a.f2c_filter, geometry=a.geometry, graph_chunk=a.graph_chunk)
+                             f2c_filter=a.f2c_filter, geometry=a.geometry, graph_chunk=a.graph_chunk,
+                             scheme=a.scheme)
     r.pop("probe", None)
     print(json.dumps(r))
 

@@ -378,13 +378,17 @@ def main():
     if not a.no_refinement:
         from acpulse_vclbm_amd import refinement
         ref = {}
-        for op, filt in (("RR", "NOFI"), ("RR", "LAG"), ("HRR", "NOFI")):
-            x = refinement.interface_reflection(acp, op=op, f2c_filter=filt, graph_chunk=a.refine_graph_chunk)
+        # the first run is the refined acoustic-pulse case as specified: direct
+        # coupling, RR, LAG restriction, 2000 coarse steps, probe at x = 96
+        for op, filt, scheme in (("RR", "LAG", "DC1"), ("RR", "NOFI", "STD"), ("RR", "LAG", "STD"),
+                                 ("HRR", "NOFI", "STD"), ("HRR", "LAG", "DC1")):
+            x = refinement.interface_reflection(acp, op=op, f2c_filter=filt, graph_chunk=a.refine_graph_chunk,
+                                                scheme=scheme)
             x.pop("probe", None)
-            ref[f"{op} {filt}"] = x
+            ref[f"{op} {scheme} {filt}"] = x
             g = (f" ({x['ms_per_coarse_step_graph']:.3f} ms replayed from graphs of {x['graph_chunk']} steps)"
                  if "graph_chunk" in x else "")
-            print(f"# refinement (256x128x3 + fine half, {op}, F2C {filt}): interface echo "
+            print(f"# refinement (256x128x3 + fine half, {op}, {scheme}, F2C {filt}): interface echo "
                   f"{x['R_vs_uniform_coarse']:.2e} of the pulse, {x['ms_per_coarse_step']:.3f} ms per coarse step"
                   f"{g}", file=sys.stderr, flush=True)
     lodi = None

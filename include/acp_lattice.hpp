@@ -293,6 +293,7 @@ public:
             detail::check(acp_refined_create(&cd, x0_, x1_, nullptr, &r_), "Lattice::createLattice");
             detail::check(acp_refined_level(r_, 0, &lv_[0]), "Lattice::createLattice");
             detail::check(acp_refined_level(r_, 1, &lv_[1]), "Lattice::createLattice");
+            apply_coupling_keys();
         } else {
             detail::die("Lattice::createLattice: MaxLevel " + std::to_string(d.max_level) +
                         ": at most one refined level\n");
@@ -309,6 +310,37 @@ public:
         initialized_ = false;
         invalidate();
     }
+    // The case file's coupling keys (BoundaryConditions.txt:52-90):
+    // Coupling-Approach STD / DC1 / DC2 with Coarse-to-Fine, Fine-to-Coarse
+    // and Hanging-Node-Reconstruction, F2C-Filter NOFI / LAG and Scaling-Style
+    // SRTSTYLE.  A value the engine does not provide ends the program with the
+    // key and the value, as the reference ends on a malformed case file.
+    void apply_coupling_keys() {
+        const std::string filt = bc_.getF2Cfilter();
+        if (filt != "NOFI" && filt != "LAG")
+            detail::die("Lattice::createLattice: F2C-Filter " + filt + " is not available (NOFI or LAG)\n");
+        detail::check(acp_refined_set_filter(r_, filt == "LAG" ? ACP_F2C_LAG : ACP_F2C_NOFI), "Lattice::createLattice");
+        if (bc_.getScalingStyle() != "SRTSTYLE")
+            detail::die("Lattice::createLattice: Scaling-Style " + bc_.getScalingStyle() +
+                        " is not available (SRTSTYLE)\n");
+        const std::string cp = bc_.getCouplingProcedure();
+        if (cp != "STD" && cp != "DC1" && cp != "DC2")
+            detail::die("Lattice::createLattice: Coupling-Approach " + cp + " (STD, DC1 or DC2)\n");
+        auto links = [](const std::string& key, const std::string& v, const char* m, const char* mp, const char* all) {
+            if (v == m) return (int32_t)ACP_LINKS_MISSING;
+            if (v == mp) return (int32_t)ACP_LINKS_MISSING_COMMON;
+            if (v == all) return (int32_t)ACP_LINKS_ALL;
+            detail::die("Lattice::createLattice: " + key + " " + v + " (" + m + ", " + mp + " or " + all + ")\n");
+            return (int32_t)-1;
+        };
+        const int32_t c2f = links("Coarse-to-Fine", bc_.getCouplingReplacementC2F(), "Mf", "MfP", "fAll");
+        const int32_t f2c = links("Fine-to-Coarse", bc_.getCouplingReplacementF2C(), "Mc", "McP", "cAll");
+        const int32_t hang = links("Hanging-Node-Reconstruction", bc_.getHangingNodeReconstruction(), "hMf", "hMfP", "hAll");
+        const std::string where = "Lattice::createLattice: Coupling-Approach " + cp;
+        detail::check(acp_refined_set_coupling(r_, cp == "STD" ? ACP_COUPLING_STD : ACP_COUPLING_DC, c2f, f2c, hang),
+                      where.c_str());
+    }
+
     // Lattice.cpp:237-1394: the neighbour, LODI and partner relations are a
     // function of the grid and its boundary kind; the engine derives them
     // when the device state is created (the LODI schedule on the host, the

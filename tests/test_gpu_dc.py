@@ -124,3 +124,38 @@ def test_hrr_switches_between_couplings_bitexact(gpu):
             o.step(3)
             for lv in (0, 1):
                 assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (scheme, lv)
+
+
+def test_refined_pulse_case_as_specified_matches_oracle(gpu):
+    """The refined acoustic-pulse case as specified, at full size: coarse
+    256 x 128 x 3 with LODI x faces and periodic y, z, fine level (ratio 2)
+    over coarse x in [128, 255] ending on a LODI face, direct coupling, RR,
+    F2C filter LAG, pulse 1e-3 of half-width 8 at (64, 64), probe at
+    (96, 64, 1).  Its first 40 coarse steps: the probe series and both levels'
+    states equal the oracle's bit for bit (the oracle restates the reference's
+    arithmetic at a few MLUPS, so the 2000-step run is the GPU's alone)."""
+    import os
+
+    from acpulse_vclbm_amd.refinement import TAU
+    steps = 40
+    p = ol.make_params(256, 128, 3, op="RR", order=3, periodic=3, tau=TAU, mrt_mode=1, cs=CS, spacing=DX, rho0=RHO0,
+                       nthreads=min(16, os.cpu_count() or 1))
+    o = ol.OracleRefined(p, 128, 255)
+    o.set_coupling("DC1", "Mf", "Mc", "hMf")
+    o.set_filter("LAG")
+    o.levels[0].set_state(ol.pulse_ic(p, 1e-3, 64.0, 64.0, 8.0))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, 1e-3, 2 * (64.0 - 128), 128.0, 16.0))
+    print("oracle", steps, "steps", flush=True)
+    po = o.step(steps, probes=[(96, 64, 1)])[:, 0]
+    d = acp.make_desc(256, 128, 3, op="RR", order=3, tau=TAU, boundary="LODI-X", mrt_mode="RELAXED")
+    with acp.Refined(d, 128, 255) as r:
+        r.set_coupling("DC1", "Mf", "Mc", "hMf")
+        r.set_filter("LAG")
+        r.init_pulse(1e-3, 64.0, 64.0, 8.0)
+        c = r.level(0)
+        c.set_probes([(96, 64, 1)], capacity=steps)
+        r.step(steps)
+        pg = c.read_probes()[:, 0].copy()
+        assert np.array_equal(pg, po)
+        for lv in (0, 1):
+            assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), lv

@@ -107,13 +107,23 @@ def test_stepping_member_of_another_operator_exits(gpu, tmp_path):
     assert "timeSteppingRR" in r.stderr and "one operator per case" in r.stderr
 
 
-def _refined_text(nx, ny, nz, op, region, steps):
+def _refined_text(nx, ny, nz, op, region, steps, coupling=None):
     txt = bc_text(nx, ny, nz, op=op, tau=0.56, steps=steps).replace("MaxLevel 0", "MaxLevel 1")
+    if coupling:
+        scheme, c2f, f2c, hang, filt = coupling
+        txt = (txt.replace("Coupling-Approach STD", f"Coupling-Approach {scheme}")
+               .replace("Coarse-to-Fine fAll", f"Coarse-to-Fine {c2f}").replace("Fine-to-Coarse cAll", f"Fine-to-Coarse {f2c}")
+               .replace("Hanging-Node-Reconstruction hAll", f"Hanging-Node-Reconstruction {hang}")
+               .replace("F2C-Filter NOFI", f"F2C-Filter {filt}"))
     return txt + f"Boundary LODI-X\nRefined-Region {region[0]} {region[1]}\n"
 
 
+COUPLINGS = [None, ("DC1", "Mf", "Mc", "hMf", "LAG"), ("DC2", "MfP", "Mc", "hMfP", "NOFI")]
+
+
+@pytest.mark.parametrize("coupling", COUPLINGS, ids=["STD-NOFI", "DC1-LAG", "DC2-MfP"])
 @pytest.mark.parametrize("op", ["RR", "HRR"])
-def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op):
+def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, coupling):
     """MaxLevel 1 with this engine's refined duct geometry: the reference's
     initial pulse on both levels (against the oracle's, bit for bit), one
     nestedTimeStepping step and then solve(flagL0, flagL1) steps, equal to
@@ -121,11 +131,14 @@ def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op):
     coarse/fine partners."""
     nx, ny, nz, steps, x0, x1 = 24, 8, 3, 4, 12, 23
     f = tmp_path / "bc.txt"
-    f.write_text(_refined_text(nx, ny, nz, op, (x0, x1), steps))
+    f.write_text(_refined_text(nx, ny, nz, op, (x0, x1), steps, coupling))
     d = acp.read_case_file(str(f))
     d.max_level = 0
-    print("case refined", op, flush=True)
+    print("case refined", op, coupling, flush=True)
     with acp.Refined(d, x0, x1) as rc:
+        if coupling:
+            rc.set_coupling(*coupling[:4])
+            rc.set_filter(coupling[4])
         rc.init_reference()
         p0 = ol.make_params(nx, ny, nz, op=op, tau=0.56, cs=CS, spacing=DX, rho0=RHO0, periodic=3)
         p1 = ol.make_params(2 * (x1 - x0) + 1, 2 * ny, 2 * nz, op=op, tau=0.56, cs=CS, spacing=DX / 2, rho0=RHO0,
@@ -149,3 +162,17 @@ def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op):
     assert rec["partner"][(0, mid, ny // 2, nz // 2)] == (1, 2 * (mid - x0), 2 * (ny // 2), 2 * (nz // 2))
     assert rec["node"][(1, 0, ny, nz)][1] == "gridcoupling"
     assert rec["node"][(1, fnx - 1, 3, 1)][1] == "boundary"
+
+
+@pytest.mark.parametrize("key,value", [("F2C-Filter NOFI", "F2C-Filter TOU"), ("Scaling-Style SRTSTYLE", "Scaling-Style MRTSTYLE"),
+                                       ("Coupling-Approach STD", "Coupling-Approach XYZ"),
+                                       ("Coarse-to-Fine fAll", "Coarse-to-Fine Mf")])
+def test_two_level_coupling_keys_the_engine_lacks_end_the_program(gpu, tmp_path, key, value):
+    """A two-level case file asking for what the engine does not provide (the
+    TOU filter, moment-based scaling, an unknown scheme, or the standard
+    coupling with only the missing links) ends the program naming the key."""
+    f = tmp_path / "bc.txt"
+    f.write_text(_refined_text(24, 8, 3, "RR", (12, 23), 2).replace(key, value))
+    r = _run(f, "nested")
+    assert r.returncode == 1
+    assert value.split()[0] in r.stderr, r.stderr
