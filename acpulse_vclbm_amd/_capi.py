@@ -58,7 +58,7 @@ class Request(ctypes.Structure):
         ("seed", ctypes.c_uint64), ("rho_amp", ctypes.c_double), ("u_amp", ctypes.c_double),
         ("probe", ctypes.c_int32 * 3), ("probe_out", ctypes.POINTER(ctypes.c_double)),
         ("arrival", ctypes.c_double), ("tag", ctypes.c_uint64),
-        ("refine", ctypes.c_int32 * 2), ("f2c_filter", ctypes.c_int32),
+        ("refine", ctypes.c_int32 * 2), ("f2c_filter", ctypes.c_int32), ("coupling", ctypes.c_int32),
     ]
 
 

@@ -163,6 +163,10 @@ int create_job_case(acp_server* s, Job& j) {
                                    &j.r);
         if (rc == ACP_OK) rc = acp_refined_level(j.r, 0, &j.c);
         if (rc == ACP_OK) rc = acp_refined_set_filter(j.r, q.f2c_filter);
+        if (rc == ACP_OK) {
+            const int32_t links = q.coupling == ACP_COUPLING_DC ? ACP_LINKS_MISSING : ACP_LINKS_ALL;
+            rc = acp_refined_set_coupling(j.r, q.coupling, links, links, links);
+        }
         if (rc == ACP_OK) rc = acp_refined_init_pulse(j.r, q.amp, q.x0, q.y0, q.halfwidth);
     } else {
         rc = acp_case_create_in(&q.desc, s->streams[j.slot], s->arena + j.off, j.bytes, &j.c);
@@ -418,6 +422,8 @@ extern "C" int acp_server_submit(acp_server* s, const acp_request* r) {
             return acp::fail(ACP_E_UNSUPPORTED, "acp_server_submit: two-level requests start from a pulse");
         if (r->f2c_filter != ACP_F2C_NOFI && r->f2c_filter != ACP_F2C_LAG)
             return acp::fail(ACP_E_UNSUPPORTED, "acp_server_submit: fine-to-coarse filter NOFI or LAG");
+        if (r->coupling != ACP_COUPLING_STD && r->coupling != ACP_COUPLING_DC)
+            return acp::fail(ACP_E_ARG, "acp_server_submit: coupling ACP_COUPLING_STD or ACP_COUPLING_DC");
         if (int rc = acp_refined_footprint(&d, r->refine[0], r->refine[1], np, cap, &bytes)) return rc;
     } else if (int rc = acp_case_footprint(&d, np, cap, &bytes)) {
         return rc;

@@ -21,9 +21,9 @@ Grids are nx x ny x 3, tau = 0.5 + 3e-4, N_eq = 3, the probe at
     coarse grid with LODI x faces, periodic in y and z, and a fine level
     (ratio 2) over x in [nx/2, nx - 1], which ends on a LODI face at the
     high-x end; the pulse at (nx/4, ny/2) in the coarse half, the probe read
-    on the coarse level inside the fine region; standard coupling with the
-    F2C filter NOFI.  The direct coupling is not available: requests that draw
-    it run with the standard coupling and are counted as such.
+    on the coarse level inside the fine region; F2C filter NOFI; the
+    standard coupling for requests that draw "overlap", the direct coupling
+    (missing links: Mf, Mc, hMf) for those that draw "direct".
 """
 import ctypes
 import time
@@ -100,7 +100,7 @@ def mix_counts(reqs):
     """How the refinement flags of a request list were served."""
     return {"requests": len(reqs), "refined_flag": sum(bool(r["refined"]) for r in reqs),
             "two_level": sum(two_level(r) for r in reqs),
-            "direct_run_as_overlap": sum(two_level(r) and r["scheme"] == "direct" for r in reqs)}
+            "direct_coupling": sum(two_level(r) and r["scheme"] == "direct" for r in reqs)}
 
 
 class Server:
@@ -126,6 +126,7 @@ class Server:
         if two_level(r):
             q.refine[0], q.refine[1] = fine_region(r)
             q.f2c_filter = _capi.F2C_FILTER["NOFI"]
+            q.coupling = 1 if r.get("scheme") == "direct" else 0
         pi, pj, pk = probe_point(r)
         q.probe[0], q.probe[1], q.probe[2] = pi, pj, pk
         tag = int(r["tag"])
@@ -219,6 +220,8 @@ def run_alone(r, graph_chunk=0):
     if two_level(r):
         with Refined(request_desc(r), *fine_region(r)) as rc:
             rc.set_filter("NOFI")
+            if r.get("scheme") == "direct":
+                rc.set_coupling("DC1", "Mf", "Mc", "hMf")
             rc.init_pulse(r["amp"], cx, cy, r["halfwidth"])
             c = rc.level(0)
             c.set_probes([probe_point(r)], capacity=max(r["steps"], 1))

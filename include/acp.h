@@ -386,6 +386,8 @@ typedef struct acp_request {
                                    LODI_X coarse level, pulse initial condition); refine[1] <= refine[0]:
                                    one level */
     int32_t f2c_filter;         /* ACP_F2C_* of a two-level case */
+    int32_t coupling;           /* ACP_COUPLING_* of a two-level case; the direct coupling exchanges the
+                                   missing links (Mf, Mc, hMf) */
 } acp_request;
 
 typedef struct acp_result {

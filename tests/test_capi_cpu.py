@@ -191,3 +191,31 @@ def test_lodi_launch_order_keeps_the_reference_reads(dims):
                 continue  # an interior node: its B entries are final before the LODI pass
             assert launch[x] != launch[a], f"node {a} reads node {x} written in the same launch"
             assert (launch[x] < launch[a]) == (pos[x] < pos[a]), (a, x)
+
+
+STRUCTS = [("acp_case_desc", _capi.CaseDesc), ("acp_case_info", _capi.CaseInfo), ("acp_server_desc", _capi.ServerDesc),
+           ("acp_request", _capi.Request), ("acp_result", _capi.Result), ("acp_server_stats", _capi.ServerStats)]
+
+
+def test_ctypes_structures_match_the_header(tmp_path):
+    """Size and every field offset of the binding's structures against the C
+    compiler's layout of include/acp.h (a mismatch would hand the library
+    misplaced fields)."""
+    import subprocess
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    lines = ["#include <stddef.h>", "#include <stdio.h>", '#include "acp.h"', "int main(void) {"]
+    for cname, py in STRUCTS:
+        lines.append(f'    printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'    printf("{cname} {f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", f"-I{inc}", str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in STRUCTS:
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert got[(cname, f[0])] == getattr(py, f[0]).offset, (cname, f[0])

@@ -215,6 +215,8 @@ def _oracle_request(r):
         x0, x1 = serve.fine_region(r)
         o = ol.OracleRefined(p, x0, x1)
         o.set_filter("NOFI")
+        if r.get("scheme") == "direct":
+            o.set_coupling("DC1", "Mf", "Mc", "hMf")
         o.levels[0].set_state(ol.pulse_ic(p, r["amp"], cx, cy, r["halfwidth"]))
         o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, r["amp"], 2 * (cx - x0), 2 * cy, 2 * r["halfwidth"]))
         series = o.step(r["steps"], probe)[:, 0]
@@ -234,6 +236,7 @@ def test_served_results_match_the_oracle(gpu):
     cases follow the oracle's probe series within 2e-4 of the pulse amplitude."""
     reqs = _small_mix(12, seed=5)
     assert any(serve.two_level(r) for r in reqs) and any(not serve.two_level(r) for r in reqs)
+    assert {r["scheme"] for r in reqs if serve.two_level(r)} == {"overlap", "direct"}
     assert {r["precision"] for r in reqs} == {"F64", "F32-SHIFTED"}
     results, summary = serve.run_mix(reqs, hbm_budget_bytes=1 << 30, max_active=4, graph_chunk=8, chunks_ahead=2,
                                      timeout_s=120)
