@@ -230,9 +230,9 @@ class Refined:
         check(lib().acp_refined_set_coupling_precision(self.h, PREC[precision]))
 
     def set_filter(self, name):
-        """Fine-to-coarse filter (F2C-Filter): "NOFI" (default) or "LAG"."""
+        """Fine-to-coarse filter (F2C-Filter): "NOFI" (default), "LAG" or "TOU"."""
         if name not in F2C_FILTER:
-            raise AcpError(_capi.ACP_E_UNSUPPORTED, f"fine-to-coarse filter {name!r}: NOFI or LAG")
+            raise AcpError(_capi.ACP_E_UNSUPPORTED, f"fine-to-coarse filter {name!r}: NOFI, LAG or TOU")
         check(lib().acp_refined_set_filter(self.h, F2C_FILTER[name]))
 
     def set_coupling(self, scheme="STD", c2f=None, f2c=None, hanging=None):

@@ -18,7 +18,7 @@ ACP_OK, ACP_E_ARG, ACP_E_CONFIG, ACP_E_OOM, ACP_E_HIP, ACP_E_STATE, ACP_E_UNSUPP
 OP = {"SRT": 0, "MRT-GS": 1, "MRT-RM": 2, "RR": 3, "HRR": 4}
 PREC = {"F64": 0, "F32-SHIFTED": 1, "F16-SHIFTED": 2}
 BOUNDARY = {"LODI": 0, "PERIODIC": 1, "OPEN-X": 2, "LODI-X": 3, "OPEN-LODI-X": 4}
-F2C_FILTER = {"NOFI": 0, "LAG": 1}
+F2C_FILTER = {"NOFI": 0, "LAG": 1, "TOU": 2}
 MRT_MODE = {"REFERENCE": 0, "RELAXED": 1}
 
 

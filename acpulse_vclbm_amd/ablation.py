@@ -25,7 +25,7 @@ R_rel = |R - R_f64| / R_f64.
 Part C, the coupling of the refined case in float64 or float32
 (acpulse_vclbm_amd.refinement.restriction_ablation): the refined pulse case
 (coarse 256 x 128 x 3 duct with LODI x faces, fine over x in [128, 255], RR,
-2000 coarse steps) in the shifted tiers with
+direct coupling, LAG restriction, 2000 coarse steps) in the shifted tiers with
 the coarse-to-fine interpolation, rescaling and fine-to-coarse restriction in
 float64 (on SI populations) or in float32 (on lattice-unit deviations):
 probe-series L2 error against the float64 refined run and the relative error

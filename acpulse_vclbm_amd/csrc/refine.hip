@@ -170,7 +170,16 @@ __global__ __launch_bounds__(256) void c2f_between(const double* __restrict__ Fc
 // 130 and runs at three waves, where four would spill).
 constexpr int kLagNodes = 13;  // coarse nodes per 256-thread workgroup (247 lanes)
 
-template <int ORDER>
+//
+// F2C-Filter TOU (TOU = true; Touil, Ricot and Leveque 2014, doi
+// 10.1016/j.jcp.2013.07.037, selected at Lattice.cpp:5179-5211) shares the
+// first half: each population is filtered along its own lattice direction,
+//   nbar_q = (1/4 n_q(x - e_q) + 1/2 n_q(x)) + 1/4 n_q(x + e_q),  nbar_18 = n_18(x),
+// n_q(x - e_q) being lane 1 + OPP[q]'s and n_q(x + e_q) lane 1 + q's; after a
+// second barrier every lane sums the filtered part's mass M and momentum J
+// over q = 0..18 and clears them, nbar_q - w_q (M + xi_q . J / cs^2), so that
+// the coarse node keeps its coincident fine node's density and velocity.
+template <int ORDER, bool TOU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ? 4 : 3))) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                                int fny, int fnz, double ainv, double* __restrict__ C, int cnx,
                                                int cny, int cnz, int x0, int width, const PopLayout CN,
@@ -212,14 +221,48 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
         }
     }
     __syncthreads();
-    if (!active) return;
     const int q = s;
-    double acc = neq[node][0][q];
-#pragma unroll
-    for (int m = 1; m < 19; ++m) acc += neq[node][m][q];
     const int I = (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
-    C[pidx(CN, q, lc)] = eq0[node][q] + ainv * (acc / 19.);
+    if constexpr (TOU) {
+        __shared__ double nbar[kLagNodes][19];
+        if (active) {
+            double nb = neq[node][0][q];
+            static_for<0, 18>([&](auto qc) {
+                constexpr int qq = decltype(qc)::value;
+                if (q == qq) nb = (0.25 * neq[node][1 + OPP[qq]][qq] + 0.5 * neq[node][0][qq]) + 0.25 * neq[node][1 + qq][qq];
+            });
+            nbar[node][q] = nb;
+        }
+        __syncthreads();
+        if (!active) return;
+        double M = 0., Jx = 0., Jy = 0., Jz = 0.;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int qq = decltype(qc)::value;
+            const double v = nbar[node][qq];
+            M += v;
+            if constexpr (EX[qq] != 0) Jx += (EX[qq] > 0 ? k.c : -k.c) * v;
+            if constexpr (EY[qq] != 0) Jy += (EY[qq] > 0 ? k.c : -k.c) * v;
+            if constexpr (EZ[qq] != 0) Jz += (EZ[qq] > 0 ? k.c : -k.c) * v;
+        });
+        double sv = M, w = 0.;
+        static_for<0, 19>([&](auto qc) {
+            constexpr int qq = decltype(qc)::value;
+            if (q == qq) {
+                if constexpr (EX[qq] != 0) sv += k.rcs2 * ((EX[qq] > 0 ? k.c : -k.c) * Jx);
+                if constexpr (EY[qq] != 0) sv += k.rcs2 * ((EY[qq] > 0 ? k.c : -k.c) * Jy);
+                if constexpr (EZ[qq] != 0) sv += k.rcs2 * ((EZ[qq] > 0 ? k.c : -k.c) * Jz);
+                w = weight(qq);
+            }
+        });
+        C[pidx(CN, q, lc)] = eq0[node][q] + ainv * (nbar[node][q] - w * sv);
+    } else {
+        if (!active) return;
+        double acc = neq[node][0][q];
+#pragma unroll
+        for (int m = 1; m < 19; ++m) acc += neq[node][m][q];
+        C[pidx(CN, q, lc)] = eq0[node][q] + ainv * (acc / 19.);
+    }
 }
 
 // Coarse nodes with x in [x0 + 1, x1 - 1] from their coincident fine nodes
@@ -396,6 +439,79 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
     }
 }
 
+// TOU restriction in the shifted tiers: the same filter as f2c_lag's TOU form,
+// one thread per coarse node, in float64 on SI populations (F64) or in float32
+// on lattice-unit deviations.
+template <typename T, int ORDER, bool F64>
+__device__ __forceinline__ void tou_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
+                                           int fnz, int fx, int fy, int fz, double ainv, T* __restrict__ C,
+                                           const PopLayout& CN, int64_t lc, const PopLayout& FN) {
+    using R = std::conditional_t<F64, double, float>;
+    const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
+    T raw[19];
+    load_raw19(F, FN, l0, raw);
+    R v[19], eq[19], n0[19], up[19], dn[19], nb[19];
+    auto convert = [&](const T in[19], R out[19]) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (F64) out[q] = g_to_f(ks, in[q], q);
+            else out[q] = (float)in[q] * ks.inv_s;
+        });
+    };
+    auto equilibrium = [&](const R f[19], R e[19]) {
+        if constexpr (F64) {
+            double r, x1, x2, x3;
+            moments64(k, f, r, x1, x2, x3);
+            feq64<ORDER>(k, r, x1, x2, x3, e);
+        } else {
+            float r, x1, x2, x3;
+            moments_dev(f, r, x1, x2, x3);
+            feq_dev<ORDER>(r, x1, x2, x3, e, ks.k6);
+        }
+    };
+    convert(raw, v);
+    equilibrium(v, eq);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) n0[q] = v[q] - eq[q];
+    static_for<0, 18>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int y = (fy + EY[j] + fny) % fny, z = (fz + EZ[j] + fnz) % fnz;
+        const int64_t l = fx + EX[j] + (int64_t)fnx * (y + (int64_t)fny * z);
+        T rn[19];
+        R g[19], e[19];
+        load_raw19(F, FN, l, rn);
+        convert(rn, g);
+        equilibrium(g, e);
+        up[j] = g[j] - e[j];
+        dn[OPP[j]] = g[OPP[j]] - e[OPP[j]];
+    });
+    const R quarter = (R)0.25, half = (R)0.5;
+#pragma unroll
+    for (int q = 0; q < 18; ++q) nb[q] = (quarter * dn[q] + half * n0[q]) + quarter * up[q];
+    nb[18] = n0[18];
+    // lattice speed and 1 / cs^2: SI (float64) or lattice units (float32)
+    const R c = F64 ? (R)k.c : (R)1, rcs2 = F64 ? (R)k.rcs2 : (R)3;
+    R M = 0, Jx = 0, Jy = 0, Jz = 0;
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        M += nb[q];
+        if constexpr (EX[q] != 0) Jx += (EX[q] > 0 ? c : -c) * nb[q];
+        if constexpr (EY[q] != 0) Jy += (EY[q] > 0 ? c : -c) * nb[q];
+        if constexpr (EZ[q] != 0) Jz += (EZ[q] > 0 ? c : -c) * nb[q];
+    });
+    const R ai = (R)ainv;
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        R sv = M;
+        if constexpr (EX[q] != 0) sv += rcs2 * ((EX[q] > 0 ? c : -c) * Jx);
+        if constexpr (EY[q] != 0) sv += rcs2 * ((EY[q] > 0 ? c : -c) * Jy);
+        if constexpr (EZ[q] != 0) sv += rcs2 * ((EZ[q] > 0 ? c : -c) * Jz);
+        const R out = eq[q] + ai * (nb[q] - (R)weight(q) * sv);
+        if constexpr (F64) C[pidx(CN, q, lc)] = f_to_g<T>(ks, out, q);
+        else C[pidx(CN, q, lc)] = (T)(out * ks.s);
+    });
+}
+
 // The coarse-to-fine kernels of the shifted tiers (see c2f_coincident and
 // c2f_between); the averages are formed from the coincident values as
 // stored, in float64 (F64 coupling) or float32.
@@ -468,7 +584,7 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
     });
 }
 
-template <typename T, int ORDER, bool F64, bool LAG>
+template <typename T, int ORDER, bool F64, int FILT>
 __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                              const T* __restrict__ F, int fnx, int fny, int fnz, double ainv,
                                              T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width,
@@ -482,8 +598,10 @@ __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const K
     const int J = (int)(r % cny), K = (int)(r / cny);
     const int I = (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
-    if constexpr (LAG) {
+    if constexpr (FILT == ACP_F2C_LAG) {
         lag_node_s<T, ORDER, F64>(k, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN);
+    } else if constexpr (FILT == ACP_F2C_TOU) {
+        tou_node_s<T, ORDER, F64>(k, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN);
     } else {
         const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
         couple_node<T, ORDER, F64>(k, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
@@ -946,7 +1064,7 @@ struct acp_refined {
     int x0 = 0, x1 = 0;
     double a = 1.0;        // tau_f / (2 tau_c)
     bool f64_coupling = true;  // shifted tiers: coupling arithmetic in float64 (else float32)
-    bool lag = false;          // fine-to-coarse filter LAG (else NOFI)
+    int filter = ACP_F2C_NOFI;  // fine-to-coarse filter (F2C-Filter)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int32_t graph_chunk = 0;              // coarse steps per captured graph (0: plain launches)
@@ -1006,18 +1124,21 @@ hipError_t launch_f2c_s(acp_refined* r) {
     const int width = r->x1 - r->x0 - 1;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-#define F2CS(O, LG)                                                                                              \
-    hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64, LG>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, \
+#define F2CS(O, FI)                                                                                              \
+    hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64, FI>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, \
                        f->g.ny, f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
-    if (r->lag) {
-        if (c->d.order == 2) F2CS(2, true);
-        else if (c->d.order == 3) F2CS(3, true);
-        else F2CS(4, true);
+#define F2CS_O(FI)                       \
+    if (c->d.order == 2) F2CS(2, FI);    \
+    else if (c->d.order == 3) F2CS(3, FI); \
+    else F2CS(4, FI)
+    if (r->filter == ACP_F2C_LAG) {
+        F2CS_O(ACP_F2C_LAG);
+    } else if (r->filter == ACP_F2C_TOU) {
+        F2CS_O(ACP_F2C_TOU);
     } else {
-        if (c->d.order == 2) F2CS(2, false);
-        else if (c->d.order == 3) F2CS(3, false);
-        else F2CS(4, false);
+        F2CS_O(ACP_F2C_NOFI);
     }
+#undef F2CS_O
 #undef F2CS
     return hipGetLastError();
 }
@@ -1061,15 +1182,23 @@ hipError_t launch_f2c(acp_refined* r) {
 #define F2C(O)                                                                                                  \
     hipLaunchKernelGGL((acp::refine::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, \
                        c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
-#define F2CLAG(O)                                                                                                \
-    hipLaunchKernelGGL((acp::refine::f2c_lag<O>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) /             \
-                                                                   acp::refine::kLagNodes)),                     \
+#define F2CLAG(O, TOU)                                                                                           \
+    hipLaunchKernelGGL((acp::refine::f2c_lag<O, TOU>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) /        \
+                                                                        acp::refine::kLagNodes)),                \
                        b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, c->A, c->g.nx, c->g.ny, \
                        c->g.nz, r->x0, width, c->g.L, f->g.L)
-    if (r->lag) {
-        if (c->d.order == 2) F2CLAG(2);
-        else if (c->d.order == 3) F2CLAG(3);
-        else F2CLAG(4);
+    if (r->filter != ACP_F2C_NOFI) {
+        const bool tou = r->filter == ACP_F2C_TOU;
+        if (c->d.order == 2) {
+            if (tou) F2CLAG(2, true);
+            else F2CLAG(2, false);
+        } else if (c->d.order == 3) {
+            if (tou) F2CLAG(3, true);
+            else F2CLAG(3, false);
+        } else {
+            if (tou) F2CLAG(4, true);
+            else F2CLAG(4, false);
+        }
     } else {
         if (c->d.order == 2) F2C(2);
         else if (c->d.order == 3) F2C(3);
@@ -1528,11 +1657,11 @@ extern "C" int acp_refined_set_coupling_precision(acp_refined* r, int32_t precis
 
 extern "C" int acp_refined_set_filter(acp_refined* r, int32_t filter) {
     if (!r) return fail(ACP_E_ARG, "acp_refined_set_filter: null argument");
-    if (filter != ACP_F2C_NOFI && filter != ACP_F2C_LAG)
-        return fail(ACP_E_UNSUPPORTED, "fine-to-coarse filter: ACP_F2C_NOFI or ACP_F2C_LAG");
+    if (filter != ACP_F2C_NOFI && filter != ACP_F2C_LAG && filter != ACP_F2C_TOU)
+        return fail(ACP_E_UNSUPPORTED, "fine-to-coarse filter: ACP_F2C_NOFI, ACP_F2C_LAG or ACP_F2C_TOU");
     if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
     drop_graphs(r);
-    r->lag = filter == ACP_F2C_LAG;
+    r->filter = filter;
     return ACP_OK;
 }
 

@@ -420,8 +420,8 @@ extern "C" int acp_server_submit(acp_server* s, const acp_request* r) {
     if (two_level(*r)) {
         if (r->init_kind != ACP_INIT_PULSE)
             return acp::fail(ACP_E_UNSUPPORTED, "acp_server_submit: two-level requests start from a pulse");
-        if (r->f2c_filter != ACP_F2C_NOFI && r->f2c_filter != ACP_F2C_LAG)
-            return acp::fail(ACP_E_UNSUPPORTED, "acp_server_submit: fine-to-coarse filter NOFI or LAG");
+        if (r->f2c_filter != ACP_F2C_NOFI && r->f2c_filter != ACP_F2C_LAG && r->f2c_filter != ACP_F2C_TOU)
+            return acp::fail(ACP_E_UNSUPPORTED, "acp_server_submit: fine-to-coarse filter NOFI, LAG or TOU");
         if (r->coupling != ACP_COUPLING_STD && r->coupling != ACP_COUPLING_DC)
             return acp::fail(ACP_E_ARG, "acp_server_submit: coupling ACP_COUPLING_STD or ACP_COUPLING_DC");
         if (int rc = acp_refined_footprint(&d, r->refine[0], r->refine[1], np, cap, &bytes)) return rc;

@@ -122,19 +122,20 @@ def interface_reflection(acp, op="RR", steps=None, amp=1e-3, precision="F64", co
     return out
 
 
-def restriction_ablation(acp, op="RR", f2c_filter="NOFI"):
-    """Refined pulse case in the shifted tiers with the coupling arithmetic in
-    float64 and in float32, against the float64 refined run: probe-series L2
-    error and the relative error of the interface reflection coefficient."""
-    ref = interface_reflection(acp, op, with_fine=False, f2c_filter=f2c_filter)
+def restriction_ablation(acp, op="RR", f2c_filter="LAG", scheme="DC1"):
+    """Refined pulse case (by default as specified: direct coupling, RR, LAG)
+    in the shifted tiers with the coupling arithmetic in float64 and in
+    float32, against the float64 refined run: probe-series L2 error and the
+    relative error of the interface reflection coefficient."""
+    ref = interface_reflection(acp, op, with_fine=False, f2c_filter=f2c_filter, scheme=scheme)
     p64, r64 = ref["probe"], ref["R_vs_uniform_coarse"]
     rho0 = acp.make_desc(8, 8, 8, tau=0.6).rho0
     sig = np.linalg.norm(p64 - rho0)
-    out = {"operator": op, "f2c_filter": f2c_filter, "R_f64": r64}
+    out = {"operator": op, "f2c_filter": f2c_filter, "coupling_approach": scheme, "R_f64": r64}
     for prec in ("F32-SHIFTED", "F16-SHIFTED"):
         for coupling in ("F64", "F32-SHIFTED"):
             x = interface_reflection(acp, op, precision=prec, coupling=coupling, with_fine=False,
-                                     f2c_filter=f2c_filter)
+                                     f2c_filter=f2c_filter, scheme=scheme)
             out[f"{prec} coupling {coupling}"] = {
                 "probe_l2": float(np.linalg.norm(x["probe"] - p64) / sig),
                 "R": x["R_vs_uniform_coarse"],
@@ -154,7 +155,7 @@ def main():
     ap.add_argument("--precision", default="F64")
     ap.add_argument("--coupling", default="F64")
     ap.add_argument("--steps", type=int, default=None)
-    ap.add_argument("--f2c-filter", default="NOFI", choices=["NOFI", "LAG"])
+    ap.add_argument("--f2c-filter", default="NOFI", choices=["NOFI", "LAG", "TOU"])
     ap.add_argument("--scheme", default="STD", choices=["STD", "DC1", "DC2"])
     ap.add_argument("--geometry", default="duct", choices=sorted(GEOMETRY))
     ap.add_argument("--graph-chunk", type=int, default=0, help="also time graph replays of this many coarse steps (even; 0: no)")

@@ -301,8 +301,12 @@ int acp_refined_set_coupling_precision(acp_refined* r, int32_t precision);
 /* F2C-Filter (BoundaryConditions.txt:69; selected at Lattice.cpp:5179-5211):
  * NOFI hands each coarse node its coincident fine node's non-equilibrium, LAG
  * the mean non-equilibrium of that fine node and its 18 D3Q19 neighbours
- * (Lagrava et al. 2012).  The anisotropic filter TOU is not available. */
-enum { ACP_F2C_NOFI = 0, ACP_F2C_LAG = 1 };
+ * (Lagrava et al. 2012), TOU an anisotropic filter (Touil et al. 2014): each
+ * population's non-equilibrium averaged along its own lattice direction with
+ * weights 1/4, 1/2, 1/4 (the rest population unfiltered), cleared of the mass
+ * and momentum it carries so that the coarse node keeps the fine node's
+ * density and velocity. */
+enum { ACP_F2C_NOFI = 0, ACP_F2C_LAG = 1, ACP_F2C_TOU = 2 };
 int acp_refined_set_filter(acp_refined* r, int32_t filter);
 /* Coupling-Approach (BoundaryConditions.txt:52-53, 71-83): the standard
  * coupling (STD, the default; every link replaced: Coarse-to-Fine fAll,

@@ -312,14 +312,15 @@ public:
     }
     // The case file's coupling keys (BoundaryConditions.txt:52-90):
     // Coupling-Approach STD / DC1 / DC2 with Coarse-to-Fine, Fine-to-Coarse
-    // and Hanging-Node-Reconstruction, F2C-Filter NOFI / LAG and Scaling-Style
+    // and Hanging-Node-Reconstruction, F2C-Filter NOFI / LAG / TOU and Scaling-Style
     // SRTSTYLE.  A value the engine does not provide ends the program with the
     // key and the value, as the reference ends on a malformed case file.
     void apply_coupling_keys() {
         const std::string filt = bc_.getF2Cfilter();
-        if (filt != "NOFI" && filt != "LAG")
-            detail::die("Lattice::createLattice: F2C-Filter " + filt + " is not available (NOFI or LAG)\n");
-        detail::check(acp_refined_set_filter(r_, filt == "LAG" ? ACP_F2C_LAG : ACP_F2C_NOFI), "Lattice::createLattice");
+        if (filt != "NOFI" && filt != "LAG" && filt != "TOU")
+            detail::die("Lattice::createLattice: F2C-Filter " + filt + " (NOFI, LAG or TOU)\n");
+        detail::check(acp_refined_set_filter(r_, filt == "LAG" ? ACP_F2C_LAG : (filt == "TOU" ? ACP_F2C_TOU : ACP_F2C_NOFI)),
+                      "Lattice::createLattice");
         if (bc_.getScalingStyle() != "SRTSTYLE")
             detail::die("Lattice::createLattice: Scaling-Style " + bc_.getScalingStyle() +
                         " is not available (SRTSTYLE)\n");

@@ -1215,7 +1215,7 @@ struct acpo_refined {
     acpo_case* f;
     int x0, x1;
     double a;
-    int filter; /* 0 NOFI, 1 LAG */
+    int filter; /* 0 NOFI, 1 LAG, 2 TOU */
     int scheme; /* 0 standard coupling (STD), 1 direct coupling (DC) */
     int c2f, f2c, hang; /* DC link sets: 0 missing, 1 missing and common (MfP, McP, hMfP), 2 all (hAll) */
     /* HRR: strain rates handed between the levels, per coupled face:
@@ -1368,6 +1368,64 @@ static void lag19(const acpo_consts* k, int order, const acpo_case* f, int fx, i
     for (int q = 0; q < 19; ++q) v[q] = eq[q] + ainv * (acc[q] / 19.);
 }
 
+/* the direction opposite q (q < 18) */
+static int opposite(int q) {
+    for (int j = 0; j < 18; ++j)
+        if (E[j][0] == -E[q][0] && E[j][1] == -E[q][1] && E[j][2] == -E[q][2]) return j;
+    return 18;
+}
+
+/* F2C-Filter TOU (Touil, Ricot and Leveque 2014, doi
+ * 10.1016/j.jcp.2013.07.037; named at BoundaryConditions.txt:69, selected at
+ * Lattice.cpp:5179-5211, its arithmetic in the absent GridCoupling.cpp;
+ * restated, parity unpinned): an anisotropic filter of the fine
+ * non-equilibrium, each population along its own lattice direction.
+ * Population q < 18 takes the three-point binomial mean
+ *   nbar_q = (1/4 n_q(x - e_q) + 1/2 n_q(x)) + 1/4 n_q(x + e_q),
+ * n_q(y) = f_q(y) - feq_q(rho(y), u(y)) at fine node y; the rest population
+ * keeps n_18(x).  The filtered part is then cleared of the mass and momentum
+ * it carries, nbar_q -= w_q (M + xi_q . J / cs^2) with M = sum_q nbar_q,
+ * J = sum_q xi_q nbar_q, so that the coarse node keeps the density and
+ * velocity of its coincident fine node; the coarse node receives
+ * feq(x) + (1/a) nbar. */
+static void tou19(const acpo_consts* k, int order, const acpo_case* f, int fx, int fy, int fz, double ainv,
+                  double v[19]) {
+    const int fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
+    double rho, u[3], eq[19], n0[19], up[19], dn[19], g[19], e[19], nb[19];
+    const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
+    for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + l0];
+    acpo_moments(k, v, NULL, &rho, u);
+    acpo_feq(k, order, rho, u, eq);
+    for (int q = 0; q < 19; ++q) n0[q] = v[q] - eq[q];
+    for (int j = 0; j < 18; ++j) {
+        /* fine node x + e_j: population j one step ahead along e_j, population
+         * opposite(j) one step behind along its own direction */
+        const int x = fx + E[j][0], y = (fy + E[j][1] + fny) % fny, z = (fz + E[j][2] + fnz) % fnz;
+        const int64_t l = x + (int64_t)fnx * (y + (int64_t)fny * z);
+        double r2, u2[3];
+        for (int q = 0; q < 19; ++q) g[q] = f->A[(size_t)q * f->N + l];
+        acpo_moments(k, g, NULL, &r2, u2);
+        acpo_feq(k, order, r2, u2, e);
+        const int o = opposite(j);
+        up[j] = g[j] - e[j];
+        dn[o] = g[o] - e[o];
+    }
+    for (int q = 0; q < 18; ++q) nb[q] = (0.25 * dn[q] + 0.5 * n0[q]) + 0.25 * up[q];
+    nb[18] = n0[18];
+    double M = 0., J[3] = {0., 0., 0.};
+    for (int q = 0; q < 19; ++q) {
+        M += nb[q];
+        for (int a = 0; a < 3; ++a)
+            if (E[q][a]) J[a] += k->xi[q][a] * nb[q];
+    }
+    for (int q = 0; q < 19; ++q) {
+        double s = M;
+        for (int a = 0; a < 3; ++a)
+            if (E[q][a]) s += k->rcs2 * (k->xi[q][a] * J[a]);
+        v[q] = eq[q] + ainv * (nb[q] - k->w[q] * s);
+    }
+}
+
 static void refined_f2c(acpo_refined* r) {
     acpo_case* c = r->c;
     acpo_case* f = r->f;
@@ -1383,6 +1441,8 @@ static void refined_f2c(acpo_refined* r) {
                 double v[19];
                 if (r->filter == 1) {
                     lag19(&c->k, c->p.order, f, 2 * (I - r->x0), 2 * J, 2 * K, ainv, v);
+                } else if (r->filter == 2) {
+                    tou19(&c->k, c->p.order, f, 2 * (I - r->x0), 2 * J, 2 * K, ainv, v);
                 } else {
                     for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + lf];
                     rescale19(&c->k, c->p.order, v, ainv);
@@ -1568,7 +1628,7 @@ int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int
 }
 
 int acpo_refined_set_filter(acpo_refined* r, int filter) {
-    if (filter != 0 && filter != 1) return -1;
+    if (filter < 0 || filter > 2) return -1;
     r->filter = filter;
     return 0;
 }

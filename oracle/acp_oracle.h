@@ -139,7 +139,7 @@ void acpo_refined_step(acpo_refined* r, int n);
 /* the same, recording sum_q f_q at coarse nodes probe_idx after each coarse
  * step, before the coupling writes into the coarse level */
 void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out);
-/* fine-to-coarse filter: 0 NOFI (default), 1 LAG; -1 for any other value */
+/* fine-to-coarse filter: 0 NOFI (default), 1 LAG, 2 TOU; -1 for any other value */
 int acpo_refined_set_filter(acpo_refined* r, int filter);
 /* coupling scheme: 0 standard (STD, default: every link, c2f = f2c = hang =
  * 2), 1 direct coupling (DC); link sets 0 missing (Mf, Mc, hMf), 1 missing and

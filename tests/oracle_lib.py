@@ -234,8 +234,8 @@ class OracleRefined:
         return out.reshape(n, len(probes))
 
     def set_filter(self, name):
-        """Fine-to-coarse filter: "NOFI" or "LAG"."""
-        if lib().acpo_refined_set_filter(self.h, {"NOFI": 0, "LAG": 1}[name]) != 0:
+        """Fine-to-coarse filter: "NOFI", "LAG" or "TOU"."""
+        if lib().acpo_refined_set_filter(self.h, {"NOFI": 0, "LAG": 1, "TOU": 2}[name]) != 0:
             raise ValueError(name)
 
     def set_coupling(self, scheme="STD", c2f="fAll", f2c="cAll", hanging="hAll"):

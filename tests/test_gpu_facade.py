@@ -118,10 +118,11 @@ def _refined_text(nx, ny, nz, op, region, steps, coupling=None):
     return txt + f"Boundary LODI-X\nRefined-Region {region[0]} {region[1]}\n"
 
 
-COUPLINGS = [None, ("DC1", "Mf", "Mc", "hMf", "LAG"), ("DC2", "MfP", "Mc", "hMfP", "NOFI")]
+COUPLINGS = [None, ("DC1", "Mf", "Mc", "hMf", "LAG"), ("DC2", "MfP", "Mc", "hMfP", "NOFI"),
+             ("STD", "fAll", "cAll", "hAll", "TOU")]
 
 
-@pytest.mark.parametrize("coupling", COUPLINGS, ids=["STD-NOFI", "DC1-LAG", "DC2-MfP"])
+@pytest.mark.parametrize("coupling", COUPLINGS, ids=["STD-NOFI", "DC1-LAG", "DC2-MfP", "STD-TOU"])
 @pytest.mark.parametrize("op", ["RR", "HRR"])
 def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, coupling):
     """MaxLevel 1 with this engine's refined duct geometry: the reference's
@@ -164,12 +165,12 @@ def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, couplin
     assert rec["node"][(1, fnx - 1, 3, 1)][1] == "boundary"
 
 
-@pytest.mark.parametrize("key,value", [("F2C-Filter NOFI", "F2C-Filter TOU"), ("Scaling-Style SRTSTYLE", "Scaling-Style MRTSTYLE"),
+@pytest.mark.parametrize("key,value", [("F2C-Filter NOFI", "F2C-Filter XYZ"), ("Scaling-Style SRTSTYLE", "Scaling-Style MRTSTYLE"),
                                        ("Coupling-Approach STD", "Coupling-Approach XYZ"),
                                        ("Coarse-to-Fine fAll", "Coarse-to-Fine Mf")])
 def test_two_level_coupling_keys_the_engine_lacks_end_the_program(gpu, tmp_path, key, value):
-    """A two-level case file asking for what the engine does not provide (the
-    TOU filter, moment-based scaling, an unknown scheme, or the standard
+    """A two-level case file asking for what the engine does not provide (an
+    unknown filter, moment-based scaling, an unknown scheme, or the standard
     coupling with only the missing links) ends the program naming the key."""
     f = tmp_path / "bc.txt"
     f.write_text(_refined_text(24, 8, 3, "RR", (12, 23), 2).replace(key, value))

@@ -146,12 +146,92 @@ def test_lag_is_the_19_point_mean_of_the_nonequilibrium(axis):
     assert not np.any(lag[:, :, :, : x0 + 1]) and not np.any(lag[:, :, :, x1:])
 
 
-def test_lag_interface_reflection_is_small():
-    """The interface-reflection check of the NOFI coupling, with LAG."""
+E19 = np.array([(1, 0, 0), (0, 1, 0), (-1, 0, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 0, 1), (1, 0, -1),
+                (0, 1, 1), (0, 1, -1), (-1, 0, 1), (-1, 0, -1), (0, -1, 1), (0, -1, -1), (-1, 1, 0), (1, -1, 0),
+                (1, 1, 0), (-1, -1, 0), (0, 0, 0)])
+
+
+def _tou_case():
+    nx, ny, nz, x0, x1 = 20, 8, 3, 6, 14
+    p = ol.make_params(nx, ny, nz, op="SRT", periodic=True, tau=0.6, nthreads=4)
+    fp = ol.make_params(2 * (x1 - x0) + 1, 2 * ny, 2 * nz, op="SRT", periodic=2, tau=0.6, nthreads=4)
+    return p, fp, x0, x1
+
+
+@pytest.mark.parametrize("axis,shear,factor", [("x", (0, 1), 0.5), ("z", (0, 1), 0.0), ("y", (1, 2), 0.5),
+                                               ("x", (1, 2), 0.0)])
+def test_tou_filters_each_population_along_its_own_direction(axis, shear, factor):
+    """TOU: a non-equilibrium part h * beta * (s - S)^2 quadratic along one axis
+    s, h a pure shear (h_q = w_q xi_a xi_b, no mass, no momentum, so every
+    node has the same equilibrium).  At the coarse node (s = S) the fine value
+    is zero; the three-point mean along e_q gives beta h_q e_qs^2 / 2, which is
+    h_q beta / 2 on the shear's links when they all move along s (xy along x,
+    yz along y) and zero when none does (xy along z, yz along x), and carries
+    no mass or momentum.  So TOU - NOFI = factor * beta * h / a exactly in
+    real arithmetic; LAG (isotropic) would give 10/19 in both."""
+    p, fp, x0, x1 = _tou_case()
+    a_, b_ = shear
+    h = p.rho0 * W * (E19[:, a_] * E19[:, b_]) * 1e-3
+    fnx, fny, fnz = fp.nx, fp.ny, fp.nz
+    k, j, i = np.meshgrid(np.arange(fnz), np.arange(fny), np.arange(fnx), indexing="ij")
+    s = {"x": i, "y": j, "z": k}[axis].astype(float)
+    feq0 = ol.equilibrium_field(fp, np.full(1, p.rho0), np.zeros((3, 1)))[:, 0]
+    beta = 0.01
+    I, J, K = x0 + 2, 2, 1  # the curvature is centred on this coarse node's fine partner
+    S = {"x": 2 * (I - x0), "y": 2 * J, "z": 2 * K}[axis]
+    f = feq0[:, None] + h[:, None] * beta * (s.reshape(-1)[None, :] - S) ** 2
+    tou, a = _restricted("TOU", f, p, x0, x1)
+    nofi, _ = _restricted("NOFI", f, p, x0, x1)
+    d = tou[:, K, J, I] - nofi[:, K, J, I]
+    want = factor * beta * h / a
+    assert np.max(np.abs(d - want)) <= 1e-9 * np.max(np.abs(h / a))
+
+
+def test_tou_keeps_the_coincident_density_and_velocity():
+    """Random fine states: every restricted coarse node has the density and
+    velocity of its coincident fine node (the filtered non-equilibrium carries
+    no mass or momentum), and TOU differs from NOFI and from LAG."""
+    p, fp, x0, x1 = _tou_case()
+    rng = np.random.default_rng(11)
+    FN = fp.nx * fp.ny * fp.nz
+    f = p.rho0 * W[:, None] * (1.0 + 1e-3 * rng.standard_normal((19, FN)))
+    tou, _ = _restricted("TOU", f, p, x0, x1)
+    nofi, _ = _restricted("NOFI", f, p, x0, x1)
+    lag, _ = _restricted("LAG", f, p, x0, x1)
+    fr = f.reshape(19, fp.nz, fp.ny, fp.nx)
+    for I in range(x0 + 1, x1):
+        for J in range(p.ny):
+            for K in range(p.nz):
+                mc = ol.macros(p, tou[:, K, J, I][:, None])
+                mf = ol.macros(fp, fr[:, 2 * K, 2 * J, 2 * (I - x0)][:, None])
+                assert np.allclose(np.array(mc[0]), np.array(mf[0]), rtol=1e-14, atol=0)
+                for a_ in range(3):
+                    assert abs(mc[1][a_][0] - mf[1][a_][0]) <= 1e-13 * p.cs
+    inside = (slice(None), slice(None), slice(None), slice(x0 + 1, x1))
+    assert not np.allclose(tou[inside], nofi[inside], rtol=1e-12, atol=0)
+    assert not np.allclose(tou[inside], lag[inside], rtol=1e-12, atol=0)
+
+
+@pytest.mark.parametrize("op,moving", [("SRT", True), ("RR", False), ("HRR", True)])
+def test_tou_uniform_state_stays_uniform(op, moving):
+    r, p = refined(op)
+    r.set_filter("TOU")
+    u = np.array([0.02 * p.cs, -0.01 * p.cs, 0.0]) if moving else np.zeros(3)
+    for lv in r.levels:
+        lv.set_state(ol.equilibrium_field(lv.p, np.full(lv.N, p.rho0), np.repeat(u[:, None], lv.N, axis=1)))
+    r.step(6)
+    ref = r.levels[0].state()[:, 0]
+    for lv in r.levels:
+        assert np.max(np.abs(lv.state() - ref[:, None])) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("filt", ["LAG", "TOU"])
+def test_filtered_interface_reflection_is_small(filt):
+    """The interface-reflection check of the NOFI coupling, with LAG and TOU."""
     nx, ny, x0, x1, steps = 64, 32, 32, 64, 60
     p = ol.make_params(nx, ny, 3, op="SRT", periodic=True, tau=0.55, nthreads=8)
     r = ol.OracleRefined(p, x0, x1)
-    r.set_filter("LAG")
+    r.set_filter(filt)
     amp, cx, cy, b = 1e-3, 16.0, 16.0, 4.0
     r.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
     r.levels[1].set_state(ol.pulse_ic(r.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
