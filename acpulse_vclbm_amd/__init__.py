@@ -258,6 +258,10 @@ class Refined:
         """Replay `chunk` coarse steps (even) from captured HIP graphs; 0 turns it off."""
         check(lib().acp_refined_enable_graphs(self.h, int(chunk)))
 
+    def write_vtk(self, stem):
+        """<stem>.vtk of both levels (Lattice::writeResultsVTK of a two-level lattice)."""
+        check(lib().acp_refined_write_vtk(self.h, str(stem).encode()))
+
     def checksum(self):
         """Order-independent checksum of both levels' states (coarse + fine, mod 2^64)."""
         return (self._levels[0].checksum() + self._levels[1].checksum()) % (1 << 64)

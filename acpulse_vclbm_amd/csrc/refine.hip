@@ -50,6 +50,7 @@
 #include <cstring>
 #include <initializer_list>
 #include <type_traits>
+#include <vector>
 
 #include "acp.h"
 #include "engine.hpp"
@@ -1603,6 +1604,11 @@ extern "C" int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32
     return refined_create(coarse, x0, x1, stream, nullptr, 0, out);
 }
 
+extern "C" int acp_refined_fine_desc(const acp_case_desc* coarse, int32_t x0, int32_t x1, acp_case_desc* fine) {
+    if (!coarse || !fine) return fail(ACP_E_ARG, "acp_refined_fine_desc: null argument");
+    return fine_desc(*coarse, x0, x1, *fine);
+}
+
 extern "C" int acp_refined_footprint(const acp_case_desc* coarse, int32_t x0, int32_t x1, int32_t nprobe,
                                      int32_t probe_capacity, int64_t* bytes) {
     if (!coarse || !bytes) return fail(ACP_E_ARG, "acp_refined_footprint: null argument");
@@ -1732,6 +1738,25 @@ extern "C" int acp_refined_init_reference(acp_refined* r) {
     // the fine level's first node is coarse node x0, fine index 2 x0 of the domain
     if (rc == ACP_OK) rc = acp::init_reference_level(r->fine, 1, 2 * r->x0, r->coarse->d);
     return rc;
+}
+
+extern "C" int acp_refined_write_vtk(acp_refined* r, const char* stem) {
+    if (!r || !stem) return fail(ACP_E_ARG, "acp_refined_write_vtk: null argument");
+    if (!r->coarse->initialized || !r->fine->initialized) return fail(ACP_E_STATE, "refined case not initialised");
+    std::vector<double> arr[2][2];  // [level][current, other]
+    acp_case* lv[2] = {r->coarse, r->fine};
+    for (int l = 0; l < 2; ++l) {
+        const size_t n = (size_t)19 * lv[l]->g.N;
+        arr[l][0].resize(n);
+        if (int rc = acp_case_get_populations(lv[l], arr[l][0].data())) return rc;
+        if (lv[l]->d.nonreflecting) {
+            arr[l][1].resize(n);
+            if (int rc = acp_case_get_previous_populations(lv[l], arr[l][1].data())) return rc;
+        }
+    }
+    auto other = [&](int l) { return arr[l][1].empty() ? arr[l][0].data() : arr[l][1].data(); };
+    return acp::write_vtk2_host(r->coarse->d, r->coarse->hk, arr[0][0].data(), other(0), r->fine->d, r->fine->hk,
+                                arr[1][0].data(), other(1), r->x0, r->x1, stem);
 }
 
 extern "C" int acp_refined_step(acp_refined* r, int32_t n) {

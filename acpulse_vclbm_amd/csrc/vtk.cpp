@@ -1,4 +1,4 @@
-// Single-level VTK output with the byte layout of Lattice::writeResultsVTK
+// VTK output with the byte layout of Lattice::writeResultsVTK
 // (Lattice.cpp:2212-2243 header, 4094-4349 single level): legacy ASCII
 // UNSTRUCTURED_GRID of VOXEL cells with POINT_DATA Pressure, Density, Omega and
 // Velocity, numbers as C++ streams print them with scientific | showpoint and
@@ -35,6 +35,183 @@ struct Out {
 }  // namespace
 
 namespace acp {
+
+namespace {
+
+// Whether node (i, j, k) of a level is a LODI boundary node (of its boundary kind).
+bool lodi_node(const acp_case_desc& d, int i, int j, int kz) {
+    if (d.boundary == ACP_BC_LODI)
+        return i == 0 || j == 0 || kz == 0 || i == d.nx - 1 || j == d.ny - 1 || kz == d.nz - 1;
+    if (d.boundary == ACP_BC_LODI_X) return i == 0 || i == d.nx - 1;
+    return d.boundary == ACP_BC_OPEN_LODI_X && i == d.nx - 1;
+}
+
+// Density and velocity of every node of a level as calcDensity / calcVelocity
+// print them; LODI nodes with i > 0 from the other array under Non-Reflecting.
+void level_macros(const acp_case_desc& d, const K64& k, const double* fcur, const double* fprev, const double* psi,
+                  std::vector<double>& rho, std::vector<double>& ux, std::vector<double>& uy, std::vector<double>& uz) {
+    const int nx = d.nx, ny = d.ny, nz = d.nz;
+    const int64_t N = (int64_t)nx * ny * nz;
+    rho.resize((size_t)N), ux.resize((size_t)N), uy.resize((size_t)N), uz.resize((size_t)N);
+    for (int kz = 0; kz < nz; ++kz)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t l = i + (int64_t)nx * (j + (int64_t)ny * kz);
+                const double* F = (d.nonreflecting && i > 0 && lodi_node(d, i, j, kz)) ? fprev : fcur;
+                double f[19], ps[19];
+                for (int q = 0; q < 19; ++q) f[q] = F[(size_t)q * N + l];
+                if (d.cubic) {
+                    for (int q = 0; q < 19; ++q) ps[q] = psi ? psi[(size_t)q * N + l] : 0.;
+                    moments64_psi(k, f, ps, rho[l], ux[l], uy[l], uz[l]);
+                } else {
+                    moments64(k, f, rho[l], ux[l], uy[l], uz[l]);
+                }
+            }
+}
+
+int write_file(const std::string& text, const char* stem) {
+    std::string fname = std::string(stem) + ".vtk";
+    std::ofstream fh(fname, std::ios::binary);
+    if (!fh) return fail(ACP_E_IO, "An error occured while writing paraview file " + fname);
+    fh.write(text.data(), (std::streamsize)text.size());
+    if (!fh) return fail(ACP_E_IO, "short write " + fname);
+    return ACP_OK;
+}
+
+void voxel(Out& o, long long n0, long long sx, long long sxy) {
+    const long long v[8] = {n0, n0 + 1, n0 + sx, n0 + sx + 1, n0 + sxy, n0 + sxy + 1, n0 + sxy + sx, n0 + sxy + sx + 1};
+    o.str("8 ");
+    for (int a = 0; a < 8; ++a) {
+        o.inum(v[a]);
+        o.str(a == 7 ? "\n" : " ");
+    }
+}
+
+}  // namespace
+
+// Two-level output (the multi-level branch of Lattice::writeResultsVTK,
+// Lattice.cpp:2243-4092, for this engine's geometry): the reference writes
+// the fine regions' nodes first and then the coarse nodes of the unrefined
+// core, interface nodes on both levels, VOXEL cells of each level between its
+// printed nodes, and POINT_DATA Omega, NodeID, Pressure, Density, Velocity in
+// that point order.  Here the fine level covers coarse x in [x0, x1]: its
+// nodes come first (k, j, i ascending, x = (2 x0 + i) spacing / 2), then the
+// coarse nodes outside the fine region's interior, columns 0..x0 and x1..nx-1
+// (none beyond x1 when x1 = nx wraps a periodic level).  NodeID: 0 fluid, 1
+// LODI boundary, 2 grid-coupling (the fine level's coupled faces).
+int write_vtk2_host(const acp_case_desc& cd, const K64& ck, const double* ccur, const double* cprev,
+                    const acp_case_desc& fd, const K64& fk, const double* fcur, const double* fprev, int x0, int x1,
+                    const char* stem) {
+    const int nx = cd.nx, ny = cd.ny, nz = cd.nz, fnx = fd.nx, fny = fd.ny, fnz = fd.nz;
+    const int64_t FN = (int64_t)fnx * fny * fnz;
+    std::vector<int> cols;
+    for (int i = 0; i <= x0; ++i) cols.push_back(i);
+    for (int i = x1; i < nx; ++i) cols.push_back(i);
+    const long long ncol = (long long)cols.size();
+    const long long CP = ncol * ny * nz, P = FN + CP;
+    const double sp0 = cd.spacing / std::pow(2., 0), sp1 = cd.spacing / std::pow(2., 1);
+    Out o;
+    o.s.reserve((size_t)P * 160 + 1024);
+    o.str("# vtk DataFile Version 2.0\n");
+    o.str("LBM results for grid refinement benchmark in channel flow\n");
+    o.str("ASCII\n");
+    o.str("DATASET UNSTRUCTURED_GRID\n");
+    o.str("POINTS ");
+    const size_t pnt_pos = o.s.size();
+    o.str((std::string(14, ' ') + " float\n").c_str());
+    auto point = [&](double x, double y, double z) {
+        o.num(x);
+        o.str(" ");
+        o.num(y);
+        o.str(" ");
+        o.num(z);
+        o.str("\n");
+    };
+    for (int kz = 0; kz < fnz; ++kz)
+        for (int j = 0; j < fny; ++j)
+            for (int i = 0; i < fnx; ++i) point((2 * x0 + i) * sp1, j * sp1, kz * sp1);
+    for (int kz = 0; kz < nz; ++kz)
+        for (int j = 0; j < ny; ++j)
+            for (int i : cols) point(i * sp0, j * sp0, kz * sp0);
+    o.str("\n");
+    o.str("CELLS ");
+    const size_t cell_pos = o.s.size();
+    o.str((std::string(9, ' ') + " " + std::string(9, ' ') + "\n").c_str());
+    long long cells = 0;
+    for (int kz = 0; kz < fnz - 1; ++kz)
+        for (int j = 0; j < fny - 1; ++j)
+            for (int i = 0; i < fnx - 1; ++i, ++cells)
+                voxel(o, i + (long long)fnx * (j + (long long)fny * kz), fnx, (long long)fnx * fny);
+    for (int kz = 0; kz < nz - 1; ++kz)
+        for (int j = 0; j < ny - 1; ++j)
+            for (long long c = 0; c + 1 < ncol; ++c) {
+                if (cols[c + 1] != cols[c] + 1) continue;  // no cell across the fine region
+                voxel(o, FN + c + ncol * (j + (long long)ny * kz), ncol, ncol * ny);
+                ++cells;
+            }
+    o.str("\n");
+    o.str("CELL_TYPES ");
+    o.inum(cells);
+    o.str("\n");
+    for (long long c = 0; c < cells; ++c) o.str("11\n");
+    o.str("\n");
+    o.str("POINT_DATA ");
+    o.inum(P);
+    o.str("\n");
+    std::vector<double> fr, fu, fv, fw, cr, cu, cv, cw;
+    level_macros(fd, fk, fcur, fprev, nullptr, fr, fu, fv, fw);
+    level_macros(cd, ck, ccur, cprev, nullptr, cr, cu, cv, cw);
+    // every printed point of both levels in the POINTS order: fn(level, l)
+    auto each = [&](auto fn) {
+        for (int64_t l = 0; l < FN; ++l) fn(1, l, (int)(l % fnx));
+        for (int kz = 0; kz < nz; ++kz)
+            for (int j = 0; j < ny; ++j)
+                for (int i : cols) fn(0, i + (int64_t)nx * (j + (int64_t)ny * kz), i);
+    };
+    o.str("SCALARS Omega float 1\n");
+    o.str("LOOKUP_TABLE default\n");
+    each([&](int lv, int64_t, int) {
+        o.num(lv ? fk.omega : ck.omega);
+        o.str("\n");
+    });
+    o.str("SCALARS NodeID float 1\n");
+    o.str("LOOKUP_TABLE default\n");
+    each([&](int lv, int64_t l, int i) {
+        const acp_case_desc& d = lv ? fd : cd;
+        const int j = (int)((l / d.nx) % d.ny), kz = (int)(l / ((int64_t)d.nx * d.ny));
+        int id = lodi_node(d, i, j, kz) ? 1 : 0;
+        if (lv && !id && (i == 0 || (fd.boundary == ACP_BC_OPEN_X && i == fnx - 1))) id = 2;
+        o.inum(id);
+        o.str("\n");
+    });
+    const double cs2 = std::pow(cd.cs, 2.);
+    o.str("SCALARS Pressure float 1\n");
+    o.str("LOOKUP_TABLE default\n");
+    each([&](int lv, int64_t l, int) {
+        o.num((lv ? fr : cr)[l] * cs2);
+        o.str("\n");
+    });
+    o.str("SCALARS Density float 1\n");
+    o.str("LOOKUP_TABLE default\n");
+    each([&](int lv, int64_t l, int) {
+        o.num((lv ? fr : cr)[l]);
+        o.str("\n");
+    });
+    o.str("\nVECTORS Velocity float \n");
+    each([&](int lv, int64_t l, int) {
+        o.num((lv ? fu : cu)[l]);
+        o.str(" ");
+        o.num((lv ? fv : cv)[l]);
+        o.str(" ");
+        o.num((lv ? fw : cw)[l]);
+        o.str(" \n");
+    });
+    const std::string pc = std::to_string(P);
+    o.s.replace(pnt_pos, pc.size(), pc);
+    const std::string cc = std::to_string(cells) + " " + std::to_string(9 * cells);
+    o.s.replace(cell_pos, cc.size(), cc);
+    return write_file(o.s, stem);
+}
 
 int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, const double* fprev, const double* psi,
                    const char* stem) {
@@ -90,28 +267,8 @@ int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, con
     o.str("\n");
 
     // density and velocity as calcDensity / calcVelocity print them
-    std::vector<double> rho((size_t)N), ux((size_t)N), uy((size_t)N), uz((size_t)N);
-    for (int kz = 0; kz < nz; ++kz)
-        for (int j = 0; j < ny; ++j)
-            for (int i = 0; i < nx; ++i) {
-                const int64_t l = i + (int64_t)nx * (j + (int64_t)ny * kz);
-                bool bnd;
-                if (d.boundary == ACP_BC_LODI)
-                    bnd = i == 0 || j == 0 || kz == 0 || i == nx - 1 || j == ny - 1 || kz == nz - 1;
-                else if (d.boundary == ACP_BC_LODI_X)
-                    bnd = i == 0 || i == nx - 1;
-                else
-                    bnd = d.boundary == ACP_BC_OPEN_LODI_X && i == nx - 1;
-                const double* F = (d.nonreflecting && i > 0 && bnd) ? fprev : fcur;
-                double f[19], ps[19];
-                for (int q = 0; q < 19; ++q) f[q] = F[(size_t)q * N + l];
-                if (d.cubic) {
-                    for (int q = 0; q < 19; ++q) ps[q] = psi ? psi[(size_t)q * N + l] : 0.;
-                    moments64_psi(k, f, ps, rho[l], ux[l], uy[l], uz[l]);
-                } else {
-                    moments64(k, f, rho[l], ux[l], uy[l], uz[l]);
-                }
-            }
+    std::vector<double> rho, ux, uy, uz;
+    level_macros(d, k, fcur, fprev, psi, rho, ux, uy, uz);
     const double cs2 = std::pow(d.cs, 2.);
     o.str("SCALARS Pressure float 1\n");
     o.str("LOOKUP_TABLE default\n");
@@ -146,12 +303,7 @@ int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, con
     o.s.replace(pnt_pos, pc.size(), pc);
     const std::string cc = std::to_string(cells) + " " + std::to_string(9 * cells);
     o.s.replace(cell_pos, cc.size(), cc);
-    std::string fname = std::string(stem) + ".vtk";
-    std::ofstream fh(fname, std::ios::binary);
-    if (!fh) return fail(ACP_E_IO, "An error occured while writing paraview file " + fname);
-    fh.write(o.s.data(), (std::streamsize)o.s.size());
-    if (!fh) return fail(ACP_E_IO, "short write " + fname);
-    return ACP_OK;
+    return write_file(o.s, stem);
 }
 
 }  // namespace acp
@@ -163,4 +315,17 @@ extern "C" int acp_write_vtk_host(const acp_case_desc* d, const double* fcur, co
     acp::K64 k;
     acp::k64_init(k, *d);
     return acp::write_vtk_host(*d, k, fcur, fprev, psi, stem);
+}
+
+extern "C" int acp_write_vtk2_host(const acp_case_desc* coarse, const double* ccur, const double* cprev, int32_t x0,
+                                   int32_t x1, const double* fcur, const double* fprev, const char* stem) {
+    if (!coarse || !ccur || !fcur || !stem) return acp::fail(ACP_E_ARG, "acp_write_vtk2_host: null argument");
+    if (coarse->nonreflecting && (!cprev || !fprev))
+        return acp::fail(ACP_E_ARG, "Non-Reflecting output needs the previous arrays");
+    acp_case_desc fd;
+    if (int rc = acp_refined_fine_desc(coarse, x0, x1, &fd)) return rc;
+    acp::K64 ck, fk;
+    acp::k64_init(ck, *coarse);
+    acp::k64_init(fk, fd);
+    return acp::write_vtk2_host(*coarse, ck, ccur, cprev, fd, fk, fcur, fprev, x0, x1, stem);
 }

@@ -294,6 +294,19 @@ int acp_refined_init_pulse(acp_refined* r, double amp, double x0, double y0, dou
 int acp_refined_init_reference(acp_refined* r);
 /* n coarse steps (2n fine steps). */
 int acp_refined_step(acp_refined* r, int32_t n);
+/* Lattice::writeResultsVTK of a two-level lattice (Lattice.cpp:2212-4092,
+ * multi-level branch): <stem>.vtk with the fine level's nodes first, then the
+ * coarse nodes outside the fine region's interior (columns 0..x0 and
+ * x1..nx-1), VOXEL cells of each level, POINT_DATA Omega, NodeID (0 fluid, 1
+ * LODI boundary, 2 grid-coupling), Pressure, Density and Velocity in that
+ * point order, numbers as the single-level writer prints them. */
+int acp_refined_write_vtk(acp_refined* r, const char* stem);
+/* The same from host arrays (the coarse level's description and current /
+ * other arrays, [19][N] each; the other arrays only under Non-Reflecting). */
+int acp_write_vtk2_host(const acp_case_desc* coarse, const double* ccur, const double* cprev, int32_t x0, int32_t x1,
+                        const double* fcur, const double* fprev, const char* stem);
+/* The fine level's description of a refined case (acp_refined_create's checks). */
+int acp_refined_fine_desc(const acp_case_desc* coarse, int32_t x0, int32_t x1, acp_case_desc* fine);
 /* Shifted tiers: run the coupling arithmetic (time interpolation, rescaling,
  * restriction) in float64 on SI populations (ACP_PREC_F64, the default) or in
  * float32 on lattice-unit deviations (ACP_PREC_F32_SHIFTED). */

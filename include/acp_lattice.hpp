@@ -411,13 +411,13 @@ public:
         nested(ACP_OP_HRR, "nestedTimeSteppingHRR", alternatingL1, alternatingL0);
     }
 
-    // Lattice.cpp:2212-2243, 4094-4349: <filename>.vtk from the newest state
+    // Lattice.cpp:2212-4349: <filename>.vtk from the newest state of every
+    // level (the two-level layout of acp_refined_write_vtk)
     virtual void writeResultsVTK(std::string filename, bool& alternating, bool& /*alternatingL1*/) {
         if (!both_[0] && alternating != cur_[0])
             detail::die("Lattice::writeResultsVTK: the flag names the array of the previous step\n");
-        if (r_) detail::die("Lattice::writeResultsVTK: two-level output is not available\n");
-        if (acp_case_write_vtk(lv_[0], filename.c_str()) != ACP_OK)
-            std::cerr << "An error occured while writing paraview file\n";
+        const int rc = r_ ? acp_refined_write_vtk(r_, filename.c_str()) : acp_case_write_vtk(lv_[0], filename.c_str());
+        if (rc != ACP_OK) std::cerr << "An error occured while writing paraview file\n";
     }
 
     // Lattice.h:119: population dir of node (i, j, k) of `level` from the

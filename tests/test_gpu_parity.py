@@ -586,3 +586,58 @@ def test_boundary_rows_through_lds_match_oracle_bitexact(gpu, op, order, bnd, cu
                 c.step(3)
                 rho, _ = c.macros()
             assert np.max(np.abs(rho - r64)) <= tol * np.max(np.abs(r64 - p.rho0)), prec
+
+
+@pytest.mark.parametrize("coupling", ["STD", "DC1"])
+def test_acp_run_driver_two_level(gpu, tmp_path, coupling):
+    """The driver on a MaxLevel 1 case file (this engine's refined geometry:
+    Boundary LODI-X, Refined-Region 12 23, the fine level reaching the high-x
+    face): the reference's two-level summary, <Save-as>_<t>.vtk at t = 0 and
+    every WriteInterval, and the last file equal to the two-level writer's
+    output of the oracle's refined run of the same case (the reference's
+    initial pulse on both levels)."""
+    import subprocess
+    from make_golden import bc_text
+    nx, ny, nz, steps, x0, x1 = 24, 10, 3, 4, 12, 23
+    txt = bc_text(nx, ny, nz, op="RR", order=3, tau=0.56, spacing=0.02, steps=steps).replace("MaxLevel 0", "MaxLevel 1")
+    txt = txt.replace("WriteInterval 1000000", "WriteInterval 2").replace("Save-as /tmp/acp_golden",
+                                                                            f"Save-as {tmp_path}/run")
+    links = ("Mf", "Mc", "hMf") if coupling != "STD" else ("fAll", "cAll", "hAll")
+    txt = (txt.replace("Coupling-Approach STD", f"Coupling-Approach {coupling}")
+           .replace("Coarse-to-Fine fAll", f"Coarse-to-Fine {links[0]}")
+           .replace("Fine-to-Coarse cAll", f"Fine-to-Coarse {links[1]}")
+           .replace("Hanging-Node-Reconstruction hAll", f"Hanging-Node-Reconstruction {links[2]}")
+           .replace("F2C-Filter NOFI", "F2C-Filter LAG"))
+    txt += f"Boundary LODI-X\nRefined-Region {x0} {x1}\n"
+    (tmp_path / "bc.txt").write_text(txt)
+    exe = os.path.join(os.path.dirname(acp._capi.LIB_PATH), "acp_run")
+    r = subprocess.run([exe, str(tmp_path / "bc.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "dimensionless collision frequency on fine grid" in r.stdout
+    assert f"coupling approach = {coupling}" in r.stdout and "f2c filtering = LAG" in r.stdout
+    assert "solved 100.00% of the case" in r.stdout
+    for t in (0, 2, 4):
+        assert os.path.exists(tmp_path / f"run_{t}.vtk")
+    d = acp.read_case_file(str(tmp_path / "bc.txt"))
+    d.max_level = 0
+    p0 = ol.make_params(nx, ny, nz, op="RR", order=3, tau=0.56, spacing=0.02, periodic=3)
+    o = ol.OracleRefined(p0, x0, x1)
+    o.set_filter("LAG")
+    o.set_coupling(coupling, *links)
+    p1 = o.levels[1].p
+    o.levels[0].set_state(ol.reference_ic_level(p0, 0, 0, 0.02, d.domain[0], d.domain[1]))
+    o.levels[1].set_state(ol.reference_ic_level(p1, 1, 2 * x0, 0.02, d.domain[0], d.domain[1]))
+    o.step(steps)
+    D = acp._capi.dptr
+    st = [(o.levels[lv].state(), o.levels[lv].prev()) for lv in (0, 1)]
+    acp._capi.check(acp.lib().acp_write_vtk2_host(d, D(st[0][0]), D(st[0][1]), x0, x1, D(st[1][0]), D(st[1][1]),
+                                                   str(tmp_path / "oracle").encode()))
+    assert open(tmp_path / "run_4.vtk", "rb").read() == open(tmp_path / "oracle.vtk", "rb").read()
+    # and the library's own two-level writer on the GPU state gives the same bytes
+    with acp.Refined(d, x0, x1) as rc:
+        rc.set_filter("LAG")
+        rc.set_coupling(coupling, *links)
+        rc.init_reference()
+        rc.step(steps)
+        rc.write_vtk(str(tmp_path / "gpu"))
+    assert open(tmp_path / "gpu.vtk", "rb").read() == open(tmp_path / "oracle.vtk", "rb").read()
