@@ -58,25 +58,25 @@ void k64_init(K64& k, const acp_case_desc& d) {
     k.pA = std::pow(k.c, 2.) - k.cs2;
     k.pB = std::pow(0., 2.) - k.cs2;
     k.cc = k.c * k.c;
+    // Node.cpp:1016-1052: xi xi - cs^2 is hA for a non-zero component and hB
+    // for a zero one; the third-order values are sums and differences of
+    // (xi xi - cs^2) xi_b, the fourth-order ones depend on which components
+    // are non-zero (h3_of, h4_pattern)
+    const double PA = k.hA * k.c, PB = k.hB * k.c;
+    k.h3m[0] = PB + PB;
+    k.h3m[1] = PA + PB;
+    k.h3m[2] = PA + PA;
+    k.h3m[3] = PA - PB;
     const double o6 = 1. / 6., t7 = 2. / 7., t5 = 2. / 5.;
-    for (int q = 0; q < 19; ++q) {
-        const double x = EX[q] == 0 ? 0. : (EX[q] > 0 ? k.c : -k.c);
-        const double y = EY[q] == 0 ? 0. : (EY[q] > 0 ? k.c : -k.c);
-        const double z = EZ[q] == 0 ? 0. : (EZ[q] > 0 ? k.c : -k.c);
-        const double xx = x * x - k.cs2, yy = y * y - k.cs2, zz = z * z - k.cs2;   // Node.cpp:1016-1018
-        const double XXY = xx * y, XYY = yy * x, XXZ = xx * z, XZZ = zz * x, YYZ = yy * z, YZZ = zz * y;
-        k.h3[0][q] = XXY + YZZ;
-        k.h3[1][q] = XXZ + YYZ;
-        k.h3[2][q] = XYY + XZZ;
-        k.h3[3][q] = YZZ - XXY;
-        k.h3[4][q] = YYZ - XXZ;
-        k.h3[5][q] = XZZ - XYY;
-        const double a = xx * yy + o6 * zz;                                          // Node.cpp:1040-1052
+    const double e7 = 8. / 7., e45 = 56. / 45., e27 = 40. / 27.;
+    for (int p = 0; p < 8; ++p) {
+        const double xx = (p & 1) ? k.hA : k.hB, yy = (p & 2) ? k.hA : k.hB, zz = (p & 4) ? k.hA : k.hB;
+        const double a = xx * yy + o6 * zz;
         const double b = xx * zz + o6 * yy + t7 * a;
         const double c = yy * zz + o6 * xx + t7 * a + t5 * b;
-        k.h4[0][q] = a;
-        k.h4[1][q] = b;
-        k.h4[2][q] = c;
+        k.h4e[p][0] = e7 * a;
+        k.h4e[p][1] = e45 * b;
+        k.h4e[p][2] = e27 * c;
     }
     k.half_dt = 0.5 * k.dt;
     k.rec_cs4_strain = 1. / (k.cs2 * k.cs2);

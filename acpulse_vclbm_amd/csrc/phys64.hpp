@@ -19,8 +19,17 @@ struct K64 {
     double hA, hB;                   // xi*xi - cs^2 for a non-zero / zero component
     double pA, pB;                   // pow(xi, 2.) - cs^2 (Node.cpp:2514-2516)
     double cc;                       // c*c, the magnitude of xi_a*xi_b
-    double h3[6][19];                // third-order orthogonal Hermite values (Node.cpp:1024-1037)
-    double h4[3][19];                // fourth-order fully orthogonal Hermite values (Node.cpp:1040-1052)
+    // Third-order orthogonal Hermite values (Node.cpp:1024-1037): h3[a][q] is
+    // a sign times one of four magnitudes, fixed per (a, q) at compile time
+    // (h3_of), since xi = 0 or +-c and IEEE products and sums are symmetric in
+    // sign: PB + PB, PA + PB, PA + PA, PA - PB with PA = (c c - cs^2) c and
+    // PB = (0 0 - cs^2) c.  Holding 4 values instead of 114 keeps the tables
+    // in scalar registers of the collision kernels.
+    double h3m[4];
+    // fourth-order fully orthogonal Hermite values (Node.cpp:1040-1052), with
+    // their weights 8/7, 56/45, 40/27 already applied, per pattern of
+    // non-zero components (bit 0 x, bit 1 y, bit 2 z)
+    double h4e[8][3];
     double half_dt;                  // 0.5*dt
     double rec_cs4_strain;           // 1/(cs2*cs2) (Node.cpp:1758)
     double inv_pre;                  // -((dt*cs2)/(omega*nu)) (Node.cpp:1449)
@@ -107,6 +116,44 @@ __host__ __device__ __forceinline__ void moments64_psi(const K64& k, const doubl
     uz = vz / r;
 }
 
+// sign (0: the value is zero) and magnitude index into K64::h3m of h3[a][q]
+struct H3Ref {
+    int sign, idx;
+};
+__host__ __device__ constexpr H3Ref h3_of(int a, int q) {
+    const int ex = EX[q], ey = EY[q], ez = EZ[q];
+    const int nx = ex != 0, ny = ey != 0, nz = ez != 0;
+    switch (a) {
+        case 0: return {ey, nx + nz};                           // xx y + zz y
+        case 1: return {ez, nx + ny};                           // xx z + yy z
+        case 2: return {ex, ny + nz};                           // yy x + zz x
+        case 3: return {nz == nx ? 0 : ey * (nz ? 1 : -1), 3};  // zz y - xx y
+        case 4: return {ny == nx ? 0 : ez * (ny ? 1 : -1), 3};  // yy z - xx z
+        default: return {nz == ny ? 0 : ex * (nz ? 1 : -1), 3}; // zz x - yy x
+    }
+}
+__host__ __device__ constexpr int h4_pattern(int q) { return (EX[q] != 0) | ((EY[q] != 0) << 1) | ((EZ[q] != 0) << 2); }
+
+// sum_a h3[a][q] * v[a] * (a < 3 ? 3 : 1), left to right, as Node.cpp sums it;
+// terms whose Hermite value is zero add +-0 to a non-zero partial sum or to a
+// sum that only ever meets non-zero values, so they are left out
+template <int Q>
+__host__ __device__ __forceinline__ double h3_sum(const K64& k, const double v[6]) {
+    double s = 0.;
+    bool first = true;
+    static_for<0, 6>([&](auto ac) {
+        constexpr int a = decltype(ac)::value;
+        constexpr H3Ref h = h3_of(a, Q);
+        if constexpr (h.sign != 0) {
+            const double hv = h.sign > 0 ? k.h3m[h.idx] : -k.h3m[h.idx];
+            const double t = a < 3 ? hv * v[a] * 3. : hv * v[a];
+            s = first ? t : s + t;
+            first = false;
+        }
+    });
+    return s;
+}
+
 // ---------------------------------------------------------------- equilibrium (a4)
 // Node.cpp:739-813 (N=2), 816-949 (N=3), 952-1132 (N=4)
 template <int ORDER>
@@ -142,13 +189,10 @@ __host__ __device__ __forceinline__ void feq64(const K64& k, double rho, double 
         if constexpr (EX[q] * EZ[q] != 0) s2 = s2 + (EX[q] * EZ[q] > 0 ? k.cc : -k.cc) * xz * 2.;
         if constexpr (EY[q] * EZ[q] != 0) s2 = s2 + (EY[q] * EZ[q] > 0 ? k.cc : -k.cc) * yz * 2.;
         double x = 1. * 1. + k.rcs2 * s1 + k.c2 * s2;
-        if constexpr (ORDER >= 3) {
-            const double s3 = k.h3[0][q] * a3[0] * 3. + k.h3[1][q] * a3[1] * 3. + k.h3[2][q] * a3[2] * 3. +
-                              k.h3[3][q] * a3[3] + k.h3[4][q] * a3[4] + k.h3[5][q] * a3[5];
-            x = x + k.c3 * s3;
-        }
+        if constexpr (ORDER >= 3) x = x + k.c3 * h3_sum<q>(k, a3);
         if constexpr (ORDER >= 4) {
-            const double s4 = e7 * k.h4[0][q] * A + e45 * k.h4[1][q] * B + e27 * k.h4[2][q] * C;
+            constexpr int p = h4_pattern(q);
+            const double s4 = k.h4e[p][0] * A + k.h4e[p][1] * B + k.h4e[p][2] * C;
             x = x + k.c4 * s4;
         }
         out[q] = rho * weight(q) * x;
@@ -209,13 +253,10 @@ __host__ __device__ __forceinline__ void offeq64(const K64& k, const double a2[6
         if constexpr (EX[q] * EZ[q] != 0) s2 = s2 + (EX[q] * EZ[q] > 0 ? k.cc : -k.cc) * XZ * 2.;
         if constexpr (EY[q] * EZ[q] != 0) s2 = s2 + (EY[q] * EZ[q] > 0 ? k.cc : -k.cc) * YZ * 2.;
         double x = k.c2 * s2;
-        if constexpr (ORDER >= 3) {
-            const double s3 = k.h3[0][q] * b3[0] * 3. + k.h3[1][q] * b3[1] * 3. + k.h3[2][q] * b3[2] * 3. +
-                              k.h3[3][q] * b3[3] + k.h3[4][q] * b3[4] + k.h3[5][q] * b3[5];
-            x = x + k.c3 * s3;
-        }
+        if constexpr (ORDER >= 3) x = x + k.c3 * h3_sum<q>(k, b3);
         if constexpr (ORDER >= 4) {
-            const double s4 = e7 * k.h4[0][q] * A + e45 * k.h4[1][q] * B + e27 * k.h4[2][q] * C;
+            constexpr int p = h4_pattern(q);
+            const double s4 = k.h4e[p][0] * A + k.h4e[p][1] * B + k.h4e[p][2] * C;
             x = x + k.c4 * s4;
         }
         out[q] = weight(q) * x;
