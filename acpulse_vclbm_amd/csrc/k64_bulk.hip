@@ -20,6 +20,32 @@
 
 namespace acp {
 
+// The pre-collision (rho, ux, uy, uz) of the six neighbours along links 0..5
+// from the macro planes M.  On an open x face (a fine level's coupled face,
+// PERIODIC form with xopen) the missing neighbour's macros come by quadratic
+// extrapolation, u(-1) = 3 u(0) - 3 u(1) + u(2), which makes the central
+// difference the second-order one-sided one.
+template <typename P, bool PERIODIC, typename I>
+__device__ __forceinline__ void load_neighbour_macros(const double* __restrict__ M, I l, const PopLayout& L, int i,
+                                                      int nx, int xopen, I oxp, I oyp, I oxm, I oym, I ozp, I ozm,
+                                                      double nb[6][4]) {
+    const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) nb[m][v] = P::soa(M, v, (I)(l + o[m]), L.N);
+    if constexpr (PERIODIC) {
+        if (((xopen & 1) && i == 0) || ((xopen & 2) && i == nx - 1)) {
+            const int m = i == 0 ? 2 : 0;
+            const I s1 = i == 0 ? (I)1 : (I)-1;
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                nb[m][v] = (3. * P::soa(M, v, l, L.N) - 3. * P::soa(M, v, (I)(l + s1), L.N)) +
+                           P::soa(M, v, (I)(l + s1 + s1), L.N);
+        }
+    }
+}
+
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
@@ -36,14 +62,6 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
     const I sxy = (I)nx * (I)ny;
     const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
     const K64& k = *kp;
-
-    double f[19];
-    if (active) {
-        static_for<0, 19>([&](auto qc) {
-            constexpr int q = decltype(qc)::value;
-            f[q] = ld_pop<NT>(&P::at(A, q, l, L));
-        });
-    }
 
     int bi = 0, bj = 0, bk = 0;
     if constexpr (!PERIODIC) {
@@ -67,9 +85,33 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if (kz == 0) ozm = sxy * (I)(nz - 1);
     }
 
+    // HRR without the cubic terms: the strain rate from the neighbours'
+    // macros first, so that their 24 values and the node's 19 populations are
+    // never live together (the schedule barrier keeps the population loads
+    // below); collide_hrr64_strain then equals collide_rr64's hybrid form
+    double S[6] = {0., 0., 0., 0., 0., 0.};
+    if constexpr (OP == OP_HRR && !CUBIC) {
+        if (active && !boundary) {
+            double nb[6][4];
+            load_neighbour_macros<P, PERIODIC>(M, l, L, i, nx, xopen, oxp, oyp, oxm, oym, ozp, ozm, nb);
+            strain_psi64<false>(k, nb, S, nullptr);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+    }
+
+    double f[19];
+    if (active) {
+        static_for<0, 19>([&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            f[q] = ld_pop<NT>(&P::at(A, q, l, L));
+        });
+    }
+
     if (active && !boundary) {
         if constexpr (OP == OP_SRT) {
             collide_srt64<ORDER, false>(k, f, nullptr);
+        } else if constexpr (OP == OP_HRR && !CUBIC) {
+            collide_hrr64_strain<ORDER>(k, f, S);
         } else if constexpr (OP == OP_RR || OP == OP_HRR) {
             double ps[19];
             if constexpr (CUBIC) {
@@ -79,26 +121,8 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
                 });
             }
             double nb[6][4];
-            if constexpr (OP == OP_HRR || CUBIC) {
-                const I o[6] = {oxp, oyp, oxm, oym, ozp, ozm};
-#pragma unroll
-                for (int m = 0; m < 6; ++m)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) nb[m][v] = P::soa(M, v, (I)(l + o[m]), L.N);
-                if constexpr (PERIODIC) {
-                    // open x face: the missing neighbour's macros by quadratic
-                    // extrapolation, u(-1) = 3 u(0) - 3 u(1) + u(2), which makes the
-                    // central difference the second-order one-sided one
-                    if (((xopen & 1) && i == 0) || ((xopen & 2) && i == nx - 1)) {
-                        const int m = i == 0 ? 2 : 0;
-                        const I s1 = i == 0 ? (I)1 : (I)-1;
-#pragma unroll
-                        for (int v = 0; v < 4; ++v)
-                            nb[m][v] = (3. * P::soa(M, v, l, L.N) - 3. * P::soa(M, v, (I)(l + s1), L.N)) +
-                                       P::soa(M, v, (I)(l + s1 + s1), L.N);
-                    }
-                }
-            }
+            if constexpr (OP == OP_HRR || CUBIC)
+                load_neighbour_macros<P, PERIODIC>(M, l, L, i, nx, xopen, oxp, oyp, oxm, oym, ozp, ozm, nb);
             collide_rr64<ORDER, OP == OP_HRR, CUBIC>(k, f, ps, nb);
             if constexpr (OP == OP_HRR || CUBIC) {
                 if (psi != nullptr) {
