@@ -1,6 +1,7 @@
 // Float64 auxiliary kernels: LODI boundary pass, HRR macro pre-pass, device
 // initial condition, probes and the mass reduction.
 #include "k64.hpp"
+#include "plane.hpp"
 #include "kshift.hpp"
 
 namespace acp {
@@ -150,13 +151,20 @@ hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, c
 // ------------------------------------------------------------------ macro pre-pass
 // Pre-collision density and velocity of every node (calcDensityPreCol /
 // calcVelocityPreCol, Node.cpp:32-51, 121-154) for the HRR strain stencil.
+// Populations larger than the last-level cache are read with the
+// non-temporal hint (NT), as the bulk kernels read them; the macro planes are
+// stored plainly, since the bulk kernel reads each of them from six nodes.
+template <bool NT>
 __global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                double* __restrict__ M, const PopLayout L) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t N = L.N;
     if (l >= N) return;
     double f[19], r, ux, uy, uz;
-    load19(A, L, l, f);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] = ld_pop<NT>(&A[pidx(L, q, l)]);
+    });
     moments64(*kp, f, r, ux, uy, uz);
     M[l] = r;
     M[N + l] = ux;
@@ -166,7 +174,11 @@ __global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const
 
 hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Grid& g, bool, hipStream_t s) {
     const int bs = 256;
-    hipLaunchKernelGGL(macro64, dim3((unsigned)((g.N + bs - 1) / bs)), dim3(bs), 0, s, k, A, macro, g.L);
+    const dim3 grid((unsigned)((g.N + bs - 1) / bs));
+    if (stream_populations(g.L, (int)sizeof(double)))
+        hipLaunchKernelGGL(macro64<true>, grid, dim3(bs), 0, s, k, A, macro, g.L);
+    else
+        hipLaunchKernelGGL(macro64<false>, grid, dim3(bs), 0, s, k, A, macro, g.L);
     return hipGetLastError();
 }
 

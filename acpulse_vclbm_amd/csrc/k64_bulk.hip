@@ -46,8 +46,13 @@ __device__ __forceinline__ void load_neighbour_macros(const double* __restrict__
     }
 }
 
+// HRR without the cubic terms, orders 2 and 3, is held to 128 registers (four
+// waves per SIMD): at 512^3 order 3 runs 9.4 GLUPS there, with a few bytes of
+// spill, against 8.2 at the 130 registers and three waves it takes
+// unconstrained.  Order 4 would spill about 100 bytes and ran slower (7.6
+// against 7.8 GLUPS), the cubic forms several hundred.
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT>
-__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
+__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(OP == OP_HRR && !CUBIC && ORDER < 4 ? ACP_HRR_WAVES : ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
                                                      const PopLayout L, int xopen, int row_lds, int xlodi) {

@@ -26,6 +26,9 @@
 #ifndef ACP_BULK_WAVES
 #define ACP_BULK_WAVES 1
 #endif
+#ifndef ACP_HRR_WAVES
+#define ACP_HRR_WAVES 4
+#endif
 
 namespace acp {
 
