@@ -274,10 +274,13 @@ int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* r
  * Every coarse node strictly between x0 and x1 is restricted from the fine
  * level.  A fine LODI face uses the level-0 domain length and its own spacing
  * and time step, as the reference's LODIpressureOutlet does on every level
- * (Node.cpp:237-311).  Every operator without the cubic-Mach correction
- * (HRR's strain stencil extrapolates at the fine level's open faces, see
- * ACP_BC_OPEN_X), every storage tier (both levels use the coarse
- * description's tier). */
+ * (Node.cpp:237-311).  Every operator without the cubic-Mach correction,
+ * every storage tier (both levels use the coarse description's tier).  HRR
+ * hands strain rates across the interface as the reference's partner
+ * functions compute them (Node.cpp:1892-2490): the fine face nodes collide
+ * with their coarse partners' (at t and t + 1/2), the coarse grid-coupling
+ * columns with their fine partners' (acp_refined_set_coupling for the direct
+ * coupling's form). */
 typedef struct acp_refined acp_refined;
 int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, acp_refined** out);
 void acp_refined_destroy(acp_refined* r);
