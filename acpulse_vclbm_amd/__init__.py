@@ -14,7 +14,7 @@ import numpy as np
 from . import _capi
 from ._capi import (ACP_OK, AcpError, BOUNDARY, F2C_FILTER, CaseDesc, CaseInfo, MRT_MODE, OP, PREC, check, dptr, lib)
 
-__all__ = ["Case", "Control", "Lattice", "Refined", "CaseDesc", "AcpError", "make_desc", "lib", "lodi_schedule"]
+__all__ = ["Case", "Control", "Lattice", "Pack", "Refined", "CaseDesc", "AcpError", "make_desc", "lib", "lodi_schedule"]
 
 
 def make_desc(nx, ny, nz, op="SRT", order=3, cubic=False, transverse=False, boundary="LODI",
@@ -287,6 +287,57 @@ class Refined:
 
     def __exit__(self, *a):
         self.close()
+
+
+class Pack:
+    """Resident one-level cases of one tier and equilibrium order advanced by
+    shared kernel launches (acp_pack_* of include/acp.h).  A member keeps its
+    own results: they equal stepping it alone, bit for bit."""
+
+    def __init__(self, precision="F64", order=3, stream=None):
+        prec = PREC[precision] if isinstance(precision, str) else int(precision)
+        self.h = ctypes.c_void_p()
+        st = ctypes.c_void_p(int(stream)) if stream else None
+        check(lib().acp_pack_create(prec, int(order), st, ctypes.byref(self.h)))
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().acp_pack_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, case, steps):
+        check(lib().acp_pack_add(self.h, case.h, int(steps)))
+
+    def step(self, n=1):
+        check(lib().acp_pack_step(self.h, int(n)))
+
+    def enable_graphs(self, chunk):
+        check(lib().acp_pack_enable_graphs(self.h, int(chunk)))
+
+    def steps_left(self, case):
+        n = ctypes.c_int32()
+        check(lib().acp_pack_steps_left(self.h, case.h, ctypes.byref(n)))
+        return n.value
+
+    def remove(self, case):
+        check(lib().acp_pack_remove(self.h, case.h))
+
+    def members(self):
+        n = ctypes.c_int32()
+        check(lib().acp_pack_members(self.h, ctypes.byref(n)))
+        return n.value
 
 
 def lodi_schedule(nx, ny, nz):

@@ -48,7 +48,7 @@ class CaseInfo(ctypes.Structure):
 class ServerDesc(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_active", ctypes.c_int32),
                 ("graph_chunk", ctypes.c_int32), ("launch_steps", ctypes.c_int32), ("chunks_ahead", ctypes.c_int32),
-                ("admit_threads", ctypes.c_int32)]
+                ("admit_threads", ctypes.c_int32), ("pack", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):
@@ -154,6 +154,14 @@ def lib():
         "acp_refined_footprint": (ctypes.c_int, [D, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int64)]),
         "acp_refined_create_in": (ctypes.c_int, [D, i32, i32, vp, vp, ctypes.c_int64, ctypes.POINTER(vp)]),
         "acp_refined_enable_graphs": (ctypes.c_int, [vp, i32]),
+        "acp_pack_create": (ctypes.c_int, [i32, i32, vp, ctypes.POINTER(vp)]),
+        "acp_pack_destroy": (None, [vp]),
+        "acp_pack_add": (ctypes.c_int, [vp, vp, i32]),
+        "acp_pack_step": (ctypes.c_int, [vp, i32]),
+        "acp_pack_enable_graphs": (ctypes.c_int, [vp, i32]),
+        "acp_pack_steps_left": (ctypes.c_int, [vp, vp, ctypes.POINTER(i32)]),
+        "acp_pack_remove": (ctypes.c_int, [vp, vp]),
+        "acp_pack_members": (ctypes.c_int, [vp, ctypes.POINTER(i32)]),
         "acp_server_create": (ctypes.c_int, [ctypes.POINTER(ServerDesc), ctypes.POINTER(vp)]),
         "acp_server_submit": (ctypes.c_int, [vp, ctypes.POINTER(Request)]),
         "acp_server_poll": (ctypes.c_int, [vp, ctypes.POINTER(Result), i32, i32, ctypes.POINTER(i32)]),

@@ -459,8 +459,11 @@ int download_array(acp_case* c, bool current, double* f) {
     return rc;
 }
 
+// A case that a pack steps belongs to the pack until acp_pack_remove: its
+// host-side arrays and counters are stale until then.
 int check_case(const acp_case* c) {
     if (!c) return fail(ACP_E_ARG, "null case handle");
+    if (c->pack) return fail(ACP_E_STATE, "the case is stepped by a pack: acp_pack_remove it first");
     return ACP_OK;
 }
 
@@ -682,13 +685,14 @@ int create_case(const acp_case_desc* desc, void* stream, void* region, int64_t r
 
 extern "C" void acp_case_destroy(acp_case* c) {
     if (!c) return;
+    acp::pack_detach(c);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     release(c);
     delete c;
 }
 
 extern "C" int acp_case_info_get(const acp_case* c, acp_case_info* out) {
-    if (check_case(c) || !out) return fail(ACP_E_ARG, "acp_case_info_get: null argument");
+    if (!c || !out) return fail(ACP_E_ARG, "acp_case_info_get: null argument");
     std::memset(out, 0, sizeof(*out));
     out->nodes = c->g.N;
     out->device_bytes = c->device_bytes;

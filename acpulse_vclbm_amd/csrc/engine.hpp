@@ -71,6 +71,11 @@ int write_vtk2_host(const acp_case_desc& cd, const K64& ck, const double* ccur, 
 int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, const double* fprev, const double* psi,
                    const char* stem);
 
+// Packed stepping (pack.cpp): whether a case may join a pack of a tier and
+// order, and the removal of a member case that is being destroyed.
+int pack_eligible(const acp_case* c, int32_t prec, int32_t order);
+void pack_detach(acp_case* c);
+
 }  // namespace acp
 
 struct acp_case {
@@ -123,5 +128,8 @@ struct acp_case {
     int32_t fix_col[2] = {0, 0};
     const double* fix_S[2] = {nullptr, nullptr};
     std::vector<void*> extra;  // buffers of case_alloc_extra (freed with the case)
+    // the pack that steps the case (acp_pack_add .. acp_pack_remove): its
+    // arrays, step and probe counters are the pack's until then
+    acp_pack* pack = nullptr;
     int64_t device_bytes = 0;
 };

@@ -1,8 +1,9 @@
 // Float64 auxiliary kernels: LODI boundary pass, HRR macro pre-pass, device
 // initial condition, probes and the mass reduction.
 #include "k64.hpp"
-#include "plane.hpp"
 #include "kshift.hpp"
+#include "pack.hpp"
+#include "plane.hpp"
 
 namespace acp {
 
@@ -27,14 +28,9 @@ __device__ __forceinline__ void load19_soa(const double* __restrict__ F, int64_t
 // plus their cubic-Mach terms), tangent neighbours from A; writes feq_N of the
 // characteristic update into the node's B entries.
 template <int ORDER, bool CUBIC>
-__global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                              double* __restrict__ B, const double* __restrict__ psi,
-                                              const LodiNode* __restrict__ nodes, int64_t count, const PopLayout PL,
-                                              int transverse) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= count) return;
-    const K64& k = *kp;
-    const LodiNode ln = nodes[s];
+__device__ __forceinline__ void lodi64_node(const K64& k, const double* __restrict__ A, double* __restrict__ B,
+                                            const double* __restrict__ psi, const LodiNode& ln, const PopLayout& PL,
+                                            int transverse) {
     double f[19], ps[19];
     double rb, vb[3], r1, v1[3], r2, v2[3];
     load19(A, PL, ln.node, f);
@@ -128,6 +124,47 @@ __global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const 
     });
 }
 
+template <int ORDER, bool CUBIC>
+__global__ __launch_bounds__(256) void lodi64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                              double* __restrict__ B, const double* __restrict__ psi,
+                                              const LodiNode* __restrict__ nodes, int64_t count, const PopLayout PL,
+                                              int transverse) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const LodiNode ln = nodes[s];
+    lodi64_node<ORDER, CUBIC>(*kp, A, B, psi, ln, PL, transverse);
+}
+
+// Packed form (pack.hpp): launch `phase` of every member's LODI pass, one
+// persistent grid over 256-record chunks.
+template <int ORDER>
+__global__ __launch_bounds__(kPackChunk) void lodi64_packed(const PackTable* __restrict__ t, int phase) {
+    const PackList& pl = t->lodi[phase];
+    const int total = pl.prefix[pl.n];
+    int m = 0;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        m = pack_member(pl, w, m);
+        const PackSlot& s = t->slot[pl.slot[m]];
+        if (s.steps_left <= 0) continue;
+        const int64_t r = s.phase_start[phase] + (int64_t)(w - pl.prefix[m]) * kPackChunk + threadIdx.x;
+        if (r >= s.phase_start[phase + 1]) continue;
+        const LodiNode ln = s.lodi[r];
+        lodi64_node<ORDER, false>(*(const K64*)s.k, (const double*)s.A, (double*)s.B, nullptr, ln, s.L,
+                                  s.transverse);
+    }
+}
+
+hipError_t launch_lodi64_packed(int order, int phase, const PackTable* t, hipStream_t s) {
+    const dim3 grid(kPackLodiGrid), block(kPackChunk);
+    switch (order) {
+        case 2: hipLaunchKernelGGL(lodi64_packed<2>, grid, block, 0, s, t, phase); break;
+        case 3: hipLaunchKernelGGL(lodi64_packed<3>, grid, block, 0, s, t, phase); break;
+        case 4: hipLaunchKernelGGL(lodi64_packed<4>, grid, block, 0, s, t, phase); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, const double* A, double* B,
                          const double* psi, const LodiNode* nodes, int64_t count, const PopLayout& L, hipStream_t s) {
     if (count <= 0) return hipSuccess;
@@ -155,21 +192,47 @@ hipError_t launch_lodi64(int order, bool cubic, bool transverse, const K64* k, c
 // non-temporal hint (NT), as the bulk kernels read them; the macro planes are
 // stored plainly, since the bulk kernel reads each of them from six nodes.
 template <bool NT>
-__global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                               double* __restrict__ M, const PopLayout L) {
-    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void macro64_node(const K64& k, const double* __restrict__ A, double* __restrict__ M,
+                                             const PopLayout& L, int64_t l) {
     const int64_t N = L.N;
-    if (l >= N) return;
     double f[19], r, ux, uy, uz;
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         f[q] = ld_pop<NT>(&A[pidx(L, q, l)]);
     });
-    moments64(*kp, f, r, ux, uy, uz);
+    moments64(k, f, r, ux, uy, uz);
     M[l] = r;
     M[N + l] = ux;
     M[2 * N + l] = uy;
     M[3 * N + l] = uz;
+}
+
+template <bool NT>
+__global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                               double* __restrict__ M, const PopLayout L) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L.N) return;
+    macro64_node<NT>(*kp, A, M, L, l);
+}
+
+// Packed form: the pre-pass of every HRR member of a pack.
+__global__ __launch_bounds__(kPackChunk) void macro64_packed(const PackTable* __restrict__ t) {
+    const PackList& pl = t->bulk[OP_HRR];
+    const int total = pl.prefix[pl.n];
+    int m = 0;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        m = pack_member(pl, w, m);
+        const PackSlot& s = t->slot[pl.slot[m]];
+        if (s.steps_left <= 0) continue;
+        const int64_t l = (int64_t)(w - pl.prefix[m]) * kPackChunk + threadIdx.x;
+        if (l >= s.L.N) continue;
+        macro64_node<false>(*(const K64*)s.k, (const double*)s.A, (double*)s.M, s.L, l);
+    }
+}
+
+hipError_t launch_macro64_packed(const PackTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(macro64_packed, dim3(kPackBulkGrid), dim3(kPackChunk), 0, s, t);
+    return hipGetLastError();
 }
 
 hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Grid& g, bool, hipStream_t s) {
@@ -348,9 +411,9 @@ hipError_t launch_init_rows(int prec, int order, const K64* k, const KShift* ks,
 // `slot` of the series (if below the capacity), and thread 0 advances the
 // counter after the barrier.  Keeping the row index on the device lets a
 // captured graph of several steps write a new row on every replay.
-__global__ __launch_bounds__(1024) void probe64(const double* __restrict__ A, const PopLayout L,
-                                                const int64_t* __restrict__ nodes, int nprobe,
-                                                double* __restrict__ out, int32_t* __restrict__ slot, int32_t cap) {
+__device__ __forceinline__ void probe64_row(const double* __restrict__ A, const PopLayout& L,
+                                            const int64_t* __restrict__ nodes, int nprobe, double* __restrict__ out,
+                                            int32_t* __restrict__ slot, int32_t cap) {
     const int32_t row = *slot;
     for (int p = threadIdx.x; p < nprobe; p += blockDim.x) {
         const int64_t l = nodes[p];
@@ -361,6 +424,46 @@ __global__ __launch_bounds__(1024) void probe64(const double* __restrict__ A, co
     }
     __syncthreads();
     if (threadIdx.x == 0) *slot = row + 1;
+}
+
+__global__ __launch_bounds__(1024) void probe64(const double* __restrict__ A, const PopLayout L,
+                                                const int64_t* __restrict__ nodes, int nprobe,
+                                                double* __restrict__ out, int32_t* __restrict__ slot, int32_t cap) {
+    probe64_row(A, L, nodes, nprobe, out, slot, cap);
+}
+
+// Packed form: the probe row of every member, one member after the other in
+// one workgroup (the step's new populations are B, before the swap).
+__global__ __launch_bounds__(256) void probe64_packed(const PackTable* __restrict__ t) {
+    for (int i = 0; i < t->nslot; ++i) {
+        const PackSlot& s = t->slot[i];
+        if (s.steps_left <= 0 || s.nprobe <= 0) continue;
+        probe64_row((const double*)s.B, s.L, s.probe_nodes, s.nprobe, s.probe_buf, s.probe_slot, s.probe_cap);
+    }
+}
+
+hipError_t launch_probe64_packed(const PackTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(probe64_packed, dim3(1), dim3(256), 0, s, t);
+    return hipGetLastError();
+}
+
+// Last kernel of a packed step: the arrays of every member with steps left
+// change roles and its count goes down by one.
+__global__ __launch_bounds__(64) void pack_advance(PackTable* __restrict__ t) {
+    const int i = threadIdx.x;
+    if (i >= t->nslot) return;
+    PackSlot& s = t->slot[i];
+    if (s.steps_left <= 0) return;
+    void* a = s.A;
+    s.A = s.B;
+    s.B = a;
+    s.steps_left -= 1;
+}
+
+hipError_t launch_pack_advance(PackTable* t, hipStream_t s) {
+    static_assert(kPackSlots <= 64, "one thread per pack slot");
+    hipLaunchKernelGGL(pack_advance, dim3(1), dim3(64), 0, s, t);
+    return hipGetLastError();
 }
 
 hipError_t launch_probe64(const double* A, const PopLayout& L, const int64_t* nodes, int nprobe, double* out,

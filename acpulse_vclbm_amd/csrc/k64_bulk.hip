@@ -16,6 +16,7 @@
 // previous contents, as in the reference.  In periodic mode every node is
 // interior and the neighbours wrap.
 #include "k64.hpp"
+#include "pack.hpp"
 #include "plane.hpp"
 
 namespace acp {
@@ -51,19 +52,15 @@ __device__ __forceinline__ void load_neighbour_macros(const double* __restrict__
 // spill, against 8.2 at the 130 registers and three waves it takes
 // unconstrained.  Order 4 would spill about 100 bytes and ran slower (7.6
 // against 7.8 GLUPS), the cubic forms several hundred.
+// The update of node (i, j, kz): collide and push.  `active` is false for the
+// threads beyond a row held by one workgroup (row_lds), which keep only their
+// part in the row's LDS exchange; row_lds is uniform per workgroup.
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT>
-__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(OP == OP_HRR && !CUBIC && ORDER < 4 ? ACP_HRR_WAVES : ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                                     double* __restrict__ B, double* __restrict__ psi,
-                                                     const double* __restrict__ M, int nx, int ny, int nz,
-                                                     const PopLayout L, int xopen, int row_lds, int xlodi) {
+__device__ __forceinline__ void bulk64_node(const K64* kp, const double* A, double* B, double* psi, const double* M,
+                                            int nx, int ny, int nz, const PopLayout& L,
+                                            int xopen, int row_lds, int xlodi, int i, int j, int kz, bool active) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    // a row held by one workgroup (row_lds) keeps the threads beyond it for the
-    // LDS exchange of its stores; they load, compute and store nothing
-    const bool active = i < nx;
-    if (!active && !row_lds) return;
-    const int j = blockIdx.y, kz = blockIdx.z;
     const I sxy = (I)nx * (I)ny;
     const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
     const K64& k = *kp;
@@ -199,6 +196,43 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(OP =
     st_pop<NT>(&P::at(B, 18, l, L), f[18]);
 }
 
+template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT>
+__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(OP == OP_HRR && !CUBIC && ORDER < 4 ? ACP_HRR_WAVES : ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                                     double* __restrict__ B, double* __restrict__ psi,
+                                                     const double* __restrict__ M, int nx, int ny, int nz,
+                                                     const PopLayout L, int xopen, int row_lds, int xlodi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    // a row held by one workgroup (row_lds) keeps the threads beyond it for the
+    // LDS exchange of its stores; they load, compute and store nothing
+    const bool active = i < nx;
+    if (!active && !row_lds) return;
+    bulk64_node<OP, ORDER, CUBIC, PERIODIC, NARROW, NT>(kp, A, B, psi, M, nx, ny, nz, L, xopen, row_lds, xlodi, i,
+                                                        (int)blockIdx.y, (int)blockIdx.z, active);
+}
+
+// Packed form (pack.hpp): one persistent grid over the 256-node chunks of
+// every member of a pack with operator OP.  Members are one-level grids with
+// boundary nodes on every face (LODI), addressed in the narrow form with
+// plain loads and stores; each node is updated exactly as bulk64 updates it.
+template <int OP, int ORDER>
+__global__ __launch_bounds__(kPackChunk) __attribute__((amdgpu_waves_per_eu(OP == OP_HRR && ORDER < 4 ? ACP_HRR_WAVES : ACP_BULK_WAVES))) void bulk64_packed(const PackTable* __restrict__ t) {
+    const PackList& pl = t->bulk[OP];
+    const int total = pl.prefix[pl.n];
+    int m = 0;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        m = pack_member(pl, w, m);
+        const PackSlot& s = t->slot[pl.slot[m]];
+        if (s.steps_left <= 0) continue;
+        const uint32_t l = (uint32_t)(w - pl.prefix[m]) * kPackChunk + threadIdx.x;
+        if (l >= (uint32_t)s.L.N) continue;
+        const uint32_t nx = (uint32_t)s.nx, nxy = nx * (uint32_t)s.ny;
+        const uint32_t kz = l / nxy, r = l - kz * nxy, j = r / nx, i = r - j * nx;
+        bulk64_node<OP, ORDER, false, false, true, false>((const K64*)s.k, (const double*)s.A, (double*)s.B, nullptr,
+                                                          (const double*)s.M, s.nx, s.ny, s.nz, s.L, 0, 0, 0, (int)i,
+                                                          (int)j, (int)kz, true);
+    }
+}
+
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC>
 static hipError_t go(const K64* k, const double* A, double* B, double* psi, const double* M, const Grid& g,
                      hipStream_t s) {
@@ -254,6 +288,30 @@ hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64
         case OP_MRT_RM: return go_p<OP_MRT_RM, 3, false>(periodic, k, A, B, psi, macro, g, s);
         case OP_MRT_GS: return go_p<OP_MRT_GS, 3, false>(periodic, k, A, B, psi, macro, g, s);
         case OP_STREAM: return go_p<OP_STREAM, 3, false>(periodic, k, A, B, psi, macro, g, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int OP>
+static hipError_t packed_o(int order, const PackTable* t, hipStream_t s) {
+    const dim3 grid(kPackBulkGrid), block(kPackChunk);
+    switch (order) {
+        case 2: hipLaunchKernelGGL((bulk64_packed<OP, 2>), grid, block, 0, s, t); break;
+        case 3: hipLaunchKernelGGL((bulk64_packed<OP, 3>), grid, block, 0, s, t); break;
+        case 4: hipLaunchKernelGGL((bulk64_packed<OP, 4>), grid, block, 0, s, t); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bulk64_packed(int op, int order, const PackTable* t, hipStream_t s) {
+    switch (op) {
+        case OP_SRT: return packed_o<OP_SRT>(order, t, s);
+        case OP_RR: return packed_o<OP_RR>(order, t, s);
+        case OP_HRR: return packed_o<OP_HRR>(order, t, s);
+        case OP_MRT_RM: return packed_o<OP_MRT_RM>(3, t, s);
+        case OP_MRT_GS: return packed_o<OP_MRT_GS>(3, t, s);
+        case OP_STREAM: return packed_o<OP_STREAM>(3, t, s);
         default: return hipErrorInvalidValue;
     }
 }

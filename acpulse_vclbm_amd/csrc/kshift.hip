@@ -8,6 +8,7 @@
 #include <algorithm>
 
 #include "kshift.hpp"
+#include "pack.hpp"
 #include "physf.hpp"
 #include "plane.hpp"
 
@@ -25,6 +26,18 @@ __device__ __forceinline__ void load_dev(const T* __restrict__ F, const PopLayou
         constexpr int q = decltype(qc)::value;
         d[q] = (float)F[pidx(L, q, l)] * inv_s;
     });
+}
+
+// A deviation (float32, storage scale) converted to the storage type.  For
+// float16 the product that made v is first rounded to float32 in every
+// kernel: otherwise the compiler folds a multiply with a single use into the
+// conversion (one rounding, v_fma_mixlo_f16) in some kernels and not in
+// others, and the same node update would round differently in a case
+// stepped alone and in a pack.  The empty asm only pins v to a register.
+template <typename T>
+__device__ __forceinline__ T to_store(float v) {
+    if constexpr (!std::is_same_v<T, float>) __asm__ volatile("" : "+v"(v));
+    return (T)v;
 }
 
 // Collision of one node in lattice units on its stored deviations g (in place):
@@ -261,18 +274,15 @@ hipError_t launch_fix_hrr_shift(int prec, int order, const KShift* k, const void
 }
 
 // ------------------------------------------------------------------ bulk collide + push
+// The update of node (i, j, kz); `active` false: a thread beyond a row held
+// by one workgroup, which only takes part in its LDS exchange (see bulk64).
 template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW, bool NT>
-__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
-    const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
-    int ny, int nz, const PopLayout L, int xopen, int row_lds, int xlodi) {
+__device__ __forceinline__ void bulk_shift_node(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                T* __restrict__ B, const float* __restrict__ M, int nx, int ny, int nz,
+                                                const PopLayout& L, int xopen, int row_lds, int xlodi, int i, int j,
+                                                int kz, bool active) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    // threads beyond a row held by one workgroup only take part in its LDS
-    // exchange (see bulk64)
-    const bool active = i < nx;
-    if (!active && !row_lds) return;
-    const int j = blockIdx.y, kz = blockIdx.z;
     const I sxy = (I)nx * (I)ny;
     const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
     const KShift& k = *kp;
@@ -323,16 +333,16 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
             if constexpr (EZ[q] < 0) base += ozm;
             if constexpr (EX[q] == 0) {
                 if (active && row_source_pushes<PERIODIC>(q, i, nx, bj, bk, xopen, xlodi))
-                    st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), (T)g[q]);
+                    st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), to_store<T>(g[q]));
             } else {
                 __syncthreads();
-                if (active) row[(i + EX[q] + nx) % nx] = (T)g[q];
+                if (active) row[(i + EX[q] + nx) % nx] = to_store<T>(g[q]);
                 __syncthreads();
                 if (active && row_source_pushes<PERIODIC>(q, i - EX[q], nx, bj, bk, xopen, xlodi))
                     st_pop<NT>(&P::at(B, q, (I)(base + (I)i), L), row[i]);
             }
         });
-        if (active) st_pop<NT>(&P::at(B, 18, l, L), (T)g[18]);
+        if (active) st_pop<NT>(&P::at(B, 18, l, L), to_store<T>(g[18]));
         return;
     }
 
@@ -356,9 +366,43 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_
         if constexpr (EY[q] < 0) dd += oym;
         if constexpr (EZ[q] > 0) dd += ozp;
         if constexpr (EZ[q] < 0) dd += ozm;
-        st_pop<NT>(&P::at(B, q, dd, L), (T)g[q]);
+        st_pop<NT>(&P::at(B, q, dd, L), to_store<T>(g[q]));
     });
-    st_pop<NT>(&P::at(B, 18, l, L), (T)g[18]);
+    st_pop<NT>(&P::at(B, 18, l, L), to_store<T>(g[18]));
+}
+
+template <typename T, int OP, int ORDER, bool PERIODIC, bool NARROW, bool NT>
+__global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift(
+    const KShift* __restrict__ kp, const T* __restrict__ A, T* __restrict__ B, const float* __restrict__ M, int nx,
+    int ny, int nz, const PopLayout L, int xopen, int row_lds, int xlodi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = i < nx;
+    if (!active && !row_lds) return;
+    bulk_shift_node<T, OP, ORDER, PERIODIC, NARROW, NT>(kp, A, B, M, nx, ny, nz, L, xopen, row_lds, xlodi, i,
+                                                        (int)blockIdx.y, (int)blockIdx.z, active);
+}
+
+// Packed form (pack.hpp): one persistent grid over the 256-node chunks of
+// every member of a pack with operator OP (one-level grids with LODI faces,
+// narrow addressing, plain loads and stores), each node as bulk_shift.
+template <typename T, int OP, int ORDER>
+__global__ __launch_bounds__(kPackChunk) __attribute__((amdgpu_waves_per_eu(ACP_BULK_WAVES))) void bulk_shift_packed(
+    const PackTable* __restrict__ t) {
+    const PackList& pl = t->bulk[OP];
+    const int total = pl.prefix[pl.n];
+    int m = 0;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        m = pack_member(pl, w, m);
+        const PackSlot& s = t->slot[pl.slot[m]];
+        if (s.steps_left <= 0) continue;
+        const uint32_t l = (uint32_t)(w - pl.prefix[m]) * kPackChunk + threadIdx.x;
+        if (l >= (uint32_t)s.L.N) continue;
+        const uint32_t nx = (uint32_t)s.nx, nxy = nx * (uint32_t)s.ny;
+        const uint32_t kz = l / nxy, r = l - kz * nxy, j = r / nx, i = r - j * nx;
+        bulk_shift_node<T, OP, ORDER, false, true, false>((const KShift*)s.k, (const T*)s.A, (T*)s.B,
+                                                          (const float*)s.M, s.nx, s.ny, s.nz, s.L, 0, 0, 0, (int)i,
+                                                          (int)j, (int)kz, true);
+    }
 }
 
 // Periodic rows of nx = 2 * blockDim.x nodes (nx a multiple of 128, no open
@@ -533,6 +577,37 @@ static hipError_t bulk_t(int op, int order, bool periodic, const KShift* k, cons
 #undef BO
 }
 
+template <typename T>
+static hipError_t bulk_packed_t(int op, int order, const PackTable* t, hipStream_t s) {
+    const dim3 grid(kPackBulkGrid), block(kPackChunk);
+#define BP(OPC, O) hipLaunchKernelGGL((bulk_shift_packed<T, OPC, O>), grid, block, 0, s, t)
+#define BPO(OPC)                                  \
+    switch (order) {                              \
+        case 2: BP(OPC, 2); break;                \
+        case 3: BP(OPC, 3); break;                \
+        case 4: BP(OPC, 4); break;                \
+        default: return hipErrorInvalidValue;     \
+    }                                             \
+    break
+    switch (op) {
+        case OP_SRT: BPO(OP_SRT);
+        case OP_RR: BPO(OP_RR);
+        case OP_HRR: BPO(OP_HRR);
+        case OP_MRT_RM: BP(OP_MRT_RM, 3); break;
+        case OP_MRT_GS: BP(OP_MRT_GS, 3); break;
+        case OP_STREAM: BP(OP_STREAM, 3); break;
+        default: return hipErrorInvalidValue;
+    }
+#undef BPO
+#undef BP
+    return hipGetLastError();
+}
+
+hipError_t launch_bulk_shift_packed(int prec, int op, int order, const PackTable* t, hipStream_t s) {
+    if (prec == ACP_PREC_F32_SHIFTED) return bulk_packed_t<float>(op, order, t, s);
+    return bulk_packed_t<_Float16>(op, order, t, s);
+}
+
 hipError_t launch_bulk_shift(int prec, int op, int order, bool periodic, const KShift* k, const void* A, void* B,
                              const float* macro, const Grid& g, hipStream_t s) {
     if (prec == ACP_PREC_F32_SHIFTED) return bulk_t<float>(op, order, periodic, k, A, B, macro, g, s);
@@ -541,18 +616,47 @@ hipError_t launch_bulk_shift(int prec, int op, int order, bool periodic, const K
 
 // ------------------------------------------------------------------ HRR pre-pass
 template <typename T>
-__global__ __launch_bounds__(256) void macro_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
-                                                   float* __restrict__ M, const PopLayout L) {
-    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void macro_shift_node(const KShift& k, const T* __restrict__ A, float* __restrict__ M,
+                                                 const PopLayout& L, int64_t l) {
     const int64_t N = L.N;
-    if (l >= N) return;
     float d[19], dr, ux, uy, uz;
-    load_dev(A, L, l, kp->inv_s, d);
+    load_dev(A, L, l, k.inv_s, d);
     moments_dev(d, dr, ux, uy, uz);
     M[l] = dr;
     M[N + l] = ux;
     M[2 * N + l] = uy;
     M[3 * N + l] = uz;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void macro_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                   float* __restrict__ M, const PopLayout L) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= L.N) return;
+    macro_shift_node(*kp, A, M, L, l);
+}
+
+// Packed form: the pre-pass of every HRR member of a pack.
+template <typename T>
+__global__ __launch_bounds__(kPackChunk) void macro_shift_packed(const PackTable* __restrict__ t) {
+    const PackList& pl = t->bulk[OP_HRR];
+    const int total = pl.prefix[pl.n];
+    int m = 0;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        m = pack_member(pl, w, m);
+        const PackSlot& s = t->slot[pl.slot[m]];
+        if (s.steps_left <= 0) continue;
+        const int64_t l = (int64_t)(w - pl.prefix[m]) * kPackChunk + threadIdx.x;
+        if (l >= s.L.N) continue;
+        macro_shift_node(*(const KShift*)s.k, (const T*)s.A, (float*)s.M, s.L, l);
+    }
+}
+
+hipError_t launch_macro_shift_packed(int prec, const PackTable* t, hipStream_t s) {
+    const dim3 grid(kPackBulkGrid), block(kPackChunk);
+    if (prec == ACP_PREC_F32_SHIFTED) hipLaunchKernelGGL(macro_shift_packed<float>, grid, block, 0, s, t);
+    else hipLaunchKernelGGL(macro_shift_packed<_Float16>, grid, block, 0, s, t);
+    return hipGetLastError();
 }
 
 hipError_t launch_macro_shift(int prec, const KShift* k, const void* A, float* macro, const Grid& g, hipStream_t s) {
@@ -568,13 +672,8 @@ hipError_t launch_macro_shift(int prec, const KShift* k, const void* A, float* m
 // Node.cpp:229-504 in lattice units on deviations: (rho_b - rho0) and the density
 // differences of the one-sided gradient are formed from the stored deviations.
 template <typename T, int ORDER>
-__global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
-                                                  T* __restrict__ B, const LodiNode* __restrict__ nodes, int64_t count,
-                                                  const PopLayout L, int transverse) {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= count) return;
-    const KShift& k = *kp;
-    const LodiNode ln = nodes[s];
+__device__ __forceinline__ void lodi_shift_node(const KShift& k, const T* __restrict__ A, T* __restrict__ B,
+                                                const LodiNode& ln, const PopLayout& L, int transverse) {
     float d[19];
     float rb, vb[3], r1, v1[3], r2, v2[3];
     load_dev(A, L, ln.node, k.inv_s, d);
@@ -646,8 +745,51 @@ __global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp,
     feq_dev<ORDER>(dr_new, u[0], u[1], u[2], eq, k.k6);
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        B[pidx(L, q, ln.node)] = (T)(eq[q] * k.s);
+        B[pidx(L, q, ln.node)] = to_store<T>(eq[q] * k.s);
     });
+}
+
+template <typename T, int ORDER>
+__global__ __launch_bounds__(256) void lodi_shift(const KShift* __restrict__ kp, const T* __restrict__ A,
+                                                  T* __restrict__ B, const LodiNode* __restrict__ nodes, int64_t count,
+                                                  const PopLayout L, int transverse) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= count) return;
+    const LodiNode ln = nodes[s];
+    lodi_shift_node<T, ORDER>(*kp, A, B, ln, L, transverse);
+}
+
+// Packed form: launch `phase` of every member's LODI pass.
+template <typename T, int ORDER>
+__global__ __launch_bounds__(kPackChunk) void lodi_shift_packed(const PackTable* __restrict__ t, int phase) {
+    const PackList& pl = t->lodi[phase];
+    const int total = pl.prefix[pl.n];
+    int m = 0;
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        m = pack_member(pl, w, m);
+        const PackSlot& s = t->slot[pl.slot[m]];
+        if (s.steps_left <= 0) continue;
+        const int64_t r = s.phase_start[phase] + (int64_t)(w - pl.prefix[m]) * kPackChunk + threadIdx.x;
+        if (r >= s.phase_start[phase + 1]) continue;
+        const LodiNode ln = s.lodi[r];
+        lodi_shift_node<T, ORDER>(*(const KShift*)s.k, (const T*)s.A, (T*)s.B, ln, s.L, s.transverse);
+    }
+}
+
+hipError_t launch_lodi_shift_packed(int prec, int order, int phase, const PackTable* t, hipStream_t s) {
+    const dim3 grid(kPackLodiGrid), block(kPackChunk);
+#define LSP(T, O) hipLaunchKernelGGL((lodi_shift_packed<T, O>), grid, block, 0, s, t, phase)
+    if (prec == ACP_PREC_F32_SHIFTED) {
+        if (order == 2) LSP(float, 2);
+        else if (order == 3) LSP(float, 3);
+        else LSP(float, 4);
+    } else {
+        if (order == 2) LSP(_Float16, 2);
+        else if (order == 3) LSP(_Float16, 3);
+        else LSP(_Float16, 4);
+    }
+#undef LSP
+    return hipGetLastError();
 }
 
 hipError_t launch_lodi_shift(int prec, int order, bool transverse, const KShift* k, const void* A, void* B,
@@ -777,18 +919,42 @@ hipError_t launch_init_random_shift(int prec, int order, const KShift* k, void* 
 // ------------------------------------------------------------------ probes and mass (float64)
 // One block; the row index is kept on the device (see probe64).
 template <typename T>
-__global__ __launch_bounds__(1024) void probe_shift(const KShift* __restrict__ kp, const T* __restrict__ A, const PopLayout L,
-                                                    const int64_t* __restrict__ nodes, int nprobe,
-                                                    double* __restrict__ out, int32_t* __restrict__ slot,
-                                                    int32_t cap) {
+__device__ __forceinline__ void probe_shift_row(const KShift& k, const T* __restrict__ A, const PopLayout& L,
+                                                const int64_t* __restrict__ nodes, int nprobe,
+                                                double* __restrict__ out, int32_t* __restrict__ slot, int32_t cap) {
     const int32_t row = *slot;
     for (int p = threadIdx.x; p < nprobe; p += blockDim.x) {
         double r, ux, uy, uz;
-        macro_f64(*kp, A, L, nodes[p], r, ux, uy, uz);
+        macro_f64(k, A, L, nodes[p], r, ux, uy, uz);
         if (row < cap) out[(int64_t)row * nprobe + p] = r;
     }
     __syncthreads();
     if (threadIdx.x == 0) *slot = row + 1;
+}
+
+template <typename T>
+__global__ __launch_bounds__(1024) void probe_shift(const KShift* __restrict__ kp, const T* __restrict__ A, const PopLayout L,
+                                                    const int64_t* __restrict__ nodes, int nprobe,
+                                                    double* __restrict__ out, int32_t* __restrict__ slot,
+                                                    int32_t cap) {
+    probe_shift_row(*kp, A, L, nodes, nprobe, out, slot, cap);
+}
+
+// Packed form: the probe row of every member (see probe64_packed).
+template <typename T>
+__global__ __launch_bounds__(256) void probe_shift_packed(const PackTable* __restrict__ t) {
+    for (int i = 0; i < t->nslot; ++i) {
+        const PackSlot& s = t->slot[i];
+        if (s.steps_left <= 0 || s.nprobe <= 0) continue;
+        probe_shift_row(*(const KShift*)s.k, (const T*)s.B, s.L, s.probe_nodes, s.nprobe, s.probe_buf, s.probe_slot,
+                        s.probe_cap);
+    }
+}
+
+hipError_t launch_probe_shift_packed(int prec, const PackTable* t, hipStream_t s) {
+    if (prec == ACP_PREC_F32_SHIFTED) hipLaunchKernelGGL(probe_shift_packed<float>, dim3(1), dim3(256), 0, s, t);
+    else hipLaunchKernelGGL(probe_shift_packed<_Float16>, dim3(1), dim3(256), 0, s, t);
+    return hipGetLastError();
 }
 
 hipError_t launch_probe_shift(int prec, const KShift* k, const void* A, const PopLayout& L, const int64_t* nodes,

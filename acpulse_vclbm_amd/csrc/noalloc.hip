@@ -62,6 +62,7 @@ struct DeviceScope {
 
 extern "C" int acp_case_get_macros_noalloc(acp_case* c, double* rho, double* u) {
     if (!c) return acp::fail(ACP_E_ARG, "null case handle");
+    if (c->pack) return acp::fail(ACP_E_STATE, "the case is stepped by a pack: acp_pack_remove it first");
     if (!rho || !u) return acp::fail(ACP_E_ARG, "null output array");
     const int64_t N = c->g.N;
     if (N > kScratchNodes)

@@ -11,7 +11,10 @@
 //   * creates and initialises admitted cases on an admission thread;
 //   * drives every resident case on its own non-blocking stream from a driver
 //     thread, keeping a bounded number of step chunks (captured HIP graphs)
-//     queued per case, so that the cases' kernels overlap on the device;
+//     queued per case, so that the cases' kernels overlap on the device; or,
+//     with packing on, steps the eligible one-level cases of one tier and
+//     order together (acp_pack_*): one pack per tier and order, on a stream
+//     of its own, with the same bounded number of chunks queued per pack;
 //   * on completion reads the probe series, mass and state checksum, releases
 //     the case's arena range and slot, and posts a result with its times.
 #include <hip/hip_runtime.h>
@@ -29,6 +32,7 @@
 
 #include "acp.h"
 #include "engine.hpp"
+#include "pack.hpp"
 
 namespace {
 
@@ -81,6 +85,17 @@ struct Job {
     double t_submit = 0, t_admit = 0, t_start = 0;
     int status = ACP_OK;
     std::string error;
+    int64_t seq = 0;  // submission number
+};
+
+// The eligible one-level cases of one tier and order, stepped together.
+struct PackRun {
+    int32_t prec = 0, order = 0;
+    acp_pack* p = nullptr;
+    hipStream_t stream = nullptr;
+    std::vector<Job> members;
+    // launched chunks: completion event and the members whose last step it holds
+    std::deque<std::pair<hipEvent_t, std::vector<int64_t>>> inflight;
 };
 
 }  // namespace
@@ -296,23 +311,136 @@ void finish_job(acp_server* s, Job& j) {
 void drive_loop(acp_server* s) {
     (void)hipSetDevice(s->device);
     std::vector<Job> active;
+    std::vector<PackRun> packs;
     std::vector<hipEvent_t> pool;
     const int32_t chunk = s->cfg.graph_chunk > 0 ? s->cfg.graph_chunk : std::max(s->cfg.launch_steps, 1);
     const int32_t ahead = std::max(s->cfg.chunks_ahead, 1);
+    auto take_event = [&](hipEvent_t* ev) -> int {
+        if (!pool.empty()) {
+            *ev = pool.back();
+            pool.pop_back();
+            return ACP_OK;
+        }
+        if (hipEventCreateWithFlags(ev, hipEventDisableTiming) != hipSuccess) {
+            *ev = nullptr;
+            return acp::fail(ACP_E_HIP, "hipEventCreate");
+        }
+        return ACP_OK;
+    };
     while (true) {
+        std::vector<Job> arrived;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             while (!s->ready.empty()) {
-                active.push_back(std::move(s->ready.front()));
+                arrived.push_back(std::move(s->ready.front()));
                 s->ready.pop_front();
             }
-            if (active.empty()) {
+            bool packed = false;
+            for (const PackRun& pr : packs) packed = packed || !pr.members.empty();
+            if (active.empty() && arrived.empty() && !packed) {
                 if (s->admit_running == 0) break;
                 s->cv_drive.wait_for(lk, std::chrono::milliseconds(20));
                 continue;
             }
         }
         bool progressed = false;
+        // arrived cases join the pack of their tier and order when packing is
+        // on and they are eligible; the others run on their own streams
+        for (Job& j : arrived) {
+            if (s->cfg.pack && !j.r && j.status == ACP_OK && j.req.steps > 0 &&
+                acp::pack_eligible(j.c, j.req.desc.precision, j.req.desc.order) == ACP_OK) {
+                PackRun* pr = nullptr;
+                for (PackRun& x : packs)
+                    if (x.prec == j.req.desc.precision && x.order == j.req.desc.order) pr = &x;
+                if (!pr) {
+                    PackRun x;
+                    x.prec = j.req.desc.precision;
+                    x.order = j.req.desc.order;
+                    int rc = hipStreamCreateWithFlags(&x.stream, hipStreamNonBlocking) == hipSuccess
+                                 ? acp_pack_create(x.prec, x.order, x.stream, &x.p)
+                                 : acp::fail(ACP_E_HIP, "pack stream");
+                    if (rc == ACP_OK && s->cfg.graph_chunk > 0) rc = acp_pack_enable_graphs(x.p, s->cfg.graph_chunk);
+                    if (rc == ACP_OK) {
+                        packs.push_back(std::move(x));
+                        pr = &packs.back();
+                    } else {
+                        if (x.p) acp_pack_destroy(x.p);
+                        if (x.stream) (void)hipStreamDestroy(x.stream);
+                    }
+                }
+                if (pr && (int)pr->members.size() < acp::kPackSlots && acp_pack_add(pr->p, j.c, j.req.steps) == ACP_OK) {
+                    pr->members.push_back(std::move(j));
+                    progressed = true;
+                    continue;
+                }
+            }
+            active.push_back(std::move(j));
+        }
+        for (PackRun& pr : packs) {
+            // retire finished chunks; finish the members whose last step they held
+            while (!pr.inflight.empty()) {
+                const hipError_t e = hipEventQuery(pr.inflight.front().first);
+                if (e == hipErrorNotReady) break;
+                for (Job& j : pr.members)
+                    if (e != hipSuccess && j.status == ACP_OK) {
+                        j.status = ACP_E_HIP;
+                        j.error = hipGetErrorString(e);
+                    }
+                const std::vector<int64_t> fin = std::move(pr.inflight.front().second);
+                pool.push_back(pr.inflight.front().first);
+                pr.inflight.pop_front();
+                for (int64_t seq : fin)
+                    for (size_t i = 0; i < pr.members.size(); ++i)
+                        if (pr.members[i].seq == seq) {
+                            Job& j = pr.members[i];
+                            const int rc = acp_pack_remove(pr.p, j.c);
+                            if (rc != ACP_OK && j.status == ACP_OK) {
+                                j.status = rc;
+                                j.error = acp_last_error();
+                            }
+                            finish_job(s, j);
+                            pr.members.erase(pr.members.begin() + i);
+                            break;
+                        }
+                progressed = true;
+            }
+            // launch chunks while members have steps left
+            while ((int32_t)pr.inflight.size() < ahead) {
+                std::vector<int64_t> fin;
+                bool any = false;
+                for (Job& j : pr.members) {
+                    if (j.launched >= j.req.steps) continue;
+                    any = true;
+                    if (j.launched == 0) j.t_start = s->now();
+                    if (j.launched + chunk >= j.req.steps) fin.push_back(j.seq);
+                }
+                if (!any) break;
+                hipEvent_t ev = nullptr;
+                int rc = acp_pack_step(pr.p, chunk);
+                if (rc == ACP_OK) rc = take_event(&ev);
+                if (rc == ACP_OK && hipEventRecord(ev, pr.stream) != hipSuccess)
+                    rc = acp::fail(ACP_E_HIP, "hipEventRecord");
+                if (rc != ACP_OK) {
+                    // the pack stops: every member finishes with the error
+                    if (ev) pool.push_back(ev);
+                    (void)hipStreamSynchronize(pr.stream);
+                    const std::string msg = acp_last_error();
+                    for (Job& j : pr.members) {
+                        (void)acp_pack_remove(pr.p, j.c);
+                        j.status = rc;
+                        j.error = msg;
+                        finish_job(s, j);
+                    }
+                    pr.members.clear();
+                    for (auto& x : pr.inflight) pool.push_back(x.first);
+                    pr.inflight.clear();
+                    break;
+                }
+                for (Job& j : pr.members) j.launched = std::min(j.req.steps, j.launched + chunk);
+                pr.inflight.emplace_back(ev, std::move(fin));
+                progressed = true;
+            }
+        }
         for (size_t i = 0; i < active.size();) {
             Job& j = active[i];
             while (!j.inflight.empty()) {
@@ -362,6 +490,10 @@ void drive_loop(acp_server* s) {
             }
         }
         if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(50));
+    }
+    for (PackRun& pr : packs) {
+        acp_pack_destroy(pr.p);
+        (void)hipStreamDestroy(pr.stream);
     }
     for (hipEvent_t e : pool) (void)hipEventDestroy(e);
 }
@@ -437,6 +569,7 @@ extern "C" int acp_server_submit(acp_server* s, const acp_request* r) {
     std::lock_guard<std::mutex> lk(s->mu);
     if (s->stopping) return acp::fail(ACP_E_STATE, "acp_server_submit: server is shutting down");
     j.t_submit = s->now();
+    j.seq = s->submitted;
     auto pos = std::upper_bound(s->queue.begin(), s->queue.end(), r->arrival,
                                 [](double a, const Job& x) { return a < x.req.arrival; });
     s->queue.insert(pos, std::move(j));

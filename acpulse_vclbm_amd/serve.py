@@ -107,10 +107,10 @@ class Server:
     """acp_server_* handle: submit requests, poll results, read statistics."""
 
     def __init__(self, hbm_budget_bytes, max_active=8, graph_chunk=16, launch_steps=16, chunks_ahead=4,
-                 admit_threads=4):
+                 admit_threads=4, pack=False):
         L = lib()
         d = ServerDesc(int(hbm_budget_bytes), int(max_active), int(graph_chunk), int(launch_steps),
-                       int(chunks_ahead), int(admit_threads))
+                       int(chunks_ahead), int(admit_threads), int(bool(pack)))
         h = ctypes.c_void_p()
         check(L.acp_server_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
@@ -175,10 +175,11 @@ class Server:
 
 
 def run_mix(reqs, hbm_budget_bytes, max_active=8, graph_chunk=16, launch_steps=16, chunks_ahead=4,
-            admit_threads=4, timeout_s=3600.0):
+            admit_threads=4, timeout_s=3600.0, pack=False):
     """Offer `reqs` (arrival times relative to the start) to a fresh server and
-    collect every result.  Returns (results sorted by tag, summary dict)."""
-    with Server(hbm_budget_bytes, max_active, graph_chunk, launch_steps, chunks_ahead, admit_threads) as srv:
+    collect every result; `pack`: eligible one-level cases share packed step
+    launches.  Returns (results sorted by tag, summary dict)."""
+    with Server(hbm_budget_bytes, max_active, graph_chunk, launch_steps, chunks_ahead, admit_threads, pack) as srv:
         for r in reqs:
             srv.submit(r)
         results = []
@@ -253,9 +254,11 @@ def main():
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--graph-chunk", type=int, default=16)
     ap.add_argument("--budget-gib", type=float, default=64.0)
+    ap.add_argument("--pack", action="store_true", help="one-level cases share packed step launches")
     a = ap.parse_args()
     reqs = make_requests(a.requests, seed=1, rate=a.rate or None, step_scale=a.step_scale)
-    _, s = run_mix(reqs, int(a.budget_gib * (1 << 30)), max_active=a.max_active, graph_chunk=a.graph_chunk)
+    _, s = run_mix(reqs, int(a.budget_gib * (1 << 30)), max_active=a.max_active, graph_chunk=a.graph_chunk,
+                   pack=a.pack)
     print(json.dumps(s))
 
 

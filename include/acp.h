@@ -367,6 +367,40 @@ int acp_refined_create_in(const acp_case_desc* coarse, int32_t x0, int32_t x1, v
                           int64_t region_bytes, acp_refined** out);
 int acp_refined_enable_graphs(acp_refined* r, int32_t chunk);
 
+/* ------------------------------------------------------------------ packed stepping
+ * No counterpart in the reference, which runs one case per process
+ * (main.cpp:37-271).  A pack advances several resident one-level cases of one
+ * storage tier and equilibrium order with shared kernel launches: per step
+ * the HRR pre-pass, one bulk collide-and-push launch per operator present,
+ * one launch per LODI phase, one probe launch and one bookkeeping launch,
+ * each a persistent grid walking a member table held in device memory.
+ * Members join (acp_pack_add) and leave (acp_pack_remove) between steps; each
+ * takes exactly the steps it was given, and its populations, probe series and
+ * counters are bit-identical to stepping it alone with acp_case_step.
+ * Eligible cases: LODI on every face (ACP_BC_LODI), no cubic-Mach correction,
+ * not a level of a two-level case, at most 32 members.  While a case is in a
+ * pack, its own stepping, probe set-up and read-back calls return
+ * ACP_E_STATE. */
+typedef struct acp_pack acp_pack;
+/* precision ACP_PREC_*, order 2..4; stream NULL: the pack creates its own */
+int acp_pack_create(int32_t precision, int32_t order, void* stream, acp_pack** out);
+/* Waits for the pack's work, removes every member (as acp_pack_remove) and
+ * frees the pack. */
+void acp_pack_destroy(acp_pack* p);
+/* Join with `steps` steps to take (initialised case; waits for its stream). */
+int acp_pack_add(acp_pack* p, acp_case* c, int32_t steps);
+/* Launch n packed steps on the pack's stream; a member takes a step only while
+ * it has steps left.  Chunks of acp_pack_enable_graphs steps replay a
+ * captured graph. */
+int acp_pack_step(acp_pack* p, int32_t n);
+int acp_pack_enable_graphs(acp_pack* p, int32_t chunk);
+/* Steps of a member not yet launched. */
+int acp_pack_steps_left(const acp_pack* p, const acp_case* c, int32_t* left);
+/* Leave the pack (waits for the launches that stepped it): the case's arrays,
+ * step and probe counters are its own again. */
+int acp_pack_remove(acp_pack* p, acp_case* c);
+int acp_pack_members(const acp_pack* p, int32_t* members);
+
 /* ------------------------------------------------------------------ serving
  * No counterpart in the reference, which runs one case per process
  * (main.cpp:37-271).  A server keeps many independent cases resident on one
@@ -378,6 +412,9 @@ int acp_refined_enable_graphs(acp_refined* r, int32_t chunk);
  *   - each resident case runs on its own stream in chunks of `graph_chunk`
  *     steps replayed from captured HIP graphs (or `launch_steps` plain steps
  *     when graph_chunk is 0), with at most `chunks_ahead` chunks queued;
+ *   - with `pack` set, eligible one-level cases of one tier and order are
+ *     stepped together instead (acp_pack_*): one pack per tier and order,
+ *     each on its own stream, chunks of `graph_chunk` steps;
  *   - a finished case posts an acp_result and gives back its range and slot.
  * Results are those of the same case run alone: cases share no state. */
 enum { ACP_INIT_REFERENCE = 0, ACP_INIT_PULSE = 1, ACP_INIT_RANDOM = 2 };
@@ -389,6 +426,8 @@ typedef struct acp_server_desc {
     int32_t launch_steps;       /* steps per acp_case_step call when graph_chunk is 0 */
     int32_t chunks_ahead;       /* chunks queued per case before the driver waits on it */
     int32_t admit_threads;      /* host threads creating and initialising admitted cases (0: 4) */
+    int32_t pack;               /* 1: eligible one-level cases (acp_pack_add) of one tier and order share
+                                   packed step launches on one stream; 0: every case on its own stream */
 } acp_server_desc;
 
 typedef struct acp_request {
