@@ -48,7 +48,7 @@ class CaseInfo(ctypes.Structure):
 class ServerDesc(ctypes.Structure):
     _fields_ = [("hbm_budget_bytes", ctypes.c_int64), ("max_active", ctypes.c_int32),
                 ("graph_chunk", ctypes.c_int32), ("launch_steps", ctypes.c_int32), ("chunks_ahead", ctypes.c_int32),
-                ("admit_threads", ctypes.c_int32), ("pack", ctypes.c_int32)]
+                ("admit_threads", ctypes.c_int32), ("pack_max_nodes", ctypes.c_int32)]
 
 
 class Request(ctypes.Structure):

@@ -12,9 +12,10 @@
 //   * drives every resident case on its own non-blocking stream from a driver
 //     thread, keeping a bounded number of step chunks (captured HIP graphs)
 //     queued per case, so that the cases' kernels overlap on the device; or,
-//     with packing on, steps the eligible one-level cases of one tier and
-//     order together (acp_pack_*): one pack per tier and order, on a stream
-//     of its own, with the same bounded number of chunks queued per pack;
+//     with packing on, steps the eligible small one-level cases of one tier
+//     and order together (acp_pack_*): one pack per tier and order, on a
+//     stream of its own, with the same bounded number of chunks queued per
+//     pack;
 //   * on completion reads the probe series, mass and state checksum, releases
 //     the case's arena range and slot, and posts a result with its times.
 #include <hip/hip_runtime.h>
@@ -347,7 +348,9 @@ void drive_loop(acp_server* s) {
         // arrived cases join the pack of their tier and order when packing is
         // on and they are eligible; the others run on their own streams
         for (Job& j : arrived) {
-            if (s->cfg.pack && !j.r && j.status == ACP_OK && j.req.steps > 0 &&
+            const int64_t nodes = (int64_t)j.req.desc.nx * j.req.desc.ny * j.req.desc.nz;
+            if (s->cfg.pack_max_nodes > 0 && nodes <= s->cfg.pack_max_nodes && !j.r && j.status == ACP_OK &&
+                j.req.steps > 0 &&
                 acp::pack_eligible(j.c, j.req.desc.precision, j.req.desc.order) == ACP_OK) {
                 PackRun* pr = nullptr;
                 for (PackRun& x : packs)
@@ -503,7 +506,7 @@ void drive_loop(acp_server* s) {
 extern "C" int acp_server_create(const acp_server_desc* desc, acp_server** out) {
     if (!desc || !out) return acp::fail(ACP_E_ARG, "acp_server_create: null argument");
     *out = nullptr;
-    if (desc->max_active < 1 || desc->hbm_budget_bytes < 256 || desc->chunks_ahead < 0 ||
+    if (desc->max_active < 1 || desc->hbm_budget_bytes < 256 || desc->chunks_ahead < 0 || desc->pack_max_nodes < 0 ||
         (desc->graph_chunk != 0 && (desc->graph_chunk < 2 || desc->graph_chunk % 2 != 0)))
         return acp::fail(ACP_E_ARG, "acp_server_create: need max_active >= 1, a budget, an even graph chunk or 0");
     acp_server* s = new acp_server();

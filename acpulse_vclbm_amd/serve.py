@@ -107,10 +107,10 @@ class Server:
     """acp_server_* handle: submit requests, poll results, read statistics."""
 
     def __init__(self, hbm_budget_bytes, max_active=8, graph_chunk=16, launch_steps=16, chunks_ahead=4,
-                 admit_threads=4, pack=False):
+                 admit_threads=4, pack_max_nodes=0):
         L = lib()
         d = ServerDesc(int(hbm_budget_bytes), int(max_active), int(graph_chunk), int(launch_steps),
-                       int(chunks_ahead), int(admit_threads), int(bool(pack)))
+                       int(chunks_ahead), int(admit_threads), int(pack_max_nodes))
         h = ctypes.c_void_p()
         check(L.acp_server_create(ctypes.byref(d), ctypes.byref(h)))
         self.h = h
@@ -175,17 +175,23 @@ class Server:
 
 
 def run_mix(reqs, hbm_budget_bytes, max_active=8, graph_chunk=16, launch_steps=16, chunks_ahead=4,
-            admit_threads=4, timeout_s=3600.0, pack=False):
+            admit_threads=4, timeout_s=3600.0, pack_max_nodes=0, on_result=None):
     """Offer `reqs` (arrival times relative to the start) to a fresh server and
-    collect every result; `pack`: eligible one-level cases share packed step
-    launches.  Returns (results sorted by tag, summary dict)."""
-    with Server(hbm_budget_bytes, max_active, graph_chunk, launch_steps, chunks_ahead, admit_threads, pack) as srv:
+    collect every result; eligible one-level cases of at most `pack_max_nodes`
+    nodes share packed step launches; `on_result(result)` is called as each
+    result arrives.  Returns (results sorted by tag, summary dict)."""
+    with Server(hbm_budget_bytes, max_active, graph_chunk, launch_steps, chunks_ahead, admit_threads,
+                pack_max_nodes) as srv:
         for r in reqs:
             srv.submit(r)
         results = []
         t_end = time.monotonic() + timeout_s
         while len(results) < len(reqs):
-            results += srv.poll(64, wait_ms=200)
+            got = srv.poll(64, wait_ms=200)
+            if on_result:
+                for x in got:
+                    on_result(x)
+            results += got
             if time.monotonic() > t_end:
                 raise TimeoutError(f"{len(results)} of {len(reqs)} cases finished within {timeout_s} s")
         stats = srv.stats()
@@ -244,22 +250,63 @@ def run_alone(r, graph_chunk=0):
 
 
 def main():
+    """Run the seed-1 request mix on one GPU and print its summary as JSON.
+
+    The full mix as specified is ``--requests 256 --step-scale 1``.
+    ``--sequential-f64`` runs every request in float64, one at a time with one
+    launch sequence per step (the sequential baseline); ``--rate-per-hour``
+    offers Poisson arrivals (otherwise every request arrives at once, which
+    measures capacity).  ``--progress`` prints a line per finished case."""
     import argparse
     import json
 
     ap = argparse.ArgumentParser(description="run the serving request mix on one GPU")
     ap.add_argument("--requests", type=int, default=48)
     ap.add_argument("--step-scale", type=float, default=0.25)
-    ap.add_argument("--rate", type=float, default=0.0, help="Poisson arrivals per second; 0: all at once")
+    ap.add_argument("--rate-per-hour", type=float, default=0.0, help="Poisson arrivals per hour; 0: all at once")
     ap.add_argument("--max-active", type=int, default=16)
     ap.add_argument("--graph-chunk", type=int, default=16)
     ap.add_argument("--budget-gib", type=float, default=64.0)
-    ap.add_argument("--pack", action="store_true", help="one-level cases share packed step launches")
+    ap.add_argument("--pack-max-nodes", type=int, default=0,
+                    help="one-level cases of at most this many nodes share packed step launches")
+    ap.add_argument("--sequential-f64", action="store_true", help="float64, one case at a time")
+    ap.add_argument("--all-f64", action="store_true", help="every request in float64")
+    ap.add_argument("--progress", action="store_true")
     a = ap.parse_args()
-    reqs = make_requests(a.requests, seed=1, rate=a.rate or None, step_scale=a.step_scale)
-    _, s = run_mix(reqs, int(a.budget_gib * (1 << 30)), max_active=a.max_active, graph_chunk=a.graph_chunk,
-                   pack=a.pack)
-    print(json.dumps(s))
+    rate = a.rate_per_hour / 3600.0 if a.rate_per_hour > 0 else None
+    reqs = make_requests(a.requests, seed=1, rate=rate, step_scale=a.step_scale)
+    if a.sequential_f64 or a.all_f64:
+        reqs = [dict(r, precision="F64") for r in reqs]
+    t0 = time.monotonic()
+    done = [0]
+
+    def progress(x):
+        done[0] += 1
+        print(f"{done[0]}/{len(reqs)} tag {x['tag']} status {x['status']} steps {x['steps']} "
+              f"latency {x['t_done'] - x['t_arrival']:.2f} s wall {time.monotonic() - t0:.1f} s", flush=True)
+
+    kw = dict(on_result=progress if a.progress else None, pack_max_nodes=a.pack_max_nodes)
+    budget = int(a.budget_gib * (1 << 30))
+    if a.sequential_f64:
+        _, s = run_mix(reqs, budget, max_active=1, graph_chunk=0, launch_steps=1, chunks_ahead=64, admit_threads=1,
+                       **kw)
+    else:
+        _, s = run_mix(reqs, budget, max_active=a.max_active, graph_chunk=a.graph_chunk, **kw)
+    s.update({"requests": len(reqs), "step_scale": a.step_scale, "rate_per_hour": a.rate_per_hour,
+              "sequential_f64": a.sequential_f64, "all_f64": a.all_f64, "pack_max_nodes": a.pack_max_nodes,
+              "max_active": 1 if a.sequential_f64 else a.max_active, "mix": mix_counts(reqs),
+              "node_updates": int(sum(_node_updates(r) for r in reqs))})
+    print(json.dumps(s), flush=True)
+
+
+def _node_updates(r):
+    """Node updates of a request: steps x nodes, a two-level case counting its
+    fine level twice per coarse step."""
+    n = r["nx"] * r["ny"] * r["nz"]
+    if two_level(r):
+        x0, x1 = fine_region(r)
+        n += 2 * (2 * (x1 - x0) + 1) * (2 * r["ny"]) * (2 * r["nz"])
+    return n * r["steps"]
 
 
 if __name__ == "__main__":

@@ -166,14 +166,16 @@ def node_updates_per_step(serve, r):
     return n
 
 
-def serving(acp, n_req, step_scale, max_active, graph_chunk):
+def serving(acp, n_req, step_scale, max_active, graph_chunk, pack_max_nodes=0):
     """Serving workload: `n_req` acoustic-pulse requests of the request mix
     (seed 1; steps scaled by `step_scale`).  First the same requests, all in
     float64, run one at a time with one launch sequence per step (sequential
     capacity) and, mixed as requested, all at once to the concurrent server
     (its capacity); then Poisson arrivals at 1.2x the sequential capacity are
     offered to the concurrent server (mixed tiers, captured-graph chunks) and,
-    for comparison, to the sequential float64 server."""
+    for comparison, to the sequential float64 server.  Packed step launches
+    (one-level cases of at most `pack_max_nodes` nodes) are measured on the
+    same mix and on a mix of small one-level cases, where they pay."""
     from acpulse_vclbm_amd import serve
 
     budget = 64 << 30
@@ -192,6 +194,28 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
     _, seq = serve.run_mix(seq_reqs, budget, max_active=1, graph_chunk=0, launch_steps=1, chunks_ahead=64,
                            admit_threads=1)
     keep = ("cases_per_hour", "latency_p50_s", "latency_p99_s", "makespan_s", "failed")
+    packed = None
+    if pack_max_nodes > 0:
+        _, srv_cap_p = serve.run_mix(base, budget, max_active=max_active, graph_chunk=graph_chunk, chunks_ahead=4,
+                                     pack_max_nodes=pack_max_nodes)
+        # small one-level cases (128-192 nodes square): bound by launch latency
+        small = [dict(r, refined=False) for r in serve.make_requests(48, seed=3, step_scale=step_scale,
+                                                                     nxy=(128, 192))]
+        _, sm_streams = serve.run_mix(small, budget, max_active=32, graph_chunk=graph_chunk, chunks_ahead=4)
+        _, sm_packed = serve.run_mix(small, budget, max_active=32, graph_chunk=graph_chunk, chunks_ahead=4,
+                                     pack_max_nodes=pack_max_nodes)
+        packed = {
+            "pack_max_nodes": pack_max_nodes,
+            "server_capacity_per_hour": round(srv_cap_p["cases_per_hour"], 1),
+            "small_cases": {"workload": "48 one-level requests, nx, ny in 128-192, nz 3, seed 3, "
+                                        f"step counts x {step_scale}, max_active 32",
+                            "streams_cases_per_hour": round(sm_streams["cases_per_hour"], 1),
+                            "packed_cases_per_hour": round(sm_packed["cases_per_hour"], 1),
+                            "streams_p50_p99_s": [round(sm_streams["latency_p50_s"], 3),
+                                                  round(sm_streams["latency_p99_s"], 3)],
+                            "packed_p50_p99_s": [round(sm_packed["latency_p50_s"], 3),
+                                                 round(sm_packed["latency_p99_s"], 3)]},
+        }
     full = n_req == 256 and step_scale == 1.0
     return {
         "workload": ("the full request mix: 256 requests, 1000-8000 steps" if full else
@@ -205,6 +229,7 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk):
         "sequential_f64_same_arrivals": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in seq.items()
                                          if k in keep},
         "mix": serve.mix_counts(reqs),
+        "packed": packed,
         "node_steps": sum(node_updates_per_step(serve, r) * r["steps"] for r in reqs),
     }
 
@@ -317,6 +342,8 @@ def main():
     ap.add_argument("--serve-step-scale", type=float, default=0.25, help="1.0 for the full step counts")
     ap.add_argument("--serve-max-active", type=int, default=16)
     ap.add_argument("--serve-graph-chunk", type=int, default=16)
+    ap.add_argument("--serve-pack-max-nodes", type=int, default=1 << 18,
+                    help="one-level cases of at most this many nodes share packed launches (0: no packing)")
     ap.add_argument("--refine-graph-chunk", type=int, default=16,
                     help="coarse steps per graph replay in the refined-step timing (even; 0: plain launches only)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -355,7 +382,8 @@ def main():
               file=sys.stderr, flush=True)
     srv = None
     if not a.no_serving:
-        srv = serving(acp, a.serve_requests, a.serve_step_scale, a.serve_max_active, a.serve_graph_chunk)
+        srv = serving(acp, a.serve_requests, a.serve_step_scale, a.serve_max_active, a.serve_graph_chunk,
+                      a.serve_pack_max_nodes)
         print(f"# serving: capacity {srv['server_capacity_per_hour']:.0f} cases/h (sequential float64: "
               f"{srv['sequential_f64_capacity_per_hour']:.0f}); at 1.2x the sequential capacity p50 "
               f"{srv['server']['latency_p50_s']:.2f} s, p99 {srv['server']['latency_p99_s']:.2f} s (sequential: "

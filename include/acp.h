@@ -412,9 +412,11 @@ int acp_pack_members(const acp_pack* p, int32_t* members);
  *   - each resident case runs on its own stream in chunks of `graph_chunk`
  *     steps replayed from captured HIP graphs (or `launch_steps` plain steps
  *     when graph_chunk is 0), with at most `chunks_ahead` chunks queued;
- *   - with `pack` set, eligible one-level cases of one tier and order are
- *     stepped together instead (acp_pack_*): one pack per tier and order,
- *     each on its own stream, chunks of `graph_chunk` steps;
+ *   - with `pack_max_nodes` set, eligible one-level cases of at most that
+ *     many nodes are stepped together instead (acp_pack_*): one pack per tier
+ *     and order, each on its own stream, chunks of `graph_chunk` steps.  Small
+ *     cases are bound by launch latency, which packing removes; large ones
+ *     run faster on their own streams, where their kernels overlap;
  *   - a finished case posts an acp_result and gives back its range and slot.
  * Results are those of the same case run alone: cases share no state. */
 enum { ACP_INIT_REFERENCE = 0, ACP_INIT_PULSE = 1, ACP_INIT_RANDOM = 2 };
@@ -426,8 +428,9 @@ typedef struct acp_server_desc {
     int32_t launch_steps;       /* steps per acp_case_step call when graph_chunk is 0 */
     int32_t chunks_ahead;       /* chunks queued per case before the driver waits on it */
     int32_t admit_threads;      /* host threads creating and initialising admitted cases (0: 4) */
-    int32_t pack;               /* 1: eligible one-level cases (acp_pack_add) of one tier and order share
-                                   packed step launches on one stream; 0: every case on its own stream */
+    int32_t pack_max_nodes;     /* > 0: eligible one-level cases (acp_pack_add) of at most this many nodes
+                                   share packed step launches, one pack per tier and order; 0: every case
+                                   on its own stream */
 } acp_server_desc;
 
 typedef struct acp_request {

@@ -150,14 +150,14 @@ def _small_mix(n, seed):
 
 
 def test_packed_server_results_equal_alone(gpu):
-    """The serving mix with packing on (one-level cases of both tiers in packs,
-    two-level cases on their own streams): each result equals the same request
-    run alone, bit for bit."""
+    """The serving mix with packing on for every size (one-level cases of both
+    tiers in packs, two-level cases on their own streams): each result equals
+    the same request run alone, bit for bit."""
     reqs = _small_mix(16, seed=5)
     assert sum(not serve.two_level(r) for r in reqs) >= 6
     alone = [serve.run_alone(r) for r in reqs]
     results, summary = serve.run_mix(reqs, hbm_budget_bytes=1 << 30, max_active=8, graph_chunk=8, chunks_ahead=2,
-                                     timeout_s=120, pack=True)
+                                     timeout_s=120, pack_max_nodes=1 << 30)
     assert summary["failed"] == 0, [(r["tag"], r["status"], r["error"]) for r in results if r["status"]]
     for r, a, q in zip(results, alone, reqs):
         assert r["steps"] == q["steps"]
