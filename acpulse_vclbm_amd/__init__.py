@@ -155,7 +155,7 @@ class Case:
         """Run every full chunk of `chunk` (even) steps as one captured-graph replay; 0 switches off."""
         check(lib().acp_case_enable_graphs(self.h, int(chunk)))
 
-    BULK_VARIANTS = {"auto": 0, "wide_nt": 1, "narrow_nt": 2, "narrow_plain": 3}
+    BULK_VARIANTS = {"auto": 0, "wide_nt": 1, "narrow_nt": 2, "narrow_plain": 3, "mfma": 4}
 
     def set_bulk_variant(self, name):
         """Force the bulk kernels' addressing and streaming form ("auto",
@@ -234,6 +234,13 @@ class Refined:
         if name not in F2C_FILTER:
             raise AcpError(_capi.ACP_E_UNSUPPORTED, f"fine-to-coarse filter {name!r}: NOFI, LAG or TOU")
         check(lib().acp_refined_set_filter(self.h, F2C_FILTER[name]))
+
+    def set_scaling(self, style):
+        """Scaling-Style of the standard coupling: "SRTSTYLE" or "MRTSTYLE"."""
+        styles = {"SRTSTYLE": 0, "MRTSTYLE": 1}
+        if style not in styles:
+            raise AcpError(_capi.ACP_E_ARG, f"Scaling-Style {style!r}: SRTSTYLE or MRTSTYLE")
+        check(lib().acp_refined_set_scaling(self.h, styles[style]))
 
     def set_coupling(self, scheme="STD", c2f=None, f2c=None, hanging=None):
         """Coupling-Approach "STD" (default) or "DC1" / "DC2" (direct coupling),

@@ -146,6 +146,7 @@ def lib():
         "acp_refined_step": (ctypes.c_int, [vp, i32]),
         "acp_refined_set_coupling_precision": (ctypes.c_int, [vp, i32]),
         "acp_refined_set_filter": (ctypes.c_int, [vp, i32]),
+        "acp_refined_set_scaling": (ctypes.c_int, [vp, i32]),
         "acp_refined_set_coupling": (ctypes.c_int, [vp, i32, i32, i32, i32]),
         "acp_refined_write_vtk": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "acp_write_vtk2_host": (ctypes.c_int, [ctypes.POINTER(CaseDesc), dp, dp, i32, i32, dp, dp, ctypes.c_char_p]),

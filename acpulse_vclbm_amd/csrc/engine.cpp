@@ -388,6 +388,7 @@ void release(acp_case* c) {
     if (!c->arena)
         for (void* p : ptrs)
             if (p) (void)hipFree(p);
+    if (c->mfma_ct) (void)hipFree(c->mfma_ct);
     if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -794,7 +795,9 @@ int step_eager(acp_case* c, int32_t n) {
         hipError_t e = hipSuccess;
         if (!shifted(c)) {
             if (c->need_macro) e = acp::launch_macro64(c->dk, c->A, c->M, c->g, c->periodic, c->stream);
-            if (e == hipSuccess)
+            if (e == hipSuccess && c->g.variant == acp::BULK_MFMA)
+                e = acp::launch_bulk64_mfma(c->dk, c->A, c->B, c->mfma_ct, c->g, c->stream);
+            else if (e == hipSuccess)
                 e = acp::launch_bulk64(c->bulk_op, c->d.order, c->d.cubic != 0, c->periodic, c->dk, c->A, c->B,
                                        c->psi, c->M, c->g, c->stream);
             for (int x = 0; x < c->nfix && e == hipSuccess; ++x)
@@ -903,8 +906,19 @@ extern "C" int acp_case_enable_graphs(acp_case* c, int32_t chunk) {
 
 extern "C" int acp_case_set_bulk_variant(acp_case* c, int32_t variant) {
     if (int rc = check_case(c)) return rc;
-    if (variant < acp::BULK_AUTO || variant > acp::BULK_NARROW_PLAIN)
-        return fail(ACP_E_ARG, "unknown bulk-kernel variant");
+    if (variant < acp::BULK_AUTO || variant > acp::BULK_MFMA) return fail(ACP_E_ARG, "unknown bulk-kernel variant");
+    if (variant == acp::BULK_MFMA) {
+        if (shifted(c) || c->bulk_op != acp::OP_MRT_RM || !c->periodic || c->g.xopen || c->g.xlodi ||
+            c->g.nx % 256 != 0)
+            return fail(ACP_E_UNSUPPORTED, "the matrix-core form needs float64 MRT-RM (relaxed) on a periodic grid "
+                                           "with nx a multiple of 256");
+        if (!c->mfma_ct) {
+            double h[acp::kMfmaRows * acp::kMfmaCols];
+            acp::mrt_rm_collision_matrix(c->hk, h);
+            HIPCHK(hipMalloc((void**)&c->mfma_ct, sizeof(h)));
+            HIPCHK(upload(c, c->mfma_ct, h, sizeof(h)));
+        }
+    }
     const int eb = shifted(c) ? (int)elem_bytes(c->precision) : 8;
     if ((variant == acp::BULK_NARROW_NT || variant == acp::BULK_NARROW_PLAIN) &&
         !acp::narrow_planes(c->g.L, eb, shifted(c) ? 4 : 8))

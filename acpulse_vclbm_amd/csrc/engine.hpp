@@ -131,5 +131,7 @@ struct acp_case {
     // the pack that steps the case (acp_pack_add .. acp_pack_remove): its
     // arrays, step and probe counters are the pack's until then
     acp_pack* pack = nullptr;
+    // collision matrix of the matrix-core MRT-RM form (BULK_MFMA), outside any region
+    double* mfma_ct = nullptr;
     int64_t device_bytes = 0;
 };

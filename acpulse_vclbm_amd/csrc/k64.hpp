@@ -14,6 +14,11 @@ struct KShift;
 
 // bulk-kernel operator codes
 enum BulkOp : int { OP_SRT = 0, OP_MRT_GS = 1, OP_MRT_RM = 2, OP_RR = 3, OP_HRR = 4, OP_STREAM = 5 };
+// The MRT-RM collision on the matrix cores (bulk variant BULK_MFMA; periodic
+// float64 grids only, see k64_bulk.hip): a kernel form, never a case's operator.
+constexpr int OP_MRT_RM_MFMA = 6;
+// Ct of that form: the case's collision matrix, transposed and zero-padded.
+constexpr int kMfmaRows = 20, kMfmaCols = 32;
 
 // One boundary node handled by the LODI kernel (Node.cpp:229-504): its index, the
 // first and second normal neighbours (read from the B array), the four tangent
@@ -44,6 +49,14 @@ struct Grid {
 
 hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64* k, const double* A, double* B,
                          double* psi, const double* macro, const Grid& g, hipStream_t s);
+// The matrix-core form of the relaxed MRT-RM bulk kernel on a periodic grid
+// (nx a multiple of 64): Ct [kMfmaRows][kMfmaCols] as written by
+// mrt_rm_collision_matrix.
+hipError_t launch_bulk64_mfma(const K64* k, const double* A, double* B, const double* Ct, const Grid& g,
+                              hipStream_t s);
+// C^T of f* = f + C (feq - f), the relaxed MRT-RM collision as one matrix
+// (C = M^-1 S M), zero-padded to [kMfmaRows][kMfmaCols] (host arithmetic).
+void mrt_rm_collision_matrix(const K64& k, double* Ct);
 hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Grid& g, bool periodic,
                           hipStream_t s);
 // Workgroup size of a LODI phase: a phase of fewer than 16 Ki nodes (the x

@@ -47,6 +47,63 @@ __device__ __forceinline__ void load_neighbour_macros(const double* __restrict__
     }
 }
 
+// ---------------------------------------------------------------- MFMA trial
+// The relaxed MRT-RM collision (collide_mrt_rm64) as one 19 x 19 matrix per
+// case:  f* = f + C (feq - f),  C = M^-1 S M, with M the raw-moment transform,
+// S the rates (omega for moments 5-9, 1 for the others) and M^-1 its inverse;
+// in exact arithmetic the same update.  The 64 nodes of a wave form one block
+// X = feq - f (64 x 19), staged through LDS (rows padded to 65 doubles against
+// bank conflicts), and X C^T runs on the matrix cores: 4 x 2 output tiles of
+// 16 nodes x 16 populations, each five steps of v_mfma_f64_16x16x4f64 over the
+// 19 populations padded to 20.  Operand layout of that instruction for lane l
+// (checked on the device): A[l % 16][l / 16], B[l / 16][l % 16], and D rows
+// l / 16 + 4 v, column l % 16, v = 0..3.  Every thread of the workgroup
+// must take part (the launcher admits only grids whose rows fill whole
+// workgroups).  Ct: C^T, [kMfmaRows][kMfmaCols], zero beyond 19 x 19.
+typedef double mfma_d4 __attribute__((ext_vector_type(4)));
+constexpr int kMfmaBlock = 256;  // threads of a workgroup of the trial kernel (LDS: 4 waves x 20 x 65 doubles)
+
+__device__ __forceinline__ void collide_mrt_rm_mfma(const K64& k, double f[19], const double* __restrict__ Ct) {
+    __shared__ double X[kMfmaBlock / 64][kMfmaRows][65];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double m[19], eq[19];
+    rm_moments64(k, f, m);
+    rm_feq64(k, m[0], m[1] / m[0], m[2] / m[0], m[3] / m[0], eq);
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        X[w][q][lane] = eq[q] - f[q];
+    });
+    X[w][19][lane] = 0.;
+    __syncthreads();
+    const int r = lane >> 4, c = lane & 15;
+    mfma_d4 acc[2][4];
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            acc[nt][mt] = mfma_d4{0., 0., 0., 0.};
+#pragma unroll
+            for (int kk = 0; kk < 5; ++kk) {
+                const double a = X[w][4 * kk + r][16 * mt + c];
+                const double b = Ct[(4 * kk + r) * kMfmaCols + 16 * nt + c];
+                acc[nt][mt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[nt][mt], 0, 0, 0);
+            }
+        }
+    __syncthreads();
+#pragma unroll
+    for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                if (16 * nt + c < 19) X[w][16 * nt + c][16 * mt + r + 4 * v] = acc[nt][mt][v];
+    __syncthreads();
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        f[q] += X[w][q][lane];
+    });
+}
+
 // HRR without the cubic terms, orders 2 and 3, is held to 128 registers (four
 // waves per SIMD): at 512^3 order 3 runs 9.4 GLUPS there, with a few bytes of
 // spill, against 8.2 at the 130 registers and three waves it takes
@@ -136,6 +193,8 @@ __device__ __forceinline__ void bulk64_node(const K64* kp, const double* A, doub
             }
         } else if constexpr (OP == OP_MRT_RM) {
             collide_mrt_rm64(k, f);
+        } else if constexpr (OP == OP_MRT_RM_MFMA) {
+            collide_mrt_rm_mfma(k, f, M);  // M carries the collision matrix in this form
         } else if constexpr (OP == OP_MRT_GS) {
             collide_mrt_gs64(k, f);
         }
@@ -290,6 +349,53 @@ hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64
         case OP_STREAM: return go_p<OP_STREAM, 3, false>(periodic, k, A, B, psi, macro, g, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+// The trial kernel: the bulk kernel's periodic form with the matrix-core
+// collision, 256-thread workgroups along x.
+template <bool NARROW, bool NT>
+__global__ __launch_bounds__(kMfmaBlock) void bulk64_mfma(const K64* __restrict__ kp, const double* __restrict__ A,
+                                                          double* __restrict__ B, const double* __restrict__ Ct,
+                                                          int nx, int ny, int nz, const PopLayout L) {
+    const int i = blockIdx.x * kMfmaBlock + threadIdx.x;
+    bulk64_node<OP_MRT_RM_MFMA, 3, false, true, NARROW, NT>(kp, A, B, nullptr, Ct, nx, ny, nz, L, 0, 0, 0, i,
+                                                            (int)blockIdx.y, (int)blockIdx.z, true);
+}
+
+hipError_t launch_bulk64_mfma(const K64* k, const double* A, double* B, const double* Ct, const Grid& g,
+                              hipStream_t s) {
+    if (g.nx % kMfmaBlock != 0 || g.xopen || g.xlodi) return hipErrorInvalidValue;
+    const dim3 grid(g.nx / kMfmaBlock, g.ny, g.nz), block(kMfmaBlock);
+    const BulkForm form = bulk_form(g.L, sizeof(double), sizeof(double), BULK_AUTO);
+    if (!form.narrow) hipLaunchKernelGGL((bulk64_mfma<false, true>), grid, block, 0, s, k, A, B, Ct, g.nx, g.ny, g.nz, g.L);
+    else if (form.nt) hipLaunchKernelGGL((bulk64_mfma<true, true>), grid, block, 0, s, k, A, B, Ct, g.nx, g.ny, g.nz, g.L);
+    else hipLaunchKernelGGL((bulk64_mfma<true, false>), grid, block, 0, s, k, A, B, Ct, g.nx, g.ny, g.nz, g.L);
+    return hipGetLastError();
+}
+
+void mrt_rm_collision_matrix(const K64& k, double* Ct) {
+    // M (moments of the unit populations), M^-1 (populations of the unit
+    // moments), S; C = M^-1 S M
+    double Mm[19][19], Mi[19][19];
+    for (int q = 0; q < 19; ++q) {
+        double e[19] = {0.}, m[19];
+        e[q] = 1.;
+        rm_moments64(k, e, m);
+        for (int r = 0; r < 19; ++r) Mm[r][q] = m[r];
+    }
+    for (int r = 0; r < 19; ++r) {
+        double e[19] = {0.}, f[19];
+        e[r] = 1.;
+        rm_inverse64(k, e, f);
+        for (int q = 0; q < 19; ++q) Mi[q][r] = f[q];
+    }
+    for (int i = 0; i < kMfmaRows * kMfmaCols; ++i) Ct[i] = 0.;
+    for (int q = 0; q < 19; ++q)        // output population
+        for (int p = 0; p < 19; ++p) {  // input population
+            double s = 0.;
+            for (int r = 0; r < 19; ++r) s += Mi[q][r] * ((r >= 5 && r <= 9) ? k.omega : 1.) * Mm[r][p];
+            Ct[p * kMfmaCols + q] = s;
+        }
 }
 
 template <int OP>

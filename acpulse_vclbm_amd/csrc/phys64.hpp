@@ -546,8 +546,9 @@ __host__ __device__ constexpr int gs_coef(int r, int q) {
 }
 constexpr int GS_PW[19] = {0, 2, 4, 1, 3, 1, 3, 1, 3, 2, 4, 2, 4, 2, 2, 2, 3, 3, 3};
 
-__host__ __device__ __forceinline__ void collide_mrt_gs64(const K64& k, double f[19]) {
-    double m[19];
+// Gram-Schmidt moments of a population set (calcAllMoments_GSbasis,
+// Node.cpp:2934-3004: each row summed over q in order, then times its power of c)
+__host__ __device__ __forceinline__ void gs_moments64(const K64& k, const double f[19], double m[19]) {
     static_for<0, 19>([&](auto rc) {
         constexpr int r = decltype(rc)::value;
         double s = 0.;
@@ -562,6 +563,31 @@ __host__ __device__ __forceinline__ void collide_mrt_gs64(const K64& k, double f
         if constexpr (r == 0) m[r] = s;
         else m[r] = s * k.gs_cp[GS_PW[r]];
     });
+}
+
+// Populations of a set of Gram-Schmidt moments
+// (transformMomentsToDistributions_GSbasis, Node.cpp:2870-2931: each
+// population summed over the rows in order, starting from the first term)
+__host__ __device__ __forceinline__ void gs_inverse64(const K64& k, const double m[19], double f[19]) {
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        double s = 0.;
+        bool first = true;
+        static_for<0, 19>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if constexpr (gs_coef(r, q) != 0) {
+                const double t = k.gs_pre[r][q] * m[r];
+                s = first ? t : s + t;
+                first = false;
+            }
+        });
+        f[q] = s;
+    });
+}
+
+__host__ __device__ __forceinline__ void collide_mrt_gs64(const K64& k, double f[19]) {
+    double m[19];
+    gs_moments64(k, f, m);
     // equilibrium moments (Node.cpp:2838-2867) and rates (Node.cpp:2815-2825)
     const double ir = 1. / m[0], jx = m[3], jy = m[5], jz = m[7];
     const double xx = jx * jx, yy = jy * jy, zz = jz * jz;
@@ -579,19 +605,11 @@ __host__ __device__ __forceinline__ void collide_mrt_gs64(const K64& k, double f
         else if constexpr (r == 1 || r == 9 || r == 11 || r == 13 || r == 14 || r == 15) dm[r] = k.omega * (meq[r] - m[r]);
         else dm[r] = 1. * (meq[r] - m[r]);
     });
+    double df[19];
+    gs_inverse64(k, dm, df);
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        double s = 0.;
-        bool first = true;
-        static_for<0, 19>([&](auto rc) {
-            constexpr int r = decltype(rc)::value;
-            if constexpr (gs_coef(r, q) != 0) {
-                const double t = k.gs_pre[r][q] * dm[r];
-                s = first ? t : s + t;
-                first = false;
-            }
-        });
-        f[q] += s;
+        f[q] += df[q];
     });
 }
 

@@ -152,7 +152,7 @@ inline bool narrow_planes(const PopLayout& L, int pop_bytes, int plane_bytes) {
 // last-level cache.  Tests force the forms that only large grids take onto
 // small grids, so that every instantiation the large cases run is compared
 // with the oracle; a narrow form is never forced onto arrays it cannot address.
-enum BulkVariant : int { BULK_AUTO = 0, BULK_WIDE_NT = 1, BULK_NARROW_NT = 2, BULK_NARROW_PLAIN = 3 };
+enum BulkVariant : int { BULK_AUTO = 0, BULK_WIDE_NT = 1, BULK_NARROW_NT = 2, BULK_NARROW_PLAIN = 3, BULK_MFMA = 4 };
 struct BulkForm {
     bool narrow, nt;
 };

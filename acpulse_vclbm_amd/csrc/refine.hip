@@ -60,14 +60,48 @@ namespace acp {
 
 namespace refine {
 
-// density and velocity (SI) and feq of one node's populations
+// Scaling-Style of the standard coupling (BoundaryConditions.txt:89, chosen at
+// Lattice.cpp:4634-4704): the non-equilibrium part n = f - feq of a
+// population set handed between the levels is scaled by one factor a
+// (SRTSTYLE: tau_f / (2 tau_c) coarse to fine, its inverse fine to coarse) or,
+// MRTSTYLE, moment by moment in the Gram-Schmidt basis of the MRT-GS operator
+// by m[r]: tau_f / (2 tau_c) for the moments that operator relaxes with omega,
+// 1/2 for those it relaxes at rate 1, 0 for the conserved ones, the inverses
+// fine to coarse (the oracle's scale_neq; the reference's
+// interpolTimeScalingCoarseToFineMRT_GSbasis and scalingFineToCoarseMRT*_GSbasis
+// are declared at GridCoupling.h:30-36 in the absent GridCoupling.cpp).
+// Passed to the kernels by value.
+struct NeqScale {
+    int mrt;
+    double a;
+    double m[19];
+};
+
+__device__ __forceinline__ void scale_neq(const K64& k, double n[19], const NeqScale& s) {
+    if (!s.mrt) {
+#pragma unroll
+        for (int q = 0; q < 19; ++q) n[q] = s.a * n[q];
+        return;
+    }
+    double m[19];
+    gs_moments64(k, n, m);
+#pragma unroll
+    for (int r = 0; r < 19; ++r) m[r] = m[r] * s.m[r];
+    gs_inverse64(k, m, n);
+}
+
+// density and velocity (SI) and feq of one node's populations; the
+// non-equilibrium part rescaled
 template <int ORDER>
-__device__ __forceinline__ void rescale(const K64& k, double f[19], double a) {
-    double rho, ux, uy, uz, eq[19];
+__device__ __forceinline__ void rescale(const K64& k, double f[19], const NeqScale& s) {
+    double rho, ux, uy, uz, eq[19], n[19];
     moments64(k, f, rho, ux, uy, uz);
     feq64<ORDER>(k, rho, ux, uy, uz, eq);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) f[q] = eq[q] + a * (f[q] - eq[q]);
+    for (int q = 0; q < 19; ++q) n[q] = f[q] - eq[q];
+    scale_neq(k, n, s);
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = eq[q] + n[q];
 }
 
 // The populations a coincident fine face node takes from coarse node lc:
@@ -77,7 +111,7 @@ __device__ __forceinline__ void rescale(const K64& k, double f[19], double a) {
 template <int ORDER>
 __device__ __forceinline__ void c2f_value(const K64& k, const double* __restrict__ Cprev,
                                           const double* __restrict__ Ccur, const PopLayout& CN, int64_t lc, int half,
-                                          double a, double f[19]) {
+                                          const NeqScale& a, double f[19]) {
     const int64_t o = (lc >> CN.shift) * CN.block + (lc & CN.mask);
     double prev[19];
 #pragma unroll
@@ -100,7 +134,7 @@ __device__ __forceinline__ void c2f_value(const K64& k, const double* __restrict
 // average their 2 (mid nodes) or 4 (centre nodes) coincident neighbours.
 template <int ORDER>
 __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp, const double* __restrict__ Cprev,
-                                                      const double* __restrict__ Ccur, int half, double a, int cnx,
+                                                      const double* __restrict__ Ccur, int half, const NeqScale a, int cnx,
                                                       int cny, int cnz, int ci0, int ci1, double* __restrict__ F,
                                                       int fnx, int fny, const PopLayout CN, const PopLayout FN,
                                                       int nfaces) {
@@ -182,7 +216,7 @@ constexpr int kLagNodes = 13;  // coarse nodes per 256-thread workgroup (247 lan
 // the coarse node keeps its coincident fine node's density and velocity.
 template <int ORDER, bool TOU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ? 4 : 3))) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
-                                               int fny, int fnz, double ainv, double* __restrict__ C, int cnx,
+                                               int fny, int fnz, const NeqScale ainv, double* __restrict__ C, int cnx,
                                                int cny, int cnz, int x0, int width, const PopLayout CN,
                                                const PopLayout FN) {
     __shared__ double neq[kLagNodes][19][19];  // [node][fine node s][direction]
@@ -225,6 +259,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
     const int q = s;
     const int I = (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
+    double nq = 0.;  // the non-equilibrium part lane q hands to population q
     if constexpr (TOU) {
         __shared__ double nbar[kLagNodes][19];
         if (active) {
@@ -236,42 +271,93 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
             nbar[node][q] = nb;
         }
         __syncthreads();
-        if (!active) return;
-        double M = 0., Jx = 0., Jy = 0., Jz = 0.;
-        static_for<0, 19>([&](auto qc) {
-            constexpr int qq = decltype(qc)::value;
-            const double v = nbar[node][qq];
-            M += v;
-            if constexpr (EX[qq] != 0) Jx += (EX[qq] > 0 ? k.c : -k.c) * v;
-            if constexpr (EY[qq] != 0) Jy += (EY[qq] > 0 ? k.c : -k.c) * v;
-            if constexpr (EZ[qq] != 0) Jz += (EZ[qq] > 0 ? k.c : -k.c) * v;
-        });
-        double sv = M, w = 0.;
-        static_for<0, 19>([&](auto qc) {
-            constexpr int qq = decltype(qc)::value;
-            if (q == qq) {
-                if constexpr (EX[qq] != 0) sv += k.rcs2 * ((EX[qq] > 0 ? k.c : -k.c) * Jx);
-                if constexpr (EY[qq] != 0) sv += k.rcs2 * ((EY[qq] > 0 ? k.c : -k.c) * Jy);
-                if constexpr (EZ[qq] != 0) sv += k.rcs2 * ((EZ[qq] > 0 ? k.c : -k.c) * Jz);
-                w = weight(qq);
-            }
-        });
-        C[pidx(CN, q, lc)] = eq0[node][q] + ainv * (nbar[node][q] - w * sv);
-    } else {
-        if (!active) return;
+        if (active) {
+            double M = 0., Jx = 0., Jy = 0., Jz = 0.;
+            static_for<0, 19>([&](auto qc) {
+                constexpr int qq = decltype(qc)::value;
+                const double v = nbar[node][qq];
+                M += v;
+                if constexpr (EX[qq] != 0) Jx += (EX[qq] > 0 ? k.c : -k.c) * v;
+                if constexpr (EY[qq] != 0) Jy += (EY[qq] > 0 ? k.c : -k.c) * v;
+                if constexpr (EZ[qq] != 0) Jz += (EZ[qq] > 0 ? k.c : -k.c) * v;
+            });
+            double sv = M, w = 0.;
+            static_for<0, 19>([&](auto qc) {
+                constexpr int qq = decltype(qc)::value;
+                if (q == qq) {
+                    if constexpr (EX[qq] != 0) sv += k.rcs2 * ((EX[qq] > 0 ? k.c : -k.c) * Jx);
+                    if constexpr (EY[qq] != 0) sv += k.rcs2 * ((EY[qq] > 0 ? k.c : -k.c) * Jy);
+                    if constexpr (EZ[qq] != 0) sv += k.rcs2 * ((EZ[qq] > 0 ? k.c : -k.c) * Jz);
+                    w = weight(qq);
+                }
+            });
+            nq = nbar[node][q] - w * sv;
+        }
+    } else if (active) {
         double acc = neq[node][0][q];
 #pragma unroll
         for (int m = 1; m < 19; ++m) acc += neq[node][m][q];
-        C[pidx(CN, q, lc)] = eq0[node][q] + ainv * (acc / 19.);
+        nq = acc / 19.;
     }
+    if (!ainv.mrt) {
+        if (active) C[pidx(CN, q, lc)] = eq0[node][q] + ainv.a * nq;
+        return;
+    }
+    // MRTSTYLE: the node's 19 parts gathered in LDS, lane r forms moment r
+    // (gs_moments64's order), scales it, and lane q the population q of the
+    // scaled moments (gs_inverse64's order)
+    __shared__ double part[kLagNodes][19];
+    if (active) part[node][q] = nq;
+    __syncthreads();
+    double mr = 0.;
+    if (active) {
+        static_for<0, 19>([&](auto rc) {
+            constexpr int r = decltype(rc)::value;
+            if (q == r) {
+                double sum = 0.;
+                static_for<0, 19>([&](auto qc) {
+                    constexpr int qq = decltype(qc)::value;
+                    constexpr int cf = gs_coef(r, qq);
+                    if constexpr (cf == 1) sum += part[node][qq];
+                    else if constexpr (cf == -1) sum -= part[node][qq];
+                    else if constexpr (cf == 2) sum += 2. * part[node][qq];
+                    else if constexpr (cf == -2) sum -= 2. * part[node][qq];
+                });
+                mr = (r == 0 ? sum : sum * k.gs_cp[GS_PW[r]]) * ainv.m[r];
+            }
+        });
+    }
+    __syncthreads();
+    if (active) part[node][q] = mr;
+    __syncthreads();
+    if (!active) return;
+    double out = 0.;
+    static_for<0, 19>([&](auto qc) {
+        constexpr int qq = decltype(qc)::value;
+        if (q == qq) {
+            double sum = 0.;
+            bool first = true;
+            static_for<0, 19>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                if constexpr (gs_coef(r, qq) != 0) {
+                    const double t = k.gs_pre[r][qq] * part[node][r];
+                    sum = first ? t : sum + t;
+                    first = false;
+                }
+            });
+            out = sum;
+        }
+    });
+    C[pidx(CN, q, lc)] = eq0[node][q] + out;
 }
 
 // Coarse nodes with x in [x0 + 1, x1 - 1] from their coincident fine nodes
 // (F2C-Filter NOFI; LAG is f2c_lag).
 template <int ORDER>
 __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
-                                           int fny, int fnz, double ainv, double* __restrict__ C, int cnx, int cny,
-                                           int cnz, int x0, int width, const PopLayout CN, const PopLayout FN) {
+                                           int fny, int fnz, const NeqScale ainv, double* __restrict__ C, int cnx,
+                                           int cny, int cnz, int x0, int width, const PopLayout CN,
+                                           const PopLayout FN) {
     __shared__ K64 sk;
     const K64& k = stage_k64(kp, sk);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -318,7 +404,7 @@ __device__ __forceinline__ void load_raw19(const T* __restrict__ A, const PopLay
 
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void couple_value(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
-                                             const PopLayout& CN, int64_t lc, int half, double a, T out[19]) {
+                                             const PopLayout& CN, int64_t lc, int half, const NeqScale& a, T out[19]) {
     // every load ahead of the arithmetic (one memory latency for all of them)
     T cur[19], prev[19];
     load_raw19(Ccur, CN, lc, cur);
@@ -349,7 +435,7 @@ __device__ __forceinline__ void couple_value(const K64& k, const KShift& ks, con
         });
         moments_dev(d, dr, ux, uy, uz);
         feq_dev<ORDER>(dr, ux, uy, uz, eq, ks.k6);
-        const float af = (float)a;
+        const float af = (float)a.a;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             out[q] = (T)((eq[q] + af * (d[q] - eq[q])) * ks.s);
@@ -359,7 +445,7 @@ __device__ __forceinline__ void couple_value(const K64& k, const KShift& ks, con
 
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
-                                            const PopLayout& CN, int64_t lc, int half, double a, T* F,
+                                            const PopLayout& CN, int64_t lc, int half, const NeqScale& a, T* F,
                                             const PopLayout& FN, int64_t lf) {
     T v[19];
     couple_value<T, ORDER, F64>(k, ks, Cprev, Ccur, CN, lc, half, a, v);
@@ -371,7 +457,7 @@ __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, cons
 // on SI populations (F64) or in float32 on lattice-unit deviations.
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
-                                           int fnz, int fx, int fy, int fz, double ainv, T* __restrict__ C,
+                                           int fnz, int fx, int fy, int fz, const NeqScale& ainv, T* __restrict__ C,
                                            const PopLayout& CN, int64_t lc, const PopLayout& FN) {
     const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
     T raw[19];
@@ -402,9 +488,12 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
 #pragma unroll
             for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
         });
+#pragma unroll
+        for (int q = 0; q < 19; ++q) acc[q] = acc[q] / 19.;
+        scale_neq(k, acc, ainv);
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            C[pidx(CN, q, lc)] = f_to_g<T>(ks, eq[q] + ainv * (acc[q] / 19.), q);
+            C[pidx(CN, q, lc)] = f_to_g<T>(ks, eq[q] + acc[q], q);
         });
     } else {
         float d[19], eq[19], acc[19], dr, ux, uy, uz;
@@ -432,7 +521,7 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
 #pragma unroll
             for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
         });
-        const float af = (float)ainv;
+        const float af = (float)ainv.a;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
             C[pidx(CN, q, lc)] = (T)((eq[q] + af * (acc[q] * (1.f / 19.f))) * ks.s);
@@ -445,7 +534,7 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
 // on lattice-unit deviations.
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void tou_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
-                                           int fnz, int fx, int fy, int fz, double ainv, T* __restrict__ C,
+                                           int fnz, int fx, int fy, int fz, const NeqScale& ainv, T* __restrict__ C,
                                            const PopLayout& CN, int64_t lc, const PopLayout& FN) {
     using R = std::conditional_t<F64, double, float>;
     const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
@@ -500,14 +589,25 @@ __device__ __forceinline__ void tou_node_s(const K64& k, const KShift& ks, const
         if constexpr (EY[q] != 0) Jy += (EY[q] > 0 ? c : -c) * nb[q];
         if constexpr (EZ[q] != 0) Jz += (EZ[q] > 0 ? c : -c) * nb[q];
     });
-    const R ai = (R)ainv;
+    R n[19];
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         R sv = M;
         if constexpr (EX[q] != 0) sv += rcs2 * ((EX[q] > 0 ? c : -c) * Jx);
         if constexpr (EY[q] != 0) sv += rcs2 * ((EY[q] > 0 ? c : -c) * Jy);
         if constexpr (EZ[q] != 0) sv += rcs2 * ((EZ[q] > 0 ? c : -c) * Jz);
-        const R out = eq[q] + ai * (nb[q] - (R)weight(q) * sv);
+        n[q] = nb[q] - (R)weight(q) * sv;
+    });
+    if constexpr (F64) {
+        scale_neq(k, n, ainv);
+    } else {
+        const R ai = (R)ainv.a;
+#pragma unroll
+        for (int q = 0; q < 19; ++q) n[q] = ai * n[q];
+    }
+    static_for<0, 19>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        const R out = eq[q] + n[q];
         if constexpr (F64) C[pidx(CN, q, lc)] = f_to_g<T>(ks, out, q);
         else C[pidx(CN, q, lc)] = (T)(out * ks.s);
     });
@@ -519,7 +619,7 @@ __device__ __forceinline__ void tou_node_s(const K64& k, const KShift& ks, const
 template <typename T, int ORDER, bool F64>
 __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                                         const T* __restrict__ Cprev, const T* __restrict__ Ccur,
-                                                        int half, double a, int cnx, int cny, int cnz, int ci0,
+                                                        int half, const NeqScale a, int cnx, int cny, int cnz, int ci0,
                                                         int ci1, T* __restrict__ F, int fnx, int fny,
                                                         const PopLayout CN, const PopLayout FN, int nfaces) {
     __shared__ K64 sk;
@@ -587,7 +687,7 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
 
 template <typename T, int ORDER, bool F64, int FILT>
 __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
-                                             const T* __restrict__ F, int fnx, int fny, int fnz, double ainv,
+                                             const T* __restrict__ F, int fnx, int fny, int fnz, const NeqScale ainv,
                                              T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width,
                                              const PopLayout CN, const PopLayout FN) {
     __shared__ K64 sk;
@@ -1064,6 +1164,7 @@ struct acp_refined {
     acp_case* fine = nullptr;
     int x0 = 0, x1 = 0;
     double a = 1.0;        // tau_f / (2 tau_c)
+    bool mrt_style = false;  // Scaling-Style MRTSTYLE (else SRTSTYLE)
     bool f64_coupling = true;  // shifted tiers: coupling arithmetic in float64 (else float32)
     int filter = ACP_F2C_NOFI;  // fine-to-coarse filter (F2C-Filter)
     hipStream_t stream = nullptr;
@@ -1092,6 +1193,22 @@ using acp::fail;
 // fine level's high-x face is a LODI face.
 int coupled_faces(const acp_refined* r) { return (r->fine->g.xlodi & 2) ? 1 : 2; }
 
+// The Scaling-Style factors of the standard coupling, coarse to fine or fine
+// to coarse (the Gram-Schmidt moments the MRT-GS operator relaxes with omega,
+// Node.cpp:2815-2825, take the SRTSTYLE factor; the conserved ones 0; the
+// others 1/2, or 2 fine to coarse).
+acp::refine::NeqScale neq_scale(const acp_refined* r, bool f2c) {
+    acp::refine::NeqScale s{};
+    s.mrt = r->mrt_style ? 1 : 0;
+    s.a = f2c ? 1.0 / r->a : r->a;
+    for (int m = 0; m < 19; ++m) {
+        const bool conserved = m == 0 || m == 3 || m == 5 || m == 7;
+        const bool omega = m == 1 || m == 9 || m == 11 || m == 13 || m == 14 || m == 15;
+        s.m[m] = conserved ? 0. : (omega ? s.a : (f2c ? 2.0 : 0.5));
+    }
+    return s;
+}
+
 // Coarse-to-fine on the coupled fine faces: coincident nodes, then the
 // others.  These are a few thousand latency-bound threads each: 64-thread
 // workgroups spread them over more CUs.
@@ -1105,7 +1222,8 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
     const dim3 b(64), g1((unsigned)((nf * cny * cnz + 63) / 64)), g2((unsigned)((nf * fny * fnz + 63) / 64));
 #define C2FS(O)                                                                                                  \
     hipLaunchKernelGGL((acp::refine::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks,            \
-                       (const T*)c->SB, (const T*)c->SA, half, r->a, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx,     \
+                       (const T*)c->SB, (const T*)c->SA, half, neq_scale(r, false), cnx, cny, cnz, r->x0 % cnx,   \
+                       r->x1 % cnx,                                                                               \
                        (T*)f->SA, fnx, fny, c->g.L, f->g.L, nf)
     if (c->d.order == 2) C2FS(2);
     else if (c->d.order == 3) C2FS(3);
@@ -1127,7 +1245,8 @@ hipError_t launch_f2c_s(acp_refined* r) {
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define F2CS(O, FI)                                                                                              \
     hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64, FI>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, \
-                       f->g.ny, f->g.nz, 1.0 / r->a, (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
+                       f->g.ny, f->g.nz, neq_scale(r, true), (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width,   \
+                       c->g.L, f->g.L)
 #define F2CS_O(FI)                       \
     if (c->d.order == 2) F2CS(2, FI);    \
     else if (c->d.order == 3) F2CS(3, FI); \
@@ -1156,7 +1275,8 @@ hipError_t launch_c2f(acp_refined* r, int half) {
     const int nf = coupled_faces(r);
     const dim3 b(64), g1((unsigned)((nf * cny * cnz + 63) / 64)), g2((unsigned)((nf * fny * fnz + 63) / 64));
 #define C2F(O)                                                                                                    \
-    hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half, r->a, cnx, \
+    hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half,           \
+                       neq_scale(r, false), cnx,                                                                  \
                        cny, cnz, r->x0 % cnx, r->x1 % cnx, f->A, fnx, fny, c->g.L, f->g.L, nf)
     if (c->d.order == 2) C2F(2);
     else if (c->d.order == 3) C2F(3);
@@ -1181,12 +1301,14 @@ hipError_t launch_f2c(acp_refined* r) {
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define F2C(O)                                                                                                  \
-    hipLaunchKernelGGL((acp::refine::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, \
+    hipLaunchKernelGGL((acp::refine::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz,       \
+                       neq_scale(r, true),                                                                        \
                        c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
 #define F2CLAG(O, TOU)                                                                                           \
     hipLaunchKernelGGL((acp::refine::f2c_lag<O, TOU>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) /        \
                                                                         acp::refine::kLagNodes)),                \
-                       b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, 1.0 / r->a, c->A, c->g.nx, c->g.ny, \
+                       b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, neq_scale(r, true), c->A, c->g.nx, \
+                       c->g.ny,                                                                                   \
                        c->g.nz, r->x0, width, c->g.L, f->g.L)
     if (r->filter != ACP_F2C_NOFI) {
         const bool tou = r->filter == ACP_F2C_TOU;
@@ -1655,9 +1777,23 @@ extern "C" int acp_refined_set_coupling_precision(acp_refined* r, int32_t precis
         return fail(ACP_E_ARG, "coupling precision: ACP_PREC_F64 or ACP_PREC_F32_SHIFTED");
     if (r->coarse->precision == ACP_PREC_F64 && precision != ACP_PREC_F64)
         return fail(ACP_E_UNSUPPORTED, "a float64 refined case couples in float64");
+    if (r->mrt_style && precision != ACP_PREC_F64)
+        return fail(ACP_E_UNSUPPORTED, "Scaling-Style MRTSTYLE needs the coupling arithmetic in float64");
     if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
     drop_graphs(r);
     r->f64_coupling = precision == ACP_PREC_F64;
+    return ACP_OK;
+}
+
+extern "C" int acp_refined_set_scaling(acp_refined* r, int32_t style) {
+    if (!r) return fail(ACP_E_ARG, "acp_refined_set_scaling: null argument");
+    if (style != ACP_SCALING_SRT && style != ACP_SCALING_MRT)
+        return fail(ACP_E_ARG, "Scaling-Style: ACP_SCALING_SRT or ACP_SCALING_MRT");
+    if (style == ACP_SCALING_MRT && !r->f64_coupling)
+        return fail(ACP_E_UNSUPPORTED, "Scaling-Style MRTSTYLE needs the coupling arithmetic in float64");
+    if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
+    drop_graphs(r);
+    r->mrt_style = style == ACP_SCALING_MRT;
     return ACP_OK;
 }
 

@@ -55,12 +55,13 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def run_tier(acp, torch, precision, n, steps, warmup, seed=0, op="MRT-RM", order=3):
+def run_tier(acp, torch, precision, n, steps, warmup, seed=0, op="MRT-RM", order=3, variant="auto"):
     stream = torch.cuda.Stream()
     d = acp.make_desc(n, n, n, op=op, order=order, boundary="PERIODIC", mrt_mode="RELAXED",
                       precision=precision, tau=0.503, sigma=0.98)
     case = acp.Case(d, stream=stream.cuda_stream)
     try:
+        case.set_bulk_variant(variant)
         case.init_random(seed, 1e-3, 1e-3)
         m0 = case.mass()
         case.step(warmup)
@@ -338,6 +339,7 @@ def main():
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--no-refinement", action="store_true")
     ap.add_argument("--no-lodi", action="store_true")
+    ap.add_argument("--no-mfma-trial", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48, help="256 for the full request mix")
     ap.add_argument("--serve-step-scale", type=float, default=0.25, help="1.0 for the full step counts")
     ap.add_argument("--serve-max-active", type=int, default=16)
@@ -375,6 +377,15 @@ def main():
               f"{100 * res[t]['frac']:.1f}% of HBM peak", file=sys.stderr, flush=True)
     main_t = "F64" if "F64" in res else tiers[0]
     r = res[main_t]
+    mfma = None
+    if not a.no_mfma_trial and "F64" in res and a.op == "MRT-RM" and a.n % 256 == 0:
+        # the matrix-core trial of the MRT-RM collision (ACP_BULK_MFMA), same case and timing
+        m = run_tier(acp, torch, "F64", a.n, a.steps, a.warmup, op=a.op, order=a.order, variant="mfma")
+        mfma = {"glups": round(m["glups"], 4), "ms_per_step": round(m["ms_per_step"], 4),
+                "frac": round(m["frac"], 4), "mass_drift_rel": m["mass_drift_rel"],
+                "over_vector_form": round(m["glups"] / res["F64"]["glups"], 3)}
+        print(f"# MFMA trial (MRT-RM collision matrix on v_mfma_f64_16x16x4f64): {m['glups']:.2f} GLUPS, "
+              f"{mfma['over_vector_form']:.3f}x the vector form", file=sys.stderr, flush=True)
     ceil = None if a.no_copy_ceiling else copy_ceiling(acp, torch, a.n)
     if ceil is not None:
         print(f"# copy ceiling: engine layout (blocks of {LAYOUT_BLOCK_NODES} nodes) {ceil['layout_gbs']:.0f} GB/s, "
@@ -447,6 +458,7 @@ def main():
                      "frac": round(r["frac"], 4),
                      "traffic": traffic_from_profiles(f"bulk64_mrtrm_periodic_{a.n}.txt")},
         "tiers": {t: {k: (round(v, 6) if isinstance(v, float) else v) for k, v in res[t].items()} for t in res},
+        "mfma_trial": mfma,
         "copy_ceiling": None if ceil is None else {
             "layout_gbs": round(ceil["layout_gbs"], 1), "layout_block_nodes": LAYOUT_BLOCK_NODES,
             "planes19_gbs": round(ceil["planes19_gbs"], 1), "planes1_gbs": round(ceil["planes1_gbs"], 1),

@@ -179,8 +179,14 @@ int acp_case_enable_graphs(acp_case* c, int32_t chunk);
  * cache.  The other values force one form on any grid, so that tests can
  * compare the forms only large grids take (the 512^3 float64 grid runs
  * ACP_BULK_WIDE_NT) with the oracle on small grids; results are identical in
- * every form.  A 32-bit form on arrays that need 64-bit offsets is ACP_E_ARG. */
-enum { ACP_BULK_AUTO = 0, ACP_BULK_WIDE_NT = 1, ACP_BULK_NARROW_NT = 2, ACP_BULK_NARROW_PLAIN = 3 };
+ * every form.  A 32-bit form on arrays that need 64-bit offsets is ACP_E_ARG.
+ * ACP_BULK_MFMA is the matrix-core trial of the relaxed MRT-RM collision:
+ * f* = f + C (feq - f) with the case's 19 x 19 collision matrix C = M^-1 S M
+ * on v_mfma_f64_16x16x4f64 (float64, periodic grids with nx a multiple of
+ * 256, operator MRT-RM with ACP_MRT_RELAXED; otherwise ACP_E_UNSUPPORTED).  It
+ * equals the default form in exact arithmetic, not to the bit (the matrix
+ * product rounds in its own order). */
+enum { ACP_BULK_AUTO = 0, ACP_BULK_WIDE_NT = 1, ACP_BULK_NARROW_NT = 2, ACP_BULK_NARROW_PLAIN = 3, ACP_BULK_MFMA = 4 };
 int acp_case_set_bulk_variant(acp_case* c, int32_t variant);
 
 /* No counterpart in the reference: device bytes a case of `desc` holds, with
@@ -324,6 +330,20 @@ int acp_refined_set_coupling_precision(acp_refined* r, int32_t precision);
  * density and velocity. */
 enum { ACP_F2C_NOFI = 0, ACP_F2C_LAG = 1, ACP_F2C_TOU = 2 };
 int acp_refined_set_filter(acp_refined* r, int32_t filter);
+/* Scaling-Style (BoundaryConditions.txt:89; selected at Lattice.cpp:4634-4704
+ * between interpolTimeScalingCoarseToFine and its _MRT_GSbasis twin,
+ * GridCoupling.h:30-36) of the standard coupling: the non-equilibrium part
+ * handed between the levels is scaled by one factor, a = tau_f / (2 tau_c)
+ * coarse to fine and 1/a fine to coarse (ACP_SCALING_SRT, SRTSTYLE, the
+ * default), or moment by moment in the Gram-Schmidt basis of the MRT-GS
+ * operator, each moment by its own relaxation-time ratio (ACP_SCALING_MRT,
+ * MRTSTYLE): the moments that operator relaxes with omega by a, those it
+ * relaxes at rate 1 on both levels by 1/2 (2 fine to coarse), the conserved
+ * ones by 0.  Applies to the coarse-to-fine rescaling and to every
+ * fine-to-coarse filter; the direct coupling has no moment-space form and
+ * ignores it.  MRTSTYLE needs the coupling arithmetic in float64. */
+enum { ACP_SCALING_SRT = 0, ACP_SCALING_MRT = 1 };
+int acp_refined_set_scaling(acp_refined* r, int32_t style);
 /* Coupling-Approach (BoundaryConditions.txt:52-53, 71-83): the standard
  * coupling (STD, the default; every link replaced: Coarse-to-Fine fAll,
  * Fine-to-Coarse cAll, Hanging-Node-Reconstruction hAll) or the direct

@@ -313,7 +313,7 @@ public:
     // The case file's coupling keys (BoundaryConditions.txt:52-90):
     // Coupling-Approach STD / DC1 / DC2 with Coarse-to-Fine, Fine-to-Coarse
     // and Hanging-Node-Reconstruction, F2C-Filter NOFI / LAG / TOU and Scaling-Style
-    // SRTSTYLE.  A value the engine does not provide ends the program with the
+    // SRTSTYLE / MRTSTYLE.  A value the engine does not provide ends the program with the
     // key and the value, as the reference ends on a malformed case file.
     void apply_coupling_keys() {
         const std::string filt = bc_.getF2Cfilter();
@@ -321,9 +321,11 @@ public:
             detail::die("Lattice::createLattice: F2C-Filter " + filt + " (NOFI, LAG or TOU)\n");
         detail::check(acp_refined_set_filter(r_, filt == "LAG" ? ACP_F2C_LAG : (filt == "TOU" ? ACP_F2C_TOU : ACP_F2C_NOFI)),
                       "Lattice::createLattice");
-        if (bc_.getScalingStyle() != "SRTSTYLE")
-            detail::die("Lattice::createLattice: Scaling-Style " + bc_.getScalingStyle() +
-                        " is not available (SRTSTYLE)\n");
+        const std::string sc = bc_.getScalingStyle();
+        if (sc != "SRTSTYLE" && sc != "MRTSTYLE")
+            detail::die("Lattice::createLattice: Scaling-Style " + sc + " (SRTSTYLE or MRTSTYLE)\n");
+        detail::check(acp_refined_set_scaling(r_, sc == "MRTSTYLE" ? ACP_SCALING_MRT : ACP_SCALING_SRT),
+                      "Lattice::createLattice");
         const std::string cp = bc_.getCouplingProcedure();
         if (cp != "STD" && cp != "DC1" && cp != "DC2")
             detail::die("Lattice::createLattice: Coupling-Approach " + cp + " (STD, DC1 or DC2)\n");

@@ -1218,6 +1218,8 @@ struct acpo_refined {
     int filter; /* 0 NOFI, 1 LAG, 2 TOU */
     int scheme; /* 0 standard coupling (STD), 1 direct coupling (DC) */
     int c2f, f2c, hang; /* DC link sets: 0 missing, 1 missing and common (MfP, McP, hMfP), 2 all (hAll) */
+    int mrt_style;      /* Scaling-Style: 0 SRTSTYLE, 1 MRTSTYLE (see scale_neq) */
+    double sc_c2f[19], sc_f2c[19]; /* MRTSTYLE factors per Gram-Schmidt moment */
     /* HRR: strain rates handed between the levels, per coupled face:
      * s_col at the coarse partners of the fine face nodes, s_face at the fine
      * face nodes, s_gc at the coarse grid-coupling column (each [6][plane]) */
@@ -1226,12 +1228,61 @@ struct acpo_refined {
     double* s_gc[2];
 };
 
-static void rescale19(const acpo_consts* k, int order, double f[19], double a) {
-    double rho, u[3], eq[19];
+/* Scaling-Style (BoundaryConditions.txt:89, selected at Lattice.cpp:4634-4704
+ * and the matching sites of every nestedTimeStepping*): how the
+ * non-equilibrium part n = f - feq(rho, u) of a population set handed between
+ * the levels of the standard coupling is rescaled.
+ *   SRTSTYLE: by one factor, a = tau_f / (2 tau_c) coarse to fine and 1/a fine
+ *     to coarse (the relaxation time scaled with the level's time step).
+ *   MRTSTYLE (interpolTimeScalingCoarseToFineMRT_GSbasis and
+ *     scalingFineToCoarseMRT{,_LAG,_TOU}_GSbasis, GridCoupling.h:30-36, whose
+ *     arithmetic is in the absent GridCoupling.cpp; restated, PARITY
+ *     UNPINNED): moment by moment in the Gram-Schmidt basis of the MRT-GS
+ *     operator (acpo_gs_moments / acpo_gs_inverse), each moment by the ratio
+ *     of its own relaxation times on the two levels, tau_f,r / (2 tau_c,r):
+ *     the moments that operator relaxes with omega (1, 9, 11, 13, 14, 15;
+ *     Node.cpp:2815-2825) by a, as SRTSTYLE scales everything, the moments it
+ *     relaxes at rate 1 on both levels by 1/2, the conserved ones (0, 3, 5, 7),
+ *     in which n vanishes, by 0; fine to coarse the inverse factors (0 for the
+ *     conserved moments).
+ * With one factor for every non-conserved moment MRTSTYLE reduces to SRTSTYLE;
+ * the tests pin that reduction and each moment's factor. */
+static void scale_neq(const acpo_refined* r, const acpo_consts* k, int f2c, double n[19]) {
+    if (!r->mrt_style) {
+        const double a = f2c ? 1.0 / r->a : r->a;
+        for (int q = 0; q < 19; ++q) n[q] = a * n[q];
+        return;
+    }
+    double m[19];
+    acpo_gs_moments(k, n, m);
+    const double* sc = f2c ? r->sc_f2c : r->sc_c2f;
+    for (int q = 0; q < 19; ++q) m[q] = m[q] * sc[q];
+    acpo_gs_inverse(k, m, n);
+}
+
+static void rescale19(const acpo_refined* r, const acpo_consts* k, int order, double f[19], int f2c) {
+    double rho, u[3], eq[19], n[19];
     acpo_moments(k, f, NULL, &rho, u);
     acpo_feq(k, order, rho, u, eq);
-    for (int q = 0; q < 19; ++q) f[q] = eq[q] + a * (f[q] - eq[q]);
+    for (int q = 0; q < 19; ++q) n[q] = f[q] - eq[q];
+    scale_neq(r, k, f2c, n);
+    for (int q = 0; q < 19; ++q) f[q] = eq[q] + n[q];
 }
+
+static int gs_omega_moment(int m) { return m == 1 || m == 9 || m == 11 || m == 13 || m == 14 || m == 15; }
+static int gs_conserved_moment(int m) { return m == 0 || m == 3 || m == 5 || m == 7; }
+
+int acpo_refined_set_scaling(acpo_refined* r, int mrt_style) {
+    if (mrt_style < 0 || mrt_style > 1) return -1;
+    r->mrt_style = mrt_style;
+    for (int m = 0; m < 19; ++m) {
+        r->sc_c2f[m] = gs_conserved_moment(m) ? 0. : (gs_omega_moment(m) ? r->a : 0.5);
+        r->sc_f2c[m] = gs_conserved_moment(m) ? 0. : (gs_omega_moment(m) ? 1.0 / r->a : 2.0);
+    }
+    return 0;
+}
+
+void acpo_refined_scale_neq(const acpo_refined* r, int f2c, double n[19]) { scale_neq(r, &r->c->k, f2c, n); }
 
 acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
     if (coarse->periodic == 1) {
@@ -1262,6 +1313,7 @@ acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
     r->x1 = x1;
     r->c2f = r->f2c = r->hang = 2; /* STD: every link */
     r->a = (1. / r->f->k.omega) / (2. * (1. / r->c->k.omega));
+    acpo_refined_set_scaling(r, 0);
     if (coarse->op == ACPO_HRR) {
         const int nf = fp.periodic == 4 ? 1 : 2;
         const int64_t cplane = (int64_t)coarse->ny * coarse->nz, fplane = (int64_t)fp.ny * fp.nz;
@@ -1315,7 +1367,7 @@ static void refined_c2f(acpo_refined* r, int half) {
                 double v[19];
                 for (int q = 0; q < 19; ++q)
                     v[q] = half ? 0.5 * (c->B[(size_t)q * CN + lc] + c->A[(size_t)q * CN + lc]) : c->A[(size_t)q * CN + lc];
-                rescale19(&c->k, c->p.order, v, r->a);
+                rescale19(r, &c->k, c->p.order, v, 0);
                 for (int q = 0; q < 19; ++q) f->A[(size_t)q * FN + lf] = v[q];
             }
         for (int k = 0; k < fnz; ++k)
@@ -1348,7 +1400,7 @@ static void refined_c2f(acpo_refined* r, int half) {
  * coarse node is the mean over the coincident fine node and its 18 D3Q19
  * neighbours, each taken against its own equilibrium; the equilibrium part is
  * the coincident node's.  Summation: the node itself, then directions 0..17. */
-static void lag19(const acpo_consts* k, int order, const acpo_case* f, int fx, int fy, int fz, double ainv,
+static void lag19(const acpo_refined* r, const acpo_consts* k, int order, const acpo_case* f, int fx, int fy, int fz,
                   double v[19]) {
     const int fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
     double rho, u[3], eq[19], acc[19], g[19], e[19];
@@ -1365,7 +1417,9 @@ static void lag19(const acpo_consts* k, int order, const acpo_case* f, int fx, i
         acpo_feq(k, order, rho, u, e);
         for (int q = 0; q < 19; ++q) acc[q] += g[q] - e[q];
     }
-    for (int q = 0; q < 19; ++q) v[q] = eq[q] + ainv * (acc[q] / 19.);
+    for (int q = 0; q < 19; ++q) acc[q] = acc[q] / 19.;
+    scale_neq(r, k, 1, acc);
+    for (int q = 0; q < 19; ++q) v[q] = eq[q] + acc[q];
 }
 
 /* the direction opposite q (q < 18) */
@@ -1388,7 +1442,7 @@ static int opposite(int q) {
  * J = sum_q xi_q nbar_q, so that the coarse node keeps the density and
  * velocity of its coincident fine node; the coarse node receives
  * feq(x) + (1/a) nbar. */
-static void tou19(const acpo_consts* k, int order, const acpo_case* f, int fx, int fy, int fz, double ainv,
+static void tou19(const acpo_refined* r, const acpo_consts* k, int order, const acpo_case* f, int fx, int fy, int fz,
                   double v[19]) {
     const int fnx = f->p.nx, fny = f->p.ny, fnz = f->p.nz;
     double rho, u[3], eq[19], n0[19], up[19], dn[19], g[19], e[19], nb[19];
@@ -1418,12 +1472,15 @@ static void tou19(const acpo_consts* k, int order, const acpo_case* f, int fx, i
         for (int a = 0; a < 3; ++a)
             if (E[q][a]) J[a] += k->xi[q][a] * nb[q];
     }
+    double n[19];
     for (int q = 0; q < 19; ++q) {
         double s = M;
         for (int a = 0; a < 3; ++a)
             if (E[q][a]) s += k->rcs2 * (k->xi[q][a] * J[a]);
-        v[q] = eq[q] + ainv * (nb[q] - k->w[q] * s);
+        n[q] = nb[q] - k->w[q] * s;
     }
+    scale_neq(r, k, 1, n);
+    for (int q = 0; q < 19; ++q) v[q] = eq[q] + n[q];
 }
 
 static void refined_f2c(acpo_refined* r) {
@@ -1431,7 +1488,6 @@ static void refined_f2c(acpo_refined* r) {
     acpo_case* f = r->f;
     const int cnx = c->p.nx, cny = c->p.ny, cnz = c->p.nz;
     const int fnx = f->p.nx, fny = f->p.ny;
-    const double ainv = 1.0 / r->a;
 #pragma omp parallel for collapse(2) schedule(static) num_threads(c->p.nthreads > 0 ? c->p.nthreads : 1)
     for (int I = r->x0 + 1; I <= r->x1 - 1; ++I)
         for (int K = 0; K < cnz; ++K)
@@ -1440,12 +1496,12 @@ static void refined_f2c(acpo_refined* r) {
                 const int64_t lf = 2 * (I - r->x0) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
                 double v[19];
                 if (r->filter == 1) {
-                    lag19(&c->k, c->p.order, f, 2 * (I - r->x0), 2 * J, 2 * K, ainv, v);
+                    lag19(r, &c->k, c->p.order, f, 2 * (I - r->x0), 2 * J, 2 * K, v);
                 } else if (r->filter == 2) {
-                    tou19(&c->k, c->p.order, f, 2 * (I - r->x0), 2 * J, 2 * K, ainv, v);
+                    tou19(r, &c->k, c->p.order, f, 2 * (I - r->x0), 2 * J, 2 * K, v);
                 } else {
                     for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + lf];
-                    rescale19(&c->k, c->p.order, v, ainv);
+                    rescale19(r, &c->k, c->p.order, v, 1);
                 }
                 for (int q = 0; q < 19; ++q) c->A[(size_t)q * c->N + lc] = v[q];
             }

@@ -141,6 +141,12 @@ void acpo_refined_step(acpo_refined* r, int n);
 void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out);
 /* fine-to-coarse filter: 0 NOFI (default), 1 LAG, 2 TOU; -1 for any other value */
 int acpo_refined_set_filter(acpo_refined* r, int filter);
+/* Scaling-Style of the standard coupling: 0 SRTSTYLE, 1 MRTSTYLE (the direct
+ * coupling has no moment-space variant and ignores it) */
+int acpo_refined_set_scaling(acpo_refined* r, int mrt_style);
+/* the rescaling of a non-equilibrium part n under the current Scaling-Style,
+ * coarse to fine (f2c 0) or fine to coarse (f2c 1), in place */
+void acpo_refined_scale_neq(const acpo_refined* r, int f2c, double n[19]);
 /* coupling scheme: 0 standard (STD, default: every link, c2f = f2c = hang =
  * 2), 1 direct coupling (DC); link sets 0 missing (Mf, Mc, hMf), 1 missing and
  * common (MfP, McP, hMfP), 2 all (hAll, hanging nodes only); -1 for an

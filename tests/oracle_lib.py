@@ -101,6 +101,8 @@ def lib():
         L.acpo_refined_set_filter.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.acpo_refined_restrict.argtypes = [ctypes.c_void_p]
         L.acpo_refined_set_coupling.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
+        L.acpo_refined_set_scaling.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.acpo_refined_scale_neq.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
         _lib = L
     return _lib
 
@@ -250,6 +252,17 @@ class OracleRefined:
     def restrict(self):
         """The fine-to-coarse restriction alone, on the levels' current states."""
         lib().acpo_refined_restrict(self.h)
+
+    def set_scaling(self, style):
+        """Scaling-Style of the standard coupling: "SRTSTYLE" or "MRTSTYLE"."""
+        if lib().acpo_refined_set_scaling(self.h, {"SRTSTYLE": 0, "MRTSTYLE": 1}[style]) != 0:
+            raise ValueError(style)
+
+    def scale_neq(self, n, f2c=False):
+        """A non-equilibrium part rescaled coarse to fine (or fine to coarse)."""
+        out = np.ascontiguousarray(n, dtype=np.float64).copy()
+        lib().acpo_refined_scale_neq(self.h, int(bool(f2c)), dptr(out))
+        return out
 
 
 def feq(p, order, rho, u):

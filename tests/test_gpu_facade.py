@@ -110,19 +110,21 @@ def test_stepping_member_of_another_operator_exits(gpu, tmp_path):
 def _refined_text(nx, ny, nz, op, region, steps, coupling=None):
     txt = bc_text(nx, ny, nz, op=op, tau=0.56, steps=steps).replace("MaxLevel 0", "MaxLevel 1")
     if coupling:
-        scheme, c2f, f2c, hang, filt = coupling
+        scheme, c2f, f2c, hang, filt = coupling[:5]
+        scaling = coupling[5] if len(coupling) > 5 else "SRTSTYLE"
         txt = (txt.replace("Coupling-Approach STD", f"Coupling-Approach {scheme}")
                .replace("Coarse-to-Fine fAll", f"Coarse-to-Fine {c2f}").replace("Fine-to-Coarse cAll", f"Fine-to-Coarse {f2c}")
                .replace("Hanging-Node-Reconstruction hAll", f"Hanging-Node-Reconstruction {hang}")
-               .replace("F2C-Filter NOFI", f"F2C-Filter {filt}"))
+               .replace("F2C-Filter NOFI", f"F2C-Filter {filt}")
+               .replace("Scaling-Style SRTSTYLE", f"Scaling-Style {scaling}"))
     return txt + f"Boundary LODI-X\nRefined-Region {region[0]} {region[1]}\n"
 
 
 COUPLINGS = [None, ("DC1", "Mf", "Mc", "hMf", "LAG"), ("DC2", "MfP", "Mc", "hMfP", "NOFI"),
-             ("STD", "fAll", "cAll", "hAll", "TOU")]
+             ("STD", "fAll", "cAll", "hAll", "TOU"), ("STD", "fAll", "cAll", "hAll", "LAG", "MRTSTYLE")]
 
 
-@pytest.mark.parametrize("coupling", COUPLINGS, ids=["STD-NOFI", "DC1-LAG", "DC2-MfP", "STD-TOU"])
+@pytest.mark.parametrize("coupling", COUPLINGS, ids=["STD-NOFI", "DC1-LAG", "DC2-MfP", "STD-TOU", "STD-LAG-MRTSTYLE"])
 @pytest.mark.parametrize("op", ["RR", "HRR"])
 def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, coupling):
     """MaxLevel 1 with this engine's refined duct geometry: the reference's
@@ -140,6 +142,8 @@ def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, couplin
         if coupling:
             rc.set_coupling(*coupling[:4])
             rc.set_filter(coupling[4])
+            if len(coupling) > 5:
+                rc.set_scaling(coupling[5])
         rc.init_reference()
         p0 = ol.make_params(nx, ny, nz, op=op, tau=0.56, cs=CS, spacing=DX, rho0=RHO0, periodic=3)
         p1 = ol.make_params(2 * (x1 - x0) + 1, 2 * ny, 2 * nz, op=op, tau=0.56, cs=CS, spacing=DX / 2, rho0=RHO0,
@@ -165,13 +169,14 @@ def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, couplin
     assert rec["node"][(1, fnx - 1, 3, 1)][1] == "boundary"
 
 
-@pytest.mark.parametrize("key,value", [("F2C-Filter NOFI", "F2C-Filter XYZ"), ("Scaling-Style SRTSTYLE", "Scaling-Style MRTSTYLE"),
+@pytest.mark.parametrize("key,value", [("F2C-Filter NOFI", "F2C-Filter XYZ"), ("Scaling-Style SRTSTYLE", "Scaling-Style XYZ"),
                                        ("Coupling-Approach STD", "Coupling-Approach XYZ"),
                                        ("Coarse-to-Fine fAll", "Coarse-to-Fine Mf")])
 def test_two_level_coupling_keys_the_engine_lacks_end_the_program(gpu, tmp_path, key, value):
     """A two-level case file asking for what the engine does not provide (an
-    unknown filter, moment-based scaling, an unknown scheme, or the standard
-    coupling with only the missing links) ends the program naming the key."""
+    unknown filter, an unknown scaling style, an unknown scheme, or the
+    standard coupling with only the missing links) ends the program naming the
+    key."""
     f = tmp_path / "bc.txt"
     f.write_text(_refined_text(24, 8, 3, "RR", (12, 23), 2).replace(key, value))
     r = _run(f, "nested")

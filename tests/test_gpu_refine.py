@@ -228,3 +228,64 @@ def test_shifted_tier_refinement_follows_float64(gpu):
     assert a["F32-SHIFTED coupling F64"]["probe_l2"] < 1e-4
     assert a["F32-SHIFTED coupling F32-SHIFTED"]["probe_l2"] < 1e-3
     assert a["F16-SHIFTED coupling F64"]["probe_l2"] < 1e-2
+
+
+@pytest.mark.parametrize("filt", ["NOFI", "LAG", "TOU"])
+@pytest.mark.parametrize("op,order,mrt,bnd,x1", [("SRT", 3, 0, "PERIODIC", 16), ("MRT-GS", 3, 1, "PERIODIC", 16),
+                                                 ("HRR", 3, 0, "LODI-X", 23), ("RR", 4, 0, "LODI-X", 17)])
+def test_refined_mrtstyle_matches_oracle_bitexact(gpu, filt, op, order, mrt, bnd, x1):
+    """Scaling-Style MRTSTYLE: the non-equilibrium part handed between the
+    levels is rescaled moment by moment in the Gram-Schmidt basis; the GPU's
+    refined run is the oracle's to the bit, for every fine-to-coarse filter,
+    and differs from the SRTSTYLE run."""
+    nx, ny, nz, x0 = 24, 10, 3, 8
+    periodic = 1 if bnd == "PERIODIC" else 3
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=periodic, tau=0.56, mrt_mode=mrt, cs=CS,
+                       spacing=DX, rho0=RHO0, transverse=bnd != "PERIODIC")
+    o = ol.OracleRefined(p, x0, x1)
+    o.set_filter(filt)
+    o.set_scaling("MRTSTYLE")
+    amp, cx, cy, b = 1e-3, 10.0, 5.0, 2.5
+    o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, tau=0.56, boundary=bnd, transverse=bnd != "PERIODIC",
+                      mrt_mode="RELAXED" if mrt else "REFERENCE")
+    print("case", op, order, filt, bnd, flush=True)
+    with acp.Refined(d, x0, x1) as r, acp.Refined(d, x0, x1) as s:
+        for x in (r, s):
+            x.set_filter(filt)
+            x.init_pulse(amp, cx, cy, b)
+        r.set_scaling("MRTSTYLE")
+        for n in (1, 4, 9):
+            r.step(n)
+            s.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+        assert not np.array_equal(r.level(0).populations(), s.level(0).populations())
+
+
+def test_refined_mrtstyle_shifted_tiers_follow_float64(gpu):
+    """MRTSTYLE in the float32-shifted tier (float64 coupling) follows the
+    float64 oracle within 1e-4 of the largest density deviation; the float32
+    coupling arithmetic refuses it."""
+    nx, ny, nz, x0, x1 = 24, 10, 3, 8, 23
+    p = ol.make_params(nx, ny, nz, op="RR", order=3, periodic=3, tau=0.56, cs=CS, spacing=DX, rho0=RHO0)
+    o = ol.OracleRefined(p, x0, x1)
+    o.set_scaling("MRTSTYLE")
+    o.set_filter("TOU")
+    amp, cx, cy, b = 1e-3, 14.0, 5.0, 2.5
+    o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    o.step(30)
+    r64 = o.levels[0].state().sum(axis=0)
+    d = acp.make_desc(nx, ny, nz, op="RR", order=3, tau=0.56, boundary="LODI-X", precision="F32-SHIFTED")
+    with acp.Refined(d, x0, x1) as r:
+        r.set_filter("TOU")
+        r.set_scaling("MRTSTYLE")
+        with pytest.raises(acp.AcpError):
+            r.set_coupling_precision("F32-SHIFTED")
+        r.init_pulse(amp, cx, cy, b)
+        r.step(30)
+        rho, _ = r.level(0).macros()
+    assert np.max(np.abs(rho - r64)) <= 1e-4 * np.max(np.abs(r64 - RHO0))
