@@ -155,7 +155,7 @@ class Case:
         """Run every full chunk of `chunk` (even) steps as one captured-graph replay; 0 switches off."""
         check(lib().acp_case_enable_graphs(self.h, int(chunk)))
 
-    BULK_VARIANTS = {"auto": 0, "wide_nt": 1, "narrow_nt": 2, "narrow_plain": 3, "mfma": 4}
+    BULK_VARIANTS = {"auto": 0, "wide_nt": 1, "narrow_nt": 2, "narrow_plain": 3, "mfma": 4, "relative_nt": 5}
 
     def set_bulk_variant(self, name):
         """Force the bulk kernels' addressing and streaming form ("auto",

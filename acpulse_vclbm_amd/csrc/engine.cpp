@@ -906,7 +906,8 @@ extern "C" int acp_case_enable_graphs(acp_case* c, int32_t chunk) {
 
 extern "C" int acp_case_set_bulk_variant(acp_case* c, int32_t variant) {
     if (int rc = check_case(c)) return rc;
-    if (variant < acp::BULK_AUTO || variant > acp::BULK_MFMA) return fail(ACP_E_ARG, "unknown bulk-kernel variant");
+    if (variant < acp::BULK_AUTO || variant > acp::BULK_RELATIVE_NT)
+        return fail(ACP_E_ARG, "unknown bulk-kernel variant");
     if (variant == acp::BULK_MFMA) {
         if (shifted(c) || c->bulk_op != acp::OP_MRT_RM || !c->periodic || c->g.xopen || c->g.xlodi ||
             c->g.nx % 256 != 0)

@@ -115,11 +115,14 @@ __device__ __forceinline__ void collide_mrt_rm_mfma(const K64& k, double f[19], 
 template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT>
 __device__ __forceinline__ void bulk64_node(const K64* kp, const double* A, double* B, double* psi, const double* M,
                                             int nx, int ny, int nz, const PopLayout& L,
-                                            int xopen, int row_lds, int xlodi, int i, int j, int kz, bool active) {
+                                            int xopen, int row_lds, int xlodi, int i, int j, int kz, bool active,
+                                            typename Plane<NARROW>::idx l0 = 0) {
     using P = Plane<NARROW>;
     using I = typename P::idx;
     const I sxy = (I)nx * (I)ny;
-    const I l = (I)i + (I)nx * (I)j + sxy * (I)kz;
+    // l0: the first node of the block the arrays are addressed from (0, or
+    // the workgroup's block-relative base; see bulk64)
+    const I l = (I)i + (I)nx * (I)j + sxy * (I)kz - l0;
     const K64& k = *kp;
 
     int bi = 0, bj = 0, bk = 0;
@@ -255,18 +258,41 @@ __device__ __forceinline__ void bulk64_node(const K64* kp, const double* A, doub
     st_pop<NT>(&P::at(B, 18, l, L), f[18]);
 }
 
-template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT>
+// Block-relative addressing (REL, with the narrow form): on arrays too large
+// for 32-bit offsets from the array start, every access of a workgroup still
+// lies within `rel_span` blocks of the workgroup's own block (its row's
+// neighbours are at most one plane and a row away, or two planes on a grid
+// wrapped in y, whose last row pushes to the first row of the next and
+// previous planes), so the arrays are addressed from a
+// uniform base `rel_span` blocks before that block, with 32-bit offsets from
+// it: half the address registers of the 64-bit form.  Rows whose z
+// neighbours wrap to the far end of a periodic grid run in a separate launch
+// of the 64-bit form.  Planes kz = kz0 + kzstep * blockIdx.z.
+template <int OP, int ORDER, bool CUBIC, bool PERIODIC, bool NARROW, bool NT, bool REL = false>
 __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(OP == OP_HRR && !CUBIC && ORDER < 4 ? ACP_HRR_WAVES : ACP_BULK_WAVES))) void bulk64(const K64* __restrict__ kp, const double* __restrict__ A,
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
-                                                     const PopLayout L, int xopen, int row_lds, int xlodi) {
+                                                     const PopLayout L, int xopen, int row_lds, int xlodi,
+                                                     int kz0, int kzstep, int rel_span) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     // a row held by one workgroup (row_lds) keeps the threads beyond it for the
     // LDS exchange of its stores; they load, compute and store nothing
     const bool active = i < nx;
     if (!active && !row_lds) return;
-    bulk64_node<OP, ORDER, CUBIC, PERIODIC, NARROW, NT>(kp, A, B, psi, M, nx, ny, nz, L, xopen, row_lds, xlodi, i,
-                                                        (int)blockIdx.y, (int)blockIdx.z, active);
+    const int j = blockIdx.y, kz = kz0 + kzstep * (int)blockIdx.z;
+    if constexpr (REL) {
+        static_assert(NARROW, "block-relative addressing takes 32-bit offsets");
+        const int64_t first = (int64_t)nx * ((int64_t)j + (int64_t)ny * kz);
+        const int64_t b = first >> L.shift;
+        const int64_t b0 = b > rel_span ? b - rel_span : 0;
+        const int64_t l0 = b0 << L.shift;
+        bulk64_node<OP, ORDER, CUBIC, PERIODIC, true, NT>(kp, A + b0 * L.block, B + b0 * L.block,
+                                                          psi ? psi + l0 : nullptr, M ? M + l0 : nullptr, nx, ny, nz,
+                                                          L, xopen, row_lds, xlodi, i, j, kz, active, (uint32_t)l0);
+    } else {
+        bulk64_node<OP, ORDER, CUBIC, PERIODIC, NARROW, NT>(kp, A, B, psi, M, nx, ny, nz, L, xopen, row_lds, xlodi, i,
+                                                            j, kz, active);
+    }
 }
 
 // Packed form (pack.hpp): one persistent grid over the 256-node chunks of
@@ -305,13 +331,33 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
     // duct 17.3 against 20.7 us per step, a 256^3 LODI cube 1.18 against 1.32 ms).
     const bool fully_periodic = PERIODIC && !g.xopen && !g.xlodi;
     const int row_lds = g.nx == bx && (fully_periodic || (int64_t)g.ny * g.nz >= 4096);
-#define B64(NAR, NT)                                                                                          \
-    hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, NAR, NT>), grid, block, 0, s, k, A, B, psi, M, g.nx, \
-                       g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi)
-    const BulkForm form = bulk_form(g.L, sizeof(double), sizeof(double), g.variant);
-    if (!form.narrow) B64(false, true);
-    else if (form.nt) B64(true, true);
-    else B64(true, false);
+#define B64(NAR, NT, REL, GZ, KZ0, KZS)                                                                          \
+    hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, NAR, NT, REL>), dim3(grid.x, grid.y, GZ), block, 0, s, k, A, \
+                       B, psi, M, g.nx, g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi, KZ0, KZS, span)
+    // a row's accesses reach one plane and a row (z and y neighbours), and on
+    // grids wrapped in y (the periodic form) the wrapped row of the next or
+    // previous plane: nearly two planes
+    const int64_t plane = (int64_t)g.nx * g.ny;
+    const int64_t reach = (PERIODIC ? 2 * plane : plane) + 2 * (int64_t)g.nx;
+    const BulkForm form = bulk_form(g.L, sizeof(double), sizeof(double), g.variant, reach);
+    const int span = (int)rel_span_blocks(reach, g.L);
+    if (form.rel) {
+        // planes whose z neighbours wrap (periodic grids) in the 64-bit form
+        if (PERIODIC && g.nz > 2) {
+            B64(true, true, true, g.nz - 2, 1, 1);
+            B64(false, true, false, 2, 0, g.nz - 1);
+        } else if (!PERIODIC) {
+            B64(true, true, true, g.nz, 0, 1);
+        } else {
+            B64(false, true, false, g.nz, 0, 1);
+        }
+    } else if (!form.narrow) {
+        B64(false, true, false, g.nz, 0, 1);
+    } else if (form.nt) {
+        B64(true, true, false, g.nz, 0, 1);
+    } else {
+        B64(true, false, false, g.nz, 0, 1);
+    }
 #undef B64
     return hipGetLastError();
 }

@@ -152,17 +152,45 @@ inline bool narrow_planes(const PopLayout& L, int pop_bytes, int plane_bytes) {
 // last-level cache.  Tests force the forms that only large grids take onto
 // small grids, so that every instantiation the large cases run is compared
 // with the oracle; a narrow form is never forced onto arrays it cannot address.
-enum BulkVariant : int { BULK_AUTO = 0, BULK_WIDE_NT = 1, BULK_NARROW_NT = 2, BULK_NARROW_PLAIN = 3, BULK_MFMA = 4 };
-struct BulkForm {
-    bool narrow, nt;
+enum BulkVariant : int {
+    BULK_AUTO = 0,
+    BULK_WIDE_NT = 1,
+    BULK_NARROW_NT = 2,
+    BULK_NARROW_PLAIN = 3,
+    BULK_MFMA = 4,
+    BULK_RELATIVE_NT = 5
 };
-inline BulkForm bulk_form(const PopLayout& L, int pop_bytes, int plane_bytes, int variant) {
+// rel: block-relative 32-bit offsets (see bulk64): for arrays the narrow form
+// cannot address whose workgroups each stay within a window that 32-bit
+// offsets reach; the float64 bulk kernel takes it in place of the 64-bit form
+struct BulkForm {
+    bool narrow, nt, rel;
+};
+// Blocks a workgroup's accesses reach beyond its row's first block: `reach`
+// nodes (the farthest neighbour index a row's pushes take, both ways: a plane
+// and two rows, two planes when the y wrap of a periodic grid's last row lands
+// a plane back), rounded up to blocks, plus one.
+inline int64_t rel_span_blocks(int64_t reach, const PopLayout& L) {
+    return L.block ? ((reach + L.mask) >> L.shift) + 1 : 0;
+}
+// whether block-relative offsets reach every access of a workgroup: a window
+// of 2 span + 2 blocks of pop_bytes populations, node indices below 2^32 and
+// macro planes of plane_bytes addressable with 32-bit offsets
+inline bool relative_fits(const PopLayout& L, int64_t reach, int pop_bytes, int plane_bytes) {
+    if (!L.block) return false;
+    const int64_t window = (2 * rel_span_blocks(reach, L) + 2) * L.block * (int64_t)pop_bytes;
+    return window <= ((int64_t)1 << 32) && L.N < ((int64_t)1 << 32) &&
+           L.N * (int64_t)plane_bytes <= ((int64_t)1 << 32);
+}
+inline BulkForm bulk_form(const PopLayout& L, int pop_bytes, int plane_bytes, int variant, int64_t reach = 0) {
     const bool narrow = narrow_planes(L, pop_bytes, plane_bytes);
+    const bool rel_ok = reach > 0 && relative_fits(L, reach, pop_bytes, plane_bytes);
     switch (variant) {
-        case BULK_WIDE_NT: return {false, true};
-        case BULK_NARROW_NT: return {narrow, true};
-        case BULK_NARROW_PLAIN: return {narrow, !narrow};
-        default: return {narrow, !narrow || stream_populations(L, pop_bytes)};
+        case BULK_WIDE_NT: return {false, true, false};
+        case BULK_NARROW_NT: return {narrow, true, false};
+        case BULK_NARROW_PLAIN: return {narrow, !narrow, false};
+        case BULK_RELATIVE_NT: return {false, true, rel_ok};
+        default: return {narrow, !narrow || stream_populations(L, pop_bytes), !narrow && rel_ok};
     }
 }
 

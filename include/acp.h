@@ -174,19 +174,33 @@ int acp_case_enable_graphs(acp_case* c, int32_t chunk);
 /* No counterpart in the reference: the addressing and streaming form of the
  * bulk (collide + push) kernels.  ACP_BULK_AUTO (the default) follows the
  * arrays' size: 32-bit offsets from a uniform base while the per-node part of
- * every offset stays below 4 GiB, 64-bit offsets beyond; non-temporal loads
- * and stores once the two population arrays exceed the 256 MB last-level
- * cache.  The other values force one form on any grid, so that tests can
- * compare the forms only large grids take (the 512^3 float64 grid runs
+ * every offset stays below 4 GiB, block-relative (float64 bulk kernels) or
+ * 64-bit offsets beyond; non-temporal loads and stores once the two population
+ * arrays exceed the 256 MB last-level cache.  The other values force one form
+ * on any grid, so that tests can compare the forms only large grids take (the
+ * 512^3 float64 grid runs ACP_BULK_RELATIVE_NT, its wrapped planes
  * ACP_BULK_WIDE_NT) with the oracle on small grids; results are identical in
  * every form.  A 32-bit form on arrays that need 64-bit offsets is ACP_E_ARG.
+ * ACP_BULK_RELATIVE_NT forces the float64 kernels' block-relative form: 32-bit
+ * offsets from a base a few blocks before each workgroup's own block, which
+ * ACP_BULK_AUTO takes on arrays too large for offsets from the array start
+ * (the 512^3 float64 grid); rows whose z neighbours wrap to the far end of a
+ * periodic grid keep 64-bit offsets.  Grids of a single block, and the
+ * shifted tiers, run the 64-bit form under it.
  * ACP_BULK_MFMA is the matrix-core trial of the relaxed MRT-RM collision:
  * f* = f + C (feq - f) with the case's 19 x 19 collision matrix C = M^-1 S M
  * on v_mfma_f64_16x16x4f64 (float64, periodic grids with nx a multiple of
  * 256, operator MRT-RM with ACP_MRT_RELAXED; otherwise ACP_E_UNSUPPORTED).  It
  * equals the default form in exact arithmetic, not to the bit (the matrix
  * product rounds in its own order). */
-enum { ACP_BULK_AUTO = 0, ACP_BULK_WIDE_NT = 1, ACP_BULK_NARROW_NT = 2, ACP_BULK_NARROW_PLAIN = 3, ACP_BULK_MFMA = 4 };
+enum {
+    ACP_BULK_AUTO = 0,
+    ACP_BULK_WIDE_NT = 1,
+    ACP_BULK_NARROW_NT = 2,
+    ACP_BULK_NARROW_PLAIN = 3,
+    ACP_BULK_MFMA = 4,
+    ACP_BULK_RELATIVE_NT = 5
+};
 int acp_case_set_bulk_variant(acp_case* c, int32_t variant);
 
 /* No counterpart in the reference: device bytes a case of `desc` holds, with

@@ -45,10 +45,14 @@ VARIANT_CASES = [
     ("HRR", 4, True, True, "LODI-X", (41, 29, 17)),
     ("RR", 3, False, False, "OPEN-LODI-X", (41, 29, 17)),
     ("SRT", 2, False, False, "LODI", (41, 29, 17)),
+    # periodic planes of eight blocks: the last row's wrapped pushes reach
+    # nearly two planes away, which the block-relative form's window must hold
+    ("MRT-RM", 3, False, False, "PERIODIC", (256, 256, 6)),
+    ("HRR", 3, False, False, "PERIODIC", (256, 256, 6)),
 ]
 
 
-@pytest.mark.parametrize("variant", ["wide_nt", "narrow_nt", "narrow_plain"])
+@pytest.mark.parametrize("variant", ["wide_nt", "narrow_nt", "narrow_plain", "relative_nt"])
 @pytest.mark.parametrize("op,order,cubic,tr,bnd,n", VARIANT_CASES,
                          ids=[f"{a}-o{b}-{e}-{f[0]}" + ("-cubic" if c else "") for a, b, c, d, e, f in VARIANT_CASES])
 def test_float64_bulk_forms_match_oracle_bitexact(gpu, variant, op, order, cubic, tr, bnd, n):
