@@ -10,8 +10,11 @@ value) and the float32-shifted tier (reported beside it).
 
 GLUPS counts every node of the grid per step.  The roofline figure uses the
 algorithmic bytes of the bulk kernel, 2*19*8 = 304 B (float64) and 2*19*4 =
-152 B (float32) per node and step, over the measured average step time; with
-no boundary pass on a periodic grid the step is one kernel launch.
+152 B (float32) per node and step, over the measured average step time.  With
+no boundary pass on a periodic grid the float32 step is one kernel launch and
+the float64 step two launches of the bulk kernel over disjoint z-planes: the
+510 inner planes with block-relative 32-bit offsets, the two planes whose z
+neighbours wrap to the far end of the arrays with 64-bit offsets.
 
 Serving (reported beside, in "serving"): the acoustic-pulse request mix of
 acpulse_vclbm_amd.serve (nx, ny in [128, 1024], nz = 3, five collision
@@ -236,12 +239,13 @@ def serving(acp, n_req, step_scale, max_active, graph_chunk, pack_max_nodes=0):
 
 
 def traffic_from_profiles(name):
-    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE,
-    read from the committed summary (profiles/<name>), or None."""
+    """HBM bytes per step (the launches of one pass over the grid) measured with
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, read from the committed summary
+    (profiles/<name>), or None."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
-    m = re.search(r"hbm_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", open(path).read())
+    m = re.search(r"hbm_bytes_per_(?:launch|step)\s*[:=]\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
 
 
