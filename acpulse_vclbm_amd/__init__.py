@@ -121,6 +121,12 @@ class Case:
         check(lib().acp_case_get_populations(self.h, dptr(f)))
         return f
 
+    def psi(self):
+        """The cubic-Mach state psi [19][N] (zero without the correction)."""
+        f = np.empty((19, self.N))
+        check(lib().acp_case_get_psi(self.h, dptr(f)))
+        return f
+
     def previous_populations(self):
         """The other array of the ping-pong pair (the one the last step read)."""
         f = np.empty((19, self.N))

@@ -518,8 +518,6 @@ int validate_desc(const acp_case_desc& d) {
     if (d.op < ACP_OP_SRT || d.op > ACP_OP_HRR) return fail(ACP_E_ARG, "unknown collision operator");
     if (d.precision < ACP_PREC_F64 || d.precision > ACP_PREC_F16_SHIFTED) return fail(ACP_E_ARG, "unknown precision");
     if (d.boundary < ACP_BC_LODI || d.boundary > ACP_BC_OPEN_LODI_X) return fail(ACP_E_ARG, "unknown boundary");
-    if ((d.boundary == ACP_BC_OPEN_X || d.boundary == ACP_BC_OPEN_LODI_X) && d.cubic)
-        return fail(ACP_E_UNSUPPORTED, "open x faces: the cubic-Mach terms are not carried across them");
     if ((d.boundary == ACP_BC_LODI_X || d.boundary == ACP_BC_OPEN_LODI_X) && d.nx < 4)
         return fail(ACP_E_ARG, "LODI x faces need at least 4 nodes in x");
     if (d.precision != ACP_PREC_F64 && d.cubic)
@@ -801,7 +799,8 @@ int step_eager(acp_case* c, int32_t n) {
                 e = acp::launch_bulk64(c->bulk_op, c->d.order, c->d.cubic != 0, c->periodic, c->dk, c->A, c->B,
                                        c->psi, c->M, c->g, c->stream);
             for (int x = 0; x < c->nfix && e == hipSuccess; ++x)
-                e = acp::launch_fix_hrr64(c->d.order, c->dk, c->A, c->B, c->g, c->fix_col[x], c->fix_S[x], c->stream);
+                e = acp::launch_fix64(c->bulk_op, c->d.order, c->d.cubic != 0, c->dk, c->A, c->B, c->psi, c->g,
+                                      c->fix_col[x], c->fix_S[x], c->stream);
             for (int p = 0; p < nph && e == hipSuccess; ++p)
                 e = acp::launch_lodi64(c->d.order, c->d.cubic != 0, c->d.transverse != 0, c->dk, c->A, c->B, c->psi,
                                        c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], c->g.L,

@@ -47,11 +47,13 @@ LodiSchedule build_lodi_schedule_x(int nx, int ny, int nz, int faces);
 int init_reference_level(acp_case* c, int level, int i0, const acp_case_desc& d0);
 
 // The nodes of x column `col` collide with a strain rate handed over by the
-// other level of a two-level step (S: float64 planes [6][ny nz], SI units)
-// and push as the bulk kernel does; launched right after the bulk kernel
-// (refine.hip, kshift.hip).
-hipError_t launch_fix_hrr64(int order, const K64* k, const double* A, double* B, const Grid& g, int col,
-                            const double* S, hipStream_t s);
+// other level of a two-level step (S: float64 planes [6][ny nz], SI units;
+// with the cubic-Mach terms followed by the handed psi [19][ny nz], which
+// replaces the nodes' psi state) and push as the bulk kernel does; launched
+// right after the bulk kernel (refine.hip, kshift.hip).  op: OP_HRR, or OP_RR
+// with the cubic-Mach terms.
+hipError_t launch_fix64(int op, int order, bool cubic, const K64* k, const double* A, double* B, double* psi,
+                        const Grid& g, int col, const double* S, hipStream_t s);
 hipError_t launch_fix_hrr_shift(int prec, int order, const KShift* k, const void* A, void* B, const Grid& g, int col,
                                 const double* S, double dt, hipStream_t s);
 
@@ -123,7 +125,8 @@ struct acp_case {
     // stale once it differs from the value they recorded
     uint64_t graph_epoch = 0;
     // x columns whose nodes collide with a strain rate handed over by the other
-    // level of a two-level HRR case (fix_S: float64 planes [6][ny nz], SI)
+    // level of a two-level HRR case (fix_S: float64 planes [6][ny nz], SI; with
+    // the cubic-Mach terms, RR or HRR, followed by psi [19][ny nz])
     int32_t nfix = 0;
     int32_t fix_col[2] = {0, 0};
     const double* fix_S[2] = {nullptr, nullptr};

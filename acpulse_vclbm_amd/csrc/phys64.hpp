@@ -391,6 +391,20 @@ __host__ __device__ __forceinline__ void collide_hrr64_strain(const K64& k, doub
     rr64_relax<ORDER, true, false>(k, f, nullptr, S, eq, ux, uy, uz);
 }
 
+// RR or HRR with the cubic-Mach terms at an interface node of a two-level
+// step: its partner hands it psi (Node.cpp:2069) and, for HRR, the strain
+// rate S before it collides, and a grid-coupling node's own strain-rate
+// function returns at once (Node.cpp:1790-1793), so the handed psi enters its
+// density and velocity (Node.cpp:11-30, 85-118), Pi1 and the result.
+template <int ORDER, bool HYBRID>
+__host__ __device__ __forceinline__ void collide_rr64_handed(const K64& k, double f[19], const double psi[19],
+                                                              const double S[6]) {
+    double rho, ux, uy, uz, eq[19];
+    moments64_psi(k, f, psi, rho, ux, uy, uz);
+    feq64<ORDER>(k, rho, ux, uy, uz, eq);
+    rr64_relax<ORDER, HYBRID, true>(k, f, psi, S, eq, ux, uy, uz);
+}
+
 // raw moments (Node.cpp:2693-2762)
 __host__ __device__ __forceinline__ void rm_moments64(const K64& k, const double f[19], double m[19]) {
     // coefficients of M_abc per link: row r, link q -> -1, 0, 1

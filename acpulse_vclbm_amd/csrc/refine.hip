@@ -738,6 +738,24 @@ __device__ __forceinline__ void macros_si(const K64& k, const KShift* ks, const 
     moments64(k, f, m4[0], m4[1], m4[2], m4[3]);
 }
 
+// The strain rate of neighbour macros nb into entry r of the planes o (nv of
+// `per` entries): S, and with the cubic-Mach terms (nv = 25) psi from the same
+// stencil, as the partner functions evaluate both (Node.cpp:1892-2490).
+__device__ __forceinline__ void store_handed(const K64& k, const double nb[6][4], int nv, double* __restrict__ o,
+                                             int per, int r) {
+    double S[6];
+    if (nv > 6) {
+        double ps[19];
+        strain_psi64<true>(k, nb, S, ps);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) o[(6 + q) * per + r] = ps[q];
+    } else {
+        strain_psi64<false>(k, nb, S, nullptr);
+    }
+#pragma unroll
+    for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
+}
+
 // The coarse partners of the fine face nodes: coarse column ci0 (face 0) and
 // ci1 (face 1), their grid-coupling neighbour (+x for face 0, -x for face 1)
 // read through its fine partner (column fi0 / fi1 of the fine level, array F).
@@ -750,7 +768,7 @@ __global__ __launch_bounds__(64) void partner_coarse(const K64* __restrict__ kcp
                                                      const T* __restrict__ Ccur, const PopLayout CL, int cnx, int cny,
                                                      int cnz, int ci0, int ci1, const T* __restrict__ F,
                                                      const PopLayout FL, int fnx, int fny, int fi0, int fi1,
-                                                     int nfaces, double* __restrict__ Sout) {
+                                                     int nfaces, int nv, double* __restrict__ Sout) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = cny * cnz;
     if (t >= nfaces * per) return;
@@ -779,11 +797,7 @@ __global__ __launch_bounds__(64) void partner_coarse(const K64* __restrict__ kcp
             macros_si(k, kcs, Ccur, CL, lc, nb[m]);
         }
     }
-    double S[6];
-    strain_psi64<false>(k, nb, S, nullptr);
-    double* o = Sout + (int64_t)face * 6 * per;
-#pragma unroll
-    for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
+    store_handed(k, nb, nv, Sout + (int64_t)face * nv * per, per, r);
 }
 
 // The fine partners of the coarse grid-coupling columns: fine node (fi, 2J,
@@ -792,7 +806,7 @@ __global__ __launch_bounds__(64) void partner_coarse(const K64* __restrict__ kcp
 template <typename T>
 __global__ __launch_bounds__(64) void partner_fine(const K64* __restrict__ kfp, const KShift* __restrict__ kfs,
                                                    const T* __restrict__ F, const PopLayout FL, int fnx, int fny,
-                                                   int fnz, int fi0, int fi1, int cny, int cnz, int nfaces,
+                                                   int fnz, int fi0, int fi1, int cny, int cnz, int nfaces, int nv,
                                                    double* __restrict__ Sout) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = cny * cnz;
@@ -807,11 +821,7 @@ __global__ __launch_bounds__(64) void partner_fine(const K64* __restrict__ kfp, 
         const int a = fi + EX[m], b = (2 * J + EY[m] + fny) % fny, d = (2 * K + EZ[m] + fnz) % fnz;
         macros_si(k, kfs, F, FL, a + (int64_t)fnx * (b + (int64_t)fny * d), nb[m]);
     }
-    double S[6];
-    strain_psi64<false>(k, nb, S, nullptr);
-    double* o = Sout + (int64_t)face * 6 * per;
-#pragma unroll
-    for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
+    store_handed(k, nb, nv, Sout + (int64_t)face * nv * per, per, r);
 }
 
 // HRR under the direct coupling at t + 1/2 (Node.cpp:1892-2111 on level 1,
@@ -828,7 +838,7 @@ __global__ __launch_bounds__(64) void dc_strain_fine(const K64* __restrict__ kcp
                                                      const T* __restrict__ Ccur, const PopLayout CL, int cnx, int cny,
                                                      int cnz, int ci0, int ci1, const T* __restrict__ F,
                                                      const PopLayout FL, int fnx, int fny, int fnz, int nfaces,
-                                                     double* __restrict__ Sout) {
+                                                     int nv, double* __restrict__ Sout) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = cny * cnz;
     if (t >= nfaces * per) return;
@@ -852,28 +862,23 @@ __global__ __launch_bounds__(64) void dc_strain_fine(const K64* __restrict__ kcp
             macros_si(k, kfs, F, FL, a + (int64_t)fnx * (b + (int64_t)fny * d), nb[m]);
         }
     }
-    double S[6];
-    strain_psi64<false>(k, nb, S, nullptr);
-    double* o = Sout + (int64_t)face * 6 * per;
-#pragma unroll
-    for (int v = 0; v < 6; ++v) o[v * per + r] = S[v];
+    store_handed(k, nb, nv, Sout + (int64_t)face * nv * per, per, r);
 }
 
-// The fine face nodes' strain rates from their coarse partners' (coarse
-// column planes Scol, per face [6][cny cnz]) into per-face [6][fny fnz].
+// The fine face nodes' strain rates (and psi) from their coarse partners'
+// (coarse column planes Scol, per face [nv][cny cnz]) into per-face [nv][fny fnz].
 __global__ __launch_bounds__(256) void face_strain(const double* __restrict__ Scol, int cny, int cnz, int fny, int fnz,
-                                                   int nfaces, double* __restrict__ Sface) {
+                                                   int nfaces, int nv, double* __restrict__ Sface) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = fny * fnz, cper = cny * cnz;
     if (t >= nfaces * per) return;
     const int face = t / per, r = t - face * per;
     const int j = r % fny, kk = r / fny;
     const int J = j / 2, K = kk / 2, J1 = (J + 1) % cny, K1 = (K + 1) % cnz;
-    const double* S = Scol + (int64_t)face * 6 * cper;
-    double* o = Sface + (int64_t)face * 6 * per;
+    const double* S = Scol + (int64_t)face * nv * cper;
+    double* o = Sface + (int64_t)face * nv * per;
     const int a = J + cny * K, b = J1 + cny * K, c = J + cny * K1, d = J1 + cny * K1;
-#pragma unroll
-    for (int v = 0; v < 6; ++v) {
+    for (int v = 0; v < nv; ++v) {
         const double* P = S + v * cper;
         double x;
         if (!(j & 1) && !(kk & 1)) x = P[a];
@@ -1116,11 +1121,14 @@ __global__ __launch_bounds__(64) void dc_between(const KShift* __restrict__ ksp,
 // S (planes [6][ny nz] over (j, k)) and push as the bulk kernel pushes them:
 // all 18 links, y and z wrapped, x wrapped when the level is periodic in x and
 // otherwise dropped when they leave the grid.  Launched right after the bulk
-// kernel, it overwrites what that kernel pushed from these nodes.
-template <int ORDER>
-__global__ __launch_bounds__(64) void fix_hrr64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                                double* __restrict__ B, int nx, int ny, int nz, const PopLayout L,
-                                                int col, int xwrap, const double* __restrict__ S) {
+// kernel, it overwrites what that kernel pushed from these nodes.  KIND 0:
+// HRR; with the cubic-Mach terms (KIND 1 HRR, 2 RR) the planes continue with
+// the handed psi ([19][ny nz]), which the collision takes and which replaces
+// the node's psi state (written there by the bulk kernel).
+template <int ORDER, int KIND>
+__global__ __launch_bounds__(64) void fix64(const K64* __restrict__ kp, const double* __restrict__ A,
+                                            double* __restrict__ B, double* __restrict__ psi, int nx, int ny, int nz,
+                                            const PopLayout L, int col, int xwrap, const double* __restrict__ S) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = ny * nz;
     if (t >= per) return;
@@ -1132,7 +1140,16 @@ __global__ __launch_bounds__(64) void fix_hrr64(const K64* __restrict__ kp, cons
     double s6[6];
 #pragma unroll
     for (int v = 0; v < 6; ++v) s6[v] = S[v * per + t];
-    collide_hrr64_strain<ORDER>(*kp, f, s6);
+    if constexpr (KIND == 0) {
+        collide_hrr64_strain<ORDER>(*kp, f, s6);
+    } else {
+        double ps[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) ps[q] = S[(6 + q) * per + t];
+        collide_rr64_handed<ORDER, KIND == 1>(*kp, f, ps, s6);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) psi[q * L.N + l] = ps[q];
+    }
     static_for<0, 18>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         int a = col + EX[q];
@@ -1144,15 +1161,23 @@ __global__ __launch_bounds__(64) void fix_hrr64(const K64* __restrict__ kp, cons
     B[pidx(L, 18, l)] = f[18];
 }
 
-hipError_t launch_fix_hrr64(int order, const K64* k, const double* A, double* B, const Grid& g, int col,
-                            const double* S, hipStream_t s) {
+hipError_t launch_fix64(int op, int order, bool cubic, const K64* k, const double* A, double* B, double* psi,
+                        const Grid& g, int col, const double* S, hipStream_t s) {
     const int per = g.ny * g.nz;
     const dim3 grid((unsigned)((per + 63) / 64)), block(64);
     const int xwrap = !g.xopen && !g.xlodi;
-#define FIX(O) hipLaunchKernelGGL((fix_hrr64<O>), grid, block, 0, s, k, A, B, g.nx, g.ny, g.nz, g.L, col, xwrap, S)
-    if (order == 2) FIX(2);
-    else if (order == 3) FIX(3);
-    else FIX(4);
+    const int kind = !cubic ? 0 : (op == OP_HRR ? 1 : 2);
+    if (kind == 0 && op != OP_HRR) return hipErrorInvalidValue;
+#define FIX(O, KD) \
+    hipLaunchKernelGGL((fix64<O, KD>), grid, block, 0, s, k, A, B, psi, g.nx, g.ny, g.nz, g.L, col, xwrap, S)
+#define FIXK(O) \
+    if (kind == 0) FIX(O, 0); \
+    else if (kind == 1) FIX(O, 1); \
+    else FIX(O, 2)
+    if (order == 2) { FIXK(2); }
+    else if (order == 3) { FIXK(3); }
+    else { FIXK(4); }
+#undef FIXK
 #undef FIX
     return hipGetLastError();
 }
@@ -1172,10 +1197,13 @@ struct acp_refined {
     int32_t graph_chunk = 0;              // coarse steps per captured graph (0: plain launches)
     hipGraphExec_t gexec[4] = {nullptr, nullptr, nullptr, nullptr};  // by coarse + 2 * fine parity
     uint64_t epoch[2] = {0, 0};  // the levels' graph epochs the captured graphs hold
-    // HRR strain-rate transfer (per coupled face, float64 planes of six
-    // components): s_col at the coarse partners of the fine face nodes
-    // [nf][6][cny cnz], s_face at the fine face nodes [nf][6][fny fnz], s_gc
-    // at the coarse grid-coupling columns [nf][6][cny cnz]; null without HRR
+    // HRR strain-rate transfer, with the cubic-Mach terms also for RR (per
+    // coupled face, float64 planes of nv components: S[6], then psi[19] with
+    // the cubic-Mach terms): s_col at the coarse partners of the fine face
+    // nodes [nf][nv][cny cnz], s_face at the fine face nodes [nf][nv][fny fnz],
+    // s_gc at the coarse grid-coupling columns [nf][nv][cny cnz]; null when
+    // nothing is handed
+    int nv = 6;
     double* s_col = nullptr;
     double* s_face = nullptr;
     double* s_gc = nullptr;
@@ -1448,31 +1476,32 @@ hipError_t launch_transfer_t(acp_refined* r, int half) {
         // the level-1 stencil into the planes s_gc (no grid-coupling column
         // inside the fine region under the direct coupling), spread over the faces
         hipLaunchKernelGGL((acp::refine::dc_strain_fine<T>), gc, b64, 0, r->stream, c->dk, c->dks, f->dks, Cprev, Ccur,
-                           c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, fnz, nf, r->s_gc);
+                           c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, fnz, nf, r->nv,
+                           r->s_gc);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(acp::refine::face_strain, gface, b256, 0, r->stream, (const double*)r->s_gc, cny, cnz, fny,
-                           fnz, nf, r->s_face);
+                           fnz, nf, r->nv, r->s_face);
         return hipGetLastError();
     }
     if (half)
         hipLaunchKernelGGL((acp::refine::partner_coarse<T, true>), gc, b64, 0, r->stream, c->dk, c->dks, f->dks, Cprev,
                            Ccur, c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, 2, fnx - 3, nf,
-                           r->s_col);
+                           r->nv, r->s_col);
     else
         hipLaunchKernelGGL((acp::refine::partner_coarse<T, false>), gc, b64, 0, r->stream, c->dk, c->dks, f->dks, Ccur,
                            Ccur, c->g.L, cnx, cny, cnz, r->x0 % cnx, r->x1 % cnx, F, f->g.L, fnx, fny, 2, fnx - 3, nf,
-                           r->s_col);
+                           r->nv, r->s_col);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(acp::refine::face_strain, gface, b256, 0, r->stream, (const double*)r->s_col, cny, cnz, fny, fnz,
-                       nf, r->s_face);
+                       nf, r->nv, r->s_face);
     e = hipGetLastError();
     // under the direct coupling the coarse interface column collides with the
     // coarse partners' strain rates themselves (s_col)
     if (e != hipSuccess || half || r->scheme == ACP_COUPLING_DC) return e;
     hipLaunchKernelGGL((acp::refine::partner_fine<T>), gc, b64, 0, r->stream, f->dk, f->dks, F, f->g.L, fnx, fny, fnz, 2,
-                       fnx - 3, cny, cnz, nf, r->s_gc);
+                       fnx - 3, cny, cnz, nf, r->nv, r->s_gc);
     return hipGetLastError();
 }
 
@@ -1485,12 +1514,18 @@ hipError_t launch_transfer(acp_refined* r, int half) {
     }
 }
 
-// Bytes of the strain-rate transfer buffers of an HRR refined case (every
-// buffer a multiple of 256 bytes; none for other operators).
+// Whether the interface columns collide with values handed by the other
+// level: HRR's strain rate, and with the cubic-Mach terms psi (RR and HRR).
+bool handed(const acp_case_desc& d) { return d.op == ACP_OP_HRR || (d.op == ACP_OP_RR && d.cubic); }
+// planes per interface node: S[6], then psi[19] with the cubic-Mach terms
+int handed_planes(const acp_case_desc& d) { return d.cubic ? 25 : 6; }
+
+// Bytes of the transfer buffers of a refined case (every buffer a multiple of
+// 256 bytes; none when nothing is handed).
 int64_t transfer_bytes(const acp_case_desc& d, const acp_case_desc& fd) {
-    if (d.op != ACP_OP_HRR) return 0;
-    const int64_t nf = fd.boundary == ACP_BC_OPEN_LODI_X ? 1 : 2;
-    const int64_t col = acp::round_alloc(nf * 6 * d.ny * d.nz * 8), face = acp::round_alloc(nf * 6 * fd.ny * fd.nz * 8);
+    if (!handed(d)) return 0;
+    const int64_t nf = fd.boundary == ACP_BC_OPEN_LODI_X ? 1 : 2, nv = handed_planes(d);
+    const int64_t col = acp::round_alloc(nf * nv * d.ny * d.nz * 8), face = acp::round_alloc(nf * nv * fd.ny * fd.nz * 8);
     return 2 * col + face;
 }
 
@@ -1502,7 +1537,8 @@ namespace {
 int fine_desc(const acp_case_desc& d, int32_t x0, int32_t x1, acp_case_desc& fd) {
     if (d.boundary != ACP_BC_PERIODIC && d.boundary != ACP_BC_LODI_X)
         return fail(ACP_E_UNSUPPORTED, "refined cases need a periodic or LODI_X coarse level");
-    if (d.cubic) return fail(ACP_E_UNSUPPORTED, "refined cases: the cubic-Mach terms are not coupled");
+    if (d.cubic && d.op != ACP_OP_RR && d.op != ACP_OP_HRR)
+        return fail(ACP_E_UNSUPPORTED, "refined cases: the cubic-Mach terms belong to RR and HRR");
     if (d.boundary == ACP_BC_PERIODIC && (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx || x1 - x0 >= d.nx - 1))
         return fail(ACP_E_ARG, "fine region [x0, x1] must satisfy 1 <= x0, x0 + 2 <= x1 <= nx, x1 - x0 < nx - 1");
     if (d.boundary == ACP_BC_LODI_X && (x0 < 1 || x1 <= x0 + 1 || x1 > d.nx - 1))
@@ -1516,9 +1552,9 @@ int fine_desc(const acp_case_desc& d, int32_t x0, int32_t x1, acp_case_desc& fd)
     fd.domain[1] = (fd.ny - 0.5) * fd.spacing;
     fd.domain[2] = (fd.nz - 0.5) * fd.spacing;
     fd.boundary = (d.boundary == ACP_BC_LODI_X && x1 == d.nx - 1) ? ACP_BC_OPEN_LODI_X : ACP_BC_OPEN_X;
-    if (d.op == ACP_OP_HRR && fd.boundary == ACP_BC_OPEN_X && x1 - x0 < 3)
-        return fail(ACP_E_ARG, "HRR refined cases with two coupled faces need x1 - x0 >= 3 (one grid-coupling column "
-                               "inside each face)");
+    if (handed(d) && fd.boundary == ACP_BC_OPEN_X && x1 - x0 < 3)
+        return fail(ACP_E_ARG, "HRR (or cubic-Mach) refined cases with two coupled faces need x1 - x0 >= 3 (one "
+                               "grid-coupling column inside each face)");
     return ACP_OK;
 }
 
@@ -1605,16 +1641,18 @@ int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* st
             return acp::hip_fail(e, "fine-level constants");
         }
     }
-    // HRR: the strain-rate transfer buffers, on the coarse level (counted in
-    // its footprint), and the interface columns that collide with them: the
-    // coarse grid-coupling columns x0 + 1 and x1 - 1, the fine face columns
-    if (d.op == ACP_OP_HRR) {
-        const int nf = coupled_faces(r);
+    // HRR (and RR with the cubic-Mach terms): the transfer buffers, on the
+    // coarse level (counted in its footprint), and the interface columns that
+    // collide with them: the coarse grid-coupling columns x0 + 1 and x1 - 1,
+    // the fine face columns
+    r->nv = handed_planes(d);
+    if (handed(d)) {
+        const int nf = coupled_faces(r), nv = r->nv;
         const int64_t cp = (int64_t)d.ny * d.nz, fp = (int64_t)fd.ny * fd.nz;
         void *pc = nullptr, *pg = nullptr, *pf = nullptr;
-        int arc = acp::case_alloc_extra(r->coarse, nf * 6 * cp * 8, &pc);
-        if (arc == ACP_OK) arc = acp::case_alloc_extra(r->coarse, nf * 6 * cp * 8, &pg);
-        if (arc == ACP_OK) arc = acp::case_alloc_extra(r->coarse, nf * 6 * fp * 8, &pf);
+        int arc = acp::case_alloc_extra(r->coarse, nf * nv * cp * 8, &pc);
+        if (arc == ACP_OK) arc = acp::case_alloc_extra(r->coarse, nf * nv * cp * 8, &pg);
+        if (arc == ACP_OK) arc = acp::case_alloc_extra(r->coarse, nf * nv * fp * 8, &pf);
         if (arc != ACP_OK) {
             acp_case_destroy(r->coarse);
             acp_case_destroy(r->fine);
@@ -1629,9 +1667,9 @@ int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* st
         r->fine->nfix = nf;
         for (int s = 0; s < nf; ++s) {
             r->coarse->fix_col[s] = s == 0 ? x0 + 1 : (x1 - 1) % d.nx;
-            r->coarse->fix_S[s] = r->s_gc + (int64_t)s * 6 * cp;
+            r->coarse->fix_S[s] = r->s_gc + (int64_t)s * nv * cp;
             r->fine->fix_col[s] = s == 0 ? 0 : fd.nx - 1;
-            r->fine->fix_S[s] = r->s_face + (int64_t)s * 6 * fp;
+            r->fine->fix_S[s] = r->s_face + (int64_t)s * nv * fp;
         }
     }
     // tau = 1 / omega of each level (Control.cpp:54 with each level's time step)
@@ -1826,18 +1864,18 @@ extern "C" int acp_refined_set_coupling(acp_refined* r, int32_t scheme, int32_t 
     if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
     drop_graphs(r);
     r->scheme = scheme;
-    // HRR: the coarse column that collides with handed strain rates is the
-    // grid-coupling column inside the fine region (standard coupling) or the
-    // interface column itself (direct coupling)
-    const int64_t cp = (int64_t)r->coarse->g.ny * r->coarse->g.nz;
+    // HRR (and RR with the cubic-Mach terms): the coarse column that collides
+    // with handed values is the grid-coupling column inside the fine region
+    // (standard coupling) or the interface column itself (direct coupling)
+    const int64_t cp = (int64_t)r->coarse->g.ny * r->coarse->g.nz * r->nv;
     for (int s = 0; s < r->coarse->nfix; ++s) {
         const int cnx = r->coarse->g.nx;
         if (scheme == ACP_COUPLING_STD) {
             r->coarse->fix_col[s] = s == 0 ? r->x0 + 1 : (r->x1 - 1) % cnx;
-            r->coarse->fix_S[s] = r->s_gc + (int64_t)s * 6 * cp;
+            r->coarse->fix_S[s] = r->s_gc + (int64_t)s * cp;
         } else {
             r->coarse->fix_col[s] = s == 0 ? r->x0 % cnx : r->x1 % cnx;
-            r->coarse->fix_S[s] = r->s_col + (int64_t)s * 6 * cp;
+            r->coarse->fix_S[s] = r->s_col + (int64_t)s * cp;
         }
     }
     r->c2f_set = c2f;

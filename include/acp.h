@@ -55,10 +55,9 @@ enum { ACP_PREC_F64 = 0, ACP_PREC_F32_SHIFTED = 1, ACP_PREC_F16_SHIFTED = 2 };
  *   OPEN_LODI_X  face x = 0 open, face x = nx - 1 LODI: the fine level of a
  *                refined case whose region reaches the high-x face of a
  *                LODI_X coarse level (acp_refined_create makes it).
- * HRR's strain stencil takes the missing neighbour beyond an open face by
- * quadratic extrapolation (u(-1) = 3 u(0) - 3 u(1) + u(2), the second-order
- * one-sided difference); open faces are not available with the cubic-Mach
- * correction. */
+ * HRR's strain stencil, and the cubic-Mach terms' (RR and HRR), take the
+ * missing neighbour beyond an open face by quadratic extrapolation
+ * (u(-1) = 3 u(0) - 3 u(1) + u(2), the second-order one-sided difference). */
 enum { ACP_BC_LODI = 0, ACP_BC_PERIODIC = 1, ACP_BC_OPEN_X = 2, ACP_BC_LODI_X = 3, ACP_BC_OPEN_LODI_X = 4 };
 /* MRT operators: the reference computes the relaxed populations and discards them
  * (Node.cpp:2603-2604, 2833-2834), i.e. MRT steps are pure streaming + LODI. */
@@ -294,13 +293,17 @@ int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* r
  * Every coarse node strictly between x0 and x1 is restricted from the fine
  * level.  A fine LODI face uses the level-0 domain length and its own spacing
  * and time step, as the reference's LODIpressureOutlet does on every level
- * (Node.cpp:237-311).  Every operator without the cubic-Mach correction,
- * every storage tier (both levels use the coarse description's tier).  HRR
- * hands strain rates across the interface as the reference's partner
- * functions compute them (Node.cpp:1892-2490): the fine face nodes collide
- * with their coarse partners' (at t and t + 1/2), the coarse grid-coupling
- * columns with their fine partners' (acp_refined_set_coupling for the direct
- * coupling's form). */
+ * (Node.cpp:237-311).  Every operator, every storage tier (both levels use
+ * the coarse description's tier); the cubic-Mach correction with RR and HRR
+ * in float64.  HRR hands strain rates across the interface as the
+ * reference's partner functions compute them (Node.cpp:1892-2490): the fine
+ * face nodes collide with their coarse partners' (at t and t + 1/2), the
+ * coarse grid-coupling columns with their fine partners'
+ * (acp_refined_set_coupling for the direct coupling's form).  With the
+ * cubic-Mach correction the same functions hand psi from the same stencils
+ * (Node.cpp:2069), and those interface nodes of RR and HRR collide with the
+ * handed psi in their moments, Pi1 and result; every other node keeps the
+ * psi of its own collision. */
 typedef struct acp_refined acp_refined;
 int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, acp_refined** out);
 void acp_refined_destroy(acp_refined* r);

@@ -579,6 +579,32 @@ void acpo_collide_hrr_strain(const acpo_consts* k, const acpo_params* p, double 
     for (int q = 0; q < 19; ++q) f[q] = eq[q] + (1 - om) * f1[q];
 }
 
+/* RR or HRR at a grid-coupling node with the cubic-Mach terms: its partner
+ * hands it psi (Node.cpp:2069 setCubicMachCorrection; 2094, 2474 on the
+ * partner functions' own nodes) before it collides, and its own
+ * calcStrainRateTensorFD... returns at once for a grid-coupling node
+ * (Node.cpp:1790-1793), so the handed psi enters its density and velocity
+ * (Node.cpp:11-30, 85-118), Pi1 (2493-2558) and the result (627, 713); HRR
+ * also takes the handed strain rate S. */
+void acpo_collide_handed(const acpo_consts* k, const acpo_params* p, double f[19], const double S[6],
+                         const double psi[19]) {
+    const double h = 0.5 * k->dt;
+    const double om = k->omega;
+    double rho, u[3], eq[19], a2[6], f1[19];
+    acpo_moments(k, f, psi, &rho, u);
+    acpo_feq(k, p->order, rho, u, eq);
+    acpo_pi1(k, f, eq, psi, a2);
+    if (p->op == ACPO_HRR) {
+        double dens, uu[3];
+        acpo_moments(k, f, psi, &dens, uu); /* Node.cpp:1444 */
+        (void)uu;
+        const double dyn = k->nu * dens;
+        for (int i = 0; i < 6; ++i) a2[i] = k->sigma * a2[i] + (1. - k->sigma) * k->inv_pre * (2. * dyn * S[i]);
+    }
+    acpo_offeq(k, p->order, a2, u, f1);
+    for (int q = 0; q < 19; ++q) f[q] = eq[q] + (1 - om) * f1[q] + h * psi[q];
+}
+
 void acpo_collide(const acpo_consts* k, const acpo_params* p, double f[19], double* psi, const double nb[6][4]) {
     const int cubic = p->cubic;
     const double h = 0.5 * k->dt;
@@ -667,7 +693,9 @@ struct acpo_case {
     int64_t* ph_start;  /* [max_phase + 1] offsets into ph_order */
     int64_t* ph_order;  /* LODI slots sorted by phase, reference order within a phase */
     /* x columns whose interior nodes collide with a strain rate handed over by
-     * the other level of a two-level step (HRR): fix_S[x] holds S[6][ny*nz] */
+     * the other level of a two-level step (HRR, and RR or HRR with the
+     * cubic-Mach terms): fix_S[x] holds S[6][ny*nz], followed by psi[19][ny*nz]
+     * with the cubic-Mach terms */
     int nfix;
     int fix_col[2];
     const double* fix_S[2];
@@ -1055,7 +1083,13 @@ static void one_step(acpo_case* c) {
                         const int64_t t = jj + (int64_t)ny * kk, n2 = (int64_t)ny * nz;
                         const double S[6] = {c->fix_S[fx][t], c->fix_S[fx][n2 + t], c->fix_S[fx][2 * n2 + t],
                                              c->fix_S[fx][3 * n2 + t], c->fix_S[fx][4 * n2 + t], c->fix_S[fx][5 * n2 + t]};
-                        acpo_collide_hrr_strain(&c->k, p, f, S);
+                        if (p->cubic) {
+                            for (int q = 0; q < 19; ++q) ps[q] = c->fix_S[fx][(6 + q) * n2 + t];
+                            acpo_collide_handed(&c->k, p, f, S, ps);
+                            for (int q = 0; q < 19; ++q) c->psi[(size_t)q * N + l] = ps[q];
+                        } else {
+                            acpo_collide_hrr_strain(&c->k, p, f, S);
+                        }
                     } else {
                         for (int q = 0; q < 19; ++q) ps[q] = c->psi[(size_t)q * N + l];
                         acpo_collide(&c->k, p, f, ps, (const double(*)[4])nb);
@@ -1220,9 +1254,12 @@ struct acpo_refined {
     int c2f, f2c, hang; /* DC link sets: 0 missing, 1 missing and common (MfP, McP, hMfP), 2 all (hAll) */
     int mrt_style;      /* Scaling-Style: 0 SRTSTYLE, 1 MRTSTYLE (see scale_neq) */
     double sc_c2f[19], sc_f2c[19]; /* MRTSTYLE factors per Gram-Schmidt moment */
-    /* HRR: strain rates handed between the levels, per coupled face:
-     * s_col at the coarse partners of the fine face nodes, s_face at the fine
-     * face nodes, s_gc at the coarse grid-coupling column (each [6][plane]) */
+    /* HRR, and RR or HRR with the cubic-Mach terms: strain rates (and psi)
+     * handed between the levels, per coupled face: s_col at the coarse
+     * partners of the fine face nodes, s_face at the fine face nodes, s_gc at
+     * the coarse grid-coupling column (each [nv][plane]: S[6], then psi[19]
+     * with the cubic-Mach terms) */
+    int nv;
     double* s_col[2];
     double* s_face[2];
     double* s_gc[2];
@@ -1292,9 +1329,12 @@ acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
     } else {
         return NULL;
     }
-    /* HRR hands strain rates to a grid-coupling column next to each coupled
-     * fine face; with two coupled faces the two columns must differ */
-    if (coarse->op == ACPO_HRR && !(coarse->periodic == 3 && x1 == coarse->nx - 1) && x1 - x0 < 3) return NULL;
+    /* HRR (and RR with the cubic-Mach terms) hands strain rates and psi to a
+     * grid-coupling column next to each coupled fine face; with two coupled
+     * faces the two columns must differ */
+    const int handed = coarse->op == ACPO_HRR || (coarse->op == ACPO_RR && coarse->cubic);
+    if (coarse->cubic && coarse->op != ACPO_RR && coarse->op != ACPO_HRR) return NULL;
+    if (handed && !(coarse->periodic == 3 && x1 == coarse->nx - 1) && x1 - x0 < 3) return NULL;
     acpo_params fp = *coarse;
     fp.nx = 2 * (x1 - x0) + 1;
     fp.ny = 2 * coarse->ny;
@@ -1314,15 +1354,16 @@ acpo_refined* acpo_refined_create(const acpo_params* coarse, int x0, int x1) {
     r->c2f = r->f2c = r->hang = 2; /* STD: every link */
     r->a = (1. / r->f->k.omega) / (2. * (1. / r->c->k.omega));
     acpo_refined_set_scaling(r, 0);
-    if (coarse->op == ACPO_HRR) {
+    r->nv = coarse->cubic ? 25 : 6;
+    if (handed) {
         const int nf = fp.periodic == 4 ? 1 : 2;
         const int64_t cplane = (int64_t)coarse->ny * coarse->nz, fplane = (int64_t)fp.ny * fp.nz;
         r->c->nfix = nf;
         r->f->nfix = nf;
         for (int s = 0; s < nf; ++s) {
-            r->s_col[s] = (double*)calloc((size_t)6 * cplane, sizeof(double));
-            r->s_gc[s] = (double*)calloc((size_t)6 * cplane, sizeof(double));
-            r->s_face[s] = (double*)calloc((size_t)6 * fplane, sizeof(double));
+            r->s_col[s] = (double*)calloc((size_t)r->nv * cplane, sizeof(double));
+            r->s_gc[s] = (double*)calloc((size_t)r->nv * cplane, sizeof(double));
+            r->s_face[s] = (double*)calloc((size_t)r->nv * fplane, sizeof(double));
             /* the coarse grid-coupling columns x0 + 1 and x1 - 1 (one coarse cell
              * inside the fine region, Overlap 1), the fine face columns 0 and
              * nx_f - 1 */
@@ -1695,15 +1736,29 @@ void acpo_refined_restrict(acpo_refined* r) { refined_f2c(r); }
  * strain rates across the interface with the partner functions of
  * Node.cpp:1892-2490 (acpo_strain_*_partner, pinned against it), called at
  * Lattice.cpp:12142 (t + 1/2), 12884 (fine partners before f2c) and 13211
- * (t + 1); a grid-coupling node collides with the strain rate it received
- * (Node.cpp:1795-1798).  The interpolation of the handed strain rates to the
- * fine face nodes lives in the absent GridCoupling.cpp; it is restated like the
- * populations' coarse-to-fine: coincident nodes take their coarse partner's
- * value, mid nodes the mean of two, centre nodes of four. */
+ * (t + 1) for HRR and at 10252, 10995 and 11323 for RR; a grid-coupling node
+ * collides with the strain rate it received (Node.cpp:1795-1798).  The same
+ * functions evaluate the cubic-Mach psi from the same stencil and hand it on
+ * (Node.cpp:2069); with the cubic-Mach terms psi travels with S, and RR's
+ * grid-coupling columns collide with it too.  The interpolation of the handed
+ * values to the fine face nodes lives in the absent GridCoupling.cpp; it is
+ * restated like the populations' coarse-to-fine: coincident nodes take their
+ * coarse partner's value, mid nodes the mean of two, centre nodes of four.
+ * One difference is kept on purpose: the reference's fine-partner function
+ * writes psi into the fine partner node's own state (Node.cpp:2474), which
+ * that node's next density then reads; here the fine partner keeps the psi
+ * of its own collision and the handed value goes to the coarse column only. */
 
 static void gather19(const acpo_case* c, const double* A, int i, int j, int k, double out[19]) {
     const int64_t l = lin(c, i, j, k);
     for (int q = 0; q < 19; ++q) out[q] = A[(size_t)q * c->N + l];
+}
+
+/* entry t of the planes P ([nv][plane]): S, and psi with the cubic-Mach terms */
+static void store_handed(const acpo_refined* r, double* P, int64_t plane, int64_t t, const double S[6],
+                         const double psi[19]) {
+    for (int v = 0; v < 6; ++v) P[v * plane + t] = S[v];
+    for (int v = 6; v < r->nv; ++v) P[v * plane + t] = psi[v - 6];
 }
 
 /* the coarse partners of the fine face nodes (coarse column under each coupled
@@ -1737,7 +1792,7 @@ static void refined_partner_coarse(acpo_refined* r, int half, const double* Cpre
                                                           (const double(*)[19])oth, S, psi);
                 else acpo_strain_coarse_partner(&c->k, c->p.spacing / pow(2., 0), gc, (const double(*)[19])nb,
                                                 (const double(*)[19])oth, S, psi);
-                for (int v = 0; v < 6; ++v) r->s_col[s][v * plane + J + (int64_t)cny * K] = S[v];
+                store_handed(r, r->s_col[s], plane, J + (int64_t)cny * K, S, psi);
             }
     }
 }
@@ -1752,7 +1807,7 @@ static void refined_face_strain(acpo_refined* r, double* const* src) {
         for (int k = 0; k < fnz; ++k)
             for (int j = 0; j < fny; ++j) {
                 const int J = j / 2, K = k / 2, J1 = (J + 1) % cny, K1 = (K + 1) % cnz;
-                for (int v = 0; v < 6; ++v) {
+                for (int v = 0; v < r->nv; ++v) {
                     const double* S = src[s] + v * cp;
                     double x;
                     if (!(j & 1) && !(k & 1)) x = S[J + (int64_t)cny * K];
@@ -1781,7 +1836,7 @@ static void refined_partner_fine(acpo_refined* r, const double* Fcur) {
                 for (int m = 0; m < 6; ++m)
                     gather19(f, Fcur, fi + E[m][0], (2 * J + E[m][1] + fny) % fny, (2 * K + E[m][2] + fnz) % fnz, nb[m]);
                 acpo_strain_fine_partner(&f->k, f->p.spacing / pow(2., 0), (const double(*)[19])nb, S, psi);
-                for (int v = 0; v < 6; ++v) r->s_gc[s][v * plane + J + (int64_t)cny * K] = S[v];
+                store_handed(r, r->s_gc[s], plane, J + (int64_t)cny * K, S, psi);
             }
     }
 }
@@ -1820,7 +1875,7 @@ static void refined_dc_strain_fine(acpo_refined* r, const double* Cprev, const d
                 }
                 acpo_strain_direct_coupling(&f->k, 1, f->p.spacing, missing, (const double(*)[19])pre,
                                             (const double(*)[19])nb, (const double(*)[19])oth, S, psi);
-                for (int v = 0; v < 6; ++v) r->s_gc[s][v * plane + J + (int64_t)cny * K] = S[v];
+                store_handed(r, r->s_gc[s], plane, J + (int64_t)cny * K, S, psi);
             }
     }
 }
@@ -1856,9 +1911,9 @@ void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_g
     const int nf = r->c->nfix;
     const int64_t cp = (int64_t)r->c->p.ny * r->c->p.nz, fp = (int64_t)r->f->p.ny * r->f->p.nz;
     for (int s = 0; s < nf; ++s) {
-        if (s_col) memcpy(s_col + s * 6 * cp, r->s_col[s], sizeof(double) * 6 * cp);
-        if (s_gc) memcpy(s_gc + s * 6 * cp, r->s_gc[s], sizeof(double) * 6 * cp);
-        if (s_face) memcpy(s_face + s * 6 * fp, r->s_face[s], sizeof(double) * 6 * fp);
+        if (s_col) memcpy(s_col + s * r->nv * cp, r->s_col[s], sizeof(double) * r->nv * cp);
+        if (s_gc) memcpy(s_gc + s * r->nv * cp, r->s_gc[s], sizeof(double) * r->nv * cp);
+        if (s_face) memcpy(s_face + s * r->nv * fp, r->s_face[s], sizeof(double) * r->nv * fp);
     }
 }
 

@@ -96,6 +96,10 @@ void acpo_collide(const acpo_consts* k, const acpo_params* p, double f[19], doub
  * collision of a grid-coupling node, which keeps the strain rate its partner
  * hands it (Node.cpp:1795-1798) */
 void acpo_collide_hrr_strain(const acpo_consts* k, const acpo_params* p, double f[19], const double S[6]);
+/* RR or HRR with the cubic-Mach terms at a grid-coupling node: psi (and, for
+ * HRR, S) handed by its partner enter its moments, Pi1 and the result */
+void acpo_collide_handed(const acpo_consts* k, const acpo_params* p, double f[19], const double S[6],
+                         const double psi[19]);
 
 /* ---- whole lattice ---- */
 typedef struct acpo_case acpo_case;
@@ -160,10 +164,12 @@ int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int
 void acpo_dc_matrix(const acpo_consts* k, double a, int set, int n, double m[6]);
 void acpo_dc_state(const acpo_consts* k, int order, double a, const double m[6], int set, int n, const double cv[19],
                    const double fv[19], double e[19]);
-/* HRR: the strain-rate transfer of a step's start (half = 0) or of t + 1/2
- * (half = 1) alone, on the levels' current arrays (coarse A at t + 1, B at t);
- * copies the planes out where the pointers are not NULL: s_col and s_gc
- * [faces][6][cny cnz], s_face [faces][6][fny fnz] */
+/* HRR (and RR or HRR with the cubic-Mach terms): the strain-rate transfer of
+ * a step's start (half = 0) or of t + 1/2 (half = 1) alone, on the levels'
+ * current arrays (coarse A at t + 1, B at t); copies the planes out where the
+ * pointers are not NULL: s_col and s_gc [faces][nv][cny cnz], s_face
+ * [faces][nv][fny fnz], nv = 6 (S) or 25 (S, then psi[19]) with the
+ * cubic-Mach terms */
 void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_gc, double* s_face);
 /* the fine-to-coarse restriction alone, on the levels' current A arrays */
 void acpo_refined_restrict(acpo_refined* r);

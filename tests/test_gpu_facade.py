@@ -107,8 +107,8 @@ def test_stepping_member_of_another_operator_exits(gpu, tmp_path):
     assert "timeSteppingRR" in r.stderr and "one operator per case" in r.stderr
 
 
-def _refined_text(nx, ny, nz, op, region, steps, coupling=None):
-    txt = bc_text(nx, ny, nz, op=op, tau=0.56, steps=steps).replace("MaxLevel 0", "MaxLevel 1")
+def _refined_text(nx, ny, nz, op, region, steps, coupling=None, cubic=False):
+    txt = bc_text(nx, ny, nz, op=op, tau=0.56, steps=steps, cubic=cubic).replace("MaxLevel 0", "MaxLevel 1")
     if coupling:
         scheme, c2f, f2c, hang, filt = coupling[:5]
         scaling = coupling[5] if len(coupling) > 5 else "SRTSTYLE"
@@ -132,12 +132,25 @@ def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, couplin
     nestedTimeStepping step and then solve(flagL0, flagL1) steps, equal to
     acp_refined_step's, with getDistribution on both levels and the
     coarse/fine partners."""
+    _two_level_program(tmp_path, op, coupling, False)
+
+
+@pytest.mark.parametrize("op,coupling", [("RR", COUPLINGS[0]), ("HRR", COUPLINGS[1])], ids=["RR-STD", "HRR-DC1"])
+def test_two_level_lattice_program_with_cubic_mach_terms(gpu, tmp_path, op, coupling):
+    """The same with Cubic-Mach-Correction yes: the case file's two-level run
+    equals acp_refined_step's with the cubic-Mach terms handed across the
+    interface."""
+    _two_level_program(tmp_path, op, coupling, True)
+
+
+def _two_level_program(tmp_path, op, coupling, cubic):
     nx, ny, nz, steps, x0, x1 = 24, 8, 3, 4, 12, 23
     f = tmp_path / "bc.txt"
-    f.write_text(_refined_text(nx, ny, nz, op, (x0, x1), steps, coupling))
+    f.write_text(_refined_text(nx, ny, nz, op, (x0, x1), steps, coupling, cubic))
     d = acp.read_case_file(str(f))
+    assert d.cubic == int(cubic)
     d.max_level = 0
-    print("case refined", op, coupling, flush=True)
+    print("case refined", op, coupling, "cubic" if cubic else "", flush=True)
     with acp.Refined(d, x0, x1) as rc:
         if coupling:
             rc.set_coupling(*coupling[:4])

@@ -184,10 +184,85 @@ def test_open_x_hrr_matches_oracle_bitexact(gpu, order):
     assert np.max(np.abs(rho32 - r64)) <= 2e-5 * np.max(np.abs(r64 - RHO0))
 
 
+def _moving_pulse(lp, amp, cx, cy, b, U, scale, shift):
+    """The pulse's density (centre and width in coarse nodes, scaled to the
+    level's nodes) in a uniform flow U along x, as feq_N."""
+    f = ol.pulse_ic(lp, amp, (cx - shift) * scale, cy * scale, b * scale)
+    rho, _ = ol.macros(lp, f)
+    N = lp.nx * lp.ny * lp.nz
+    return ol.equilibrium_field(lp, rho, np.vstack([np.full(N, U), np.zeros(N), np.zeros(N)]))
+
+
+@pytest.mark.parametrize("op,order,bnd,x1,scheme,filt", [
+    ("RR", 3, "PERIODIC", 16, "STD", "NOFI"), ("HRR", 3, "PERIODIC", 16, "STD", "LAG"),
+    ("RR", 4, "LODI-X", 17, "DC1", "LAG"), ("HRR", 4, "LODI-X", 23, "STD", "TOU"),
+    ("HRR", 3, "LODI-X", 17, "DC1", "NOFI"), ("RR", 2, "LODI-X", 23, "DC1", "NOFI")])
+def test_refined_cubic_matches_oracle_bitexact(gpu, op, order, bnd, x1, scheme, filt):
+    """The cubic-Mach terms across the interface (psi handed by the partner
+    functions with the strain rate, the interface nodes colliding with it): a
+    pulse in a mean flow crossing the interface, both levels' populations and
+    psi bit for bit against the oracle."""
+    nx, ny, nz, x0 = 24, 10, 3, 8
+    per = 1 if bnd == "PERIODIC" else 3
+    p = ol.make_params(nx, ny, nz, op=op, order=order, cubic=True, periodic=per, tau=0.6, cs=CS, spacing=DX,
+                       rho0=RHO0, transverse=bnd != "PERIODIC", sigma=0.7)
+    o = ol.OracleRefined(p, x0, x1)
+    o.set_filter(filt)
+    if scheme != "STD":
+        o.set_coupling(scheme, "Mf", "Mc", "hMf")
+    amp, cx, cy, b, U = 1e-3, 12.0, 5.0, 2.5, 0.08 * CS
+    f0 = [_moving_pulse(p, amp, cx, cy, b, U, 1.0, 0), _moving_pulse(o.levels[1].p, amp, cx, cy, b, U, 2.0, x0)]
+    for lv in (0, 1):
+        o.levels[lv].set_state(f0[lv])
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, cubic=True, tau=0.6, sigma=0.7, boundary=bnd,
+                      transverse=bnd != "PERIODIC")
+    print("case", op, order, bnd, x1, scheme, filt, flush=True)
+    with acp.Refined(d, x0, x1) as r:
+        r.set_filter(filt)
+        if scheme != "STD":
+            r.set_coupling(scheme, "Mf", "Mc", "hMf")
+        for lv in (0, 1):
+            r.level(lv).set_populations(f0[lv])
+        for n in (1, 4, 11):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+                assert np.array_equal(r.level(lv).psi(), o.levels[lv].psi()), (n, lv)
+        assert np.max(np.abs(r.level(1).psi())) > 0
+
+
+@pytest.mark.parametrize("op", ["RR", "HRR"])
+def test_open_x_cubic_matches_oracle_bitexact(gpu, op):
+    """The cubic-Mach terms with open x faces (the fine level's boundary kind):
+    psi of a face node from the extrapolated neighbour, bit for bit."""
+    nx, ny, nz = 12, 8, 5
+    p = ol.make_params(nx, ny, nz, op=op, order=3, cubic=True, periodic=2, tau=0.56, sigma=0.7, cs=CS, spacing=DX,
+                       rho0=RHO0)
+    rng = np.random.default_rng(5)
+    N = nx * ny * nz
+    f0 = ol.equilibrium_field(p, RHO0 * (1.0 + 0.01 * rng.uniform(-1, 1, N)), 0.05 * CS * rng.uniform(-1, 1, (3, N)))
+    o = ol.OracleCase(p)
+    o.set_state(f0)
+    o.step(6)
+    d = acp.make_desc(nx, ny, nz, op=op, order=3, cubic=True, tau=0.56, sigma=0.7, boundary="OPEN-X")
+    with acp.Case(d) as c:
+        c.set_populations(f0)
+        c.step(6)
+        assert np.array_equal(c.populations(), o.state())
+        assert np.array_equal(c.psi(), o.psi())
+
+
 def test_refined_rejects_unsupported():
-    d = acp.make_desc(24, 10, 3, op="HRR", cubic=True, tau=0.6, boundary="PERIODIC")
+    d = acp.make_desc(24, 10, 3, op="SRT", cubic=True, tau=0.6, boundary="PERIODIC")
     with pytest.raises(acp.AcpError):
-        acp.Refined(d, 8, 16)  # the cubic-Mach terms are not coupled
+        acp.Refined(d, 8, 16)  # the cubic-Mach terms belong to RR and HRR
+    d = acp.make_desc(24, 10, 3, op="RR", cubic=True, tau=0.6, boundary="PERIODIC")
+    with pytest.raises(acp.AcpError):
+        acp.Refined(d, 8, 10)  # a grid-coupling column inside each coupled face
+    d = acp.make_desc(24, 10, 3, op="HRR", cubic=True, tau=0.6, boundary="PERIODIC", precision="F32-SHIFTED")
+    with pytest.raises(acp.AcpError):
+        acp.Refined(d, 8, 16)  # the cubic-Mach terms in float64 only
     d = acp.make_desc(24, 10, 3, op="SRT", tau=0.6)
     with pytest.raises(acp.AcpError):
         acp.Refined(d, 8, 16)  # LODI coarse level

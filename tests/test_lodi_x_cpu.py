@@ -191,7 +191,9 @@ def test_capi_lodi_x_footprint_and_checks():
     with pytest.raises(acp.AcpError):
         acp.footprint(acp.make_desc(3, 24, 6, op="SRT", tau=0.56, boundary="LODI-X"))
     with pytest.raises(acp.AcpError):
-        acp.footprint(acp.make_desc(40, 24, 6, op="RR", cubic=True, tau=0.56, boundary="OPEN-LODI-X"))
+        acp.footprint(acp.make_desc(40, 24, 6, op="RR", cubic=True, tau=0.56, boundary="OPEN-LODI-X",
+                                    precision="F32-SHIFTED"))
+    acp.footprint(acp.make_desc(40, 24, 6, op="RR", cubic=True, tau=0.56, boundary="OPEN-LODI-X"))
     acp.footprint(acp.make_desc(40, 24, 6, op="RR", cubic=True, tau=0.56, boundary="LODI-X"))
     d = acp.make_desc(40, 24, 6, op="SRT", tau=0.56, boundary="LODI-X")
     for good in [(1, 3), (8, 39), (20, 30)]:
