@@ -132,12 +132,29 @@ __device__ __forceinline__ void c2f_value(const K64& k, const double* __restrict
 // First the coincident nodes (even j and k, one thread per coarse (J, K) of
 // either face) take their coarse node's value; then the other face nodes
 // average their 2 (mid nodes) or 4 (centre nodes) coincident neighbours.
+// Link sets of the standard coupling (BoundaryConditions.txt:73-86), bit q
+// for direction q: the links the fine face nodes at coarse positions take from
+// the coarse level (Coarse-to-Fine fAll / MfP / Mf) and those the other face
+// nodes average (Hanging-Node-Reconstruction), per face; the coarse columns
+// that take only some restricted links (Fine-to-Coarse Mc: the grid-coupling
+// columns x0 + 1 and x1 - 1, -1 for none) and those links.  Every other
+// restricted node takes all 19.
+constexpr uint32_t kAllLinks = (1u << 19) - 1;
+struct StdLinks {
+    uint32_t face[2], hang[2];
+    int gc[2];
+    uint32_t gcm[2];
+};
+__device__ __forceinline__ uint32_t gc_mask(const StdLinks& lk, int I) {
+    return I == lk.gc[0] ? lk.gcm[0] : (I == lk.gc[1] ? lk.gcm[1] : kAllLinks);
+}
+
 template <int ORDER>
 __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp, const double* __restrict__ Cprev,
                                                       const double* __restrict__ Ccur, int half, const NeqScale a, int cnx,
                                                       int cny, int cnz, int ci0, int ci1, double* __restrict__ F,
                                                       int fnx, int fny, const PopLayout CN, const PopLayout FN,
-                                                      int nfaces) {
+                                                      int nfaces, const StdLinks lk) {
     __shared__ K64 sk;
     const K64& k = stage_k64(kp, sk);
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -150,8 +167,10 @@ __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp
     const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
     double f[19];
     c2f_value<ORDER>(k, Cprev, Ccur, CN, lc, half, a, f);
+    const uint32_t m = lk.face[face];
 #pragma unroll
-    for (int q = 0; q < 19; ++q) F[pidx(FN, q, lf)] = f[q];
+    for (int q = 0; q < 19; ++q)
+        if ((m >> q) & 1u) F[pidx(FN, q, lf)] = f[q];
 }
 
 // c2f_between: Fc and F are the same array; the kernel reads only coincident nodes (even j
@@ -159,7 +178,8 @@ __global__ __launch_bounds__(256) void c2f_coincident(const K64* __restrict__ kp
 // both read and written and both pointers may be restrict-qualified, which
 // lets every load of a thread be issued before its first store.
 __global__ __launch_bounds__(256) void c2f_between(const double* __restrict__ Fc, double* __restrict__ F, int fnx,
-                                                   int fny, int fnz, const PopLayout FN, int nfaces) {
+                                                   int fny, int fnz, const PopLayout FN, int nfaces,
+                                                   const StdLinks lk) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = fny * fnz;
     if (t >= nfaces * per) return;
@@ -173,17 +193,20 @@ __global__ __launch_bounds__(256) void c2f_between(const double* __restrict__ Fc
         return fi + (int64_t)fnx * (jj + (int64_t)fny * kk);
     };
     const int64_t l = at(j, k);
+    const uint32_t m = lk.hang[face];
     if ((j & 1) && (k & 1)) {
         const int64_t a = at(j - 1, k - 1), b = at(j + 1, k - 1), c = at(j - 1, k + 1), d = at(j + 1, k + 1);
 #pragma unroll
         for (int q = 0; q < 19; ++q)
-            F[pidx(FN, q, l)] =
-                ((Fc[pidx(FN, q, a)] + Fc[pidx(FN, q, b)]) + (Fc[pidx(FN, q, c)] + Fc[pidx(FN, q, d)])) * 0.25;
+            if ((m >> q) & 1u)
+                F[pidx(FN, q, l)] =
+                    ((Fc[pidx(FN, q, a)] + Fc[pidx(FN, q, b)]) + (Fc[pidx(FN, q, c)] + Fc[pidx(FN, q, d)])) * 0.25;
     } else {
         const int64_t a = (j & 1) ? at(j - 1, k) : at(j, k - 1);
         const int64_t b = (j & 1) ? at(j + 1, k) : at(j, k + 1);
 #pragma unroll
-        for (int q = 0; q < 19; ++q) F[pidx(FN, q, l)] = (Fc[pidx(FN, q, a)] + Fc[pidx(FN, q, b)]) * 0.5;
+        for (int q = 0; q < 19; ++q)
+            if ((m >> q) & 1u) F[pidx(FN, q, l)] = (Fc[pidx(FN, q, a)] + Fc[pidx(FN, q, b)]) * 0.5;
     }
 }
 
@@ -218,7 +241,7 @@ template <int ORDER, bool TOU>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ? 4 : 3))) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                                int fny, int fnz, const NeqScale ainv, double* __restrict__ C, int cnx,
                                                int cny, int cnz, int x0, int width, const PopLayout CN,
-                                               const PopLayout FN) {
+                                               const PopLayout FN, const StdLinks lk) {
     __shared__ double neq[kLagNodes][19][19];  // [node][fine node s][direction]
     __shared__ double eq0[kLagNodes][19];      // the coincident node's equilibrium
     const K64& k = *kp;
@@ -259,6 +282,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
     const int q = s;
     const int I = (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
+    const bool take = (gc_mask(lk, I) >> q) & 1u;  // Fine-to-Coarse link set of this column
     double nq = 0.;  // the non-equilibrium part lane q hands to population q
     if constexpr (TOU) {
         __shared__ double nbar[kLagNodes][19];
@@ -300,7 +324,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
         nq = acc / 19.;
     }
     if (!ainv.mrt) {
-        if (active) C[pidx(CN, q, lc)] = eq0[node][q] + ainv.a * nq;
+        if (active && take) C[pidx(CN, q, lc)] = eq0[node][q] + ainv.a * nq;
         return;
     }
     // MRTSTYLE: the node's 19 parts gathered in LDS, lane r forms moment r
@@ -348,7 +372,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
             out = sum;
         }
     });
-    C[pidx(CN, q, lc)] = eq0[node][q] + out;
+    if (take) C[pidx(CN, q, lc)] = eq0[node][q] + out;
 }
 
 // Coarse nodes with x in [x0 + 1, x1 - 1] from their coincident fine nodes
@@ -357,7 +381,7 @@ template <int ORDER>
 __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                            int fny, int fnz, const NeqScale ainv, double* __restrict__ C, int cnx,
                                            int cny, int cnz, int x0, int width, const PopLayout CN,
-                                           const PopLayout FN) {
+                                           const PopLayout FN, const StdLinks lk) {
     __shared__ K64 sk;
     const K64& k = stage_k64(kp, sk);
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -372,8 +396,10 @@ __global__ __launch_bounds__(256) void f2c(const K64* __restrict__ kp, const dou
 #pragma unroll
     for (int q = 0; q < 19; ++q) f[q] = F[pidx(FN, q, lf)];
     rescale<ORDER>(k, f, ainv);
+    const uint32_t m = gc_mask(lk, I);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) C[pidx(CN, q, lc)] = f[q];
+    for (int q = 0; q < 19; ++q)
+        if ((m >> q) & 1u) C[pidx(CN, q, lc)] = f[q];
 }
 
 // ---- shifted tiers: stored g = s (f / rho0 - w) ---------------------------
@@ -446,11 +472,12 @@ __device__ __forceinline__ void couple_value(const K64& k, const KShift& ks, con
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, const T* Cprev, const T* Ccur,
                                             const PopLayout& CN, int64_t lc, int half, const NeqScale& a, T* F,
-                                            const PopLayout& FN, int64_t lf) {
+                                            const PopLayout& FN, int64_t lf, uint32_t m) {
     T v[19];
     couple_value<T, ORDER, F64>(k, ks, Cprev, Ccur, CN, lc, half, a, v);
 #pragma unroll
-    for (int q = 0; q < 19; ++q) F[pidx(FN, q, lf)] = v[q];
+    for (int q = 0; q < 19; ++q)
+        if ((m >> q) & 1u) F[pidx(FN, q, lf)] = v[q];
 }
 
 // LAG restriction in the shifted tiers: the same mean as f2c_lag, in float64
@@ -458,7 +485,7 @@ __device__ __forceinline__ void couple_node(const K64& k, const KShift& ks, cons
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
                                            int fnz, int fx, int fy, int fz, const NeqScale& ainv, T* __restrict__ C,
-                                           const PopLayout& CN, int64_t lc, const PopLayout& FN) {
+                                           const PopLayout& CN, int64_t lc, const PopLayout& FN, uint32_t m) {
     const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
     T raw[19];
     load_raw19(F, FN, l0, raw);
@@ -493,7 +520,7 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
         scale_neq(k, acc, ainv);
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            C[pidx(CN, q, lc)] = f_to_g<T>(ks, eq[q] + acc[q], q);
+            if ((m >> q) & 1u) C[pidx(CN, q, lc)] = f_to_g<T>(ks, eq[q] + acc[q], q);
         });
     } else {
         float d[19], eq[19], acc[19], dr, ux, uy, uz;
@@ -524,7 +551,7 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
         const float af = (float)ainv.a;
         static_for<0, 19>([&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            C[pidx(CN, q, lc)] = (T)((eq[q] + af * (acc[q] * (1.f / 19.f))) * ks.s);
+            if ((m >> q) & 1u) C[pidx(CN, q, lc)] = (T)((eq[q] + af * (acc[q] * (1.f / 19.f))) * ks.s);
         });
     }
 }
@@ -535,7 +562,7 @@ __device__ __forceinline__ void lag_node_s(const K64& k, const KShift& ks, const
 template <typename T, int ORDER, bool F64>
 __device__ __forceinline__ void tou_node_s(const K64& k, const KShift& ks, const T* __restrict__ F, int fnx, int fny,
                                            int fnz, int fx, int fy, int fz, const NeqScale& ainv, T* __restrict__ C,
-                                           const PopLayout& CN, int64_t lc, const PopLayout& FN) {
+                                           const PopLayout& CN, int64_t lc, const PopLayout& FN, uint32_t m) {
     using R = std::conditional_t<F64, double, float>;
     const int64_t l0 = fx + (int64_t)fnx * (fy + (int64_t)fny * fz);
     T raw[19];
@@ -608,6 +635,7 @@ __device__ __forceinline__ void tou_node_s(const K64& k, const KShift& ks, const
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         const R out = eq[q] + n[q];
+        if (!((m >> q) & 1u)) return;
         if constexpr (F64) C[pidx(CN, q, lc)] = f_to_g<T>(ks, out, q);
         else C[pidx(CN, q, lc)] = (T)(out * ks.s);
     });
@@ -621,7 +649,8 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
                                                         const T* __restrict__ Cprev, const T* __restrict__ Ccur,
                                                         int half, const NeqScale a, int cnx, int cny, int cnz, int ci0,
                                                         int ci1, T* __restrict__ F, int fnx, int fny,
-                                                        const PopLayout CN, const PopLayout FN, int nfaces) {
+                                                        const PopLayout CN, const PopLayout FN, int nfaces,
+                                                        const StdLinks lk) {
     __shared__ K64 sk;
     const K64& k = F64 ? stage_k64(kp, sk) : *kp;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -632,13 +661,13 @@ __global__ __launch_bounds__(256) void c2f_coincident_s(const K64* __restrict__ 
     const int ci = face ? ci1 : ci0, fi = face ? fnx - 1 : 0;
     const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
     const int64_t lf = fi + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
-    couple_node<T, ORDER, F64>(k, *ksp, Cprev, Ccur, CN, lc, half, a, F, FN, lf);
+    couple_node<T, ORDER, F64>(k, *ksp, Cprev, Ccur, CN, lc, half, a, F, FN, lf, lk.face[face]);
 }
 
 template <typename T, bool F64>
 __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ ksp, const T* __restrict__ Fc,
                                                      T* __restrict__ F, int fnx, int fny, int fnz, const PopLayout FN,
-                                                     int nfaces) {
+                                                     int nfaces, const StdLinks lk) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int per = fny * fnz;
     if (t >= nfaces * per) return;
@@ -663,6 +692,7 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
         nb[1] = (j & 1) ? at(j + 1, k) : at(j, k + 1);
         n = 2;
     }
+    const uint32_t m = lk.hang[face];
     T r0[19], r1[19], r2[19], r3[19];
     load_raw19(Fc, FN, nb[0], r0);
     load_raw19(Fc, FN, nb[1], r1);
@@ -672,6 +702,7 @@ __global__ __launch_bounds__(256) void c2f_between_s(const KShift* __restrict__ 
     }
     static_for<0, 19>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
+        if (!((m >> q) & 1u)) return;
         if constexpr (F64) {
             double v = n == 4 ? ((g_to_f(ks, r0[q], q) + g_to_f(ks, r1[q], q)) +
                                  (g_to_f(ks, r2[q], q) + g_to_f(ks, r3[q], q))) * 0.25
@@ -689,7 +720,7 @@ template <typename T, int ORDER, bool F64, int FILT>
 __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const KShift* __restrict__ ksp,
                                              const T* __restrict__ F, int fnx, int fny, int fnz, const NeqScale ainv,
                                              T* __restrict__ C, int cnx, int cny, int cnz, int x0, int width,
-                                             const PopLayout CN, const PopLayout FN) {
+                                             const PopLayout CN, const PopLayout FN, const StdLinks lk) {
     __shared__ K64 sk;
     const K64& k = F64 ? stage_k64(kp, sk) : *kp;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -699,13 +730,14 @@ __global__ __launch_bounds__(256) void f2c_s(const K64* __restrict__ kp, const K
     const int J = (int)(r % cny), K = (int)(r / cny);
     const int I = (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
+    const uint32_t m = gc_mask(lk, I);
     if constexpr (FILT == ACP_F2C_LAG) {
-        lag_node_s<T, ORDER, F64>(k, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN);
+        lag_node_s<T, ORDER, F64>(k, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN, m);
     } else if constexpr (FILT == ACP_F2C_TOU) {
-        tou_node_s<T, ORDER, F64>(k, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN);
+        tou_node_s<T, ORDER, F64>(k, *ksp, F, fnx, fny, fnz, 2 * (1 + di), 2 * J, 2 * K, ainv, C, CN, lc, FN, m);
     } else {
         const int64_t lf = 2 * (1 + di) + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
-        couple_node<T, ORDER, F64>(k, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc);
+        couple_node<T, ORDER, F64>(k, *ksp, F, F, FN, lf, 0, ainv, C, CN, lc, m);
     }
 }
 
@@ -1237,6 +1269,30 @@ acp::refine::NeqScale neq_scale(const acp_refined* r, bool f2c) {
     return s;
 }
 
+bool dc_link(int set, int q, int n);
+
+// The standard coupling's link sets (see StdLinks); every link under the
+// direct coupling, whose pair exchange hands the interface column its links.
+acp::refine::StdLinks std_links(const acp_refined* r) {
+    acp::refine::StdLinks l{};
+    const bool std_ = r->scheme == ACP_COUPLING_STD;
+    for (int s = 0; s < 2; ++s) {
+        const int n = s == 0 ? 1 : -1;
+        for (int q = 0; q < 19; ++q) {
+            if (!std_ || dc_link(r->c2f_set, q, n)) l.face[s] |= 1u << q;
+            if (!std_ || dc_link(r->hang_set, q, n)) l.hang[s] |= 1u << q;
+            if (dc_link(r->f2c_set, q, -n)) l.gcm[s] |= 1u << q;
+        }
+        l.gc[s] = -1;
+    }
+    if (std_ && r->f2c_set != ACP_LINKS_ALL) {
+        const int cnx = r->coarse->g.nx;
+        l.gc[0] = (r->x0 + 1) % cnx;
+        if (coupled_faces(r) == 2) l.gc[1] = (r->x1 - 1) % cnx;
+    }
+    return l;
+}
+
 // Coarse-to-fine on the coupled fine faces: coincident nodes, then the
 // others.  These are a few thousand latency-bound threads each: 64-thread
 // workgroups spread them over more CUs.
@@ -1247,12 +1303,13 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
     const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
     const int nf = coupled_faces(r);
+    const acp::refine::StdLinks lk = std_links(r);
     const dim3 b(64), g1((unsigned)((nf * cny * cnz + 63) / 64)), g2((unsigned)((nf * fny * fnz + 63) / 64));
 #define C2FS(O)                                                                                                  \
     hipLaunchKernelGGL((acp::refine::c2f_coincident_s<T, O, F64>), g1, b, 0, r->stream, c->dk, c->dks,            \
                        (const T*)c->SB, (const T*)c->SA, half, neq_scale(r, false), cnx, cny, cnz, r->x0 % cnx,   \
                        r->x1 % cnx,                                                                               \
-                       (T*)f->SA, fnx, fny, c->g.L, f->g.L, nf)
+                       (T*)f->SA, fnx, fny, c->g.L, f->g.L, nf, lk)
     if (c->d.order == 2) C2FS(2);
     else if (c->d.order == 3) C2FS(3);
     else C2FS(4);
@@ -1260,7 +1317,7 @@ hipError_t launch_c2f_s(acp_refined* r, int half) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((acp::refine::c2f_between_s<T, F64>), g2, b, 0, r->stream, f->dks, (const T*)f->SA,
-                       (T*)f->SA, fnx, fny, fnz, f->g.L, nf);
+                       (T*)f->SA, fnx, fny, fnz, f->g.L, nf, lk);
     return hipGetLastError();
 }
 
@@ -1270,11 +1327,12 @@ hipError_t launch_f2c_s(acp_refined* r) {
     acp_case* f = r->fine;
     const int width = r->x1 - r->x0 - 1;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
+    const acp::refine::StdLinks lk = std_links(r);
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define F2CS(O, FI)                                                                                              \
     hipLaunchKernelGGL((acp::refine::f2c_s<T, O, F64, FI>), g, b, 0, r->stream, c->dk, c->dks, (const T*)f->SA, f->g.nx, \
                        f->g.ny, f->g.nz, neq_scale(r, true), (T*)c->SA, c->g.nx, c->g.ny, c->g.nz, r->x0, width,   \
-                       c->g.L, f->g.L)
+                       c->g.L, f->g.L, lk)
 #define F2CS_O(FI)                       \
     if (c->d.order == 2) F2CS(2, FI);    \
     else if (c->d.order == 3) F2CS(3, FI); \
@@ -1301,11 +1359,12 @@ hipError_t launch_c2f(acp_refined* r, int half) {
     const int cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
     const int fnx = f->g.nx, fny = f->g.ny, fnz = f->g.nz;
     const int nf = coupled_faces(r);
+    const acp::refine::StdLinks lk = std_links(r);
     const dim3 b(64), g1((unsigned)((nf * cny * cnz + 63) / 64)), g2((unsigned)((nf * fny * fnz + 63) / 64));
 #define C2F(O)                                                                                                    \
     hipLaunchKernelGGL((acp::refine::c2f_coincident<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half,           \
                        neq_scale(r, false), cnx,                                                                  \
-                       cny, cnz, r->x0 % cnx, r->x1 % cnx, f->A, fnx, fny, c->g.L, f->g.L, nf)
+                       cny, cnz, r->x0 % cnx, r->x1 % cnx, f->A, fnx, fny, c->g.L, f->g.L, nf, lk)
     if (c->d.order == 2) C2F(2);
     else if (c->d.order == 3) C2F(3);
     else C2F(4);
@@ -1313,7 +1372,7 @@ hipError_t launch_c2f(acp_refined* r, int half) {
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(acp::refine::c2f_between, g2, b, 0, r->stream, (const double*)f->A, f->A, fnx, fny, fnz,
-                       f->g.L, nf);
+                       f->g.L, nf, lk);
     return hipGetLastError();
 }
 
@@ -1327,17 +1386,18 @@ hipError_t launch_f2c(acp_refined* r) {
         return r->f64_coupling ? launch_f2c_s<_Float16, true>(r) : launch_f2c_s<_Float16, false>(r);
     if (width <= 0) return hipSuccess;
     const int64_t n = (int64_t)width * c->g.ny * c->g.nz;
+    const acp::refine::StdLinks lk = std_links(r);
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
 #define F2C(O)                                                                                                  \
     hipLaunchKernelGGL((acp::refine::f2c<O>), g, b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz,       \
                        neq_scale(r, true),                                                                        \
-                       c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L)
+                       c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L, lk)
 #define F2CLAG(O, TOU)                                                                                           \
     hipLaunchKernelGGL((acp::refine::f2c_lag<O, TOU>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) /        \
                                                                         acp::refine::kLagNodes)),                \
                        b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, neq_scale(r, true), c->A, c->g.nx, \
                        c->g.ny,                                                                                   \
-                       c->g.nz, r->x0, width, c->g.L, f->g.L)
+                       c->g.nz, r->x0, width, c->g.L, f->g.L, lk)
     if (r->filter != ACP_F2C_NOFI) {
         const bool tou = r->filter == ACP_F2C_TOU;
         if (c->d.order == 2) {
@@ -1850,9 +1910,12 @@ extern "C" int acp_refined_set_coupling(acp_refined* r, int32_t scheme, int32_t 
     auto valid = [](int32_t v) { return v == ACP_LINKS_MISSING || v == ACP_LINKS_MISSING_COMMON || v == ACP_LINKS_ALL; };
     if (!valid(c2f) || !valid(f2c) || !valid(hanging)) return fail(ACP_E_ARG, "coupling link sets: ACP_LINKS_*");
     if (scheme == ACP_COUPLING_STD) {
-        if (c2f != ACP_LINKS_ALL || f2c != ACP_LINKS_ALL || hanging != ACP_LINKS_ALL)
-            return fail(ACP_E_UNSUPPORTED, "the standard coupling replaces every link (Coarse-to-Fine fAll, "
-                                           "Fine-to-Coarse cAll, Hanging-Node-Reconstruction hAll)");
+        if (f2c == ACP_LINKS_MISSING_COMMON)
+            return fail(ACP_E_UNSUPPORTED, "Fine-to-Coarse McP belongs to the direct coupling (the standard "
+                                           "coupling takes Mc or cAll)");
+        if (f2c == ACP_LINKS_MISSING && coupled_faces(r) == 2 && r->x1 - r->x0 < 3)
+            return fail(ACP_E_ARG, "Fine-to-Coarse Mc with two coupled faces needs x1 - x0 >= 3 (one grid-coupling "
+                                   "column inside each face)");
     } else if (scheme == ACP_COUPLING_DC) {
         if (c2f == ACP_LINKS_ALL || f2c == ACP_LINKS_ALL)
             return fail(ACP_E_ARG, "direct coupling: Coarse-to-Fine Mf or MfP, Fine-to-Coarse Mc or McP");

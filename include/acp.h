@@ -378,7 +378,15 @@ int acp_refined_set_scaling(acp_refined* r, int32_t style);
  * those of the coarse node completed with the rescaled fine-side links.
  * Hanging fine face nodes average their coincident neighbours' links of
  * `hanging` (hMf ACP_LINKS_MISSING, hMfP ACP_LINKS_MISSING_COMMON, hAll
- * ACP_LINKS_ALL).  Every operator and tier; HRR hands its strain rates as
+ * ACP_LINKS_ALL).  The standard coupling takes link sets too
+ * (BoundaryConditions.txt:73-86): the fine face nodes at coarse positions
+ * take from the coarse node the links of `c2f` (Mf, MfP or fAll) and keep what
+ * the fine step pushed on the others, the hanging face nodes average those of
+ * `hanging`, and with Fine-to-Coarse Mc (ACP_LINKS_MISSING) the coarse
+ * grid-coupling columns x0 + 1 and x1 - 1 take from the fine level only their
+ * links arriving from inside the fine region (cAll: every link; McP belongs
+ * to the direct coupling).  The other restricted coarse nodes take every
+ * link.  Every operator and tier; HRR hands its strain rates as
  * the reference's DirectCoupling function computes them (Node.cpp:1892-2111):
  * the coarse interface column and its fine partners collide with the coarse
  * stencil that reads the missing inner neighbour through the fine level, the

@@ -1391,6 +1391,19 @@ void acpo_refined_destroy(acpo_refined* r) {
 acpo_case* acpo_refined_level(acpo_refined* r, int level) { return level == 0 ? r->c : r->f; }
 double acpo_refined_scale(const acpo_refined* r) { return r->a; }
 
+/* Link sets of the coupling keys (BoundaryConditions.txt:73-86; Coarse-to-Fine,
+ * Fine-to-Coarse, Hanging-Node-Reconstruction): 0 the missing links, those
+ * arriving with e_x = n (n = +1 at a fine face at x0, -1 at x1; -n for the
+ * coarse side), 1 also the common ones (e_x = 0), 2 every link. */
+static int dc_link(int set, int q, int n) { return set == 2 || E[q][0] == n || (set == 1 && E[q][0] == 0); }
+
+/* Standard coupling, coarse to fine (time interpolation at t + 1/2, the
+ * coarse state at t + 1): the fine face nodes at coarse positions take their
+ * coarse node's links of the Coarse-to-Fine set (fAll every link, Mf the
+ * missing ones, MfP also the common ones; the others keep what the fine step
+ * pushed), rescaled; the face nodes between them the mean of their two (mid)
+ * or four (centre) coincident neighbours over the Hanging-Node-Reconstruction
+ * set. */
 static void refined_c2f(acpo_refined* r, int half) {
     acpo_case* c = r->c;
     acpo_case* f = r->f;
@@ -1400,7 +1413,7 @@ static void refined_c2f(acpo_refined* r, int half) {
     const int faces[2][2] = {{r->x0 % cnx, 0}, {r->x1 % cnx, fnx - 1}};
     const int nfaces = f->p.periodic == 4 ? 1 : 2; /* a fine LODI face is not coupled */
     for (int s = 0; s < nfaces; ++s) {
-        const int ci = faces[s][0], fi = faces[s][1];
+        const int ci = faces[s][0], fi = faces[s][1], n = s == 0 ? 1 : -1;
         for (int K = 0; K < cnz; ++K)
             for (int J = 0; J < cny; ++J) {
                 const int64_t lc = ci + (int64_t)cnx * (J + (int64_t)cny * K);
@@ -1409,7 +1422,8 @@ static void refined_c2f(acpo_refined* r, int half) {
                 for (int q = 0; q < 19; ++q)
                     v[q] = half ? 0.5 * (c->B[(size_t)q * CN + lc] + c->A[(size_t)q * CN + lc]) : c->A[(size_t)q * CN + lc];
                 rescale19(r, &c->k, c->p.order, v, 0);
-                for (int q = 0; q < 19; ++q) f->A[(size_t)q * FN + lf] = v[q];
+                for (int q = 0; q < 19; ++q)
+                    if (dc_link(r->c2f, q, n)) f->A[(size_t)q * FN + lf] = v[q];
             }
         for (int k = 0; k < fnz; ++k)
             for (int j = 0; j < fny; ++j) {
@@ -1419,6 +1433,7 @@ static void refined_c2f(acpo_refined* r, int half) {
                 if ((j & 1) && (k & 1)) {
                     const int64_t a = AT(j - 1, k - 1), b = AT(j + 1, k - 1), cc = AT(j - 1, k + 1), d = AT(j + 1, k + 1);
                     for (int q = 0; q < 19; ++q) {
+                        if (!dc_link(r->hang, q, n)) continue;
                         const double* F = f->A + (size_t)q * FN;
                         f->A[(size_t)q * FN + l] = ((F[a] + F[b]) + (F[cc] + F[d])) * 0.25;
                     }
@@ -1426,6 +1441,7 @@ static void refined_c2f(acpo_refined* r, int half) {
                     const int64_t a = (j & 1) ? AT(j - 1, k) : AT(j, k - 1);
                     const int64_t b = (j & 1) ? AT(j + 1, k) : AT(j, k + 1);
                     for (int q = 0; q < 19; ++q) {
+                        if (!dc_link(r->hang, q, n)) continue;
                         const double* F = f->A + (size_t)q * FN;
                         f->A[(size_t)q * FN + l] = (F[a] + F[b]) * 0.5;
                     }
@@ -1524,11 +1540,19 @@ static void tou19(const acpo_refined* r, const acpo_consts* k, int order, const 
     for (int q = 0; q < 19; ++q) v[q] = eq[q] + n[q];
 }
 
+/* Fine to coarse: every coarse node strictly inside the fine region takes
+ * its coincident fine node (filtered by F2C-Filter), rescaled.  Under the
+ * standard coupling with Fine-to-Coarse Mc the coarse grid-coupling columns
+ * (x0 + 1, and x1 - 1 when that face is coupled) take only their missing
+ * links, those arriving from inside the fine region (e_x = -1 at x0 + 1, +1 at
+ * x1 - 1); under cAll, and under the direct coupling, which hands the
+ * interface column its links in the pair exchange, every link. */
 static void refined_f2c(acpo_refined* r) {
     acpo_case* c = r->c;
     acpo_case* f = r->f;
     const int cnx = c->p.nx, cny = c->p.ny, cnz = c->p.nz;
     const int fnx = f->p.nx, fny = f->p.ny;
+    const int partial = r->scheme == 0 && r->f2c != 2, two = f->p.periodic != 4;
 #pragma omp parallel for collapse(2) schedule(static) num_threads(c->p.nthreads > 0 ? c->p.nthreads : 1)
     for (int I = r->x0 + 1; I <= r->x1 - 1; ++I)
         for (int K = 0; K < cnz; ++K)
@@ -1544,7 +1568,9 @@ static void refined_f2c(acpo_refined* r) {
                     for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + lf];
                     rescale19(r, &c->k, c->p.order, v, 1);
                 }
-                for (int q = 0; q < 19; ++q) c->A[(size_t)q * c->N + lc] = v[q];
+                const int n = !partial ? 0 : (I == r->x0 + 1 ? -1 : ((two && I == r->x1 - 1) ? 1 : 0));
+                for (int q = 0; q < 19; ++q)
+                    if (n == 0 || dc_link(r->f2c, q, n)) c->A[(size_t)q * c->N + lc] = v[q];
             }
 }
 
@@ -1574,7 +1600,6 @@ static void refined_f2c(acpo_refined* r) {
  * ones take the mean of their two (mid) or four (centre) coincident
  * neighbours (Hanging-Node-Reconstruction): hMf for the missing links,
  * hMfP also for the common ones, hAll for every link. */
-static int dc_link(int set, int q, int n) { return set == 2 || E[q][0] == n || (set == 1 && E[q][0] == 0); }
 
 /* Mass and momentum (rho, rho u_x, rho u_y, rho u_z) of the links of set
  * `set` arriving from the fine side (e_x = -n; with the common links for set
@@ -1697,7 +1722,10 @@ static void refined_dc(acpo_refined* r, int half) {
 
 int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int hang) {
     if (scheme == 0) {
-        if (c2f != 2 || f2c != 2 || hang != 2) return -1; /* STD replaces every link */
+        /* STD: Mf, MfP or fAll; Mc or cAll; any hanging set.  With two coupled
+         * faces Mc needs two distinct grid-coupling columns */
+        if (c2f < 0 || c2f > 2 || (f2c != 0 && f2c != 2) || hang < 0 || hang > 2) return -1;
+        if (f2c == 0 && r->f->p.periodic != 4 && r->x1 - r->x0 < 3) return -1;
     } else if (scheme == 1) {
         if (c2f < 0 || c2f > 1 || f2c < 0 || f2c > 1 || hang < 0 || hang > 2) return -1;
         if (c2f == 1 && f2c == 1) return -1; /* either MfP-Mc or Mf-McP */
@@ -1731,6 +1759,7 @@ int acpo_refined_set_filter(acpo_refined* r, int filter) {
 }
 
 void acpo_refined_restrict(acpo_refined* r) { refined_f2c(r); }
+void acpo_refined_prolong(acpo_refined* r, int half) { refined_c2f(r, half); }
 
 /* ---- HRR strain-rate transfer between the levels.  The reference hands
  * strain rates across the interface with the partner functions of

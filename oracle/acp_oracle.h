@@ -151,10 +151,11 @@ int acpo_refined_set_scaling(acpo_refined* r, int mrt_style);
 /* the rescaling of a non-equilibrium part n under the current Scaling-Style,
  * coarse to fine (f2c 0) or fine to coarse (f2c 1), in place */
 void acpo_refined_scale_neq(const acpo_refined* r, int f2c, double n[19]);
-/* coupling scheme: 0 standard (STD, default: every link, c2f = f2c = hang =
- * 2), 1 direct coupling (DC); link sets 0 missing (Mf, Mc, hMf), 1 missing and
- * common (MfP, McP, hMfP), 2 all (hAll, hanging nodes only); -1 for an
- * invalid combination (MfP with McP) */
+/* coupling scheme: 0 standard (STD, default c2f = f2c = hang = 2), 1 direct
+ * coupling (DC); link sets 0 missing (Mf, Mc, hMf), 1 missing and common
+ * (MfP, McP, hMfP), 2 all (fAll, cAll, hAll; under DC hAll only); -1 for an
+ * invalid combination (MfP with McP under DC, McP under STD, STD's Mc with two
+ * coupled faces and x1 - x0 < 3) */
 int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int hang);
 /* direct coupling of one coincident pair on the face with outward fine-region
  * normal -n (n = +1 at x0): the chord matrix of the face and the pair's
@@ -173,6 +174,10 @@ void acpo_dc_state(const acpo_consts* k, int order, double a, const double m[6],
 void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_gc, double* s_face);
 /* the fine-to-coarse restriction alone, on the levels' current A arrays */
 void acpo_refined_restrict(acpo_refined* r);
+/* the standard coupling's coarse-to-fine alone (half = 1: the coarse state
+ * interpolated to t + 1/2 from its B and A arrays; 0: its A array) into the
+ * fine level's A array */
+void acpo_refined_prolong(acpo_refined* r, int half);
 #ifdef __cplusplus
 }
 #endif

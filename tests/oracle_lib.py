@@ -100,6 +100,7 @@ def lib():
         L.acpo_refined_step_probes.argtypes = [ctypes.c_void_p, ctypes.c_int, ip, ctypes.c_int, dp]
         L.acpo_refined_set_filter.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.acpo_refined_restrict.argtypes = [ctypes.c_void_p]
+        L.acpo_refined_prolong.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.acpo_refined_set_coupling.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 4
         L.acpo_refined_set_scaling.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.acpo_refined_scale_neq.argtypes = [ctypes.c_void_p, ctypes.c_int, dp]
@@ -252,6 +253,11 @@ class OracleRefined:
     def restrict(self):
         """The fine-to-coarse restriction alone, on the levels' current states."""
         lib().acpo_refined_restrict(self.h)
+
+    def prolong(self, half=0):
+        """The standard coupling's coarse-to-fine alone (half: the coarse state
+        at t + 1/2 from both of its arrays)."""
+        lib().acpo_refined_prolong(self.h, int(half))
 
     def set_scaling(self, style):
         """Scaling-Style of the standard coupling: "SRTSTYLE" or "MRTSTYLE"."""

@@ -140,8 +140,11 @@ def test_forbidden_combinations_are_refused():
     with pytest.raises(ValueError):
         r.set_coupling("DC1", "fAll", "Mc", "hMf")  # fAll with STD only
     with pytest.raises(ValueError):
-        r.set_coupling("STD", "Mf", "cAll", "hAll")  # the standard coupling replaces every link
+        r.set_coupling("STD", "Mf", "McP", "hAll")  # McP with DC only
+    with pytest.raises(ValueError):
+        refined("RR", x1=10)[0].set_coupling("STD", "Mf", "Mc", "hAll")  # one grid-coupling column per face
     r.set_coupling("DC2", "Mf", "McP", "hMfP")
+    r.set_coupling("STD", "MfP", "Mc", "hMf")
 
 
 def _pulse(scheme, links=("Mf", "Mc", "hMf"), op="RR"):

@@ -103,11 +103,15 @@ def test_direct_coupling_refusals(gpu):
         with pytest.raises(acp.AcpError):
             r.set_coupling("DC2", "fAll", "Mc", "hMf")
         with pytest.raises(acp.AcpError):
-            r.set_coupling("STD", "Mf", "cAll", "hAll")
+            r.set_coupling("STD", "Mf", "McP", "hAll")  # McP belongs to the direct coupling
         with pytest.raises(acp.AcpError):
             r.set_coupling("DC3")
         r.set_coupling("DC2", "Mf", "McP", "hMfP")
+        r.set_coupling("STD", "Mf", "Mc", "hMf")
         r.set_coupling("STD")
+    with acp.Refined(d, 8, 10) as r:
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("STD", "Mf", "Mc", "hMf")  # one grid-coupling column inside each face
 
 
 def test_hrr_switches_between_couplings_bitexact(gpu):

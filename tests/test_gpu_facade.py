@@ -121,10 +121,12 @@ def _refined_text(nx, ny, nz, op, region, steps, coupling=None, cubic=False):
 
 
 COUPLINGS = [None, ("DC1", "Mf", "Mc", "hMf", "LAG"), ("DC2", "MfP", "Mc", "hMfP", "NOFI"),
-             ("STD", "fAll", "cAll", "hAll", "TOU"), ("STD", "fAll", "cAll", "hAll", "LAG", "MRTSTYLE")]
+             ("STD", "fAll", "cAll", "hAll", "TOU"), ("STD", "fAll", "cAll", "hAll", "LAG", "MRTSTYLE"),
+             ("STD", "Mf", "Mc", "hMf", "NOFI")]
 
 
-@pytest.mark.parametrize("coupling", COUPLINGS, ids=["STD-NOFI", "DC1-LAG", "DC2-MfP", "STD-TOU", "STD-LAG-MRTSTYLE"])
+@pytest.mark.parametrize("coupling", COUPLINGS, ids=["STD-NOFI", "DC1-LAG", "DC2-MfP", "STD-TOU", "STD-LAG-MRTSTYLE",
+                                                     "STD-Mf-Mc"])
 @pytest.mark.parametrize("op", ["RR", "HRR"])
 def test_two_level_lattice_program_matches_the_engine(gpu, tmp_path, op, coupling):
     """MaxLevel 1 with this engine's refined duct geometry: the reference's
@@ -184,12 +186,12 @@ def _two_level_program(tmp_path, op, coupling, cubic):
 
 @pytest.mark.parametrize("key,value", [("F2C-Filter NOFI", "F2C-Filter XYZ"), ("Scaling-Style SRTSTYLE", "Scaling-Style XYZ"),
                                        ("Coupling-Approach STD", "Coupling-Approach XYZ"),
-                                       ("Coarse-to-Fine fAll", "Coarse-to-Fine Mf")])
+                                       ("Fine-to-Coarse cAll", "Fine-to-Coarse McP")])
 def test_two_level_coupling_keys_the_engine_lacks_end_the_program(gpu, tmp_path, key, value):
     """A two-level case file asking for what the engine does not provide (an
     unknown filter, an unknown scaling style, an unknown scheme, or the
-    standard coupling with only the missing links) ends the program naming the
-    key."""
+    standard coupling with the direct coupling's McP) ends the program naming
+    the key."""
     f = tmp_path / "bc.txt"
     f.write_text(_refined_text(24, 8, 3, "RR", (12, 23), 2).replace(key, value))
     r = _run(f, "nested")

@@ -184,6 +184,78 @@ def test_open_x_hrr_matches_oracle_bitexact(gpu, order):
     assert np.max(np.abs(rho32 - r64)) <= 2e-5 * np.max(np.abs(r64 - RHO0))
 
 
+@pytest.mark.parametrize("op,order,mrt,bnd,x1,links,filt", [
+    ("RR", 3, 0, "PERIODIC", 16, ("Mf", "Mc", "hMf"), "NOFI"),
+    ("HRR", 3, 0, "LODI-X", 17, ("MfP", "Mc", "hMfP"), "LAG"),
+    ("SRT", 3, 0, "LODI-X", 23, ("Mf", "Mc", "hAll"), "TOU"),
+    ("MRT-RM", 3, 1, "PERIODIC", 16, ("fAll", "Mc", "hMf"), "NOFI"),
+    ("RR", 4, 0, "LODI-X", 17, ("MfP", "cAll", "hMf"), "LAG")])
+def test_refined_std_link_sets_match_oracle_bitexact(gpu, op, order, mrt, bnd, x1, links, filt):
+    """The standard coupling with the case file's link sets (Coarse-to-Fine
+    Mf / MfP, Fine-to-Coarse Mc, Hanging-Node-Reconstruction hMf / hMfP): both
+    levels bit for bit against the oracle through a pulse crossing the
+    interface, and the restriction alone over random states."""
+    nx, ny, nz, x0 = 24, 10, 3, 8
+    per = 1 if bnd == "PERIODIC" else 3
+    p = ol.make_params(nx, ny, nz, op=op, order=order, periodic=per, tau=0.6, mrt_mode=mrt, cs=CS, spacing=DX,
+                       rho0=RHO0, transverse=bnd != "PERIODIC")
+    o = ol.OracleRefined(p, x0, x1)
+    o.set_filter(filt)
+    o.set_coupling("STD", *links)
+    amp, cx, cy, b = 1e-3, 12.0, 5.0, 2.5
+    o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+    o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+    d = acp.make_desc(nx, ny, nz, op=op, order=order, tau=0.6, boundary=bnd, transverse=bnd != "PERIODIC",
+                      mrt_mode="RELAXED" if mrt else "REFERENCE")
+    print("case", op, order, bnd, x1, links, filt, flush=True)
+    with acp.Refined(d, x0, x1) as r:
+        r.set_filter(filt)
+        r.set_coupling("STD", *links)
+        r.init_pulse(amp, cx, cy, b)
+        for n in (1, 4, 11):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+        rng = np.random.default_rng(7)
+        W = np.array([1 / 18] * 6 + [1 / 36] * 12 + [1 / 3])
+        fs = [RHO0 * W[:, None] * (1.0 + 1e-3 * rng.standard_normal((19, lv.N))) for lv in o.levels]
+        for lv in (0, 1):
+            o.levels[lv].set_state(fs[lv])
+            r.level(lv).set_populations(fs[lv])
+        o.restrict()
+        r.restrict()
+        assert np.array_equal(r.level(0).populations(), o.levels[0].state())
+
+
+@pytest.mark.parametrize("prec,coupling,tol", [("F32-SHIFTED", "F64", 1e-4), ("F32-SHIFTED", "F32-SHIFTED", 1e-3),
+                                               ("F16-SHIFTED", "F64", 2e-2)])
+def test_refined_std_link_sets_shifted_tiers_follow_float64(gpu, prec, coupling, tol):
+    """The standard coupling with Mf / Mc / hMf and each filter in the shifted
+    tiers: the coarse density after 30 coarse steps within `tol` of the
+    largest density deviation of the float64 oracle's run."""
+    nx, ny, nz, x0, x1 = 24, 10, 3, 8, 17
+    for filt in ("NOFI", "LAG", "TOU"):
+        p = ol.make_params(nx, ny, nz, op="RR", order=3, periodic=3, tau=0.6, cs=CS, spacing=DX, rho0=RHO0)
+        o = ol.OracleRefined(p, x0, x1)
+        o.set_filter(filt)
+        o.set_coupling("STD", "Mf", "Mc", "hMf")
+        amp, cx, cy, b = 1e-3, 12.0, 5.0, 2.5
+        o.levels[0].set_state(ol.pulse_ic(p, amp, cx, cy, b))
+        o.levels[1].set_state(ol.pulse_ic(o.levels[1].p, amp, 2 * (cx - x0), 2 * cy, 2 * b))
+        o.step(30)
+        r64 = o.levels[0].state().sum(axis=0)
+        d = acp.make_desc(nx, ny, nz, op="RR", order=3, tau=0.6, boundary="LODI-X", precision=prec, f16_scale=1000.0)
+        with acp.Refined(d, x0, x1) as r:
+            r.set_coupling_precision(coupling)
+            r.set_filter(filt)
+            r.set_coupling("STD", "Mf", "Mc", "hMf")
+            r.init_pulse(amp, cx, cy, b)
+            r.step(30)
+            rho, _ = r.level(0).macros()
+        assert np.max(np.abs(rho - r64)) <= tol * np.max(np.abs(r64 - RHO0)), filt
+
+
 def _moving_pulse(lp, amp, cx, cy, b, U, scale, shift):
     """The pulse's density (centre and width in coarse nodes, scaled to the
     level's nodes) in a uniform flow U along x, as feq_N."""
