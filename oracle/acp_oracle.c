@@ -699,6 +699,12 @@ struct acpo_case {
     int nfix;
     int fix_col[2];
     const double* fix_S[2];
+    /* or a list of such nodes (the reference's shell geometry): fix_pos[l] is
+     * node l's position in the list (-1 for none), fix_P the planes
+     * [nv][fix_n] over the list */
+    int32_t* fix_pos;
+    const double* fix_P;
+    int64_t fix_n;
 };
 
 static inline int64_t lin(const acpo_case* c, int i, int j, int k) {
@@ -916,7 +922,7 @@ acpo_case* acpo_create(const acpo_params* p) {
 
 void acpo_destroy(acpo_case* c) {
     if (!c) return;
-    free(c->A); free(c->B); free(c->psi); free(c->macro); free(c->mask); free(c->interior);
+    free(c->A); free(c->B); free(c->psi); free(c->macro); free(c->mask); free(c->interior); free(c->fix_pos);
     free(c->l_node); free(c->l_n1); free(c->l_n2);
     for (int t = 0; t < 4; ++t) free(c->l_t[t]);
     free(c->l_dir); free(c->l_corner); free(c->l_phase);
@@ -1076,15 +1082,16 @@ static void one_step(acpo_case* c) {
                             for (int v = 0; v < 4; ++v) nb[m][v] = c->macro[(size_t)v * N + o];
                         }
                     }
-                    int fx = -1;
+                    const double* FP = NULL;
+                    int64_t t = 0, n2 = 0;
                     for (int x = 0; x < c->nfix; ++x)
-                        if (ii == c->fix_col[x]) fx = x;
-                    if (fx >= 0) {
-                        const int64_t t = jj + (int64_t)ny * kk, n2 = (int64_t)ny * nz;
-                        const double S[6] = {c->fix_S[fx][t], c->fix_S[fx][n2 + t], c->fix_S[fx][2 * n2 + t],
-                                             c->fix_S[fx][3 * n2 + t], c->fix_S[fx][4 * n2 + t], c->fix_S[fx][5 * n2 + t]};
+                        if (ii == c->fix_col[x]) FP = c->fix_S[x], t = jj + (int64_t)ny * kk, n2 = (int64_t)ny * nz;
+                    if (c->fix_pos && c->fix_pos[l] >= 0) FP = c->fix_P, t = c->fix_pos[l], n2 = c->fix_n;
+                    if (FP) {
+                        const double S[6] = {FP[t], FP[n2 + t], FP[2 * n2 + t], FP[3 * n2 + t], FP[4 * n2 + t],
+                                             FP[5 * n2 + t]};
                         if (p->cubic) {
-                            for (int q = 0; q < 19; ++q) ps[q] = c->fix_S[fx][(6 + q) * n2 + t];
+                            for (int q = 0; q < 19; ++q) ps[q] = FP[(6 + q) * n2 + t];
                             acpo_collide_handed(&c->k, p, f, S, ps);
                             for (int q = 0; q < 19; ++q) c->psi[(size_t)q * N + l] = ps[q];
                         } else {
@@ -1263,6 +1270,17 @@ struct acpo_refined {
     double* s_col[2];
     double* s_face[2];
     double* s_gc[2];
+    /* the reference's geometry (shell = 1, see acpo_refined_create_shell) */
+    int shell;
+    int hb[6];          /* the fine hole box (fine indices, x0 x1 y0 y1 z0 z1); its surface the fine grid-coupling nodes */
+    int cb[6];          /* the coarse core box (coarse indices); its surface the coarse grid-coupling nodes */
+    int64_t nfs, *fs;   /* the fine grid-coupling nodes (fine node indices, the coincident ones first) */
+    int64_t nfc;        /* ... of which coincident with coarse nodes */
+    uint32_t *fs_c2f, *fs_hang; /* their Coarse-to-Fine and Hanging-Node-Reconstruction link masks */
+    int64_t ncr, *cr;   /* the restricted coarse nodes: the core box's surface (ncs of them), then outside it */
+    int64_t ncs;
+    uint32_t* cr_mask;  /* their Fine-to-Coarse link masks */
+    double *sh_face, *sh_gc; /* handed values [nv][nfs] and [nv][ncs] */
 };
 
 /* Scaling-Style (BoundaryConditions.txt:89, selected at Lattice.cpp:4634-4704
@@ -1383,6 +1401,7 @@ void acpo_refined_destroy(acpo_refined* r) {
         free(r->s_face[s]);
         free(r->s_gc[s]);
     }
+    free(r->fs); free(r->fs_c2f); free(r->fs_hang); free(r->cr); free(r->cr_mask); free(r->sh_face); free(r->sh_gc);
     acpo_destroy(r->c);
     acpo_destroy(r->f);
     free(r);
@@ -1720,7 +1739,19 @@ static void refined_dc(acpo_refined* r, int half) {
     }
 }
 
+static void shell_masks(acpo_refined* r);
+static void shell_c2f(acpo_refined* r, int half);
+static void shell_f2c(acpo_refined* r);
+
 int acpo_refined_set_coupling(acpo_refined* r, int scheme, int c2f, int f2c, int hang) {
+    if (r->shell) { /* the standard coupling (Overlap 1) with its link sets */
+        if (scheme != 0 || c2f < 0 || c2f > 2 || (f2c != 0 && f2c != 2) || hang < 0 || hang > 2) return -1;
+        r->c2f = c2f;
+        r->f2c = f2c;
+        r->hang = hang;
+        shell_masks(r);
+        return 0;
+    }
     if (scheme == 0) {
         /* STD: Mf, MfP or fAll; Mc or cAll; any hanging set.  With two coupled
          * faces Mc needs two distinct grid-coupling columns */
@@ -1758,8 +1789,14 @@ int acpo_refined_set_filter(acpo_refined* r, int filter) {
     return 0;
 }
 
-void acpo_refined_restrict(acpo_refined* r) { refined_f2c(r); }
-void acpo_refined_prolong(acpo_refined* r, int half) { refined_c2f(r, half); }
+void acpo_refined_restrict(acpo_refined* r) {
+    if (r->shell) shell_f2c(r);
+    else refined_f2c(r);
+}
+void acpo_refined_prolong(acpo_refined* r, int half) {
+    if (r->shell) shell_c2f(r, half);
+    else refined_c2f(r, half);
+}
 
 /* ---- HRR strain-rate transfer between the levels.  The reference hands
  * strain rates across the interface with the partner functions of
@@ -1935,7 +1972,316 @@ static void refined_transfer(acpo_refined* r, int half) {
     }
 }
 
+/* ---- the reference's two-level geometry (MaxLevel 1; createLattice,
+ * Lattice.cpp:6-213): a fine shell around a coarse core.  Level 0 holds
+ * grid-coupling nodes on the surface of the core box [RefinementLayerX - 1,
+ * imax - RefinementLayerYZ + 1] x [RefinementLayerYZ - 1, jmax -
+ * RefinementLayerYZ + 1] x [..z..] (Overlap 1, the standard coupling);
+ * level 1 covers the domain at half the spacing with boundary nodes on its
+ * faces and grid-coupling nodes on the surface of the hole box
+ * [2 RefinementLayerX, 2 (imax - RefinementLayerYZ)] x [2 RefinementLayerYZ,
+ * 2 (jmax - RefinementLayerYZ)] x [..z..]; the fine level is the domain
+ * outside the hole, the coarse level the core.
+ * Both levels are stepped over the whole domain here (LODI on every face, the
+ * fine LODI relaxing over the level-0 length).  The hole's fine nodes and the
+ * coarse nodes outside the core box stand for nodes the reference does not
+ * step; they are overwritten wherever the coupling reads them:
+ *   - every fine grid-coupling node takes from the coarse level the links
+ *     arriving from inside the hole (Coarse-to-Fine Mf), also those arriving
+ *     along the hole's surface (MfP), or all (fAll); coincident nodes from
+ *     their coarse node, the others the mean of their two or four coincident
+ *     neighbours along the surface (Hanging-Node-Reconstruction);
+ *   - every coarse node outside the core box, the coarse domain faces
+ *     excepted (they keep their LODI update over the fine LODI faces), takes
+ *     its coincident fine node: the core box's surface the links arriving
+ *     from outside it (Fine-to-Coarse Mc) or all (cAll), the nodes outside it
+ *     all links.
+ * HRR (and RR or HRR with the cubic-Mach terms) hands strain rates and psi
+ * as on the x band: the coincident fine grid-coupling nodes from their
+ * coarse partners (whose neighbours on the core box's surface are read
+ * through their fine partners, Node.cpp:2114-2380), the other fine
+ * grid-coupling nodes the mean of those along the surface, the core box's
+ * surface from its fine partners (Node.cpp:2383-2490).  PARITY UNPINNED like
+ * the rest of the coupling; pinned by invariants. */
+
+static int box_in(const int b[6], int i, int j, int k) {
+    return i >= b[0] && i <= b[1] && j >= b[2] && j <= b[3] && k >= b[4] && k <= b[5];
+}
+static int box_inner(const int b[6], int i, int j, int k) {
+    return i > b[0] && i < b[1] && j > b[2] && j < b[3] && k > b[4] && k < b[5];
+}
+static int box_surface(const int b[6], int i, int j, int k) { return box_in(b, i, j, k) && !box_inner(b, i, j, k); }
+
+static void coords(const acpo_case* c, int64_t l, int* i, int* j, int* k) {
+    *i = (int)(l % c->p.nx);
+    *j = (int)((l / c->p.nx) % c->p.ny);
+    *k = (int)(l / ((int64_t)c->p.nx * c->p.ny));
+}
+
+/* the link masks of the current link sets */
+static void shell_masks(acpo_refined* r) {
+    const acpo_case* f = r->f;
+    const acpo_case* c = r->c;
+    for (int64_t t = 0; t < r->nfs; ++t) {
+        int i, j, k;
+        coords(f, r->fs[t], &i, &j, &k);
+        uint32_t miss = 0, common = 0;
+        for (int q = 0; q < 19; ++q) {
+            const int a = i - E[q][0], b = j - E[q][1], d = k - E[q][2];
+            if (box_inner(r->hb, a, b, d)) miss |= 1u << q;
+            else if (box_surface(r->hb, a, b, d)) common |= 1u << q;
+        }
+        const uint32_t all = (1u << 19) - 1;
+        r->fs_c2f[t] = r->c2f == 2 ? all : (r->c2f == 1 ? (miss | common) : miss);
+        r->fs_hang[t] = r->hang == 2 ? all : (r->hang == 1 ? (miss | common) : miss);
+    }
+    for (int64_t t = 0; t < r->ncr; ++t) {
+        uint32_t m = (1u << 19) - 1;
+        if (t < r->ncs && r->f2c != 2) {
+            int I, J, K;
+            coords(c, r->cr[t], &I, &J, &K);
+            m = 0;
+            for (int q = 0; q < 19; ++q)
+                if (!box_in(r->cb, I - E[q][0], J - E[q][1], K - E[q][2])) m |= 1u << q;
+        }
+        r->cr_mask[t] = m;
+    }
+}
+
+acpo_refined* acpo_refined_create_shell(const acpo_params* coarse, int layer_yz, int layer_x) {
+    const int nx = coarse->nx, ny = coarse->ny, nz = coarse->nz;
+    if (coarse->periodic != 0 || layer_yz < 2 || layer_x < 2) return NULL;
+    const int hc[6] = {layer_x, nx - 1 - layer_yz, layer_yz, ny - 1 - layer_yz, layer_yz, nz - 1 - layer_yz};
+    if (hc[1] - hc[0] < 2 || hc[3] - hc[2] < 2 || hc[5] - hc[4] < 2) return NULL;
+    if (coarse->cubic && coarse->op != ACPO_RR && coarse->op != ACPO_HRR) return NULL;
+    const int handed = coarse->op == ACPO_HRR || (coarse->op == ACPO_RR && coarse->cubic);
+    acpo_params fp = *coarse;
+    fp.nx = 2 * nx - 1;
+    fp.ny = 2 * ny - 1;
+    fp.nz = 2 * nz - 1;
+    fp.spacing = coarse->spacing / 2.;
+    acpo_refined* r = (acpo_refined*)calloc(1, sizeof(acpo_refined));
+    r->c = acpo_create(coarse);
+    r->f = acpo_create(&fp);
+    r->f->k.dom_len[0] = nx * coarse->spacing;
+    r->f->k.dom_len[1] = ny * coarse->spacing;
+    r->f->k.dom_len[2] = nz * coarse->spacing;
+    r->shell = 1;
+    r->c2f = r->f2c = r->hang = 2;
+    r->a = (1. / r->f->k.omega) / (2. * (1. / r->c->k.omega));
+    acpo_refined_set_scaling(r, 0);
+    r->nv = coarse->cubic ? 25 : 6;
+    for (int d = 0; d < 6; ++d) r->hb[d] = 2 * hc[d];
+    const int cb[6] = {hc[0] - 1, hc[1] + 1, hc[2] - 1, hc[3] + 1, hc[4] - 1, hc[5] + 1};
+    memcpy(r->cb, cb, sizeof(cb));
+    /* the fine grid-coupling nodes, coincident ones first, each group in node order */
+    const acpo_case* f = r->f;
+    int64_t n = 0;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int64_t l = 0; l < f->N; ++l) {
+            int i, j, k;
+            coords(f, l, &i, &j, &k);
+            if (!box_surface(r->hb, i, j, k)) continue;
+            const int co = !(i & 1) && !(j & 1) && !(k & 1);
+            if (co != (pass == 0)) continue;
+            if (r->fs) r->fs[n] = l;
+            ++n;
+        }
+    r->nfs = n;
+    r->fs = (int64_t*)malloc(sizeof(int64_t) * n);
+    n = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t l = 0; l < f->N; ++l) {
+            int i, j, k;
+            coords(f, l, &i, &j, &k);
+            if (!box_surface(r->hb, i, j, k)) continue;
+            const int co = !(i & 1) && !(j & 1) && !(k & 1);
+            if (co != (pass == 0)) continue;
+            r->fs[n++] = l;
+        }
+        if (pass == 0) r->nfc = n;
+    }
+    /* the restricted coarse nodes: the core box's surface, then the nodes
+     * outside it off the domain faces */
+    const acpo_case* c = r->c;
+    r->cr = (int64_t*)malloc(sizeof(int64_t) * c->N);
+    n = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t l = 0; l < c->N; ++l) {
+            int I, J, K;
+            coords(c, l, &I, &J, &K);
+            const int face = I == 0 || I == nx - 1 || J == 0 || J == ny - 1 || K == 0 || K == nz - 1;
+            if (pass == 0 ? box_surface(r->cb, I, J, K) : (!box_in(r->cb, I, J, K) && !face)) r->cr[n++] = l;
+        }
+        if (pass == 0) r->ncs = n;
+    }
+    r->ncr = n;
+    r->fs_c2f = (uint32_t*)malloc(sizeof(uint32_t) * r->nfs);
+    r->fs_hang = (uint32_t*)malloc(sizeof(uint32_t) * r->nfs);
+    r->cr_mask = (uint32_t*)malloc(sizeof(uint32_t) * (r->ncr ? r->ncr : 1));
+    shell_masks(r);
+    if (handed) {
+        r->sh_face = (double*)calloc((size_t)r->nv * r->nfs, sizeof(double));
+        r->sh_gc = (double*)calloc((size_t)r->nv * r->ncs, sizeof(double));
+        r->f->fix_pos = (int32_t*)malloc(sizeof(int32_t) * f->N);
+        for (int64_t l = 0; l < f->N; ++l) r->f->fix_pos[l] = -1;
+        for (int64_t t = 0; t < r->nfs; ++t) r->f->fix_pos[r->fs[t]] = (int32_t)t;
+        r->f->fix_P = r->sh_face;
+        r->f->fix_n = r->nfs;
+        r->c->fix_pos = (int32_t*)malloc(sizeof(int32_t) * c->N);
+        for (int64_t l = 0; l < c->N; ++l) r->c->fix_pos[l] = -1;
+        for (int64_t t = 0; t < r->ncs; ++t) r->c->fix_pos[r->cr[t]] = (int32_t)t;
+        r->c->fix_P = r->sh_gc;
+        r->c->fix_n = r->ncs;
+    }
+    return r;
+}
+
+/* the neighbours a non-coincident fine grid-coupling node averages: its two
+ * (one odd index) or four (two odd indices) coincident neighbours along the
+ * surface, in the order (-,-), (+,-), (-,+), (+,+) of its odd axes */
+static int shell_between(const acpo_case* f, int64_t l, int64_t nb[4]) {
+    int x[3];
+    coords(f, l, &x[0], &x[1], &x[2]);
+    int odd[2], no = 0;
+    for (int d = 0; d < 3; ++d)
+        if (x[d] & 1) odd[no++] = d;
+    const int64_t st[3] = {1, f->p.nx, (int64_t)f->p.nx * f->p.ny};
+    if (no == 1) {
+        nb[0] = l - st[odd[0]];
+        nb[1] = l + st[odd[0]];
+        return 2;
+    }
+    nb[0] = l - st[odd[0]] - st[odd[1]];
+    nb[1] = l + st[odd[0]] - st[odd[1]];
+    nb[2] = l - st[odd[0]] + st[odd[1]];
+    nb[3] = l + st[odd[0]] + st[odd[1]];
+    return 4;
+}
+
+static void shell_c2f(acpo_refined* r, int half) {
+    acpo_case* c = r->c;
+    acpo_case* f = r->f;
+    const int64_t CN = c->N, FN = f->N;
+    for (int64_t t = 0; t < r->nfc; ++t) {
+        int i, j, k;
+        coords(f, r->fs[t], &i, &j, &k);
+        const int64_t lc = lin(c, i / 2, j / 2, k / 2);
+        double v[19];
+        for (int q = 0; q < 19; ++q)
+            v[q] = half ? 0.5 * (c->B[(size_t)q * CN + lc] + c->A[(size_t)q * CN + lc]) : c->A[(size_t)q * CN + lc];
+        rescale19(r, &c->k, c->p.order, v, 0);
+        for (int q = 0; q < 19; ++q)
+            if ((r->fs_c2f[t] >> q) & 1u) f->A[(size_t)q * FN + r->fs[t]] = v[q];
+    }
+    for (int64_t t = r->nfc; t < r->nfs; ++t) {
+        int64_t nb[4];
+        const int n = shell_between(f, r->fs[t], nb);
+        for (int q = 0; q < 19; ++q) {
+            if (!((r->fs_hang[t] >> q) & 1u)) continue;
+            const double* F = f->A + (size_t)q * FN;
+            f->A[(size_t)q * FN + r->fs[t]] = n == 4 ? ((F[nb[0]] + F[nb[1]]) + (F[nb[2]] + F[nb[3]])) * 0.25
+                                                     : (F[nb[0]] + F[nb[1]]) * 0.5;
+        }
+    }
+}
+
+static void shell_f2c(acpo_refined* r) {
+    acpo_case* c = r->c;
+    acpo_case* f = r->f;
+#pragma omp parallel for schedule(static) num_threads(c->p.nthreads > 0 ? c->p.nthreads : 1)
+    for (int64_t t = 0; t < r->ncr; ++t) {
+        int I, J, K;
+        coords(c, r->cr[t], &I, &J, &K);
+        double v[19];
+        if (r->filter == 1) {
+            lag19(r, &c->k, c->p.order, f, 2 * I, 2 * J, 2 * K, v);
+        } else if (r->filter == 2) {
+            tou19(r, &c->k, c->p.order, f, 2 * I, 2 * J, 2 * K, v);
+        } else {
+            const int64_t lf = lin(f, 2 * I, 2 * J, 2 * K);
+            for (int q = 0; q < 19; ++q) v[q] = f->A[(size_t)q * f->N + lf];
+            rescale19(r, &c->k, c->p.order, v, 1);
+        }
+        for (int q = 0; q < 19; ++q)
+            if ((r->cr_mask[t] >> q) & 1u) c->A[(size_t)q * c->N + r->cr[t]] = v[q];
+    }
+}
+
+/* strain rates (and psi) handed at a step's start (half = 0: coarse partners
+ * of the coincident fine grid-coupling nodes from the coarse state at t, the
+ * core box's surface from its fine partners) or at t + 1/2 (half = 1: the
+ * coarse partners from the coarse states at t and t + 1 and the fine state at
+ * t + 1/2); the other fine grid-coupling nodes take the mean along the
+ * surface */
+static void shell_transfer(acpo_refined* r, int half) {
+    if (!r->sh_face) return;
+    acpo_case* c = r->c;
+    acpo_case* f = r->f;
+    const double* Cprev = half ? c->B : c->A;
+    const double* Ccur = c->A;
+    const double* F = f->A;
+    for (int64_t t = 0; t < r->nfc; ++t) {
+        int i, j, k;
+        coords(f, r->fs[t], &i, &j, &k);
+        const int P[3] = {i / 2, j / 2, k / 2};
+        double nb[6][19], pre[6][19], oth[6][19], S[6], psi[19];
+        int gc[6];
+        for (int m = 0; m < 6; ++m) {
+            const int a = P[0] + E[m][0], b = P[1] + E[m][1], d = P[2] + E[m][2];
+            gc[m] = box_surface(r->cb, a, b, d);
+            gather19(c, Ccur, a, b, d, nb[m]);
+            gather19(c, Cprev, a, b, d, pre[m]);
+            if (gc[m]) gather19(f, F, 2 * a, 2 * b, 2 * d, oth[m]);
+            else memcpy(oth[m], nb[m], sizeof(oth[m]));
+        }
+        if (half) acpo_strain_half_coarse_partner(&c->k, c->p.spacing, gc, (const double(*)[19])pre,
+                                                  (const double(*)[19])nb, (const double(*)[19])oth, S, psi);
+        else acpo_strain_coarse_partner(&c->k, c->p.spacing, gc, (const double(*)[19])nb, (const double(*)[19])oth,
+                                        S, psi);
+        store_handed(r, r->sh_face, r->nfs, t, S, psi);
+    }
+    for (int64_t t = r->nfc; t < r->nfs; ++t) {
+        int64_t nb[4];
+        const int n = shell_between(f, r->fs[t], nb);
+        int32_t pos[4];
+        for (int e = 0; e < n; ++e) pos[e] = f->fix_pos[nb[e]];
+        for (int v = 0; v < r->nv; ++v) {
+            const double* P = r->sh_face + (size_t)v * r->nfs;
+            r->sh_face[(size_t)v * r->nfs + t] =
+                n == 4 ? ((P[pos[0]] + P[pos[1]]) + (P[pos[2]] + P[pos[3]])) * 0.25 : (P[pos[0]] + P[pos[1]]) * 0.5;
+        }
+    }
+    if (half) return;
+    for (int64_t t = 0; t < r->ncs; ++t) {
+        int I, J, K;
+        coords(c, r->cr[t], &I, &J, &K);
+        double nb[6][19], S[6], psi[19];
+        for (int m = 0; m < 6; ++m) gather19(f, F, 2 * I + E[m][0], 2 * J + E[m][1], 2 * K + E[m][2], nb[m]);
+        acpo_strain_fine_partner(&f->k, f->p.spacing, (const double(*)[19])nb, S, psi);
+        store_handed(r, r->sh_gc, r->ncs, t, S, psi);
+    }
+}
+
+int64_t acpo_refined_shell_lists(const acpo_refined* r, int64_t* fs, int64_t* cr, uint32_t* fs_c2f, uint32_t* fs_hang,
+                                 uint32_t* cr_mask, int64_t counts[4]) {
+    if (!r->shell) return -1;
+    counts[0] = r->nfs, counts[1] = r->nfc, counts[2] = r->ncr, counts[3] = r->ncs;
+    if (fs) memcpy(fs, r->fs, sizeof(int64_t) * r->nfs);
+    if (cr) memcpy(cr, r->cr, sizeof(int64_t) * r->ncr);
+    if (fs_c2f) memcpy(fs_c2f, r->fs_c2f, sizeof(uint32_t) * r->nfs);
+    if (fs_hang) memcpy(fs_hang, r->fs_hang, sizeof(uint32_t) * r->nfs);
+    if (cr_mask) memcpy(cr_mask, r->cr_mask, sizeof(uint32_t) * r->ncr);
+    return 0;
+}
+
 void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_gc, double* s_face) {
+    if (r->shell) {
+        shell_transfer(r, half);
+        if (r->sh_face && s_face) memcpy(s_face, r->sh_face, sizeof(double) * r->nv * r->nfs);
+        if (r->sh_gc && s_gc) memcpy(s_gc, r->sh_gc, sizeof(double) * r->nv * r->ncs);
+        return;
+    }
     refined_transfer(r, half);
     const int nf = r->c->nfix;
     const int64_t cp = (int64_t)r->c->p.ny * r->c->p.nz, fp = (int64_t)r->f->p.ny * r->f->p.nz;
@@ -1953,6 +2299,22 @@ void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_g
  * own step, before the coupling writes into it. */
 void acpo_refined_step_probes(acpo_refined* r, int n, const int64_t* probe_idx, int nprobe, double* probe_out) {
     for (int s = 0; s < n; ++s) {
+        if (r->shell) {
+            shell_transfer(r, 0);
+            one_step(r->c);
+            for (int i = 0; i < nprobe; ++i) {
+                double v = 0.;
+                for (int q = 0; q < 19; ++q) v += r->c->A[(size_t)q * r->c->N + probe_idx[i]];
+                probe_out[(size_t)s * nprobe + i] = v;
+            }
+            one_step(r->f);
+            shell_transfer(r, 1);
+            shell_c2f(r, 1);
+            one_step(r->f);
+            shell_c2f(r, 0);
+            shell_f2c(r);
+            continue;
+        }
         refined_transfer(r, 0);
         one_step(r->c);
         for (int i = 0; i < nprobe; ++i) {

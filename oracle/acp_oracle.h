@@ -172,6 +172,20 @@ void acpo_dc_state(const acpo_consts* k, int order, double a, const double m[6],
  * [faces][nv][fny fnz], nv = 6 (S) or 25 (S, then psi[19]) with the
  * cubic-Mach terms */
 void acpo_refined_transfer(acpo_refined* r, int half, double* s_col, double* s_gc, double* s_face);
+/* The reference's two-level geometry (MaxLevel 1, Lattice.cpp:6-213): the
+ * coarse level `coarse` (LODI on every face) is the core, the fine level
+ * (2 nx - 1, 2 ny - 1, 2 nz - 1 nodes, LODI faces) the shell outside the hole
+ * box [2 layer_x, 2 (nx - 1 - layer_yz)] x [2 layer_yz, 2 (ny - 1 - layer_yz)]
+ * x [2 layer_yz, 2 (nz - 1 - layer_yz)] (RefinementLayerX, RefinementLayerYZ);
+ * the standard coupling with Overlap 1.  NULL for other boundaries, layers
+ * below 2 or a hole without inner nodes. */
+acpo_refined* acpo_refined_create_shell(const acpo_params* coarse, int layer_yz, int layer_x);
+/* its node lists: the fine grid-coupling nodes (coincident ones first), the
+ * restricted coarse nodes (the core box's surface first) and their link
+ * masks; counts = {fine grid-coupling nodes, of which coincident, restricted
+ * coarse nodes, of which on the core box's surface}; -1 for the x band */
+int64_t acpo_refined_shell_lists(const acpo_refined* r, int64_t* fs, int64_t* cr, uint32_t* fs_c2f,
+                                 uint32_t* fs_hang, uint32_t* cr_mask, int64_t counts[4]);
 /* the fine-to-coarse restriction alone, on the levels' current A arrays */
 void acpo_refined_restrict(acpo_refined* r);
 /* the standard coupling's coarse-to-fine alone (half = 1: the coarse state

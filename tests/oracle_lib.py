@@ -91,6 +91,11 @@ def lib():
         L.acpo_pulse_ic.argtypes = [P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
         L.acpo_refined_create.argtypes = [P, ctypes.c_int, ctypes.c_int]
         L.acpo_refined_create.restype = ctypes.c_void_p
+        L.acpo_refined_create_shell.argtypes = [P, ctypes.c_int, ctypes.c_int]
+        L.acpo_refined_create_shell.restype = ctypes.c_void_p
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.acpo_refined_shell_lists.argtypes = [ctypes.c_void_p, ip, ip, u32p, u32p, u32p, ip]
+        L.acpo_refined_shell_lists.restype = ctypes.c_int64
         L.acpo_refined_destroy.argtypes = [ctypes.c_void_p]
         L.acpo_refined_level.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.acpo_refined_level.restype = ctypes.c_void_p
@@ -204,7 +209,20 @@ class OracleRefined:
     """Two-level oracle run (coarse p periodic or with LODI x faces, fine over
     coarse x in [x0, x1])."""
 
-    def __init__(self, p, x0, x1):
+    def __init__(self, p, x0, x1, shell=None):
+        """shell = (layer_yz, layer_x): the reference's geometry, a fine shell
+        around a coarse core (p with LODI faces); x0, x1 are then unused."""
+        if shell is not None:
+            self.h = lib().acpo_refined_create_shell(ctypes.byref(p), shell[0], shell[1])
+            if not self.h:
+                raise ValueError("oracle rejected the shell case")
+            fp = Params.from_buffer_copy(p)
+            fp.nx, fp.ny, fp.nz = 2 * p.nx - 1, 2 * p.ny - 1, 2 * p.nz - 1
+            fp.spacing = p.spacing / 2.0
+            self.levels = [_Level(lib().acpo_refined_level(self.h, 0), p),
+                           _Level(lib().acpo_refined_level(self.h, 1), fp)]
+            self.x0 = self.x1 = None
+            return
         self.h = lib().acpo_refined_create(ctypes.byref(p), x0, x1)
         if not self.h:
             raise ValueError("oracle rejected the refined case")
@@ -253,6 +271,21 @@ class OracleRefined:
     def restrict(self):
         """The fine-to-coarse restriction alone, on the levels' current states."""
         lib().acpo_refined_restrict(self.h)
+
+    def shell_lists(self):
+        """The shell geometry's node lists: fine grid-coupling nodes (coincident
+        first) with their Coarse-to-Fine and hanging masks, the restricted
+        coarse nodes (core-box surface first) with their masks, and the counts
+        (fine, coincident, coarse, core-box surface)."""
+        counts = np.zeros(4, dtype=np.int64)
+        ip = ctypes.POINTER(ctypes.c_int64)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        lib().acpo_refined_shell_lists(self.h, None, None, None, None, None, counts.ctypes.data_as(ip))
+        fs, cr = np.zeros(counts[0], np.int64), np.zeros(counts[2], np.int64)
+        m1, m2, m3 = np.zeros(counts[0], np.uint32), np.zeros(counts[0], np.uint32), np.zeros(counts[2], np.uint32)
+        lib().acpo_refined_shell_lists(self.h, fs.ctypes.data_as(ip), cr.ctypes.data_as(ip), m1.ctypes.data_as(u32p),
+                                       m2.ctypes.data_as(u32p), m3.ctypes.data_as(u32p), counts.ctypes.data_as(ip))
+        return {"fs": fs, "fs_c2f": m1, "fs_hang": m2, "cr": cr, "cr_mask": m3, "counts": counts}
 
     def prolong(self, half=0):
         """The standard coupling's coarse-to-fine alone (half: the coarse state
