@@ -205,9 +205,25 @@ class Refined:
     face and only the face at x0 is coupled.  level(0) / level(1) give the
     levels' cases for probes, read-back and output; step with Refined.step."""
 
-    def __init__(self, desc, x0, x1, stream=None):
+    def __init__(self, desc, x0=0, x1=0, stream=None, shell=None):
+        """shell = (layer_yz, layer_x): the reference's geometry, a fine shell
+        around a coarse core (acp_refined_create_shell; `desc` with LODI faces,
+        x0 and x1 unused)."""
         self.h = ctypes.c_void_p()
         st = ctypes.c_void_p(int(stream)) if stream else None
+        if shell is not None:
+            check(lib().acp_refined_create_shell(ctypes.byref(desc), int(shell[0]), int(shell[1]), st,
+                                                 ctypes.byref(self.h)))
+            fd = CaseDesc.from_buffer_copy(desc)
+            fd.nx, fd.ny, fd.nz = 2 * desc.nx - 1, 2 * desc.ny - 1, 2 * desc.nz - 1
+            fd.spacing = desc.spacing / 2.0
+            self.x0 = self.x1 = 0
+            self._levels = []
+            for lv, d in ((0, desc), (1, fd)):
+                hc = ctypes.c_void_p()
+                check(lib().acp_refined_level(self.h, lv, ctypes.byref(hc)))
+                self._levels.append(_LevelCase(hc, d))
+            return
         check(lib().acp_refined_create(ctypes.byref(desc), int(x0), int(x1), st, ctypes.byref(self.h)))
         fd = CaseDesc.from_buffer_copy(desc)
         fd.nx, fd.ny, fd.nz = 2 * (x1 - x0) + 1, 2 * desc.ny, 2 * desc.nz

@@ -139,6 +139,7 @@ def lib():
         "acp_case_checksum": (ctypes.c_int, [vp, ctypes.POINTER(ctypes.c_uint64)]),
         "acp_case_set_bulk_variant": (ctypes.c_int, [vp, ctypes.c_int32]),
         "acp_refined_create": (ctypes.c_int, [D, i32, i32, vp, ctypes.POINTER(vp)]),
+        "acp_refined_create_shell": (ctypes.c_int, [D, i32, i32, vp, ctypes.POINTER(vp)]),
         "acp_refined_destroy": (None, [vp]),
         "acp_refined_level": (ctypes.c_int, [vp, i32, ctypes.POINTER(vp)]),
         "acp_refined_init_pulse": (ctypes.c_int, [vp, dbl, dbl, dbl, dbl]),

@@ -801,6 +801,9 @@ int step_eager(acp_case* c, int32_t n) {
             for (int x = 0; x < c->nfix && e == hipSuccess; ++x)
                 e = acp::launch_fix64(c->bulk_op, c->d.order, c->d.cubic != 0, c->dk, c->A, c->B, c->psi, c->g,
                                       c->fix_col[x], c->fix_S[x], c->stream);
+            if (e == hipSuccess && c->fix_list_n > 0)
+                e = acp::launch_fix64_list(c->bulk_op, c->d.order, c->d.cubic != 0, c->dk, c->A, c->B, c->psi, c->g,
+                                           c->fix_list, c->fix_list_n, c->fix_list_P, c->stream);
             for (int p = 0; p < nph && e == hipSuccess; ++p)
                 e = acp::launch_lodi64(c->d.order, c->d.cubic != 0, c->d.transverse != 0, c->dk, c->A, c->B, c->psi,
                                        c->lodi + c->phase_start[p], c->phase_start[p + 1] - c->phase_start[p], c->g.L,

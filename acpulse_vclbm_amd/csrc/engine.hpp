@@ -54,6 +54,10 @@ int init_reference_level(acp_case* c, int level, int i0, const acp_case_desc& d0
 // with the cubic-Mach terms.
 hipError_t launch_fix64(int op, int order, bool cubic, const K64* k, const double* A, double* B, double* psi,
                         const Grid& g, int col, const double* S, hipStream_t s);
+// The same for a list of nodes (none on a face of the grid), P planes
+// [nv][n] over the list.
+hipError_t launch_fix64_list(int op, int order, bool cubic, const K64* k, const double* A, double* B, double* psi,
+                             const Grid& g, const int64_t* list, int64_t n, const double* P, hipStream_t s);
 hipError_t launch_fix_hrr_shift(int prec, int order, const KShift* k, const void* A, void* B, const Grid& g, int col,
                                 const double* S, double dt, hipStream_t s);
 
@@ -130,6 +134,12 @@ struct acp_case {
     int32_t nfix = 0;
     int32_t fix_col[2] = {0, 0};
     const double* fix_S[2] = {nullptr, nullptr};
+    // or a list of such nodes (the shell geometry of a two-level case, float64):
+    // fix_list [fix_list_n] node indices off the grid's faces, fix_list_P
+    // planes [nv][fix_list_n] over the list
+    const int64_t* fix_list = nullptr;
+    int64_t fix_list_n = 0;
+    const double* fix_list_P = nullptr;
     std::vector<void*> extra;  // buffers of case_alloc_extra (freed with the case)
     // the pack that steps the case (acp_pack_add .. acp_pack_remove): its
     // arrays, step and probe counters are the pack's until then

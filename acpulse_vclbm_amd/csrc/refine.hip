@@ -237,23 +237,34 @@ constexpr int kLagNodes = 13;  // coarse nodes per 256-thread workgroup (247 lan
 // second barrier every lane sums the filtered part's mass M and momentum J
 // over q = 0..18 and clears them, nbar_q - w_q (M + xi_q . J / cs^2), so that
 // the coarse node keeps its coincident fine node's density and velocity.
-template <int ORDER, bool TOU>
+// LIST: the coarse nodes list[t] (the shell geometry's restricted nodes) with
+// their masks lmask[t] in place of the columns x0 + 1 .. x1 - 1.
+template <int ORDER, bool TOU, bool LIST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ? 4 : 3))) void f2c_lag(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
                                                int fny, int fnz, const NeqScale ainv, double* __restrict__ C, int cnx,
                                                int cny, int cnz, int x0, int width, const PopLayout CN,
-                                               const PopLayout FN, const StdLinks lk) {
+                                               const PopLayout FN, const StdLinks lk,
+                                               const int64_t* __restrict__ list, const uint32_t* __restrict__ lmask,
+                                               int64_t nlist) {
     __shared__ double neq[kLagNodes][19][19];  // [node][fine node s][direction]
     __shared__ double eq0[kLagNodes][19];      // the coincident node's equilibrium
     const K64& k = *kp;
     const int node = (int)threadIdx.x / 19, s = (int)threadIdx.x % 19;
     const int64_t t = (int64_t)blockIdx.x * kLagNodes + node;
-    const bool active = node < kLagNodes && t < (int64_t)width * cny * cnz;
-    int di = 0, J = 0, K = 0;
+    const bool active = node < kLagNodes && t < (LIST ? nlist : (int64_t)width * cny * cnz);
+    int di = 0, J = 0, K = 0, Il = 0;
     if (active) {
-        di = (int)(t % width);
-        const int64_t r = t / width;
-        J = (int)(r % cny);
-        K = (int)(r / cny);
+        if constexpr (LIST) {
+            const int64_t lc = list[t];
+            Il = (int)(lc % cnx);
+            J = (int)((lc / cnx) % cny);
+            K = (int)(lc / ((int64_t)cnx * cny));
+        } else {
+            di = (int)(t % width);
+            const int64_t r = t / width;
+            J = (int)(r % cny);
+            K = (int)(r / cny);
+        }
         int dx = 0, dy = 0, dz = 0;
         static_for<0, 18>([&](auto jc) {
             constexpr int j = decltype(jc)::value;
@@ -263,7 +274,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
                 dz = EZ[j];
             }
         });
-        const int fx = 2 * (1 + di), fy = 2 * J, fz = 2 * K;
+        const int fx = LIST ? 2 * Il : 2 * (1 + di), fy = 2 * J, fz = 2 * K;
         const int y = (fy + dy + fny) % fny, z = (fz + dz + fnz) % fnz;
         const int64_t l = fx + dx + (int64_t)fnx * (y + (int64_t)fny * z);
         double g[19], e[19], rho, ux, uy, uz;
@@ -280,9 +291,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ORDER < 4 ?
     }
     __syncthreads();
     const int q = s;
-    const int I = (x0 + 1 + di) % cnx;
+    const int I = LIST ? Il : (x0 + 1 + di) % cnx;
     const int64_t lc = I + (int64_t)cnx * (J + (int64_t)cny * K);
-    const bool take = (gc_mask(lk, I) >> q) & 1u;  // Fine-to-Coarse link set of this column
+    // Fine-to-Coarse link set of this column (or listed node)
+    const bool take = ((LIST ? (active ? lmask[t] : 0u) : gc_mask(lk, I)) >> q) & 1u;
     double nq = 0.;  // the non-equilibrium part lane q hands to population q
     if constexpr (TOU) {
         __shared__ double nbar[kLagNodes][19];
@@ -1193,6 +1205,231 @@ __global__ __launch_bounds__(64) void fix64(const K64* __restrict__ kp, const do
     B[pidx(L, 18, l)] = f[18];
 }
 
+// fix64 over a list of nodes (list[t], none on a face of the grid, so every
+// push stays inside it), handed values P [nv][n] over the list.
+template <int ORDER, int KIND>
+__global__ __launch_bounds__(64) void fix64_list(const K64* __restrict__ kp, const double* __restrict__ A,
+                                                 double* __restrict__ B, double* __restrict__ psi, int nx, int ny,
+                                                 const PopLayout L, const int64_t* __restrict__ list, int64_t n,
+                                                 const double* __restrict__ P) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t l = list[t];
+    const int64_t sxy = (int64_t)nx * ny;
+    double f[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = A[pidx(L, q, l)];
+    double s6[6];
+#pragma unroll
+    for (int v = 0; v < 6; ++v) s6[v] = P[v * n + t];
+    if constexpr (KIND == 0) {
+        collide_hrr64_strain<ORDER>(*kp, f, s6);
+    } else {
+        double ps[19];
+#pragma unroll
+        for (int q = 0; q < 19; ++q) ps[q] = P[(6 + q) * n + t];
+        collide_rr64_handed<ORDER, KIND == 1>(*kp, f, ps, s6);
+#pragma unroll
+        for (int q = 0; q < 19; ++q) psi[q * L.N + l] = ps[q];
+    }
+    static_for<0, 18>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        B[pidx(L, q, l + EX[q] + (int64_t)nx * EY[q] + sxy * EZ[q])] = f[q];
+    });
+    B[pidx(L, 18, l)] = f[18];
+}
+
+hipError_t launch_fix64_list(int op, int order, bool cubic, const K64* k, const double* A, double* B, double* psi,
+                             const Grid& g, const int64_t* list, int64_t n, const double* P, hipStream_t s) {
+    const dim3 grid((unsigned)((n + 63) / 64)), block(64);
+    const int kind = !cubic ? 0 : (op == OP_HRR ? 1 : 2);
+    if (kind == 0 && op != OP_HRR) return hipErrorInvalidValue;
+#define FIXL(O, KD) hipLaunchKernelGGL((fix64_list<O, KD>), grid, block, 0, s, k, A, B, psi, g.nx, g.ny, g.L, list, n, P)
+#define FIXLK(O) \
+    if (kind == 0) FIXL(O, 0); \
+    else if (kind == 1) FIXL(O, 1); \
+    else FIXL(O, 2)
+    if (order == 2) { FIXLK(2); }
+    else if (order == 3) { FIXLK(3); }
+    else { FIXLK(4); }
+#undef FIXLK
+#undef FIXL
+    return hipGetLastError();
+}
+
+namespace refine {
+
+// ---- the reference's two-level geometry (MaxLevel 1; createLattice,
+// Lattice.cpp:6-213): a fine shell around a coarse core.  The fine level
+// covers the domain (2 nx - 1 nodes per row), the coarse level too; the fine
+// grid-coupling nodes are the surface of the hole box hb, the coarse
+// grid-coupling nodes the surface of the core box cb, one coarse cell outside
+// it (Overlap 1).  The kernels run over node lists built on the host: the fine
+// grid-coupling nodes (coincident ones first) with their Coarse-to-Fine and
+// hanging link masks, the restricted coarse nodes (the core box's surface
+// first) with their Fine-to-Coarse masks.
+struct Box6 {
+    int b[6];
+};
+__device__ __forceinline__ bool box_surface(const Box6& B, int i, int j, int k) {
+    const bool in = i >= B.b[0] && i <= B.b[1] && j >= B.b[2] && j <= B.b[3] && k >= B.b[4] && k <= B.b[5];
+    const bool inner = i > B.b[0] && i < B.b[1] && j > B.b[2] && j < B.b[3] && k > B.b[4] && k < B.b[5];
+    return in && !inner;
+}
+
+// coincident fine grid-coupling nodes from their coarse node (time
+// interpolation, rescaling), the links of their Coarse-to-Fine mask
+template <int ORDER>
+__global__ __launch_bounds__(64) void shell_c2f_co(const K64* __restrict__ kp, const double* __restrict__ Cprev,
+                                                   const double* __restrict__ Ccur, int half, const NeqScale a, int cnx,
+                                                   int cny, const PopLayout CL, double* __restrict__ F, int fnx,
+                                                   int fny, const PopLayout FL, const int64_t* __restrict__ fs,
+                                                   const uint32_t* __restrict__ mask, int64_t nfc) {
+    __shared__ K64 sk;
+    const K64& k = stage_k64(kp, sk);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfc) return;
+    const int64_t l = fs[t];
+    const int i = (int)(l % fnx), j = (int)((l / fnx) % fny), kk = (int)(l / ((int64_t)fnx * fny));
+    const int64_t lc = i / 2 + (int64_t)cnx * (j / 2 + (int64_t)cny * (kk / 2));
+    double f[19];
+    c2f_value<ORDER>(k, Cprev, Ccur, CL, lc, half, a, f);
+    const uint32_t m = mask[t];
+#pragma unroll
+    for (int q = 0; q < 19; ++q)
+        if ((m >> q) & 1u) F[pidx(FL, q, l)] = f[q];
+}
+
+// the other fine grid-coupling nodes: the mean of their two (one odd index)
+// or four (two odd indices) coincident neighbours along the surface, in the
+// order (-,-), (+,-), (-,+), (+,+) of their odd axes, over their hanging mask;
+// Fc and F the same array (coincident nodes read, the others written)
+__global__ __launch_bounds__(64) void shell_c2f_between(const double* __restrict__ Fc, double* __restrict__ F, int fnx,
+                                                        int fny, const PopLayout FL, const int64_t* __restrict__ fs,
+                                                        const uint32_t* __restrict__ hmask, int64_t nfc, int64_t nfs) {
+    const int64_t t = nfc + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfs) return;
+    const int64_t l = fs[t];
+    const int x[3] = {(int)(l % fnx), (int)((l / fnx) % fny), (int)(l / ((int64_t)fnx * fny))};
+    const int64_t st[3] = {1, fnx, (int64_t)fnx * fny};
+    int odd[2] = {0, 0}, no = 0;
+    for (int d = 0; d < 3; ++d)
+        if (x[d] & 1) odd[no++] = d;
+    const uint32_t m = hmask[t];
+    if (no == 1) {
+        const int64_t a = l - st[odd[0]], b = l + st[odd[0]];
+#pragma unroll
+        for (int q = 0; q < 19; ++q)
+            if ((m >> q) & 1u) F[pidx(FL, q, l)] = (Fc[pidx(FL, q, a)] + Fc[pidx(FL, q, b)]) * 0.5;
+    } else {
+        const int64_t a = l - st[odd[0]] - st[odd[1]], b = l + st[odd[0]] - st[odd[1]];
+        const int64_t c = l - st[odd[0]] + st[odd[1]], d = l + st[odd[0]] + st[odd[1]];
+#pragma unroll
+        for (int q = 0; q < 19; ++q)
+            if ((m >> q) & 1u)
+                F[pidx(FL, q, l)] =
+                    ((Fc[pidx(FL, q, a)] + Fc[pidx(FL, q, b)]) + (Fc[pidx(FL, q, c)] + Fc[pidx(FL, q, d)])) * 0.25;
+    }
+}
+
+// the restricted coarse nodes from their coincident fine node (F2C-Filter
+// NOFI; LAG and TOU are f2c_lag over the same list), over their masks
+template <int ORDER>
+__global__ __launch_bounds__(256) void shell_f2c(const K64* __restrict__ kp, const double* __restrict__ F, int fnx,
+                                                 int fny, const NeqScale ainv, double* __restrict__ C, int cnx,
+                                                 int cny, const PopLayout CL, const PopLayout FL,
+                                                 const int64_t* __restrict__ cr, const uint32_t* __restrict__ mask,
+                                                 int64_t ncr) {
+    __shared__ K64 sk;
+    const K64& k = stage_k64(kp, sk);
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncr) return;
+    const int64_t lc = cr[t];
+    const int I = (int)(lc % cnx), J = (int)((lc / cnx) % cny), K = (int)(lc / ((int64_t)cnx * cny));
+    const int64_t lf = 2 * I + (int64_t)fnx * (2 * J + (int64_t)fny * (2 * K));
+    double f[19];
+#pragma unroll
+    for (int q = 0; q < 19; ++q) f[q] = F[pidx(FL, q, lf)];
+    rescale<ORDER>(k, f, ainv);
+    const uint32_t m = mask[t];
+#pragma unroll
+    for (int q = 0; q < 19; ++q)
+        if ((m >> q) & 1u) C[pidx(CL, q, lc)] = f[q];
+}
+
+// HRR (and the cubic-Mach terms): the coarse partners of the coincident fine
+// grid-coupling nodes (Node.cpp:2114-2380), their neighbours on the core box's
+// surface read through their fine partners; HALF at t + 1/2 (the other
+// neighbours the mean of t and t + 1).  Into S [nv][nfs] at the node's list
+// position.
+template <bool HALF>
+__global__ __launch_bounds__(64) void shell_partner_coarse(const K64* __restrict__ kcp, const double* __restrict__ Cprev,
+                                                           const double* __restrict__ Ccur, const PopLayout CL, int cnx,
+                                                           int cny, const double* __restrict__ F, const PopLayout FL,
+                                                           int fnx, int fny, const int64_t* __restrict__ fs,
+                                                           int64_t nfc, int64_t nfs, const Box6 cb, int nv,
+                                                           double* __restrict__ S) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfc) return;
+    const int64_t l = fs[t];
+    const int P[3] = {(int)(l % fnx) / 2, (int)((l / fnx) % fny) / 2, (int)(l / ((int64_t)fnx * fny)) / 2};
+    const K64& k = *kcp;
+    double nb[6][4];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        const int a = P[0] + EX[m], b = P[1] + EY[m], d = P[2] + EZ[m];
+        if (box_surface(cb, a, b, d)) {
+            macros_si<double>(k, nullptr, F, FL, 2 * a + (int64_t)fnx * (2 * b + (int64_t)fny * (2 * d)), nb[m]);
+            continue;
+        }
+        const int64_t lc = a + (int64_t)cnx * (b + (int64_t)cny * d);
+        if constexpr (HALF) {
+            double p4[4], c4[4];
+            macros_si<double>(k, nullptr, Cprev, CL, lc, p4);
+            macros_si<double>(k, nullptr, Ccur, CL, lc, c4);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) nb[m][v] = 0.5 * (p4[v] + c4[v]);
+        } else {
+            macros_si<double>(k, nullptr, Ccur, CL, lc, nb[m]);
+        }
+    }
+    store_handed(k, nb, nv, S, (int)nfs, (int)t);
+}
+
+// the other fine grid-coupling nodes: the mean of their coincident
+// neighbours' handed values (list positions nbpos [4] per node, -1 unused)
+__global__ __launch_bounds__(256) void shell_spread(const int32_t* __restrict__ nbpos, int64_t nfc, int64_t nfs, int nv,
+                                                    double* __restrict__ S) {
+    const int64_t t = nfc + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nfs) return;
+    const int32_t* p = nbpos + 4 * (t - nfc);
+    for (int v = 0; v < nv; ++v) {
+        const double* V = S + (int64_t)v * nfs;
+        S[(int64_t)v * nfs + t] = p[2] < 0 ? (V[p[0]] + V[p[1]]) * 0.5 : ((V[p[0]] + V[p[1]]) + (V[p[2]] + V[p[3]])) * 0.25;
+    }
+}
+
+// the fine partners of the core box's surface (Node.cpp:2383-2490): fine node
+// 2 Q from its six fine neighbours; into S [nv][ncs]
+__global__ __launch_bounds__(64) void shell_partner_fine(const K64* __restrict__ kfp, const double* __restrict__ F,
+                                                         const PopLayout FL, int fnx, int fny, int cnx, int cny,
+                                                         const int64_t* __restrict__ cr, int64_t ncs, int nv,
+                                                         double* __restrict__ S) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ncs) return;
+    const int64_t lc = cr[t];
+    const int I = (int)(lc % cnx), J = (int)((lc / cnx) % cny), K = (int)(lc / ((int64_t)cnx * cny));
+    const K64& k = *kfp;
+    double nb[6][4];
+#pragma unroll
+    for (int m = 0; m < 6; ++m)
+        macros_si<double>(k, nullptr, F, FL,
+                          2 * I + EX[m] + (int64_t)fnx * (2 * J + EY[m] + (int64_t)fny * (2 * K + EZ[m])), nb[m]);
+    store_handed(k, nb, nv, S, (int)ncs, (int)t);
+}
+
+}  // namespace refine
+
 hipError_t launch_fix64(int op, int order, bool cubic, const K64* k, const double* A, double* B, double* psi,
                         const Grid& g, int col, const double* S, hipStream_t s) {
     const int per = g.ny * g.nz;
@@ -1243,6 +1480,22 @@ struct acp_refined {
     // sets (ACP_LINKS_*: Coarse-to-Fine, Fine-to-Coarse, Hanging-Node-Reconstruction)
     int scheme = ACP_COUPLING_STD;
     int c2f_set = ACP_LINKS_ALL, f2c_set = ACP_LINKS_ALL, hang_set = ACP_LINKS_ALL;
+    // the reference's geometry, a fine shell around a coarse core
+    // (acp_refined_create_shell): the fine hole box hb and the coarse core box
+    // cb; on the host the fine grid-coupling nodes (coincident first) and the
+    // restricted coarse nodes (the core box's surface first), on the device the
+    // same lists with their link masks, the hanging nodes' coincident
+    // neighbours (list positions) and the handed values [nv][nfs] + [nv][ncs]
+    bool shell = false;
+    acp::refine::Box6 hb{}, cb{};
+    std::vector<int64_t> h_fs, h_cr;
+    int64_t nfs = 0, nfc = 0, ncr = 0, ncs = 0;
+    int64_t* d_fs = nullptr;
+    uint32_t* d_fs_mask = nullptr;  // [2][nfs]: Coarse-to-Fine, then hanging
+    int32_t* d_fs_nb = nullptr;     // [nfs - nfc][4]
+    int64_t* d_cr = nullptr;
+    uint32_t* d_cr_mask = nullptr;
+    double* d_sh = nullptr;
 };
 
 namespace {
@@ -1349,7 +1602,90 @@ hipError_t launch_f2c_s(acp_refined* r) {
     return hipGetLastError();
 }
 
+// ---- the shell geometry's launches (float64)
+hipError_t launch_shell_c2f(acp_refined* r, int half) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int fnx = f->g.nx, fny = f->g.ny;
+    const dim3 b(64), g1((unsigned)((r->nfc + 63) / 64)), g2((unsigned)((r->nfs - r->nfc + 63) / 64));
+#define SC2F(O)                                                                                                   \
+    hipLaunchKernelGGL((acp::refine::shell_c2f_co<O>), g1, b, 0, r->stream, c->dk, c->B, c->A, half,             \
+                       neq_scale(r, false), c->g.nx, c->g.ny, c->g.L, f->A, fnx, fny, f->g.L, r->d_fs, r->d_fs_mask, \
+                       r->nfc)
+    if (c->d.order == 2) SC2F(2);
+    else if (c->d.order == 3) SC2F(3);
+    else SC2F(4);
+#undef SC2F
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || r->nfs == r->nfc) return e;
+    hipLaunchKernelGGL(acp::refine::shell_c2f_between, g2, b, 0, r->stream, (const double*)f->A, f->A, fnx, fny,
+                       f->g.L, r->d_fs, r->d_fs_mask + r->nfs, r->nfc, r->nfs);
+    return hipGetLastError();
+}
+
+hipError_t launch_shell_f2c(acp_refined* r) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int64_t n = r->ncr;
+    if (n == 0) return hipSuccess;
+    const acp::refine::StdLinks lk{};
+    const dim3 b(256);
+#define SF2C(O)                                                                                                   \
+    hipLaunchKernelGGL((acp::refine::shell_f2c<O>), dim3((unsigned)((n + 255) / 256)), b, 0, r->stream, c->dk,    \
+                       f->A, f->g.nx, f->g.ny, neq_scale(r, true), c->A, c->g.nx, c->g.ny, c->g.L, f->g.L, r->d_cr, \
+                       r->d_cr_mask, n)
+#define SLAG(O, TOU)                                                                                              \
+    hipLaunchKernelGGL((acp::refine::f2c_lag<O, TOU, true>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) /  \
+                                                                              acp::refine::kLagNodes)),          \
+                       b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, neq_scale(r, true), c->A, c->g.nx, \
+                       c->g.ny, c->g.nz, 0, 0, c->g.L, f->g.L, lk, r->d_cr, r->d_cr_mask, n)
+    const int o = c->d.order;
+    if (r->filter == ACP_F2C_NOFI) {
+        if (o == 2) SF2C(2);
+        else if (o == 3) SF2C(3);
+        else SF2C(4);
+    } else if (r->filter == ACP_F2C_LAG) {
+        if (o == 2) SLAG(2, false);
+        else if (o == 3) SLAG(3, false);
+        else SLAG(4, false);
+    } else {
+        if (o == 2) SLAG(2, true);
+        else if (o == 3) SLAG(3, true);
+        else SLAG(4, true);
+    }
+#undef SLAG
+#undef SF2C
+    return hipGetLastError();
+}
+
+hipError_t launch_shell_transfer(acp_refined* r, int half) {
+    if (!r->d_sh) return hipSuccess;
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int fnx = f->g.nx, fny = f->g.ny, cnx = c->g.nx, cny = c->g.ny;
+    double* Sface = r->d_sh;
+    double* Sgc = r->d_sh + (int64_t)r->nv * r->nfs;
+    const dim3 b64(64), b256(256), g1((unsigned)((r->nfc + 63) / 64));
+    if (half)
+        hipLaunchKernelGGL((acp::refine::shell_partner_coarse<true>), g1, b64, 0, r->stream, c->dk, c->B, c->A, c->g.L,
+                           cnx, cny, f->A, f->g.L, fnx, fny, r->d_fs, r->nfc, r->nfs, r->cb, r->nv, Sface);
+    else
+        hipLaunchKernelGGL((acp::refine::shell_partner_coarse<false>), g1, b64, 0, r->stream, c->dk, c->A, c->A,
+                           c->g.L, cnx, cny, f->A, f->g.L, fnx, fny, r->d_fs, r->nfc, r->nfs, r->cb, r->nv, Sface);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess && r->nfs > r->nfc) {
+        hipLaunchKernelGGL(acp::refine::shell_spread, dim3((unsigned)((r->nfs - r->nfc + 255) / 256)), b256, 0,
+                           r->stream, r->d_fs_nb, r->nfc, r->nfs, r->nv, Sface);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess || half || r->ncs == 0) return e;
+    hipLaunchKernelGGL(acp::refine::shell_partner_fine, dim3((unsigned)((r->ncs + 63) / 64)), b64, 0, r->stream, f->dk,
+                       f->A, f->g.L, fnx, fny, cnx, cny, r->d_cr, r->ncs, r->nv, Sgc);
+    return hipGetLastError();
+}
+
 hipError_t launch_c2f(acp_refined* r, int half) {
+    if (r->shell) return launch_shell_c2f(r, half);
     acp_case* c = r->coarse;
     acp_case* f = r->fine;
     if (c->precision == ACP_PREC_F32_SHIFTED)
@@ -1377,6 +1713,7 @@ hipError_t launch_c2f(acp_refined* r, int half) {
 }
 
 hipError_t launch_f2c(acp_refined* r) {
+    if (r->shell) return launch_shell_f2c(r);
     acp_case* c = r->coarse;
     acp_case* f = r->fine;
     const int width = r->x1 - r->x0 - 1;
@@ -1393,11 +1730,11 @@ hipError_t launch_f2c(acp_refined* r) {
                        neq_scale(r, true),                                                                        \
                        c->A, c->g.nx, c->g.ny, c->g.nz, r->x0, width, c->g.L, f->g.L, lk)
 #define F2CLAG(O, TOU)                                                                                           \
-    hipLaunchKernelGGL((acp::refine::f2c_lag<O, TOU>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) /        \
-                                                                        acp::refine::kLagNodes)),                \
+    hipLaunchKernelGGL((acp::refine::f2c_lag<O, TOU, false>), dim3((unsigned)((n + acp::refine::kLagNodes - 1) / \
+                                                                               acp::refine::kLagNodes)),         \
                        b, 0, r->stream, c->dk, f->A, f->g.nx, f->g.ny, f->g.nz, neq_scale(r, true), c->A, c->g.nx, \
                        c->g.ny,                                                                                   \
-                       c->g.nz, r->x0, width, c->g.L, f->g.L, lk)
+                       c->g.nz, r->x0, width, c->g.L, f->g.L, lk, nullptr, nullptr, (int64_t)0)
     if (r->filter != ACP_F2C_NOFI) {
         const bool tou = r->filter == ACP_F2C_TOU;
         if (c->d.order == 2) {
@@ -1566,6 +1903,7 @@ hipError_t launch_transfer_t(acp_refined* r, int half) {
 }
 
 hipError_t launch_transfer(acp_refined* r, int half) {
+    if (r->shell) return launch_shell_transfer(r, half);
     if (!r->s_col) return hipSuccess;
     switch (r->coarse->precision) {
         case ACP_PREC_F64: return launch_transfer_t<double>(r, half);
@@ -1739,6 +2077,139 @@ int refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* st
     return ACP_OK;
 }
 
+
+// ---- the shell geometry on the host: node lists (in the oracle's order) and
+// link masks
+bool box_in_h(const acp::refine::Box6& B, int i, int j, int k) {
+    return i >= B.b[0] && i <= B.b[1] && j >= B.b[2] && j <= B.b[3] && k >= B.b[4] && k <= B.b[5];
+}
+bool box_inner_h(const acp::refine::Box6& B, int i, int j, int k) {
+    return i > B.b[0] && i < B.b[1] && j > B.b[2] && j < B.b[3] && k > B.b[4] && k < B.b[5];
+}
+
+// the link masks of the current link sets, uploaded (Coarse-to-Fine and
+// hanging: the links arriving from inside the hole, with MfP / hMfP also those
+// arriving along its surface; Fine-to-Coarse Mc: the core box's surface takes
+// the links arriving from outside the box)
+int shell_upload_masks(acp_refined* r) {
+    const int fnx = r->fine->g.nx, fny = r->fine->g.ny, cnx = r->coarse->g.nx, cny = r->coarse->g.ny;
+    std::vector<uint32_t> fm(2 * r->nfs), cm(r->ncr);
+    for (int64_t t = 0; t < r->nfs; ++t) {
+        const int64_t l = r->h_fs[t];
+        const int i = (int)(l % fnx), j = (int)((l / fnx) % fny), k = (int)(l / ((int64_t)fnx * fny));
+        uint32_t miss = 0, common = 0;
+        for (int q = 0; q < 19; ++q) {
+            const int a = i - acp::EX[q], b = j - acp::EY[q], d = k - acp::EZ[q];
+            if (box_inner_h(r->hb, a, b, d)) miss |= 1u << q;
+            else if (box_in_h(r->hb, a, b, d)) common |= 1u << q;
+        }
+        const uint32_t all = acp::refine::kAllLinks;
+        fm[t] = r->c2f_set == ACP_LINKS_ALL ? all : (r->c2f_set == ACP_LINKS_MISSING_COMMON ? (miss | common) : miss);
+        fm[r->nfs + t] =
+            r->hang_set == ACP_LINKS_ALL ? all : (r->hang_set == ACP_LINKS_MISSING_COMMON ? (miss | common) : miss);
+    }
+    for (int64_t t = 0; t < r->ncr; ++t) {
+        uint32_t m = acp::refine::kAllLinks;
+        if (t < r->ncs && r->f2c_set != ACP_LINKS_ALL) {
+            const int64_t l = r->h_cr[t];
+            const int I = (int)(l % cnx), J = (int)((l / cnx) % cny), K = (int)(l / ((int64_t)cnx * cny));
+            m = 0;
+            for (int q = 0; q < 19; ++q)
+                if (!box_in_h(r->cb, I - acp::EX[q], J - acp::EY[q], K - acp::EZ[q])) m |= 1u << q;
+        }
+        cm[t] = m;
+    }
+    hipError_t e = hipMemcpy(r->d_fs_mask, fm.data(), sizeof(uint32_t) * fm.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !cm.empty())
+        e = hipMemcpy(r->d_cr_mask, cm.data(), sizeof(uint32_t) * cm.size(), hipMemcpyHostToDevice);
+    return e == hipSuccess ? ACP_OK : acp::hip_fail(e, "shell link masks");
+}
+
+void shell_free(acp_refined* r) {
+    for (void* p : {(void*)r->d_fs, (void*)r->d_fs_mask, (void*)r->d_fs_nb, (void*)r->d_cr, (void*)r->d_cr_mask,
+                    (void*)r->d_sh})
+        if (p) (void)hipFree(p);
+    r->d_fs = nullptr, r->d_fs_mask = nullptr, r->d_fs_nb = nullptr, r->d_cr = nullptr, r->d_cr_mask = nullptr;
+    r->d_sh = nullptr;
+}
+
+// the lists, their device copies and the levels' fix-node lists
+int shell_build(acp_refined* r) {
+    acp_case* c = r->coarse;
+    acp_case* f = r->fine;
+    const int fnx = f->g.nx, fny = f->g.ny, cnx = c->g.nx, cny = c->g.ny, cnz = c->g.nz;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t l = 0; l < f->g.N; ++l) {
+            const int i = (int)(l % fnx), j = (int)((l / fnx) % fny), k = (int)(l / ((int64_t)fnx * fny));
+            if (!box_in_h(r->hb, i, j, k) || box_inner_h(r->hb, i, j, k)) continue;
+            const bool co = !(i & 1) && !(j & 1) && !(k & 1);
+            if (co == (pass == 0)) r->h_fs.push_back(l);
+        }
+        if (pass == 0) r->nfc = (int64_t)r->h_fs.size();
+    }
+    r->nfs = (int64_t)r->h_fs.size();
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t l = 0; l < c->g.N; ++l) {
+            const int I = (int)(l % cnx), J = (int)((l / cnx) % cny), K = (int)(l / ((int64_t)cnx * cny));
+            const bool face = I == 0 || I == cnx - 1 || J == 0 || J == cny - 1 || K == 0 || K == cnz - 1;
+            const bool take = pass == 0 ? (box_in_h(r->cb, I, J, K) && !box_inner_h(r->cb, I, J, K))
+                                        : (!box_in_h(r->cb, I, J, K) && !face);
+            if (take) r->h_cr.push_back(l);
+        }
+        if (pass == 0) r->ncs = (int64_t)r->h_cr.size();
+    }
+    r->ncr = (int64_t)r->h_cr.size();
+    // the hanging nodes' coincident neighbours along the surface, as list positions
+    std::vector<int32_t> pos(f->g.N, -1);
+    for (int64_t t = 0; t < r->nfc; ++t) pos[r->h_fs[t]] = (int32_t)t;
+    std::vector<int32_t> nbp(4 * (r->nfs - r->nfc), -1);
+    const int64_t st[3] = {1, fnx, (int64_t)fnx * fny};
+    for (int64_t t = r->nfc; t < r->nfs; ++t) {
+        const int64_t l = r->h_fs[t];
+        const int x[3] = {(int)(l % fnx), (int)((l / fnx) % fny), (int)(l / ((int64_t)fnx * fny))};
+        int odd[2] = {0, 0}, no = 0;
+        for (int d = 0; d < 3; ++d)
+            if (x[d] & 1) odd[no++] = d;
+        int32_t* p = &nbp[4 * (t - r->nfc)];
+        if (no == 1) {
+            p[0] = pos[l - st[odd[0]]];
+            p[1] = pos[l + st[odd[0]]];
+        } else {
+            p[0] = pos[l - st[odd[0]] - st[odd[1]]];
+            p[1] = pos[l + st[odd[0]] - st[odd[1]]];
+            p[2] = pos[l - st[odd[0]] + st[odd[1]]];
+            p[3] = pos[l + st[odd[0]] + st[odd[1]]];
+        }
+        for (int e = 0; e < (no == 1 ? 2 : 4); ++e)
+            if (p[e] < 0) return fail(ACP_E_STATE, "shell geometry: a hanging node without coincident neighbours");
+    }
+    const bool handed = c->d.op == ACP_OP_HRR || (c->d.op == ACP_OP_RR && c->d.cubic);
+    hipError_t e = hipMalloc((void**)&r->d_fs, sizeof(int64_t) * r->nfs);
+    if (e == hipSuccess) e = hipMalloc((void**)&r->d_fs_mask, sizeof(uint32_t) * 2 * r->nfs);
+    if (e == hipSuccess) e = hipMalloc((void**)&r->d_fs_nb, sizeof(int32_t) * std::max<size_t>(nbp.size(), 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&r->d_cr, sizeof(int64_t) * std::max<int64_t>(r->ncr, 1));
+    if (e == hipSuccess) e = hipMalloc((void**)&r->d_cr_mask, sizeof(uint32_t) * std::max<int64_t>(r->ncr, 1));
+    if (e == hipSuccess && handed) {
+        e = hipMalloc((void**)&r->d_sh, sizeof(double) * r->nv * (r->nfs + r->ncs));
+        if (e == hipSuccess) e = hipMemset(r->d_sh, 0, sizeof(double) * r->nv * (r->nfs + r->ncs));
+    }
+    if (e == hipSuccess) e = hipMemcpy(r->d_fs, r->h_fs.data(), sizeof(int64_t) * r->nfs, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !nbp.empty())
+        e = hipMemcpy(r->d_fs_nb, nbp.data(), sizeof(int32_t) * nbp.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && r->ncr)
+        e = hipMemcpy(r->d_cr, r->h_cr.data(), sizeof(int64_t) * r->ncr, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return acp::hip_fail(e, "shell geometry lists");
+    if (handed) {
+        f->fix_list = r->d_fs;
+        f->fix_list_n = r->nfs;
+        f->fix_list_P = r->d_sh;
+        c->fix_list = r->d_cr;
+        c->fix_list_n = r->ncs;
+        c->fix_list_P = r->d_sh + (int64_t)r->nv * r->nfs;
+    }
+    return shell_upload_masks(r);
+}
+
 // One refined coarse step as plain launches on the case's stream.
 int step_once(acp_refined* r) {
     hipError_t e = launch_transfer(r, 0);
@@ -1859,10 +2330,77 @@ extern "C" int acp_refined_enable_graphs(acp_refined* r, int32_t chunk) {
     return ACP_OK;
 }
 
+extern "C" int acp_refined_create_shell(const acp_case_desc* coarse, int32_t layer_yz, int32_t layer_x, void* stream,
+                                        acp_refined** out) {
+    if (!coarse || !out) return fail(ACP_E_ARG, "acp_refined_create_shell: null argument");
+    *out = nullptr;
+    const acp_case_desc& d = *coarse;
+    if (d.boundary != ACP_BC_LODI)
+        return fail(ACP_E_UNSUPPORTED, "shell geometry: the coarse level has LODI faces (ACP_BC_LODI)");
+    if (d.precision != ACP_PREC_F64) return fail(ACP_E_UNSUPPORTED, "shell geometry: float64 only");
+    if (d.cubic && d.op != ACP_OP_RR && d.op != ACP_OP_HRR)
+        return fail(ACP_E_UNSUPPORTED, "refined cases: the cubic-Mach terms belong to RR and HRR");
+    if (layer_yz < 2 || layer_x < 2)
+        return fail(ACP_E_ARG, "shell geometry: RefinementLayerYZ and RefinementLayerX of at least 2 (the core box's "
+                               "grid-coupling nodes off the domain faces)");
+    const int hc[6] = {layer_x, d.nx - 1 - layer_yz, layer_yz, d.ny - 1 - layer_yz, layer_yz, d.nz - 1 - layer_yz};
+    if (hc[1] - hc[0] < 2 || hc[3] - hc[2] < 2 || hc[5] - hc[4] < 2)
+        return fail(ACP_E_ARG, "shell geometry: the refinement layers leave no coarse core");
+    acp_case_desc fd = d;
+    fd.nx = 2 * d.nx - 1;
+    fd.ny = 2 * d.ny - 1;
+    fd.nz = 2 * d.nz - 1;
+    fd.spacing = d.spacing / 2.;
+    for (int a = 0; a < 3; ++a) fd.domain[a] = ((a == 0 ? fd.nx : (a == 1 ? fd.ny : fd.nz)) - 0.5) * fd.spacing;
+    acp_refined* r = new acp_refined();
+    r->shell = true;
+    for (int a = 0; a < 6; ++a) {
+        r->hb.b[a] = 2 * hc[a];
+        r->cb.b[a] = hc[a] + (a % 2 ? 1 : -1);
+    }
+    r->nv = handed_planes(d);
+    if (stream) {
+        r->stream = (hipStream_t)stream;
+    } else {
+        hipError_t e = hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            delete r;
+            return acp::hip_fail(e, "hipStreamCreate");
+        }
+        r->own_stream = true;
+    }
+    int rc = acp_case_create(&d, r->stream, &r->coarse);
+    if (rc == ACP_OK) rc = acp_case_create(&fd, r->stream, &r->fine);
+    if (rc == ACP_OK) {
+        // the fine LODI faces relax over the level-0 domain length (Node.cpp:237-238, 311)
+        acp_case* f = r->fine;
+        const int n[3] = {d.nx, d.ny, d.nz};
+        for (int a = 0; a < 3; ++a) f->hk.dom_len[a] = n[a] * d.spacing;
+        hipError_t e = hipMemcpyAsync(f->dk, &f->hk, sizeof(acp::K64), hipMemcpyHostToDevice, r->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(r->stream);
+        if (e != hipSuccess) rc = acp::hip_fail(e, "fine-level constants");
+    }
+    if (rc == ACP_OK) rc = shell_build(r);
+    if (rc != ACP_OK) {
+        shell_free(r);
+        acp_case_destroy(r->coarse);
+        acp_case_destroy(r->fine);
+        if (r->own_stream) (void)hipStreamDestroy(r->stream);
+        delete r;
+        return rc;
+    }
+    r->a = (1. / r->fine->hk.omega) / (2. * (1. / r->coarse->hk.omega));
+    *out = r;
+    acp::clear_error();
+    return ACP_OK;
+}
+
 extern "C" void acp_refined_destroy(acp_refined* r) {
     if (!r) return;
     if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
     drop_graphs(r);
+    if (r->shell) (void)hipStreamSynchronize(r->stream);
+    shell_free(r);
     acp_case_destroy(r->fine);
     acp_case_destroy(r->coarse);
     if (r->own_stream && r->stream) (void)hipStreamDestroy(r->stream);
@@ -1909,6 +2447,20 @@ extern "C" int acp_refined_set_coupling(acp_refined* r, int32_t scheme, int32_t 
     if (!r) return fail(ACP_E_ARG, "acp_refined_set_coupling: null argument");
     auto valid = [](int32_t v) { return v == ACP_LINKS_MISSING || v == ACP_LINKS_MISSING_COMMON || v == ACP_LINKS_ALL; };
     if (!valid(c2f) || !valid(f2c) || !valid(hanging)) return fail(ACP_E_ARG, "coupling link sets: ACP_LINKS_*");
+    if (r->shell) {
+        if (scheme != ACP_COUPLING_STD)
+            return fail(ACP_E_UNSUPPORTED, "shell geometry: the standard coupling (Overlap 1) only");
+        if (f2c == ACP_LINKS_MISSING_COMMON)
+            return fail(ACP_E_UNSUPPORTED, "Fine-to-Coarse McP belongs to the direct coupling (the standard "
+                                           "coupling takes Mc or cAll)");
+        if (r->graph_chunk > 0) (void)hipStreamSynchronize(r->stream);
+        drop_graphs(r);
+        (void)hipStreamSynchronize(r->stream);
+        r->c2f_set = c2f;
+        r->f2c_set = f2c;
+        r->hang_set = hanging;
+        return shell_upload_masks(r);
+    }
     if (scheme == ACP_COUPLING_STD) {
         if (f2c == ACP_LINKS_MISSING_COMMON)
             return fail(ACP_E_UNSUPPORTED, "Fine-to-Coarse McP belongs to the direct coupling (the standard "
@@ -1979,6 +2531,7 @@ extern "C" int acp_refined_init_reference(acp_refined* r) {
 
 extern "C" int acp_refined_write_vtk(acp_refined* r, const char* stem) {
     if (!r || !stem) return fail(ACP_E_ARG, "acp_refined_write_vtk: null argument");
+    if (r->shell) return fail(ACP_E_UNSUPPORTED, "two-level VTK output of the shell geometry is not available");
     if (!r->coarse->initialized || !r->fine->initialized) return fail(ACP_E_STATE, "refined case not initialised");
     std::vector<double> arr[2][2];  // [level][current, other]
     acp_case* lv[2] = {r->coarse, r->fine};

@@ -306,6 +306,29 @@ int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* r
  * psi of its own collision. */
 typedef struct acp_refined acp_refined;
 int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void* stream, acp_refined** out);
+/* The reference's own two-level geometry (MaxLevel 1; createLattice,
+ * Lattice.cpp:6-213, with RefinementLayerYZ = layer_yz, RefinementLayerX =
+ * layer_x, Overlap 1): a fine shell around a coarse core.  The coarse level
+ * `coarse` has LODI on every face (ACP_BC_LODI); the fine level covers the
+ * domain at half the spacing (2 nx - 1, 2 ny - 1, 2 nz - 1 nodes, LODI faces
+ * relaxing over the level-0 domain length) outside the hole box
+ * [2 layer_x, 2 (nx - 1 - layer_yz)] x [2 layer_yz, 2 (ny - 1 - layer_yz)] x
+ * [2 layer_yz, 2 (nz - 1 - layer_yz)], whose surface holds the fine
+ * grid-coupling nodes; the coarse grid-coupling nodes are the surface of the
+ * core box one coarse cell outside it.  The standard coupling with its link
+ * sets (acp_refined_set_coupling; the links a fine grid-coupling node misses
+ * are those arriving from inside the hole, setConnectivity's empty
+ * neighbours), the filters and Scaling-Styles of acp_refined_*, HRR's strain
+ * rates and the cubic-Mach psi handed by the partner functions at faces,
+ * edges and corners of the interface.  Both levels step the whole domain:
+ * the hole's fine nodes and the coarse nodes outside the core box stand for
+ * nodes the reference does not step and are overwritten where the coupling
+ * reads them (every coarse node outside the core box off the domain faces
+ * takes its coincident fine node).  Float64; layers of at least 2 that leave
+ * a core of at least 3 nodes per direction; no placement in a region and no
+ * two-level VTK output (ACP_E_UNSUPPORTED). */
+int acp_refined_create_shell(const acp_case_desc* coarse, int32_t layer_yz, int32_t layer_x, void* stream,
+                             acp_refined** out);
 void acp_refined_destroy(acp_refined* r);
 /* The level's case (0 coarse, 1 fine) for probes, read-back and output; step
  * it only through acp_refined_step.  Probes record each level's state after

@@ -1,0 +1,118 @@
+"""The reference's two-level geometry on the GPU (acp_refined_create_shell): a
+fine shell around a coarse core, both levels bit for bit against the
+oracle's restatement in float64."""
+import numpy as np
+import pytest
+
+import acpulse_vclbm_amd as acp
+import oracle_lib as ol
+from make_golden import CS, DX, RHO0
+
+pytestmark = pytest.mark.gpu
+
+N, LAYERS = (21, 17, 13), (3, 6)
+
+
+def _pair(op, order, mrt, cubic, links, filt, transverse=False):
+    p = ol.make_params(*N, op=op, order=order, periodic=False, tau=0.6, mrt_mode=mrt, cubic=cubic, cs=CS, spacing=DX,
+                       rho0=RHO0, transverse=transverse, sigma=0.7)
+    o = ol.OracleRefined(p, 0, 0, shell=LAYERS)
+    o.set_coupling("STD", *links)
+    o.set_filter(filt)
+    d = acp.make_desc(*N, op=op, order=order, tau=0.6, sigma=0.7, cubic=cubic, boundary="LODI", transverse=transverse,
+                      mrt_mode="RELAXED" if mrt else "REFERENCE")
+    r = acp.Refined(d, shell=LAYERS)
+    r.set_coupling("STD", *links)
+    r.set_filter(filt)
+    return o, r, p
+
+
+def _pulse(lp, amp, c, b, U=0.0):
+    """A Gaussian density pulse (centre c, half-width b in coarse nodes) in a
+    uniform flow U along x, at a level's nodes, as feq."""
+    h = lp.spacing / DX
+    x = np.tile(np.arange(lp.nx), lp.ny * lp.nz) * h
+    y = np.tile(np.repeat(np.arange(lp.ny), lp.nx), lp.nz) * h
+    z = np.repeat(np.arange(lp.nz), lp.nx * lp.ny) * h
+    r2 = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2
+    rho = RHO0 * (1 + amp * np.exp(-np.log(2) * r2 / b ** 2))
+    u = np.vstack([np.full(x.size, U), np.zeros(x.size), np.zeros(x.size)])
+    return ol.equilibrium_field(lp, rho, u)
+
+
+CASES = [("SRT", 3, 0, False, ("fAll", "cAll", "hAll"), "NOFI"),
+         ("RR", 3, 0, False, ("Mf", "Mc", "hMf"), "LAG"),
+         ("HRR", 3, 0, False, ("fAll", "cAll", "hAll"), "NOFI"),
+         ("HRR", 4, 0, False, ("MfP", "Mc", "hMfP"), "TOU"),
+         ("MRT-RM", 3, 1, False, ("MfP", "cAll", "hMf"), "NOFI"),
+         ("MRT-GS", 2, 1, False, ("fAll", "Mc", "hAll"), "LAG"),
+         ("RR", 4, 0, True, ("fAll", "cAll", "hAll"), "NOFI"),
+         ("HRR", 3, 0, True, ("Mf", "Mc", "hMf"), "LAG")]
+
+
+@pytest.mark.parametrize("op,order,mrt,cubic,links,filt", CASES,
+                         ids=[f"{c[0]}-o{c[1]}-{'-'.join(c[4])}-{c[5]}" + ("-cubic" if c[3] else "") for c in CASES])
+def test_shell_matches_oracle_bitexact(gpu, op, order, mrt, cubic, links, filt):
+    """A pulse in a mean flow crossing the interface at its faces, edges and
+    corners: both levels' populations (and psi with the cubic-Mach terms) bit
+    for bit after 1, 4 and 11 coarse steps."""
+    print("case shell", op, order, links, filt, cubic, flush=True)
+    o, r, p = _pair(op, order, mrt, cubic, links, filt, transverse=op == "HRR")
+    U = 0.0 if op.startswith("MRT") else 0.05 * CS
+    f0 = [_pulse(lv.p, 1e-3, (9.0, 8.0, 6.0), 2.0, U) for lv in o.levels]
+    with r:
+        for lv in (0, 1):
+            o.levels[lv].set_state(f0[lv])
+            r.level(lv).set_populations(f0[lv])
+        for n in (1, 4, 11):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                assert np.array_equal(r.level(lv).populations(), o.levels[lv].state()), (n, lv)
+                if cubic:
+                    assert np.array_equal(r.level(lv).psi(), o.levels[lv].psi()), (n, lv)
+
+
+@pytest.mark.parametrize("filt", ["NOFI", "LAG", "TOU"])
+def test_shell_restriction_alone_matches_oracle(gpu, filt):
+    o, r, p = _pair("SRT", 3, 0, False, ("fAll", "Mc", "hAll"), filt)
+    rng = np.random.default_rng(5)
+    W = np.array([1 / 18] * 6 + [1 / 36] * 12 + [1 / 3])
+    fs = [RHO0 * W[:, None] * (1.0 + 1e-3 * rng.standard_normal((19, lv.N))) for lv in o.levels]
+    with r:
+        for lv in (0, 1):
+            o.levels[lv].set_state(fs[lv])
+            r.level(lv).set_populations(fs[lv])
+        o.restrict()
+        r.restrict()
+        assert np.array_equal(r.level(0).populations(), o.levels[0].state())
+
+
+def test_shell_graphs_equal_plain_launches(gpu):
+    states = []
+    for chunk in (0, 4):
+        _, r, p = _pair("HRR", 3, 0, False, ("fAll", "cAll", "hAll"), "LAG")
+        with r:
+            r.init_pulse(1e-3, 9.0, 8.0, 2.0)
+            r.enable_graphs(chunk)
+            r.step(12)
+            states.append([r.level(lv).populations() for lv in (0, 1)])
+    for lv in (0, 1):
+        assert np.array_equal(states[0][lv], states[1][lv])
+
+
+def test_shell_refusals(gpu):
+    kw = dict(op="RR", tau=0.6, boundary="LODI")
+    for desc, layers in ((acp.make_desc(*N, **dict(kw, boundary="PERIODIC")), LAYERS),
+                         (acp.make_desc(*N, **dict(kw, precision="F32-SHIFTED")), LAYERS),
+                         (acp.make_desc(*N, **kw), (1, 6)), (acp.make_desc(*N, **kw), (6, 6)),
+                         (acp.make_desc(*N, **dict(kw, op="SRT", cubic=True)), LAYERS)):
+        with pytest.raises(acp.AcpError):
+            acp.Refined(desc, shell=layers)
+    with acp.Refined(acp.make_desc(*N, **kw), shell=LAYERS) as r:
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("DC1", "Mf", "Mc", "hMf")
+        with pytest.raises(acp.AcpError):
+            r.set_coupling("STD", "Mf", "McP", "hMf")
+        with pytest.raises(acp.AcpError):
+            r.write_vtk("x")
