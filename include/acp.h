@@ -279,8 +279,9 @@ int64_t acp_lodi_launches(int32_t nx, int32_t ny, int32_t nz, acp_lodi_record* r
  * selects (Coupling-Approach STD, Overlap 1, Coarse-to-Fine fAll,
  * Fine-to-Coarse cAll, F2C-Filter NOFI, Scaling-Style SRTSTYLE;
  * BoundaryConditions.txt:52-106), whose arithmetic (GridCoupling.cpp) the
- * reference does not contain; restated from the published scheme.  Geometry
- * differs from the reference's fine shell (Lattice.cpp:38-126): the coarse
+ * reference does not contain; restated from the published scheme.  This is
+ * the engine's x band (the reference's own fine shell around a coarse core is
+ * acp_refined_create_shell): the coarse
  * level is an ACP_BC_PERIODIC or ACP_BC_LODI_X grid and the fine level covers
  * coarse x in [x0, x1] over every y and z at half the spacing and half the
  * time step, with ratio-2 vertex-centred coupling on its x faces:

@@ -34,13 +34,15 @@
 //     dispatches to it on the collision model as Lattice.cpp:2185-2209 does.
 //     A lattice steps with the operator its case file names: a stepping
 //     member of another operator prints a message and exits.
-//   * MaxLevel 1 builds a two-level lattice (acp_refined_*, acp.h) when the
-//     case file selects one of this engine's refined geometries: the optional
-//     key `Boundary LODI-X` or `Boundary PERIODIC` and the fine region
-//     `Refined-Region x0 x1` in coarse node indices (default for LODI-X: the
-//     right half, x0 = nx / 2, x1 = nx - 1).  The reference's own geometry, a
-//     fine shell around a coarse box (Lattice.cpp:38-126, selected by the
-//     reference's keys alone), is not available and exits with that reason.
+//   * MaxLevel 1 builds a two-level lattice (acp_refined_*, acp.h).  The
+//     reference's keys alone select its own geometry, a fine shell around a
+//     coarse core (Lattice.cpp:6-213: RefinementLayerYZ, RefinementLayerX,
+//     Overlap 1 for the standard coupling; acp_refined_create_shell); the
+//     direct coupling (Overlap 0) is not available on it and exits with that
+//     reason.  The engine key `Boundary LODI-X` or `Boundary PERIODIC` with
+//     the fine region `Refined-Region x0 x1` in coarse node indices (default
+//     for LODI-X: the right half, x0 = nx / 2, x1 = nx - 1) selects this
+//     engine's x band instead.
 //   * NodeView (node(level, i, j, k)) gives the getters of the reference's
 //     Node (Node.h:38-99) on a host copy of the newest state: populations of
 //     both arrays, density and velocity, cubic-Mach terms, node kind, LODI
@@ -132,8 +134,8 @@ public:
     int getTimeStepMax() { return d_.timesteps; }
     int getWriteInterval() { return d_.write_interval; }
     int getNumberOfThreads() { return d_.threads; }
-    int getRefinementLayer() { return ref_layer_; }
-    int getRefinementLayerX() { return ref_layer_x_; }
+    int getRefinementLayer() const { return ref_layer_; }
+    int getRefinementLayerX() const { return ref_layer_x_; }
     int getOrderOfEquilibrium() { return d_.order; }
     void setTime_(double time) { time_ = time; }
     double getSpacing() { return d_.spacing; }
@@ -285,12 +287,25 @@ public:
         if (d.max_level == 0) {
             detail::check(acp_case_create(&d, nullptr, &lv_[0]), "Lattice::createLattice");
         } else if (d.max_level == 1) {
-            std::string why;
-            if (!refined_geometry(x0_, x1_, why))
-                detail::die("Lattice::createLattice: MaxLevel 1: " + why + "\n");
             acp_case_desc cd = d;
             cd.max_level = 0;
-            detail::check(acp_refined_create(&cd, x0_, x1_, nullptr, &r_), "Lattice::createLattice");
+            shell_ = d.boundary == ACP_BC_LODI;
+            if (shell_) {
+                // the reference's geometry (Lattice.cpp:6-213)
+                if (bc_.getOverlap() != 1.)
+                    detail::die("Lattice::createLattice: MaxLevel 1: Overlap " + std::to_string(bc_.getOverlap()) +
+                                ": the fine shell couples with the standard coupling (Overlap 1); the direct "
+                                "coupling (Overlap 0) is available on this engine's x band (Boundary LODI-X)\n");
+                x0_ = x1_ = 0;
+                detail::check(acp_refined_create_shell(&cd, bc_.getRefinementLayer(), bc_.getRefinementLayerX(),
+                                                       nullptr, &r_),
+                              "Lattice::createLattice");
+            } else {
+                std::string why;
+                if (!refined_geometry(x0_, x1_, why))
+                    detail::die("Lattice::createLattice: MaxLevel 1: " + why + "\n");
+                detail::check(acp_refined_create(&cd, x0_, x1_, nullptr, &r_), "Lattice::createLattice");
+            }
             detail::check(acp_refined_level(r_, 0, &lv_[0]), "Lattice::createLattice");
             detail::check(acp_refined_level(r_, 1, &lv_[1]), "Lattice::createLattice");
             apply_coupling_keys();
@@ -474,9 +489,23 @@ private:
             h.have = h.have_macros = h.have_psi = false;
         }
     }
-    int nx(int level) const { return level == 0 ? bc_.desc().nx : 2 * (x1_ - x0_) + 1; }
-    int ny(int level) const { return level == 0 ? bc_.desc().ny : 2 * bc_.desc().ny; }
-    int nz(int level) const { return level == 0 ? bc_.desc().nz : 2 * bc_.desc().nz; }
+    int nx(int level) const {
+        return level == 0 ? bc_.desc().nx : (shell_ ? 2 * bc_.desc().nx - 1 : 2 * (x1_ - x0_) + 1);
+    }
+    int ny(int level) const { return level == 0 ? bc_.desc().ny : 2 * bc_.desc().ny - (shell_ ? 1 : 0); }
+    int nz(int level) const { return level == 0 ? bc_.desc().nz : 2 * bc_.desc().nz - (shell_ ? 1 : 0); }
+    // the shell geometry's grid-coupling surfaces (createLattice): the core
+    // box on level 0, the hole box on level 1
+    bool shell_surface(int level, int i, int j, int k) const {
+        const acp_case_desc& d = bc_.desc();
+        const int ly = bc_.getRefinementLayer(), lx = bc_.getRefinementLayerX(), m = level == 0 ? 1 : 2;
+        const int o = level == 0 ? 1 : 0;  // Overlap 1 on level 0
+        const int b[6] = {m * lx - o, m * (d.nx - 1 - ly) + o, m * ly - o, m * (d.ny - 1 - ly) + o, m * ly - o,
+                          m * (d.nz - 1 - ly) + o};
+        const bool in = i >= b[0] && i <= b[1] && j >= b[2] && j <= b[3] && k >= b[4] && k <= b[5];
+        const bool inner = i > b[0] && i < b[1] && j > b[2] && j < b[3] && k > b[4] && k < b[5];
+        return in && !inner;
+    }
     int64_t lin(int level, int i, int j, int k) const {
         return i + (int64_t)nx(level) * (j + (int64_t)ny(level) * k);
     }
@@ -485,19 +514,13 @@ private:
     // reaches the high-x face of a LODI-X coarse level
     int boundary(int level) const {
         const int b = bc_.desc().boundary;
-        if (level == 0) return b;
+        if (level == 0 || shell_) return b;
         return (b == ACP_BC_LODI_X && x1_ == bc_.desc().nx - 1) ? ACP_BC_OPEN_LODI_X : ACP_BC_OPEN_X;
     }
 
     bool refined_geometry(int& x0, int& x1, std::string& why) const {
         const acp_case_desc& d = bc_.desc();
         const int* rr = bc_.refinedRegion();
-        if (d.boundary == ACP_BC_LODI) {
-            why = "the reference's fine shell around a coarse box (Lattice.cpp:38-126) is not available; this engine "
-                  "refines the slab x0 <= x <= x1 of a grid with `Boundary LODI-X` or `Boundary PERIODIC` "
-                  "(case-file keys Boundary and Refined-Region x0 x1)";
-            return false;
-        }
         if (d.boundary != ACP_BC_LODI_X && d.boundary != ACP_BC_PERIODIC) {
             why = "refinement needs Boundary LODI-X or PERIODIC";
             return false;
@@ -621,6 +644,7 @@ private:
     acp_case* lv_[2] = {nullptr, nullptr};
     acp_refined* r_ = nullptr;
     int x0_ = 0, x1_ = 0;
+    bool shell_ = false;
     int64_t N_[2] = {0, 0};
     double omega_[2] = {0., 0.}, dt_[2] = {0., 0.};
     bool cur_[2] = {true, true};   // the flag naming the array with the newest state, per level
@@ -666,6 +690,11 @@ int NodeView::getNodeID() {
 // nodes; on the fine level of a two-level lattice the coupled x faces are
 // grid-coupling nodes
 std::string NodeView::getTag() {
+    if (lat_->shell_) {
+        // createLattice with MaxLevel 1: level 0 has no boundary nodes
+        if (level_ == 1 && lodi()) return "boundary";
+        return lat_->shell_surface(level_, i_, j_, k_) ? "gridcoupling" : "fluid";
+    }
     if (lodi()) return "boundary";
     if (level_ == 1) {
         const int b = lat_->boundary(1), X = lat_->nx(1);
@@ -737,9 +766,15 @@ NodeView NodeView::getTangentNeighbor2(int dir) {
 }
 // Node::getPartner (Node.h:71): the coincident node of the other level of a
 // two-level lattice (coarse (I, J, K) with x0 <= I <= x1 and fine
-// (2 (I - x0), 2 J, 2 K))
+// (2 (I - x0), 2 J, 2 K); on the shell geometry every coarse node and fine
+// (2 I, 2 J, 2 K))
 NodeView NodeView::getPartner() {
     if (!lat_ || !lat_->r_) return NodeView();
+    if (lat_->shell_) {  // the shell geometry: coarse (I, J, K) and fine (2 I, 2 J, 2 K)
+        if (level_ == 0) return NodeView(lat_, 1, 2 * i_, 2 * j_, 2 * k_);
+        if ((i_ & 1) || (j_ & 1) || (k_ & 1)) return NodeView();
+        return NodeView(lat_, 0, i_ / 2, j_ / 2, k_ / 2);
+    }
     const int x0 = lat_->x0_, x1 = lat_->x1_;
     if (level_ == 0) {
         if (i_ < x0 || i_ > x1) return NodeView();

@@ -13,8 +13,8 @@
 //        then solve(flagL0, flagL1); prints both levels' populations
 //   lattice_check <case file> mismatch  calls a stepping member of another
 //        operator (the lattice must refuse it and exit with status 1)
-//   lattice_check <case file> create    only constructs the lattice (the
-//        reference's own two-level geometry must be refused with a reason)
+//   lattice_check <case file> create    only constructs the lattice (a
+//        geometry the engine does not build must be refused with a reason)
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -142,6 +142,18 @@ int main(int argc, char** argv) {
         for (auto& p : pts1) print_node(lat, 1, p[0], p[1], p[2]);
         print_get(lat, alt, 0, (x0 + x1) / 2, ny / 2, nz / 2);
         print_get(lat, alt1, 1, 2 * x0 + 3, ny, nz);
+        if (x0 == 0 && x1 == 0) {
+            // the shell geometry: nodes of the core box's surface (level 0) and of
+            // the hole's faces, edges and corners (level 1)
+            const int lx = bc.getRefinementLayerX(), ly = bc.getRefinementLayer();
+            const int c0[][3] = {{lx - 1, ny / 2, nz / 2}, {lx - 1, ly - 1, ly - 1}, {nx - ly, ny / 2, nz - ly},
+                                 {lx, ny / 2, nz / 2}, {nx / 2, ny / 2, nz / 2}};
+            for (auto& p : c0) print_node(lat, 0, p[0], p[1], p[2]);
+            const int f1[][3] = {{2 * lx, ny - 1, nz - 1}, {2 * lx, 2 * ly, 2 * ly}, {2 * lx, ny, nz + 1},
+                                 {2 * lx + 1, 2 * ly, nz}, {2 * (nx - 1 - ly), 2 * ly + 1, 2 * ly + 1},
+                                 {2 * lx - 2, ny - 1, nz - 1}, {1, 1, 1}};
+            for (auto& p : f1) print_node(lat, 1, p[0], p[1], p[2]);
+        }
         return 0;
     }
     std::fprintf(stderr, "unknown mode %s\n", mode.c_str());

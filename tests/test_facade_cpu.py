@@ -125,9 +125,11 @@ def test_missing_keyword_and_refinement_exit_like_the_reference(probe, tmp_path)
     assert r.returncode == 1 and 'No Value for keyword "Coupling-Approach"' in r.stderr
     r = _run(probe, tmp_path / "absent.txt")
     assert r.returncode == 1 and "Error reading boundary conditions file" in r.stderr
+    # MaxLevel 1 on a 20 x 20 x 3 grid with the reference's layers (4, 75):
+    # the fine shell leaves no coarse core
     f.write_text(txt.replace("MaxLevel 0", "MaxLevel 1"))
     r = _run(probe, f, "lattice")
-    assert r.returncode == 1 and "MaxLevel 1" in r.stderr
+    assert r.returncode == 1 and "shell geometry" in r.stderr and "core" in r.stderr
 
 
 def test_keyword_readers_through_the_c_abi(tmp_path):
@@ -144,15 +146,24 @@ def test_keyword_readers_through_the_c_abi(tmp_path):
     assert L.acp_control_read_value(str(f).encode(), b"Beta", ctypes.byref(v)) == acp._capi.ACP_E_CONFIG
 
 
-def test_reference_two_level_geometry_is_refused_with_its_reason(tmp_path):
+def test_reference_two_level_geometry_refusals_name_their_reason(tmp_path):
     """The reference's own MaxLevel 1 geometry (a fine shell around a coarse
-    box, Lattice.cpp:38-126, selected by its keys alone) is not built: the
-    Lattice constructor exits with status 1 and names the geometry that is
-    available instead, before it touches a device."""
+    core, Lattice.cpp:6-213, selected by its keys alone): layers that leave no
+    core, layers under 2 and Overlap 0 (the direct coupling, not available on
+    the shell) end the Lattice constructor with status 1 and the reason,
+    before it touches a device."""
     exe = os.path.join(ROOT, "tests", "_bin", "lattice_check")
     assert os.path.exists(exe), "tests/_bin/lattice_check is not built (__graft_entry__.build())"
     f = tmp_path / "bc.txt"
-    f.write_text(bc_text(20, 20, 7, op="HRR").replace("MaxLevel 0", "MaxLevel 1"))
-    r = subprocess.run([exe, str(f), "create"], capture_output=True, text=True, timeout=60)
-    assert r.returncode == 1
-    assert "fine shell" in r.stderr and "Refined-Region" in r.stderr and "Boundary LODI-X" in r.stderr
+    base = bc_text(20, 20, 7, op="HRR").replace("MaxLevel 0", "MaxLevel 1")
+    for text, why in ((base, "no coarse core"),
+                      (base.replace("RefinementLayerX 75", "RefinementLayerX 6").replace("RefinementLayerYZ 4",
+                                                                                          "RefinementLayerYZ 1"),
+                       "at least 2"),
+                      (base.replace("RefinementLayerX 75", "RefinementLayerX 6").replace("RefinementLayerYZ 4",
+                                                                                          "RefinementLayerYZ 2")
+                       .replace("Overlap 1", "Overlap 0"), "Overlap 0")):
+        f.write_text(text)
+        r = subprocess.run([exe, str(f), "create"], capture_output=True, text=True, timeout=60)
+        assert r.returncode == 1, text
+        assert why in r.stderr, r.stderr

@@ -184,6 +184,59 @@ def _two_level_program(tmp_path, op, coupling, cubic):
     assert rec["node"][(1, fnx - 1, 3, 1)][1] == "boundary"
 
 
+# even step counts: the program prints the array the flag true names, the current one after an even count
+@pytest.mark.parametrize("n,layers,steps,ref_steps", [((41, 41, 21), (4, 10), 4, 3), ((151, 151, 21), (4, 75), 2, 1)],
+                         ids=["41x41x21", "as-shipped-151x151x21"])
+def test_reference_two_level_geometry_through_the_facade(gpu, tmp_path, n, layers, steps, ref_steps):
+    """The reference's own MaxLevel 1 geometry from its keys alone (a fine
+    shell around a coarse core, RefinementLayerYZ / RefinementLayerX, Overlap
+    1, HRR with the standard coupling, as the shipped BoundaryConditions.txt
+    asks; the second case at its sizes): the compiled program's
+    nestedTimeStepping / solve run equals acp_refined_create_shell's at the
+    core box's surface, the hole's faces, edges and corners and inside both
+    levels, and the engine's run equals the oracle's bit for bit."""
+    nx, ny, nz = n
+    txt = (bc_text(nx, ny, nz, op="HRR", tau=0.56, steps=steps).replace("MaxLevel 0", "MaxLevel 1")
+           .replace("RefinementLayerYZ 4", f"RefinementLayerYZ {layers[0]}")
+           .replace("RefinementLayerX 75", f"RefinementLayerX {layers[1]}"))
+    f = tmp_path / "bc.txt"
+    f.write_text(txt)
+    d = acp.read_case_file(str(f))
+    assert d.boundary == acp.BOUNDARY["LODI"]
+    d.max_level = 0
+    print("case shell", n, layers, flush=True)
+    with acp.Refined(d, shell=layers) as rc:
+        rc.init_reference()
+        p0 = ol.make_params(nx, ny, nz, op="HRR", tau=0.56, cs=CS, spacing=DX, rho0=RHO0, periodic=0,
+                            nthreads=min(16, os.cpu_count() or 1))
+        o = ol.OracleRefined(p0, 0, 0, shell=layers)
+        for lv in (0, 1):
+            init = ol.reference_ic_level(o.levels[lv].p, lv, 0, DX, d.domain[0], d.domain[1])
+            assert np.array_equal(rc.level(lv).populations(), init), lv
+            o.levels[lv].set_state(init)
+        rc.step(ref_steps)
+        o.step(ref_steps)
+        for lv in (0, 1):
+            assert np.array_equal(rc.level(lv).populations(), o.levels[lv].state()), lv
+        rc.step(steps - ref_steps)
+        lv = [rc.level(0).populations(), rc.level(1).populations()]
+    r = _run(f, "nested")
+    assert r.returncode == 0, r.stderr
+    rec = _parse(r.stdout)
+    dims = [(nx, ny), (2 * nx - 1, 2 * ny - 1)]
+    assert len(rec["pop"]) >= 14
+    for key, vals in rec["pop"].items():
+        level, i, j, k = key
+        X, Y = dims[level]
+        assert [float(v) for v in vals] == list(lv[level][:, i + X * (j + Y * k)]), key
+    lx, ly = layers[1], layers[0]
+    assert rec["node"][(0, lx - 1, ny // 2, nz // 2)][1] == "gridcoupling"
+    assert rec["node"][(0, nx // 2, ny // 2, nz // 2)][1] == "fluid"
+    assert rec["node"][(1, 2 * lx, 2 * ly, 2 * ly)][1] == "gridcoupling"
+    assert rec["node"][(1, 2 * lx - 2, ny - 1, nz - 1)][1] == "fluid"
+    assert rec["partner"][(0, lx - 1, ny // 2, nz // 2)] == (1, 2 * (lx - 1), 2 * (ny // 2), 2 * (nz // 2))
+
+
 @pytest.mark.parametrize("key,value", [("F2C-Filter NOFI", "F2C-Filter XYZ"), ("Scaling-Style SRTSTYLE", "Scaling-Style XYZ"),
                                        ("Coupling-Approach STD", "Coupling-Approach XYZ"),
                                        ("Fine-to-Coarse cAll", "Fine-to-Coarse McP")])
