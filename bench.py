@@ -98,6 +98,29 @@ def run_tier(acp, torch, precision, n, steps, warmup, seed=0, op="MRT-RM", order
 LAYOUT_BLOCK_NODES = 8192  # float64 / float32 population layout blocks (csrc/engine.cpp)
 
 
+def shell_case(acp, torch, steps=160, warmup=32):
+    """The reference's own two-level geometry as its shipped case file sets it
+    up: a 151 x 151 x 21 coarse grid with LODI faces (spacing 0.02 m, air), a
+    fine shell (301 x 301 x 41 nodes over the domain, RefinementLayerYZ 4,
+    RefinementLayerX 75) around the coarse core, HRR with sigma 0.98, the standard
+    coupling with every link, the reference's initial pulse.  Time per coarse
+    step (one coarse and two fine steps with the coupling), plain launches:
+    the host clock over `steps` steps, the case stream synchronized before and
+    after."""
+    d = acp.make_desc(151, 151, 21, op="HRR", order=3, tau=0.56, boundary="LODI")
+    out = {"workload": "reference two-level geometry 151x151x21, fine shell layers 4 / 75 (301x301x41 fine nodes), HRR, "
+                       "STD", "coarse_nodes": 151 * 151 * 21, "fine_nodes": 301 * 301 * 41}
+    with acp.Refined(d, shell=(4, 75)) as r:
+        r.init_reference()
+        r.step(warmup)
+        r.level(0).sync()
+        t0 = time.perf_counter()
+        r.step(steps)
+        r.level(0).sync()
+        out["ms_per_coarse_step"] = round((time.perf_counter() - t0) / steps * 1e3, 4)
+    return out
+
+
 def copy_ceiling(acp, torch, n, reps=5):
     """Device-copy bandwidth for the same bytes as one float64 step: the 19
     populations of n^3 nodes copied one node per thread in the engine's block
@@ -434,6 +457,10 @@ def main():
             print(f"# refinement (256x128x3 + fine half, {op}, {scheme}, F2C {filt}): interface echo "
                   f"{x['R_vs_uniform_coarse']:.2e} of the pulse, {x['ms_per_coarse_step']:.3f} ms per coarse step"
                   f"{g}", file=sys.stderr, flush=True)
+        sh = shell_case(acp, torch)
+        ref["shell"] = sh
+        print(f"# refinement, the reference's geometry ({sh['workload']}): {sh['ms_per_coarse_step']:.3f} ms per coarse "
+              f"step", file=sys.stderr, flush=True)
     lodi = None
     if not a.no_lodi:
         lodi = lodi_path(acp, torch)
