@@ -31,7 +31,7 @@ enum {
     ACP_E_OOM = 3,         /* device memory allocation failed */
     ACP_E_HIP = 4,         /* a HIP runtime call failed */
     ACP_E_STATE = 5,       /* call out of order (e.g. step before init) */
-    ACP_E_UNSUPPORTED = 6, /* feature not available in this build (e.g. MaxLevel > 0) */
+    ACP_E_UNSUPPORTED = 6, /* feature not available in this build (e.g. acp_case_create with MaxLevel > 0) */
     ACP_E_IO = 7           /* output file could not be written */
 };
 
