@@ -151,6 +151,8 @@ def lib():
         "acp_refined_set_coupling": (ctypes.c_int, [vp, i32, i32, i32, i32]),
         "acp_refined_write_vtk": (ctypes.c_int, [vp, ctypes.c_char_p]),
         "acp_write_vtk2_host": (ctypes.c_int, [ctypes.POINTER(CaseDesc), dp, dp, i32, i32, dp, dp, ctypes.c_char_p]),
+        "acp_write_vtk_shell_host": (ctypes.c_int, [ctypes.POINTER(CaseDesc), i32, i32, dp, dp, dp, dp, dp, dp,
+                                                    ctypes.c_char_p]),
         "acp_refined_fine_desc": (ctypes.c_int, [ctypes.POINTER(CaseDesc), i32, i32, ctypes.POINTER(CaseDesc)]),
         "acp_refined_restrict": (ctypes.c_int, [vp]),
         "acp_refined_footprint": (ctypes.c_int, [D, i32, i32, i32, i32, ctypes.POINTER(ctypes.c_int64)]),

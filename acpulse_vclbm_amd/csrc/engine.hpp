@@ -73,7 +73,11 @@ int case_step_launch(acp_case* c, int32_t n);
 // array for the Non-Reflecting boundary quirk, cubic-Mach psi or null).
 int write_vtk2_host(const acp_case_desc& cd, const K64& ck, const double* ccur, const double* cprev,
                     const acp_case_desc& fd, const K64& fk, const double* fcur, const double* fprev, int x0, int x1,
-                    const char* stem);
+                    const char* stem, const double* cpsi = nullptr, const double* fpsi = nullptr);
+// the same for the shell geometry (RefinementLayerYZ ly, RefinementLayerX lx)
+int write_vtk_shell_host(const acp_case_desc& cd, const K64& ck, const double* ccur, const double* cprev,
+                         const double* cpsi, const acp_case_desc& fd, const K64& fk, const double* fcur,
+                         const double* fprev, const double* fpsi, int ly, int lx, const char* stem);
 int write_vtk_host(const acp_case_desc& d, const K64& k, const double* fcur, const double* fprev, const double* psi,
                    const char* stem);
 

@@ -2531,7 +2531,6 @@ extern "C" int acp_refined_init_reference(acp_refined* r) {
 
 extern "C" int acp_refined_write_vtk(acp_refined* r, const char* stem) {
     if (!r || !stem) return fail(ACP_E_ARG, "acp_refined_write_vtk: null argument");
-    if (r->shell) return fail(ACP_E_UNSUPPORTED, "two-level VTK output of the shell geometry is not available");
     if (!r->coarse->initialized || !r->fine->initialized) return fail(ACP_E_STATE, "refined case not initialised");
     std::vector<double> arr[2][2];  // [level][current, other]
     acp_case* lv[2] = {r->coarse, r->fine};
@@ -2545,8 +2544,22 @@ extern "C" int acp_refined_write_vtk(acp_refined* r, const char* stem) {
         }
     }
     auto other = [&](int l) { return arr[l][1].empty() ? arr[l][0].data() : arr[l][1].data(); };
+    // the cubic-Mach state enters the printed density and velocity (calcDensity, Node.cpp:11-30)
+    std::vector<double> psi[2];
+    if (r->coarse->d.cubic)
+        for (int l = 0; l < 2; ++l) {
+            psi[l].resize((size_t)19 * lv[l]->g.N);
+            if (int rc = acp_case_get_psi(lv[l], psi[l].data())) return rc;
+        }
+    const double* cpsi = psi[0].empty() ? nullptr : psi[0].data();
+    const double* fpsi = psi[1].empty() ? nullptr : psi[1].data();
+    if (r->shell) {
+        const int ly = r->hb.b[2] / 2, lx = r->hb.b[0] / 2;
+        return acp::write_vtk_shell_host(r->coarse->d, r->coarse->hk, arr[0][0].data(), other(0), cpsi, r->fine->d,
+                                         r->fine->hk, arr[1][0].data(), other(1), fpsi, ly, lx, stem);
+    }
     return acp::write_vtk2_host(r->coarse->d, r->coarse->hk, arr[0][0].data(), other(0), r->fine->d, r->fine->hk,
-                                arr[1][0].data(), other(1), r->x0, r->x1, stem);
+                                arr[1][0].data(), other(1), r->x0, r->x1, stem, cpsi, fpsi);
 }
 
 extern "C" int acp_refined_step(acp_refined* r, int32_t n) {

@@ -326,8 +326,8 @@ int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void
  * nodes the reference does not step and are overwritten where the coupling
  * reads them (every coarse node outside the core box off the domain faces
  * takes its coincident fine node).  Float64; layers of at least 2 that leave
- * a core of at least 3 nodes per direction; no placement in a region and no
- * two-level VTK output (ACP_E_UNSUPPORTED). */
+ * a core of at least 3 nodes per direction; no placement in a region
+ * (acp_refined_create_in is for the x band). */
 int acp_refined_create_shell(const acp_case_desc* coarse, int32_t layer_yz, int32_t layer_x, void* stream,
                              acp_refined** out);
 void acp_refined_destroy(acp_refined* r);
@@ -349,12 +349,23 @@ int acp_refined_step(acp_refined* r, int32_t n);
  * coarse nodes outside the fine region's interior (columns 0..x0 and
  * x1..nx-1), VOXEL cells of each level, POINT_DATA Omega, NodeID (0 fluid, 1
  * LODI boundary, 2 grid-coupling), Pressure, Density and Velocity in that
- * point order, numbers as the single-level writer prints them. */
+ * point order, numbers as the single-level writer prints them.  A shell case
+ * (acp_refined_create_shell) writes the reference's own two-level layout:
+ * the level-1 bottom, top, left and right slabs and inlet and outlet blocks
+ * of the shell, then the level-0 core, with the reference's cell indexing
+ * (the core's cells keep the level-1 plane size as z stride, as the
+ * reference's writer does).  With the cubic-Mach terms the density and
+ * velocity include psi, as calcDensity does. */
 int acp_refined_write_vtk(acp_refined* r, const char* stem);
 /* The same from host arrays (the coarse level's description and current /
  * other arrays, [19][N] each; the other arrays only under Non-Reflecting). */
 int acp_write_vtk2_host(const acp_case_desc* coarse, const double* ccur, const double* cprev, int32_t x0, int32_t x1,
                         const double* fcur, const double* fprev, const char* stem);
+/* The shell geometry's two-level output from host arrays (the psi arrays may
+ * be NULL; no counterpart in the reference beyond its writer). */
+int acp_write_vtk_shell_host(const acp_case_desc* coarse, int32_t layer_yz, int32_t layer_x, const double* ccur,
+                             const double* cprev, const double* cpsi, const double* fcur, const double* fprev,
+                             const double* fpsi, const char* stem);
 /* The fine level's description of a refined case (acp_refined_create's checks). */
 int acp_refined_fine_desc(const acp_case_desc* coarse, int32_t x0, int32_t x1, acp_case_desc* fine);
 /* Shifted tiers: run the coupling arithmetic (time interpolation, rescaling,

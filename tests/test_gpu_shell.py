@@ -114,5 +114,30 @@ def test_shell_refusals(gpu):
             r.set_coupling("DC1", "Mf", "Mc", "hMf")
         with pytest.raises(acp.AcpError):
             r.set_coupling("STD", "Mf", "McP", "hMf")
-        with pytest.raises(acp.AcpError):
-            r.write_vtk("x")
+
+
+@pytest.mark.parametrize("cubic", [False, True])
+def test_shell_vtk_is_the_host_writer_of_the_oracle_states(gpu, tmp_path, cubic):
+    """acp_refined_write_vtk of a shell case after 4 coarse steps (Non-Reflecting
+    output, the cubic-Mach terms in the density) is byte for byte the host
+    writer's file of the oracle's states."""
+    o, r, p = _pair("HRR" if cubic else "RR", 3, 0, cubic, ("fAll", "cAll", "hAll"), "NOFI")
+    f0 = [_pulse(lv.p, 1e-3, (9.0, 8.0, 6.0), 2.0, 0.05 * CS) for lv in o.levels]
+    with r:
+        for lv in (0, 1):
+            o.levels[lv].set_state(f0[lv])
+            r.level(lv).set_populations(f0[lv])
+        r.step(4)
+        o.step(4)
+        r.write_vtk(str(tmp_path / "gpu"))
+    d = acp.make_desc(*N, op="HRR" if cubic else "RR", order=3, tau=0.6, sigma=0.7, cubic=cubic, boundary="LODI")
+    assert d.nonreflecting
+    D = acp._capi.dptr
+    st = [(lv.state(), lv.prev(), lv.psi() if cubic else None) for lv in o.levels]
+    acp._capi.check(acp.lib().acp_write_vtk_shell_host(d, LAYERS[0], LAYERS[1], D(st[0][0]), D(st[0][1]),
+                                                        D(st[0][2]) if cubic else None, D(st[1][0]), D(st[1][1]),
+                                                        D(st[1][2]) if cubic else None,
+                                                        str(tmp_path / "host").encode()))
+    a = open(tmp_path / "gpu.vtk", "rb").read()
+    b = open(tmp_path / "host.vtk", "rb").read()
+    assert len(a) > 100000 and a == b
