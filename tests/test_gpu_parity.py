@@ -641,3 +641,41 @@ def test_acp_run_driver_two_level(gpu, tmp_path, coupling):
         rc.step(steps)
         rc.write_vtk(str(tmp_path / "gpu"))
     assert open(tmp_path / "gpu.vtk", "rb").read() == open(tmp_path / "oracle.vtk", "rb").read()
+
+
+def test_acp_run_driver_reference_two_level_geometry(gpu, tmp_path):
+    """The driver on a MaxLevel 1 case file with the reference's keys alone
+    (its own geometry, a fine shell around a coarse core: RefinementLayerYZ 3,
+    RefinementLayerX 6, Overlap 1, HRR with the standard coupling): the
+    two-level summary, <Save-as>_<t>.vtk at t = 0 and every WriteInterval, and
+    the last file equal to the shell writer's output of the oracle's run of
+    the same case (the reference's initial pulse on both levels)."""
+    import subprocess
+    from make_golden import bc_text
+    nx, ny, nz, steps, ly, lx = 21, 17, 13, 4, 3, 6
+    txt = (bc_text(nx, ny, nz, op="HRR", order=3, tau=0.56, spacing=0.02, steps=steps)
+           .replace("MaxLevel 0", "MaxLevel 1").replace("RefinementLayerYZ 4", f"RefinementLayerYZ {ly}")
+           .replace("RefinementLayerX 75", f"RefinementLayerX {lx}")
+           .replace("WriteInterval 1000000", "WriteInterval 2").replace("Save-as /tmp/acp_golden",
+                                                                        f"Save-as {tmp_path}/run"))
+    (tmp_path / "bc.txt").write_text(txt)
+    exe = os.path.join(os.path.dirname(acp._capi.LIB_PATH), "acp_run")
+    r = subprocess.run([exe, str(tmp_path / "bc.txt")], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "dimensionless collision frequency on fine grid" in r.stdout
+    assert "coupling approach = STD" in r.stdout and "solved 100.00% of the case" in r.stdout
+    assert "error" not in r.stderr.lower(), r.stderr
+    for t in (0, 2, 4):
+        assert os.path.exists(tmp_path / f"run_{t}.vtk")
+    d = acp.read_case_file(str(tmp_path / "bc.txt"))
+    d.max_level = 0
+    p0 = ol.make_params(nx, ny, nz, op="HRR", order=3, tau=0.56, spacing=0.02, periodic=0)
+    o = ol.OracleRefined(p0, 0, 0, shell=(ly, lx))
+    for lv in (0, 1):
+        o.levels[lv].set_state(ol.reference_ic_level(o.levels[lv].p, lv, 0, 0.02, d.domain[0], d.domain[1]))
+    o.step(steps)
+    D = acp._capi.dptr
+    st = [(o.levels[lv].state(), o.levels[lv].prev()) for lv in (0, 1)]
+    acp._capi.check(acp.lib().acp_write_vtk_shell_host(d, ly, lx, D(st[0][0]), D(st[0][1]), None, D(st[1][0]),
+                                                        D(st[1][1]), None, str(tmp_path / "oracle").encode()))
+    assert open(tmp_path / "run_4.vtk", "rb").read() == open(tmp_path / "oracle.vtk", "rb").read()
