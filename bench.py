@@ -365,6 +365,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-ablation", action="store_true")
     ap.add_argument("--no-refinement", action="store_true")
+    ap.add_argument("--shell-only", action="store_true",
+                    help="time only the reference's two-level geometry (shell_case) and print its JSON")
     ap.add_argument("--no-lodi", action="store_true")
     ap.add_argument("--no-mfma-trial", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=48, help="256 for the full request mix")
@@ -390,6 +392,9 @@ def main():
         print("bench.py needs a GPU", file=sys.stderr)
         sys.exit(1)
     torch.cuda.set_device(0)
+    if a.shell_only:
+        print(json.dumps({"refinement_shell": shell_case(acp, torch)}))
+        return
     if a.calibrate_copy:
         elems = 512 * 1024 * 1024
         ms = acp.copy_probe(1, elems, 2, torch.cuda.current_stream().cuda_stream)
