@@ -38,6 +38,20 @@ static_assert(sizeof(LodiNode) == 64, "one LODI record per 64 bytes");
 // sets).  A LODI face holds boundary nodes as in the reference: they do not
 // collide, push only their inward x links, and the LODI pass updates them.
 // L: layout of the population arrays (plane.hpp).
+//
+// Nodes strictly inside the box [b0, b1] x [b2, b3] x [b4, b5] are no part of
+// the lattice: the hole of the reference's two-level shell geometry, whose
+// volume the coarse level holds.  The bulk kernels and the macro pre-pass skip
+// them; nothing reads them (the fine level's couplings, restriction and output
+// touch the box surface and the nodes outside).  The all-zero box holds no node.
+struct Hole {
+    int b[6];
+    __host__ __device__ __forceinline__ bool inside(int i, int j, int k) const {
+        return i > b[0] && i < b[1] && j > b[2] && j < b[3] && k > b[4] && k < b[5];
+    }
+    bool empty() const { return b[1] - b[0] < 2 || b[3] - b[2] < 2 || b[5] - b[4] < 2; }
+};
+
 struct Grid {
     int nx, ny, nz;
     int64_t N;
@@ -45,6 +59,7 @@ struct Grid {
     PopLayout L;
     int xlodi = 0;
     int variant = BULK_AUTO;  // addressing and streaming form of the bulk kernels (plane.hpp)
+    Hole hole{};
 };
 
 hipError_t launch_bulk64(int op, int order, bool cubic, bool periodic, const K64* k, const double* A, double* B,

@@ -209,9 +209,14 @@ __device__ __forceinline__ void macro64_node(const K64& k, const double* __restr
 
 template <bool NT>
 __global__ __launch_bounds__(256) void macro64(const K64* __restrict__ kp, const double* __restrict__ A,
-                                               double* __restrict__ M, const PopLayout L) {
+                                               double* __restrict__ M, const PopLayout L, const Hole hole, int nx,
+                                               int ny) {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= L.N) return;
+    if (hole.b[1] > hole.b[0]) {
+        const int64_t r = l / nx;
+        if (hole.inside((int)(l - r * nx), (int)(r % ny), (int)(r / ny))) return;
+    }
     macro64_node<NT>(*kp, A, M, L, l);
 }
 
@@ -239,9 +244,9 @@ hipError_t launch_macro64(const K64* k, const double* A, double* macro, const Gr
     const int bs = 256;
     const dim3 grid((unsigned)((g.N + bs - 1) / bs));
     if (stream_populations(g.L, (int)sizeof(double)))
-        hipLaunchKernelGGL(macro64<true>, grid, dim3(bs), 0, s, k, A, macro, g.L);
+        hipLaunchKernelGGL(macro64<true>, grid, dim3(bs), 0, s, k, A, macro, g.L, g.hole, g.nx, g.ny);
     else
-        hipLaunchKernelGGL(macro64<false>, grid, dim3(bs), 0, s, k, A, macro, g.L);
+        hipLaunchKernelGGL(macro64<false>, grid, dim3(bs), 0, s, k, A, macro, g.L, g.hole, g.nx, g.ny);
     return hipGetLastError();
 }
 

@@ -273,13 +273,15 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(OP =
                                                      double* __restrict__ B, double* __restrict__ psi,
                                                      const double* __restrict__ M, int nx, int ny, int nz,
                                                      const PopLayout L, int xopen, int row_lds, int xlodi,
-                                                     int kz0, int kzstep, int rel_span) {
+                                                     int kz0, int kzstep, int rel_span, const Hole hole) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     // a row held by one workgroup (row_lds) keeps the threads beyond it for the
     // LDS exchange of its stores; they load, compute and store nothing
     const bool active = i < nx;
     if (!active && !row_lds) return;
     const int j = blockIdx.y, kz = kz0 + kzstep * (int)blockIdx.z;
+    // nodes of a hole (never with row_lds: the launcher takes the plain rows)
+    if (!row_lds && hole.inside(i, j, kz)) return;
     if constexpr (REL) {
         static_assert(NARROW, "block-relative addressing takes 32-bit offsets");
         const int64_t first = (int64_t)nx * ((int64_t)j + (int64_t)ny * kz);
@@ -330,10 +332,11 @@ static hipError_t go(const K64* k, const double* A, double* B, double* psi, cons
     // barriers cost more than the aligned stores gain (measured: a 256 x 128 x 3
     // duct 17.3 against 20.7 us per step, a 256^3 LODI cube 1.18 against 1.32 ms).
     const bool fully_periodic = PERIODIC && !g.xopen && !g.xlodi;
-    const int row_lds = g.nx == bx && (fully_periodic || (int64_t)g.ny * g.nz >= 4096);
+    const int row_lds = g.nx == bx && g.hole.empty() && (fully_periodic || (int64_t)g.ny * g.nz >= 4096);
 #define B64(NAR, NT, REL, GZ, KZ0, KZS)                                                                          \
     hipLaunchKernelGGL((bulk64<OP, ORDER, CUBIC, PERIODIC, NAR, NT, REL>), dim3(grid.x, grid.y, GZ), block, 0, s, k, A, \
-                       B, psi, M, g.nx, g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi, KZ0, KZS, span)
+                       B, psi, M, g.nx, g.ny, g.nz, g.L, g.xopen, row_lds, g.xlodi, KZ0, KZS, span, \
+                       g.hole)
     // a row's accesses reach one plane and a row (z and y neighbours), and on
     // grids wrapped in y (the periodic form) the wrapped row of the next or
     // previous plane: nearly two planes

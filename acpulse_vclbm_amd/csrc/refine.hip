@@ -2390,6 +2390,10 @@ extern "C" int acp_refined_create_shell(const acp_case_desc* coarse, int32_t lay
         return rc;
     }
     r->a = (1. / r->fine->hk.omega) / (2. * (1. / r->coarse->hk.omega));
+    // the fine nodes strictly inside the core box are no part of the shell
+    // (the reference's level-1 parts leave them out): the fine level's bulk
+    // kernels skip them
+    for (int a = 0; a < 6; ++a) r->fine->g.hole.b[a] = r->hb.b[a];
     *out = r;
     acp::clear_error();
     return ACP_OK;

@@ -321,11 +321,13 @@ int acp_refined_create(const acp_case_desc* coarse, int32_t x0, int32_t x1, void
  * are those arriving from inside the hole, setConnectivity's empty
  * neighbours), the filters and Scaling-Styles of acp_refined_*, HRR's strain
  * rates and the cubic-Mach psi handed by the partner functions at faces,
- * edges and corners of the interface.  Both levels step the whole domain:
- * the hole's fine nodes and the coarse nodes outside the core box stand for
- * nodes the reference does not step and are overwritten where the coupling
- * reads them (every coarse node outside the core box off the domain faces
- * takes its coincident fine node).  Float64; layers of at least 2 that leave
+ * edges and corners of the interface.  The fine level does not step the
+ * nodes strictly inside the hole box (no part of the reference's level 1;
+ * their populations are left as they are and nothing reads them).  The
+ * coarse level steps the whole domain: its nodes outside the core box stand
+ * for nodes the reference does not step and are overwritten where the
+ * coupling reads them (every coarse node outside the core box off the domain
+ * faces takes its coincident fine node).  Float64; layers of at least 2 that leave
  * a core of at least 3 nodes per direction; no placement in a region
  * (acp_refined_create_in is for the x band). */
 int acp_refined_create_shell(const acp_case_desc* coarse, int32_t layer_yz, int32_t layer_x, void* stream,

@@ -705,7 +705,17 @@ struct acpo_case {
     int32_t* fix_pos;
     const double* fix_P;
     int64_t fix_n;
+    /* nodes strictly inside the box hole[6] (x0 x1 y0 y1 z0 z1) are no part of
+     * the lattice and are not stepped: the fine level of the reference's shell
+     * geometry, whose level-1 parts (Lattice.cpp:2243-4092) leave out the volume
+     * the coarse core holds; all zero for none */
+    int hole[6];
 };
+
+static inline int in_hole(const acpo_case* c, int i, int j, int k) {
+    const int* h = c->hole;
+    return i > h[0] && i < h[1] && j > h[2] && j < h[3] && k > h[4] && k < h[5];
+}
 
 static inline int64_t lin(const acpo_case* c, int i, int j, int k) {
     return (int64_t)i + (int64_t)c->p.nx * ((int64_t)j + (int64_t)c->p.ny * k);
@@ -1038,6 +1048,8 @@ static void macro_pass(acpo_case* c) {
 #pragma omp parallel for schedule(static) num_threads(c->p.nthreads > 0 ? c->p.nthreads : 1)
     for (int64_t l = 0; l < N; ++l) {
         double f[19], r, u[3];
+        const int64_t row = l / c->p.nx;
+        if (in_hole(c, (int)(l - row * c->p.nx), (int)(row % c->p.ny), (int)(row / c->p.ny))) continue;
         load19(c->A, N, l, f);
         acpo_moments(&c->k, f, NULL, &r, u);
         c->macro[l] = r; c->macro[N + l] = u[0]; c->macro[2 * N + l] = u[1]; c->macro[3 * N + l] = u[2];
@@ -1055,6 +1067,7 @@ static void one_step(acpo_case* c) {
     for (int kk = 0; kk < nz; ++kk)
         for (int jj = 0; jj < ny; ++jj)
             for (int ii = 0; ii < nx; ++ii) {
+                if (in_hole(c, ii, jj, kk)) continue;
                 const int64_t l = lin(c, ii, jj, kk);
                 double f[19];
                 load19(c->A, N, l, f);
@@ -2071,7 +2084,7 @@ acpo_refined* acpo_refined_create_shell(const acpo_params* coarse, int layer_yz,
     r->a = (1. / r->f->k.omega) / (2. * (1. / r->c->k.omega));
     acpo_refined_set_scaling(r, 0);
     r->nv = coarse->cubic ? 25 : 6;
-    for (int d = 0; d < 6; ++d) r->hb[d] = 2 * hc[d];
+    for (int d = 0; d < 6; ++d) r->hb[d] = r->f->hole[d] = 2 * hc[d];
     const int cb[6] = {hc[0] - 1, hc[1] + 1, hc[2] - 1, hc[3] + 1, hc[4] - 1, hc[5] + 1};
     memcpy(r->cb, cb, sizeof(cb));
     /* the fine grid-coupling nodes, coincident ones first, each group in node order */

@@ -141,3 +141,33 @@ def test_shell_vtk_is_the_host_writer_of_the_oracle_states(gpu, tmp_path, cubic)
     a = open(tmp_path / "gpu.vtk", "rb").read()
     b = open(tmp_path / "host.vtk", "rb").read()
     assert len(a) > 100000 and a == b
+
+
+@pytest.mark.parametrize("op,order,cubic,links", [("HRR", 3, False, ("Mf", "Mc", "hMf")),
+                                                  ("RR", 4, True, ("fAll", "cAll", "hAll"))])
+def test_shell_leaves_the_hole_unread(gpu, op, order, cubic, links):
+    """NaN planted in the fine nodes strictly inside the hole box (which the
+    fine level does not step): both levels equal the oracle's bit for bit,
+    NaN where the oracle holds it, and no NaN outside the hole."""
+    print("case hole", op, order, cubic, flush=True)
+    o, r, p = _pair(op, order, 0, cubic, links, "LAG", transverse=op == "HRR")
+    f0 = [_pulse(lv.p, 1e-3, (9.0, 8.0, 6.0), 2.0, 0.05 * CS) for lv in o.levels]
+    lp = o.levels[1].p
+    i = np.tile(np.arange(lp.nx), lp.ny * lp.nz)
+    j = np.tile(np.repeat(np.arange(lp.ny), lp.nx), lp.nz)
+    k = np.repeat(np.arange(lp.nz), lp.nx * lp.ny)
+    lyz, lx = LAYERS
+    b = (2 * lx, 2 * (N[0] - 1 - lyz), 2 * lyz, 2 * (N[1] - 1 - lyz), 2 * lyz, 2 * (N[2] - 1 - lyz))
+    hole = (i > b[0]) & (i < b[1]) & (j > b[2]) & (j < b[3]) & (k > b[4]) & (k < b[5])
+    f0[1][:, hole] = np.nan
+    with r:
+        for lv in (0, 1):
+            o.levels[lv].set_state(f0[lv])
+            r.level(lv).set_populations(f0[lv])
+        for n in (1, 6):
+            r.step(n)
+            o.step(n)
+            for lv in (0, 1):
+                g = r.level(lv).populations()
+                assert np.array_equal(g, o.levels[lv].state(), equal_nan=True), (n, lv)
+                assert not np.isnan(g[:, ~hole] if lv else g).any(), (n, lv)

@@ -256,3 +256,46 @@ def test_pulse_in_the_core_reflects_little(op, links):
     late = np.max(np.abs(pr - pu)) / sig
     assert early < 1e-3, early
     assert late < 0.08, late
+
+
+def _hole_interior(lp, layers, n, shrink=0):
+    """Mask of the level-1 nodes strictly inside the hole box (2 RefinementLayerX,
+    2 (imax - RefinementLayerYZ)) x ..., which the fine level leaves out (the
+    box shrunk by `shrink` nodes on every side)."""
+    lyz, lx = layers
+    b = (2 * lx, 2 * (n[0] - 1 - lyz), 2 * lyz, 2 * (n[1] - 1 - lyz), 2 * lyz, 2 * (n[2] - 1 - lyz))
+    b = [v + (shrink if a % 2 == 0 else -shrink) for a, v in enumerate(b)]
+    i = np.tile(np.arange(lp.nx), lp.ny * lp.nz)
+    j = np.tile(np.repeat(np.arange(lp.ny), lp.nx), lp.nz)
+    k = np.repeat(np.arange(lp.nz), lp.nx * lp.ny)
+    return (i > b[0]) & (i < b[1]) & (j > b[2]) & (j < b[3]) & (k > b[4]) & (k < b[5])
+
+
+@pytest.mark.parametrize("op,order,cubic,links,filt", [("SRT", 3, False, ("fAll", "cAll", "hAll"), "NOFI"),
+                                                       ("HRR", 3, False, ("Mf", "Mc", "hMf"), "LAG"),
+                                                       ("RR", 4, True, ("MfP", "cAll", "hMfP"), "TOU"),
+                                                       ("MRT-RM", 3, False, ("Mf", "Mc", "hAll"), "NOFI")])
+def test_nothing_reads_the_hole(op, order, cubic, links, filt):
+    """The fine nodes strictly inside the hole box are not stepped, and no
+    coupling, restriction or stencil reads them: NaN planted there (in both
+    population arrays) stays there (the box surface pushes into the first
+    layer), and every other node of both levels is, bit for bit, what it is
+    with ordinary values in the hole."""
+    n, layers = (21, 17, 13), (3, 6)
+    runs = []
+    for plant in (0.0, np.nan):
+        r, p = _shell(op=op, n=n, layers=layers, cubic=cubic, order=order, transverse=op == "HRR")
+        r.set_coupling("STD", *links)
+        r.set_filter(filt)
+        for lv in r.levels:
+            f = _pulse3d(lv.p, p.spacing, 1e-3, (9.0, 8.0, 6.0), 2.0)
+            if lv is r.levels[1]:
+                hole = _hole_interior(lv.p, layers, n)
+                f[:, hole] = plant if np.isnan(plant) else f[:, hole] * 1.01
+            lv.set_state(f)
+        r.step(7)
+        runs.append([lv.state() for lv in r.levels])
+    (c0, f0), (c1, f1) = runs
+    assert np.array_equal(c0, c1) and not np.isnan(c1).any()
+    assert np.all(np.isnan(f1[:, _hole_interior(r.levels[1].p, layers, n, shrink=1)]))
+    assert np.array_equal(f0[:, ~hole], f1[:, ~hole]) and not np.isnan(f1[:, ~hole]).any()
