@@ -49,6 +49,10 @@ struct Hole {
     __host__ __device__ __forceinline__ bool inside(int i, int j, int k) const {
         return i > b[0] && i < b[1] && j > b[2] && j < b[3] && k > b[4] && k < b[5];
     }
+    // the x row (j, k) crosses the hole: its nodes b0 + 1 .. b1 - 1 are skipped
+    __host__ __device__ __forceinline__ bool crosses(int j, int k) const {
+        return b[1] - b[0] > 1 && j > b[2] && j < b[3] && k > b[4] && k < b[5];
+    }
     bool empty() const { return b[1] - b[0] < 2 || b[3] - b[2] < 2 || b[5] - b[4] < 2; }
 };
 

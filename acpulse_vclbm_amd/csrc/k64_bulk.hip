@@ -274,14 +274,16 @@ __global__ __launch_bounds__(kBulkBlock) __attribute__((amdgpu_waves_per_eu(OP =
                                                      const double* __restrict__ M, int nx, int ny, int nz,
                                                      const PopLayout L, int xopen, int row_lds, int xlodi,
                                                      int kz0, int kzstep, int rel_span, const Hole hole) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y, kz = kz0 + kzstep * (int)blockIdx.z;
+    // a row crossing a hole (never with row_lds: the launcher takes the plain
+    // rows) packs the nodes on either side of it into the first lanes, so the
+    // waves over the hole exit at once instead of running part-filled
+    if (!row_lds && hole.crosses(j, kz) && i > hole.b[0]) i += hole.b[1] - hole.b[0] - 1;
     // a row held by one workgroup (row_lds) keeps the threads beyond it for the
     // LDS exchange of its stores; they load, compute and store nothing
     const bool active = i < nx;
     if (!active && !row_lds) return;
-    const int j = blockIdx.y, kz = kz0 + kzstep * (int)blockIdx.z;
-    // nodes of a hole (never with row_lds: the launcher takes the plain rows)
-    if (!row_lds && hole.inside(i, j, kz)) return;
     if constexpr (REL) {
         static_assert(NARROW, "block-relative addressing takes 32-bit offsets");
         const int64_t first = (int64_t)nx * ((int64_t)j + (int64_t)ny * kz);
